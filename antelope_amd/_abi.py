@@ -1,0 +1,117 @@
+"""ctypes mirror of include/antelope/sbe_c.h (structs and constants).
+
+Kept in one place so the product binding (antelope_amd.solver) and the test
+harness describe the boundary identically. Field order and types must match
+the header; tests/test_abi.py checks sizes and offsets against a compiled
+probe of the header.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+SBE_ABI_VERSION = 1
+
+SBE_OK = 0
+SBE_ERR_CONFIG = -1
+SBE_ERR_MODEL = -2
+SBE_ERR_OOM = -3
+SBE_ERR_HIP = -4
+SBE_ERR_NAN = -5
+SBE_ERR_STATE = -6
+SBE_ERR_RANGE = -7
+SBE_ERR_UNSUPPORTED = -8
+SBE_ERR_NODEVICE = -9
+
+STATUS_NAMES = {
+    0: "SBE_OK", -1: "SBE_ERR_CONFIG", -2: "SBE_ERR_MODEL", -3: "SBE_ERR_OOM", -4: "SBE_ERR_HIP",
+    -5: "SBE_ERR_NAN", -6: "SBE_ERR_STATE", -7: "SBE_ERR_RANGE", -8: "SBE_ERR_UNSUPPORTED",
+    -9: "SBE_ERR_NODEVICE",
+}
+
+SBE_ENV_GAUSS, SBE_ENV_GAUSS_CLIPPED, SBE_ENV_COS4, SBE_ENV_COS4_CLIPPED, SBE_ENV_SIN2 = range(5)
+ENVELOPES = {
+    "gauss": SBE_ENV_GAUSS,
+    "gauss_clipped": SBE_ENV_GAUSS_CLIPPED,
+    "cos4": SBE_ENV_COS4,
+    "cos4_clipped": SBE_ENV_COS4_CLIPPED,
+    "sin2": SBE_ENV_SIN2,
+}
+
+SBE_GAUGE_LENGTH_WANNIER, SBE_GAUGE_LENGTH_HAMILTONIAN, SBE_GAUGE_VELOCITY_HAMILTONIAN = range(3)
+SBE_GRID_REFERENCE, SBE_GRID_MP = 0, 1
+SBE_INIT_UNIFORM_VALENCE, SBE_INIT_FERMI_DIRAC = 0, 1
+SBE_PREC_FP64, SBE_PREC_FP32 = 0, 1
+SBE_RK4, SBE_RK5_FEHLBERG, SBE_RK_CASH_KARP = 4, 5, 45
+SBE_EFIELD_REFERENCE, SBE_EFIELD_CONSISTENT = 0, 1
+SBE_MODEL_HALDANE, SBE_MODEL_WANNIER = 1, 2
+SBE_BASIS_ORBITAL, SBE_BASIS_BAND = 0, 1
+
+c_double_p = C.POINTER(C.c_double)
+c_int32_p = C.POINTER(C.c_int32)
+
+
+class sbe_pulse(C.Structure):
+    _fields_ = [
+        ("E0", C.c_double), ("w0", C.c_double), ("ellip", C.c_double), ("ncycles", C.c_double),
+        ("cep", C.c_double), ("t0", C.c_double), ("phix", C.c_double),
+        ("ncycles_clipped", C.c_double), ("thetaz", C.c_double), ("phiz", C.c_double),
+        ("envelope", C.c_int32), ("reserved", C.c_int32),
+    ]
+
+
+class sbe_config(C.Structure):
+    _fields_ = [
+        ("abi_version", C.c_int32), ("gauge", C.c_int32), ("npoints", C.c_int32 * 3),
+        ("rk_order", C.c_int32), ("dt", C.c_double), ("T1", C.c_double), ("T2", C.c_double),
+        ("ksfactor", C.c_double * 3), ("inter_intra", C.c_int32), ("grid", C.c_int32),
+        ("has_bz_axes", C.c_int32), ("has_bz_origin", C.c_int32), ("bz_axes", C.c_double * 9),
+        ("bz_origin", C.c_double * 3), ("init_type", C.c_int32), ("precision", C.c_int32),
+        ("fermi_energy", C.c_double), ("npulses", C.c_int32), ("efield_mode", C.c_int32),
+        ("pulses", C.POINTER(sbe_pulse)), ("offset_before", C.c_double),
+        ("offset_after", C.c_double), ("rtol", C.c_double), ("atol", C.c_double),
+    ]
+
+
+class sbe_model(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32), ("nband", C.c_int32), ("nval", C.c_int32),
+        ("is_dipole_zero", C.c_int32), ("a0", C.c_double), ("t1", C.c_double),
+        ("t2", C.c_double), ("phi0", C.c_double), ("M0", C.c_double), ("nrpts", C.c_int32),
+        ("reserved", C.c_int32), ("lattice", C.c_double * 9), ("rvec", c_int32_p),
+        ("rweight", c_double_p), ("ham", c_double_p), ("pos", c_double_p),
+    ]
+
+
+class sbe_info(C.Structure):
+    _fields_ = [
+        ("nt", C.c_int64), ("nk_total", C.c_int64), ("k_begin", C.c_int64), ("k_end", C.c_int64),
+        ("nband", C.c_int32), ("nstage", C.c_int32), ("dt", C.c_double), ("atmin", C.c_double),
+        ("atmax", C.c_double), ("dV", C.c_double), ("total_weight", C.c_double),
+        ("steps_done", C.c_int64), ("device_bytes", C.c_int64),
+    ]
+
+
+# Exported C-ABI symbols with their ctypes signatures (argtypes, restype).
+SIGNATURES = {
+    "sbe_create": ([C.POINTER(sbe_config), C.POINTER(sbe_model), C.c_int64, C.c_int64,
+                    C.POINTER(C.c_void_p)], C.c_int),
+    "sbe_run": ([C.c_void_p, C.c_int64, C.c_int64], C.c_int),
+    "sbe_synchronize": ([C.c_void_p], C.c_int),
+    "sbe_get_series": ([C.c_void_p, c_double_p, c_double_p, c_double_p, c_double_p], C.c_int),
+    "sbe_get_state": ([C.c_void_p, C.c_int32, c_double_p], C.c_int),
+    "sbe_get_info": ([C.c_void_p, C.POINTER(sbe_info)], C.c_int),
+    "sbe_query_footprint": ([C.POINTER(sbe_config), C.POINTER(sbe_model), C.c_int64, C.c_int64,
+                             C.POINTER(C.c_size_t)], C.c_int),
+    "sbe_laser_eval": ([C.POINTER(sbe_config), C.c_int64, c_double_p, c_double_p, c_double_p,
+                        c_double_p, C.POINTER(C.c_int64)], C.c_int),
+    "sbe_destroy": ([C.c_void_p], None),
+    "sbe_last_error": ([C.c_void_p], C.c_char_p),
+}
+
+
+def bind(lib: C.CDLL, signatures=SIGNATURES) -> C.CDLL:
+    for name, (argtypes, restype) in signatures.items():
+        fn = getattr(lib, name)
+        fn.argtypes = argtypes
+        fn.restype = restype
+    return lib

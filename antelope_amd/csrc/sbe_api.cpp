@@ -1,0 +1,483 @@
+// sbe_api.cpp - the C ABI of include/antelope/sbe_c.h.
+//
+// Host-side orchestration only: validation, the k-independent tables, device
+// memory and the launch sequence. All per-k arithmetic runs in the HIP
+// kernels (haldane.hip, series.hip); there is no CPU fallback. Every error
+// is reported through the return code and sbe_last_error, never by exiting.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "sbe_internal.hpp"
+
+using sbe::kBlock;
+
+namespace {
+
+thread_local std::string g_thread_error;
+
+int set_error(sbe_ctx *ctx, int code, const std::string &msg)
+{
+    if (ctx) ctx->err = msg;
+    g_thread_error = msg;
+    return code;
+}
+
+int hip_check(sbe_ctx *ctx, hipError_t e, const char *what)
+{
+    if (e == hipSuccess) return SBE_OK;
+    const int code = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? SBE_ERR_OOM : SBE_ERR_HIP;
+    return set_error(ctx, code, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Upper bound of the per-chunk partial-sum buffer.
+constexpr size_t kPartialBytes = size_t(128) << 20;
+
+struct Plan {
+    int64_t nk = 0, nk_stride = 0;
+    int32_t nwaves = 0;
+    int nv = 0;
+    int64_t chunk = 0;
+    int64_t nt = 0;
+    int nstage = 0;
+    size_t bytes_state = 0, bytes_kpts = 0, bytes_tab = 0, bytes_series = 0, bytes_partials = 0;
+    size_t total() const { return bytes_state + bytes_kpts + bytes_tab + bytes_series + bytes_partials; }
+};
+
+int validate(const sbe_config *cfg, const sbe_model *m, std::string &err)
+{
+    if (!cfg || !m) {
+        err = "null config or model";
+        return SBE_ERR_CONFIG;
+    }
+    if (cfg->abi_version != SBE_ABI_VERSION) {
+        err = "sbe_config.abi_version mismatch (expected " + std::to_string(SBE_ABI_VERSION) + ")";
+        return SBE_ERR_CONFIG;
+    }
+    if (cfg->gauge == SBE_GAUGE_LENGTH_HAMILTONIAN) {
+        err = "LengthHamiltonian gauge is not implemented (as in the reference, SBEs.h:806-810)";
+        return SBE_ERR_UNSUPPORTED;
+    }
+    if (cfg->gauge != SBE_GAUGE_LENGTH_WANNIER) {
+        err = "only the LengthWannier gauge is built";
+        return SBE_ERR_UNSUPPORTED;
+    }
+    if (cfg->precision != SBE_PREC_FP64 && cfg->precision != SBE_PREC_FP32) {
+        err = "unknown precision";
+        return SBE_ERR_CONFIG;
+    }
+    if (m->kind == SBE_MODEL_HALDANE) {
+        if (m->nband != 2) {
+            err = "Haldane model needs nband = 2";
+            return SBE_ERR_MODEL;
+        }
+        if (!(m->a0 > 0.0) || !std::isfinite(m->t1) || !std::isfinite(m->t2) || !std::isfinite(m->phi0) ||
+            !std::isfinite(m->M0)) {
+            err = "Haldane parameters must be finite with a0 > 0";
+            return SBE_ERR_MODEL;
+        }
+        if (m->nval < 0 || m->nval > 2) {
+            err = "Haldane nval must be 0..2";
+            return SBE_ERR_MODEL;
+        }
+        return SBE_OK;
+    }
+    if (m->kind == SBE_MODEL_WANNIER) {
+        err = "the Wannier (N-band) path is not available in this build";
+        return SBE_ERR_UNSUPPORTED;
+    }
+    err = "unknown model kind";
+    return SBE_ERR_MODEL;
+}
+
+int make_plan(const sbe_config &cfg, const sbe_model &m, int64_t nk, int64_t nt, int nstage, Plan &p)
+{
+    p.nk = nk;
+    p.nk_stride = (nk + kBlock - 1) / kBlock * kBlock;
+    p.nwaves = static_cast<int32_t>(p.nk_stride / 64);
+    p.nv = sbe::kHaldaneNV;
+    p.nt = nt;
+    p.nstage = nstage;
+    const size_t row = static_cast<size_t>(p.nwaves) * p.nv * sizeof(double);
+    p.chunk = std::max<int64_t>(1, std::min<int64_t>(nt, static_cast<int64_t>(kPartialBytes / row)));
+    const size_t real = cfg.precision == SBE_PREC_FP32 ? sizeof(float) : sizeof(double);
+    const int comps = cfg.precision == SBE_PREC_FP32 ? 8 : 4;
+    p.bytes_state = static_cast<size_t>(p.nk_stride) * comps * real;
+    p.bytes_kpts = static_cast<size_t>(p.nk_stride) * 4 * sizeof(double);
+    p.bytes_tab = static_cast<size_t>(nt) * nstage * 12 * (sizeof(double) + sizeof(float));
+    p.bytes_series = static_cast<size_t>(nt) * p.nv * sizeof(double);
+    p.bytes_partials = static_cast<size_t>(p.chunk) * row;
+    (void)m;
+    return SBE_OK;
+}
+
+template <typename T>
+T *dalloc(sbe_ctx *ctx, size_t bytes, int &rc)
+{
+    void *p = nullptr;
+    if (rc != SBE_OK) return nullptr;
+    hipError_t e = hipMalloc(&p, bytes ? bytes : 1);
+    if (e != hipSuccess) {
+        rc = hip_check(ctx, e, "hipMalloc");
+        return nullptr;
+    }
+    ctx->owned.push_back({p, bytes});
+    ctx->device_bytes += static_cast<int64_t>(bytes);
+    return static_cast<T *>(p);
+}
+
+// RK amplification of dn/dt = -g1 n over one step (population decay with T1
+// on is the same for every k: n_k(t_m) = n_k(0) R^m).
+double rk_linear_factor(const sbe::Tableau &tab, double z)
+{
+    double kk[7];
+    for (int i = 0; i < tab.nstage; ++i) {
+        double y = 1.0;
+        for (int j = 0; j < i; ++j) y += tab.a[i][j] * kk[j];
+        kk[i] = z * y;
+    }
+    double r = 1.0;
+    for (int i = 0; i < tab.nstage; ++i) r += tab.b[i] * kk[i];
+    return r;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *sbe_last_error(const sbe_ctx *ctx)
+{
+    if (ctx) return ctx->err.c_str();
+    return g_thread_error.c_str();
+}
+
+int sbe_laser_eval(const sbe_config *cfg, int64_t n, const double *t, double *A, double *E, double *atmin,
+                   int64_t *nt)
+{
+    if (!cfg) return set_error(nullptr, SBE_ERR_CONFIG, "null config");
+    sbe::Laser L;
+    std::string err;
+    int rc = L.init(*cfg, err);
+    if (rc) return set_error(nullptr, rc, err);
+    for (int64_t i = 0; i < n; ++i) {
+        double a[3], e[3];
+        L.eval(t[i], a, e);
+        for (int j = 0; j < 3; ++j) {
+            if (A) A[3 * i + j] = a[j];
+            if (E) E[3 * i + j] = e[j];
+        }
+    }
+    if (atmin) *atmin = L.atmin;
+    if (nt) *nt = L.nt;
+    return SBE_OK;
+}
+
+int sbe_query_footprint(const sbe_config *cfg, const sbe_model *model, int64_t k_begin, int64_t k_end,
+                        size_t *bytes)
+{
+    std::string err;
+    int rc = validate(cfg, model, err);
+    if (rc) return set_error(nullptr, rc, err);
+    sbe::Laser L;
+    rc = L.init(*cfg, err);
+    if (rc) return set_error(nullptr, rc, err);
+    sbe::Tableau tab;
+    rc = tab.init(cfg->rk_order, err);
+    if (rc) return set_error(nullptr, rc, err);
+    int64_t ntot = 1;
+    for (int d = 0; d < 3; ++d) ntot *= cfg->npoints[d] > 0 ? cfg->npoints[d] : 1;
+    if (k_begin == -1 && k_end == -1) {
+        k_begin = 0;
+        k_end = ntot;
+    }
+    if (k_begin < 0 || k_end > ntot || k_begin >= k_end) return set_error(nullptr, SBE_ERR_RANGE, "bad k range");
+    Plan p;
+    make_plan(*cfg, *model, k_end - k_begin, L.nt, tab.nstage, p);
+    if (bytes) *bytes = p.total();
+    return SBE_OK;
+}
+
+int sbe_create(const sbe_config *cfg, const sbe_model *model, int64_t k_begin, int64_t k_end, sbe_ctx **out)
+{
+    if (!out) return set_error(nullptr, SBE_ERR_CONFIG, "null output pointer");
+    *out = nullptr;
+    std::string err;
+    int rc = validate(cfg, model, err);
+    if (rc) return set_error(nullptr, rc, err);
+
+    sbe_ctx *ctx = new (std::nothrow) sbe_ctx();
+    if (!ctx) return set_error(nullptr, SBE_ERR_OOM, "host allocation failed");
+    ctx->cfg = *cfg;
+    ctx->pulses.assign(cfg->pulses, cfg->pulses + cfg->npulses);
+    ctx->cfg.pulses = ctx->pulses.data();
+    ctx->model = *model;
+    ctx->N = model->nband;
+
+    if ((rc = ctx->laser.init(ctx->cfg, err)) || (rc = ctx->tab.init(ctx->cfg.rk_order, err)) ||
+        (rc = ctx->grid.build(ctx->cfg, ctx->model, k_begin, k_end, err))) {
+        set_error(nullptr, rc, err);
+        delete ctx;
+        return rc;
+    }
+    if (ctx->tab.embedded) {
+        set_error(nullptr, SBE_ERR_UNSUPPORTED, "the adaptive Cash-Karp stepper runs through the batched solver");
+        delete ctx;
+        return SBE_ERR_UNSUPPORTED;
+    }
+
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        set_error(nullptr, SBE_ERR_NODEVICE, "no HIP device visible");
+        delete ctx;
+        return SBE_ERR_NODEVICE;
+    }
+    (void)hipGetDevice(&ctx->device);
+
+    Plan p;
+    const int64_t nk = ctx->grid.k_end - ctx->grid.k_begin;
+    make_plan(ctx->cfg, ctx->model, nk, ctx->laser.nt, ctx->tab.nstage, p);
+    ctx->nk = p.nk;
+    ctx->nk_stride = p.nk_stride;
+    ctx->nwaves = p.nwaves;
+    ctx->nv = p.nv;
+    ctx->chunk_steps = p.chunk;
+
+    rc = hip_check(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
+    ctx->d_state = dalloc<void>(ctx, p.bytes_state, rc);
+    ctx->d_kpts = dalloc<double>(ctx, p.bytes_kpts, rc);
+    ctx->d_phase_tab = dalloc<double>(ctx, p.bytes_tab, rc);
+    ctx->d_series = dalloc<double>(ctx, p.bytes_series, rc);
+    ctx->d_partials = dalloc<double>(ctx, p.bytes_partials, rc);
+    if (rc) {
+        set_error(nullptr, rc, ctx->err);
+        sbe_destroy(ctx);
+        return rc;
+    }
+    ctx->d_weight = ctx->d_kpts + 3 * ctx->nk_stride;
+
+    // k points (SoA, zero-padded) and weights
+    {
+        std::vector<double> hk(static_cast<size_t>(ctx->nk_stride) * 4, 0.0);
+        for (int64_t k = 0; k < nk; ++k) {
+            for (int c = 0; c < 3; ++c) hk[c * ctx->nk_stride + k] = ctx->grid.k[3 * k + c];
+            hk[3 * ctx->nk_stride + k] = ctx->grid.w[k];
+        }
+        rc = hip_check(ctx, hipMemcpy(ctx->d_kpts, hk.data(), p.bytes_kpts, hipMemcpyHostToDevice), "upload k");
+    }
+
+    // Haldane geometry (Haldane.h:134-143) and per-stage laser phase tables
+    sbe::HaldaneArgs &h = ctx->hargs;
+    const double a0 = model->a0;
+    const double va[3][2] = {{0., a0}, {-std::sqrt(3.) / 2 * a0, -0.5 * a0}, {std::sqrt(3.) / 2 * a0, -0.5 * a0}};
+    const double vb[3][2] = {{std::sqrt(3.) * a0, 0.}, {-std::sqrt(3.) / 2 * a0, 1.5 * a0},
+                             {-std::sqrt(3.) / 2 * a0, -1.5 * a0}};
+    std::memcpy(h.va, va, sizeof va);
+    std::memcpy(h.vb, vb, sizeof vb);
+    h.t1 = model->t1;
+    h.c0 = 2.0 * model->t2 * std::cos(model->phi0);
+    h.c3 = 2.0 * model->t2 * std::sin(model->phi0);
+    h.M0 = model->M0;
+    h.gam1 = ctx->cfg.T1 <= 1e-16 ? 0.0 : 1.0 / ctx->cfg.T1;  // SBEs.h:484-493
+    h.gam2 = ctx->cfg.T2 <= 1e-16 ? 0.0 : 1.0 / ctx->cfg.T2;
+    h.dt = ctx->laser.dt;
+    h.nk = nk;
+    h.nk_stride = ctx->nk_stride;
+    h.nwaves = ctx->nwaves;
+    h.inter_intra = ctx->cfg.inter_intra ? 1 : 0;
+    h.precision = ctx->cfg.precision;
+    h.nval = model->nval;
+    h.kpts = ctx->d_kpts;
+    h.weight = ctx->d_weight;
+    h.state = ctx->d_state;
+    h.partials = ctx->d_partials;
+    {
+        const int S = ctx->tab.nstage;
+        const int64_t nt = ctx->laser.nt;
+        const size_t n64 = static_cast<size_t>(nt) * S * 12;
+        std::vector<double> tab(n64);
+        std::vector<float> tab32(n64);
+        for (int64_t n = 0; n < nt; ++n) {
+            const double tn = ctx->laser.atmin + ctx->laser.dt * static_cast<double>(n);  // antelope.cpp:299
+            for (int s = 0; s < S; ++s) {
+                double A[3], E[3];
+                ctx->laser.eval(tn + ctx->tab.c[s] * ctx->laser.dt, A, E);  // SBEs.h:612
+                double *row = tab.data() + (static_cast<size_t>(n) * S + s) * 12;
+                for (int j = 0; j < 3; ++j) {
+                    const double pa = A[0] * va[j][0] + A[1] * va[j][1];
+                    const double pb = A[0] * vb[j][0] + A[1] * vb[j][1];
+                    row[2 * j] = std::cos(pa);
+                    row[2 * j + 1] = std::sin(pa);
+                    row[6 + 2 * j] = std::cos(pb);
+                    row[6 + 2 * j + 1] = std::sin(pb);
+                }
+            }
+        }
+        for (size_t i = 0; i < n64; ++i) tab32[i] = static_cast<float>(tab[i]);
+        float *d32 = reinterpret_cast<float *>(ctx->d_phase_tab + n64);
+        if (!rc) rc = hip_check(ctx, hipMemcpy(ctx->d_phase_tab, tab.data(), n64 * sizeof(double), hipMemcpyHostToDevice), "upload table");
+        if (!rc) rc = hip_check(ctx, hipMemcpy(d32, tab32.data(), n64 * sizeof(float), hipMemcpyHostToDevice), "upload table");
+        h.phase_tab = ctx->d_phase_tab;
+        h.phase_tab32 = d32;
+    }
+    if (!rc) rc = hip_check(ctx, hipMemsetAsync(ctx->d_series, 0, p.bytes_series, ctx->stream), "clear series");
+    if (!rc) rc = hip_check(ctx, sbe::launch_haldane_init(h, ctx->cfg.init_type, ctx->cfg.fermi_energy, ctx->stream), "init kernel");
+    if (!rc) rc = hip_check(ctx, hipStreamSynchronize(ctx->stream), "init sync");
+    if (rc) {
+        set_error(nullptr, rc, ctx->err);
+        sbe_destroy(ctx);
+        return rc;
+    }
+    // occupied-band count per k is the same for every k (FD is normalised to 1)
+    const double n0 = ctx->cfg.init_type == SBE_INIT_FERMI_DIRAC ? 1.0 : static_cast<double>(std::min(model->nval, 2));
+    ctx->n_sum = n0 * ctx->grid.total_weight;
+    *out = ctx;
+    return SBE_OK;
+}
+
+int sbe_run(sbe_ctx *ctx, int64_t step_begin, int64_t step_end)
+{
+    if (!ctx) return set_error(nullptr, SBE_ERR_STATE, "null context");
+    if (step_begin != ctx->steps_done)
+        return set_error(ctx, SBE_ERR_STATE,
+                         "steps run in order: next step is " + std::to_string(ctx->steps_done));
+    if (step_end > ctx->laser.nt || step_end < step_begin)
+        return set_error(ctx, SBE_ERR_RANGE, "step_end outside [step_begin, Nt]");
+    const bool t1_on = ctx->hargs.gam1 != 0.0;
+    for (int64_t s = step_begin; s < step_end; s += ctx->chunk_steps) {
+        const int64_t n = std::min<int64_t>(ctx->chunk_steps, step_end - s);
+        sbe::HaldaneArgs a = ctx->hargs;
+        a.step0 = s;
+        a.nsteps = static_cast<int32_t>(n);
+        int rc = hip_check(ctx, sbe::launch_haldane_steps(a, ctx->cfg.rk_order, t1_on, ctx->stream), "step kernel");
+        if (!rc)
+            rc = hip_check(ctx,
+                           sbe::launch_series_reduce(ctx->d_partials, n, ctx->nwaves, ctx->nv,
+                                                     ctx->d_series + s * ctx->nv, ctx->stream),
+                           "series kernel");
+        if (rc) return rc;
+        ctx->steps_done = s + n;
+    }
+    return SBE_OK;
+}
+
+int sbe_synchronize(sbe_ctx *ctx)
+{
+    if (!ctx) return set_error(nullptr, SBE_ERR_STATE, "null context");
+    return hip_check(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+}
+
+int sbe_get_series(sbe_ctx *ctx, double *j_inter, double *j_intra, double *rho_bar, double *band_pop)
+{
+    if (!ctx) return set_error(nullptr, SBE_ERR_STATE, "null context");
+    const int64_t nt = ctx->laser.nt;
+    std::vector<double> s(static_cast<size_t>(nt) * ctx->nv);
+    int rc = hip_check(ctx, hipMemcpyAsync(s.data(), ctx->d_series, s.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "copy series");
+    if (!rc) rc = hip_check(ctx, hipStreamSynchronize(ctx->stream), "sync series");
+    if (rc) return rc;
+    // n(t_m) = n(0) R^m for every k when T1 is on (R: RK factor of dn/dt = -n/T1)
+    const double R = ctx->hargs.gam1 != 0.0 ? rk_linear_factor(ctx->tab, -ctx->hargs.gam1 * ctx->laser.dt) : 1.0;
+    double nsum = ctx->n_sum;
+    bool finite = true;
+    for (int64_t n = 0; n < nt; ++n, nsum *= R) {
+        const double *r = s.data() + n * ctx->nv;
+        const bool done = n < ctx->steps_done;
+        for (int i = 0; i < ctx->nv && done; ++i) finite = finite && std::isfinite(r[i]);
+        const double ns = done ? nsum : 0.0;
+        if (j_inter) { j_inter[3 * n] = r[0]; j_inter[3 * n + 1] = r[1]; j_inter[3 * n + 2] = 0.0; }
+        if (j_intra) { j_intra[3 * n] = r[2]; j_intra[3 * n + 1] = r[3]; j_intra[3 * n + 2] = 0.0; }
+        if (rho_bar) {
+            double *o = rho_bar + n * 8;  // [2][2][re,im], rho = (n + s.sigma)/2
+            o[0] = 0.5 * (ns + r[6]); o[1] = 0.0;
+            o[2] = 0.5 * r[4];        o[3] = -0.5 * r[5];
+            o[4] = 0.5 * r[4];        o[5] = 0.5 * r[5];
+            o[6] = 0.5 * (ns - r[6]); o[7] = 0.0;
+        }
+        if (band_pop) {
+            band_pop[2 * n] = 0.5 * (ns - r[7]);
+            band_pop[2 * n + 1] = 0.5 * (ns + r[7]);
+        }
+    }
+    if (!finite) return set_error(ctx, SBE_ERR_NAN, "non-finite value in the propagated state");
+    return SBE_OK;
+}
+
+int sbe_get_state(sbe_ctx *ctx, int32_t basis, double *rho)
+{
+    if (!ctx) return set_error(nullptr, SBE_ERR_STATE, "null context");
+    if (!rho) return set_error(ctx, SBE_ERR_CONFIG, "null output");
+    const size_t bytes = static_cast<size_t>(ctx->nk) * 8 * sizeof(double);
+    double *d_out = nullptr;
+    int rc = hip_check(ctx, hipMalloc(&d_out, bytes), "hipMalloc");
+    if (!rc) rc = hip_check(ctx, sbe::launch_haldane_export(ctx->hargs, d_out, ctx->stream), "export kernel");
+    if (!rc) rc = hip_check(ctx, hipMemcpyAsync(rho, d_out, bytes, hipMemcpyDeviceToHost, ctx->stream), "copy state");
+    if (!rc) rc = hip_check(ctx, hipStreamSynchronize(ctx->stream), "sync state");
+    if (d_out) (void)hipFree(d_out);
+    if (rc) return rc;
+    if (basis == SBE_BASIS_BAND) {
+        // U^dag rho U at k + A(t_now) with the eigenvectors of d.sigma
+        const double t = ctx->laser.atmin + ctx->laser.dt * static_cast<double>(ctx->steps_done);
+        double A[3], E[3];
+        ctx->laser.eval(t, A, E);
+        const sbe::HaldaneArgs &h = ctx->hargs;
+        for (int64_t k = 0; k < ctx->nk; ++k) {
+            const double kx = ctx->grid.k[3 * k] + A[0], ky = ctx->grid.k[3 * k + 1] + A[1];
+            double sa = 0, ca = 0, sb = 0;
+            for (int j = 0; j < 3; ++j) {
+                ca += std::cos(kx * h.va[j][0] + ky * h.va[j][1]);
+                sa += std::sin(kx * h.va[j][0] + ky * h.va[j][1]);
+                sb += std::sin(kx * h.vb[j][0] + ky * h.vb[j][1]);
+            }
+            const double d[3] = {h.t1 * ca, h.t1 * sa, h.M0 - h.c3 * sb};
+            const double dn = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+            double *o = rho + 8 * k;
+            const double n = o[0] + o[6], sx = 2 * o[2], sy = 2 * o[5], sz = o[0] - o[6];
+            double sd = dn > 0 ? (sx * d[0] + sy * d[1] + sz * d[2]) / dn : -sz;
+            // perpendicular part magnitude goes to the coherence, real by phase choice
+            const double s2 = sx * sx + sy * sy + sz * sz;
+            const double perp = std::sqrt(std::max(0.0, s2 - sd * sd));
+            o[0] = 0.5 * (n - sd); o[1] = 0.0;
+            o[2] = 0.5 * perp;     o[3] = 0.0;
+            o[4] = 0.5 * perp;     o[5] = 0.0;
+            o[6] = 0.5 * (n + sd); o[7] = 0.0;
+        }
+    } else if (basis != SBE_BASIS_ORBITAL) {
+        return set_error(ctx, SBE_ERR_CONFIG, "unknown basis");
+    }
+    return SBE_OK;
+}
+
+int sbe_get_info(const sbe_ctx *ctx, sbe_info *info)
+{
+    if (!ctx || !info) return set_error(nullptr, SBE_ERR_STATE, "null context or info");
+    info->nt = ctx->laser.nt;
+    info->nk_total = ctx->grid.ntotal;
+    info->k_begin = ctx->grid.k_begin;
+    info->k_end = ctx->grid.k_end;
+    info->nband = ctx->N;
+    info->nstage = ctx->tab.nstage;
+    info->dt = ctx->laser.dt;
+    info->atmin = ctx->laser.atmin;
+    info->atmax = ctx->laser.atmax;
+    info->dV = ctx->grid.dV;
+    info->total_weight = ctx->grid.total_weight;
+    info->steps_done = ctx->steps_done;
+    info->device_bytes = ctx->device_bytes;
+    return SBE_OK;
+}
+
+void sbe_destroy(sbe_ctx *ctx)
+{
+    if (!ctx) return;
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto &b : ctx->owned) (void)hipFree(b.ptr);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+}  // extern "C"

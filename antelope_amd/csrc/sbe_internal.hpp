@@ -1,0 +1,142 @@
+// sbe_internal.hpp - host-side context and kernel launchers of the MI355X
+// SBE propagator. Not part of the C ABI (include/antelope/sbe_c.h is).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "antelope/sbe_c.h"
+
+namespace sbe {
+
+// ---------------------------------------------------------------- laser --
+// One pulse with its derived constants (reference laser.h:68-141).
+struct PulseConst {
+    double E0x, E0y, A0x, A0y, w0, cep, t0;
+    double twidth, twidth_clipped, envfactor, period0;
+    double xdir[3], ydir[3];
+    int env;
+};
+
+// Sum of pulses minus its value at atmin (laser.h:249-322), FP64 on the host.
+// The kernels never evaluate the laser: they read per-stage tables built
+// from this class.
+struct Laser {
+    std::vector<PulseConst> pulses;
+    double atmin = 0, atmax = 0, dt = 0;
+    double A_at_min[3] = {0, 0, 0}, E_at_min[3] = {0, 0, 0};
+    int64_t nt = 0;
+    bool consistent = false;
+
+    int init(const sbe_config &cfg, std::string &err);
+    void eval(double t, double A[3], double E[3]) const;
+};
+
+// ----------------------------------------------------------------- grid --
+// Block [k_begin,k_end) of the flattened k grid (momaxis.h) on the host.
+struct Grid {
+    int n[3] = {1, 1, 1};
+    int64_t ntotal = 0, k_begin = 0, k_end = 0;
+    double axes[9] = {0}, origin[3] = {0};
+    double dV = 0, total_weight = 0;
+    std::vector<double> k;  // [nk][3]
+    std::vector<double> w;  // [nk]
+
+    int build(const sbe_config &cfg, const sbe_model &m, int64_t kb, int64_t ke,
+              std::string &err);
+};
+
+// Explicit Runge-Kutta tableau (SBEs.h:395-416; Cash-Karp 1990 for the
+// adaptive pair).
+struct Tableau {
+    int nstage = 0;
+    double c[7] = {0}, b[7] = {0}, bhat[7] = {0}, a[7][7] = {{0}};
+    int order = 0;
+    bool embedded = false;
+    int init(int rk_order, std::string &err);
+};
+
+// Number of FP64 quantities reduced over k per time sample for the 2-band
+// path: inter x,y; intra x,y; sum w s_x, s_y, s_z; sum w s.dhat.
+constexpr int kHaldaneNV = 8;
+
+// Kernel parameters of the 2-band (Haldane) path, passed by value.
+struct HaldaneArgs {
+    // model: d(k) = (t1 sum cos(k.a_j), t1 sum sin(k.a_j),
+    //                M0 - c3 sum sin(k.b_j)),  d0 = c0 sum cos(k.b_j)
+    double t1, c0, c3, M0;
+    double va[3][2], vb[3][2];
+    double gam1, gam2;  // 1/T1, 1/T2 (0 = off)
+    double dt;
+    int64_t nk;         // k points in this block
+    int64_t nk_stride;  // padded SoA stride
+    int64_t step0;      // first step of this launch (index into the tables)
+    int32_t nsteps;     // steps in this launch
+    int32_t nwaves;     // waves in the grid (partials row length)
+    int32_t inter_intra;
+    int32_t precision;
+    const double *kpts;     // [3][nk_stride]
+    const double *weight;   // [nk_stride] (0 on padding)
+    const double *phase_tab;  // [nt][nstage][6][2] e^{i A(t_n + c_s dt).v_j}
+    const float *phase_tab32; // same table rounded to FP32 (FP32 state only)
+    int32_t nval;             // occupied bands of the initial state
+    int32_t reserved;
+    void *state;            // SoA [4][nk_stride] (fp64) or [8][nk_stride] (fp32 + compensation)
+    double *partials;       // [nsteps][nwaves][kHaldaneNV]
+};
+
+// ------------------------------------------------------------------ ctx --
+enum class PathKind { Haldane2 = 0, NBand = 1 };
+
+struct DeviceBuffer {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace sbe
+
+struct sbe_ctx {
+    sbe_config cfg{};
+    std::vector<sbe_pulse> pulses;
+    sbe_model model{};
+    sbe::Laser laser;
+    sbe::Grid grid;
+    sbe::Tableau tab;
+    sbe::PathKind path = sbe::PathKind::Haldane2;
+    int N = 2;
+    int64_t nk = 0, nk_stride = 0;
+    int64_t steps_done = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<sbe::DeviceBuffer> owned;
+    int64_t device_bytes = 0;
+
+    // device data
+    double *d_kpts = nullptr;       // [3][nk_stride]
+    double *d_weight = nullptr;     // [nk_stride]
+    double *d_phase_tab = nullptr;  // per-stage laser phase table
+    void *d_state = nullptr;
+    double *d_partials = nullptr;   // [chunk][nwaves][nv]
+    double *d_series = nullptr;     // [nt][nv]
+    int64_t chunk_steps = 0;
+    int32_t nwaves = 0;
+    int nv = 0;
+    double n_sum = 0;  // sum_k w_k n_k (constant when T1 is off)
+
+    sbe::HaldaneArgs hargs{};
+    std::string err;
+};
+
+namespace sbe {
+// Launchers (haldane.hip, series.hip).
+hipError_t launch_haldane_init(const HaldaneArgs &a, int init_type, double ef, hipStream_t s);
+hipError_t launch_haldane_steps(const HaldaneArgs &a, int rk_order, bool t1_on, hipStream_t s);
+hipError_t launch_series_reduce(const double *partials, int64_t nsteps, int32_t nrows, int nv,
+                                double *series, hipStream_t s);
+hipError_t launch_haldane_export(const HaldaneArgs &a, double *out, hipStream_t s);
+int haldane_waves(int64_t nk);
+constexpr int kBlock = 256;
+}  // namespace sbe

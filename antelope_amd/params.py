@@ -1,0 +1,179 @@
+"""Run configuration and material models, mirroring the reference's inputs.
+
+`RunConfig` holds the `calc` and `laser` groups of inputParam.cfg
+(SBEs.h:432-561); `Pulse` the arguments of Pulse::Pulse (laser.h:68-141);
+`HaldaneModel` the `Haldane` group (Materials/Haldane.h:46-84); `WannierModel`
+the tight-binding tables of Materials/Wannier90.h:107-218 (read from a
+file there, given as arrays here). All values are in atomic units and
+radians; `HaldaneModel.from_reference` and `Pulse.from_reference` take the
+reference's Angstrom / degree inputs.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _abi as A
+
+AU_ANGSTROM = 0.529177210903  # constant.h:34
+AU_EV = 27.211386245988       # constant.h:35
+
+
+@dataclass
+class Pulse:
+    E0: float
+    w0: float
+    ncycles: float
+    ellip: float = 0.0
+    cep: float = 0.0            # rad
+    t0: float = 0.0
+    phix: float = 0.0           # rad
+    envelope: str = "sin2"
+    ncycles_clipped: float = 0.0
+    thetaz: float = 0.0         # rad
+    phiz: float = 0.0           # rad
+
+    @classmethod
+    def from_reference(cls, E0, w0, ncycles, ellip, cep_deg, t0, phix_deg, env_name,
+                       thetaz_deg=0.0, phiz_deg=0.0, ncycles_clipped=0.0) -> "Pulse":
+        """Degrees -> radians as SBEs.h:531-537."""
+        d = math.pi / 180.0
+        return cls(E0=E0, w0=w0, ncycles=ncycles, ellip=ellip, cep=cep_deg * d, t0=t0,
+                   phix=phix_deg * d, envelope=env_name, ncycles_clipped=ncycles_clipped,
+                   thetaz=thetaz_deg * d, phiz=phiz_deg * d)
+
+    def to_c(self) -> A.sbe_pulse:
+        if self.envelope not in A.ENVELOPES:
+            raise ValueError(f"invalid envelope type {self.envelope!r}")
+        return A.sbe_pulse(self.E0, self.w0, self.ellip, self.ncycles, self.cep, self.t0,
+                           self.phix, self.ncycles_clipped, self.thetaz, self.phiz,
+                           A.ENVELOPES[self.envelope], 0)
+
+
+_RK = {4: A.SBE_RK4, 5: A.SBE_RK5_FEHLBERG, 45: A.SBE_RK_CASH_KARP, "cash_karp": A.SBE_RK_CASH_KARP}
+
+
+@dataclass
+class RunConfig:
+    npoints: Sequence[int]
+    pulses: Sequence[Pulse]
+    dt: float = 0.1               # SBEs.h:455
+    T1: float = -1.0
+    T2: float = -1.0
+    rk_order: object = 5          # SBEs.h:438
+    inter_intra: bool = False     # SBEs.h:442
+    ksfactor: Sequence[float] = (1.0, 1.0, 1.0)
+    grid: str = "reference"       # or "mp"
+    bz_axes: Optional[Sequence[float]] = None
+    bz_origin: Optional[Sequence[float]] = None
+    init: str = "uniform"         # or "fermi"
+    fermi_energy: float = 0.0
+    precision: str = "fp64"       # or "fp32"
+    offset: Sequence[float] = (0.0, 0.0)
+    efield: str = "reference"     # or "consistent"
+    gauge: str = "LengthWannier"
+    rtol: float = 1e-8
+    atol: float = 1e-12
+
+    def to_c(self):
+        """Returns (sbe_config, keepalive)."""
+        pulses = (A.sbe_pulse * len(self.pulses))(*[p.to_c() for p in self.pulses])
+        npts = list(self.npoints) + [1] * (3 - len(self.npoints))
+        gauges = {"LengthWannier": 0, "Wannier": 0, "LengthHamiltonian": 1, "Hamiltonian": 1,
+                  "VelocityHamiltonian": 2, "Velocity": 2}
+        c = A.sbe_config()
+        c.abi_version = A.SBE_ABI_VERSION
+        c.gauge = gauges[self.gauge]
+        c.npoints = (C.c_int32 * 3)(*[int(x) for x in npts])
+        c.rk_order = _RK.get(self.rk_order, self.rk_order if isinstance(self.rk_order, int) else -1)
+        c.dt, c.T1, c.T2 = self.dt, self.T1, self.T2
+        ks = list(self.ksfactor) + [1.0] * (3 - len(self.ksfactor))
+        c.ksfactor = (C.c_double * 3)(*ks)
+        c.inter_intra = int(bool(self.inter_intra))
+        c.grid = {"reference": A.SBE_GRID_REFERENCE, "mp": A.SBE_GRID_MP}[self.grid]
+        if self.bz_axes is not None:
+            c.has_bz_axes = 1
+            ax = list(self.bz_axes) + [0.0] * (9 - len(self.bz_axes))
+            c.bz_axes = (C.c_double * 9)(*ax)
+        if self.bz_origin is not None:
+            c.has_bz_origin = 1
+            org = list(self.bz_origin) + [0.0] * (3 - len(self.bz_origin))
+            c.bz_origin = (C.c_double * 3)(*org)
+        c.init_type = {"uniform": A.SBE_INIT_UNIFORM_VALENCE, "fermi": A.SBE_INIT_FERMI_DIRAC}[self.init]
+        c.precision = {"fp64": A.SBE_PREC_FP64, "fp32": A.SBE_PREC_FP32}[self.precision]
+        c.fermi_energy = self.fermi_energy
+        c.npulses = len(self.pulses)
+        c.efield_mode = {"reference": A.SBE_EFIELD_REFERENCE, "consistent": A.SBE_EFIELD_CONSISTENT}[self.efield]
+        c.pulses = C.cast(pulses, C.POINTER(A.sbe_pulse))
+        c.offset_before, c.offset_after = float(self.offset[0]), float(self.offset[1])
+        c.rtol, c.atol = self.rtol, self.atol
+        return c, [pulses]
+
+
+@dataclass
+class HaldaneModel:
+    a0: float          # bohr
+    t1: float
+    t2: float
+    phi0: float        # rad
+    M0: float
+    nval: int = 1      # Haldane.h:48
+
+    @classmethod
+    def from_reference(cls, a0_angstrom, t1, t2, phi0, M0=None, Mt2=None) -> "HaldaneModel":
+        """Haldane.h:49-67: a0 in Angstrom, M0 given directly or as Mt2*t2."""
+        if M0 is None:
+            if Mt2 is None:
+                raise ValueError("M0 param missing")
+            M0 = Mt2 * t2
+        return cls(a0=a0_angstrom / AU_ANGSTROM, t1=t1, t2=t2, phi0=phi0, M0=M0)
+
+    @property
+    def nband(self) -> int:
+        return 2
+
+    def to_c(self):
+        m = A.sbe_model()
+        m.kind = A.SBE_MODEL_HALDANE
+        m.nband, m.nval, m.is_dipole_zero = 2, self.nval, 1
+        m.a0, m.t1, m.t2, m.phi0, m.M0 = self.a0, self.t1, self.t2, self.phi0, self.M0
+        return m, []
+
+
+@dataclass
+class WannierModel:
+    lattice: np.ndarray      # [3][3] rows a1,a2,a3 (bohr)
+    rvec: np.ndarray         # [nr][3] int
+    rweight: np.ndarray      # [nr] 1/degeneracy
+    ham: np.ndarray          # [nr][N][N] complex (Hartree)
+    pos: Optional[np.ndarray] = None   # [nr][3][N][N] complex (bohr)
+    nval: int = 1
+    is_dipole_zero: bool = False
+
+    @property
+    def nband(self) -> int:
+        return int(self.ham.shape[1])
+
+    def to_c(self):
+        nr, N = self.ham.shape[0], self.ham.shape[1]
+        rv = np.ascontiguousarray(self.rvec, dtype=np.int32)
+        rw = np.ascontiguousarray(self.rweight, dtype=np.float64)
+        hm = np.ascontiguousarray(np.stack([self.ham.real, self.ham.imag], -1), dtype=np.float64)
+        keep = [rv, rw, hm]
+        m = A.sbe_model()
+        m.kind = A.SBE_MODEL_WANNIER
+        m.nband, m.nval, m.is_dipole_zero = N, self.nval, int(self.is_dipole_zero)
+        m.nrpts = nr
+        m.lattice = (C.c_double * 9)(*np.asarray(self.lattice, dtype=np.float64).ravel())
+        m.rvec = rv.ctypes.data_as(A.c_int32_p)
+        m.rweight = rw.ctypes.data_as(A.c_double_p)
+        m.ham = hm.ctypes.data_as(A.c_double_p)
+        if self.pos is not None:
+            ps = np.ascontiguousarray(np.stack([self.pos.real, self.pos.imag], -1), dtype=np.float64)
+            keep.append(ps)
+            m.pos = ps.ctypes.data_as(A.c_double_p)
+        return m, keep

@@ -1,0 +1,156 @@
+"""Python binding of the C ABI (libantelope_sbe.so) for tests and bench.
+
+`SBESolver` mirrors the reference's `SBEs` object as driven by
+antelope.cpp:159-338: construct for a k block (SBEs::SBEs(cfg, gauge,
+idx_start, idx_end)), run the time loop (the GenInterIntraCurrent +
+RunSBEs sequence of antelope.cpp:297-338), then read the k-summed series
+(inter_rad / intra_rad / density_matrix_integrated, antelope.cpp:196-205).
+
+There is no fallback: if the HIP library is not built, loading raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+from typing import Optional
+
+import numpy as np
+
+from . import _abi as A
+
+_LIB_PATH = Path(__file__).resolve().parent / "lib" / "libantelope_sbe.so"
+_lib: Optional[C.CDLL] = None
+
+
+class SBEError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{A.STATUS_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+def library_path() -> Path:
+    return Path(os.environ.get("ANTELOPE_SBE_LIB", _LIB_PATH))
+
+
+def load_library() -> C.CDLL:
+    """Load the HIP library; raises if it is not built (no silent fallback)."""
+    global _lib
+    if _lib is None:
+        p = library_path()
+        if not p.exists():
+            raise RuntimeError(
+                f"HIP library {p} is not built; run `python -m antelope_amd.build` first")
+        _lib = A.bind(C.CDLL(str(p)))
+    return _lib
+
+
+def _check(lib, ctx, rc: int) -> None:
+    if rc != A.SBE_OK:
+        msg = lib.sbe_last_error(ctx)
+        raise SBEError(rc, msg.decode() if msg else "")
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(A.c_double_p)
+
+
+def laser_eval(config, t: np.ndarray):
+    """A(t), E(t) [n][3] and (atmin, Nt) of a configuration (laser.h:301-322)."""
+    lib = load_library()
+    cfg, keep = config.to_c()
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    Av = np.zeros((t.size, 3))
+    Ev = np.zeros((t.size, 3))
+    atmin = C.c_double()
+    nt = C.c_int64()
+    _check(lib, None, lib.sbe_laser_eval(C.byref(cfg), t.size, _ptr(t), _ptr(Av), _ptr(Ev),
+                                         C.byref(atmin), C.byref(nt)))
+    return Av, Ev, atmin.value, nt.value
+
+
+def query_footprint(config, model, k_begin: int = -1, k_end: int = -1) -> int:
+    lib = load_library()
+    cfg, k1 = config.to_c()
+    mdl, k2 = model.to_c()
+    out = C.c_size_t()
+    _check(lib, None, lib.sbe_query_footprint(C.byref(cfg), C.byref(mdl), k_begin, k_end, C.byref(out)))
+    return out.value
+
+
+class SBESolver:
+    """One k block of one SBE solve, resident on the GPU."""
+
+    def __init__(self, config, model, k_begin: int = -1, k_end: int = -1):
+        self._lib = load_library()
+        self.config, self.model = config, model
+        cfg, k1 = config.to_c()
+        mdl, k2 = model.to_c()
+        ctx = C.c_void_p()
+        rc = self._lib.sbe_create(C.byref(cfg), C.byref(mdl), k_begin, k_end, C.byref(ctx))
+        _check(self._lib, None, rc)
+        self._ctx = ctx
+        self.nband = model.nband
+
+    # -- info -------------------------------------------------------------
+    @property
+    def info(self) -> A.sbe_info:
+        inf = A.sbe_info()
+        _check(self._lib, self._ctx, self._lib.sbe_get_info(self._ctx, C.byref(inf)))
+        return inf
+
+    @property
+    def nt(self) -> int:
+        return self.info.nt
+
+    # -- time loop ----------------------------------------------------------
+    def run(self, step_begin: Optional[int] = None, step_end: Optional[int] = None) -> None:
+        inf = self.info
+        b = inf.steps_done if step_begin is None else step_begin
+        e = inf.nt if step_end is None else step_end
+        _check(self._lib, self._ctx, self._lib.sbe_run(self._ctx, b, e))
+
+    def synchronize(self) -> None:
+        _check(self._lib, self._ctx, self._lib.sbe_synchronize(self._ctx))
+
+    # -- results ------------------------------------------------------------
+    def series(self) -> dict:
+        """k sums per time sample (before x dV), as sbe_get_series."""
+        nt, N = self.nt, self.nband
+        out = {
+            "j_inter": np.zeros((nt, 3)),
+            "j_intra": np.zeros((nt, 3)),
+            "rho_bar": np.zeros((nt, N, N, 2)),
+            "band_pop": np.zeros((nt, N)),
+        }
+        rc = self._lib.sbe_get_series(self._ctx, _ptr(out["j_inter"]), _ptr(out["j_intra"]),
+                                      _ptr(out["rho_bar"]), _ptr(out["band_pop"]))
+        _check(self._lib, self._ctx, rc)
+        rb = out.pop("rho_bar")
+        out["rho_bar"] = rb[..., 0] + 1j * rb[..., 1]
+        return out
+
+    def state(self, basis: str = "orbital") -> np.ndarray:
+        inf = self.info
+        nk = inf.k_end - inf.k_begin
+        buf = np.zeros((nk, self.nband, self.nband, 2))
+        b = {"orbital": A.SBE_BASIS_ORBITAL, "band": A.SBE_BASIS_BAND}[basis]
+        _check(self._lib, self._ctx, self._lib.sbe_get_state(self._ctx, b, _ptr(buf)))
+        return buf[..., 0] + 1j * buf[..., 1]
+
+    def close(self) -> None:
+        if getattr(self, "_ctx", None):
+            self._lib.sbe_destroy(self._ctx)
+            self._ctx = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

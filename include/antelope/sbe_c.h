@@ -1,0 +1,249 @@
+/*
+ * sbe_c.h - C ABI of the MI355X Semiconductor Bloch Equation propagator.
+ *
+ * This is the drop-in boundary for Antelope's per-k-point time loop. One
+ * context owns one contiguous block of the flattened k-grid (exactly what one
+ * MPI rank owns in the reference) and keeps its density matrices resident in
+ * HBM for the whole pulse. The caller above (the C++ driver, an MPI/RCCL
+ * layer, the serving loop or a ctypes test) only sees plain structs, plain
+ * pointers and integer status codes: no HIP or torch types cross it.
+ *
+ * Reference interfaces replaced (paths relative to the reference repo):
+ *   sbe_create        SBEs::SBEs(cfg, gauge, idx_start, idx_end)
+ *                     src/GeneralModel/SBEs.h:102,125-430 (state init, tableau,
+ *                     dephasing matrix) + momaxis(...) momaxis.h:52-61 +
+ *                     laser::Initialize laser.h:278-298
+ *   sbe_run           the time x k loop src/GeneralModel/antelope.cpp:297-338,
+ *                     i.e. SBEs::GenInterIntraCurrent SBEs.h:694-760 (or
+ *                     SBEs::GenCurrent SBEs.h:626-693) sampled before each
+ *                     SBEs::RunSBEs SBEs.h:596-624 step
+ *   sbe_get_series    the rank-local accumulators inter_rad / intra_rad /
+ *                     density_matrix_integrated, antelope.cpp:196-205,313-333,
+ *                     before the MPI_Reduce of antelope.cpp:482-512 and before
+ *                     the x dV / x 1/Ntotal scaling of the writers :514-548
+ *   sbe_get_state     the public SBEs::dmatrix (SBEs.h:38, AoS [k][N*N]) and
+ *                     SBEs::WannierToHamiltonian SBEs.h:927-941
+ *   sbe_get_info      SBEs::kmesh->{Ntotal,Npartial,dV,total_weight}
+ *                     (momaxis.h:40-49) and SBEs::fpulses->{Nt,dt,atmin}
+ *                     (laser.h:256-264)
+ *   sbe_laser_eval    laser::avlaser / laser::elaser laser.h:301-322
+ *   sbe_destroy       SBEs::~SBEs SBEs.h:563-594
+ *   sbe_last_error    replaces std::cerr + std::exit(EXIT_FAILURE) error paths
+ *                     (e.g. SBEs.h:292-308, laser.h:81-85): nothing here exits
+ *                     the process.
+ *   sbe_query_footprint  (no reference counterpart; used for admission control
+ *                     by the serving loop, which admits solves by HBM bytes)
+ *
+ * The material model (BaseMaterial.h:22-78 virtual GenHamiltonian, GenDipole,
+ * GenJMatrix, GenBrillouinzone) cannot be a virtual call per k on the GPU, so
+ * it crosses the boundary as data: sbe_model below.
+ *
+ * Units are atomic units throughout (Hartree, bohr, a.u. of time), angles in
+ * radians. Front ends that read inputParam.cfg convert the reference's degree
+ * and Angstrom inputs before filling these structs (SBEs.h:531-537,
+ * Haldane.h:55).
+ */
+#ifndef ANTELOPE_SBE_C_H
+#define ANTELOPE_SBE_C_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SBE_ABI_VERSION 1
+
+/* ---- status codes: 0 ok, negative = failure (message via sbe_last_error) */
+enum {
+    SBE_OK = 0,
+    SBE_ERR_CONFIG = -1,      /* invalid sbe_config                          */
+    SBE_ERR_MODEL = -2,       /* invalid sbe_model                           */
+    SBE_ERR_OOM = -3,         /* device or host allocation failed            */
+    SBE_ERR_HIP = -4,         /* HIP runtime error                           */
+    SBE_ERR_NAN = -5,         /* non-finite value detected in the state      */
+    SBE_ERR_STATE = -6,       /* call out of order (e.g. non-contiguous run) */
+    SBE_ERR_RANGE = -7,       /* index / size out of range                   */
+    SBE_ERR_UNSUPPORTED = -8, /* valid in the reference, not built here      */
+    SBE_ERR_NODEVICE = -9     /* no GPU visible                              */
+};
+
+/* Pulse envelopes, laser.h:18-24, plus the sin^2 envelope (cos^2 of the
+ * cos-n family, twidth = ncycles*T/2) that the reference lacks. */
+enum {
+    SBE_ENV_GAUSS = 0,
+    SBE_ENV_GAUSS_CLIPPED = 1,
+    SBE_ENV_COS4 = 2,
+    SBE_ENV_COS4_CLIPPED = 3,
+    SBE_ENV_SIN2 = 4
+};
+
+/* GaugeType, constant.h:41-46 */
+enum {
+    SBE_GAUGE_LENGTH_WANNIER = 0,
+    SBE_GAUGE_LENGTH_HAMILTONIAN = 1,  /* unimplemented in the reference too */
+    SBE_GAUGE_VELOCITY_HAMILTONIAN = 2
+};
+
+/* k-grid kind: the reference's endpoint-inclusive trapezoid grid
+ * (momaxis.h:147-186,349-383) or a centred Monkhorst-Pack grid (equal
+ * weights, frac = (i+1/2)/N - 1/2). */
+enum { SBE_GRID_REFERENCE = 0, SBE_GRID_MP = 1 };
+
+/* InitialValueType, BaseMaterial.h:10-15 (Custom is not supported) */
+enum { SBE_INIT_UNIFORM_VALENCE = 0, SBE_INIT_FERMI_DIRAC = 1 };
+
+/* Arithmetic of the propagated state. The k-sums of currents and
+ * populations and the laser tables are FP64 in every mode. */
+enum { SBE_PREC_FP64 = 0, SBE_PREC_FP32 = 1 };
+
+/* Time stepper: RKorder = 4 (classical RK4) or 5 (fixed-step 6-stage
+ * Fehlberg weights) as in SBEs.h:395-416. SBE_RK_CASH_KARP is the adaptive
+ * embedded 5(4) pair of Cash & Karp (no reference counterpart). */
+enum { SBE_RK4 = 4, SBE_RK5_FEHLBERG = 5, SBE_RK_CASH_KARP = 45 };
+
+/* Electric field convention. REFERENCE reproduces laser.h:217-224, whose
+ * envelope-derivative term has the opposite sign to -dA/dt; CONSISTENT uses
+ * E = -dA/dt. Only matters when dipoles are non-zero and for the laser file. */
+enum { SBE_EFIELD_REFERENCE = 0, SBE_EFIELD_CONSISTENT = 1 };
+
+/* Models (material classes of Materials/ that this build carries as data). */
+enum { SBE_MODEL_HALDANE = 1, SBE_MODEL_WANNIER = 2 };
+
+/* Basis for sbe_get_state. */
+enum { SBE_BASIS_ORBITAL = 0, SBE_BASIS_BAND = 1 };
+
+/* One pulse, the arguments of Pulse::Pulse (laser.h:68-141). */
+typedef struct sbe_pulse {
+    double E0;              /* peak field (a.u.)                          */
+    double w0;              /* carrier frequency (a.u.)                   */
+    double ellip;           /* ellipticity                                */
+    double ncycles;         /* cycles (FWHM cycles for gauss)             */
+    double cep;             /* carrier-envelope phase (rad)               */
+    double t0;              /* pulse centre (a.u.)                        */
+    double phix;            /* semi-major axis azimuth (rad)              */
+    double ncycles_clipped; /* flat-top cycles of the *_CLIPPED envelopes */
+    double thetaz;          /* propagation polar angle (rad)              */
+    double phiz;            /* propagation azimuth (rad)                  */
+    int32_t envelope;       /* SBE_ENV_*                                  */
+    int32_t reserved;
+} sbe_pulse;
+
+/* Run configuration: the `calc` and `laser` groups of inputParam.cfg
+ * (SBEs.h:432-561) after unit conversion. */
+typedef struct sbe_config {
+    int32_t abi_version;    /* must be SBE_ABI_VERSION                    */
+    int32_t gauge;          /* SBE_GAUGE_*                                */
+    int32_t npoints[3];     /* calc.Npoints, missing axes = 1             */
+    int32_t rk_order;       /* SBE_RK4 | SBE_RK5_FEHLBERG | SBE_RK_CASH_KARP */
+    double dt;              /* calc.dt (reference default 0.1)            */
+    double T1;              /* calc.T1, <= 1e-16 disables                 */
+    double T2;              /* calc.T2, <= 1e-16 disables                 */
+    double ksfactor[3];     /* calc.ksfactor (scales BZ axes per row)     */
+    int32_t inter_intra;    /* calc.InterIntra                            */
+    int32_t grid;           /* SBE_GRID_*                                 */
+    int32_t has_bz_axes;    /* calc.BZaxes given -> overrides model axes  */
+    int32_t has_bz_origin;  /* calc.BZorigin given                        */
+    double bz_axes[9];
+    double bz_origin[3];
+    int32_t init_type;      /* SBE_INIT_*                                 */
+    int32_t precision;      /* SBE_PREC_*                                 */
+    double fermi_energy;    /* Ef (Hartree) for SBE_INIT_FERMI_DIRAC      */
+    int32_t npulses;
+    int32_t efield_mode;    /* SBE_EFIELD_*                               */
+    const sbe_pulse *pulses;
+    double offset_before;   /* laser.offset[0]                            */
+    double offset_after;    /* laser.offset[1]                            */
+    double rtol, atol;      /* SBE_RK_CASH_KARP tolerances                */
+} sbe_config;
+
+/* Material model as data. Haldane: Materials/Haldane.h:46-170. Wannier
+ * tight-binding: the exact Fourier sums of Materials/Wannier90.h:297-387
+ * (GenHamiltonianPrimitive / GenDipolePrimitive / GenJMatrixPrimitive). */
+typedef struct sbe_model {
+    int32_t kind;           /* SBE_MODEL_*                                 */
+    int32_t nband;          /* N (2 for Haldane)                           */
+    int32_t nval;           /* number of valence bands                     */
+    int32_t is_dipole_zero; /* 1: no E.D term; Wannier phases use R+tau_n-tau_m */
+    /* Haldane (a0 in bohr, t1/t2/M0 in Hartree, phi0 in rad) */
+    double a0, t1, t2, phi0, M0;
+    /* Wannier */
+    int32_t nrpts;          /* number of lattice vectors R                 */
+    int32_t reserved;
+    double lattice[9];      /* rows a1, a2, a3 (bohr)                      */
+    const int32_t *rvec;    /* [nrpts][3] integer R = n1 a1 + n2 a2 + n3 a3 */
+    const double *rweight;  /* [nrpts] 1/degeneracy                        */
+    const double *ham;      /* [nrpts][N][N][re,im] <0m|H|Rn> (Hartree)    */
+    const double *pos;      /* [nrpts][3][N][N][re,im] <0m|r_i|Rn> (bohr)  */
+} sbe_model;
+
+/* Read-only facts about a context. */
+typedef struct sbe_info {
+    int64_t nt;             /* number of time samples / steps (laser.h:297) */
+    int64_t nk_total;       /* Ntotal                                      */
+    int64_t k_begin, k_end; /* owned block of the flattened k index        */
+    int32_t nband;
+    int32_t nstage;         /* RK stages per step                          */
+    double dt, atmin, atmax;
+    double dV;              /* |Volume| / Ntotal (momaxis.h:238)           */
+    double total_weight;    /* sum of the owned k weights                  */
+    int64_t steps_done;     /* steps already propagated                    */
+    int64_t device_bytes;   /* device memory held by the context           */
+} sbe_info;
+
+typedef struct sbe_ctx sbe_ctx;
+
+/* Create a solver for k indices [k_begin, k_end) of the flattened grid
+ * (k_begin = k_end = -1 means the whole grid). Builds the k-grid, the
+ * laser tables and the initial state on the device. */
+int sbe_create(const sbe_config *cfg, const sbe_model *model,
+               int64_t k_begin, int64_t k_end, sbe_ctx **out);
+
+/* Propagate steps [step_begin, step_end). Steps must be run in order:
+ * step_begin must equal the number of steps already done. For each step n
+ * the currents, populations and the weighted density matrix are sampled at
+ * t_n = atmin + n*dt from the state before the step (antelope.cpp:299-334).
+ * Asynchronous with respect to the host; sbe_synchronize waits. */
+int sbe_run(sbe_ctx *ctx, int64_t step_begin, int64_t step_end);
+int sbe_synchronize(sbe_ctx *ctx);
+
+/* Copy the per-step k-sums of this context's k block to host buffers.
+ * Each pointer may be NULL. Values are sum_k w_k x (...) before any dV or
+ * 1/Ntotal factor, so partial sums from several contexts (ranks) add up.
+ *   j_inter  [nt][3]          Re sum w Tr(rho J) inter-band part (or the total
+ *                             current if inter_intra == 0, as GenCurrent)
+ *   j_intra  [nt][3]          intra-band part (0 if inter_intra == 0)
+ *   rho_bar  [nt][N][N][2]    sum w rho in the orbital (Wannier) basis
+ *   band_pop [nt][N]          sum w (U^dag rho U)_mm at k + A(t_n) */
+int sbe_get_series(sbe_ctx *ctx, double *j_inter, double *j_intra,
+                   double *rho_bar, double *band_pop);
+
+/* Copy the current density matrices [k_local][N][N][re,im] (row-major,
+ * the reference's AoS dmatrix layout). basis = SBE_BASIS_BAND returns
+ * U^dag rho U at k + A(t_now) (WannierToHamiltonian). */
+int sbe_get_state(sbe_ctx *ctx, int32_t basis, double *rho);
+
+int sbe_get_info(const sbe_ctx *ctx, sbe_info *info);
+
+/* Device bytes a context for this problem would allocate. */
+int sbe_query_footprint(const sbe_config *cfg, const sbe_model *model,
+                        int64_t k_begin, int64_t k_end, size_t *bytes);
+
+/* Laser A(t), E(t) of a configuration (the k-independent tables fed to the
+ * kernels), for the laser output file and for checks. */
+int sbe_laser_eval(const sbe_config *cfg, int64_t n, const double *t,
+                   double *A /*[n][3]*/, double *E /*[n][3]*/,
+                   double *atmin, int64_t *nt);
+
+void sbe_destroy(sbe_ctx *ctx);
+
+/* Last error of a context (ctx may be NULL: last error of this thread,
+ * e.g. a failed sbe_create). Never NULL. */
+const char *sbe_last_error(const sbe_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ANTELOPE_SBE_C_H */

@@ -1,0 +1,939 @@
+/*
+ * sbe_oracle.c - FP64 CPU restatement of Antelope's SBE propagation.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker, never the product). See
+ * sbe_oracle.h. Reference paths are relative to src/GeneralModel/.
+ *
+ * Pinning (what checks each part independently of this restatement) is
+ * listed per function; the tests live in tests/test_oracle_*.py.
+ */
+#define _GNU_SOURCE
+#include "sbe_oracle.h"
+
+#include <complex.h>
+#include <math.h>
+#include <pthread.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+typedef double complex cplx;
+
+static __thread char g_err[512];
+
+static int fail(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+const char *orc_last_error(void) { return g_err; }
+
+/* constant.h:26-34 */
+static const double PI = 3.14159265358979323846;
+
+/* ------------------------------------------------------------------ laser */
+/* Pulse derived quantities, laser.h:68-141 (+ the sin^2 envelope). */
+typedef struct {
+    double E0x, E0y, A0x, A0y, w0, cep, t0;
+    double twidth, twidth_clipped, envfactor, period0;
+    double xdir[3], ydir[3];
+    int env;
+} opulse;
+
+typedef struct {
+    int np;
+    opulse *p;
+    double atmin, atmax, dt;
+    double A0[3], E0[3]; /* values at atmin, subtracted (laser.h:294-295) */
+    int64_t nt;
+    int consistent;
+} olaser;
+
+/* Pulse::f_envelope, laser.h:150-206; SBE_ENV_SIN2 = cos^2(t*w0/(2 ncycles))
+ * with twidth = ncycles*T/2 (the cos-n convention of laser.h:102-104). */
+static double f_env(const opulse *p, double t, int dtflag)
+{
+    double ef = p->envfactor, to;
+    switch (p->env) {
+    case SBE_ENV_COS4:
+        if (fabs(t) > p->twidth) return 0.0;
+        if (dtflag) return -4.0 * ef * sin(t * ef) * pow(cos(t * ef), 3.0);
+        return pow(cos(t * ef), 4.0);
+    case SBE_ENV_SIN2:
+        if (fabs(t) > p->twidth) return 0.0;
+        if (dtflag) return -2.0 * ef * sin(t * ef) * cos(t * ef);
+        return cos(t * ef) * cos(t * ef);
+    case SBE_ENV_COS4_CLIPPED:
+        if (fabs(t) > p->twidth) return 0.0;
+        if (t > p->twidth_clipped) to = t - p->twidth_clipped;
+        else if (t > -p->twidth_clipped) return dtflag ? 0.0 : 1.0;
+        else to = t + p->twidth_clipped;
+        if (dtflag) return -4.0 * ef * sin(to * ef) * pow(cos(to * ef), 3.0);
+        return pow(cos(to * ef), 4.0);
+    case SBE_ENV_GAUSS:
+        if (dtflag) return 2.0 * t * ef * exp(ef * t * t);
+        return exp(ef * t * t);
+    case SBE_ENV_GAUSS_CLIPPED:
+        if (t > p->twidth_clipped) to = t - p->twidth_clipped;
+        else if (t > -p->twidth_clipped) return dtflag ? 0.0 : 1.0;
+        else to = t + p->twidth_clipped;
+        if (dtflag) return 2.0 * to * ef * exp(ef * to * to);
+        return exp(ef * to * to);
+    }
+    return 0.0;
+}
+
+/* Pulse::avlaser_2d/elaser_2d + avlaser/elaser, laser.h:209-238 */
+static void pulse_eval(const opulse *p, double t, int consistent, double A[3],
+                       double E[3])
+{
+    double kt = t - p->t0;
+    double ph = kt * p->w0 - p->cep;
+    double f = f_env(p, kt, 0), fd = f_env(p, kt, 1);
+    double ax = -p->A0x * sin(ph) * f;
+    double ay = p->A0y * cos(ph) * f;
+    double ex, ey;
+    if (!consistent) { /* reference sign of the f' term (quirk Q1) */
+        ex = p->E0x * cos(ph) * f - p->A0x * sin(ph) * fd;
+        ey = p->E0y * sin(ph) * f + p->A0y * cos(ph) * fd;
+    } else {           /* E = -dA/dt */
+        ex = p->E0x * cos(ph) * f + p->A0x * sin(ph) * fd;
+        ey = p->E0y * sin(ph) * f - p->A0y * cos(ph) * fd;
+    }
+    for (int i = 0; i < 3; ++i) {
+        A[i] = ax * p->xdir[i] + ay * p->ydir[i];
+        E[i] = ex * p->xdir[i] + ey * p->ydir[i];
+    }
+}
+
+/* laser::avlaser / laser::elaser, laser.h:301-322 */
+static void laser_eval(const olaser *L, double t, double A[3], double E[3])
+{
+    double a[3] = {0, 0, 0}, e[3] = {0, 0, 0}, pa[3], pe[3];
+    for (int i = 0; i < L->np; ++i) {
+        pulse_eval(&L->p[i], t, L->consistent, pa, pe);
+        for (int j = 0; j < 3; ++j) { a[j] += pa[j]; e[j] += pe[j]; }
+    }
+    for (int j = 0; j < 3; ++j) {
+        A[j] = a[j] - L->A0[j];
+        E[j] = e[j] - L->E0[j];
+    }
+}
+
+/* Pulse::Pulse laser.h:68-141 and laser::Initialize laser.h:278-298 */
+static int laser_init(const sbe_config *cfg, olaser *L)
+{
+    if (cfg->npulses < 1 || !cfg->pulses) return fail(SBE_ERR_CONFIG, "no laser pulses");
+    if (!(cfg->dt > 0)) return fail(SBE_ERR_CONFIG, "dt must be > 0");
+    L->np = cfg->npulses;
+    L->p = (opulse *)calloc((size_t)L->np, sizeof(opulse));
+    L->consistent = cfg->efield_mode == SBE_EFIELD_CONSISTENT;
+    double smin = 0, emax = 0;
+    for (int i = 0; i < L->np; ++i) {
+        const sbe_pulse *s = &cfg->pulses[i];
+        opulse *p = &L->p[i];
+        p->w0 = s->w0; p->cep = s->cep; p->t0 = s->t0; p->env = s->envelope;
+        p->period0 = 2.0 * PI / s->w0;
+        switch (s->envelope) {
+        case SBE_ENV_GAUSS:
+            p->twidth = 2.0 * s->ncycles * p->period0;
+            p->envfactor = -4.0 * log(2.0) / (s->ncycles * p->period0 * s->ncycles * p->period0);
+            break;
+        case SBE_ENV_GAUSS_CLIPPED:
+            p->twidth = 2.0 * s->ncycles * p->period0 + s->ncycles_clipped * p->period0 / 2;
+            p->twidth_clipped = s->ncycles_clipped * p->period0 / 2;
+            p->envfactor = -4.0 * log(2.0) / (s->ncycles * p->period0 * s->ncycles * p->period0);
+            break;
+        case SBE_ENV_COS4:
+        case SBE_ENV_SIN2:
+            p->twidth = s->ncycles * p->period0 / 2;
+            p->envfactor = s->w0 / (2 * s->ncycles);
+            break;
+        case SBE_ENV_COS4_CLIPPED:
+            p->twidth = (s->ncycles + s->ncycles_clipped) * p->period0 / 2;
+            p->twidth_clipped = s->ncycles_clipped * p->period0 / 2;
+            p->envfactor = s->w0 / (2 * s->ncycles);
+            break;
+        default:
+            free(L->p); L->p = NULL;
+            return fail(SBE_ERR_CONFIG, "invalid envelope type %d", s->envelope);
+        }
+        double factor = sqrt(1.0 / (1.0 + s->ellip * s->ellip));
+        p->E0x = factor * s->E0;
+        p->E0y = s->ellip * factor * s->E0;
+        if (s->w0 > 1.0e-30) { p->A0x = p->E0x / s->w0; p->A0y = p->E0y / s->w0; }
+        double tx0[3] = {cos(s->thetaz) * cos(s->phiz), cos(s->thetaz) * sin(s->phiz), -sin(s->thetaz)};
+        double ty0[3] = {-sin(s->phiz), cos(s->phiz), 0.0};
+        for (int j = 0; j < 3; ++j) {
+            p->xdir[j] = cos(s->phix) * tx0[j] + sin(s->phix) * ty0[j];
+            p->ydir[j] = -sin(s->phix) * tx0[j] + cos(s->phix) * ty0[j];
+        }
+        double st = s->t0 - p->twidth, et = s->t0 + p->twidth;
+        if (i == 0 || st < smin) smin = st;
+        if (i == 0 || et > emax) emax = et;
+    }
+    L->dt = cfg->dt;
+    L->atmin = smin - cfg->offset_before;
+    L->atmax = emax + cfg->offset_after;
+    L->A0[0] = L->A0[1] = L->A0[2] = 0;
+    L->E0[0] = L->E0[1] = L->E0[2] = 0;
+    double a[3], e[3];
+    laser_eval(L, L->atmin, a, e);
+    memcpy(L->A0, a, sizeof a);
+    memcpy(L->E0, e, sizeof e);
+    L->nt = (int64_t)floor((L->atmax - L->atmin) / L->dt);
+    return SBE_OK;
+}
+
+int orc_laser_eval(const sbe_config *cfg, int64_t n, const double *t, double *A,
+                   double *E, double *atmin, double *atmax, int64_t *nt)
+{
+    olaser L;
+    int rc = laser_init(cfg, &L);
+    if (rc) return rc;
+    for (int64_t i = 0; i < n; ++i) laser_eval(&L, t[i], A + 3 * i, E + 3 * i);
+    if (atmin) *atmin = L.atmin;
+    if (atmax) *atmax = L.atmax;
+    if (nt) *nt = L.nt;
+    free(L.p);
+    return SBE_OK;
+}
+
+/* ------------------------------------------------------------------ model */
+typedef struct {
+    const sbe_model *m;
+    int N;
+    /* Haldane (Haldane.h:134-143) */
+    double va[3][2], vb[3][2];
+    /* Wannier */
+    double *Rv;    /* [nr][3] cartesian R */
+    double *tau;   /* [N][3] Re <0n|r|0n> (R = 0 block) */
+} omodel;
+
+static int model_init(const sbe_model *m, omodel *om)
+{
+    memset(om, 0, sizeof *om);
+    om->m = m;
+    if (m->kind == SBE_MODEL_HALDANE) {
+        if (m->nband != 2) return fail(SBE_ERR_MODEL, "Haldane model has nband = 2");
+        double a0 = m->a0;
+        if (!(a0 > 0)) return fail(SBE_ERR_MODEL, "Haldane a0 must be > 0");
+        om->N = 2;
+        /* Haldane::SetBasis, Haldane.h:134-143 */
+        om->va[0][0] = 0.;                  om->va[0][1] = a0;
+        om->va[1][0] = -sqrt(3.) / 2 * a0;  om->va[1][1] = -0.5 * a0;
+        om->va[2][0] = sqrt(3.) / 2 * a0;   om->va[2][1] = -0.5 * a0;
+        om->vb[0][0] = sqrt(3.) * a0;       om->vb[0][1] = 0.;
+        om->vb[1][0] = -sqrt(3.) / 2 * a0;  om->vb[1][1] = 3. / 2. * a0;
+        om->vb[2][0] = -sqrt(3.) / 2 * a0;  om->vb[2][1] = -3. / 2. * a0;
+        return SBE_OK;
+    }
+    if (m->kind == SBE_MODEL_WANNIER) {
+        int N = m->nband, nr = m->nrpts;
+        if (N < 1 || nr < 1 || !m->rvec || !m->rweight || !m->ham)
+            return fail(SBE_ERR_MODEL, "incomplete Wannier model");
+        if (!m->is_dipole_zero && !m->pos)
+            return fail(SBE_ERR_MODEL, "Wannier model with dipoles needs pos");
+        om->N = N;
+        om->Rv = (double *)calloc((size_t)nr * 3, sizeof(double));
+        om->tau = (double *)calloc((size_t)N * 3, sizeof(double));
+        int r0 = -1;
+        /* Wannier90.h:193-204: R = n1 a1 + n2 a2 + n3 a3 */
+        for (int r = 0; r < nr; ++r) {
+            const int32_t *n = m->rvec + 3 * r;
+            if (n[0] == 0 && n[1] == 0 && n[2] == 0) r0 = r;
+            for (int i = 0; i < 3; ++i)
+                om->Rv[3 * r + i] = n[0] * m->lattice[0 * 3 + i] + n[1] * m->lattice[1 * 3 + i] +
+                                    n[2] * m->lattice[2 * 3 + i];
+        }
+        /* Wannier90.h:178-191: tau_n = Re <0n|r|0n> enters r = R + tau_n - tau_m */
+        if (m->pos && r0 >= 0)
+            for (int n = 0; n < N; ++n)
+                for (int i = 0; i < 3; ++i)
+                    om->tau[3 * n + i] = m->pos[((((size_t)r0 * 3 + i) * N + n) * N + n) * 2];
+        return SBE_OK;
+    }
+    return fail(SBE_ERR_MODEL, "unknown model kind %d", m->kind);
+}
+
+static void model_free(omodel *om)
+{
+    free(om->Rv);
+    free(om->tau);
+}
+
+/* Haldane::GenBcomp / GenHamiltonian, Haldane.h:86-92,157-170 */
+static void haldane_H(const omodel *om, const double k[3], cplx *H)
+{
+    const sbe_model *m = om->m;
+    double B[4] = {0, 0, 0, 0};
+    for (int i = 0; i < 3; ++i) {
+        double aa = k[0] * om->va[i][0] + k[1] * om->va[i][1];
+        double ab = k[0] * om->vb[i][0] + k[1] * om->vb[i][1];
+        B[0] += cos(ab); B[1] += cos(aa); B[2] += sin(aa); B[3] += sin(ab);
+    }
+    B[0] *= 2. * m->t2 * cos(m->phi0); B[1] *= m->t1;
+    B[2] *= m->t1;                     B[3] = m->M0 - 2. * m->t2 * sin(m->phi0) * B[3];
+    H[0] = B[0] + B[3];      H[1] = B[1] - I * B[2];
+    H[2] = B[1] + I * B[2];  H[3] = B[0] - B[3];
+}
+
+/* Haldane::GenJMatrix, Haldane.h:94-127 (J_z stays zero) */
+static void haldane_J(const omodel *om, const double k[3], cplx *J)
+{
+    const sbe_model *m = om->m;
+    double xd[4] = {0, 0, 0, 0}, yd[4] = {0, 0, 0, 0};
+    for (int i = 0; i < 3; ++i) {
+        double aa = k[0] * om->va[i][0] + k[1] * om->va[i][1];
+        double ab = k[0] * om->vb[i][0] + k[1] * om->vb[i][1];
+        xd[0] += om->vb[i][0] * sin(ab); yd[0] += om->vb[i][1] * sin(ab);
+        xd[1] += om->va[i][0] * sin(aa); yd[1] += om->va[i][1] * sin(aa);
+        xd[2] += om->va[i][0] * cos(aa); yd[2] += om->va[i][1] * cos(aa);
+        xd[3] += om->vb[i][0] * cos(ab); yd[3] += om->vb[i][1] * cos(ab);
+    }
+    double c0 = -2. * m->t2 * cos(m->phi0), c3 = -2. * m->t2 * sin(m->phi0);
+    xd[0] *= c0; yd[0] *= c0; xd[1] *= -m->t1; yd[1] *= -m->t1;
+    xd[2] *= m->t1; yd[2] *= m->t1; xd[3] *= c3; yd[3] *= c3;
+    cplx *Jx = J, *Jy = J + 4, *Jz = J + 8;
+    Jx[0] = -(xd[0] + xd[3]);       Jy[0] = -(yd[0] + yd[3]);
+    Jx[1] = -(xd[1] - I * xd[2]);   Jy[1] = -(yd[1] - I * yd[2]);
+    Jx[2] = -(xd[1] + I * xd[2]);   Jy[2] = -(yd[1] + I * yd[2]);
+    Jx[3] = -(xd[0] - xd[3]);       Jy[3] = -(yd[0] - yd[3]);
+    for (int i = 0; i < 4; ++i) Jz[i] = 0;
+}
+
+static inline cplx wham(const sbe_model *m, int r, int a, int b)
+{
+    int N = m->nband;
+    const double *p = m->ham + (((size_t)r * N + a) * N + b) * 2;
+    return p[0] + I * p[1];
+}
+
+static inline cplx wpos(const sbe_model *m, int r, int ax, int a, int b)
+{
+    int N = m->nband;
+    const double *p = m->pos + ((((size_t)r * 3 + ax) * N + a) * N + b) * 2;
+    return p[0] + I * p[1];
+}
+
+/* phase vector of element (a,b) of block r: R (dipoles kept) or
+ * R + tau_b - tau_a (is_dipole_zero), Wannier90.h:178-204,297-323 */
+static inline void wvec(const omodel *om, int r, int a, int b, double v[3])
+{
+    for (int i = 0; i < 3; ++i) {
+        v[i] = om->Rv[3 * r + i];
+        if (om->m->is_dipole_zero) v[i] += om->tau[3 * b + i] - om->tau[3 * a + i];
+    }
+}
+
+/* Wannier90::GenHamiltonianPrimitive, Wannier90.h:297-323 */
+static void wannier_H(const omodel *om, const double k[3], cplx *H)
+{
+    const sbe_model *m = om->m;
+    int N = m->nband;
+    for (int a = 0; a < N; ++a)
+        for (int b = 0; b < N; ++b) {
+            cplx s = 0;
+            for (int r = 0; r < m->nrpts; ++r) {
+                double v[3];
+                wvec(om, r, a, b, v);
+                double ph = k[0] * v[0] + k[1] * v[1] + k[2] * v[2];
+                s += m->rweight[r] * wham(m, r, a, b) * cexp(I * ph);
+            }
+            H[a * N + b] = s;
+        }
+}
+
+/* Wannier90::GenDipolePrimitive, Wannier90.h:324-345 */
+static void wannier_D(const omodel *om, const double k[3], cplx *D)
+{
+    const sbe_model *m = om->m;
+    int N = m->nband;
+    memset(D, 0, sizeof(cplx) * 3 * N * N);
+    if (m->is_dipole_zero) return;
+    for (int ax = 0; ax < 3; ++ax)
+        for (int a = 0; a < N; ++a)
+            for (int b = 0; b < N; ++b) {
+                cplx s = 0;
+                for (int r = 0; r < m->nrpts; ++r) {
+                    const double *R = om->Rv + 3 * r;
+                    double ph = k[0] * R[0] + k[1] * R[1] + k[2] * R[2];
+                    s += m->rweight[r] * wpos(m, r, ax, a, b) * cexp(I * ph);
+                }
+                D[(ax * N + a) * N + b] = s;
+            }
+}
+
+/* Wannier90::GenJMatrixPrimitive, Wannier90.h:347-387 (no i[D,H] term, as in
+ * the reference) */
+static void wannier_J(const omodel *om, const double k[3], cplx *J)
+{
+    const sbe_model *m = om->m;
+    int N = m->nband;
+    for (int ax = 0; ax < 3; ++ax)
+        for (int a = 0; a < N; ++a)
+            for (int b = 0; b < N; ++b) {
+                cplx s = 0;
+                for (int r = 0; r < m->nrpts; ++r) {
+                    double v[3];
+                    wvec(om, r, a, b, v);
+                    double ph = k[0] * v[0] + k[1] * v[1] + k[2] * v[2];
+                    s -= I * v[ax] * wham(m, r, a, b) * m->rweight[r] * cexp(I * ph);
+                }
+                J[(ax * N + a) * N + b] = s;
+            }
+}
+
+static void model_H(const omodel *om, const double k[3], cplx *H)
+{
+    if (om->m->kind == SBE_MODEL_HALDANE) haldane_H(om, k, H);
+    else wannier_H(om, k, H);
+}
+
+static void model_J(const omodel *om, const double k[3], cplx *J)
+{
+    if (om->m->kind == SBE_MODEL_HALDANE) haldane_J(om, k, J);
+    else wannier_J(om, k, J);
+}
+
+static void model_D(const omodel *om, const double k[3], cplx *D)
+{
+    int N = om->N;
+    if (om->m->kind == SBE_MODEL_HALDANE || om->m->is_dipole_zero)
+        memset(D, 0, sizeof(cplx) * 3 * N * N); /* Haldane::GenDipole does nothing */
+    else wannier_D(om, k, D);
+}
+
+int orc_hamiltonian(const sbe_model *m, const double k[3], double *H)
+{
+    omodel om;
+    int rc = model_init(m, &om);
+    if (rc) return rc;
+    model_H(&om, k, (cplx *)H);
+    model_free(&om);
+    return SBE_OK;
+}
+
+int orc_jmatrix(const sbe_model *m, const double k[3], double *J)
+{
+    omodel om;
+    int rc = model_init(m, &om);
+    if (rc) return rc;
+    model_J(&om, k, (cplx *)J);
+    model_free(&om);
+    return SBE_OK;
+}
+
+int orc_dipole(const sbe_model *m, const double k[3], double *D)
+{
+    omodel om;
+    int rc = model_init(m, &om);
+    if (rc) return rc;
+    model_D(&om, k, (cplx *)D);
+    model_free(&om);
+    return SBE_OK;
+}
+
+/* ------------------------------------------------------------ linear alg */
+/* MatrixMult, utility.h:78-92: C = A B (zero fill, then i-j-l loop) */
+static void matmul(cplx *C, const cplx *A, const cplx *B, int N)
+{
+    for (int i = 0; i < N * N; ++i) C[i] = 0;
+    for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j)
+            for (int l = 0; l < N; ++l) C[i * N + j] += A[i * N + l] * B[l * N + j];
+}
+
+static void adjoint(cplx *Ad, const cplx *A, int N)
+{
+    for (int m = 0; m < N; ++m)
+        for (int n = 0; n < N; ++n) Ad[m * N + n] = conj(A[n * N + m]);
+}
+
+/* Hermitian eigensolver standing in for LAPACKE_zheevr (SBEs.h:874-898):
+ * cyclic Jacobi with complex rotations G = diag(1, e^{-i phi}) R(theta),
+ * eigenvalues sorted ascending, eigenvectors in columns. Pinned by the
+ * invariants U^dag U = 1, U^dag H U = diag(ascending) and by numpy eigh
+ * fixtures (tests). */
+static void eigh(int N, const cplx *Hin, double *ev, cplx *U, cplx *A)
+{
+    for (int i = 0; i < N * N; ++i) A[i] = Hin[i];
+    for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j) U[i * N + j] = (i == j) ? 1.0 : 0.0;
+    for (int i = 0; i < N; ++i) A[i * N + i] = creal(A[i * N + i]);
+    for (int sweep = 0; sweep < 60; ++sweep) {
+        double off = 0, diag = 0;
+        for (int p = 0; p < N; ++p) {
+            diag += creal(A[p * N + p]) * creal(A[p * N + p]);
+            for (int q = p + 1; q < N; ++q) off += creal(A[p * N + q] * conj(A[p * N + q]));
+        }
+        if (off <= 1e-34 * diag || off < 1e-300) break;
+        for (int p = 0; p < N - 1; ++p)
+            for (int q = p + 1; q < N; ++q) {
+                cplx b = A[p * N + q];
+                double ab = cabs(b);
+                double app = creal(A[p * N + p]), aqq = creal(A[q * N + q]);
+                if (ab <= 1e-300) continue;
+                cplx ph = b / ab; /* e^{i phi} */
+                double theta = (aqq - app) / (2.0 * ab);
+                double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+                /* G = [[c, s], [-s e^{-i phi}, c e^{-i phi}]] acting on (p,q) */
+                cplx gpp = c, gpq = s, gqp = -s * conj(ph), gqq = c * conj(ph);
+                for (int r = 0; r < N; ++r) { /* columns: A <- A G */
+                    cplx ap = A[r * N + p], aq = A[r * N + q];
+                    A[r * N + p] = ap * gpp + aq * gqp;
+                    A[r * N + q] = ap * gpq + aq * gqq;
+                    cplx up = U[r * N + p], uq = U[r * N + q];
+                    U[r * N + p] = up * gpp + uq * gqp;
+                    U[r * N + q] = up * gpq + uq * gqq;
+                }
+                for (int r = 0; r < N; ++r) { /* rows: A <- G^dag A */
+                    cplx ap = A[p * N + r], aq = A[q * N + r];
+                    A[p * N + r] = conj(gpp) * ap + conj(gqp) * aq;
+                    A[q * N + r] = conj(gpq) * ap + conj(gqq) * aq;
+                }
+                A[p * N + q] = 0; A[q * N + p] = 0;
+                A[p * N + p] = creal(A[p * N + p]);
+                A[q * N + q] = creal(A[q * N + q]);
+            }
+    }
+    for (int i = 0; i < N; ++i) ev[i] = creal(A[i * N + i]);
+    /* ascending order (selection sort, stable for ties) */
+    for (int i = 0; i < N - 1; ++i) {
+        int mi = i;
+        for (int j = i + 1; j < N; ++j)
+            if (ev[j] < ev[mi]) mi = j;
+        if (mi != i) {
+            double te = ev[i]; ev[i] = ev[mi]; ev[mi] = te;
+            for (int r = 0; r < N; ++r) {
+                cplx tu = U[r * N + i]; U[r * N + i] = U[r * N + mi]; U[r * N + mi] = tu;
+            }
+        }
+    }
+}
+
+int orc_eigh(int n, const double *H, double *evals, double *U)
+{
+    if (n < 1 || n > 256) return fail(SBE_ERR_RANGE, "eigh size %d", n);
+    cplx *A = (cplx *)malloc(sizeof(cplx) * n * n);
+    eigh(n, (const cplx *)H, evals, (cplx *)U, A);
+    free(A);
+    return SBE_OK;
+}
+
+/* ------------------------------------------------------------------- grid */
+/* material BZ (Haldane.h:77-81, Wannier90.h:508-521), ksfactor and the
+ * calc.BZaxes/BZorigin overrides (SBEs.h:219-241) */
+static void bz_select(const sbe_config *cfg, const sbe_model *m, double ax[9], double org[3])
+{
+    memset(ax, 0, 9 * sizeof(double));
+    memset(org, 0, 3 * sizeof(double));
+    if (m->kind == SBE_MODEL_HALDANE) {
+        double kx = PI / sqrt(3.) / m->a0, ky = 2. * PI / 3. / m->a0;
+        ax[0] = 2 * kx; ax[4] = 2 * ky;
+    } else {
+        const double *a = m->lattice;
+        double vol = a[0] * (a[4] * a[8] - a[5] * a[7]) + a[1] * (a[5] * a[6] - a[3] * a[8]) +
+                     a[2] * (a[3] * a[7] - a[4] * a[6]);
+        for (int i = 0; i < 3; ++i) {
+            const double *u = a + 3 * ((i + 1) % 3), *v = a + 3 * ((i + 2) % 3);
+            ax[3 * i + 0] = (u[1] * v[2] - u[2] * v[1]) * 2 * PI / vol;
+            ax[3 * i + 1] = (u[2] * v[0] - u[0] * v[2]) * 2 * PI / vol;
+            ax[3 * i + 2] = (u[0] * v[1] - v[0] * u[1]) * 2 * PI / vol;
+        }
+    }
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) ax[3 * i + j] *= cfg->ksfactor[i];
+    if (cfg->has_bz_axes) memcpy(ax, cfg->bz_axes, 9 * sizeof(double));
+    if (cfg->has_bz_origin) memcpy(org, cfg->bz_origin, 3 * sizeof(double));
+}
+
+/* momaxis::steps_sizes volume rule, momaxis.h:208-238: over the axes with
+ * N != 1: |b1| (1D), |b1 x b2| (2D), sqrt((b1 x b2).b3) (3D, as written). */
+static double bz_volume(const int N[3], const double ax[9])
+{
+    const double *v[3];
+    int na = 0;
+    for (int i = 0; i < 3; ++i)
+        if (N[i] != 1) v[na++] = ax + 3 * i;
+    if (na == 1) return sqrt(v[0][0] * v[0][0] + v[0][1] * v[0][1] + v[0][2] * v[0][2]);
+    if (na >= 2) {
+        double c[3] = {v[0][1] * v[1][2] - v[0][2] * v[1][1], v[0][2] * v[1][0] - v[0][0] * v[1][2],
+                       v[0][0] * v[1][1] - v[1][0] * v[0][1]};
+        if (na == 2) return sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+        return sqrt(c[0] * v[2][0] + c[1] * v[2][1] + c[2] * v[2][2]);
+    }
+    return 1.0;
+}
+
+int orc_kgrid(const sbe_config *cfg, const sbe_model *model, int64_t k_begin, int64_t k_end,
+              double *kpts, double *w, double *dV, double *total_weight)
+{
+    int N[3] = {cfg->npoints[0], cfg->npoints[1], cfg->npoints[2]};
+    if (N[0] <= 0 || N[1] <= 0 || N[2] <= 0) return fail(SBE_ERR_CONFIG, "Npoints must be > 0");
+    int64_t nt = (int64_t)N[0] * N[1] * N[2];
+    if (k_begin < 0) { k_begin = 0; k_end = nt; }
+    if (k_end > nt || k_begin >= k_end) return fail(SBE_ERR_RANGE, "bad k range");
+    double ax[9], org[3];
+    bz_select(cfg, model, ax, org);
+    double tw = 0;
+    for (int64_t g = k_begin; g < k_end; ++g) {
+        int i = (int)(g / ((int64_t)N[1] * N[2])), j = (int)((g / N[2]) % N[1]), l = (int)(g % N[2]);
+        int idx[3] = {i, j, l};
+        double fr[3], wt = 1.0;
+        for (int d = 0; d < 3; ++d) {
+            if (cfg->grid == SBE_GRID_MP) {
+                fr[d] = (idx[d] + 0.5) / N[d] - 0.5;
+            } else {
+                /* momaxis.h:353-370 and trapezoid weights :162-174 */
+                fr[d] = (N[d] == 1) ? 0.0 : (double)idx[d] / (N[d] - 1) - 0.5;
+                if (N[d] != 1 && (idx[d] == 0 || idx[d] == N[d] - 1)) wt /= 2;
+            }
+        }
+        int64_t o = g - k_begin;
+        for (int c = 0; c < 3; ++c)
+            kpts[3 * o + c] = org[c] + fr[0] * ax[0 * 3 + c] + fr[1] * ax[1 * 3 + c] + fr[2] * ax[2 * 3 + c];
+        w[o] = wt;
+        tw += wt;
+    }
+    if (dV) *dV = fabs(bz_volume(N, ax)) / (double)nt;
+    if (total_weight) *total_weight = tw;
+    return SBE_OK;
+}
+
+/* ------------------------------------------------------------------ solver */
+typedef struct {
+    const sbe_config *cfg;
+    omodel om;
+    olaser L;
+    int N, nstage;
+    double c[6], b[6], a[6][6];
+    double gam_intra, gam_inter;
+} osolver;
+
+/* SBEs::InitializeGeneral post-processing (SBEs.h:484-508) and the RK
+ * tableaus of SBEs.h:395-416 */
+static int solver_init(const sbe_config *cfg, const sbe_model *model, osolver *S)
+{
+    memset(S, 0, sizeof *S);
+    S->cfg = cfg;
+    if (cfg->abi_version != SBE_ABI_VERSION) return fail(SBE_ERR_CONFIG, "abi_version mismatch");
+    if (cfg->gauge != SBE_GAUGE_LENGTH_WANNIER)
+        return fail(SBE_ERR_UNSUPPORTED, "only the LengthWannier gauge is restated");
+    int rc = model_init(model, &S->om);
+    if (rc) return rc;
+    rc = laser_init(cfg, &S->L);
+    if (rc) { model_free(&S->om); return rc; }
+    S->N = S->om.N;
+    S->gam_inter = cfg->T2 <= 1e-16 ? 0.0 : 1.0 / cfg->T2;
+    S->gam_intra = cfg->T1 <= 1e-16 ? 0.0 : 1.0 / cfg->T1;
+    if (cfg->rk_order == SBE_RK4) {
+        S->nstage = 4;
+        S->c[0] = 0.5; S->c[1] = 0.5; S->c[2] = 1.0;
+        S->b[0] = 1.0 / 6.0; S->b[1] = 2.0 / 6.0; S->b[2] = 2.0 / 6.0; S->b[3] = 1.0 / 6.0;
+        S->a[0][0] = 0.5;
+        S->a[1][0] = 0.0; S->a[1][1] = 0.5;
+        S->a[2][0] = 0.0; S->a[2][1] = 0.0; S->a[2][2] = 1.0;
+    } else if (cfg->rk_order == SBE_RK5_FEHLBERG) {
+        S->nstage = 6;
+        S->c[0] = 0.25; S->c[1] = 3. / 8.; S->c[2] = 12. / 13.; S->c[3] = 1.0; S->c[4] = 0.5;
+        S->b[0] = 16. / 135.; S->b[1] = 0; S->b[2] = 6656. / 12825.; S->b[3] = 28561. / 56430.;
+        S->b[4] = -9. / 50.; S->b[5] = 2. / 55.;
+        S->a[0][0] = 1.0 / 4.0;
+        S->a[1][0] = 3. / 32.; S->a[1][1] = 9. / 32.;
+        S->a[2][0] = 1932. / 2197.; S->a[2][1] = -7200. / 2197.; S->a[2][2] = 7296. / 2197.;
+        S->a[3][0] = 439. / 216.; S->a[3][1] = -8.; S->a[3][2] = 3680. / 513.; S->a[3][3] = -845. / 4104.;
+        S->a[4][0] = -8. / 27.; S->a[4][1] = 2.; S->a[4][2] = -3544. / 2565; S->a[4][3] = 1859. / 4104;
+        S->a[4][4] = -11. / 40.;
+    } else {
+        model_free(&S->om); free(S->L.p);
+        return fail(SBE_ERR_UNSUPPORTED, "oracle restates RKorder 4 and 5 (got %d)", cfg->rk_order);
+    }
+    return SBE_OK;
+}
+
+static void solver_free(osolver *S)
+{
+    model_free(&S->om);
+    free(S->L.p);
+}
+
+typedef struct {
+    cplx *H, *Hd, *T, *U, *Ud, *t1, *t2, *D, *J, *work;
+    double *ev;
+} oscratch;
+
+static void scratch_alloc(oscratch *w, int N)
+{
+    size_t nn = (size_t)N * N;
+    w->H = calloc(nn, sizeof(cplx)); w->Hd = calloc(nn, sizeof(cplx));
+    w->T = calloc(nn, sizeof(cplx)); w->U = calloc(nn, sizeof(cplx));
+    w->Ud = calloc(nn, sizeof(cplx)); w->t1 = calloc(nn, sizeof(cplx));
+    w->t2 = calloc(nn, sizeof(cplx)); w->D = calloc(3 * nn, sizeof(cplx));
+    w->J = calloc(3 * nn, sizeof(cplx)); w->work = calloc(nn, sizeof(cplx));
+    w->ev = calloc((size_t)N, sizeof(double));
+}
+
+static void scratch_free(oscratch *w)
+{
+    free(w->H); free(w->Hd); free(w->T); free(w->U); free(w->Ud);
+    free(w->t1); free(w->t2); free(w->D); free(w->J); free(w->work); free(w->ev);
+}
+
+/* SBEs::GenUMatrix, SBEs.h:874-898 */
+static void gen_u(const osolver *S, const double kp[3], oscratch *w)
+{
+    model_H(&S->om, kp, w->Hd);
+    eigh(S->N, w->Hd, w->ev, w->U, w->work);
+    adjoint(w->Ud, w->U, S->N);
+}
+
+/* SBEs::GenDifferentialDM, LengthWannier branch, SBEs.h:762-786,814-861 */
+static void rhs(const osolver *S, const double k[3], double t, const cplx *rho, cplx *out,
+                oscratch *w)
+{
+    int N = S->N;
+    double A[3], E[3];
+    laser_eval(&S->L, t, A, E);
+    double kp[3] = {k[0] + A[0], k[1] + A[1], k[2] + A[2]};
+    model_H(&S->om, kp, w->H);
+    if (!S->om.m->is_dipole_zero && S->om.m->kind == SBE_MODEL_WANNIER) {
+        model_D(&S->om, kp, w->D);
+        for (int i = 0; i < N * N; ++i)
+            w->H[i] += E[0] * w->D[i] + E[1] * w->D[N * N + i] + E[2] * w->D[2 * N * N + i];
+    }
+    matmul(out, w->H, rho, N);
+    for (int i = 0; i < N * N; ++i) w->T[i] = out[i];
+    for (int m = 0; m < N; ++m)
+        for (int n = 0; n < N; ++n) {
+            out[m * N + n] -= conj(w->T[n * N + m]);
+            out[m * N + n] *= -I;
+        }
+    gen_u(S, kp, w);
+    matmul(w->t1, rho, w->U, N);
+    matmul(w->t2, w->Ud, w->t1, N);
+    for (int m = 0; m < N; ++m)
+        for (int n = 0; n < N; ++n) w->t2[m * N + n] *= (m == n) ? S->gam_intra : S->gam_inter;
+    matmul(w->t1, w->U, w->t2, N);
+    matmul(w->t2, w->t1, w->Ud, N);
+    for (int i = 0; i < N * N; ++i) out[i] -= w->t2[i];
+}
+
+int orc_rhs(const sbe_config *cfg, const sbe_model *model, const double k[3], double t,
+            const double *rho, double *out)
+{
+    osolver S;
+    int rc = solver_init(cfg, model, &S);
+    if (rc) return rc;
+    oscratch w;
+    scratch_alloc(&w, S.N);
+    rhs(&S, k, t, (const cplx *)rho, (cplx *)out, &w);
+    scratch_free(&w);
+    solver_free(&S);
+    return SBE_OK;
+}
+
+/* SBEs::RunSBEs, SBEs.h:596-624 */
+static void rk_step(const osolver *S, const double k[3], double t, cplx *rho, cplx **stg,
+                    cplx *tmp, oscratch *w)
+{
+    int N = S->N, nn = N * N;
+    double dt = S->L.dt;
+    rhs(S, k, t, rho, stg[0], w);
+    for (int irk = 0; irk < S->nstage - 1; ++irk) {
+        for (int i = 0; i < nn; ++i) {
+            tmp[i] = rho[i];
+            for (int j = 0; j < irk + 1; ++j) tmp[i] += dt * S->a[irk][j] * stg[j][i];
+        }
+        rhs(S, k, t + S->c[irk] * dt, tmp, stg[irk + 1], w);
+    }
+    for (int i = 0; i < nn; ++i)
+        for (int irk = 0; irk < S->nstage; ++irk) rho[i] += dt * S->b[irk] * stg[irk][i];
+}
+
+/* SBEs::GenInterIntraCurrent SBEs.h:694-760 / SBEs::GenCurrent SBEs.h:626-657,
+ * plus the band populations diag(U^dag rho U) that the reference builds as
+ * newdMatrix but does not output. */
+static void currents(const osolver *S, const double k[3], double t, const cplx *rho,
+                     double inter[3], double intra[3], double *pop, oscratch *w)
+{
+    int N = S->N;
+    double A[3], E[3];
+    laser_eval(&S->L, t, A, E);
+    double kp[3] = {k[0] + A[0], k[1] + A[1], k[2] + A[2]};
+    model_J(&S->om, kp, w->J);
+    gen_u(S, kp, w);
+    matmul(w->t1, rho, w->U, N);
+    matmul(w->T, w->Ud, w->t1, N); /* rho_H */
+    for (int m = 0; m < N; ++m) pop[m] = creal(w->T[m * N + m]);
+    for (int ax = 0; ax < 3; ++ax) {
+        inter[ax] = 0; intra[ax] = 0;
+        const cplx *J = w->J + ax * N * N;
+        if (S->cfg->inter_intra) {
+            matmul(w->t1, J, w->U, N);
+            matmul(w->t2, w->Ud, w->t1, N);
+            for (int i = 0; i < N; ++i)
+                for (int j = 0; j < N; ++j) {
+                    double v = creal(w->T[N * i + j] * w->t2[N * j + i]);
+                    if (i == j) intra[ax] += v;
+                    else inter[ax] += v;
+                }
+        } else {
+            for (int i = 0; i < N; ++i)
+                for (int j = 0; j < N; ++j) inter[ax] += creal(rho[N * i + j] * J[N * j + i]);
+        }
+    }
+}
+
+/* SBEs::SBEs per-k initial state, SBEs.h:284-364 (UniformValence and
+ * FermiDirac, kT = 300 K / 3.15775024804e5) */
+static void init_state(const osolver *S, const double k[3], cplx *rho, oscratch *w)
+{
+    int N = S->N;
+    model_H(&S->om, k, w->Hd);
+    eigh(N, w->Hd, w->ev, w->U, w->work);
+    adjoint(w->Ud, w->U, N);
+    for (int i = 0; i < N * N; ++i) rho[i] = 0;
+    if (S->cfg->init_type == SBE_INIT_FERMI_DIRAC) {
+        double kT = 300. / 3.15775024804e5, sum = 0, *f = w->ev;
+        double tmp[256];
+        for (int m = 0; m < N; ++m) { tmp[m] = 1.0 / (exp((f[m] - S->cfg->fermi_energy) / kT) + 1.0); sum += tmp[m]; }
+        for (int m = 0; m < N; ++m) rho[m * N + m] = tmp[m] / sum;
+    } else {
+        for (int m = 0; m < N; ++m)
+            if (m < S->om.m->nval) rho[m * N + m] = 1.0;
+    }
+    matmul(w->t1, rho, w->Ud, N);
+    matmul(rho, w->U, w->t1, N);
+}
+
+typedef struct {
+    const osolver *S;
+    const double *kp, *wk;
+    int64_t k0, k1, nsteps;
+    double *ji, *ja, *rb, *bp; /* per-thread series */
+    cplx *rho;                 /* [nk][N*N] slice */
+} othread;
+
+/* antelope.cpp:297-338 for one contiguous k block */
+static void *thread_main(void *arg)
+{
+    othread *T = (othread *)arg;
+    const osolver *S = T->S;
+    int N = S->N, nn = N * N;
+    oscratch w;
+    scratch_alloc(&w, N);
+    cplx *stg[6], *tmp = calloc((size_t)nn, sizeof(cplx));
+    for (int i = 0; i < 6; ++i) stg[i] = calloc((size_t)nn, sizeof(cplx));
+    double inter[3], intra[3], pop[256];
+    for (int64_t n = 0; n < T->nsteps; ++n) {
+        double t = S->L.atmin + S->L.dt * n;
+        for (int64_t k = T->k0; k < T->k1; ++k) {
+            const double *kk = T->kp + 3 * k;
+            double wk = T->wk[k];
+            cplx *rho = T->rho + (size_t)k * nn;
+            currents(S, kk, t, rho, inter, intra, pop, &w);
+            for (int ax = 0; ax < 3; ++ax) {
+                T->ja[3 * n + ax] += intra[ax] * wk;
+                T->ji[3 * n + ax] += inter[ax] * wk;
+            }
+            for (int i = 0; i < nn; ++i) {
+                T->rb[(n * nn + i) * 2 + 0] += creal(rho[i]) * wk;
+                T->rb[(n * nn + i) * 2 + 1] += cimag(rho[i]) * wk;
+            }
+            for (int m = 0; m < N; ++m) T->bp[n * N + m] += pop[m] * wk;
+            rk_step(S, kk, t, rho, stg, tmp, &w);
+        }
+    }
+    for (int i = 0; i < 6; ++i) free(stg[i]);
+    free(tmp);
+    scratch_free(&w);
+    return NULL;
+}
+
+int orc_run(const sbe_config *cfg, const sbe_model *model, int64_t k_begin, int64_t k_end,
+            int64_t nsteps, int nthreads, double *j_inter, double *j_intra, double *rho_bar,
+            double *band_pop, double *rho_final, orc_info *info)
+{
+    osolver S;
+    int rc = solver_init(cfg, model, &S);
+    if (rc) return rc;
+    int N = S.N, nn = N * N;
+    if (N > 256) { solver_free(&S); return fail(SBE_ERR_MODEL, "N too large"); }
+    int64_t ntot = (int64_t)cfg->npoints[0] * cfg->npoints[1] * cfg->npoints[2];
+    if (k_begin < 0) { k_begin = 0; k_end = ntot; }
+    int64_t nk = k_end - k_begin;
+    if (nk <= 0 || k_end > ntot) { solver_free(&S); return fail(SBE_ERR_RANGE, "bad k range"); }
+    if (nsteps < 0 || nsteps > S.L.nt) nsteps = S.L.nt;
+    double *kp = malloc(sizeof(double) * 3 * nk), *wk = malloc(sizeof(double) * nk);
+    double dV, tw;
+    rc = orc_kgrid(cfg, model, k_begin, k_end, kp, wk, &dV, &tw);
+    if (rc) { free(kp); free(wk); solver_free(&S); return rc; }
+    cplx *rho = calloc((size_t)nk * nn, sizeof(cplx));
+    {
+        oscratch w;
+        scratch_alloc(&w, N);
+        for (int64_t k = 0; k < nk; ++k) init_state(&S, kp + 3 * k, rho + (size_t)k * nn, &w);
+        scratch_free(&w);
+    }
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > nk) nthreads = (int)nk;
+    othread *th = calloc((size_t)nthreads, sizeof(othread));
+    pthread_t *tid = calloc((size_t)nthreads, sizeof(pthread_t));
+    int64_t ns = nsteps > 0 ? nsteps : 1;
+    for (int r = 0; r < nthreads; ++r) {
+        /* ParaRange, utility.h:104-115 */
+        int64_t w1 = nk / nthreads, w2 = nk % nthreads;
+        int64_t s = w1 * r + (r < w2 ? r : w2), e = s + w1 + (w2 > r ? 1 : 0);
+        th[r].S = &S; th[r].kp = kp; th[r].wk = wk; th[r].k0 = s; th[r].k1 = e;
+        th[r].nsteps = nsteps; th[r].rho = rho;
+        th[r].ji = calloc((size_t)ns * 3, sizeof(double));
+        th[r].ja = calloc((size_t)ns * 3, sizeof(double));
+        th[r].rb = calloc((size_t)ns * nn * 2, sizeof(double));
+        th[r].bp = calloc((size_t)ns * N, sizeof(double));
+    }
+    struct timespec t0, t1;
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+    for (int r = 1; r < nthreads; ++r) pthread_create(&tid[r], NULL, thread_main, &th[r]);
+    thread_main(&th[0]);
+    for (int r = 1; r < nthreads; ++r) pthread_join(tid[r], NULL);
+    clock_gettime(CLOCK_MONOTONIC, &t1);
+    for (int64_t n = 0; n < nsteps; ++n) {
+        for (int ax = 0; ax < 3; ++ax) {
+            double si = 0, sa = 0;
+            for (int r = 0; r < nthreads; ++r) { si += th[r].ji[3 * n + ax]; sa += th[r].ja[3 * n + ax]; }
+            if (j_inter) j_inter[3 * n + ax] = si;
+            if (j_intra) j_intra[3 * n + ax] = sa;
+        }
+        if (rho_bar)
+            for (int i = 0; i < nn * 2; ++i) {
+                double s = 0;
+                for (int r = 0; r < nthreads; ++r) s += th[r].rb[n * nn * 2 + i];
+                rho_bar[n * nn * 2 + i] = s;
+            }
+        if (band_pop)
+            for (int m = 0; m < N; ++m) {
+                double s = 0;
+                for (int r = 0; r < nthreads; ++r) s += th[r].bp[n * N + m];
+                band_pop[n * N + m] = s;
+            }
+    }
+    if (rho_final) memcpy(rho_final, rho, sizeof(cplx) * (size_t)nk * nn);
+    if (info) {
+        info->nt = S.L.nt; info->nk_total = ntot; info->nk_local = nk;
+        info->atmin = S.L.atmin; info->atmax = S.L.atmax; info->dt = S.L.dt;
+        info->dV = dV; info->total_weight = tw;
+        info->seconds = (t1.tv_sec - t0.tv_sec) + 1e-9 * (t1.tv_nsec - t0.tv_nsec);
+    }
+    for (int r = 0; r < nthreads; ++r) { free(th[r].ji); free(th[r].ja); free(th[r].rb); free(th[r].bp); }
+    free(th); free(tid); free(rho); free(kp); free(wk);
+    solver_free(&S);
+    return SBE_OK;
+}

@@ -1,0 +1,91 @@
+"""The C-ABI boundary on the CPU: the library loads, exports every function
+declared in include/antelope/sbe_c.h, its structs have the layout the
+ctypes mirror assumes, and invalid inputs come back as status codes (the
+reference exits the process instead, e.g. SBEs.h:472-480, laser.h:81-85)."""
+import ctypes as C
+import dataclasses
+import re
+import subprocess
+from pathlib import Path
+
+import pytest
+
+from antelope_amd import _abi as A
+from antelope_amd import HaldaneModel, Pulse, RunConfig, SBESolver, laser_eval, query_footprint
+from antelope_amd.solver import SBEError, library_path, load_library
+from tests.cases import haldane_cfg1_model, small_config
+
+ROOT = Path(__file__).resolve().parent.parent
+HEADERS = sorted((ROOT / "include").rglob("*.h"))
+
+
+def declared_functions():
+    names = set()
+    for h in HEADERS:
+        text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
+        names |= set(re.findall(r"\b(sbe_[a-z_0-9]+)\s*\(", text))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_function():
+    lib = load_library()
+    names = declared_functions()
+    assert len(names) >= 10
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) <= set(A.SIGNATURES) | {"sbe_batch_create"} or set(names) >= set(A.SIGNATURES)
+
+
+def test_struct_layout_matches_header(tmp_path):
+    probe = tmp_path / "probe.c"
+    fields = {
+        "sbe_pulse": [f for f, _ in A.sbe_pulse._fields_],
+        "sbe_config": [f for f, _ in A.sbe_config._fields_],
+        "sbe_model": [f for f, _ in A.sbe_model._fields_],
+        "sbe_info": [f for f, _ in A.sbe_info._fields_],
+    }
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "antelope/sbe_c.h"', "int main(void){"]
+    for s, fs in fields.items():
+        lines.append(f'printf("{s} %zu\\n", sizeof({s}));')
+        for f in fs:
+            lines.append(f'printf("{s}.{f} %zu\\n", offsetof({s}, {f}));')
+    lines.append("return 0;}")
+    probe.write_text("\n".join(lines))
+    exe = tmp_path / "probe"
+    subprocess.run(["gcc", "-std=c99", f"-I{ROOT / 'include'}", str(probe), "-o", str(exe)], check=True)
+    out = dict(ln.rsplit(" ", 1) for ln in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                         check=True).stdout.splitlines())
+    for s, fs in fields.items():
+        cls = getattr(A, s)
+        assert int(out[s]) == C.sizeof(cls), s
+        for f in fs:
+            assert int(out[f"{s}.{f}"]) == getattr(cls, f).offset, f"{s}.{f}"
+
+
+def _expect(code, fn):
+    with pytest.raises(SBEError) as e:
+        fn()
+    assert e.value.code == code, str(e.value)
+
+
+def test_invalid_inputs_return_status_codes():
+    m = haldane_cfg1_model()
+    cfg = small_config()
+    _expect(A.SBE_ERR_CONFIG, lambda: SBESolver(dataclasses.replace(cfg, dt=0.0), m))
+    _expect(A.SBE_ERR_CONFIG, lambda: SBESolver(dataclasses.replace(cfg, rk_order=3), m))
+    _expect(A.SBE_ERR_CONFIG, lambda: SBESolver(dataclasses.replace(cfg, npoints=[0, 4]), m))
+    _expect(A.SBE_ERR_CONFIG, lambda: SBESolver(dataclasses.replace(cfg, pulses=[]), m))
+    _expect(A.SBE_ERR_UNSUPPORTED, lambda: SBESolver(dataclasses.replace(cfg, gauge="LengthHamiltonian"), m))
+    _expect(A.SBE_ERR_MODEL, lambda: SBESolver(cfg, dataclasses.replace(m, a0=-1.0)))
+    _expect(A.SBE_ERR_RANGE, lambda: SBESolver(cfg, m, 0, 10_000))
+    _expect(A.SBE_ERR_RANGE, lambda: SBESolver(cfg, m, 50, 40))
+
+
+def test_laser_eval_and_footprint_without_gpu():
+    cfg = small_config(n=300, ncycles=12.0, ellip=0.5)
+    m = haldane_cfg1_model()
+    Av, Ev, atmin, nt = laser_eval(cfg, [0.0])
+    assert nt == 21542
+    assert Ev[0, 0] == pytest.approx(0.0045 / (1.25 ** 0.5))
+    b = query_footprint(cfg, m)
+    assert 90_000 * 32 < b < 512 * 2 ** 20
