@@ -104,6 +104,8 @@ SIGNATURES = {
                              C.POINTER(C.c_size_t)], C.c_int),
     "sbe_laser_eval": ([C.POINTER(sbe_config), C.c_int64, c_double_p, c_double_p, c_double_p,
                         c_double_p, C.POINTER(C.c_int64)], C.c_int),
+    "sbe_set_timing": ([C.c_void_p, C.c_int32], C.c_int),
+    "sbe_get_timing": ([C.c_void_p, c_double_p, C.POINTER(C.c_int64)], C.c_int),
     "sbe_destroy": ([C.c_void_p], None),
     "sbe_last_error": ([C.c_void_p], C.c_char_p),
 }

@@ -120,25 +120,6 @@ __device__ __forceinline__ Vec3<R> dvec(const R er[6], const R ei[6], R t1, R c3
     return d;
 }
 
-// Bloch RHS: ds = 2 d x s - g2 (s - s_par) - g1 s_par;  dn = -g1 n
-template <typename R, bool T1ON>
-__device__ __forceinline__ void rhs(const Vec3<R> &d, R n, const Vec3<R> &s, R g1, R g2, R &dn,
-                                    Vec3<R> &ds)
-{
-    ds.x = R(2) * (d.y * s.z - d.z * s.y);
-    ds.y = R(2) * (d.z * s.x - d.x * s.z);
-    ds.z = R(2) * (d.x * s.y - d.y * s.x);
-    const R dd = d.x * d.x + d.y * d.y + d.z * d.z;
-    const R inv = dd > R(0) ? R(1) / dd : R(0);
-    const R sp = (s.x * d.x + s.y * d.y + s.z * d.z) * inv;  // s_par = sp * d
-    // g2 (s - sp d) + g1 sp d = g2 s + (g1 - g2) sp d
-    const R cpar = (g1 - g2) * sp;
-    ds.x -= g2 * s.x + cpar * d.x;
-    ds.y -= g2 * s.y + cpar * d.y;
-    ds.z -= g2 * s.z + cpar * d.z;
-    dn = T1ON ? -g1 * n : R(0);
-}
-
 template <typename R> struct StateT;
 template <> struct StateT<double> {
     using T = double;
@@ -158,169 +139,240 @@ __device__ __forceinline__ void cmul(R a, R b, R c, R d, R &re, R &im)
 }
 
 // ------------------------------------------------------------ main kernel
-// R: arithmetic of the propagated state (double, or float with a Kahan
-// compensation term per component). The time samples are always FP64.
-template <typename R, int RK, bool T1ON>
-__global__ __launch_bounds__(kBlock) void haldane_steps_kernel(HaldaneArgs a, const R *phase_tab_r)
+// Per-k registers of one lane.
+template <typename R>
+struct KPoint {
+    R er[6], ei[6];     // e^{i k.v_j} (a_0..a_2, b_0..b_2)
+    R n, sx, sy, sz;    // Bloch state
+    R cn, cx, cy, cz;   // Kahan compensation (FP32 state only)
+    double w;
+};
+
+// Time sample of one k point at t_n (antelope.cpp:307-333 with
+// SBEs.h:694-760): adds w x (inter_x, inter_y, intra_x, intra_y, s_x, s_y,
+// s_z, s.dhat) to v (FP64) and returns d(k + A(t_n)) for RK stage 0.
+// Evaluated in the state precision R; the k sum is always FP64. Branch-free
+// so the scheduler can interleave it with the RK chain.
+template <typename R>
+__device__ __forceinline__ Vec3<R> time_sample(const KPoint<R> &p, const R *row, const HaldaneArgs &a, R ii,
+                                               double v[8])
+{
+    R ar[3], ai[3], br[3], bi[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        cmul(p.er[j], p.ei[j], row[2 * j], row[2 * j + 1], ar[j], ai[j]);  // t1 e^{i(k+A).a_j}
+        cmul(p.er[j + 3], p.ei[j + 3], row[6 + 2 * j], row[7 + 2 * j], br[j], bi[j]);
+    }
+    const R c3 = static_cast<R>(a.c3);
+    Vec3<R> d{ar[0] + ar[1] + ar[2], ai[0] + ai[1] + ai[2], static_cast<R>(a.M0) - c3 * (bi[0] + bi[1] + bi[2])};
+    // derivatives d/dk_x, d/dk_y of (d0, dx, dy, dz)
+    R xx = 0, xy = 0, yx = 0, yy = 0, zx = 0, zy = 0, ox = 0, oy = 0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const R ax = static_cast<R>(a.va[j][0]), ay = static_cast<R>(a.va[j][1]);
+        const R bx = static_cast<R>(a.vb[j][0]), by = static_cast<R>(a.vb[j][1]);
+        xx -= ax * ai[j];
+        xy -= ay * ai[j];
+        yx += ax * ar[j];
+        yy += ay * ar[j];
+        zx += bx * br[j];
+        zy += by * br[j];
+        ox += bx * bi[j];
+        oy += by * bi[j];
+    }
+    zx *= -c3;
+    zy *= -c3;
+    const R c0 = static_cast<R>(a.c0);
+    ox *= -c0;
+    oy *= -c0;
+    const R nn = p.n - p.cn, s_x = p.sx - p.cx, s_y = p.sy - p.cy, s_z = p.sz - p.cz;
+    const R dd = d.x * d.x + d.y * d.y + d.z * d.z;
+    const R inv = dd > R(0) ? R(1) / dd : R(0);
+    const R sd = s_x * d.x + s_y * d.y + s_z * d.z;
+    const R tx = s_x * xx + s_y * yx + s_z * zx;  // s . d_x d
+    const R ty = s_x * xy + s_y * yy + s_z * zy;
+    const R px = sd * (d.x * xx + d.y * yx + d.z * zx) * inv;  // (s.dh)(dh.d_x d)
+    const R py = sd * (d.x * xy + d.y * yy + d.z * zy) * inv;
+    // InterIntra (ii = 1): inter = -(s.d_a d - px), intra = -(n d_a d0 + px)
+    // GenCurrent  (ii = 0): inter = -(n d_a d0 + s.d_a d), intra = 0
+    const R ni = R(1) - ii;
+    const double w = p.w;
+    v[0] -= static_cast<double>(tx - ii * px + ni * nn * ox) * w;
+    v[1] -= static_cast<double>(ty - ii * py + ni * nn * oy) * w;
+    v[2] -= static_cast<double>(ii * (nn * ox + px)) * w;
+    v[3] -= static_cast<double>(ii * (nn * oy + py)) * w;
+    v[4] += static_cast<double>(s_x) * w;
+    v[5] += static_cast<double>(s_y) * w;
+    v[6] += static_cast<double>(s_z) * w;
+    v[7] += static_cast<double>(dd > R(0) ? sd * rsqrt(dd) : -s_z) * w;
+    return d;
+}
+
+// Step kernel. R: arithmetic of the propagated state and of the per-k
+// samples (double, or float with a Kahan compensation term per state
+// component); the k sums are FP64 in both. DEPH: any of T1/T2 on. KPT: k
+// points per lane (independent chains give the scheduler instruction-level
+// parallelism when waves are few).
+//
+// The stage right-hand side is evaluated as half of the reference RHS,
+//   f/2 = d x s - (g2/2)(s - s_par) - (g1/2) s_par,  dn/2 = -(g1/2) n,
+// with the factor 2 folded into the RK coefficients (dt a_ij, dt b_i).
+// The NN part of the phase table carries the factor t1.
+template <typename R, int RK, bool T1ON, bool DEPH, int KPT>
+__global__ __launch_bounds__(kBlock) void haldane_steps_kernel(const HaldaneArgs a, const R *__restrict__ tabr,
+                                                               R *__restrict__ st, double *__restrict__ partials)
 {
     using C = Coef<RK>;
     constexpr int S = C::S;
     constexpr bool kComp = StateT<R>::kComp != 0;
-    const int64_t k = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;  // < nk_stride
     const int64_t ks = a.nk_stride;
-    const int wave = static_cast<int>(k >> 6);
+    const int64_t lane0 = static_cast<int64_t>(blockIdx.x) * (kBlock * KPT) + threadIdx.x;
+    const int wave = static_cast<int>((static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) >> 6);
 
-    // per-k constants
-    const double kx = a.kpts[k], ky = a.kpts[ks + k];
-    const double wk = a.weight[k];
-    double ekr[6], eki[6];
+    KPoint<R> p[KPT];
 #pragma unroll
-    for (int j = 0; j < 3; ++j) {
-        sincos(kx * a.va[j][0] + ky * a.va[j][1], &eki[j], &ekr[j]);
-        sincos(kx * a.vb[j][0] + ky * a.vb[j][1], &eki[j + 3], &ekr[j + 3]);
-    }
-    R ekr_r[6], eki_r[6];
+    for (int q = 0; q < KPT; ++q) {
+        const int64_t k = lane0 + q * kBlock;
+        const double kx = a.kpts[k], ky = a.kpts[ks + k];
+        p[q].w = a.weight[k];
 #pragma unroll
-    for (int j = 0; j < 6; ++j) {
-        ekr_r[j] = static_cast<R>(ekr[j]);
-        eki_r[j] = static_cast<R>(eki[j]);
+        for (int j = 0; j < 3; ++j) {
+            double sa, ca, sb, cb;
+            sincos(kx * a.va[j][0] + ky * a.va[j][1], &sa, &ca);
+            sincos(kx * a.vb[j][0] + ky * a.vb[j][1], &sb, &cb);
+            p[q].er[j] = static_cast<R>(ca);
+            p[q].ei[j] = static_cast<R>(sa);
+            p[q].er[j + 3] = static_cast<R>(cb);
+            p[q].ei[j + 3] = static_cast<R>(sb);
+        }
+        p[q].n = st[k];
+        p[q].sx = st[ks + k];
+        p[q].sy = st[2 * ks + k];
+        p[q].sz = st[3 * ks + k];
+        p[q].cn = p[q].cx = p[q].cy = p[q].cz = R(0);
+        if (kComp) {
+            p[q].cn = st[4 * ks + k];
+            p[q].cx = st[5 * ks + k];
+            p[q].cy = st[6 * ks + k];
+            p[q].cz = st[7 * ks + k];
+        }
     }
-    // derivative weights of the measurement (x, y components of a_j, b_j)
-    const R t1 = static_cast<R>(a.t1), c3 = static_cast<R>(a.c3), M0 = static_cast<R>(a.M0);
-    const R g1 = static_cast<R>(a.gam1), g2 = static_cast<R>(a.gam2);
-    const R dt = static_cast<R>(a.dt);
-
-    // state (SoA): n, sx, sy, sz [, compensation terms]
-    R *st = static_cast<R *>(a.state);
-    R n = st[k], sx = st[ks + k], sy = st[2 * ks + k], sz = st[3 * ks + k];
-    R cn = 0, cx = 0, cy = 0, cz = 0;
-    if (kComp) {
-        cn = st[4 * ks + k];
-        cx = st[5 * ks + k];
-        cy = st[6 * ks + k];
-        cz = st[7 * ks + k];
-    }
+    const R c3 = static_cast<R>(a.c3), M0 = static_cast<R>(a.M0);
+    const R gh1 = static_cast<R>(0.5 * a.gam1), gh2 = static_cast<R>(0.5 * a.gam2);
+    const R gh12 = static_cast<R>(0.5 * (a.gam1 - a.gam2));
+    const R ii = a.inter_intra ? R(1) : R(0);
+    const int lane = __lane_id();
 
     for (int step = 0; step < a.nsteps; ++step) {
-        const int64_t gstep = a.step0 + step;
-        const R *ptab = phase_tab_r + gstep * (S * 12);
-        const double *ptab64 = a.phase_tab + gstep * (S * 12);
+        const R *tr = tabr + (a.step0 + step) * (S * 12);
+        Vec3<R> dstage0[KPT];
 
-        // ---- time sample at t_n from the state before the step (FP64)
+        // ---- time sample at t_n from the state before the step
+        double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int q = 0; q < KPT; ++q) dstage0[q] = time_sample<R>(p[q], tr, a, ii, v);
         {
-            double er[6], ei[6];
-#pragma unroll
-            for (int j = 0; j < 6; ++j) cmul(ekr[j], eki[j], ptab64[2 * j], ptab64[2 * j + 1], er[j], ei[j]);
-            const Vec3<double> d = dvec<double>(er, ei, a.t1, a.c3, a.M0);
-            // d/dk_a of (d0, d) for a = x, y
-            double d0x = 0, d0y = 0, dxx = 0, dxy = 0, dyx = 0, dyy = 0, dzx = 0, dzy = 0;
-#pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                dxx -= a.va[j][0] * ei[j];
-                dxy -= a.va[j][1] * ei[j];
-                dyx += a.va[j][0] * er[j];
-                dyy += a.va[j][1] * er[j];
-                dzx += a.vb[j][0] * er[j + 3];
-                dzy += a.vb[j][1] * er[j + 3];
-                d0x -= a.vb[j][0] * ei[j + 3];
-                d0y -= a.vb[j][1] * ei[j + 3];
-            }
-            dxx *= a.t1; dxy *= a.t1; dyx *= a.t1; dyy *= a.t1;
-            dzx *= -a.c3; dzy *= -a.c3; d0x *= a.c0; d0y *= a.c0;
-            const double nn = static_cast<double>(n) - static_cast<double>(cn);
-            const double s_x = static_cast<double>(sx) - static_cast<double>(cx);
-            const double s_y = static_cast<double>(sy) - static_cast<double>(cy);
-            const double s_z = static_cast<double>(sz) - static_cast<double>(cz);
-            const double dd = d.x * d.x + d.y * d.y + d.z * d.z;
-            const double inv = dd > 0.0 ? 1.0 / dd : 0.0;
-            const double sd = s_x * d.x + s_y * d.y + s_z * d.z;
-            const double tx = s_x * dxx + s_y * dyx + s_z * dzx;  // s . d_x d
-            const double ty = s_x * dxy + s_y * dyy + s_z * dzy;
-            const double px = sd * (d.x * dxx + d.y * dyx + d.z * dzx) * inv;  // (s.dh)(dh.d_x d)
-            const double py = sd * (d.x * dxy + d.y * dyy + d.z * dzy) * inv;
-            double v[8];
-            if (a.inter_intra) {
-                v[0] = -(tx - px) * wk;
-                v[1] = -(ty - py) * wk;
-                v[2] = -(nn * d0x + px) * wk;
-                v[3] = -(nn * d0y + py) * wk;
-            } else {  // GenCurrent: the whole Re Tr(rho J) goes to "inter"
-                v[0] = -(nn * d0x + tx) * wk;
-                v[1] = -(nn * d0y + ty) * wk;
-                v[2] = 0.0;
-                v[3] = 0.0;
-            }
-            v[4] = s_x * wk;
-            v[5] = s_y * wk;
-            v[6] = s_z * wk;
-            v[7] = (dd > 0.0 ? sd * rsqrt(dd) : 0.0) * wk;
             const double r = wave_reduce8(v);
-            if ((__lane_id() & 7) == 0)
-                a.partials[(static_cast<int64_t>(step) * a.nwaves + wave) * kHaldaneNV + (__lane_id() >> 3)] = r;
+            // lanes 8q..8q+7 all hold value q: duplicate stores of equal data keep the step branch-free
+            partials[(static_cast<int64_t>(step) * a.nwaves + wave) * kHaldaneNV + (lane >> 3)] = r;
         }
 
-        // ---- one RK step (SBEs.h:596-624)
-        R kn[S];
-        Vec3<R> ks_[S];
+        // ---- one RK step (SBEs.h:596-624), stage RHS halved (see above)
 #pragma unroll
-        for (int i = 0; i < S; ++i) {
-            R yn = n;
-            Vec3<R> y{sx, sy, sz};
+        for (int q = 0; q < KPT; ++q) {
+            R kn[S];
+            Vec3<R> kv[S];
 #pragma unroll
-            for (int j = 0; j < i; ++j) {
-                if (C::a[i][j] != 0.0) {
-                    const R aij = static_cast<R>(C::a[i][j]) * dt;
-                    yn += aij * kn[j];
-                    y.x += aij * ks_[j].x;
-                    y.y += aij * ks_[j].y;
-                    y.z += aij * ks_[j].z;
+            for (int i = 0; i < S; ++i) {
+                R yn = p[q].n;
+                Vec3<R> y{p[q].sx, p[q].sy, p[q].sz};
+#pragma unroll
+                for (int j = 0; j < i; ++j) {
+                    if (C::a[i][j] != 0.0) {
+                        const R aij = static_cast<R>(2.0 * C::a[i][j] * a.dt);
+                        if (T1ON) yn += aij * kn[j];
+                        y.x += aij * kv[j].x;
+                        y.y += aij * kv[j].y;
+                        y.z += aij * kv[j].z;
+                    }
+                }
+                Vec3<R> d;
+                if (i == 0) {
+                    d = dstage0[q];
+                } else {
+                    const R *row = tr + i * 12;
+                    R dx = 0, dy = 0, sb = 0;
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) {
+                        dx += p[q].er[j] * row[2 * j] - p[q].ei[j] * row[2 * j + 1];
+                        dy += p[q].er[j] * row[2 * j + 1] + p[q].ei[j] * row[2 * j];
+                        sb += p[q].er[j + 3] * row[7 + 2 * j] + p[q].ei[j + 3] * row[6 + 2 * j];
+                    }
+                    d = {dx, dy, M0 - c3 * sb};
+                }
+                // f/2 = d x y - (g2/2) y + ((g2-g1)/2) (y.d / |d|^2) d
+                Vec3<R> f;
+                f.x = d.y * y.z - d.z * y.y;
+                f.y = d.z * y.x - d.x * y.z;
+                f.z = d.x * y.y - d.y * y.x;
+                if (DEPH) {
+                    const R dd = d.x * d.x + d.y * d.y + d.z * d.z;
+                    R inv = __builtin_amdgcn_rcp(dd);  // reciprocal with two Newton steps
+                    inv = inv + inv * (R(1) - dd * inv);
+                    inv = inv + inv * (R(1) - dd * inv);
+                    inv = dd > R(0) ? inv : R(0);
+                    const R c = -gh12 * (y.x * d.x + y.y * d.y + y.z * d.z) * inv;
+                    f.x += c * d.x - gh2 * y.x;
+                    f.y += c * d.y - gh2 * y.y;
+                    f.z += c * d.z - gh2 * y.z;
+                }
+                kv[i] = f;
+                kn[i] = T1ON ? -gh1 * yn : R(0);
+            }
+            R dn = 0;
+            Vec3<R> ds{0, 0, 0};
+#pragma unroll
+            for (int i = 0; i < S; ++i) {
+                if (C::b[i] != 0.0) {
+                    const R bi = static_cast<R>(2.0 * C::b[i] * a.dt);
+                    if (T1ON) dn += bi * kn[i];
+                    ds.x += bi * kv[i].x;
+                    ds.y += bi * kv[i].y;
+                    ds.z += bi * kv[i].z;
                 }
             }
-            R er[6], ei[6];
-#pragma unroll
-            for (int j = 0; j < 6; ++j) cmul(ekr_r[j], eki_r[j], ptab[i * 12 + 2 * j], ptab[i * 12 + 2 * j + 1], er[j], ei[j]);
-            const Vec3<R> d = dvec<R>(er, ei, t1, c3, M0);
-            rhs<R, T1ON>(d, yn, y, g1, g2, kn[i], ks_[i]);
-        }
-        R dn = 0;
-        Vec3<R> ds{0, 0, 0};
-#pragma unroll
-        for (int i = 0; i < S; ++i) {
-            if (C::b[i] != 0.0) {
-                const R bi = static_cast<R>(C::b[i]);
-                dn += bi * kn[i];
-                ds.x += bi * ks_[i].x;
-                ds.y += bi * ks_[i].y;
-                ds.z += bi * ks_[i].z;
+            if (kComp) {  // Kahan-compensated state update
+                auto kahan = [](R &sum, R &c, R inc) {
+                    const R y = inc - c;
+                    const R t = sum + y;
+                    c = (t - sum) - y;
+                    sum = t;
+                };
+                if (T1ON) kahan(p[q].n, p[q].cn, dn);
+                kahan(p[q].sx, p[q].cx, ds.x);
+                kahan(p[q].sy, p[q].cy, ds.y);
+                kahan(p[q].sz, p[q].cz, ds.z);
+            } else {
+                if (T1ON) p[q].n += dn;
+                p[q].sx += ds.x;
+                p[q].sy += ds.y;
+                p[q].sz += ds.z;
             }
         }
-        if (kComp) {  // Kahan-compensated state update
-            auto kahan = [](R &sum, R &c, R inc) {
-                const R y = inc - c;
-                const R t = sum + y;
-                c = (t - sum) - y;
-                sum = t;
-            };
-            if (T1ON) kahan(n, cn, dt * dn);
-            kahan(sx, cx, dt * ds.x);
-            kahan(sy, cy, dt * ds.y);
-            kahan(sz, cz, dt * ds.z);
-        } else {
-            if (T1ON) n += dt * dn;
-            sx += dt * ds.x;
-            sy += dt * ds.y;
-            sz += dt * ds.z;
-        }
     }
-    st[k] = n;
-    st[ks + k] = sx;
-    st[2 * ks + k] = sy;
-    st[3 * ks + k] = sz;
-    if (kComp) {
-        st[4 * ks + k] = cn;
-        st[5 * ks + k] = cx;
-        st[6 * ks + k] = cy;
-        st[7 * ks + k] = cz;
+#pragma unroll
+    for (int q = 0; q < KPT; ++q) {
+        const int64_t k = lane0 + q * kBlock;
+        st[k] = p[q].n;
+        st[ks + k] = p[q].sx;
+        st[2 * ks + k] = p[q].sy;
+        st[3 * ks + k] = p[q].sz;
+        if (kComp) {
+            st[4 * ks + k] = p[q].cn;
+            st[5 * ks + k] = p[q].cx;
+            st[6 * ks + k] = p[q].cy;
+            st[7 * ks + k] = p[q].cz;
+        }
     }
 }
 
@@ -409,23 +461,33 @@ hipError_t launch_haldane_init(const HaldaneArgs &a, int init_type, double ef, h
     return hipGetLastError();
 }
 
-template <typename R, int RK>
+template <typename R, int RK, int KPT>
 static void launch_steps(const HaldaneArgs &a, const R *tab, bool t1_on, hipStream_t s)
 {
-    const dim3 grid(static_cast<unsigned>(a.nk_stride / kBlock));
-    if (t1_on) haldane_steps_kernel<R, RK, true><<<grid, kBlock, 0, s>>>(a, tab);
-    else haldane_steps_kernel<R, RK, false><<<grid, kBlock, 0, s>>>(a, tab);
+    const dim3 grid(static_cast<unsigned>(a.nk_stride / (kBlock * KPT)));
+    R *st = static_cast<R *>(a.state);
+    const bool deph = a.gam1 != 0.0 || a.gam2 != 0.0;
+    if (t1_on) haldane_steps_kernel<R, RK, true, true, KPT><<<grid, kBlock, 0, s>>>(a, tab, st, a.partials);
+    else if (deph) haldane_steps_kernel<R, RK, false, true, KPT><<<grid, kBlock, 0, s>>>(a, tab, st, a.partials);
+    else haldane_steps_kernel<R, RK, false, false, KPT><<<grid, kBlock, 0, s>>>(a, tab, st, a.partials);
+}
+
+template <typename R, int RK>
+static void launch_steps_kpt(const HaldaneArgs &a, const R *tab, bool t1_on, hipStream_t s)
+{
+    if (a.kpt == 2) launch_steps<R, RK, 2>(a, tab, t1_on, s);
+    else launch_steps<R, RK, 1>(a, tab, t1_on, s);
 }
 
 hipError_t launch_haldane_steps(const HaldaneArgs &a, int rk_order, bool t1_on, hipStream_t s)
 {
     const bool f32 = a.precision == SBE_PREC_FP32;
     if (rk_order == SBE_RK4) {
-        if (f32) launch_steps<float, 4>(a, a.phase_tab32, t1_on, s);
-        else launch_steps<double, 4>(a, a.phase_tab, t1_on, s);
+        if (f32) launch_steps_kpt<float, 4>(a, a.phase_tab32, t1_on, s);
+        else launch_steps_kpt<double, 4>(a, a.phase_tab, t1_on, s);
     } else if (rk_order == SBE_RK5_FEHLBERG) {
-        if (f32) launch_steps<float, 5>(a, a.phase_tab32, t1_on, s);
-        else launch_steps<double, 5>(a, a.phase_tab, t1_on, s);
+        if (f32) launch_steps_kpt<float, 5>(a, a.phase_tab32, t1_on, s);
+        else launch_steps_kpt<double, 5>(a, a.phase_tab, t1_on, s);
     } else {
         return hipErrorInvalidValue;
     }

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -40,6 +41,7 @@ int hip_check(sbe_ctx *ctx, hipError_t e, const char *what)
 constexpr size_t kPartialBytes = size_t(128) << 20;
 
 struct Plan {
+    int kpt = 1;
     int64_t nk = 0, nk_stride = 0;
     int32_t nwaves = 0;
     int nv = 0;
@@ -99,8 +101,13 @@ int validate(const sbe_config *cfg, const sbe_model *m, std::string &err)
 int make_plan(const sbe_config &cfg, const sbe_model &m, int64_t nk, int64_t nt, int nstage, Plan &p)
 {
     p.nk = nk;
-    p.nk_stride = (nk + kBlock - 1) / kBlock * kBlock;
-    p.nwaves = static_cast<int32_t>(p.nk_stride / 64);
+    // k points per lane of the step kernel: 2 gives each lane two independent
+    // dependency chains, which pays when there are few waves per SIMD
+    p.kpt = 1;
+    if (const char *e = std::getenv("ANTELOPE_HALDANE_KPT")) p.kpt = std::atoi(e) == 2 ? 2 : 1;
+    const int64_t unit = static_cast<int64_t>(kBlock) * p.kpt;
+    p.nk_stride = (nk + unit - 1) / unit * unit;
+    p.nwaves = static_cast<int32_t>(p.nk_stride / (64 * p.kpt));
     p.nv = sbe::kHaldaneNV;
     p.nt = nt;
     p.nstage = nstage;
@@ -147,9 +154,65 @@ double rk_linear_factor(const sbe::Tableau &tab, double z)
     return r;
 }
 
+void drop_events(sbe_ctx *ctx)
+{
+    for (auto &v : ctx->ev) {
+        for (auto &p : v) {
+            (void)hipEventDestroy(p.first);
+            (void)hipEventDestroy(p.second);
+        }
+        v.clear();
+    }
+}
+
+// Launch through `fn`, bracketed by events when timing is on.
+template <typename F>
+int timed(sbe_ctx *ctx, int cls, F &&fn, const char *what)
+{
+    hipEvent_t a = nullptr, b = nullptr;
+    if (ctx->timing) {
+        int rc = hip_check(ctx, hipEventCreate(&a), "hipEventCreate");
+        if (!rc) rc = hip_check(ctx, hipEventCreate(&b), "hipEventCreate");
+        if (!rc) rc = hip_check(ctx, hipEventRecord(a, ctx->stream), "hipEventRecord");
+        if (rc) return rc;
+    }
+    int rc = hip_check(ctx, fn(), what);
+    if (ctx->timing) {
+        if (!rc) rc = hip_check(ctx, hipEventRecord(b, ctx->stream), "hipEventRecord");
+        ctx->ev[cls].push_back({a, b});
+    }
+    return rc;
+}
+
 }  // namespace
 
 extern "C" {
+
+int sbe_set_timing(sbe_ctx *ctx, int32_t enable)
+{
+    if (!ctx) return set_error(nullptr, SBE_ERR_STATE, "null context");
+    ctx->timing = enable != 0;
+    return SBE_OK;
+}
+
+int sbe_get_timing(sbe_ctx *ctx, double *ms, int64_t *launches)
+{
+    if (!ctx) return set_error(nullptr, SBE_ERR_STATE, "null context");
+    int rc = hip_check(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    for (int c = 0; c < 2 && !rc; ++c) {
+        double sum = 0.0;
+        for (auto &p : ctx->ev[c]) {
+            float t = 0.f;
+            rc = hip_check(ctx, hipEventElapsedTime(&t, p.first, p.second), "hipEventElapsedTime");
+            if (rc) break;
+            sum += t;
+        }
+        if (ms) ms[c] = sum;
+        if (launches) launches[c] = static_cast<int64_t>(ctx->ev[c].size());
+    }
+    drop_events(ctx);
+    return rc;
+}
 
 const char *sbe_last_error(const sbe_ctx *ctx)
 {
@@ -292,6 +355,7 @@ int sbe_create(const sbe_config *cfg, const sbe_model *model, int64_t k_begin, i
     h.inter_intra = ctx->cfg.inter_intra ? 1 : 0;
     h.precision = ctx->cfg.precision;
     h.nval = model->nval;
+    h.kpt = p.kpt;
     h.kpts = ctx->d_kpts;
     h.weight = ctx->d_weight;
     h.state = ctx->d_state;
@@ -311,8 +375,8 @@ int sbe_create(const sbe_config *cfg, const sbe_model *model, int64_t k_begin, i
                 for (int j = 0; j < 3; ++j) {
                     const double pa = A[0] * va[j][0] + A[1] * va[j][1];
                     const double pb = A[0] * vb[j][0] + A[1] * vb[j][1];
-                    row[2 * j] = std::cos(pa);
-                    row[2 * j + 1] = std::sin(pa);
+                    row[2 * j] = model->t1 * std::cos(pa);  // NN part carries t1
+                    row[2 * j + 1] = model->t1 * std::sin(pa);
                     row[6 + 2 * j] = std::cos(pb);
                     row[6 + 2 * j + 1] = std::sin(pb);
                 }
@@ -354,12 +418,13 @@ int sbe_run(sbe_ctx *ctx, int64_t step_begin, int64_t step_end)
         sbe::HaldaneArgs a = ctx->hargs;
         a.step0 = s;
         a.nsteps = static_cast<int32_t>(n);
-        int rc = hip_check(ctx, sbe::launch_haldane_steps(a, ctx->cfg.rk_order, t1_on, ctx->stream), "step kernel");
+        int rc = timed(ctx, 0, [&] { return sbe::launch_haldane_steps(a, ctx->cfg.rk_order, t1_on, ctx->stream); },
+                       "step kernel");
         if (!rc)
-            rc = hip_check(ctx,
-                           sbe::launch_series_reduce(ctx->d_partials, n, ctx->nwaves, ctx->nv,
-                                                     ctx->d_series + s * ctx->nv, ctx->stream),
-                           "series kernel");
+            rc = timed(ctx, 1, [&] {
+                return sbe::launch_series_reduce(ctx->d_partials, n, ctx->nwaves, ctx->nv, ctx->d_series + s * ctx->nv,
+                                                 ctx->stream);
+            }, "series kernel");
         if (rc) return rc;
         ctx->steps_done = s + n;
     }
@@ -475,6 +540,7 @@ void sbe_destroy(sbe_ctx *ctx)
 {
     if (!ctx) return;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    drop_events(ctx);
     for (auto &b : ctx->owned) (void)hipFree(b.ptr);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;

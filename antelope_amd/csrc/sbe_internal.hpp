@@ -6,6 +6,7 @@
 
 #include <cstdint>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "antelope/sbe_c.h"
@@ -80,10 +81,10 @@ struct HaldaneArgs {
     int32_t precision;
     const double *kpts;     // [3][nk_stride]
     const double *weight;   // [nk_stride] (0 on padding)
-    const double *phase_tab;  // [nt][nstage][6][2] e^{i A(t_n + c_s dt).v_j}
+    const double *phase_tab;  // [nt][nstage][12]: t1 e^{i A.a_j} (j<3), e^{i A.b_j}, at t_n + c_s dt
     const float *phase_tab32; // same table rounded to FP32 (FP32 state only)
     int32_t nval;             // occupied bands of the initial state
-    int32_t reserved;
+    int32_t kpt;              // k points per lane in the step kernel (1 or 2)
     void *state;            // SoA [4][nk_stride] (fp64) or [8][nk_stride] (fp32 + compensation)
     double *partials;       // [nsteps][nwaves][kHaldaneNV]
 };
@@ -128,6 +129,10 @@ struct sbe_ctx {
 
     sbe::HaldaneArgs hargs{};
     std::string err;
+
+    // optional kernel timing: (start, stop) event pairs per kernel class
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[2];
 };
 
 namespace sbe {

@@ -138,6 +138,16 @@ class SBESolver:
         _check(self._lib, self._ctx, self._lib.sbe_get_state(self._ctx, b, _ptr(buf)))
         return buf[..., 0] + 1j * buf[..., 1]
 
+    def set_timing(self, enable: bool = True) -> None:
+        _check(self._lib, self._ctx, self._lib.sbe_set_timing(self._ctx, int(enable)))
+
+    def timing(self) -> dict:
+        """Summed HIP-event durations (ms) of the kernels since the last call."""
+        ms = (C.c_double * 2)()
+        n = (C.c_int64 * 2)()
+        _check(self._lib, self._ctx, self._lib.sbe_get_timing(self._ctx, ms, n))
+        return {"step_ms": ms[0], "step_launches": n[0], "reduce_ms": ms[1], "reduce_launches": n[1]}
+
     def close(self) -> None:
         if getattr(self, "_ctx", None):
             self._lib.sbe_destroy(self._ctx)

@@ -1,0 +1,229 @@
+"""Throughput benchmark of the SBE time propagation on one MI355X.
+
+Workload: the 2-band Haldane model (a = 2.683 bohr, t1 = 0.075, t2 = t1/3,
+phi0 = 0.3 pi, M0 = 2.54 t2) on a 300 x 300 k grid (90,000 k points) driven
+by an elliptically polarised sin^2 pulse (ellipticity 0.5, E0 = 0.0045,
+w0 = 0.014, 12 cycles), T2 = 220, RK4 with dt = 0.25: Nt = 21542 time steps.
+A "step" is one RK4 step of all 90,000 k points, including the per-step
+k-summed currents, populations and density matrix. The run is timed for the
+FP64 path and for the FP32-state path (FP64 sums), and the FP32 path's error
+against the FP64 path is measured on the same run.
+
+Prints ONE JSON line (rank 0). Synthetic, seeded inputs only.
+
+    python bench.py [--steps K] [--warmup W] [--grid 300] [--no-cpu-baseline]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+from antelope_amd import HaldaneModel, Pulse, RunConfig, SBESolver  # noqa: E402
+from antelope_amd.spectrum import harmonic_error_db, hhg_spectrum  # noqa: E402
+
+METRIC = "k-point·RK-steps/s at ≤1e-6 rel err vs FP64 oracle; p50/p99 solve latency"
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+# FP64 flops of one k-point RK4 step of the 2-band kernel, counted from the
+# algorithm (4 stages of: e^{i(k+A)v} products, d, 2 d x s, T2 projection, RK
+# combination; plus the time sample: d, d0, dd/dk, intra/inter currents,
+# populations, 8-value wave reduction). See DESIGN.md for the count.
+FLOPS_PER_KSTEP = {"fp64": 560.0}
+# compulsory state bytes of one k-point step if the state were streamed
+# through HBM every step (read + write of the 2x2 complex rho, FP64)
+HBM_BYTES_PER_KSTEP_FP64 = 128.0
+
+
+def workload(grid: int, precision: str) -> tuple[RunConfig, HaldaneModel]:
+    t1 = 0.075
+    model = HaldaneModel(a0=2.683, t1=t1, t2=t1 / 3, phi0=0.3 * math.pi, M0=2.54 * t1 / 3)
+    pulse = Pulse(E0=0.0045, w0=0.014, ncycles=12.0, ellip=0.5, envelope="sin2")
+    cfg = RunConfig(npoints=[grid, grid], pulses=[pulse], dt=0.25, T2=220.0, rk_order=4,
+                    inter_intra=True, precision=precision)
+    return cfg, model
+
+
+def gpu_sync():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+
+
+def timed_solve(cfg, model, steps: int, warmup: int):
+    """Warm up in a throwaway context, then time `steps` steps of a fresh one."""
+    with SBESolver(cfg, model) as w:
+        w.run(0, min(warmup, w.nt))
+        w.synchronize()
+    s = SBESolver(cfg, model)
+    steps = min(steps, s.nt)
+    s.set_timing(True)
+    gpu_sync()
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.run(0, steps)
+    s.synchronize()
+    gpu_sync()
+    wall = time.perf_counter() - t0
+    tim = s.timing()
+    series = s.series()
+    nk = s.info.k_end - s.info.k_begin
+    info = s.info
+    s.close()
+    return {"wall_s": wall, "steps": steps, "nk": nk, "timing": tim, "series": series, "info": info}
+
+
+def cpu_baseline(cfg, model, budget_s: float):
+    """FP64 CPU oracle (faithful-cost restatement of the reference per-k loop)
+    on a bounded sample: a contiguous k block of the same workload for the
+    first steps of the pulse, all host threads (capped at 16)."""
+    from oracle import pyoracle
+    threads = min(os.cpu_count() or 1, 16)
+    nsteps = 200
+    ntot = cfg.npoints[0] * cfg.npoints[1]
+    nk = min(ntot, 64 * threads)
+    while True:
+        r = pyoracle.run(cfg, model, 0, nk, nsteps, threads)
+        sec = r["info"].seconds
+        if sec >= 0.5 * budget_s or nk >= ntot:
+            break
+        nk = int(min(ntot, max(2 * nk, nk * budget_s / max(sec, 1e-3))))
+    return {
+        "value": nk * nsteps / sec,
+        "unit": "k-point·RK-steps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"first {nsteps} steps of k points [0,{nk}) of the same workload, FP64 oracle "
+                  f"(per-stage Jacobi eigensolve, generic N^3 products), {sec:.1f} s wall",
+    }
+
+
+def rel_l2(x, ref):
+    d = np.linalg.norm(ref)
+    return float(np.linalg.norm(x - ref) / d) if d > 0 else float(np.linalg.norm(x))
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=0, help="timed steps (default: the whole pulse)")
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--grid", type=int, default=300)
+    ap.add_argument("--precision", choices=["both", "fp64", "fp32"], default="both")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args(argv)
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(k blocks would shard across GPUs with one final sum; see DESIGN.md).", file=sys.stderr)
+        return 2
+
+    forms = ["fp64", "fp32"] if args.precision == "both" else [args.precision]
+    res = {}
+    for p in forms:
+        cfg, model = workload(args.grid, p)
+        steps = args.steps if args.steps > 0 else 10 ** 9
+        r = timed_solve(cfg, model, steps, args.warmup)
+        print(f"[bench] {p}: {r['steps']} steps x {r['nk']} k in {r['wall_s']:.3f} s "
+              f"(step kernels {r['timing']['step_ms']:.1f} ms / {r['timing']['step_launches']} launches, "
+              f"reductions {r['timing']['reduce_ms']:.1f} ms)", file=sys.stderr, flush=True)
+        res[p] = r
+
+    out_forms = {}
+    for p, r in res.items():
+        ks = r["nk"] * r["steps"] / r["wall_s"]
+        out_forms[p] = {"k_steps_per_s": ks, "wall_s": r["wall_s"],
+                        "step_kernel_ms": r["timing"]["step_ms"], "reduce_kernel_ms": r["timing"]["reduce_ms"]}
+    if "fp64" in res and "fp32" in res:
+        a, b = res["fp64"]["series"], res["fp32"]["series"]
+        n = min(res["fp64"]["steps"], res["fp32"]["steps"])
+        j64 = (a["j_inter"] + a["j_intra"])[:n]
+        j32 = (b["j_inter"] + b["j_intra"])[:n]
+        out_forms["fp32"]["err_j_rel_l2_vs_fp64"] = rel_l2(j32, j64)
+        out_forms["fp32"]["err_pop_rel_l2_vs_fp64"] = rel_l2(b["band_pop"][:n], a["band_pop"][:n])
+        if n == res["fp64"]["info"].nt:
+            dV, dt = res["fp64"]["info"].dV, res["fp64"]["info"].dt
+            o, S64 = hhg_spectrum(a["j_inter"] * dV, a["j_intra"] * dV, dt, 0.014)
+            _, S32 = hhg_spectrum(b["j_inter"] * dV, b["j_intra"] * dV, dt, 0.014)
+            db, hcut = harmonic_error_db(o, S32, S64)
+            out_forms["fp32"]["err_hhg_db_vs_fp64"] = db
+            out_forms["fp32"]["hhg_cutoff_harmonic"] = hcut
+
+    # the reported value: the fastest form within the 1e-6 error bar
+    best = "fp64"
+    if "fp32" in out_forms and out_forms["fp32"].get("err_j_rel_l2_vs_fp64", 1.0) <= 1e-6 and \
+            (best not in out_forms or out_forms["fp32"]["k_steps_per_s"] > out_forms[best]["k_steps_per_s"]):
+        best = "fp32"
+    if best not in out_forms:
+        best = forms[0]
+    r = res[best]
+    value = out_forms[best]["k_steps_per_s"]
+    # roofline of the dominant kernel (the fused step kernel), from HIP events
+    tim = res["fp64"]["timing"] if "fp64" in res else r["timing"]
+    rfl = None
+    if "fp64" in res:
+        f = res["fp64"]
+        flops = FLOPS_PER_KSTEP["fp64"] * f["nk"] * f["steps"]
+        ach = flops / (tim["step_ms"] * 1e-3) / 1e12
+        rfl = {"bound": "mfma", "pipe": "FP64 VALU (vector FP64 peak 78.6 TF equals the FP64 MFMA peak)",
+               "kernel": "haldane_steps_kernel<double,4>", "achieved": ach, "peak": PEAK_FP64_TFLOPS,
+               "unit": "TFLOP/s", "frac": ach / PEAK_FP64_TFLOPS, "traffic": None,
+               "flops_per_kstep": FLOPS_PER_KSTEP["fp64"],
+               "avg_launch_ms": tim["step_ms"] / max(tim["step_launches"], 1),
+               "hbm_equivalent": {
+                   "note": "state bytes a per-step HBM-streaming kernel would move (128 B per k-step FP64)",
+                   "GBps": f["nk"] * f["steps"] * HBM_BYTES_PER_KSTEP_FP64 / (tim["step_ms"] * 1e-3) / 1e9,
+                   "frac_of_8TBps": f["nk"] * f["steps"] * HBM_BYTES_PER_KSTEP_FP64 / (tim["step_ms"] * 1e-3) / 8e12}}
+    prof = ROOT / "profiles" / "haldane_steps_300x300_traffic.json"
+    if rfl is not None and prof.exists() and args.grid == 300:
+        try:
+            rfl["traffic"] = json.loads(prof.read_text())["hbm_bytes_per_launch"]
+            rfl["traffic_source"] = str(prof.relative_to(ROOT))
+        except Exception:
+            pass
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cfg, model = workload(args.grid, "fp64")
+        cpu = cpu_baseline(cfg, model, args.cpu_seconds)
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "k-point·RK-steps/s",
+        "n_gpus": 1,
+        "steps": r["steps"],
+        "warmup": args.warmup,
+        "ms_per_step": r["wall_s"] * 1e3 / r["steps"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if best == "fp64" else "f32 state, f64 sums",
+        "data": "synthetic (seeded analytic Haldane model and laser pulse)",
+        "config": {"workload": f"Haldane 2-band, {args.grid}x{args.grid} k grid, elliptic 0.5 sin^2 12-cycle "
+                               f"pulse, T2=220, RK4 dt=0.25, Nt={res[best]['info'].nt}",
+                   "nk": r["nk"], "form": best},
+        "forms": out_forms,
+        "roofline": rfl,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

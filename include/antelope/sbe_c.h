@@ -236,6 +236,14 @@ int sbe_laser_eval(const sbe_config *cfg, int64_t n, const double *t,
                    double *A /*[n][3]*/, double *E /*[n][3]*/,
                    double *atmin, int64_t *nt);
 
+/* Measurement (no reference counterpart): when enabled, HIP events are
+ * recorded on the context's stream around every kernel sbe_run launches.
+ * sbe_get_timing waits for them and returns the summed durations since the
+ * previous call (then resets): ms[0] / launches[0] for the step kernels,
+ * ms[1] / launches[1] for the series reductions. */
+int sbe_set_timing(sbe_ctx *ctx, int32_t enable);
+int sbe_get_timing(sbe_ctx *ctx, double *ms /*[2]*/, int64_t *launches /*[2]*/);
+
 void sbe_destroy(sbe_ctx *ctx);
 
 /* Last error of a context (ctx may be NULL: last error of this thread,
