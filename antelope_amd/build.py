@@ -25,7 +25,7 @@ LIBNAME = "libantelope_sbe.so"
 ARCH = os.environ.get("ANTELOPE_OFFLOAD_ARCH", "gfx950")
 
 # Library sources (kernels + C ABI). The driver links against the library.
-LIB_SOURCES = ["host_setup.cpp", "sbe_api.cpp", "haldane.hip", "series.hip"]
+LIB_SOURCES = ["host_setup.cpp", "sbe_api.cpp", "nband_host.cpp", "haldane.hip", "nband.hip", "series.hip"]
 
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}",

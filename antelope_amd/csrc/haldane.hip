@@ -139,6 +139,8 @@ __device__ __forceinline__ void cmul(R a, R b, R c, R d, R &re, R &im)
 }
 
 // ------------------------------------------------------------ main kernel
+constexpr int kTile = 32;  // time steps per LDS tile of the phase table
+
 // Per-k registers of one lane.
 template <typename R>
 struct KPoint {
@@ -263,8 +265,42 @@ __global__ __launch_bounds__(kBlock) void haldane_steps_kernel(const HaldaneArgs
     const R ii = a.inter_intra ? R(1) : R(0);
     const int lane = __lane_id();
 
-    for (int step = 0; step < a.nsteps; ++step) {
-        const R *tr = tabr + (a.step0 + step) * (S * 12);
+    // The k-independent phase rows of kTile steps are staged in LDS (double
+    // buffered): every lane reads the same row, so LDS broadcasts replace a
+    // per-step scalar-cache miss that a lone wave per SIMD cannot hide.
+    constexpr int ROW = S * 12;
+    constexpr int TILE_VALS = kTile * ROW;
+    constexpr int PER_THREAD = (TILE_VALS + kBlock - 1) / kBlock;
+    __shared__ R lds_tab[2][TILE_VALS];
+    const R *gtab = tabr + a.step0 * ROW;
+    const int64_t tab_vals = static_cast<int64_t>(a.nsteps) * ROW;
+    const int ntiles = (a.nsteps + kTile - 1) / kTile;
+    R pre[PER_THREAD];
+    auto fetch = [&](int tile) {
+#pragma unroll
+        for (int r = 0; r < PER_THREAD; ++r) {
+            const int i = r * kBlock + threadIdx.x;
+            const int64_t g = static_cast<int64_t>(tile) * TILE_VALS + i;
+            pre[r] = (i < TILE_VALS && g < tab_vals) ? gtab[g] : R(0);
+        }
+    };
+    auto stash = [&](int buf) {
+#pragma unroll
+        for (int r = 0; r < PER_THREAD; ++r) {
+            const int i = r * kBlock + threadIdx.x;
+            if (i < TILE_VALS) lds_tab[buf][i] = pre[r];
+        }
+    };
+    fetch(0);
+    stash(0);
+    __syncthreads();
+
+    for (int tile = 0; tile < ntiles; ++tile) {
+    if (tile + 1 < ntiles) fetch(tile + 1);  // in flight during this tile
+    const int s_end = min(kTile, a.nsteps - tile * kTile);
+    for (int st_ = 0; st_ < s_end; ++st_) {
+        const int step = tile * kTile + st_;
+        const R *tr = &lds_tab[tile & 1][st_ * ROW];
         Vec3<R> dstage0[KPT];
 
         // ---- time sample at t_n from the state before the step
@@ -359,6 +395,11 @@ __global__ __launch_bounds__(kBlock) void haldane_steps_kernel(const HaldaneArgs
                 p[q].sz += ds.z;
             }
         }
+    }
+    // the other buffer was last read in the previous tile (all lanes passed
+    // the barrier since), so it can take the next tile now
+    if (tile + 1 < ntiles) stash((tile + 1) & 1);
+    __syncthreads();
     }
 #pragma unroll
     for (int q = 0; q < KPT; ++q) {

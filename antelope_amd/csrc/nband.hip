@@ -1,0 +1,532 @@
+// nband.hip - N-band (Wannier tight-binding) SBE kernels for gfx950.
+//
+// Reference: Materials/Wannier90.h:297-387 (H, D, J as Fourier sums over R,
+// with dipoles: isDipoleZero = false, Wannier90.h:124), SBEs.h:762-861 (RHS:
+// -i(H~rho - (H~rho)^dag) - U (Gamma o U^dag rho U) U^dag with
+// H~ = H + E.D and U the eigenvectors of H), SBEs.h:694-760 (currents),
+// SBEs.h:596-624 (RK step), SBEs.h:284-364 (initial state).
+//
+// Decomposition per RK stage:
+//  * nb_fourier_kernel: complex GEMM C[m][k][ij] = sum_R P[k][R] B[R][m,ij]
+//    over a tile of 64 k x 64 columns, with P = e^{ik.R} (fixed, built once)
+//    and B = w_R e^{iA(t).R} X_R (X = H + E.r, H, or -i R_a H), formed on
+//    the fly while staging through LDS. t and E(t) come from a device table
+//    indexed by (step, stage), so launch arguments do not carry time.
+//  * nb_rhs_kernel / nb_sample_kernel: one wave per k point; matrices live in
+//    LDS; the Hermitian eigenproblem is solved by a parallel-order complex
+//    Jacobi method (N/2 disjoint rotations per round).
+#include "nband.hpp"
+
+namespace sbe {
+namespace {
+
+using cd = double2;
+
+__device__ __forceinline__ cd cmk(double r, double i) { return make_double2(r, i); }
+__device__ __forceinline__ cd cadd(cd a, cd b) { return cmk(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ cd csub(cd a, cd b) { return cmk(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ cd cmul(cd a, cd b) { return cmk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ __forceinline__ cd cconj(cd a) { return cmk(a.x, -a.y); }
+__device__ __forceinline__ cd cscale(cd a, double s) { return cmk(a.x * s, a.y * s); }
+// acc += a * b
+__device__ __forceinline__ void cfma(cd &acc, cd a, cd b)
+{
+    acc.x += a.x * b.x - a.y * b.y;
+    acc.y += a.x * b.y + a.y * b.x;
+}
+// acc += conj(a) * b
+__device__ __forceinline__ void cfmac(cd &acc, cd a, cd b)
+{
+    acc.x += a.x * b.x + a.y * b.y;
+    acc.y += a.x * b.y - a.y * b.x;
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// ------------------------------------------------------------ phase P --
+__global__ void nb_phase_kernel(const NbArgs a, cd *__restrict__ P)
+{
+    const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t total = a.nk_stride * a.nr;
+    if (idx >= total) return;
+    const int64_t k = idx / a.nr;
+    const int r = static_cast<int>(idx % a.nr);
+    if (k >= a.nk) {
+        P[idx] = cmk(0.0, 0.0);
+        return;
+    }
+    const double *R = a.rcart + 3 * r;
+    const double ph = a.kpts[3 * k] * R[0] + a.kpts[3 * k + 1] * R[1] + a.kpts[3 * k + 2] * R[2];
+    double s, c;
+    sincos(ph, &s, &c);
+    P[idx] = cmk(c, s);
+}
+
+// --------------------------------------------------------- Fourier GEMM --
+constexpr int GM = 64, GN = 64, GK = 8;
+
+// B element (row r, column c) of the evaluation: c = mat * N^2 + ij
+__device__ __forceinline__ cd b_element(const NbArgs &a, const NbEval &ev, int r, int64_t c, cd phase)
+{
+    const int nn = a.N * a.N;
+    const int mat = static_cast<int>(c / nn);
+    const int ij = static_cast<int>(c % nn);
+    const int type = ev.mats[mat];
+    const cd h = a.ham[static_cast<int64_t>(r) * nn + ij];
+    cd x;
+    if (type == kMatHtilde) {
+        x = h;
+        const cd *p = a.pos + static_cast<int64_t>(r) * 3 * nn + ij;
+        x = cadd(x, cscale(p[0], ev.E[0]));
+        x = cadd(x, cscale(p[nn], ev.E[1]));
+        x = cadd(x, cscale(p[2 * nn], ev.E[2]));
+    } else if (type == kMatH0) {
+        x = h;
+    } else {  // J_a = -i R_a H (Wannier90.h:373-384)
+        const double Ra = a.rcart[3 * r + (type - kMatJx)];
+        x = cmk(h.y * Ra, -h.x * Ra);
+    }
+    return cmul(cscale(phase, a.rweight[r]), x);
+}
+
+__global__ __launch_bounds__(256) void nb_fourier_kernel(const NbArgs a, const NbEval ev, const cd *__restrict__ P,
+                                                         cd *__restrict__ out)
+{
+    __shared__ cd As[GK][GM];
+    __shared__ cd Bs[GK][GN];
+    __shared__ cd ph[GK];
+    const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+    const int64_t k0 = static_cast<int64_t>(blockIdx.x) * GM;
+    const int64_t c0 = static_cast<int64_t>(blockIdx.y) * GN;
+    const int nn = a.N * a.N;
+    const int64_t ncols = static_cast<int64_t>(ev.nmat) * nn;
+    cd acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = cmk(0.0, 0.0);
+
+    for (int r0 = 0; r0 < a.nr; r0 += GK) {
+        // e^{i A(t).R} for this slice
+        if (threadIdx.x < GK) {
+            const int r = r0 + threadIdx.x;
+            cd v = cmk(0.0, 0.0);
+            if (r < a.nr) {
+                const double *R = a.rcart + 3 * r;
+                double s, c;
+                sincos(ev.A[0] * R[0] + ev.A[1] * R[1] + ev.A[2] * R[2], &s, &c);
+                v = cmk(c, s);
+            }
+            ph[threadIdx.x] = v;
+        }
+        // A tile: P[k0 + m][r0 + kk]
+        for (int t = threadIdx.x; t < GM * GK; t += 256) {
+            const int m = t / GK, kk = t % GK;
+            const int64_t k = k0 + m;
+            const int r = r0 + kk;
+            As[kk][m] = (k < a.nk_stride && r < a.nr) ? P[k * a.nr + r] : cmk(0.0, 0.0);
+        }
+        __syncthreads();
+        for (int t = threadIdx.x; t < GK * GN; t += 256) {
+            const int kk = t / GN, n = t % GN;
+            const int r = r0 + kk;
+            const int64_t c = c0 + n;
+            Bs[kk][n] = (r < a.nr && c < ncols) ? b_element(a, ev, r, c, ph[kk]) : cmk(0.0, 0.0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < GK; ++kk) {
+            cd av[4], bv[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) av[i] = As[kk][ty + 16 * i];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) bv[j] = Bs[kk][tx + 16 * j];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) cfma(acc[i][j], av[i], bv[j]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int64_t k = k0 + ty + 16 * i;
+        if (k >= a.nk_stride) continue;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int64_t c = c0 + tx + 16 * j;
+            if (c >= ncols) continue;
+            const int mat = static_cast<int>(c / nn), ij = static_cast<int>(c % nn);
+            out[(static_cast<int64_t>(mat) * a.nk_stride + k) * nn + ij] = acc[i][j];
+        }
+    }
+}
+
+// ------------------------------------------------- wave matrix helpers --
+// All matrices are N x N complex, row-major, in LDS; one wave (64 lanes).
+__device__ __forceinline__ void wave_sync() { __syncthreads(); }
+
+// C = op(A) op(B); adjA/adjB select the conjugate transpose.
+__device__ void wmm(cd *C, const cd *A, const cd *B, int N, bool adjA, bool adjB)
+{
+    const int nn = N * N;
+    for (int e = threadIdx.x; e < nn; e += 64) {
+        const int i = e / N, j = e % N;
+        cd s = cmk(0.0, 0.0);
+        for (int l = 0; l < N; ++l) {
+            const cd x = adjA ? cconj(A[l * N + i]) : A[i * N + l];
+            const cd y = adjB ? cconj(B[j * N + l]) : B[l * N + j];
+            cfma(s, x, y);
+        }
+        C[e] = s;
+    }
+    wave_sync();
+}
+
+__device__ void wcopy(cd *D, const cd *S, int nn)
+{
+    for (int e = threadIdx.x; e < nn; e += 64) D[e] = S[e];
+    wave_sync();
+}
+
+// Hermitian eigensolver: A (destroyed) -> ev ascending, U columns = vectors.
+// Parallel-order cyclic Jacobi with complex rotations G = diag(1, e^{-i phi})
+// R(theta); rotation pairs follow the round-robin tournament. work: N*N.
+__device__ void weigh(cd *A, cd *U, double *ev, cd *work, int N, double4 *rot)
+{
+    const int lane = threadIdx.x;
+    const int nn = N * N;
+    const int M = (N + 1) & ~1;  // even number of players (a dummy if N odd)
+    for (int e = lane; e < nn; e += 64) {
+        const int i = e / N, j = e % N;
+        U[e] = cmk(i == j ? 1.0 : 0.0, 0.0);
+        if (i == j) A[e].y = 0.0;
+    }
+    wave_sync();
+    for (int sweep = 0; sweep < 40; ++sweep) {
+        double off = 0.0, dg = 0.0;
+        for (int e = lane; e < nn; e += 64) {
+            const int i = e / N, j = e % N;
+            if (i == j) dg += A[e].x * A[e].x;
+            else off += A[e].x * A[e].x + A[e].y * A[e].y;
+        }
+        off = wave_sum(off);
+        dg = wave_sum(dg);
+        if (off <= 1e-30 * dg || off < 1e-300) break;
+        for (int round = 0; round < M - 1; ++round) {
+            if (lane < M / 2) {
+                // round-robin: player 0 fixed, players 1..M-1 rotate
+                auto pos = [&](int s) { return s == 0 ? 0 : 1 + (s - 1 + round) % (M - 1); };
+                int p = pos(lane), q = pos(M - 1 - lane);
+                if (p > q) { const int t = p; p = q; q = t; }
+                double4 g = make_double4(1.0, 0.0, 1.0, 0.0);  // (c, s, e^{-i phi})
+                int pq = -1;
+                if (q < N) {
+                    const cd b = A[p * N + q];
+                    const double ab = sqrt(b.x * b.x + b.y * b.y);
+                    if (ab > 1e-300) {
+                        const double th = (A[q * N + q].x - A[p * N + p].x) / (2.0 * ab);
+                        const double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+                        const double c = 1.0 / sqrt(t * t + 1.0);
+                        g = make_double4(c, t * c, b.x / ab, -b.y / ab);
+                        pq = p * 64 + q;
+                    }
+                }
+                rot[lane] = g;
+                reinterpret_cast<int *>(rot + 32)[lane] = pq;
+            }
+            wave_sync();
+            const int *pqs = reinterpret_cast<const int *>(rot + 32);
+            const int half = M / 2;
+            // columns: A <- A G, U <- U G
+            for (int t = lane; t < half * N; t += 64) {
+                const int j = t / N, r = t % N;
+                const int pq = pqs[j];
+                if (pq < 0) continue;
+                const int p = pq / 64, q = pq % 64;
+                const double4 g = rot[j];
+                const cd em = cmk(g.z, g.w);  // e^{-i phi}
+                const cd gqp = cscale(em, -g.y), gqq = cscale(em, g.x);
+                cd ap = A[r * N + p], aq = A[r * N + q];
+                A[r * N + p] = cadd(cscale(ap, g.x), cmul(aq, gqp));
+                A[r * N + q] = cadd(cscale(ap, g.y), cmul(aq, gqq));
+                cd up = U[r * N + p], uq = U[r * N + q];
+                U[r * N + p] = cadd(cscale(up, g.x), cmul(uq, gqp));
+                U[r * N + q] = cadd(cscale(up, g.y), cmul(uq, gqq));
+            }
+            wave_sync();
+            // rows: A <- G^dag A
+            for (int t = lane; t < half * N; t += 64) {
+                const int j = t / N, r = t % N;
+                const int pq = pqs[j];
+                if (pq < 0) continue;
+                const int p = pq / 64, q = pq % 64;
+                const double4 g = rot[j];
+                const cd ep = cmk(g.z, -g.w);  // e^{+i phi}
+                const cd ap = A[p * N + r], aq = A[q * N + r];
+                A[p * N + r] = csub(cscale(ap, g.x), cmul(cscale(ep, g.y), aq));
+                A[q * N + r] = cadd(cscale(ap, g.y), cmul(cscale(ep, g.x), aq));
+            }
+            wave_sync();
+            // exact zeros / real diagonal of the rotated 2x2 blocks
+            if (lane < half) {
+                const int pq = pqs[lane];
+                if (pq >= 0) {
+                    const int p = pq / 64, q = pq % 64;
+                    A[p * N + q] = cmk(0.0, 0.0);
+                    A[q * N + p] = cmk(0.0, 0.0);
+                    A[p * N + p].y = 0.0;
+                    A[q * N + q].y = 0.0;
+                }
+            }
+            wave_sync();
+        }
+    }
+    // ascending order: rank of each eigenvalue (ties by index), permute U
+    if (lane < N) {
+        const double e = A[lane * N + lane].x;
+        int rank = 0;
+        for (int j = 0; j < N; ++j) {
+            const double f = A[j * N + j].x;
+            rank += (f < e) || (f == e && j < lane);
+        }
+        ev[rank] = e;
+        for (int r = 0; r < N; ++r) work[r * N + rank] = U[r * N + lane];
+    }
+    wave_sync();
+    wcopy(U, work, nn);
+}
+
+// LDS layout of one wave: kMats matrices of N*N complex, N eigenvalues,
+// 32 rotation slots + 32 int pair slots.
+__device__ __forceinline__ cd *mat_slot(cd *base, int i, int nn) { return base + static_cast<int64_t>(i) * nn; }
+
+// ----------------------------------------------------------- init state --
+// rho(k) = U diag(occupation) U^dag with U = eig(H(k)) (SBEs.h:284-364).
+__global__ __launch_bounds__(64) void nb_init_kernel(const NbArgs a, const cd *__restrict__ H0, cd *__restrict__ rho)
+{
+    extern __shared__ double4 smem4[];
+    const int N = a.N, nn = N * N;
+    cd *base = reinterpret_cast<cd *>(smem4);
+    cd *A = mat_slot(base, 0, nn), *U = mat_slot(base, 1, nn), *W = mat_slot(base, 2, nn);
+    double *ev = reinterpret_cast<double *>(mat_slot(base, 3, nn));
+    double4 *rot = reinterpret_cast<double4 *>(ev + 64);
+    const int64_t k = blockIdx.x;
+    cd *out = rho + k * nn;
+    if (k >= a.nk) {
+        for (int e = threadIdx.x; e < nn; e += 64) out[e] = cmk(0.0, 0.0);
+        return;
+    }
+    wcopy(A, H0 + k * nn, nn);
+    weigh(A, U, ev, W, N, rot);
+    // occupations: first nval bands, or Fermi-Dirac normalised to 1
+    __shared__ double occ[64];
+    if (threadIdx.x == 0) {
+        double sum = 0.0;
+        for (int m = 0; m < N; ++m) {
+            double f;
+            if (a.fermi) f = 1.0 / (exp((ev[m] - a.fermi_energy) / (300.0 / 3.15775024804e5)) + 1.0);
+            else f = m < a.nval ? 1.0 : 0.0;
+            occ[m] = f;
+            sum += f;
+        }
+        if (a.fermi)
+            for (int m = 0; m < N; ++m) occ[m] /= sum;
+    }
+    wave_sync();
+    for (int e = threadIdx.x; e < nn; e += 64) {
+        const int i = e / N, j = e % N;
+        cd s = cmk(0.0, 0.0);
+        for (int m = 0; m < N; ++m) cfma(s, cscale(U[i * N + m], occ[m]), cconj(U[j * N + m]));
+        out[e] = s;
+    }
+}
+
+// ---------------------------------------------------------- time sample --
+// Per k at t_n: U = eig(H(k+A)), rho_H = U^dag rho U, J_H = U^dag J U;
+// intra = sum_i Re(rho_H,ii J_H,ii), inter = the rest (or the total
+// Re Tr(rho J) when inter_intra == 0); band populations diag(rho_H); and
+// w rho for the orbital-basis sum. Each wave walks k = wave, wave + G, ...
+// and writes one row of partial sums: [inter(3) intra(3) pop(N) rho(2N^2)].
+__global__ __launch_bounds__(64) void nb_sample_kernel(const NbArgs a, const cd *__restrict__ mats,
+                                                       const cd *__restrict__ rho, double *__restrict__ partials,
+                                                       int64_t step_local)
+{
+    extern __shared__ double4 smem4[];
+    const int N = a.N, nn = N * N;
+    cd *base = reinterpret_cast<cd *>(smem4);
+    cd *A = mat_slot(base, 0, nn), *U = mat_slot(base, 1, nn), *W = mat_slot(base, 2, nn);
+    cd *R = mat_slot(base, 3, nn), *T = mat_slot(base, 4, nn), *X = mat_slot(base, 5, nn);
+    double *ev = reinterpret_cast<double *>(mat_slot(base, 6, nn));
+    double4 *rot = reinterpret_cast<double4 *>(ev + 64);
+    const int lane = threadIdx.x;
+    const int nv = a.nv;
+    // per-lane accumulators: rho elements e = lane + 64 t
+    constexpr int kMaxPer = 1024 / 64;
+    double racc[2 * kMaxPer];
+    for (int t = 0; t < 2 * kMaxPer; ++t) racc[t] = 0.0;
+    double cur[6] = {0, 0, 0, 0, 0, 0};
+    double pop = 0.0;  // lane m < N: population of band m
+    const cd *H0 = mats;                              // mat 0: H0
+    const cd *J = mats + a.nk_stride * nn;            // mats 1..3: J_x, J_y, J_z
+    for (int64_t k = blockIdx.x; k < a.nk; k += gridDim.x) {
+        const double w = a.weight[k];
+        wcopy(A, H0 + k * nn, nn);
+        wcopy(R, rho + k * nn, nn);
+        weigh(A, U, ev, W, N, rot);
+        wmm(T, R, U, N, false, false);
+        wmm(X, U, T, N, true, false);  // rho_H
+        if (lane < N) pop += X[lane * N + lane].x * w;
+        for (int e = lane, t = 0; e < nn; e += 64, ++t) {
+            racc[2 * t] += R[e].x * w;
+            racc[2 * t + 1] += R[e].y * w;
+        }
+        for (int ax = 0; ax < 3; ++ax) {
+            const cd *Ja = J + static_cast<int64_t>(ax) * a.nk_stride * nn + k * nn;
+            double inter = 0.0, intra = 0.0;
+            if (a.inter_intra) {
+                wcopy(A, Ja, nn);
+                wmm(T, A, U, N, false, false);
+                wmm(W, U, T, N, true, false);  // J_H
+                for (int e = lane; e < nn; e += 64) {
+                    const int i = e / N, j = e % N;
+                    const cd r = X[i * N + j], jj = W[j * N + i];
+                    const double v = r.x * jj.x - r.y * jj.y;
+                    if (i == j) intra += v;
+                    else inter += v;
+                }
+            } else {
+                for (int e = lane; e < nn; e += 64) {
+                    const int i = e / N, j = e % N;
+                    const cd r = R[i * N + j], jj = Ja[j * N + i];
+                    inter += r.x * jj.x - r.y * jj.y;
+                }
+            }
+            cur[ax] += wave_sum(inter) * w;
+            cur[3 + ax] += wave_sum(intra) * w;
+            wave_sync();
+        }
+    }
+    double *row = partials + (step_local * gridDim.x + blockIdx.x) * static_cast<int64_t>(nv);
+    if (lane < 6) row[lane] = cur[lane];
+    if (lane < N) row[6 + lane] = pop;
+    for (int e = lane, t = 0; e < nn; e += 64, ++t) {
+        row[6 + N + 2 * e] = racc[2 * t];
+        row[6 + N + 2 * e + 1] = racc[2 * t + 1];
+    }
+}
+
+// ------------------------------------------------------------- RK stage --
+// Stage i of the step: y = rho + dt sum_j a_ij k_j; k_i = f(y) with
+// f(y) = -i(H~ y - (H~ y)^dag) - U (Gamma o (U^dag y U)) U^dag; after the
+// last stage rho += dt sum_i b_i k_i (SBEs.h:596-624, 762-861).
+__global__ __launch_bounds__(64) void nb_rhs_kernel(const NbArgs a, const NbStage sg, const cd *__restrict__ mats,
+                                                    cd *__restrict__ rho, cd *__restrict__ kbuf)
+{
+    extern __shared__ double4 smem4[];
+    const int N = a.N, nn = N * N;
+    cd *base = reinterpret_cast<cd *>(smem4);
+    cd *A = mat_slot(base, 0, nn), *U = mat_slot(base, 1, nn), *W = mat_slot(base, 2, nn);
+    cd *Y = mat_slot(base, 3, nn), *T = mat_slot(base, 4, nn), *X = mat_slot(base, 5, nn);
+    double *ev = reinterpret_cast<double *>(mat_slot(base, 6, nn));
+    double4 *rot = reinterpret_cast<double4 *>(ev + 64);
+    const int64_t k = blockIdx.x;
+    if (k >= a.nk) return;
+    const int lane = threadIdx.x;
+    const int64_t kstride = a.nk_stride * nn;
+    cd *rk = rho + k * nn;
+    // stage input
+    for (int e = lane; e < nn; e += 64) {
+        cd y = rk[e];
+        for (int j = 0; j < sg.stage; ++j)
+            if (sg.a[j] != 0.0) y = cadd(y, cscale(kbuf[j * kstride + k * nn + e], sg.a[j]));
+        Y[e] = y;
+    }
+    const cd *Ht = mats + k * nn;                 // mat 0: H~ = H + E.D
+    wcopy(A, Ht, nn);
+    wmm(T, A, Y, N, false, false);                // T = H~ y
+    cd *ko = kbuf + sg.stage * kstride + k * nn;
+    // -i (T - T^dag): (-i)(t_ij - conj(t_ji))
+    for (int e = lane; e < nn; e += 64) {
+        const int i = e / N, j = e % N;
+        const cd d = csub(T[i * N + j], cconj(T[j * N + i]));
+        X[e] = cmk(d.y, -d.x);
+    }
+    wave_sync();
+    if (a.deph) {
+        const cd *H0 = mats + a.nk_stride * nn + k * nn;  // mat 1: H
+        wcopy(A, H0, nn);
+        weigh(A, U, ev, W, N, rot);
+        wmm(T, Y, U, N, false, false);
+        wmm(W, U, T, N, true, false);             // U^dag y U
+        for (int e = lane; e < nn; e += 64) {
+            const int i = e / N, j = e % N;
+            W[e] = cscale(W[e], i == j ? a.gam1 : a.gam2);
+        }
+        wave_sync();
+        wmm(T, U, W, N, false, false);
+        wmm(A, T, U, N, false, true);             // U (G o ...) U^dag
+        for (int e = lane; e < nn; e += 64) X[e] = csub(X[e], A[e]);
+        wave_sync();
+    }
+    for (int e = lane; e < nn; e += 64) ko[e] = X[e];
+    if (sg.last) {
+        for (int e = lane; e < nn; e += 64) {
+            cd r = rk[e];
+            for (int j = 0; j < sg.nstage; ++j)
+                if (sg.b[j] != 0.0) r = cadd(r, cscale(j == sg.stage ? X[e] : kbuf[j * kstride + k * nn + e], sg.b[j]));
+            rk[e] = r;
+        }
+    }
+}
+
+}  // namespace
+
+size_t nb_lds_bytes(int N)
+{
+    const size_t nn = static_cast<size_t>(N) * N;
+    return 6 * nn * sizeof(double2) + 64 * sizeof(double) + 64 * sizeof(double4);
+}
+
+hipError_t nb_launch_phase(const NbArgs &a, double2 *P, hipStream_t s)
+{
+    const int64_t total = a.nk_stride * a.nr;
+    nb_phase_kernel<<<dim3(static_cast<unsigned>((total + 255) / 256)), 256, 0, s>>>(a, P);
+    return hipGetLastError();
+}
+
+hipError_t nb_launch_fourier(const NbArgs &a, const NbEval &ev, const double2 *P, double2 *out, hipStream_t s)
+{
+    const int64_t ncols = static_cast<int64_t>(ev.nmat) * a.N * a.N;
+    const dim3 grid(static_cast<unsigned>((a.nk_stride + GM - 1) / GM), static_cast<unsigned>((ncols + GN - 1) / GN));
+    nb_fourier_kernel<<<grid, 256, 0, s>>>(a, ev, P, out);
+    return hipGetLastError();
+}
+
+hipError_t nb_launch_init(const NbArgs &a, const double2 *H0, double2 *rho, hipStream_t s)
+{
+    nb_init_kernel<<<dim3(static_cast<unsigned>(a.nk_stride)), 64, nb_lds_bytes(a.N), s>>>(a, H0, rho);
+    return hipGetLastError();
+}
+
+hipError_t nb_launch_sample(const NbArgs &a, const double2 *mats, const double2 *rho, double *partials,
+                            int64_t step_local, hipStream_t s)
+{
+    nb_sample_kernel<<<dim3(static_cast<unsigned>(a.sample_waves)), 64, nb_lds_bytes(a.N), s>>>(a, mats, rho, partials,
+                                                                                               step_local);
+    return hipGetLastError();
+}
+
+hipError_t nb_launch_rhs(const NbArgs &a, const NbStage &sg, const double2 *mats, double2 *rho, double2 *kbuf,
+                         hipStream_t s)
+{
+    nb_rhs_kernel<<<dim3(static_cast<unsigned>(a.nk)), 64, nb_lds_bytes(a.N), s>>>(a, sg, mats, rho, kbuf);
+    return hipGetLastError();
+}
+
+}  // namespace sbe

@@ -1,0 +1,58 @@
+// nband.hpp - N-band (Wannier) path: kernel arguments and launchers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace sbe {
+
+// Matrix kinds an evaluation of the Fourier sums can produce.
+enum : int { kMatHtilde = 0, kMatH0 = 1, kMatJx = 2, kMatJy = 3, kMatJz = 4 };
+
+constexpr int kNbMaxBands = 32;
+
+struct NbArgs {
+    int N;                 // bands
+    int nr;                // lattice vectors R
+    int nval;
+    int fermi;             // Fermi-Dirac initial occupations
+    double fermi_energy;
+    int inter_intra;
+    int deph;              // any dephasing on (T1 or T2)
+    double gam1, gam2;     // 1/T1, 1/T2 (0 = off)
+    int64_t nk, nk_stride;
+    int nv;                // values per sample row: 6 + N + 2 N^2
+    int sample_waves;      // rows of the per-step partial sums
+    const double *kpts;    // [nk_stride][3]
+    const double *weight;  // [nk_stride]
+    const double *rcart;   // [nr][3] cartesian R (bohr)
+    const double *rweight; // [nr] 1/degeneracy
+    const double2 *ham;    // [nr][N][N]
+    const double2 *pos;    // [nr][3][N][N]
+};
+
+// One evaluation of the Fourier sums at k + A(t) with field E(t).
+struct NbEval {
+    double A[3], E[3];
+    int nmat;
+    int mats[5];
+};
+
+// One RK stage: coefficients already multiplied by dt.
+struct NbStage {
+    int stage, nstage, last;
+    double a[8];  // dt a_{stage,j}
+    double b[8];  // dt b_j (used when last)
+};
+
+size_t nb_lds_bytes(int N);
+hipError_t nb_launch_phase(const NbArgs &a, double2 *P, hipStream_t s);
+hipError_t nb_launch_fourier(const NbArgs &a, const NbEval &ev, const double2 *P, double2 *out, hipStream_t s);
+hipError_t nb_launch_init(const NbArgs &a, const double2 *H0, double2 *rho, hipStream_t s);
+hipError_t nb_launch_sample(const NbArgs &a, const double2 *mats, const double2 *rho, double *partials,
+                            int64_t step_local, hipStream_t s);
+hipError_t nb_launch_rhs(const NbArgs &a, const NbStage &sg, const double2 *mats, double2 *rho, double2 *kbuf,
+                         hipStream_t s);
+
+}  // namespace sbe

@@ -1,0 +1,267 @@
+// nband_host.cpp - host orchestration of the N-band (Wannier) path: device
+// tables, the initial state and the per-step launch sequence
+//   sample:  Fourier sums {H, J_x, J_y, J_z} at t_n  -> nb_sample_kernel
+//   stage i: Fourier sums {H + E.D, H} at t_n + c_i dt -> nb_rhs_kernel
+// (antelope.cpp:297-338 with SBEs.h:596-624).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "nband.hpp"
+#include "sbe_internal.hpp"
+
+namespace sbe {
+
+namespace {
+
+int hipc(sbe_ctx *ctx, hipError_t e, const char *what)
+{
+    if (e == hipSuccess) return SBE_OK;
+    ctx->err = std::string(what) + ": " + hipGetErrorString(e);
+    return (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? SBE_ERR_OOM : SBE_ERR_HIP;
+}
+
+template <typename T>
+T *nalloc(sbe_ctx *ctx, size_t bytes, int &rc)
+{
+    if (rc) return nullptr;
+    void *p = nullptr;
+    const hipError_t e = hipMalloc(&p, bytes ? bytes : 1);
+    if (e != hipSuccess) {
+        rc = hipc(ctx, e, "hipMalloc");
+        return nullptr;
+    }
+    ctx->owned.push_back({p, bytes});
+    ctx->device_bytes += static_cast<int64_t>(bytes);
+    return static_cast<T *>(p);
+}
+
+constexpr size_t kNbPartialBytes = size_t(128) << 20;
+
+struct NbPlan {
+    int64_t nk_stride;
+    int sample_waves, nv;
+    int64_t chunk;
+    size_t bytes_P, bytes_rho, bytes_mats, bytes_k, bytes_tab, bytes_partials, bytes_series;
+};
+
+NbPlan nb_plan(int N, int nr, int64_t nk, int64_t nt, int nstage)
+{
+    NbPlan p{};
+    p.nk_stride = (nk + 63) / 64 * 64;
+    p.sample_waves = static_cast<int>(std::min<int64_t>(nk, 2048));
+    p.nv = 6 + N + 2 * N * N;
+    const size_t nn = static_cast<size_t>(N) * N;
+    const size_t row = static_cast<size_t>(p.sample_waves) * p.nv * sizeof(double);
+    p.chunk = std::max<int64_t>(1, std::min<int64_t>(nt, static_cast<int64_t>(kNbPartialBytes / row)));
+    p.bytes_P = static_cast<size_t>(p.nk_stride) * nr * sizeof(double2);
+    p.bytes_rho = static_cast<size_t>(p.nk_stride) * nn * sizeof(double2);
+    p.bytes_mats = 4 * p.bytes_rho;
+    p.bytes_k = static_cast<size_t>(nstage) * p.bytes_rho;
+    p.bytes_tab = static_cast<size_t>(p.nk_stride) * 4 * sizeof(double) + static_cast<size_t>(nr) * 4 * sizeof(double) +
+                  static_cast<size_t>(nr) * 4 * nn * sizeof(double2);
+    p.bytes_partials = static_cast<size_t>(p.chunk) * row;
+    p.bytes_series = static_cast<size_t>(nt) * p.nv * sizeof(double);
+    return p;
+}
+
+}  // namespace
+
+size_t nb_footprint(const sbe_model &m, int64_t nk, int64_t nt, int nstage)
+{
+    const NbPlan p = nb_plan(m.nband, m.nrpts, nk, nt, nstage);
+    return p.bytes_P + p.bytes_rho + p.bytes_mats + p.bytes_k + p.bytes_tab + p.bytes_partials + p.bytes_series;
+}
+
+int nb_validate(const sbe_model &m, std::string &err)
+{
+    if (m.nband < 1 || m.nband > kNbMaxBands) {
+        err = "Wannier path supports 1..32 bands";
+        return SBE_ERR_UNSUPPORTED;
+    }
+    if (m.nrpts < 1 || !m.rvec || !m.rweight || !m.ham) {
+        err = "Wannier model needs nrpts >= 1, rvec, rweight and ham";
+        return SBE_ERR_MODEL;
+    }
+    if (m.is_dipole_zero) {
+        err = "the GPU Wannier path uses the dipole form (isDipoleZero = false, as Wannier90.h:124 sets)";
+        return SBE_ERR_UNSUPPORTED;
+    }
+    if (!m.pos) {
+        err = "Wannier model with dipoles needs pos";
+        return SBE_ERR_MODEL;
+    }
+    if (m.nval < 0 || m.nval > m.nband) {
+        err = "nval must be within 0..nband";
+        return SBE_ERR_MODEL;
+    }
+    return SBE_OK;
+}
+
+int nb_setup(sbe_ctx *ctx)
+{
+    const sbe_model &m = ctx->model;
+    const int N = m.nband, nr = m.nrpts;
+    const int64_t nk = ctx->grid.k_end - ctx->grid.k_begin;
+    const NbPlan p = nb_plan(N, nr, nk, ctx->laser.nt, ctx->tab.nstage);
+    const size_t nn = static_cast<size_t>(N) * N;
+    ctx->nk = nk;
+    ctx->nk_stride = p.nk_stride;
+    ctx->nv = p.nv;
+    ctx->chunk_steps = p.chunk;
+    int rc = SBE_OK;
+    ctx->nb_P = nalloc<double2>(ctx, p.bytes_P, rc);
+    ctx->nb_rho = nalloc<double2>(ctx, p.bytes_rho, rc);
+    ctx->nb_mats = nalloc<double2>(ctx, p.bytes_mats, rc);
+    ctx->nb_k = nalloc<double2>(ctx, p.bytes_k, rc);
+    char *tab = nalloc<char>(ctx, p.bytes_tab, rc);
+    ctx->d_partials = nalloc<double>(ctx, p.bytes_partials, rc);
+    ctx->d_series = nalloc<double>(ctx, p.bytes_series, rc);
+    if (rc) return rc;
+
+    // host copies of the tables: k [nk_stride][3], w, R cartesian, weights, H, r
+    std::vector<double> hk(static_cast<size_t>(p.nk_stride) * 4, 0.0);
+    for (int64_t k = 0; k < nk; ++k) {
+        for (int c = 0; c < 3; ++c) hk[3 * k + c] = ctx->grid.k[3 * k + c];
+        hk[3 * p.nk_stride + k] = ctx->grid.w[k];
+    }
+    std::vector<double> hr(static_cast<size_t>(nr) * 4, 0.0);
+    for (int r = 0; r < nr; ++r) {
+        const int32_t *n = m.rvec + 3 * r;  // R = n1 a1 + n2 a2 + n3 a3 (Wannier90.h:193-204)
+        for (int i = 0; i < 3; ++i)
+            hr[3 * r + i] = n[0] * m.lattice[i] + n[1] * m.lattice[3 + i] + n[2] * m.lattice[6 + i];
+        hr[3 * nr + r] = m.rweight[r];
+    }
+    const size_t bytes_k = hk.size() * sizeof(double), bytes_r = hr.size() * sizeof(double);
+    double *dk = reinterpret_cast<double *>(tab);
+    double *dr = reinterpret_cast<double *>(tab + bytes_k);
+    double2 *dham = reinterpret_cast<double2 *>(tab + bytes_k + bytes_r);
+    double2 *dpos = dham + static_cast<size_t>(nr) * nn;
+    rc = hipc(ctx, hipMemcpy(dk, hk.data(), bytes_k, hipMemcpyHostToDevice), "upload k");
+    if (!rc) rc = hipc(ctx, hipMemcpy(dr, hr.data(), bytes_r, hipMemcpyHostToDevice), "upload R");
+    if (!rc) rc = hipc(ctx, hipMemcpy(dham, m.ham, static_cast<size_t>(nr) * nn * sizeof(double2), hipMemcpyHostToDevice), "upload H");
+    if (!rc) rc = hipc(ctx, hipMemcpy(dpos, m.pos, static_cast<size_t>(nr) * 3 * nn * sizeof(double2), hipMemcpyHostToDevice), "upload r");
+    if (rc) return rc;
+
+    NbArgs &a = ctx->nargs;
+    a.N = N;
+    a.nr = nr;
+    a.nval = m.nval;
+    a.fermi = ctx->cfg.init_type == SBE_INIT_FERMI_DIRAC;
+    a.fermi_energy = ctx->cfg.fermi_energy;
+    a.inter_intra = ctx->cfg.inter_intra ? 1 : 0;
+    a.gam2 = ctx->cfg.T2 <= 1e-16 ? 0.0 : 1.0 / ctx->cfg.T2;  // SBEs.h:484-493
+    a.gam1 = ctx->cfg.T1 <= 1e-16 ? 0.0 : 1.0 / ctx->cfg.T1;
+    a.deph = (a.gam1 != 0.0 || a.gam2 != 0.0) ? 1 : 0;
+    a.nk = nk;
+    a.nk_stride = p.nk_stride;
+    a.nv = p.nv;
+    a.sample_waves = p.sample_waves;
+    a.kpts = dk;
+    a.weight = dk + 3 * p.nk_stride;
+    a.rcart = dr;
+    a.rweight = dr + 3 * nr;
+    a.ham = dham;
+    a.pos = dpos;
+
+    // e^{ik.R}, then rho(k) from H(k) (A(atmin) = 0)
+    rc = hipc(ctx, nb_launch_phase(a, ctx->nb_P, ctx->stream), "phase kernel");
+    NbEval ev{};
+    ev.nmat = 1;
+    ev.mats[0] = kMatH0;
+    if (!rc) rc = hipc(ctx, nb_launch_fourier(a, ev, ctx->nb_P, ctx->nb_mats, ctx->stream), "fourier kernel");
+    if (!rc) rc = hipc(ctx, nb_launch_init(a, ctx->nb_mats, ctx->nb_rho, ctx->stream), "init kernel");
+    if (!rc) rc = hipc(ctx, hipStreamSynchronize(ctx->stream), "init sync");
+    return rc;
+}
+
+int nb_run(sbe_ctx *ctx, int64_t step_begin, int64_t step_end)
+{
+    const NbArgs &a = ctx->nargs;
+    const Tableau &tb = ctx->tab;
+    const double dt = ctx->laser.dt;
+    int rc = SBE_OK;
+    for (int64_t c0 = step_begin; c0 < step_end && !rc; c0 += ctx->chunk_steps) {
+        const int64_t n = std::min<int64_t>(ctx->chunk_steps, step_end - c0);
+        for (int64_t s = 0; s < n && !rc; ++s) {
+            const int64_t step = c0 + s;
+            const double tn = ctx->laser.atmin + dt * static_cast<double>(step);  // antelope.cpp:299
+            NbEval ev{};
+            double Ev[3];
+            ctx->laser.eval(tn, ev.A, Ev);
+            ev.nmat = 4;
+            ev.mats[0] = kMatH0;
+            ev.mats[1] = kMatJx;
+            ev.mats[2] = kMatJy;
+            ev.mats[3] = kMatJz;
+            rc = hipc(ctx, nb_launch_fourier(a, ev, ctx->nb_P, ctx->nb_mats, ctx->stream), "fourier kernel");
+            if (!rc)
+                rc = hipc(ctx, nb_launch_sample(a, ctx->nb_mats, ctx->nb_rho, ctx->d_partials, s, ctx->stream),
+                          "sample kernel");
+            for (int i = 0; i < tb.nstage && !rc; ++i) {
+                NbEval se{};
+                ctx->laser.eval(tn + tb.c[i] * dt, se.A, se.E);  // SBEs.h:612
+                se.nmat = a.deph ? 2 : 1;
+                se.mats[0] = kMatHtilde;
+                se.mats[1] = kMatH0;
+                NbStage sg{};
+                sg.stage = i;
+                sg.nstage = tb.nstage;
+                sg.last = i == tb.nstage - 1;
+                for (int j = 0; j < i; ++j) sg.a[j] = dt * tb.a[i][j];
+                for (int j = 0; j < tb.nstage; ++j) sg.b[j] = dt * tb.b[j];
+                rc = hipc(ctx, nb_launch_fourier(a, se, ctx->nb_P, ctx->nb_mats, ctx->stream), "fourier kernel");
+                if (!rc) rc = hipc(ctx, nb_launch_rhs(a, sg, ctx->nb_mats, ctx->nb_rho, ctx->nb_k, ctx->stream), "rhs kernel");
+            }
+        }
+        if (!rc)
+            rc = hipc(ctx, launch_series_reduce(ctx->d_partials, n, a.sample_waves, a.nv, ctx->d_series + c0 * a.nv, ctx->stream),
+                      "series kernel");
+        if (!rc) ctx->steps_done = c0 + n;
+    }
+    return rc;
+}
+
+int nb_series(sbe_ctx *ctx, double *j_inter, double *j_intra, double *rho_bar, double *band_pop)
+{
+    const int N = ctx->N, nv = ctx->nv;
+    const int64_t nt = ctx->laser.nt;
+    std::vector<double> s(static_cast<size_t>(nt) * nv);
+    int rc = hipc(ctx, hipMemcpyAsync(s.data(), ctx->d_series, s.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "copy series");
+    if (!rc) rc = hipc(ctx, hipStreamSynchronize(ctx->stream), "sync series");
+    if (rc) return rc;
+    bool finite = true;
+    for (int64_t n = 0; n < nt; ++n) {
+        const double *r = s.data() + n * nv;
+        if (n < ctx->steps_done)
+            for (int i = 0; i < nv; ++i) finite = finite && std::isfinite(r[i]);
+        for (int ax = 0; ax < 3; ++ax) {
+            if (j_inter) j_inter[3 * n + ax] = r[ax];
+            if (j_intra) j_intra[3 * n + ax] = ctx->cfg.inter_intra ? r[3 + ax] : 0.0;
+        }
+        if (band_pop)
+            for (int m = 0; m < N; ++m) band_pop[n * N + m] = r[6 + m];
+        if (rho_bar) std::memcpy(rho_bar + n * 2 * N * N, r + 6 + N, sizeof(double) * 2 * N * N);
+    }
+    if (!finite) {
+        ctx->err = "non-finite value in the propagated state";
+        return SBE_ERR_NAN;
+    }
+    return SBE_OK;
+}
+
+int nb_state(sbe_ctx *ctx, int basis, double *rho)
+{
+    if (basis != SBE_BASIS_ORBITAL) {
+        ctx->err = "band-basis export is available for the 2-band path only";
+        return SBE_ERR_UNSUPPORTED;
+    }
+    const size_t bytes = static_cast<size_t>(ctx->nk) * ctx->N * ctx->N * sizeof(double2);
+    int rc = hipc(ctx, hipMemcpyAsync(rho, ctx->nb_rho, bytes, hipMemcpyDeviceToHost, ctx->stream), "copy state");
+    if (!rc) rc = hipc(ctx, hipStreamSynchronize(ctx->stream), "sync state");
+    return rc;
+}
+
+}  // namespace sbe

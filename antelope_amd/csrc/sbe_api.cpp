@@ -90,10 +90,7 @@ int validate(const sbe_config *cfg, const sbe_model *m, std::string &err)
         }
         return SBE_OK;
     }
-    if (m->kind == SBE_MODEL_WANNIER) {
-        err = "the Wannier (N-band) path is not available in this build";
-        return SBE_ERR_UNSUPPORTED;
-    }
+    if (m->kind == SBE_MODEL_WANNIER) return sbe::nb_validate(*m, err);
     err = "unknown model kind";
     return SBE_ERR_MODEL;
 }
@@ -260,6 +257,10 @@ int sbe_query_footprint(const sbe_config *cfg, const sbe_model *model, int64_t k
         k_end = ntot;
     }
     if (k_begin < 0 || k_end > ntot || k_begin >= k_end) return set_error(nullptr, SBE_ERR_RANGE, "bad k range");
+    if (model->kind == SBE_MODEL_WANNIER) {
+        if (bytes) *bytes = sbe::nb_footprint(*model, k_end - k_begin, L.nt, tab.nstage);
+        return SBE_OK;
+    }
     Plan p;
     make_plan(*cfg, *model, k_end - k_begin, L.nt, tab.nstage, p);
     if (bytes) *bytes = p.total();
@@ -302,6 +303,18 @@ int sbe_create(const sbe_config *cfg, const sbe_model *model, int64_t k_begin, i
     }
     (void)hipGetDevice(&ctx->device);
 
+    if (model->kind == SBE_MODEL_WANNIER) {
+        ctx->path = sbe::PathKind::NBand;
+        rc = hip_check(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
+        if (!rc) rc = sbe::nb_setup(ctx);
+        if (rc) {
+            set_error(nullptr, rc, ctx->err);
+            sbe_destroy(ctx);
+            return rc;
+        }
+        *out = ctx;
+        return SBE_OK;
+    }
     Plan p;
     const int64_t nk = ctx->grid.k_end - ctx->grid.k_begin;
     make_plan(ctx->cfg, ctx->model, nk, ctx->laser.nt, ctx->tab.nstage, p);
@@ -412,6 +425,7 @@ int sbe_run(sbe_ctx *ctx, int64_t step_begin, int64_t step_end)
                          "steps run in order: next step is " + std::to_string(ctx->steps_done));
     if (step_end > ctx->laser.nt || step_end < step_begin)
         return set_error(ctx, SBE_ERR_RANGE, "step_end outside [step_begin, Nt]");
+    if (ctx->path == sbe::PathKind::NBand) return sbe::nb_run(ctx, step_begin, step_end);
     const bool t1_on = ctx->hargs.gam1 != 0.0;
     for (int64_t s = step_begin; s < step_end; s += ctx->chunk_steps) {
         const int64_t n = std::min<int64_t>(ctx->chunk_steps, step_end - s);
@@ -440,6 +454,7 @@ int sbe_synchronize(sbe_ctx *ctx)
 int sbe_get_series(sbe_ctx *ctx, double *j_inter, double *j_intra, double *rho_bar, double *band_pop)
 {
     if (!ctx) return set_error(nullptr, SBE_ERR_STATE, "null context");
+    if (ctx->path == sbe::PathKind::NBand) return sbe::nb_series(ctx, j_inter, j_intra, rho_bar, band_pop);
     const int64_t nt = ctx->laser.nt;
     std::vector<double> s(static_cast<size_t>(nt) * ctx->nv);
     int rc = hip_check(ctx, hipMemcpyAsync(s.data(), ctx->d_series, s.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "copy series");
@@ -476,6 +491,7 @@ int sbe_get_state(sbe_ctx *ctx, int32_t basis, double *rho)
 {
     if (!ctx) return set_error(nullptr, SBE_ERR_STATE, "null context");
     if (!rho) return set_error(ctx, SBE_ERR_CONFIG, "null output");
+    if (ctx->path == sbe::PathKind::NBand) return sbe::nb_state(ctx, basis, rho);
     const size_t bytes = static_cast<size_t>(ctx->nk) * 8 * sizeof(double);
     double *d_out = nullptr;
     int rc = hip_check(ctx, hipMalloc(&d_out, bytes), "hipMalloc");

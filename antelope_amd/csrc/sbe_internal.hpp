@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "antelope/sbe_c.h"
+#include "nband.hpp"
 
 namespace sbe {
 
@@ -128,6 +129,12 @@ struct sbe_ctx {
     double n_sum = 0;  // sum_k w_k n_k (constant when T1 is off)
 
     sbe::HaldaneArgs hargs{};
+    // N-band path
+    sbe::NbArgs nargs{};
+    double2 *nb_P = nullptr;     // e^{ik.R} [nk_stride][nr]
+    double2 *nb_rho = nullptr;   // [nk_stride][N][N]
+    double2 *nb_mats = nullptr;  // Fourier-sum outputs [4][nk_stride][N][N]
+    double2 *nb_k = nullptr;     // RK stage derivatives [nstage][nk_stride][N][N]
     std::string err;
 
     // optional kernel timing: (start, stop) event pairs per kernel class
@@ -143,5 +150,12 @@ hipError_t launch_series_reduce(const double *partials, int64_t nsteps, int32_t 
                                 double *series, hipStream_t s);
 hipError_t launch_haldane_export(const HaldaneArgs &a, double *out, hipStream_t s);
 int haldane_waves(int64_t nk);
+// N-band host side (nband_host.cpp)
+int nb_validate(const sbe_model &m, std::string &err);
+size_t nb_footprint(const sbe_model &m, int64_t nk, int64_t nt, int nstage);
+int nb_setup(sbe_ctx *ctx);
+int nb_run(sbe_ctx *ctx, int64_t step_begin, int64_t step_end);
+int nb_series(sbe_ctx *ctx, double *j_inter, double *j_intra, double *rho_bar, double *band_pop);
+int nb_state(sbe_ctx *ctx, int basis, double *rho);
 constexpr int kBlock = 256;
 }  // namespace sbe

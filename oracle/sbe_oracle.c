@@ -473,7 +473,7 @@ static void eigh(int N, const cplx *Hin, double *ev, cplx *U, cplx *A)
             diag += creal(A[p * N + p]) * creal(A[p * N + p]);
             for (int q = p + 1; q < N; ++q) off += creal(A[p * N + q] * conj(A[p * N + q]));
         }
-        if (off <= 1e-34 * diag || off < 1e-300) break;
+        if (off <= 1e-30 * diag || off < 1e-300) break;
         for (int p = 0; p < N - 1; ++p)
             for (int q = p + 1; q < N; ++q) {
                 cplx b = A[p * N + q];
