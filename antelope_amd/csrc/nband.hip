@@ -197,14 +197,24 @@ __device__ void wcopy(cd *D, const cd *S, int nn)
 // Hermitian eigensolver: A (destroyed) -> ev ascending, U columns = vectors.
 // Parallel-order cyclic Jacobi with complex rotations G = diag(1, e^{-i phi})
 // R(theta); rotation pairs follow the round-robin tournament. work: N*N.
-__device__ void weigh(cd *A, cd *U, double *ev, cd *work, int N, double4 *rot)
+// warm: U holds eigenvectors of a nearby matrix on entry; the sweeps then
+// start from U^dag A U, which is nearly diagonal, and converge in 1-2 sweeps.
+__device__ void weigh(cd *A, cd *U, double *ev, cd *work, int N, double4 *rot, bool warm)
 {
     const int lane = threadIdx.x;
     const int nn = N * N;
     const int M = (N + 1) & ~1;  // even number of players (a dummy if N odd)
+    if (warm) {
+        wmm(work, A, U, N, false, false);
+        wmm(A, U, work, N, true, false);
+    } else {
+        for (int e = lane; e < nn; e += 64) {
+            const int i = e / N, j = e % N;
+            U[e] = cmk(i == j ? 1.0 : 0.0, 0.0);
+        }
+    }
     for (int e = lane; e < nn; e += 64) {
         const int i = e / N, j = e % N;
-        U[e] = cmk(i == j ? 1.0 : 0.0, 0.0);
         if (i == j) A[e].y = 0.0;
     }
     wave_sync();
@@ -308,7 +318,8 @@ __device__ __forceinline__ cd *mat_slot(cd *base, int i, int nn) { return base +
 
 // ----------------------------------------------------------- init state --
 // rho(k) = U diag(occupation) U^dag with U = eig(H(k)) (SBEs.h:284-364).
-__global__ __launch_bounds__(64) void nb_init_kernel(const NbArgs a, const cd *__restrict__ H0, cd *__restrict__ rho)
+__global__ __launch_bounds__(64) void nb_init_kernel(const NbArgs a, const cd *__restrict__ H0, cd *__restrict__ rho,
+                                                     cd *__restrict__ Ust)
 {
     extern __shared__ double4 smem4[];
     const int N = a.N, nn = N * N;
@@ -323,7 +334,8 @@ __global__ __launch_bounds__(64) void nb_init_kernel(const NbArgs a, const cd *_
         return;
     }
     wcopy(A, H0 + k * nn, nn);
-    weigh(A, U, ev, W, N, rot);
+    weigh(A, U, ev, W, N, rot, false);
+    for (int e = threadIdx.x; e < nn; e += 64) Ust[k * nn + e] = U[e];
     // occupations: first nval bands, or Fermi-Dirac normalised to 1
     __shared__ double occ[64];
     if (threadIdx.x == 0) {
@@ -355,7 +367,7 @@ __global__ __launch_bounds__(64) void nb_init_kernel(const NbArgs a, const cd *_
 // and writes one row of partial sums: [inter(3) intra(3) pop(N) rho(2N^2)].
 __global__ __launch_bounds__(64) void nb_sample_kernel(const NbArgs a, const cd *__restrict__ mats,
                                                        const cd *__restrict__ rho, double *__restrict__ partials,
-                                                       int64_t step_local)
+                                                       int64_t step_local, cd *__restrict__ Ust, int ureuse)
 {
     extern __shared__ double4 smem4[];
     const int N = a.N, nn = N * N;
@@ -376,9 +388,13 @@ __global__ __launch_bounds__(64) void nb_sample_kernel(const NbArgs a, const cd 
     const cd *J = mats + a.nk_stride * nn;            // mats 1..3: J_x, J_y, J_z
     for (int64_t k = blockIdx.x; k < a.nk; k += gridDim.x) {
         const double w = a.weight[k];
-        wcopy(A, H0 + k * nn, nn);
         wcopy(R, rho + k * nn, nn);
-        weigh(A, U, ev, W, N, rot);
+        wcopy(U, Ust + k * nn, nn);
+        if (!ureuse) {  // eigenvectors at k + A(t_n), warm-started from the last ones
+            wcopy(A, H0 + k * nn, nn);
+            weigh(A, U, ev, W, N, rot, true);
+            for (int e = lane; e < nn; e += 64) Ust[k * nn + e] = U[e];
+        }
         wmm(T, R, U, N, false, false);
         wmm(X, U, T, N, true, false);  // rho_H
         if (lane < N) pop += X[lane * N + lane].x * w;
@@ -426,7 +442,7 @@ __global__ __launch_bounds__(64) void nb_sample_kernel(const NbArgs a, const cd 
 // f(y) = -i(H~ y - (H~ y)^dag) - U (Gamma o (U^dag y U)) U^dag; after the
 // last stage rho += dt sum_i b_i k_i (SBEs.h:596-624, 762-861).
 __global__ __launch_bounds__(64) void nb_rhs_kernel(const NbArgs a, const NbStage sg, const cd *__restrict__ mats,
-                                                    cd *__restrict__ rho, cd *__restrict__ kbuf)
+                                                    cd *__restrict__ rho, cd *__restrict__ kbuf, cd *__restrict__ Ust)
 {
     extern __shared__ double4 smem4[];
     const int N = a.N, nn = N * N;
@@ -459,9 +475,13 @@ __global__ __launch_bounds__(64) void nb_rhs_kernel(const NbArgs a, const NbStag
     }
     wave_sync();
     if (a.deph) {
-        const cd *H0 = mats + a.nk_stride * nn + k * nn;  // mat 1: H
-        wcopy(A, H0, nn);
-        weigh(A, U, ev, W, N, rot);
+        wcopy(U, Ust + k * nn, nn);
+        if (!sg.ureuse) {  // eigenvectors at this stage time, warm-started
+            const cd *H0 = mats + a.nk_stride * nn + k * nn;  // mat 1: H
+            wcopy(A, H0, nn);
+            weigh(A, U, ev, W, N, rot, true);
+            for (int e = lane; e < nn; e += 64) Ust[k * nn + e] = U[e];
+        }
         wmm(T, Y, U, N, false, false);
         wmm(W, U, T, N, true, false);             // U^dag y U
         for (int e = lane; e < nn; e += 64) {
@@ -508,24 +528,24 @@ hipError_t nb_launch_fourier(const NbArgs &a, const NbEval &ev, const double2 *P
     return hipGetLastError();
 }
 
-hipError_t nb_launch_init(const NbArgs &a, const double2 *H0, double2 *rho, hipStream_t s)
+hipError_t nb_launch_init(const NbArgs &a, const double2 *H0, double2 *rho, double2 *U, hipStream_t s)
 {
-    nb_init_kernel<<<dim3(static_cast<unsigned>(a.nk_stride)), 64, nb_lds_bytes(a.N), s>>>(a, H0, rho);
+    nb_init_kernel<<<dim3(static_cast<unsigned>(a.nk_stride)), 64, nb_lds_bytes(a.N), s>>>(a, H0, rho, U);
     return hipGetLastError();
 }
 
 hipError_t nb_launch_sample(const NbArgs &a, const double2 *mats, const double2 *rho, double *partials,
-                            int64_t step_local, hipStream_t s)
+                            int64_t step_local, double2 *U, int ureuse, hipStream_t s)
 {
     nb_sample_kernel<<<dim3(static_cast<unsigned>(a.sample_waves)), 64, nb_lds_bytes(a.N), s>>>(a, mats, rho, partials,
-                                                                                               step_local);
+                                                                                               step_local, U, ureuse);
     return hipGetLastError();
 }
 
 hipError_t nb_launch_rhs(const NbArgs &a, const NbStage &sg, const double2 *mats, double2 *rho, double2 *kbuf,
-                         hipStream_t s)
+                         double2 *U, hipStream_t s)
 {
-    nb_rhs_kernel<<<dim3(static_cast<unsigned>(a.nk)), 64, nb_lds_bytes(a.N), s>>>(a, sg, mats, rho, kbuf);
+    nb_rhs_kernel<<<dim3(static_cast<unsigned>(a.nk)), 64, nb_lds_bytes(a.N), s>>>(a, sg, mats, rho, kbuf, U);
     return hipGetLastError();
 }
 

@@ -42,6 +42,7 @@ struct NbEval {
 // One RK stage: coefficients already multiplied by dt.
 struct NbStage {
     int stage, nstage, last;
+    int ureuse;   // the stored eigenvectors belong to this stage's H (same time)
     double a[8];  // dt a_{stage,j}
     double b[8];  // dt b_j (used when last)
 };
@@ -49,10 +50,10 @@ struct NbStage {
 size_t nb_lds_bytes(int N);
 hipError_t nb_launch_phase(const NbArgs &a, double2 *P, hipStream_t s);
 hipError_t nb_launch_fourier(const NbArgs &a, const NbEval &ev, const double2 *P, double2 *out, hipStream_t s);
-hipError_t nb_launch_init(const NbArgs &a, const double2 *H0, double2 *rho, hipStream_t s);
+hipError_t nb_launch_init(const NbArgs &a, const double2 *H0, double2 *rho, double2 *U, hipStream_t s);
 hipError_t nb_launch_sample(const NbArgs &a, const double2 *mats, const double2 *rho, double *partials,
-                            int64_t step_local, hipStream_t s);
+                            int64_t step_local, double2 *U, int ureuse, hipStream_t s);
 hipError_t nb_launch_rhs(const NbArgs &a, const NbStage &sg, const double2 *mats, double2 *rho, double2 *kbuf,
-                         hipStream_t s);
+                         double2 *U, hipStream_t s);
 
 }  // namespace sbe

@@ -59,7 +59,7 @@ NbPlan nb_plan(int N, int nr, int64_t nk, int64_t nt, int nstage)
     p.bytes_P = static_cast<size_t>(p.nk_stride) * nr * sizeof(double2);
     p.bytes_rho = static_cast<size_t>(p.nk_stride) * nn * sizeof(double2);
     p.bytes_mats = 4 * p.bytes_rho;
-    p.bytes_k = static_cast<size_t>(nstage) * p.bytes_rho;
+    p.bytes_k = static_cast<size_t>(nstage + 1) * p.bytes_rho;  // stage derivatives + eigenvectors
     p.bytes_tab = static_cast<size_t>(p.nk_stride) * 4 * sizeof(double) + static_cast<size_t>(nr) * 4 * sizeof(double) +
                   static_cast<size_t>(nr) * 4 * nn * sizeof(double2);
     p.bytes_partials = static_cast<size_t>(p.chunk) * row;
@@ -116,6 +116,7 @@ int nb_setup(sbe_ctx *ctx)
     ctx->nb_rho = nalloc<double2>(ctx, p.bytes_rho, rc);
     ctx->nb_mats = nalloc<double2>(ctx, p.bytes_mats, rc);
     ctx->nb_k = nalloc<double2>(ctx, p.bytes_k, rc);
+    if (!rc) ctx->nb_U = ctx->nb_k + static_cast<size_t>(ctx->tab.nstage) * p.nk_stride * nn;
     char *tab = nalloc<char>(ctx, p.bytes_tab, rc);
     ctx->d_partials = nalloc<double>(ctx, p.bytes_partials, rc);
     ctx->d_series = nalloc<double>(ctx, p.bytes_series, rc);
@@ -172,7 +173,9 @@ int nb_setup(sbe_ctx *ctx)
     ev.nmat = 1;
     ev.mats[0] = kMatH0;
     if (!rc) rc = hipc(ctx, nb_launch_fourier(a, ev, ctx->nb_P, ctx->nb_mats, ctx->stream), "fourier kernel");
-    if (!rc) rc = hipc(ctx, nb_launch_init(a, ctx->nb_mats, ctx->nb_rho, ctx->stream), "init kernel");
+    if (!rc) rc = hipc(ctx, nb_launch_init(a, ctx->nb_mats, ctx->nb_rho, ctx->nb_U, ctx->stream), "init kernel");
+    ctx->nb_u_time = ctx->laser.atmin;  // U diagonalises H(k) = H(k + A(atmin))
+    ctx->nb_ht_valid = false;
     if (!rc) rc = hipc(ctx, hipStreamSynchronize(ctx->stream), "init sync");
     return rc;
 }
@@ -196,24 +199,44 @@ int nb_run(sbe_ctx *ctx, int64_t step_begin, int64_t step_end)
             ev.mats[1] = kMatJx;
             ev.mats[2] = kMatJy;
             ev.mats[3] = kMatJz;
+            // eigenvectors: reused when they already belong to H(k + A(t_n)),
+            // otherwise warm-started from the stored ones
+            const int ureuse = ctx->nb_u_time == tn;
             rc = hipc(ctx, nb_launch_fourier(a, ev, ctx->nb_P, ctx->nb_mats, ctx->stream), "fourier kernel");
             if (!rc)
-                rc = hipc(ctx, nb_launch_sample(a, ctx->nb_mats, ctx->nb_rho, ctx->d_partials, s, ctx->stream),
+                rc = hipc(ctx, nb_launch_sample(a, ctx->nb_mats, ctx->nb_rho, ctx->d_partials, s, ctx->nb_U, ureuse, ctx->stream),
                           "sample kernel");
+            ctx->nb_u_time = tn;
+            ctx->nb_ht_valid = false;  // mats slot 0 now holds H, not H~
             for (int i = 0; i < tb.nstage && !rc; ++i) {
                 NbEval se{};
-                ctx->laser.eval(tn + tb.c[i] * dt, se.A, se.E);  // SBEs.h:612
-                se.nmat = a.deph ? 2 : 1;
-                se.mats[0] = kMatHtilde;
-                se.mats[1] = kMatH0;
+                const double ts = tn + tb.c[i] * dt;
+                ctx->laser.eval(ts, se.A, se.E);  // SBEs.h:612
                 NbStage sg{};
                 sg.stage = i;
                 sg.nstage = tb.nstage;
                 sg.last = i == tb.nstage - 1;
+                sg.ureuse = ctx->nb_u_time == ts;
                 for (int j = 0; j < i; ++j) sg.a[j] = dt * tb.a[i][j];
                 for (int j = 0; j < tb.nstage; ++j) sg.b[j] = dt * tb.b[j];
-                rc = hipc(ctx, nb_launch_fourier(a, se, ctx->nb_P, ctx->nb_mats, ctx->stream), "fourier kernel");
-                if (!rc) rc = hipc(ctx, nb_launch_rhs(a, sg, ctx->nb_mats, ctx->nb_rho, ctx->nb_k, ctx->stream), "rhs kernel");
+                // H~ (and H when new eigenvectors are needed) unless slot 0 already
+                // holds H~ for the same (A, E) (RK4 stages 1 and 2 share a time)
+                const bool same_ht = ctx->nb_ht_valid && std::memcmp(ctx->nb_ht_A, se.A, sizeof se.A) == 0 &&
+                                     std::memcmp(ctx->nb_ht_E, se.E, sizeof se.E) == 0;
+                const bool need_h = a.deph && !sg.ureuse;
+                if (!same_ht || need_h) {
+                    se.nmat = need_h ? 2 : 1;
+                    se.mats[0] = kMatHtilde;
+                    se.mats[1] = kMatH0;
+                    rc = hipc(ctx, nb_launch_fourier(a, se, ctx->nb_P, ctx->nb_mats, ctx->stream), "fourier kernel");
+                    std::memcpy(ctx->nb_ht_A, se.A, sizeof se.A);
+                    std::memcpy(ctx->nb_ht_E, se.E, sizeof se.E);
+                    ctx->nb_ht_valid = true;
+                }
+                if (!rc)
+                    rc = hipc(ctx, nb_launch_rhs(a, sg, ctx->nb_mats, ctx->nb_rho, ctx->nb_k, ctx->nb_U, ctx->stream),
+                              "rhs kernel");
+                if (a.deph) ctx->nb_u_time = ts;
             }
         }
         if (!rc)

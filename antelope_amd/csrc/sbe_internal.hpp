@@ -135,6 +135,10 @@ struct sbe_ctx {
     double2 *nb_rho = nullptr;   // [nk_stride][N][N]
     double2 *nb_mats = nullptr;  // Fourier-sum outputs [4][nk_stride][N][N]
     double2 *nb_k = nullptr;     // RK stage derivatives [nstage][nk_stride][N][N]
+    double2 *nb_U = nullptr;     // last eigenvectors per k [nk_stride][N][N]
+    double nb_u_time = 0;        // time whose H the stored eigenvectors diagonalise
+    double nb_ht_A[3] = {0, 0, 0}, nb_ht_E[3] = {0, 0, 0};  // (A, E) of the H~ in mats slot 0
+    bool nb_ht_valid = false;
     std::string err;
 
     // optional kernel timing: (start, stop) event pairs per kernel class

@@ -46,6 +46,18 @@ KERNEL_NAME = {"fp64": "haldane_steps_kernel<double, 4, false, true, 1>",
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
 
 
+def wannier_workload(precision: str = "fp64"):
+    """Synthetic Wannier N = 16 model (147 R), 48 x 48 x 8 grid, linear
+    sin^2 pulse E0 = 0.003, w0 = 0.014, 8 cycles, dt = 0.1 (Nt = 35903),
+    RK4, T2 = 220, inter/intra split on."""
+    from antelope_amd.models import synthetic_wannier
+    model = synthetic_wannier(16, nmax=(3, 3, 1), seed=2024)
+    pulse = Pulse(E0=0.003, w0=0.014, ncycles=8.0, envelope="sin2")
+    cfg = RunConfig(npoints=[48, 48, 8], pulses=[pulse], dt=0.1, T2=220.0, rk_order=4,
+                    inter_intra=True, precision=precision)
+    return cfg, model
+
+
 def workload(grid: int, precision: str) -> tuple[RunConfig, HaldaneModel]:
     t1 = 0.075
     model = HaldaneModel(a0=2.683, t1=t1, t2=t1 / 3, phi0=0.3 * math.pi, M0=2.54 * t1 / 3)
@@ -116,6 +128,29 @@ def cpu_baseline(cfg, model, budget_s: float):
     }
 
 
+def bench_wannier(args) -> int:
+    """N-band path on the Wannier workload: times `steps` steps (default 50)."""
+    cfg, model = wannier_workload("fp64")
+    steps = args.steps if args.steps > 0 else 50
+    r = timed_solve(cfg, model, steps, min(args.warmup, 5))
+    ks = r["nk"] * r["steps"] / r["wall_s"]
+    # algorithmic flops per k-point RK4 step (SURVEY 8(d)): 4 stages x (H + 3 D
+    # Fourier sums, 8 N^2 N_R each, + H rho 8 N^3) + the sample's 3 J + H sums
+    N, nr = 16, int(model.rvec.shape[0])
+    fsum = 8 * N * N * nr
+    flops = 4 * (4 * fsum + 8 * N ** 3) + 4 * fsum + 8 * 8 * N ** 3
+    line = {"metric": METRIC, "value": ks, "unit": "k-point·RK-steps/s", "n_gpus": 1,
+            "steps": r["steps"], "warmup": min(args.warmup, 5), "ms_per_step": r["wall_s"] * 1e3 / r["steps"],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic (seeded SplitMix64 Wannier N=16 model, 147 R)",
+            "config": {"workload": f"Wannier N=16, 147 R, 48x48x8 k grid, sin^2 8-cycle pulse, RK4 dt=0.1, "
+                                   f"T2=220, Nt={r['info'].nt} (first {r['steps']} steps timed)", "nk": r["nk"]},
+            "algorithmic_flops_per_kstep": flops, "achieved_tflops": ks * flops / 1e12,
+            "timing": r["timing"]}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def rel_l2(x, ref):
     d = np.linalg.norm(ref)
     return float(np.linalg.norm(x - ref) / d) if d > 0 else float(np.linalg.norm(x))
@@ -127,6 +162,7 @@ def main(argv=None) -> int:
     ap.add_argument("--steps", type=int, default=0, help="timed steps (default: the whole pulse)")
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--grid", type=int, default=300)
+    ap.add_argument("--workload", choices=["haldane", "wannier16"], default="haldane")
     ap.add_argument("--precision", choices=["both", "fp64", "fp32"], default="both")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -149,6 +185,8 @@ def main(argv=None) -> int:
               "(k blocks would shard across GPUs with one final sum; see DESIGN.md).", file=sys.stderr)
         return 2
 
+    if args.workload == "wannier16":
+        return bench_wannier(args)
     forms = ["fp64", "fp32"] if args.precision == "both" else [args.precision]
     res = {}
     for p in forms:
