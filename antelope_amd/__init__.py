@@ -5,4 +5,4 @@ include/antelope/sbe_c.h); this package holds its Python binding, the
 run/model parameter mirror and the build script.
 """
 from .params import AU_ANGSTROM, AU_EV, HaldaneModel, Pulse, RunConfig, WannierModel  # noqa: F401
-from .solver import SBEError, SBESolver, laser_eval, library_path, load_library, query_footprint  # noqa: F401
+from .solver import SBEBatch, SBEError, SBESolver, laser_eval, library_path, load_library, query_footprint  # noqa: F401

@@ -106,6 +106,14 @@ SIGNATURES = {
                         c_double_p, C.POINTER(C.c_int64)], C.c_int),
     "sbe_set_timing": ([C.c_void_p, C.c_int32], C.c_int),
     "sbe_get_timing": ([C.c_void_p, c_double_p, C.POINTER(C.c_int64)], C.c_int),
+    "sbe_batch_create": ([C.c_int32, C.POINTER(sbe_config), C.POINTER(sbe_model), C.POINTER(C.c_void_p)], C.c_int),
+    "sbe_batch_run": ([C.c_void_p, C.c_int64], C.c_int),
+    "sbe_batch_synchronize": ([C.c_void_p], C.c_int),
+    "sbe_batch_get_info": ([C.c_void_p, C.c_int32, C.POINTER(sbe_info)], C.c_int),
+    "sbe_batch_get_series": ([C.c_void_p, C.c_int32, c_double_p, c_double_p, c_double_p, c_double_p], C.c_int),
+    "sbe_batch_get_timing": ([C.c_void_p, c_double_p, C.POINTER(C.c_int64)], C.c_int),
+    "sbe_batch_last_error": ([C.c_void_p], C.c_char_p),
+    "sbe_batch_destroy": ([C.c_void_p], None),
     "sbe_destroy": ([C.c_void_p], None),
     "sbe_last_error": ([C.c_void_p], C.c_char_p),
 }

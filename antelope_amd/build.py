@@ -25,7 +25,11 @@ LIBNAME = "libantelope_sbe.so"
 ARCH = os.environ.get("ANTELOPE_OFFLOAD_ARCH", "gfx950")
 
 # Library sources (kernels + C ABI). The driver links against the library.
-LIB_SOURCES = ["host_setup.cpp", "sbe_api.cpp", "nband_host.cpp", "haldane.hip", "nband.hip", "series.hip"]
+LIB_SOURCES = [
+    "host_setup.cpp", "sbe_api.cpp", "nband_host.cpp", "frontend.cpp",
+    "haldane.hip", "nband.hip", "series.hip",
+]
+DRIVER = "antelope_mi355x"
 
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}",
@@ -77,6 +81,14 @@ def build_library(jobs: int = 8) -> Path:
     return out
 
 
+def build_driver() -> Path:
+    """The reference-compatible executable (reads inputParam.cfg, writes the .dat files)."""
+    out = LIBDIR / DRIVER
+    _run([hipcc(), "-O2", "-std=c++17", f"-I{ROOT / 'include'}", str(CSRC / "driver_main.cpp"),
+          f"-L{LIBDIR}", "-lantelope_sbe", "-Wl,-rpath,$ORIGIN", "-o", str(out)])
+    return out
+
+
 def build_oracle() -> Path:
     """The CPU checker (tests only); compiled here so tests can load it."""
     _run(["make", "-s", "-C", str(ROOT / "oracle")])
@@ -90,6 +102,7 @@ def main(argv=None) -> int:
     args = ap.parse_args(argv)
     print(build_library(args.j))
     if not args.lib_only:
+        print(build_driver())
         print(build_oracle())
     return 0
 

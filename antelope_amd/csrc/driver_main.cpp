@@ -1,0 +1,67 @@
+// antelope_mi355x - the reference's executable (antelope.cpp:50-593) on one
+// MI355X: reads inputParam.cfg from the working directory (or the path
+// given), propagates the whole k grid on the GPU and writes the reference's
+// output files into the working directory (or --outdir).
+//
+//   antelope_mi355x [config] [--outdir DIR] [--quiet]
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "antelope/sbe_c.h"
+#include "antelope/sbe_frontend.h"
+
+int main(int argc, char **argv)
+{
+    std::string path = "inputParam.cfg", outdir = ".";  // antelope.cpp:93
+    bool quiet = false;
+    for (int i = 1; i < argc; ++i) {
+        if (!std::strcmp(argv[i], "--outdir") && i + 1 < argc) outdir = argv[++i];
+        else if (!std::strcmp(argv[i], "--quiet")) quiet = true;
+        else path = argv[i];
+    }
+    sbe_spec *spec = nullptr;
+    if (sbe_spec_parse_file(path.c_str(), &spec) != SBE_OK) {
+        std::fprintf(stderr, "%s\n", sbe_frontend_last_error());
+        return 1;
+    }
+    if (*sbe_spec_warnings(spec)) std::fprintf(stderr, "Warning: %s", sbe_spec_warnings(spec));
+    const sbe_config *cfg = nullptr;
+    const sbe_model *model = nullptr;
+    sbe_spec_get(spec, &cfg, &model);
+    sbe_ctx *ctx = nullptr;
+    if (sbe_create(cfg, model, -1, -1, &ctx) != SBE_OK) {
+        std::fprintf(stderr, "%s\n", sbe_last_error(nullptr));
+        sbe_spec_free(spec);
+        return 1;
+    }
+    sbe_info info;
+    sbe_get_info(ctx, &info);
+    if (!quiet)
+        std::printf("Nk = %lld, Nt = %lld, dt = %g, atmin = %g, dV = %.6e\n", (long long)info.nk_total,
+                    (long long)info.nt, info.dt, info.atmin, info.dV);
+    const int64_t chunk = info.nt / 10 > 0 ? info.nt / 10 : 1;
+    for (int64_t s = 0; s < info.nt; s += chunk) {
+        const int64_t e = s + chunk < info.nt ? s + chunk : info.nt;
+        if (sbe_run(ctx, s, e) != SBE_OK || sbe_synchronize(ctx) != SBE_OK) {
+            std::fprintf(stderr, "%s\n", sbe_last_error(ctx));
+            sbe_destroy(ctx);
+            sbe_spec_free(spec);
+            return 1;
+        }
+        if (!quiet) std::printf("%lld / %lld\n", (long long)e, (long long)info.nt);
+    }
+    const int N = info.nband;
+    std::vector<double> ji(3 * info.nt), ja(3 * info.nt), rb(2 * N * N * info.nt), bp(N * info.nt);
+    int rc = sbe_get_series(ctx, ji.data(), ja.data(), rb.data(), bp.data());
+    if (rc != SBE_OK) std::fprintf(stderr, "%s\n", sbe_last_error(ctx));
+    sbe_destroy(ctx);
+    if (rc == SBE_OK) {
+        std::printf("\n\nWriting output-data\n\n");
+        rc = sbe_write_outputs(outdir.c_str(), cfg, model, &info, ji.data(), ja.data(), rb.data(), bp.data());
+        if (rc != SBE_OK) std::fprintf(stderr, "%s\n", sbe_frontend_last_error());
+    }
+    sbe_spec_free(spec);
+    return rc == SBE_OK ? 0 : 1;
+}

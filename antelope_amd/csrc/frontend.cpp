@@ -1,0 +1,506 @@
+// frontend.cpp - the caller side of the C ABI that the reference's driver
+// provides: reading inputParam.cfg into sbe_config / sbe_model and writing
+// the reference's output files. Host-only code (no HIP calls).
+//
+// Config reading restates antelope.cpp:90-138 (file, gauge, Npoints),
+// SBEs.h:125-215 (target, initial value type), SBEs.h:432-561 (calc and
+// laser groups, defaults, degree conversion) and Materials/Haldane.h:46-84
+// (a0 in Angstrom, M0 or Mt2). The parser accepts the libconfig grammar the
+// reference's files use: settings `name = value;` or `name : value;`,
+// groups {...}, arrays [...], lists (...), numbers (int, hex, float),
+// booleans, strings (adjacent literals concatenate) and #, //, /* */
+// comments. Writers restate antelope.cpp:208-235 (outlaserdata.dat) and
+// :514-548 (current, occupation and coherence files).
+#include <cctype>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <memory>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "antelope/sbe_c.h"
+#include "antelope/sbe_frontend.h"
+#include "sbe_internal.hpp"
+
+namespace {
+
+thread_local std::string g_fe_error;
+
+int fe_fail(int code, const std::string &msg)
+{
+    g_fe_error = msg;
+    return code;
+}
+
+// ------------------------------------------------------------ settings --
+struct Setting {
+    enum Kind { Int, Float, Bool, String, Array, List, Group } kind = Group;
+    std::string name;
+    long long i = 0;
+    double f = 0.0;
+    bool b = false;
+    std::string s;
+    std::vector<Setting> items;  // Array, List, Group
+
+    const Setting *child(const std::string &n) const
+    {
+        if (kind != Group) return nullptr;
+        for (const Setting &c : items)
+            if (c.name == n) return &c;
+        return nullptr;
+    }
+    bool number(double &out) const
+    {
+        if (kind == Float) { out = f; return true; }
+        if (kind == Int) { out = static_cast<double>(i); return true; }
+        return false;
+    }
+};
+
+class Parser {
+  public:
+    explicit Parser(const std::string &text) : t_(text) {}
+    bool parse(Setting &root, std::string &err)
+    {
+        root.kind = Setting::Group;
+        if (!settings(root.items, '\0')) {
+            err = "parse error at line " + std::to_string(line_) + ": " + err_;
+            return false;
+        }
+        return true;
+    }
+
+  private:
+    const std::string &t_;
+    size_t p_ = 0;
+    int line_ = 1;
+    std::string err_;
+
+    bool fail(const std::string &m)
+    {
+        if (err_.empty()) err_ = m;
+        return false;
+    }
+    void ws()
+    {
+        for (;;) {
+            while (p_ < t_.size() && std::isspace(static_cast<unsigned char>(t_[p_]))) {
+                if (t_[p_] == '\n') ++line_;
+                ++p_;
+            }
+            if (p_ < t_.size() && t_[p_] == '#') {
+                while (p_ < t_.size() && t_[p_] != '\n') ++p_;
+            } else if (t_.compare(p_, 2, "//") == 0) {
+                while (p_ < t_.size() && t_[p_] != '\n') ++p_;
+            } else if (t_.compare(p_, 2, "/*") == 0) {
+                const size_t e = t_.find("*/", p_ + 2);
+                for (size_t q = p_; q < (e == std::string::npos ? t_.size() : e); ++q)
+                    if (t_[q] == '\n') ++line_;
+                p_ = e == std::string::npos ? t_.size() : e + 2;
+            } else {
+                return;
+            }
+        }
+    }
+    bool settings(std::vector<Setting> &out, char close)
+    {
+        for (;;) {
+            ws();
+            if (p_ >= t_.size()) return close == '\0' ? true : fail("unexpected end of file");
+            if (t_[p_] == close) return true;
+            if (t_[p_] == '@') return fail("@include is not supported");
+            Setting s;
+            if (!name(s.name)) return false;
+            ws();
+            if (p_ >= t_.size() || (t_[p_] != '=' && t_[p_] != ':')) return fail("expected '=' or ':' after " + s.name);
+            ++p_;
+            if (!value(s)) return false;
+            ws();
+            if (p_ < t_.size() && (t_[p_] == ';' || t_[p_] == ',')) ++p_;
+            out.push_back(std::move(s));
+        }
+    }
+    bool name(std::string &n)
+    {
+        const size_t b = p_;
+        if (p_ >= t_.size() || !(std::isalpha(static_cast<unsigned char>(t_[p_])) || t_[p_] == '*'))
+            return fail("expected a setting name");
+        while (p_ < t_.size() && (std::isalnum(static_cast<unsigned char>(t_[p_])) || t_[p_] == '_' ||
+                                  t_[p_] == '-' || t_[p_] == '*'))
+            ++p_;
+        n = t_.substr(b, p_ - b);
+        return true;
+    }
+    bool value(Setting &s)
+    {
+        ws();
+        if (p_ >= t_.size()) return fail("missing value");
+        const char c = t_[p_];
+        if (c == '{') {
+            ++p_;
+            s.kind = Setting::Group;
+            if (!settings(s.items, '}')) return false;
+            ++p_;
+            return true;
+        }
+        if (c == '[' || c == '(') {
+            const char close = c == '[' ? ']' : ')';
+            s.kind = c == '[' ? Setting::Array : Setting::List;
+            ++p_;
+            for (;;) {
+                ws();
+                if (p_ >= t_.size()) return fail("unterminated array/list");
+                if (t_[p_] == close) {
+                    ++p_;
+                    return true;
+                }
+                Setting v;
+                if (!value(v)) return false;
+                s.items.push_back(std::move(v));
+                ws();
+                if (p_ < t_.size() && t_[p_] == ',') ++p_;
+            }
+        }
+        if (c == '"') {
+            s.kind = Setting::String;
+            while (p_ < t_.size() && t_[p_] == '"') {
+                ++p_;
+                while (p_ < t_.size() && t_[p_] != '"') {
+                    if (t_[p_] == '\\' && p_ + 1 < t_.size()) {
+                        const char e = t_[++p_];
+                        s.s += e == 'n' ? '\n' : e == 't' ? '\t' : e;
+                    } else {
+                        s.s += t_[p_];
+                    }
+                    ++p_;
+                }
+                if (p_ >= t_.size()) return fail("unterminated string");
+                ++p_;
+                ws();  // adjacent literals concatenate
+            }
+            return true;
+        }
+        // boolean or number
+        size_t b = p_;
+        while (p_ < t_.size() && (std::isalnum(static_cast<unsigned char>(t_[p_])) || t_[p_] == '.' || t_[p_] == '+' ||
+                                  t_[p_] == '-'))
+            ++p_;
+        std::string tok = t_.substr(b, p_ - b);
+        std::string low;
+        for (char ch : tok) low += static_cast<char>(std::tolower(static_cast<unsigned char>(ch)));
+        if (low == "true" || low == "false") {
+            s.kind = Setting::Bool;
+            s.b = low == "true";
+            return true;
+        }
+        if (tok.empty()) return fail("bad value");
+        std::string num = tok;
+        while (!num.empty() && (num.back() == 'L' || num.back() == 'l')) num.pop_back();
+        const bool hex = num.size() > 2 && num[0] == '0' && (num[1] == 'x' || num[1] == 'X');
+        const bool is_float = !hex && num.find_first_of(".eE") != std::string::npos;
+        char *end = nullptr;
+        if (is_float) {
+            s.kind = Setting::Float;
+            s.f = std::strtod(num.c_str(), &end);
+        } else {
+            s.kind = Setting::Int;
+            s.i = std::strtoll(num.c_str(), &end, hex ? 16 : 10);
+        }
+        if (!end || *end != '\0') return fail("bad number '" + tok + "'");
+        return true;
+    }
+};
+
+// -------------------------------------------------------------- lookups --
+bool get_double(const Setting *g, const char *n, double &v)
+{
+    const Setting *s = g ? g->child(n) : nullptr;
+    return s && s->number(v);
+}
+bool get_int(const Setting *g, const char *n, long long &v)
+{
+    const Setting *s = g ? g->child(n) : nullptr;
+    if (!s || s->kind != Setting::Int) return false;
+    v = s->i;
+    return true;
+}
+bool get_bool(const Setting *g, const char *n, bool &v)
+{
+    const Setting *s = g ? g->child(n) : nullptr;
+    if (!s || s->kind != Setting::Bool) return false;
+    v = s->b;
+    return true;
+}
+bool get_string(const Setting *g, const char *n, std::string &v)
+{
+    const Setting *s = g ? g->child(n) : nullptr;
+    if (!s || s->kind != Setting::String) return false;
+    v = s->s;
+    return true;
+}
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kAuAngstrom = 0.529177210903;  // constant.h:34
+
+}  // namespace
+
+struct sbe_spec {
+    sbe_config cfg{};
+    sbe_model model{};
+    std::vector<sbe_pulse> pulses;
+    std::string target;
+    std::string warnings;
+};
+
+extern "C" {
+
+const char *sbe_frontend_last_error(void) { return g_fe_error.c_str(); }
+
+int sbe_spec_parse_string(const char *text, sbe_spec **out)
+{
+    if (!text || !out) return fe_fail(SBE_ERR_CONFIG, "null argument");
+    *out = nullptr;
+    const std::string src(text);
+    Setting root;
+    std::string err;
+    if (!Parser(src).parse(root, err)) return fe_fail(SBE_ERR_CONFIG, err);
+    auto spec = std::make_unique<sbe_spec>();
+    sbe_config &c = spec->cfg;
+    c.abi_version = SBE_ABI_VERSION;
+
+    // gauge (antelope.cpp:111-129); default LengthWannier with a warning
+    std::string gauge;
+    get_string(&root, "gauge", gauge);
+    if (gauge == "Wannier" || gauge == "LengthWannier") c.gauge = SBE_GAUGE_LENGTH_WANNIER;
+    else if (gauge == "Hamiltonian" || gauge == "LengthHamiltonian") c.gauge = SBE_GAUGE_LENGTH_HAMILTONIAN;
+    else if (gauge == "Velocity" || gauge == "VelocityHamiltonian") c.gauge = SBE_GAUGE_VELOCITY_HAMILTONIAN;
+    else {
+        c.gauge = SBE_GAUGE_LENGTH_WANNIER;
+        spec->warnings += "No gauge type is specified. Choose LengthWannier for default gauge.\n";
+    }
+
+    // calc group (SBEs.h:432-509)
+    const Setting *calc = root.child("calc");
+    if (!calc || calc->kind != Setting::Group) return fe_fail(SBE_ERR_CONFIG, "Setting not found while search calc group");
+    const Setting *np = calc->child("Npoints");
+    if (!np || (np->kind != Setting::Array && np->kind != Setting::List) || np->items.empty() || np->items.size() > 3)
+        return fe_fail(SBE_ERR_CONFIG, "calc.Npoints must be an array of 1 to 3 integers");
+    for (int i = 0; i < 3; ++i) c.npoints[i] = 1;
+    for (size_t i = 0; i < np->items.size(); ++i) {
+        if (np->items[i].kind != Setting::Int) return fe_fail(SBE_ERR_CONFIG, "Type exception in calc group (Npoints)");
+        c.npoints[i] = static_cast<int32_t>(np->items[i].i);
+    }
+    c.dt = 0.1;
+    get_double(calc, "dt", c.dt);
+    c.T1 = -1.0;
+    c.T2 = -1.0;
+    get_double(calc, "T1", c.T1);
+    get_double(calc, "T2", c.T2);
+    long long rk = 5;  // SBEs.h:438
+    get_int(calc, "RKorder", rk);
+    c.rk_order = static_cast<int32_t>(rk);
+    std::string method;
+    if (get_string(calc, "method", method) && (method == "cash_karp" || method == "CashKarp")) c.rk_order = SBE_RK_CASH_KARP;
+    bool ii = false;  // SBEs.h:442
+    get_bool(calc, "InterIntra", ii);
+    c.inter_intra = ii;
+    for (int i = 0; i < 3; ++i) c.ksfactor[i] = 1.0;
+    if (const Setting *ks = calc->child("ksfactor")) {
+        for (size_t i = 0; i < ks->items.size() && i < 3; ++i)
+            if (!ks->items[i].number(c.ksfactor[i])) return fe_fail(SBE_ERR_CONFIG, "Type exception in calc group (ksfactor)");
+    } else {
+        spec->warnings += "calc.ksfactor missing (the reference requires it): using [1, 1, 1]\n";
+    }
+    if (const Setting *ax = calc->child("BZaxes")) {
+        c.has_bz_axes = 1;
+        for (size_t i = 0; i < ax->items.size() && i < 9; ++i) ax->items[i].number(c.bz_axes[i]);
+    }
+    if (const Setting *og = calc->child("BZorigin")) {
+        c.has_bz_origin = 1;
+        for (size_t i = 0; i < og->items.size() && i < 3; ++i) og->items[i].number(c.bz_origin[i]);
+    }
+    std::string s;
+    if (get_string(calc, "grid", s)) c.grid = (s == "mp" || s == "MP") ? SBE_GRID_MP : SBE_GRID_REFERENCE;
+    if (get_string(calc, "precision", s)) c.precision = s == "fp32" ? SBE_PREC_FP32 : SBE_PREC_FP64;
+    c.rtol = 1e-8;
+    c.atol = 1e-12;
+    get_double(calc, "rtol", c.rtol);
+    get_double(calc, "atol", c.atol);
+
+    // laser group (SBEs.h:511-561)
+    const Setting *laser = root.child("laser");
+    const Setting *pulses = laser ? laser->child("pulses") : nullptr;
+    if (!pulses || pulses->items.empty()) return fe_fail(SBE_ERR_CONFIG, "No laser, check config file");
+    for (const Setting &p : pulses->items) {
+        sbe_pulse q{};
+        std::string env;
+        double cep, phix;
+        if (!(get_double(&p, "E0", q.E0) && get_double(&p, "ellip", q.ellip) && get_double(&p, "w0", q.w0) &&
+              get_double(&p, "ncycles", q.ncycles) && get_double(&p, "cep", cep) && get_double(&p, "t0", q.t0) &&
+              get_double(&p, "phix", phix) && get_string(&p, "env_name", env)))
+            return fe_fail(SBE_ERR_CONFIG, "Error while initilizng laser pulses. Check config file");
+        double thetaz = 0.0, phiz = 0.0, ncc = 0.0;
+        get_double(&p, "thetaz", thetaz);
+        get_double(&p, "phiz", phiz);
+        get_double(&p, "ncycles_clipped", ncc);
+        const double d = kPi / 180.0;
+        q.cep = cep * d;
+        q.phix = phix * d;
+        q.thetaz = thetaz * d;
+        q.phiz = phiz * d;
+        q.ncycles_clipped = ncc;
+        if (env == "cos4") q.envelope = SBE_ENV_COS4;
+        else if (env == "gauss") q.envelope = SBE_ENV_GAUSS;
+        else if (env == "gauss_clipped") q.envelope = SBE_ENV_GAUSS_CLIPPED;
+        else if (env == "cos4_clipped") q.envelope = SBE_ENV_COS4_CLIPPED;
+        else if (env == "sin2") q.envelope = SBE_ENV_SIN2;
+        else return fe_fail(SBE_ERR_CONFIG, "invalid envelope type " + env);
+        spec->pulses.push_back(q);
+    }
+    if (const Setting *off = laser->child("offset")) {
+        if (off->items.size() >= 2) {
+            off->items[0].number(c.offset_before);
+            off->items[1].number(c.offset_after);
+        }
+    }
+    if (get_string(laser, "efield", s)) c.efield_mode = s == "consistent" ? SBE_EFIELD_CONSISTENT : SBE_EFIELD_REFERENCE;
+
+    // target and model (SBEs.h:130-215, Haldane.h:46-84)
+    if (!get_string(&root, "target", spec->target)) return fe_fail(SBE_ERR_CONFIG, "No 'target' in input file");
+    const Setting *mg = root.child(spec->target);
+    bool custom = false;
+    if (mg) get_bool(mg, "UsingCustomInitial", custom);
+    if (custom) return fe_fail(SBE_ERR_UNSUPPORTED, "UsingCustomInitial is not supported");
+    if (mg && get_double(mg, "Ef", c.fermi_energy)) c.init_type = SBE_INIT_FERMI_DIRAC;
+    sbe_model &m = spec->model;
+    if (spec->target == "Haldane") {
+        double a0;
+        if (!(get_double(mg, "t1", m.t1) && get_double(mg, "t2", m.t2) && get_double(mg, "phi0", m.phi0) &&
+              get_double(mg, "a0", a0)))
+            return fe_fail(SBE_ERR_MODEL, "Some Haldane paramters are missing");
+        m.a0 = a0 / kAuAngstrom;
+        if (!get_double(mg, "M0", m.M0)) {
+            double mt2;
+            if (!get_double(mg, "Mt2", mt2)) return fe_fail(SBE_ERR_MODEL, "M0 param missing");
+            m.M0 = mt2 * m.t2;
+        }
+        m.kind = SBE_MODEL_HALDANE;
+        m.nband = 2;
+        m.nval = 1;
+        m.is_dipole_zero = 1;
+    } else {
+        return fe_fail(SBE_ERR_UNSUPPORTED, "target '" + spec->target +
+                                                "' is not available as a config-file model (Haldane only; "
+                                                "Wannier models are passed as arrays)");
+    }
+    c.npulses = static_cast<int32_t>(spec->pulses.size());
+    c.pulses = spec->pulses.data();
+    *out = spec.release();
+    return SBE_OK;
+}
+
+int sbe_spec_parse_file(const char *path, sbe_spec **out)
+{
+    if (!path) return fe_fail(SBE_ERR_CONFIG, "null path");
+    std::ifstream f(path);
+    if (!f) return fe_fail(SBE_ERR_CONFIG, std::string("I/O error while reading file ") + path);
+    std::stringstream ss;
+    ss << f.rdbuf();
+    return sbe_spec_parse_string(ss.str().c_str(), out);
+}
+
+int sbe_spec_get(const sbe_spec *spec, const sbe_config **cfg, const sbe_model **model)
+{
+    if (!spec) return fe_fail(SBE_ERR_CONFIG, "null spec");
+    if (cfg) *cfg = &spec->cfg;
+    if (model) *model = &spec->model;
+    return SBE_OK;
+}
+
+const char *sbe_spec_warnings(const sbe_spec *spec) { return spec ? spec->warnings.c_str() : ""; }
+
+void sbe_spec_free(sbe_spec *spec) { delete spec; }
+
+int sbe_write_outputs(const char *dir, const sbe_config *cfg, const sbe_model *model, const sbe_info *info,
+                      const double *j_inter, const double *j_intra, const double *rho_bar, const double *band_pop)
+{
+    if (!dir || !cfg || !model || !info || !j_inter || !j_intra || !rho_bar)
+        return fe_fail(SBE_ERR_CONFIG, "null argument");
+    const std::string d(dir);
+    auto open = [&](const char *name) { return std::fopen((d + "/" + name).c_str(), "w"); };
+    sbe::Laser L;
+    std::string err;
+    int rc = L.init(*cfg, err);
+    if (rc) return fe_fail(rc, err);
+    const int64_t nt = info->nt;
+    const int N = info->nband;
+    const double nk = static_cast<double>(info->nk_total);
+    const double dV = info->dV;
+    // outlaserdata.dat, antelope.cpp:208-235
+    FILE *f = open("outlaserdata.dat");
+    if (!f) return fe_fail(SBE_ERR_CONFIG, "cannot open outlaserdata.dat in " + d);
+    for (int64_t n = 0; n < nt; ++n) {
+        const double at = L.atmin + L.dt * static_cast<double>(n);
+        double A[3], E[3];
+        L.eval(at, A, E);
+        std::fprintf(f, "%e    ", at);
+        for (int i = 0; i < 3; ++i) std::fprintf(f, "%e      ", E[i]);
+        for (int i = 0; i < 3; ++i) std::fprintf(f, "%e      ", A[i]);
+        std::fprintf(f, "\n");
+    }
+    std::fclose(f);
+    // laserParameters.dat is opened and never written by the reference (antelope.cpp:179)
+    f = open("laserParameters.dat");
+    if (f) std::fclose(f);
+    // currents x dV, occupations / Ntotal, coherence / Ntotal (antelope.cpp:514-548)
+    FILE *fi = open("interband_dipole_full_evol.dat");
+    FILE *fa = open("intraband_current_full_evol.dat");
+    FILE *fo = open("occupation__full__evol.dat");
+    FILE *fc = open("coherence__full__evol.dat");
+    FILE *fb = band_pop ? open("band_population__full__evol.dat") : nullptr;
+    if (!fi || !fa || !fo || !fc || (band_pop && !fb)) {
+        for (FILE *x : {fi, fa, fo, fc, fb})
+            if (x) std::fclose(x);
+        return fe_fail(SBE_ERR_CONFIG, "cannot open output files in " + d);
+    }
+    const int v = model->nval > 0 ? model->nval - 1 : 0;
+    const int cnd = model->nval < N ? model->nval : N - 1;
+    for (int64_t n = 0; n < nt; ++n) {
+        for (int a = 0; a < 3; ++a) {
+            std::fprintf(fi, "%.16e    ", j_inter[3 * n + a] * dV);
+            std::fprintf(fa, "%.16e    ", j_intra[3 * n + a] * dV);
+        }
+        std::fprintf(fi, "\n");
+        std::fprintf(fa, "\n");
+        const double *r = rho_bar + n * 2 * N * N;
+        double tot = 0.0;
+        for (int m = 0; m < N; ++m) {
+            std::fprintf(fo, "%.16e    ", r[2 * (m * N + m)] / nk);
+            tot += r[2 * (m * N + m)];
+        }
+        std::fprintf(fo, "%.16e\n", tot / nk);
+        // the reference indexes Nval*N + Nval + 1, outside the N x N block
+        // (reads the next row); the intended valence-conduction element is written
+        const int e = v * N + cnd;
+        std::fprintf(fc, "%.16e    ", r[2 * e] / nk);
+        std::fprintf(fc, "%.16e\n", r[2 * e + 1] / nk);
+        if (fb) {  // band populations normalised by the summed k weights
+            double s = 0.0;
+            for (int m = 0; m < N; ++m) {
+                std::fprintf(fb, "%.16e    ", band_pop[n * N + m] / info->total_weight);
+                s += band_pop[n * N + m];
+            }
+            std::fprintf(fb, "%.16e\n", s / info->total_weight);
+        }
+    }
+    for (FILE *x : {fi, fa, fo, fc, fb})
+        if (x) std::fclose(x);
+    return SBE_OK;
+}
+
+}  // extern "C"

@@ -220,16 +220,17 @@ __device__ __forceinline__ Vec3<R> time_sample(const KPoint<R> &p, const R *row,
 //   f/2 = d x s - (g2/2)(s - s_par) - (g1/2) s_par,  dn/2 = -(g1/2) n,
 // with the factor 2 folded into the RK coefficients (dt a_ij, dt b_i).
 // The NN part of the phase table carries the factor t1.
+// blk: block index within the k block of this solve.
 template <typename R, int RK, bool T1ON, bool DEPH, int KPT>
-__global__ __launch_bounds__(kBlock) void haldane_steps_kernel(const HaldaneArgs a, const R *__restrict__ tabr,
-                                                               R *__restrict__ st, double *__restrict__ partials)
+__device__ __forceinline__ void steps_body(const HaldaneArgs &a, const R *__restrict__ tabr, R *__restrict__ st,
+                                           double *__restrict__ partials, int64_t blk)
 {
     using C = Coef<RK>;
     constexpr int S = C::S;
     constexpr bool kComp = StateT<R>::kComp != 0;
     const int64_t ks = a.nk_stride;
-    const int64_t lane0 = static_cast<int64_t>(blockIdx.x) * (kBlock * KPT) + threadIdx.x;
-    const int wave = static_cast<int>((static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) >> 6);
+    const int64_t lane0 = blk * (kBlock * KPT) + threadIdx.x;
+    const int wave = static_cast<int>((blk * kBlock + threadIdx.x) >> 6);
 
     KPoint<R> p[KPT];
 #pragma unroll
@@ -417,6 +418,30 @@ __global__ __launch_bounds__(kBlock) void haldane_steps_kernel(const HaldaneArgs
     }
 }
 
+template <typename R, int RK, bool T1ON, bool DEPH, int KPT>
+__global__ __launch_bounds__(kBlock) void haldane_steps_kernel(const HaldaneArgs a, const R *__restrict__ tabr,
+                                                               R *__restrict__ st, double *__restrict__ partials)
+{
+    steps_body<R, RK, T1ON, DEPH, KPT>(a, tabr, st, partials, blockIdx.x);
+}
+
+// Many independent solves in one launch: block b works for solve
+// block_inst[b] on that solve's block b - block_first[solve]. Every solve
+// carries its own model, grid, laser table, state and step count; solves
+// with nothing left to do this launch leave at once.
+template <typename R, int RK, bool T1ON, bool DEPH>
+__global__ __launch_bounds__(kBlock) void haldane_batch_kernel(const HaldaneArgs *__restrict__ insts,
+                                                               const int32_t *__restrict__ block_inst,
+                                                               const int32_t *__restrict__ block_first)
+{
+    const int inst = block_inst[blockIdx.x];
+    const HaldaneArgs a = insts[inst];
+    if (a.nsteps <= 0) return;  // whole block
+    const R *tab = StateT<R>::kComp ? reinterpret_cast<const R *>(a.phase_tab32) : reinterpret_cast<const R *>(a.phase_tab);
+    steps_body<R, RK, T1ON, DEPH, 1>(a, tab, static_cast<R *>(a.state), a.partials,
+                                     static_cast<int64_t>(blockIdx.x) - block_first[inst]);
+}
+
 // Initial state (SBEs.h:284-364): rho_H = diag(occupations) at k, rho = U rho_H U^dag
 // -> n = f0 + f1, s = (f1 - f0) dh(k). Padding points get a zero state.
 template <typename R>
@@ -529,6 +554,32 @@ hipError_t launch_haldane_steps(const HaldaneArgs &a, int rk_order, bool t1_on, 
     } else if (rk_order == SBE_RK5_FEHLBERG) {
         if (f32) launch_steps_kpt<float, 5>(a, a.phase_tab32, t1_on, s);
         else launch_steps_kpt<double, 5>(a, a.phase_tab, t1_on, s);
+    } else {
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <typename R, int RK>
+static void launch_batch_rk(const HaldaneArgs *insts, const int32_t *bi, const int32_t *bf, int nblocks, bool t1, bool deph,
+                            hipStream_t s)
+{
+    if (t1) haldane_batch_kernel<R, RK, true, true><<<nblocks, kBlock, 0, s>>>(insts, bi, bf);
+    else if (deph) haldane_batch_kernel<R, RK, false, true><<<nblocks, kBlock, 0, s>>>(insts, bi, bf);
+    else haldane_batch_kernel<R, RK, false, false><<<nblocks, kBlock, 0, s>>>(insts, bi, bf);
+}
+
+hipError_t launch_haldane_batch(const HaldaneArgs *d_insts, const int32_t *d_block_inst, const int32_t *d_block_first,
+                                int nblocks, int rk_order, int precision, bool t1_on, bool deph, hipStream_t s)
+{
+    if (nblocks <= 0) return hipSuccess;
+    const bool f32 = precision == SBE_PREC_FP32;
+    if (rk_order == SBE_RK4) {
+        if (f32) launch_batch_rk<float, 4>(d_insts, d_block_inst, d_block_first, nblocks, t1_on, deph, s);
+        else launch_batch_rk<double, 4>(d_insts, d_block_inst, d_block_first, nblocks, t1_on, deph, s);
+    } else if (rk_order == SBE_RK5_FEHLBERG) {
+        if (f32) launch_batch_rk<float, 5>(d_insts, d_block_inst, d_block_first, nblocks, t1_on, deph, s);
+        else launch_batch_rk<double, 5>(d_insts, d_block_inst, d_block_first, nblocks, t1_on, deph, s);
     } else {
         return hipErrorInvalidValue;
     }

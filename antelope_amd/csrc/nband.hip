@@ -8,7 +8,7 @@
 //
 // Decomposition per RK stage:
 //  * nb_fourier_kernel: complex GEMM C[m][k][ij] = sum_R P[k][R] B[R][m,ij]
-//    over a tile of 64 k x 64 columns, with P = e^{ik.R} (fixed, built once)
+//    on FP64 MFMA over a tile of 64 k x 64 columns, with P = e^{ik.R} (fixed, built once)
 //    and B = w_R e^{iA(t).R} X_R (X = H + E.r, H, or -i R_a H), formed on
 //    the fly while staging through LDS. t and E(t) come from a device table
 //    indexed by (step, stage), so launch arguments do not carry time.
@@ -94,22 +94,35 @@ __device__ __forceinline__ cd b_element(const NbArgs &a, const NbEval &ev, int r
     return cmul(cscale(phase, a.rweight[r]), x);
 }
 
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+// 64 k x 64 column tile per block of 4 waves; wave w owns the 32 x 32
+// sub-tile (w / 2, w % 2) as 2 x 2 MFMA tiles of 16 x 16. The complex
+// product is four real v_mfma_f64_16x16x4f64 per tile and K step:
+//   re += a_re b_re - a_im b_im,  im += a_re b_im + a_im b_re.
+// MFMA operand layout (16x16x4, FP64): A row = lane % 16, k = lane / 16;
+// B k = lane / 16, column = lane % 16; D element r of lane l is
+// row (l / 16) + 4 r, column l % 16.
 __global__ __launch_bounds__(256) void nb_fourier_kernel(const NbArgs a, const NbEval ev, const cd *__restrict__ P,
                                                          cd *__restrict__ out)
 {
     __shared__ cd As[GK][GM];
     __shared__ cd Bs[GK][GN];
     __shared__ cd ph[GK];
-    const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
     const int64_t k0 = static_cast<int64_t>(blockIdx.x) * GM;
     const int64_t c0 = static_cast<int64_t>(blockIdx.y) * GN;
     const int nn = a.N * a.N;
     const int64_t ncols = static_cast<int64_t>(ev.nmat) * nn;
-    cd acc[4][4];
+    d4 accr[2][2], acci[2][2];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = cmk(0.0, 0.0);
+        for (int j = 0; j < 2; ++j) {
+            accr[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+            acci[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+        }
 
     for (int r0 = 0; r0 < a.nr; r0 += GK) {
         // e^{i A(t).R} for this slice
@@ -140,31 +153,37 @@ __global__ __launch_bounds__(256) void nb_fourier_kernel(const NbArgs a, const N
         }
         __syncthreads();
 #pragma unroll
-        for (int kk = 0; kk < GK; ++kk) {
-            cd av[4], bv[4];
+        for (int kb = 0; kb < GK; kb += 4) {
+            const int kk = kb + (lane >> 4);
+            cd av[2], bv[2];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) av[i] = As[kk][ty + 16 * i];
+            for (int i = 0; i < 2; ++i) av[i] = As[kk][32 * wm + 16 * i + (lane & 15)];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) bv[j] = Bs[kk][tx + 16 * j];
+            for (int j = 0; j < 2; ++j) bv[j] = Bs[kk][32 * wn + 16 * j + (lane & 15)];
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int i = 0; i < 2; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) cfma(acc[i][j], av[i], bv[j]);
+                for (int j = 0; j < 2; ++j) {
+                    accr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i].x, bv[j].x, accr[i][j], 0, 0, 0);
+                    accr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-av[i].y, bv[j].y, accr[i][j], 0, 0, 0);
+                    acci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i].x, bv[j].y, acci[i][j], 0, 0, 0);
+                    acci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i].y, bv[j].x, acci[i][j], 0, 0, 0);
+                }
         }
         __syncthreads();
     }
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int64_t k = k0 + ty + 16 * i;
-        if (k >= a.nk_stride) continue;
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int64_t c = c0 + tx + 16 * j;
-            if (c >= ncols) continue;
-            const int mat = static_cast<int>(c / nn), ij = static_cast<int>(c % nn);
-            out[(static_cast<int64_t>(mat) * a.nk_stride + k) * nn + ij] = acc[i][j];
-        }
-    }
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int64_t k = k0 + 32 * wm + 16 * i + (lane >> 4) + 4 * q;
+                const int64_t c = c0 + 32 * wn + 16 * j + (lane & 15);
+                if (k >= a.nk_stride || c >= ncols) continue;
+                const int mat = static_cast<int>(c / nn), ij = static_cast<int>(c % nn);
+                out[(static_cast<int64_t>(mat) * a.nk_stride + k) * nn + ij] = cmk(accr[i][j][q], acci[i][j][q]);
+            }
 }
 
 // ------------------------------------------------- wave matrix helpers --

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -37,8 +38,11 @@ int hip_check(sbe_ctx *ctx, hipError_t e, const char *what)
     return set_error(ctx, code, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-// Upper bound of the per-chunk partial-sum buffer.
+// Upper bound of the per-chunk partial-sum buffer (per context; the members
+// of a batch use a smaller one).
 constexpr size_t kPartialBytes = size_t(128) << 20;
+constexpr size_t kBatchPartialBytes = size_t(24) << 20;
+thread_local size_t t_partial_bytes = kPartialBytes;
 
 struct Plan {
     int kpt = 1;
@@ -109,7 +113,7 @@ int make_plan(const sbe_config &cfg, const sbe_model &m, int64_t nk, int64_t nt,
     p.nt = nt;
     p.nstage = nstage;
     const size_t row = static_cast<size_t>(p.nwaves) * p.nv * sizeof(double);
-    p.chunk = std::max<int64_t>(1, std::min<int64_t>(nt, static_cast<int64_t>(kPartialBytes / row)));
+    p.chunk = std::max<int64_t>(1, std::min<int64_t>(nt, static_cast<int64_t>(t_partial_bytes / row)));
     const size_t real = cfg.precision == SBE_PREC_FP32 ? sizeof(float) : sizeof(double);
     const int comps = cfg.precision == SBE_PREC_FP32 ? 8 : 4;
     p.bytes_state = static_cast<size_t>(p.nk_stride) * comps * real;
@@ -550,6 +554,181 @@ int sbe_get_info(const sbe_ctx *ctx, sbe_info *info)
     info->steps_done = ctx->steps_done;
     info->device_bytes = ctx->device_bytes;
     return SBE_OK;
+}
+
+}  // extern "C"
+
+struct sbe_batch {
+    std::vector<sbe_ctx *> ctx;
+    hipStream_t stream = nullptr;
+    sbe::HaldaneArgs *d_insts = nullptr;
+    int32_t *d_map = nullptr;  // [2][max_blocks]: block -> solve, solve -> first block
+    int max_blocks = 0;
+    int rk = 0, prec = 0;
+    bool t1 = false, deph = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+    std::string err;
+};
+
+namespace {
+int batch_fail(sbe_batch *b, int code, const std::string &m)
+{
+    if (b) b->err = m;
+    g_thread_error = m;
+    return code;
+}
+}  // namespace
+
+extern "C" {
+
+const char *sbe_batch_last_error(const sbe_batch *b) { return b ? b->err.c_str() : g_thread_error.c_str(); }
+
+void sbe_batch_destroy(sbe_batch *b)
+{
+    if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (auto &p : b->ev) {
+        (void)hipEventDestroy(p.first);
+        (void)hipEventDestroy(p.second);
+    }
+    for (sbe_ctx *c : b->ctx) sbe_destroy(c);
+    if (b->d_insts) (void)hipFree(b->d_insts);
+    if (b->d_map) (void)hipFree(b->d_map);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+int sbe_batch_create(int32_t n, const sbe_config *cfgs, const sbe_model *models, sbe_batch **out)
+{
+    if (!out || !cfgs || !models || n < 1) return batch_fail(nullptr, SBE_ERR_CONFIG, "bad batch arguments");
+    *out = nullptr;
+    for (int i = 0; i < n; ++i) {
+        if (models[i].kind != SBE_MODEL_HALDANE)
+            return batch_fail(nullptr, SBE_ERR_UNSUPPORTED, "batched solves take 2-band (Haldane) models");
+        if (cfgs[i].rk_order != cfgs[0].rk_order || cfgs[i].precision != cfgs[0].precision)
+            return batch_fail(nullptr, SBE_ERR_CONFIG, "all solves of a batch share rk_order and precision");
+    }
+    sbe_batch *b = new (std::nothrow) sbe_batch();
+    if (!b) return batch_fail(nullptr, SBE_ERR_OOM, "host allocation failed");
+    t_partial_bytes = kBatchPartialBytes;
+    for (int i = 0; i < n; ++i) {
+        sbe_ctx *c = nullptr;
+        const int rc = sbe_create(&cfgs[i], &models[i], -1, -1, &c);
+        if (rc) {
+            t_partial_bytes = kPartialBytes;
+            const std::string m = "solve " + std::to_string(i) + ": " + g_thread_error;
+            sbe_batch_destroy(b);
+            return batch_fail(nullptr, rc, m);
+        }
+        b->ctx.push_back(c);
+        b->max_blocks += static_cast<int>(c->nk_stride / kBlock);
+        b->t1 = b->t1 || c->hargs.gam1 != 0.0;
+        b->deph = b->deph || c->hargs.gam1 != 0.0 || c->hargs.gam2 != 0.0;
+    }
+    t_partial_bytes = kPartialBytes;
+    b->rk = cfgs[0].rk_order;
+    b->prec = cfgs[0].precision;
+    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc(&b->d_insts, sizeof(sbe::HaldaneArgs) * n);
+    if (e == hipSuccess) e = hipMalloc(&b->d_map, sizeof(int32_t) * (b->max_blocks + n));
+    if (e != hipSuccess) {
+        const std::string m = std::string("batch set-up: ") + hipGetErrorString(e);
+        sbe_batch_destroy(b);
+        return batch_fail(nullptr, SBE_ERR_HIP, m);
+    }
+    *out = b;
+    return SBE_OK;
+}
+
+int sbe_batch_run(sbe_batch *b, int64_t max_steps)
+{
+    if (!b) return batch_fail(nullptr, SBE_ERR_STATE, "null batch");
+    const int n = static_cast<int>(b->ctx.size());
+    int64_t budget = max_steps > 0 ? max_steps : INT64_MAX;
+    std::vector<sbe::HaldaneArgs> insts(n);
+    std::vector<int32_t> map(b->max_blocks + n);
+    while (budget > 0) {
+        int64_t launch = INT64_MAX;
+        for (sbe_ctx *c : b->ctx)
+            if (c->steps_done < c->laser.nt) launch = std::min<int64_t>(launch, c->chunk_steps);
+        if (launch == INT64_MAX) break;  // all done
+        launch = std::min(launch, budget);
+        int nb = 0;
+        for (int i = 0; i < n; ++i) {
+            sbe_ctx *c = b->ctx[i];
+            insts[i] = c->hargs;
+            insts[i].step0 = c->steps_done;
+            insts[i].nsteps = static_cast<int32_t>(std::min<int64_t>(launch, c->laser.nt - c->steps_done));
+            map[b->max_blocks + i] = nb;
+            if (insts[i].nsteps > 0)
+                for (int64_t q = 0; q < c->nk_stride / kBlock; ++q) map[nb++] = i;
+        }
+        // solve -> first block table sits after the block -> solve table
+        int32_t *d_first = b->d_map + b->max_blocks;
+        hipError_t e = hipMemcpyAsync(b->d_insts, insts.data(), sizeof(sbe::HaldaneArgs) * n, hipMemcpyHostToDevice, b->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(b->d_map, map.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, b->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(d_first, map.data() + b->max_blocks, sizeof(int32_t) * n, hipMemcpyHostToDevice, b->stream);
+        hipEvent_t ea = nullptr, eb = nullptr;
+        if (e == hipSuccess) e = hipEventCreate(&ea);
+        if (e == hipSuccess) e = hipEventCreate(&eb);
+        if (e == hipSuccess) e = hipEventRecord(ea, b->stream);
+        if (e == hipSuccess)
+            e = sbe::launch_haldane_batch(b->d_insts, b->d_map, d_first, nb, b->rk, b->prec, b->t1, b->deph, b->stream);
+        if (e == hipSuccess) e = hipEventRecord(eb, b->stream);
+        if (ea && eb) b->ev.push_back({ea, eb});
+        for (int i = 0; i < n && e == hipSuccess; ++i) {
+            sbe_ctx *c = b->ctx[i];
+            if (insts[i].nsteps <= 0) continue;
+            e = sbe::launch_series_reduce(c->d_partials, insts[i].nsteps, c->nwaves, c->nv,
+                                          c->d_series + c->steps_done * c->nv, b->stream);
+            c->steps_done += insts[i].nsteps;
+        }
+        if (e != hipSuccess) return batch_fail(b, SBE_ERR_HIP, std::string("batch launch: ") + hipGetErrorString(e));
+        budget -= launch;
+    }
+    return SBE_OK;
+}
+
+int sbe_batch_synchronize(sbe_batch *b)
+{
+    if (!b) return batch_fail(nullptr, SBE_ERR_STATE, "null batch");
+    const hipError_t e = hipStreamSynchronize(b->stream);
+    return e == hipSuccess ? SBE_OK : batch_fail(b, SBE_ERR_HIP, hipGetErrorString(e));
+}
+
+int sbe_batch_get_info(const sbe_batch *b, int32_t i, sbe_info *info)
+{
+    if (!b || i < 0 || i >= static_cast<int32_t>(b->ctx.size())) return batch_fail(nullptr, SBE_ERR_RANGE, "bad solve index");
+    return sbe_get_info(b->ctx[i], info);
+}
+
+int sbe_batch_get_series(sbe_batch *b, int32_t i, double *j_inter, double *j_intra, double *rho_bar, double *band_pop)
+{
+    if (!b || i < 0 || i >= static_cast<int32_t>(b->ctx.size())) return batch_fail(nullptr, SBE_ERR_RANGE, "bad solve index");
+    int rc = sbe_batch_synchronize(b);
+    if (rc) return rc;
+    rc = sbe_get_series(b->ctx[i], j_inter, j_intra, rho_bar, band_pop);
+    if (rc) return batch_fail(b, rc, b->ctx[i]->err);
+    return SBE_OK;
+}
+
+int sbe_batch_get_timing(sbe_batch *b, double *ms, int64_t *launches)
+{
+    if (!b) return batch_fail(nullptr, SBE_ERR_STATE, "null batch");
+    int rc = sbe_batch_synchronize(b);
+    double sum = 0.0;
+    for (auto &p : b->ev) {
+        float t = 0.f;
+        if (!rc && hipEventElapsedTime(&t, p.first, p.second) != hipSuccess) rc = batch_fail(b, SBE_ERR_HIP, "event timing");
+        sum += t;
+        (void)hipEventDestroy(p.first);
+        (void)hipEventDestroy(p.second);
+    }
+    if (ms) *ms = sum;
+    if (launches) *launches = static_cast<int64_t>(b->ev.size());
+    b->ev.clear();
+    return rc;
 }
 
 void sbe_destroy(sbe_ctx *ctx)

@@ -153,6 +153,8 @@ hipError_t launch_haldane_steps(const HaldaneArgs &a, int rk_order, bool t1_on, 
 hipError_t launch_series_reduce(const double *partials, int64_t nsteps, int32_t nrows, int nv,
                                 double *series, hipStream_t s);
 hipError_t launch_haldane_export(const HaldaneArgs &a, double *out, hipStream_t s);
+hipError_t launch_haldane_batch(const HaldaneArgs *d_insts, const int32_t *d_block_inst, const int32_t *d_block_first,
+                                int nblocks, int rk_order, int precision, bool t1_on, bool deph, hipStream_t s);
 int haldane_waves(int64_t nk);
 // N-band host side (nband_host.cpp)
 int nb_validate(const sbe_model &m, std::string &err);

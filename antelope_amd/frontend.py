@@ -1,0 +1,100 @@
+"""Python binding of the input/output front end (include/antelope/sbe_frontend.h).
+
+`parse_config` reads an inputParam.cfg (libconfig format) the way the
+reference's driver does and returns the C structs plus their owner;
+`write_outputs` writes the reference's .dat files from k-summed series.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+from . import _abi as A
+from .solver import SBEError, load_library
+
+_SIGS = {
+    "sbe_spec_parse_string": ([C.c_char_p, C.POINTER(C.c_void_p)], C.c_int),
+    "sbe_spec_parse_file": ([C.c_char_p, C.POINTER(C.c_void_p)], C.c_int),
+    "sbe_spec_get": ([C.c_void_p, C.POINTER(C.POINTER(A.sbe_config)), C.POINTER(C.POINTER(A.sbe_model))], C.c_int),
+    "sbe_spec_warnings": ([C.c_void_p], C.c_char_p),
+    "sbe_spec_free": ([C.c_void_p], None),
+    "sbe_frontend_last_error": ([], C.c_char_p),
+    "sbe_write_outputs": ([C.c_char_p, C.POINTER(A.sbe_config), C.POINTER(A.sbe_model), C.POINTER(A.sbe_info),
+                           A.c_double_p, A.c_double_p, A.c_double_p, A.c_double_p], C.c_int),
+}
+_bound = False
+
+
+def _lib():
+    global _bound
+    lib = load_library()
+    if not _bound:
+        A.bind(lib, _SIGS)
+        _bound = True
+    return lib
+
+
+class Spec:
+    """A parsed inputParam.cfg; .config / .model are the C structs."""
+
+    def __init__(self, handle):
+        self._h = handle
+        lib = _lib()
+        pc = C.POINTER(A.sbe_config)()
+        pm = C.POINTER(A.sbe_model)()
+        lib.sbe_spec_get(self._h, C.byref(pc), C.byref(pm))
+        # Python-owned copies, so the structs outlive this object
+        self.config = A.sbe_config.from_buffer_copy(pc.contents)
+        n = self.config.npulses
+        self._pulses = (A.sbe_pulse * n)(*[A.sbe_pulse.from_buffer_copy(pc.contents.pulses[i]) for i in range(n)])
+        self.config.pulses = C.cast(self._pulses, C.POINTER(A.sbe_pulse))
+        self.config._keep = self._pulses
+        self.model = A.sbe_model.from_buffer_copy(pm.contents)
+        self.warnings = lib.sbe_spec_warnings(self._h).decode()
+
+    def to_c(self):
+        return self.config, self
+
+    @property
+    def npoints(self):
+        return list(self.config.npoints)
+
+    def close(self):
+        if self._h:
+            _lib().sbe_spec_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _check(rc):
+    if rc != A.SBE_OK:
+        raise SBEError(rc, _lib().sbe_frontend_last_error().decode())
+
+
+def parse_config(text: str | None = None, path: str | Path | None = None) -> Spec:
+    lib = _lib()
+    h = C.c_void_p()
+    if path is not None:
+        _check(lib.sbe_spec_parse_file(str(path).encode(), C.byref(h)))
+    else:
+        _check(lib.sbe_spec_parse_string(text.encode(), C.byref(h)))
+    return Spec(h)
+
+
+def write_outputs(outdir, config_c, model_c, info, series: dict) -> None:
+    lib = _lib()
+    ji = np.ascontiguousarray(series["j_inter"], dtype=np.float64)
+    ja = np.ascontiguousarray(series["j_intra"], dtype=np.float64)
+    rb = series["rho_bar"]
+    rb = np.ascontiguousarray(np.stack([rb.real, rb.imag], -1), dtype=np.float64)
+    bp = np.ascontiguousarray(series["band_pop"], dtype=np.float64)
+    p = lambda a: a.ctypes.data_as(A.c_double_p)  # noqa: E731
+    _check(lib.sbe_write_outputs(str(outdir).encode(), C.byref(config_c), C.byref(model_c), C.byref(info),
+                                 p(ji), p(ja), p(rb), p(bp)))

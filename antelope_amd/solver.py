@@ -164,3 +164,76 @@ class SBESolver:
             self.close()
         except Exception:
             pass
+
+
+class SBEBatch:
+    """Independent 2-band solves advanced together (sbe_batch_*): one launch of
+    the step kernel per chunk covers every unfinished solve."""
+
+    def __init__(self, configs, models):
+        self._lib = load_library()
+        n = len(configs)
+        cs, ms, self._keep = [], [], []
+        for c, m in zip(configs, models):
+            cc, k1 = c.to_c()
+            mm, k2 = m.to_c()
+            cs.append(cc)
+            ms.append(mm)
+            self._keep += [k1, k2]
+        carr = (A.sbe_config * n)(*cs)
+        marr = (A.sbe_model * n)(*ms)
+        h = C.c_void_p()
+        rc = self._lib.sbe_batch_create(n, carr, marr, C.byref(h))
+        if rc != A.SBE_OK:
+            raise SBEError(rc, self._lib.sbe_batch_last_error(None).decode())
+        self._h = h
+        self.n = n
+        self.nband = [m.nband for m in models]
+
+    def _chk(self, rc):
+        if rc != A.SBE_OK:
+            raise SBEError(rc, self._lib.sbe_batch_last_error(self._h).decode())
+
+    def run(self, max_steps: int = 0) -> None:
+        self._chk(self._lib.sbe_batch_run(self._h, max_steps))
+
+    def synchronize(self) -> None:
+        self._chk(self._lib.sbe_batch_synchronize(self._h))
+
+    def info(self, i: int) -> A.sbe_info:
+        inf = A.sbe_info()
+        self._chk(self._lib.sbe_batch_get_info(self._h, i, C.byref(inf)))
+        return inf
+
+    def series(self, i: int) -> dict:
+        nt, N = self.info(i).nt, self.nband[i]
+        out = {"j_inter": np.zeros((nt, 3)), "j_intra": np.zeros((nt, 3)),
+               "rho_bar": np.zeros((nt, N, N, 2)), "band_pop": np.zeros((nt, N))}
+        self._chk(self._lib.sbe_batch_get_series(self._h, i, _ptr(out["j_inter"]), _ptr(out["j_intra"]),
+                                                 _ptr(out["rho_bar"]), _ptr(out["band_pop"])))
+        rb = out.pop("rho_bar")
+        out["rho_bar"] = rb[..., 0] + 1j * rb[..., 1]
+        return out
+
+    def timing(self) -> dict:
+        ms = C.c_double()
+        n = C.c_int64()
+        self._chk(self._lib.sbe_batch_get_timing(self._h, C.byref(ms), C.byref(n)))
+        return {"step_ms": ms.value, "launches": n.value}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.sbe_batch_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

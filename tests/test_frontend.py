@@ -1,0 +1,149 @@
+"""Input/output front end on the CPU: inputParam.cfg reading (antelope.cpp:90-138,
+SBEs.h:432-561, Materials/Haldane.h:46-84) and the reference's output files
+(antelope.cpp:208-235, 514-548)."""
+import math
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from antelope_amd import _abi as A
+from antelope_amd.frontend import parse_config, write_outputs
+from antelope_amd.solver import SBEError, laser_eval
+
+GOLDEN = Path(__file__).parent / "golden" / "configs"
+REF_CFG = Path("/root/reference/src/GeneralModel/inputParam.cfg")
+
+
+def test_parse_plain_reference_case():
+    s = parse_config(path=GOLDEN / "haldane_plain_60.cfg")
+    c, m = s.config, s.model
+    assert list(c.npoints) == [60, 60, 1]
+    assert c.dt == 0.25 and c.T2 == 220.0 and c.T1 == -1.0
+    assert c.rk_order == 4 and c.inter_intra == 1 and c.gauge == A.SBE_GAUGE_LENGTH_WANNIER
+    assert c.npulses == 1
+    p = c.pulses[0]
+    assert (p.E0, p.w0, p.ncycles, p.envelope) == (0.0045, 0.014, 10.0, A.SBE_ENV_SIN2)
+    assert m.kind == A.SBE_MODEL_HALDANE and m.nband == 2 and m.nval == 1
+    assert m.a0 == pytest.approx(1.41978 / 0.529177210903, rel=1e-15)
+    assert m.M0 == pytest.approx(2.54 * 0.025, rel=1e-15)
+    assert s.warnings == ""
+
+
+BASE = """
+target = "Haldane";
+laser = { pulses = ( { E0 = 0.01; w0 = 0.05; ncycles = 4; ellip = 0.5; cep = 90.0; t0 = 1.0;
+                       phix = 180.0; env_name = "cos4"; thetaz = 45.0; } ); };
+calc = { Npoints = [8, 4]; %s };
+Haldane = { a0 = 1.0; t1 = 0.075; t2 = 0.025; phi0 = 1.16; M0 = 0.01; };
+"""
+
+
+def test_defaults_units_and_warnings():
+    s = parse_config(BASE % "")
+    c = s.config
+    assert c.dt == 0.1 and c.rk_order == 5 and c.inter_intra == 0       # SBEs.h:438-455
+    assert c.T1 == -1.0 and c.T2 == -1.0 and list(c.ksfactor) == [1.0, 1.0, 1.0]
+    assert "ksfactor" in s.warnings and "gauge" in s.warnings
+    p = c.pulses[0]
+    assert p.cep == pytest.approx(math.pi / 2) and p.phix == pytest.approx(math.pi)
+    assert p.thetaz == pytest.approx(math.pi / 4) and p.phiz == 0.0 and p.ncycles == 4.0
+    assert s.model.M0 == 0.01
+
+
+def test_libconfig_syntax_features():
+    text = """
+    // colon separators, hex, string concatenation, all comment kinds, trailing commas
+    target : "Hal" "dane";  # concatenated
+    gauge = "Wannier"
+    laser : { offset = [100, 50.5]; efield = "consistent";
+              pulses = ( { E0 = 1e-3; w0 = .05; ncycles = 2; ellip = 0; cep = 0; t0 = 0; phix = 0; env_name = "gauss" },
+                         { E0 = 2e-3; w0 = 0.1; ncycles = 3; ellip = 1; cep = 0; t0 = 10; phix = 0;
+                           env_name = "gauss_clipped"; ncycles_clipped = 2.0 }, ) };
+    calc : { Npoints = [0x10, 4L, 2]; dt = 5e-2; InterIntra = TRUE; RKorder = 4; ksfactor = [1, 2.0, 0.5];
+             BZaxes = [1., 0., 0., 0., 1., 0., 0., 0., 1.]; BZorigin = [0.1, 0.2]; grid = "mp";
+             precision = "fp32"; T2 = 220 };
+    /* block
+       comment */
+    Haldane : { a0 = 1; t1 = 0.1; t2 = 0.02; phi0 = 0; Mt2 = 3; Ef = -0.1; };
+    """
+    c = parse_config(text).config
+    assert list(c.npoints) == [16, 4, 2] and c.dt == 0.05 and c.inter_intra == 1
+    assert (c.offset_before, c.offset_after) == (100.0, 50.5)
+    assert c.efield_mode == A.SBE_EFIELD_CONSISTENT and c.grid == A.SBE_GRID_MP and c.precision == A.SBE_PREC_FP32
+    assert c.npulses == 2 and c.pulses[1].envelope == A.SBE_ENV_GAUSS_CLIPPED and c.pulses[1].ncycles_clipped == 2.0
+    assert c.has_bz_axes == 1 and c.bz_axes[8] == 1.0 and c.has_bz_origin == 1 and c.bz_origin[1] == 0.2
+    assert list(c.ksfactor) == [1.0, 2.0, 0.5] and c.T2 == 220.0
+    assert c.init_type == A.SBE_INIT_FERMI_DIRAC and c.fermi_energy == -0.1
+
+
+@pytest.mark.parametrize("text,code", [
+    ('target = "Haldane"; laser = {pulses = ();};', A.SBE_ERR_CONFIG),
+    (BASE.replace('env_name = "cos4"', 'env_name = "cos2"') % "", A.SBE_ERR_CONFIG),
+    (BASE.replace("ellip = 0.5;", "") % "", A.SBE_ERR_CONFIG),
+    (BASE.replace("M0 = 0.01;", "") % "", A.SBE_ERR_MODEL),
+    (BASE.replace('target = "Haldane"', 'target = "KaneMele"') % "", A.SBE_ERR_UNSUPPORTED),
+    ("calc = { Npoints = [4, 4] ", A.SBE_ERR_CONFIG),
+    ("calc = { Npoints = [4, 4]; dt = 1.2.3; };", A.SBE_ERR_CONFIG),
+])
+def test_parse_errors_are_status_codes(text, code):
+    with pytest.raises(SBEError) as e:
+        parse_config(text)
+    assert e.value.code == code, str(e.value)
+
+
+def test_syntax_error_reports_line():
+    with pytest.raises(SBEError) as e:
+        parse_config("a = 1;\nb = 2;\nc = = 3;\n")
+    assert "line 3" in str(e.value)
+
+
+@pytest.mark.skipif(not REF_CFG.exists(), reason="reference checkout not present")
+def test_reference_inputparam_cfg():
+    """The reference's own config parses; its Haldane2L target is not a built model."""
+    text = REF_CFG.read_text()
+    with pytest.raises(SBEError) as e:
+        parse_config(text)
+    assert e.value.code == A.SBE_ERR_UNSUPPORTED and "Haldane2L" in str(e.value)
+    s = parse_config(text.replace('target = "Haldane2L"', 'target = "Haldane"'))
+    c, m = s.config, s.model
+    assert list(c.npoints) == [201, 201, 1] and c.dt == 0.1 and c.T2 == 220.0 and c.inter_intra == 1
+    assert (c.offset_before, c.offset_after) == (100.0, 100.0) and c.pulses[0].envelope == A.SBE_ENV_GAUSS
+    assert c.pulses[0].ellip == 1.0 and m.t2 == 0.0 and m.M0 == pytest.approx(0.07202867146428378)
+
+
+def test_output_files(tmp_path):
+    s = parse_config(path=GOLDEN / "haldane_plain_60.cfg")
+    nt, N = 50, 2
+    rng = np.random.default_rng(0)
+    series = {"j_inter": rng.normal(size=(nt, 3)), "j_intra": rng.normal(size=(nt, 3)),
+              "rho_bar": rng.normal(size=(nt, N, N)) + 1j * rng.normal(size=(nt, N, N)),
+              "band_pop": rng.uniform(size=(nt, N))}
+    info = A.sbe_info()
+    info.nt, info.nband, info.nk_total, info.dV, info.total_weight = nt, N, 3600, 2.5e-3, 3481.0
+    write_outputs(tmp_path, s.config, s.model, info, series)
+    inter = np.loadtxt(tmp_path / "interband_dipole_full_evol.dat")
+    intra = np.loadtxt(tmp_path / "intraband_current_full_evol.dat")
+    np.testing.assert_allclose(inter, series["j_inter"] * 2.5e-3, rtol=1e-15)
+    np.testing.assert_allclose(intra, series["j_intra"] * 2.5e-3, rtol=1e-15)
+    occ = np.loadtxt(tmp_path / "occupation__full__evol.dat")
+    diag = np.stack([series["rho_bar"][:, 0, 0].real, series["rho_bar"][:, 1, 1].real], 1) / 3600
+    np.testing.assert_allclose(occ[:, :2], diag, rtol=1e-15)
+    np.testing.assert_allclose(occ[:, 2], diag.sum(1), rtol=1e-14)
+    coh = np.loadtxt(tmp_path / "coherence__full__evol.dat")
+    np.testing.assert_allclose(coh[:, 0] + 1j * coh[:, 1], series["rho_bar"][:, 0, 1] / 3600, rtol=1e-15)
+    bp = np.loadtxt(tmp_path / "band_population__full__evol.dat")
+    np.testing.assert_allclose(bp[:, :2], series["band_pop"] / 3481.0, rtol=1e-15)
+    assert (tmp_path / "laserParameters.dat").stat().st_size == 0
+    first = (tmp_path / "interband_dipole_full_evol.dat").read_text().splitlines()[0]
+    assert re.fullmatch(r"(-?\d\.\d{16}e[+-]\d{2}    ){3}", first), first
+    laser = np.loadtxt(tmp_path / "outlaserdata.dat")
+    assert laser.shape == (nt, 7)
+    _, _, atmin, _ = laser_eval(s, np.zeros(1))
+    Av, Ev, _, _ = laser_eval(s, atmin + 0.25 * np.arange(nt))
+    np.testing.assert_allclose(laser[:, 1:4], Ev, rtol=1e-5, atol=1e-20)  # %e keeps 6 digits
+    np.testing.assert_allclose(laser[:, 4:7], Av, rtol=1e-5, atol=1e-20)
+    assert laser[0, 0] == pytest.approx(atmin, rel=1e-6)
+    line = (tmp_path / "outlaserdata.dat").read_text().splitlines()[1]
+    assert re.fullmatch(r"-?\d\.\d{6}e[+-]\d{2}    (-?\d\.\d{6}e[+-]\d{2}      ){6}", line), line

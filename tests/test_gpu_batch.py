@@ -1,0 +1,27 @@
+"""Batched solves: every solve of a mixed batch (different models, pulses,
+grids and lengths) returns exactly the series it returns alone."""
+import numpy as np
+import pytest
+
+from antelope_amd import SBEBatch, SBESolver
+from antelope_amd.batch_workloads import haldane_batch
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_batch_equals_alone(precision):
+    cfgs, models = haldane_batch(n=5, grid=24, cycles=(1, 3), precision=precision, seed=11)
+    cfgs[1].npoints = [17, 9]           # ragged grid in the mix
+    cfgs[2].T2 = -1.0                   # one solve without dephasing
+    with SBEBatch(cfgs, models) as b:
+        b.run(700)                      # partial advance, then the rest
+        b.run()
+        got = [b.series(i) for i in range(len(cfgs))]
+        assert [b.info(i).steps_done for i in range(len(cfgs))] == [b.info(i).nt for i in range(len(cfgs))]
+    for i, (c, m) in enumerate(zip(cfgs, models)):
+        with SBESolver(c, m) as s:
+            s.run()
+            alone = s.series()
+        for k in alone:
+            assert np.array_equal(got[i][k], alone[k]), (i, k)
