@@ -91,12 +91,27 @@ class sbe_info(C.Structure):
     ]
 
 
+class sbe_request_status(C.Structure):
+    _fields_ = [("state", C.c_int32), ("status", C.c_int32), ("nt", C.c_int64), ("nband", C.c_int32),
+                ("reserved", C.c_int32), ("footprint", C.c_int64), ("t_submit", C.c_double),
+                ("t_admit", C.c_double), ("t_done", C.c_double)]
+
+
+SBE_REQ_QUEUED, SBE_REQ_RUNNING, SBE_REQ_DONE, SBE_REQ_FAILED = range(4)
+
+
+class sbe_step_stats(C.Structure):
+    _fields_ = [("blocks", C.c_int64), ("k_per_block", C.c_int64), ("accepted", C.c_int64),
+                ("rejected", C.c_int64), ("max_accepted", C.c_int64), ("min_accepted", C.c_int64)]
+
+
 # Exported C-ABI symbols with their ctypes signatures (argtypes, restype).
 SIGNATURES = {
     "sbe_create": ([C.POINTER(sbe_config), C.POINTER(sbe_model), C.c_int64, C.c_int64,
                     C.POINTER(C.c_void_p)], C.c_int),
     "sbe_run": ([C.c_void_p, C.c_int64, C.c_int64], C.c_int),
     "sbe_synchronize": ([C.c_void_p], C.c_int),
+    "sbe_is_idle": ([C.c_void_p], C.c_int),
     "sbe_get_series": ([C.c_void_p, c_double_p, c_double_p, c_double_p, c_double_p], C.c_int),
     "sbe_get_state": ([C.c_void_p, C.c_int32, c_double_p], C.c_int),
     "sbe_get_info": ([C.c_void_p, C.POINTER(sbe_info)], C.c_int),
@@ -105,6 +120,8 @@ SIGNATURES = {
     "sbe_laser_eval": ([C.POINTER(sbe_config), C.c_int64, c_double_p, c_double_p, c_double_p,
                         c_double_p, C.POINTER(C.c_int64)], C.c_int),
     "sbe_set_timing": ([C.c_void_p, C.c_int32], C.c_int),
+    "sbe_get_step_stats": ([C.c_void_p, C.POINTER(sbe_step_stats)], C.c_int),
+    "sbe_batch_get_step_stats": ([C.c_void_p, C.c_int32, C.POINTER(sbe_step_stats)], C.c_int),
     "sbe_get_timing": ([C.c_void_p, c_double_p, C.POINTER(C.c_int64)], C.c_int),
     "sbe_batch_create": ([C.c_int32, C.POINTER(sbe_config), C.POINTER(sbe_model), C.POINTER(C.c_void_p)], C.c_int),
     "sbe_batch_run": ([C.c_void_p, C.c_int64], C.c_int),
@@ -114,6 +131,17 @@ SIGNATURES = {
     "sbe_batch_get_timing": ([C.c_void_p, c_double_p, C.POINTER(C.c_int64)], C.c_int),
     "sbe_batch_last_error": ([C.c_void_p], C.c_char_p),
     "sbe_batch_destroy": ([C.c_void_p], None),
+    "sbe_server_create": ([C.c_size_t, C.c_int32, C.c_int64, C.POINTER(C.c_void_p)], C.c_int),
+    "sbe_server_submit": ([C.c_void_p, C.POINTER(sbe_config), C.POINTER(sbe_model), C.POINTER(C.c_int64)], C.c_int),
+    "sbe_server_step": ([C.c_void_p], C.c_int),
+    "sbe_server_poll": ([C.c_void_p, C.c_int64, C.POINTER(sbe_request_status)], C.c_int),
+    "sbe_server_counts": ([C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_size_t),
+                           C.POINTER(C.c_size_t)], C.c_int),
+    "sbe_server_get_series": ([C.c_void_p, C.c_int64, c_double_p, c_double_p, c_double_p, c_double_p], C.c_int),
+    "sbe_server_request_error": ([C.c_void_p, C.c_int64], C.c_char_p),
+    "sbe_server_release": ([C.c_void_p, C.c_int64], C.c_int),
+    "sbe_server_last_error": ([C.c_void_p], C.c_char_p),
+    "sbe_server_destroy": ([C.c_void_p], None),
     "sbe_destroy": ([C.c_void_p], None),
     "sbe_last_error": ([C.c_void_p], C.c_char_p),
 }

@@ -26,8 +26,8 @@ ARCH = os.environ.get("ANTELOPE_OFFLOAD_ARCH", "gfx950")
 
 # Library sources (kernels + C ABI). The driver links against the library.
 LIB_SOURCES = [
-    "host_setup.cpp", "sbe_api.cpp", "nband_host.cpp", "frontend.cpp",
-    "haldane.hip", "nband.hip", "series.hip",
+    "host_setup.cpp", "sbe_api.cpp", "nband_host.cpp", "frontend.cpp", "server.cpp",
+    "haldane.hip", "haldane_adaptive.hip", "nband.hip", "series.hip",
 ]
 DRIVER = "antelope_mi355x"
 

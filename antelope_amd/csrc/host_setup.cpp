@@ -10,53 +10,6 @@ namespace sbe {
 namespace {
 constexpr double kPi = 3.14159265358979323846;
 
-// Envelope f(t) and f'(t) of one pulse (reference laser.h:150-206). The
-// SBE_ENV_SIN2 envelope is cos^2(t w0/(2 ncycles)) on |t| <= ncycles T/2,
-// i.e. a sin^2 pulse of total length ncycles periods.
-void envelope(const PulseConst &p, double t, double &f, double &fd)
-{
-    const double ef = p.envfactor;
-    auto cos4 = [&](double x) {
-        const double c = std::cos(x * ef), s = std::sin(x * ef);
-        f = std::pow(c, 4.0);
-        fd = -4.0 * ef * s * std::pow(c, 3.0);
-    };
-    auto gauss = [&](double x) {
-        const double g = std::exp(ef * x * x);
-        f = g;
-        fd = 2.0 * x * ef * g;
-    };
-    f = 0.0;
-    fd = 0.0;
-    switch (p.env) {
-    case SBE_ENV_COS4:
-        if (std::fabs(t) <= p.twidth) cos4(t);
-        return;
-    case SBE_ENV_SIN2:
-        if (std::fabs(t) <= p.twidth) {
-            const double c = std::cos(t * ef), s = std::sin(t * ef);
-            f = c * c;
-            fd = -2.0 * ef * s * c;
-        }
-        return;
-    case SBE_ENV_COS4_CLIPPED:
-        if (std::fabs(t) > p.twidth) return;
-        if (t > p.twidth_clipped) cos4(t - p.twidth_clipped);
-        else if (t > -p.twidth_clipped) f = 1.0;
-        else cos4(t + p.twidth_clipped);
-        return;
-    case SBE_ENV_GAUSS:
-        gauss(t);
-        return;
-    case SBE_ENV_GAUSS_CLIPPED:
-        if (t > p.twidth_clipped) gauss(t - p.twidth_clipped);
-        else if (t > -p.twidth_clipped) f = 1.0;
-        else gauss(t + p.twidth_clipped);
-        return;
-    default:
-        return;
-    }
-}
 }  // namespace
 
 int Laser::init(const sbe_config &cfg, std::string &err)
@@ -149,19 +102,11 @@ void Laser::eval(double t, double A[3], double E[3]) const
 {
     double a[3] = {0, 0, 0}, e[3] = {0, 0, 0};
     for (const PulseConst &p : pulses) {
-        const double kt = t - p.t0;
-        const double ph = kt * p.w0 - p.cep;
-        const double sn = std::sin(ph), cs = std::cos(ph);
-        double f, fd;
-        envelope(p, kt, f, fd);
-        const double ax = -p.A0x * sn * f, ay = p.A0y * cs * f;
-        // laser.h:217-224 keeps +f' terms of opposite sign to -dA/dt
-        const double sgn = consistent ? -1.0 : 1.0;
-        const double ex = p.E0x * cs * f - sgn * p.A0x * sn * fd;
-        const double ey = p.E0y * sn * f + sgn * p.A0y * cs * fd;
+        double pa[3], pe[3];
+        pulse_field(p, t, consistent, pa, pe);
         for (int j = 0; j < 3; ++j) {
-            a[j] += ax * p.xdir[j] + ay * p.ydir[j];
-            e[j] += ex * p.xdir[j] + ey * p.ydir[j];
+            a[j] += pa[j];
+            e[j] += pe[j];
         }
     }
     for (int j = 0; j < 3; ++j) {

@@ -42,6 +42,7 @@ int hip_check(sbe_ctx *ctx, hipError_t e, const char *what)
 // of a batch use a smaller one).
 constexpr size_t kPartialBytes = size_t(128) << 20;
 constexpr size_t kBatchPartialBytes = size_t(24) << 20;
+constexpr size_t kAdaptiveBatchPartialBytes = size_t(96) << 20;  // longer windows: fewer shortened steps
 thread_local size_t t_partial_bytes = kPartialBytes;
 
 struct Plan {
@@ -52,8 +53,11 @@ struct Plan {
     int64_t chunk = 0;
     int64_t nt = 0;
     int nstage = 0;
-    size_t bytes_state = 0, bytes_kpts = 0, bytes_tab = 0, bytes_series = 0, bytes_partials = 0;
-    size_t total() const { return bytes_state + bytes_kpts + bytes_tab + bytes_series + bytes_partials; }
+    size_t bytes_state = 0, bytes_kpts = 0, bytes_tab = 0, bytes_series = 0, bytes_partials = 0, bytes_adaptive = 0;
+    size_t total() const
+    {
+        return bytes_state + bytes_kpts + bytes_tab + bytes_series + bytes_partials + bytes_adaptive;
+    }
 };
 
 int validate(const sbe_config *cfg, const sbe_model *m, std::string &err)
@@ -106,6 +110,7 @@ int make_plan(const sbe_config &cfg, const sbe_model &m, int64_t nk, int64_t nt,
     // dependency chains, which pays when there are few waves per SIMD
     p.kpt = 1;
     if (const char *e = std::getenv("ANTELOPE_HALDANE_KPT")) p.kpt = std::atoi(e) == 2 ? 2 : 1;
+    if (cfg.rk_order == SBE_RK_CASH_KARP) p.kpt = 1;  // the adaptive kernel takes one k point per lane
     const int64_t unit = static_cast<int64_t>(kBlock) * p.kpt;
     p.nk_stride = (nk + unit - 1) / unit * unit;
     p.nwaves = static_cast<int32_t>(p.nk_stride / (64 * p.kpt));
@@ -118,7 +123,12 @@ int make_plan(const sbe_config &cfg, const sbe_model &m, int64_t nk, int64_t nt,
     const int comps = cfg.precision == SBE_PREC_FP32 ? 8 : 4;
     p.bytes_state = static_cast<size_t>(p.nk_stride) * comps * real;
     p.bytes_kpts = static_cast<size_t>(p.nk_stride) * 4 * sizeof(double);
-    p.bytes_tab = static_cast<size_t>(nt) * nstage * 12 * (sizeof(double) + sizeof(float));
+    // the adaptive stepper reads table rows at the sample times only
+    const bool adaptive = cfg.rk_order == SBE_RK_CASH_KARP;
+    p.bytes_tab = static_cast<size_t>(nt) * (adaptive ? 1 : nstage) * 12 * (sizeof(double) + sizeof(float));
+    p.bytes_adaptive = adaptive ? static_cast<size_t>(p.nwaves) * (sizeof(double) + 2 * sizeof(int64_t)) + 256 +
+                                      sizeof(sbe::PulseConst) * sbe::kMaxDevicePulses
+                                : 0;
     p.bytes_series = static_cast<size_t>(nt) * p.nv * sizeof(double);
     p.bytes_partials = static_cast<size_t>(p.chunk) * row;
     (void)m;
@@ -294,9 +304,21 @@ int sbe_create(const sbe_config *cfg, const sbe_model *model, int64_t k_begin, i
         return rc;
     }
     if (ctx->tab.embedded) {
-        set_error(nullptr, SBE_ERR_UNSUPPORTED, "the adaptive Cash-Karp stepper runs through the batched solver");
-        delete ctx;
-        return SBE_ERR_UNSUPPORTED;
+        std::string why;
+        rc = SBE_ERR_UNSUPPORTED;
+        if (model->kind != SBE_MODEL_HALDANE) why = "the adaptive Cash-Karp stepper is built for the 2-band (Haldane) path";
+        else if (ctx->cfg.T1 > 1e-16) why = "T1 population decay is not built for the adaptive Cash-Karp stepper";
+        else if (ctx->cfg.npulses > sbe::kMaxDevicePulses)
+            why = "the adaptive Cash-Karp stepper takes at most " + std::to_string(sbe::kMaxDevicePulses) + " pulses";
+        else if (!(ctx->cfg.rtol > 0.0) || !(ctx->cfg.atol >= 0.0) || !std::isfinite(ctx->cfg.rtol) ||
+                 !std::isfinite(ctx->cfg.atol))
+            why = "calc.rtol must be > 0 and calc.atol >= 0", rc = SBE_ERR_CONFIG;
+        if (!why.empty()) {
+            set_error(nullptr, rc, why);
+            delete ctx;
+            return rc;
+        }
+        ctx->adaptive = true;
     }
 
     int ndev = 0;
@@ -334,6 +356,12 @@ int sbe_create(const sbe_config *cfg, const sbe_model *model, int64_t k_begin, i
     ctx->d_phase_tab = dalloc<double>(ctx, p.bytes_tab, rc);
     ctx->d_series = dalloc<double>(ctx, p.bytes_series, rc);
     ctx->d_partials = dalloc<double>(ctx, p.bytes_partials, rc);
+    if (ctx->adaptive) {
+        ctx->d_hstep = dalloc<double>(ctx, sizeof(double) * ctx->nwaves, rc);
+        ctx->d_counts = dalloc<int64_t>(ctx, 2 * sizeof(int64_t) * ctx->nwaves, rc);
+        ctx->d_fail = dalloc<int32_t>(ctx, sizeof(int32_t), rc);
+        ctx->d_pulses = dalloc<sbe::PulseConst>(ctx, sizeof(sbe::PulseConst) * sbe::kMaxDevicePulses, rc);
+    }
     if (rc) {
         set_error(nullptr, rc, ctx->err);
         sbe_destroy(ctx);
@@ -378,7 +406,7 @@ int sbe_create(const sbe_config *cfg, const sbe_model *model, int64_t k_begin, i
     h.state = ctx->d_state;
     h.partials = ctx->d_partials;
     {
-        const int S = ctx->tab.nstage;
+        const int S = ctx->adaptive ? 1 : ctx->tab.nstage;  // adaptive: rows at the sample times t_n
         const int64_t nt = ctx->laser.nt;
         const size_t n64 = static_cast<size_t>(nt) * S * 12;
         std::vector<double> tab(n64);
@@ -407,6 +435,31 @@ int sbe_create(const sbe_config *cfg, const sbe_model *model, int64_t k_begin, i
         h.phase_tab32 = d32;
     }
     if (!rc) rc = hip_check(ctx, hipMemsetAsync(ctx->d_series, 0, p.bytes_series, ctx->stream), "clear series");
+    if (ctx->adaptive && !rc) {
+        sbe::AdaptiveArgs &A = ctx->aargs;
+        A.h = h;
+        A.atmin = ctx->laser.atmin;
+        A.dt_out = ctx->laser.dt;
+        A.rtol = ctx->cfg.rtol;
+        A.atol = ctx->cfg.atol;
+        A.hmax = 32.0 * ctx->laser.dt;
+        A.hmin = 1e-6 * ctx->laser.dt;
+        A.npulses = static_cast<int32_t>(ctx->laser.pulses.size());
+        for (int j = 0; j < 3; ++j) A.A_at_min[j] = ctx->laser.A_at_min[j];
+        A.pulses = ctx->d_pulses;
+        A.hstep = ctx->d_hstep;
+        A.counts = ctx->d_counts;
+        A.fail = ctx->d_fail;
+        const std::vector<double> h0(static_cast<size_t>(ctx->nwaves), ctx->laser.dt);  // first proposal: dt
+        rc = hip_check(ctx, hipMemcpy(ctx->d_hstep, h0.data(), h0.size() * sizeof(double), hipMemcpyHostToDevice),
+                       "upload step sizes");
+        if (!rc)
+            rc = hip_check(ctx, hipMemcpy(ctx->d_pulses, ctx->laser.pulses.data(),
+                                          sizeof(sbe::PulseConst) * ctx->laser.pulses.size(), hipMemcpyHostToDevice),
+                           "upload pulses");
+        if (!rc) rc = hip_check(ctx, hipMemsetAsync(ctx->d_counts, 0, 2 * sizeof(int64_t) * ctx->nwaves, ctx->stream), "clear counts");
+        if (!rc) rc = hip_check(ctx, hipMemsetAsync(ctx->d_fail, 0, sizeof(int32_t), ctx->stream), "clear flag");
+    }
     if (!rc) rc = hip_check(ctx, sbe::launch_haldane_init(h, ctx->cfg.init_type, ctx->cfg.fermi_energy, ctx->stream), "init kernel");
     if (!rc) rc = hip_check(ctx, hipStreamSynchronize(ctx->stream), "init sync");
     if (rc) {
@@ -430,6 +483,24 @@ int sbe_run(sbe_ctx *ctx, int64_t step_begin, int64_t step_end)
     if (step_end > ctx->laser.nt || step_end < step_begin)
         return set_error(ctx, SBE_ERR_RANGE, "step_end outside [step_begin, Nt]");
     if (ctx->path == sbe::PathKind::NBand) return sbe::nb_run(ctx, step_begin, step_end);
+    if (ctx->adaptive) {
+        for (int64_t s = step_begin; s < step_end; s += ctx->chunk_steps) {
+            const int64_t n = std::min<int64_t>(ctx->chunk_steps, step_end - s);
+            sbe::AdaptiveArgs A = ctx->aargs;
+            A.out0 = s;
+            A.out1 = s + n;
+            A.max_iter = static_cast<int32_t>(std::min<int64_t>(INT32_MAX, 64 * n + 1024));
+            int rc = timed(ctx, 0, [&] { return sbe::launch_haldane_adaptive(A, ctx->stream); }, "adaptive step kernel");
+            if (!rc)
+                rc = timed(ctx, 1, [&] {
+                    return sbe::launch_series_reduce(ctx->d_partials, n, ctx->nwaves, ctx->nv,
+                                                     ctx->d_series + s * ctx->nv, ctx->stream);
+                }, "series kernel");
+            if (rc) return rc;
+            ctx->steps_done = s + n;
+        }
+        return SBE_OK;
+    }
     const bool t1_on = ctx->hargs.gam1 != 0.0;
     for (int64_t s = step_begin; s < step_end; s += ctx->chunk_steps) {
         const int64_t n = std::min<int64_t>(ctx->chunk_steps, step_end - s);
@@ -449,10 +520,47 @@ int sbe_run(sbe_ctx *ctx, int64_t step_begin, int64_t step_end)
     return SBE_OK;
 }
 
+int sbe_get_step_stats(sbe_ctx *ctx, sbe_step_stats *out)
+{
+    if (!ctx || !out) return set_error(ctx, SBE_ERR_STATE, "null argument");
+    std::memset(out, 0, sizeof *out);
+    if (ctx->path == sbe::PathKind::NBand) {
+        out->blocks = 1;
+        out->k_per_block = ctx->nk;
+        out->accepted = out->max_accepted = out->min_accepted = ctx->steps_done;
+        return SBE_OK;
+    }
+    out->blocks = ctx->nwaves;
+    out->k_per_block = 64;
+    if (!ctx->adaptive) {
+        out->accepted = ctx->steps_done * ctx->nwaves;
+        out->max_accepted = out->min_accepted = ctx->steps_done;
+        return SBE_OK;
+    }
+    std::vector<int64_t> c(2 * static_cast<size_t>(ctx->nwaves));
+    int rc = hip_check(ctx, hipMemcpyAsync(c.data(), ctx->d_counts, c.size() * sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream), "copy counts");
+    if (!rc) rc = hip_check(ctx, hipStreamSynchronize(ctx->stream), "sync counts");
+    if (rc) return rc;
+    out->min_accepted = INT64_MAX;
+    for (int32_t w = 0; w < ctx->nwaves; ++w) {
+        out->accepted += c[2 * w];
+        out->rejected += c[2 * w + 1];
+        out->max_accepted = std::max(out->max_accepted, c[2 * w]);
+        out->min_accepted = std::min(out->min_accepted, c[2 * w]);
+    }
+    return SBE_OK;
+}
+
 int sbe_synchronize(sbe_ctx *ctx)
 {
     if (!ctx) return set_error(nullptr, SBE_ERR_STATE, "null context");
     return hip_check(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+}
+
+int sbe_is_idle(const sbe_ctx *ctx)
+{
+    if (!ctx || !ctx->stream) return 1;
+    return hipStreamQuery(ctx->stream) == hipSuccess ? 1 : 0;
 }
 
 int sbe_get_series(sbe_ctx *ctx, double *j_inter, double *j_intra, double *rho_bar, double *band_pop)
@@ -462,8 +570,14 @@ int sbe_get_series(sbe_ctx *ctx, double *j_inter, double *j_intra, double *rho_b
     const int64_t nt = ctx->laser.nt;
     std::vector<double> s(static_cast<size_t>(nt) * ctx->nv);
     int rc = hip_check(ctx, hipMemcpyAsync(s.data(), ctx->d_series, s.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "copy series");
+    int32_t failed = 0;
+    if (!rc && ctx->adaptive)
+        rc = hip_check(ctx, hipMemcpyAsync(&failed, ctx->d_fail, sizeof failed, hipMemcpyDeviceToHost, ctx->stream), "copy flag");
     if (!rc) rc = hip_check(ctx, hipStreamSynchronize(ctx->stream), "sync series");
     if (rc) return rc;
+    if (failed)
+        return set_error(ctx, SBE_ERR_NAN, "adaptive stepper: a step size fell below 1e-6 dt or a window needed more "
+                                           "than 64 attempts per sample (non-finite state or rtol too tight)");
     // n(t_m) = n(0) R^m for every k when T1 is on (R: RK factor of dn/dt = -n/T1)
     const double R = ctx->hargs.gam1 != 0.0 ? rk_linear_factor(ctx->tab, -ctx->hargs.gam1 * ctx->laser.dt) : 1.0;
     double nsum = ctx->n_sum;
@@ -565,6 +679,8 @@ struct sbe_batch {
     int32_t *d_map = nullptr;  // [2][max_blocks]: block -> solve, solve -> first block
     int max_blocks = 0;
     int rk = 0, prec = 0;
+    bool adaptive = false;
+    sbe::AdaptiveArgs *d_ainsts = nullptr;
     bool t1 = false, deph = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
     std::string err;
@@ -593,6 +709,7 @@ void sbe_batch_destroy(sbe_batch *b)
     }
     for (sbe_ctx *c : b->ctx) sbe_destroy(c);
     if (b->d_insts) (void)hipFree(b->d_insts);
+    if (b->d_ainsts) (void)hipFree(b->d_ainsts);
     if (b->d_map) (void)hipFree(b->d_map);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
@@ -610,7 +727,8 @@ int sbe_batch_create(int32_t n, const sbe_config *cfgs, const sbe_model *models,
     }
     sbe_batch *b = new (std::nothrow) sbe_batch();
     if (!b) return batch_fail(nullptr, SBE_ERR_OOM, "host allocation failed");
-    t_partial_bytes = kBatchPartialBytes;
+    b->adaptive = cfgs[0].rk_order == SBE_RK_CASH_KARP;
+    t_partial_bytes = b->adaptive ? kAdaptiveBatchPartialBytes : kBatchPartialBytes;
     for (int i = 0; i < n; ++i) {
         sbe_ctx *c = nullptr;
         const int rc = sbe_create(&cfgs[i], &models[i], -1, -1, &c);
@@ -630,6 +748,7 @@ int sbe_batch_create(int32_t n, const sbe_config *cfgs, const sbe_model *models,
     b->prec = cfgs[0].precision;
     hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(&b->d_insts, sizeof(sbe::HaldaneArgs) * n);
+    if (e == hipSuccess && b->adaptive) e = hipMalloc(&b->d_ainsts, sizeof(sbe::AdaptiveArgs) * n);
     if (e == hipSuccess) e = hipMalloc(&b->d_map, sizeof(int32_t) * (b->max_blocks + n));
     if (e != hipSuccess) {
         const std::string m = std::string("batch set-up: ") + hipGetErrorString(e);
@@ -646,6 +765,7 @@ int sbe_batch_run(sbe_batch *b, int64_t max_steps)
     const int n = static_cast<int>(b->ctx.size());
     int64_t budget = max_steps > 0 ? max_steps : INT64_MAX;
     std::vector<sbe::HaldaneArgs> insts(n);
+    std::vector<sbe::AdaptiveArgs> ainsts(b->adaptive ? n : 0);
     std::vector<int32_t> map(b->max_blocks + n);
     while (budget > 0) {
         int64_t launch = INT64_MAX;
@@ -659,13 +779,21 @@ int sbe_batch_run(sbe_batch *b, int64_t max_steps)
             insts[i] = c->hargs;
             insts[i].step0 = c->steps_done;
             insts[i].nsteps = static_cast<int32_t>(std::min<int64_t>(launch, c->laser.nt - c->steps_done));
+            if (b->adaptive) {
+                ainsts[i] = c->aargs;
+                ainsts[i].out0 = c->steps_done;
+                ainsts[i].out1 = c->steps_done + insts[i].nsteps;
+                ainsts[i].max_iter = static_cast<int32_t>(64 * static_cast<int64_t>(insts[i].nsteps) + 1024);
+            }
             map[b->max_blocks + i] = nb;
             if (insts[i].nsteps > 0)
                 for (int64_t q = 0; q < c->nk_stride / kBlock; ++q) map[nb++] = i;
         }
         // solve -> first block table sits after the block -> solve table
         int32_t *d_first = b->d_map + b->max_blocks;
-        hipError_t e = hipMemcpyAsync(b->d_insts, insts.data(), sizeof(sbe::HaldaneArgs) * n, hipMemcpyHostToDevice, b->stream);
+        hipError_t e = b->adaptive
+                           ? hipMemcpyAsync(b->d_ainsts, ainsts.data(), sizeof(sbe::AdaptiveArgs) * n, hipMemcpyHostToDevice, b->stream)
+                           : hipMemcpyAsync(b->d_insts, insts.data(), sizeof(sbe::HaldaneArgs) * n, hipMemcpyHostToDevice, b->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(b->d_map, map.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, b->stream);
         if (e == hipSuccess)
             e = hipMemcpyAsync(d_first, map.data() + b->max_blocks, sizeof(int32_t) * n, hipMemcpyHostToDevice, b->stream);
@@ -674,7 +802,8 @@ int sbe_batch_run(sbe_batch *b, int64_t max_steps)
         if (e == hipSuccess) e = hipEventCreate(&eb);
         if (e == hipSuccess) e = hipEventRecord(ea, b->stream);
         if (e == hipSuccess)
-            e = sbe::launch_haldane_batch(b->d_insts, b->d_map, d_first, nb, b->rk, b->prec, b->t1, b->deph, b->stream);
+            e = b->adaptive ? sbe::launch_haldane_adaptive_batch(b->d_ainsts, b->d_map, d_first, nb, b->prec, b->deph, b->stream)
+                            : sbe::launch_haldane_batch(b->d_insts, b->d_map, d_first, nb, b->rk, b->prec, b->t1, b->deph, b->stream);
         if (e == hipSuccess) e = hipEventRecord(eb, b->stream);
         if (ea && eb) b->ev.push_back({ea, eb});
         for (int i = 0; i < n && e == hipSuccess; ++i) {
@@ -709,6 +838,16 @@ int sbe_batch_get_series(sbe_batch *b, int32_t i, double *j_inter, double *j_int
     int rc = sbe_batch_synchronize(b);
     if (rc) return rc;
     rc = sbe_get_series(b->ctx[i], j_inter, j_intra, rho_bar, band_pop);
+    if (rc) return batch_fail(b, rc, b->ctx[i]->err);
+    return SBE_OK;
+}
+
+int sbe_batch_get_step_stats(sbe_batch *b, int32_t i, sbe_step_stats *out)
+{
+    if (!b || i < 0 || i >= static_cast<int32_t>(b->ctx.size())) return batch_fail(nullptr, SBE_ERR_RANGE, "bad solve index");
+    int rc = sbe_batch_synchronize(b);
+    if (rc) return rc;
+    rc = sbe_get_step_stats(b->ctx[i], out);
     if (rc) return batch_fail(b, rc, b->ctx[i]->err);
     return SBE_OK;
 }

@@ -10,22 +10,15 @@
 #include <vector>
 
 #include "antelope/sbe_c.h"
+#include "laser_core.hpp"
 #include "nband.hpp"
 
 namespace sbe {
 
 // ---------------------------------------------------------------- laser --
-// One pulse with its derived constants (reference laser.h:68-141).
-struct PulseConst {
-    double E0x, E0y, A0x, A0y, w0, cep, t0;
-    double twidth, twidth_clipped, envfactor, period0;
-    double xdir[3], ydir[3];
-    int env;
-};
-
 // Sum of pulses minus its value at atmin (laser.h:249-322), FP64 on the host.
-// The kernels never evaluate the laser: they read per-stage tables built
-// from this class.
+// The fixed-step kernels read per-stage tables built from this class; the
+// adaptive stepper evaluates the same pulses (laser_core.hpp) on the device.
 struct Laser {
     std::vector<PulseConst> pulses;
     double atmin = 0, atmax = 0, dt = 0;
@@ -90,6 +83,25 @@ struct HaldaneArgs {
     double *partials;       // [nsteps][nwaves][kHaldaneNV]
 };
 
+// Kernel parameters of the adaptive Cash-Karp stepper (2-band path). Each
+// wave (64 k points) controls its own step size; samples are taken at the
+// fixed times t_n = atmin + n dt_out by the stepper's dense output. The
+// laser is evaluated on the device at the stage times.
+constexpr int kMaxDevicePulses = 8;
+struct AdaptiveArgs {
+    HaldaneArgs h;             // model, grid, state, partials (phase_tab: rows at t_n only)
+    double atmin, dt_out;
+    int64_t out0, out1;        // samples [out0, out1) of this launch; the state ends at t_out1
+    double rtol, atol, hmax, hmin;
+    int32_t max_iter;          // step attempts allowed per wave and launch
+    int32_t npulses;
+    double A_at_min[3];
+    const PulseConst *pulses;  // [npulses] in device memory
+    double *hstep;             // [nwaves] proposed step size carried between launches
+    int64_t *counts;           // [nwaves][2] accepted and rejected step attempts
+    int32_t *fail;             // set when a wave exhausts max_iter or its step falls below hmin
+};
+
 // ------------------------------------------------------------------ ctx --
 enum class PathKind { Haldane2 = 0, NBand = 1 };
 
@@ -129,6 +141,13 @@ struct sbe_ctx {
     double n_sum = 0;  // sum_k w_k n_k (constant when T1 is off)
 
     sbe::HaldaneArgs hargs{};
+    // adaptive (Cash-Karp) 2-band path
+    bool adaptive = false;
+    sbe::AdaptiveArgs aargs{};
+    sbe::PulseConst *d_pulses = nullptr;
+    double *d_hstep = nullptr;
+    int64_t *d_counts = nullptr;
+    int32_t *d_fail = nullptr;
     // N-band path
     sbe::NbArgs nargs{};
     double2 *nb_P = nullptr;     // e^{ik.R} [nk_stride][nr]
@@ -155,6 +174,10 @@ hipError_t launch_series_reduce(const double *partials, int64_t nsteps, int32_t 
 hipError_t launch_haldane_export(const HaldaneArgs &a, double *out, hipStream_t s);
 hipError_t launch_haldane_batch(const HaldaneArgs *d_insts, const int32_t *d_block_inst, const int32_t *d_block_first,
                                 int nblocks, int rk_order, int precision, bool t1_on, bool deph, hipStream_t s);
+hipError_t launch_haldane_adaptive(const AdaptiveArgs &a, hipStream_t s);
+hipError_t launch_haldane_adaptive_batch(const AdaptiveArgs *d_insts, const int32_t *d_block_inst,
+                                         const int32_t *d_block_first, int nblocks, int precision, bool deph,
+                                         hipStream_t s);
 int haldane_waves(int64_t nk);
 // N-band host side (nband_host.cpp)
 int nb_validate(const sbe_model &m, std::string &err);

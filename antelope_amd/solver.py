@@ -141,6 +141,12 @@ class SBESolver:
     def set_timing(self, enable: bool = True) -> None:
         _check(self._lib, self._ctx, self._lib.sbe_set_timing(self._ctx, int(enable)))
 
+    def step_stats(self) -> A.sbe_step_stats:
+        """Accepted / rejected steps per block of k points (sbe_get_step_stats)."""
+        st = A.sbe_step_stats()
+        _check(self._lib, self._ctx, self._lib.sbe_get_step_stats(self._ctx, C.byref(st)))
+        return st
+
     def timing(self) -> dict:
         """Summed HIP-event durations (ms) of the kernels since the last call."""
         ms = (C.c_double * 2)()
@@ -214,6 +220,11 @@ class SBEBatch:
         rb = out.pop("rho_bar")
         out["rho_bar"] = rb[..., 0] + 1j * rb[..., 1]
         return out
+
+    def step_stats(self, i: int) -> A.sbe_step_stats:
+        st = A.sbe_step_stats()
+        self._chk(self._lib.sbe_batch_get_step_stats(self._h, i, C.byref(st)))
+        return st
 
     def timing(self) -> dict:
         ms = C.c_double()

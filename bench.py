@@ -39,7 +39,7 @@ PEAK_HBM_GBS = 8000.0
 ALGO_BYTES_PER_KSTEP = {"fp64": 128.0, "fp32": 64.0}
 # Executed flops per k-point RK4 step of the step kernel, from the rocprofv3
 # counters SQ_INSTS_VALU_{FMA,MUL,ADD}_F64 (FMA = 2) on this workload (the
-# hand count of the algorithm gives the same figure; see DESIGN.md).
+# hand count of the algorithm gives the same figure).
 FLOPS_PER_KSTEP = {"fp64": 505.0, "fp32": 522.0}
 KERNEL_NAME = {"fp64": "haldane_steps_kernel<double, 4, false, true, 1>",
                "fp32": "haldane_steps_kernel<float, 4, false, true, 1>"}
@@ -134,7 +134,7 @@ def bench_wannier(args) -> int:
     steps = args.steps if args.steps > 0 else 50
     r = timed_solve(cfg, model, steps, min(args.warmup, 5))
     ks = r["nk"] * r["steps"] / r["wall_s"]
-    # algorithmic flops per k-point RK4 step (SURVEY 8(d)): 4 stages x (H + 3 D
+    # algorithmic flops per k-point RK4 step: 4 stages x (H + 3 D
     # Fourier sums, 8 N^2 N_R each, + H rho 8 N^3) + the sample's 3 J + H sums
     N, nr = 16, int(model.rvec.shape[0])
     fsum = 8 * N * N * nr
@@ -151,6 +151,45 @@ def bench_wannier(args) -> int:
     return 0
 
 
+def bench_serving(args) -> int:
+    """Continuous-batching request loop on a seeded Poisson stream (20 requests/s):
+    Haldane solves on 100^2..400^2 grids with 4..20-cycle pulses, 10% synthetic
+    Wannier solves (N in {8, 16, 32}) at reduced grid sizes. Reports aggregate
+    k-point RK steps/s over the whole stream and p50/p99 submit-to-done latency."""
+    from antelope_amd.serve import SBEServer, run_stream, serving_stream
+
+    n = args.requests
+    reqs = serving_stream(n, rate=20.0, seed=5, wannier_share=0.1)
+    # warm the kernels (first-launch code object loading) outside the stream
+    warm = serving_stream(2, rate=1e3, seed=11, wannier_share=0.5, grid_range=(16, 16), cycles=(1, 1),
+                          wannier_cycles=(1, 1))
+    with SBEServer(max_active=args.max_active, slice_steps=args.slice_steps) as srv:
+        run_stream(srv, warm, realtime=False)
+        out = run_stream(srv, reqs, realtime=True)
+    kinds = {}
+    for r in out["records"]:
+        kinds.setdefault(r["kind"], []).append(r["latency_s"])
+    print(f"[bench] serving: {out['completed']} done, {out['failed']} failed in {out['wall_s']:.2f} s, "
+          f"max {out['max_active']} active; " +
+          ", ".join(f"{k}: {len(v)} (p50 {np.percentile(v, 50):.3f} s)" for k, v in sorted(kinds.items())),
+          file=sys.stderr, flush=True)
+    line = {"metric": METRIC, "value": out["k_steps_per_s"], "unit": "k-point·RK-steps/s", "n_gpus": 1,
+            "steps": n, "warmup": len(warm), "ms_per_step": out["wall_s"] * 1e3 / n,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic (seeded SplitMix64 request stream, models and pulses)",
+            "config": {"workload": f"serving: {n} requests, Poisson 20/s, Haldane 100^2..400^2 grids, "
+                                   f"4..20 cycles, E0 log-uniform [0.001, 0.006]; 10% Wannier N in {{8,16,32}} "
+                                   f"on 8^2..16^2 grids, 2..4 cycles", "max_active": args.max_active,
+                       "slice_steps": args.slice_steps},
+            "latency_p50_s": out["p50_s"], "latency_p99_s": out["p99_s"],
+            "latency_by_kind": {k: {"n": len(v), "p50_s": float(np.percentile(v, 50)),
+                                    "p99_s": float(np.percentile(v, 99))} for k, v in sorted(kinds.items())},
+            "completed": out["completed"], "failed": out["failed"], "max_active_seen": out["max_active"],
+            "wall_s": out["wall_s"]}
+    print(json.dumps(line), flush=True)
+    return 0 if out["failed"] == 0 else 1
+
+
 def rel_l2(x, ref):
     d = np.linalg.norm(ref)
     return float(np.linalg.norm(x - ref) / d) if d > 0 else float(np.linalg.norm(x))
@@ -162,8 +201,11 @@ def main(argv=None) -> int:
     ap.add_argument("--steps", type=int, default=0, help="timed steps (default: the whole pulse)")
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--grid", type=int, default=300)
-    ap.add_argument("--workload", choices=["haldane", "wannier16"], default="haldane")
+    ap.add_argument("--workload", choices=["haldane", "wannier16", "serving"], default="haldane")
     ap.add_argument("--precision", choices=["both", "fp64", "fp32"], default="both")
+    ap.add_argument("--requests", type=int, default=100, help="serving: requests in the stream")
+    ap.add_argument("--max-active", type=int, default=16, help="serving: admitted requests at once")
+    ap.add_argument("--slice-steps", type=int, default=256, help="serving: steps per scheduling slice")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -182,11 +224,13 @@ def main(argv=None) -> int:
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(k blocks would shard across GPUs with one final sum; see DESIGN.md).", file=sys.stderr)
+              "(k blocks would shard across GPUs with one final sum of the k-summed series).", file=sys.stderr)
         return 2
 
     if args.workload == "wannier16":
         return bench_wannier(args)
+    if args.workload == "serving":
+        return bench_serving(args)
     forms = ["fp64", "fp32"] if args.precision == "both" else [args.precision]
     res = {}
     for p in forms:

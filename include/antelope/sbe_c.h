@@ -100,7 +100,11 @@ enum { SBE_PREC_FP64 = 0, SBE_PREC_FP32 = 1 };
 
 /* Time stepper: RKorder = 4 (classical RK4) or 5 (fixed-step 6-stage
  * Fehlberg weights) as in SBEs.h:395-416. SBE_RK_CASH_KARP is the adaptive
- * embedded 5(4) pair of Cash & Karp (no reference counterpart). */
+ * embedded 5(4) pair of Cash & Karp (no reference counterpart; 2-band
+ * models with T1 off): every block of 64 k points chooses its own steps
+ * under rtol/atol, and the series is still sampled at t_n = atmin + n dt,
+ * from a 4th-order dense output, so it has the layout of a fixed-step run.
+ * dt then only sets the sample spacing (and the first step size). */
 enum { SBE_RK4 = 4, SBE_RK5_FEHLBERG = 5, SBE_RK_CASH_KARP = 45 };
 
 /* Electric field convention. REFERENCE reproduces laser.h:217-224, whose
@@ -207,6 +211,9 @@ int sbe_create(const sbe_config *cfg, const sbe_model *model,
  * Asynchronous with respect to the host; sbe_synchronize waits. */
 int sbe_run(sbe_ctx *ctx, int64_t step_begin, int64_t step_end);
 int sbe_synchronize(sbe_ctx *ctx);
+/* 1 when no work of the context is pending on the GPU, else 0 (non-blocking;
+ * lets a scheduler overlap many contexts). */
+int sbe_is_idle(const sbe_ctx *ctx);
 
 /* Copy the per-step k-sums of this context's k block to host buffers.
  * Each pointer may be NULL. Values are sum_k w_k x (...) before any dV or
@@ -254,6 +261,20 @@ int sbe_batch_get_series(sbe_batch *b, int32_t i, double *j_inter, double *j_int
 int sbe_batch_get_timing(sbe_batch *b, double *ms, int64_t *launches);
 const char *sbe_batch_last_error(const sbe_batch *b);
 void sbe_batch_destroy(sbe_batch *b);
+
+/* Step statistics (no reference counterpart). The 2-band path advances its
+ * k points in blocks of k_per_block (one wave each); with the adaptive
+ * Cash-Karp stepper every block chooses its own steps. accepted / rejected
+ * sum the step attempts over the blocks; max_accepted / min_accepted are the
+ * extremes of one block. Fixed-step solves report steps_done per block and
+ * no rejections. */
+typedef struct sbe_step_stats {
+    int64_t blocks, k_per_block;
+    int64_t accepted, rejected;
+    int64_t max_accepted, min_accepted;
+} sbe_step_stats;
+int sbe_get_step_stats(sbe_ctx *ctx, sbe_step_stats *out);
+int sbe_batch_get_step_stats(sbe_batch *b, int32_t i, sbe_step_stats *out);
 
 /* Measurement (no reference counterpart): when enabled, HIP events are
  * recorded on the context's stream around every kernel sbe_run launches.

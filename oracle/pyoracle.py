@@ -35,6 +35,10 @@ _SIGS = {
     "orc_rhs": ([C.POINTER(A.sbe_config), C.POINTER(A.sbe_model), _D, C.c_double, _D, _D], C.c_int),
     "orc_run": ([C.POINTER(A.sbe_config), C.POINTER(A.sbe_model), C.c_int64, C.c_int64, C.c_int64, C.c_int,
                  _D, _D, _D, _D, _D, C.POINTER(orc_info)], C.c_int),
+    "orc_run_adaptive": ([C.POINTER(A.sbe_config), C.POINTER(A.sbe_model), C.c_int64, C.c_int64, C.c_int64,
+                          C.c_int64, C.c_int, _D, _D, _D, _D, _D, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                          C.POINTER(orc_info)], C.c_int),
+    "orc_cash_karp": ([_D, _D, _D, _D, _D], C.c_int),
     "orc_last_error": ([], C.c_char_p),
 }
 
@@ -143,3 +147,35 @@ def run(config, model, k_begin=-1, k_end=-1, nsteps=-1, nthreads=1, want_state=F
     if want_state:
         out["state"] = st[..., 0] + 1j * st[..., 1]
     return out
+
+
+def run_adaptive(config, model, k_begin=-1, k_end=-1, block=64, window=-1, nthreads=1, want_state=False):
+    """Adaptive Cash-Karp restatement (config.rk_order must be the Cash-Karp pair)."""
+    cfg, k1 = config.to_c()
+    mdl, k2 = model.to_c()
+    N = model.nband
+    _, _, _, _, nt = laser_eval(config, np.zeros(1))
+    ntot = int(np.prod(list(config.npoints)))
+    nk = ntot if k_begin < 0 else k_end - k_begin
+    nblk = (nk + block - 1) // block
+    ji, ja = np.zeros((nt, 3)), np.zeros((nt, 3))
+    rb, bp = np.zeros((nt, N, N, 2)), np.zeros((nt, N))
+    st = np.zeros((nk, N, N, 2)) if want_state else None
+    acc, rej = np.zeros(nblk, np.int64), np.zeros(nblk, np.int64)
+    info = orc_info()
+    _chk(lib().orc_run_adaptive(C.byref(cfg), C.byref(mdl), k_begin, k_end, block, window, nthreads,
+                                _p(ji), _p(ja), _p(rb), _p(bp), _p(st),
+                                acc.ctypes.data_as(C.POINTER(C.c_int64)), rej.ctypes.data_as(C.POINTER(C.c_int64)),
+                                C.byref(info)))
+    out = {"j_inter": ji, "j_intra": ja, "rho_bar": rb[..., 0] + 1j * rb[..., 1], "band_pop": bp, "info": info,
+           "accepted": acc, "rejected": rej}
+    if want_state:
+        out["state"] = st[..., 0] + 1j * st[..., 1]
+    return out
+
+
+def cash_karp():
+    """(c, a, b, bhat, beta) exactly as the restatement uses them."""
+    c, a, b, bh, beta = np.zeros(6), np.zeros((6, 5)), np.zeros(6), np.zeros(6), np.zeros((7, 4))
+    _chk(lib().orc_cash_karp(_p(c), _p(a), _p(b), _p(bh), _p(beta)))
+    return c, a, b, bh, beta

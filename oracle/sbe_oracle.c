@@ -652,6 +652,8 @@ static int solver_init(const sbe_config *cfg, const sbe_model *model, osolver *S
         S->a[3][0] = 439. / 216.; S->a[3][1] = -8.; S->a[3][2] = 3680. / 513.; S->a[3][3] = -845. / 4104.;
         S->a[4][0] = -8. / 27.; S->a[4][1] = 2.; S->a[4][2] = -3544. / 2565; S->a[4][3] = 1859. / 4104;
         S->a[4][4] = -11. / 40.;
+    } else if (cfg->rk_order == SBE_RK_CASH_KARP) {
+        S->nstage = 6; /* coefficients live in orc_run_adaptive */
     } else {
         model_free(&S->om); free(S->L.p);
         return fail(SBE_ERR_UNSUPPORTED, "oracle restates RKorder 4 and 5 (got %d)", cfg->rk_order);
@@ -866,6 +868,10 @@ int orc_run(const sbe_config *cfg, const sbe_model *model, int64_t k_begin, int6
     int rc = solver_init(cfg, model, &S);
     if (rc) return rc;
     int N = S.N, nn = N * N;
+    if (cfg->rk_order == SBE_RK_CASH_KARP) {
+        solver_free(&S);
+        return fail(SBE_ERR_UNSUPPORTED, "the adaptive stepper is restated by orc_run_adaptive");
+    }
     if (N > 256) { solver_free(&S); return fail(SBE_ERR_MODEL, "N too large"); }
     int64_t ntot = (int64_t)cfg->npoints[0] * cfg->npoints[1] * cfg->npoints[2];
     if (k_begin < 0) { k_begin = 0; k_end = ntot; }
@@ -936,4 +942,285 @@ int orc_run(const sbe_config *cfg, const sbe_model *model, int64_t k_begin, int6
     free(th); free(tid); free(rho); free(kp); free(wk);
     solver_free(&S);
     return SBE_OK;
+}
+
+/* ------------------------------------------------------ adaptive stepper */
+/* Restatement of the build's adaptive Cash-Karp 5(4) stepper (no reference
+ * counterpart; the reference steps with a fixed dt, SBEs.h:596-624). Same
+ * model, right-hand side and samples as orc_run; the step sequence is:
+ *  - blocks of `block` consecutive k points share one step size; a step is
+ *    accepted when err = max over the block's k points and the Bloch
+ *    components s_x = 2 Re rho01, s_y = -2 Im rho01, s_z = rho00 - rho11 of
+ *    |y5 - y4| / (atol + rtol max(|y|, |y5|)) is <= 1, and the next size is
+ *    h min(5, max(0.2, 0.9 err^-1/5)) (after a rejection max(0.2, ...));
+ *  - step sizes start at dt and are capped at 32 dt; a step that would end
+ *    within 1e-4 of it before a window end is stretched to end there; each
+ *    window of `window` samples (the last one shorter) ends exactly on its
+ *    sample time, and a shortened last step leaves the proposal unchanged;
+ *  - samples t_n = atmin + n dt inside an accepted step come from the 4th-
+ *    order continuous extension y + h sum_i b_i(th) f_i over the six stages
+ *    and f(t + h, y5).
+ * Cash-Karp coefficients: J.R. Cash, A.H. Karp, ACM TOMS 16 (1990) 201.
+ * The dense-output weights b_i(th) were derived for this build (order-4
+ * conditions for all th, b(1) = b, b'(0) = e_1, b'(1) = e_7); tests check the
+ * order conditions from these numbers directly. */
+static const double ck_c[6] = {0.0, 0.2, 0.3, 0.6, 1.0, 0.875};
+static const double ck_a[6][5] = {
+    {0, 0, 0, 0, 0},
+    {0.2, 0, 0, 0, 0},
+    {3. / 40., 9. / 40., 0, 0, 0},
+    {0.3, -0.9, 1.2, 0, 0},
+    {-11. / 54., 2.5, -70. / 27., 35. / 27., 0},
+    {1631. / 55296., 175. / 512., 575. / 13824., 44275. / 110592., 253. / 4096.}};
+static const double ck_b[6] = {37. / 378., 0, 250. / 621., 125. / 594., 0, 512. / 1771.};
+static const double ck_bhat[6] = {2825. / 27648., 0, 18575. / 48384., 13525. / 55296., 277. / 14336., 0.25};
+static const double ck_beta[7][4] = {
+    {1.0, -65. / 21., 677. / 189., -25. / 18.},
+    {0, 0, 0, 0},
+    {0, 2500. / 483., -38000. / 4347., 250. / 63.},
+    {0, -125. / 44., 3875. / 594., -125. / 36.},
+    {0, -45. / 28., 45. / 14., -45. / 28.},
+    {0, 1536. / 1771., -1024. / 1771., 0},
+    {0, 1.5, -4.0, 2.5}};
+
+int orc_cash_karp(double c[6], double a[6][5], double b[6], double bhat[6], double beta[7][4])
+{
+    if (c) memcpy(c, ck_c, sizeof ck_c);
+    if (a) memcpy(a, ck_a, sizeof ck_a);
+    if (b) memcpy(b, ck_b, sizeof ck_b);
+    if (bhat) memcpy(bhat, ck_bhat, sizeof ck_bhat);
+    if (beta) memcpy(beta, ck_beta, sizeof ck_beta);
+    return SBE_OK;
+}
+
+static void bloch(const cplx *r, double s[3])
+{
+    s[0] = 2.0 * creal(r[1]);
+    s[1] = -2.0 * cimag(r[1]);
+    s[2] = creal(r[0]) - creal(r[3]);
+}
+
+typedef struct {
+    const osolver *S;
+    const double *kp, *wk;
+    int64_t b0, b1, nk, block, nsteps, window;
+    double *ji, *ja, *rb, *bp;
+    cplx *rho;
+    int64_t *acc, *rej;
+    int failed;
+} oathread;
+
+static void *adaptive_main(void *arg)
+{
+    oathread *T = (oathread *)arg;
+    const osolver *S = T->S;
+    const int N = 2, nn = 4;
+    const double dt = S->L.dt, atmin = S->L.atmin;
+    const double rtol = S->cfg->rtol, atol = S->cfg->atol, hmax = 32.0 * dt, hmin = 1e-6 * dt;
+    oscratch w;
+    scratch_alloc(&w, N);
+    const int64_t B = T->block;
+    cplx *st = calloc((size_t)B * 7 * nn, sizeof(cplx)); /* stages f_1..f_7 per k */
+    cplx *y5 = calloc((size_t)B * nn, sizeof(cplx)), *tmp = calloc((size_t)nn, sizeof(cplx));
+    double inter[3], intra[3], pop[2];
+    for (int64_t blk = T->b0; blk < T->b1; ++blk) {
+        const int64_t k0 = blk * B, k1 = k0 + B < T->nk ? k0 + B : T->nk, nb = k1 - k0;
+        double h = dt;
+        int64_t nacc = 0, nrej = 0;
+        /* the currents of sample n from state r (antelope.cpp:307-333) */
+#define ORC_SAMPLE(n, kk, r)                                                             \
+        do {                                                                             \
+            const double *kq = T->kp + 3 * (kk);                                         \
+            const double wq = T->wk[kk];                                                 \
+            currents(S, kq, atmin + dt * (double)(n), (r), inter, intra, pop, &w);      \
+            for (int ax = 0; ax < 3; ++ax) {                                             \
+                T->ja[3 * (n) + ax] += intra[ax] * wq;                                   \
+                T->ji[3 * (n) + ax] += inter[ax] * wq;                                   \
+            }                                                                            \
+            for (int i = 0; i < nn; ++i) {                                               \
+                T->rb[((n) * nn + i) * 2 + 0] += creal((r)[i]) * wq;                     \
+                T->rb[((n) * nn + i) * 2 + 1] += cimag((r)[i]) * wq;                     \
+            }                                                                            \
+            for (int q = 0; q < N; ++q) T->bp[(n) * N + q] += pop[q] * wq;              \
+        } while (0)
+        for (int64_t w0 = 0; w0 < T->nsteps && !T->failed; w0 += T->window) {
+            const int64_t w1 = w0 + T->window < T->nsteps ? w0 + T->window : T->nsteps;
+            double t = atmin + dt * (double)w0;
+            const double t_end = atmin + dt * (double)w1;
+            int64_t m = w0;
+            for (int64_t k = k0; k < k1; ++k) {
+                const cplx *r = T->rho + (size_t)k * nn;
+                ORC_SAMPLE(m, k, r);
+                rhs(S, T->kp + 3 * k, t, r, st + ((k - k0) * 7 + 0) * nn, &w);
+            }
+            ++m;
+            int64_t iter = 0;
+            while (t < t_end) {
+                if (++iter > 64 * (w1 - w0) + 1024 || !(h >= hmin)) { T->failed = 1; break; }
+                double hh = h < hmax ? h : hmax;
+                const int last = t + 1.0001 * hh >= t_end;
+                if (last) hh = t_end - t;
+                double err = 0.0;
+                for (int64_t k = k0; k < k1; ++k) {
+                    const cplx *r = T->rho + (size_t)k * nn;
+                    cplx *f = st + (k - k0) * 7 * nn;
+                    for (int i = 1; i < 6; ++i) {
+                        for (int e = 0; e < nn; ++e) {
+                            tmp[e] = r[e];
+                            for (int j = 0; j < i; ++j) tmp[e] += hh * ck_a[i][j] * f[j * nn + e];
+                        }
+                        rhs(S, T->kp + 3 * k, t + ck_c[i] * hh, tmp, f + i * nn, &w);
+                    }
+                    cplx *o = y5 + (k - k0) * nn, ev[4];
+                    for (int e = 0; e < nn; ++e) {
+                        o[e] = r[e];
+                        ev[e] = 0;
+                        for (int i = 0; i < 6; ++i) {
+                            o[e] += hh * ck_b[i] * f[i * nn + e];
+                            ev[e] += hh * (ck_b[i] - ck_bhat[i]) * f[i * nn + e];
+                        }
+                    }
+                    double sy[3], s5[3], se[3];
+                    bloch(r, sy);
+                    bloch(o, s5);
+                    bloch(ev, se);
+                    for (int c = 0; c < 3; ++c) {
+                        const double sc = atol + rtol * fmax(fabs(sy[c]), fabs(s5[c]));
+                        const double q = se[c] == 0.0 ? 0.0 : fabs(se[c]) / sc;
+                        err = q > err || isnan(q) ? q : err;
+                    }
+                }
+                if (err <= 1.0) {
+                    const double tn = last ? t_end : t + hh;
+                    for (int64_t k = k0; k < k1; ++k) {
+                        cplx *f = st + (k - k0) * 7 * nn;
+                        rhs(S, T->kp + 3 * k, tn, y5 + (k - k0) * nn, f + 6 * nn, &w);
+                    }
+                    for (; m < w1; ++m) {
+                        const double tm = atmin + dt * (double)m;
+                        if (tm > tn) break;
+                        const double th = (tm - t) / hh;
+                        double bt[7];
+                        for (int i = 0; i < 7; ++i)
+                            bt[i] = th * (ck_beta[i][0] + th * (ck_beta[i][1] + th * (ck_beta[i][2] + th * ck_beta[i][3])));
+                        for (int64_t k = k0; k < k1; ++k) {
+                            const cplx *r = T->rho + (size_t)k * nn, *f = st + (k - k0) * 7 * nn;
+                            for (int e = 0; e < nn; ++e) {
+                                tmp[e] = r[e];
+                                for (int i = 0; i < 7; ++i) tmp[e] += hh * bt[i] * f[i * nn + e];
+                            }
+                            ORC_SAMPLE(m, k, tmp);
+                        }
+                    }
+                    for (int64_t k = k0; k < k1; ++k) {
+                        memcpy(T->rho + (size_t)k * nn, y5 + (k - k0) * nn, sizeof(cplx) * nn);
+                        cplx *f = st + (k - k0) * 7 * nn;
+                        memcpy(f, f + 6 * nn, sizeof(cplx) * nn); /* f(t+h, y5) is the next first stage */
+                    }
+                    t = tn;
+                    ++nacc;
+                    const double fac = err > 0.0 ? fmin(5.0, fmax(0.2, 0.9 * pow(err, -0.2))) : 5.0;
+                    if (!last) h = hh * fac;
+                } else {
+                    h = hh * fmax(0.2, 0.9 * pow(err, -0.2));
+                    ++nrej;
+                }
+            }
+        }
+#undef ORC_SAMPLE
+        (void)nb;
+        if (T->acc) T->acc[blk] = nacc;
+        if (T->rej) T->rej[blk] = nrej;
+    }
+    free(st); free(y5); free(tmp);
+    scratch_free(&w);
+    return NULL;
+}
+
+int orc_run_adaptive(const sbe_config *cfg, const sbe_model *model, int64_t k_begin, int64_t k_end,
+                     int64_t block, int64_t window, int nthreads, double *j_inter, double *j_intra,
+                     double *rho_bar, double *band_pop, double *rho_final, int64_t *accepted,
+                     int64_t *rejected, orc_info *info)
+{
+    if (cfg->rk_order != SBE_RK_CASH_KARP) return fail(SBE_ERR_CONFIG, "orc_run_adaptive needs SBE_RK_CASH_KARP");
+    if (!(cfg->rtol > 0.0) || !(cfg->atol >= 0.0)) return fail(SBE_ERR_CONFIG, "rtol must be > 0, atol >= 0");
+    if (cfg->T1 > 1e-16) return fail(SBE_ERR_UNSUPPORTED, "T1 is not restated for the adaptive stepper");
+    osolver S;
+    int rc = solver_init(cfg, model, &S);
+    if (rc) return rc;
+    const int N = S.N, nn = N * N;
+    if (N != 2) { solver_free(&S); return fail(SBE_ERR_UNSUPPORTED, "the adaptive stepper is restated for N = 2"); }
+    int64_t ntot = (int64_t)cfg->npoints[0] * cfg->npoints[1] * cfg->npoints[2];
+    if (k_begin < 0) { k_begin = 0; k_end = ntot; }
+    int64_t nk = k_end - k_begin;
+    if (nk <= 0 || k_end > ntot) { solver_free(&S); return fail(SBE_ERR_RANGE, "bad k range"); }
+    if (block < 1) block = 64;
+    const int64_t nt = S.L.nt;
+    if (window < 1 || window > nt) window = nt;
+    double *kp = malloc(sizeof(double) * 3 * nk), *wk = malloc(sizeof(double) * nk);
+    double dV, tw;
+    rc = orc_kgrid(cfg, model, k_begin, k_end, kp, wk, &dV, &tw);
+    if (rc) { free(kp); free(wk); solver_free(&S); return rc; }
+    cplx *rho = calloc((size_t)nk * nn, sizeof(cplx));
+    {
+        oscratch w;
+        scratch_alloc(&w, N);
+        for (int64_t k = 0; k < nk; ++k) init_state(&S, kp + 3 * k, rho + (size_t)k * nn, &w);
+        scratch_free(&w);
+    }
+    const int64_t nblk = (nk + block - 1) / block;
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > nblk) nthreads = (int)nblk;
+    oathread *th = calloc((size_t)nthreads, sizeof(oathread));
+    pthread_t *tid = calloc((size_t)nthreads, sizeof(pthread_t));
+    for (int r = 0; r < nthreads; ++r) {
+        int64_t w1 = nblk / nthreads, w2 = nblk % nthreads;
+        int64_t s = w1 * r + (r < w2 ? r : w2), e = s + w1 + (w2 > r ? 1 : 0);
+        th[r].S = &S; th[r].kp = kp; th[r].wk = wk; th[r].b0 = s; th[r].b1 = e; th[r].nk = nk;
+        th[r].block = block; th[r].nsteps = nt; th[r].window = window; th[r].rho = rho;
+        th[r].acc = accepted; th[r].rej = rejected;
+        th[r].ji = calloc((size_t)nt * 3, sizeof(double));
+        th[r].ja = calloc((size_t)nt * 3, sizeof(double));
+        th[r].rb = calloc((size_t)nt * nn * 2, sizeof(double));
+        th[r].bp = calloc((size_t)nt * N, sizeof(double));
+    }
+    struct timespec t0, t1;
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+    for (int r = 1; r < nthreads; ++r) pthread_create(&tid[r], NULL, adaptive_main, &th[r]);
+    adaptive_main(&th[0]);
+    for (int r = 1; r < nthreads; ++r) pthread_join(tid[r], NULL);
+    clock_gettime(CLOCK_MONOTONIC, &t1);
+    int failed = 0;
+    for (int r = 0; r < nthreads; ++r) failed |= th[r].failed;
+    for (int64_t n = 0; n < nt; ++n) {
+        for (int ax = 0; ax < 3; ++ax) {
+            double si = 0, sa = 0;
+            for (int r = 0; r < nthreads; ++r) { si += th[r].ji[3 * n + ax]; sa += th[r].ja[3 * n + ax]; }
+            if (j_inter) j_inter[3 * n + ax] = si;
+            if (j_intra) j_intra[3 * n + ax] = sa;
+        }
+        if (rho_bar)
+            for (int i = 0; i < nn * 2; ++i) {
+                double s = 0;
+                for (int r = 0; r < nthreads; ++r) s += th[r].rb[n * nn * 2 + i];
+                rho_bar[n * nn * 2 + i] = s;
+            }
+        if (band_pop)
+            for (int m = 0; m < N; ++m) {
+                double s = 0;
+                for (int r = 0; r < nthreads; ++r) s += th[r].bp[n * N + m];
+                band_pop[n * N + m] = s;
+            }
+    }
+    if (rho_final) memcpy(rho_final, rho, sizeof(cplx) * (size_t)nk * nn);
+    if (info) {
+        info->nt = nt; info->nk_total = ntot; info->nk_local = nk;
+        info->atmin = S.L.atmin; info->atmax = S.L.atmax; info->dt = S.L.dt;
+        info->dV = dV; info->total_weight = tw;
+        info->seconds = (t1.tv_sec - t0.tv_sec) + 1e-9 * (t1.tv_nsec - t0.tv_nsec);
+    }
+    for (int r = 0; r < nthreads; ++r) { free(th[r].ji); free(th[r].ja); free(th[r].rb); free(th[r].bp); }
+    free(th); free(tid); free(rho); free(kp); free(wk);
+    solver_free(&S);
+    return failed ? fail(SBE_ERR_NAN, "adaptive step size underflow") : SBE_OK;
 }

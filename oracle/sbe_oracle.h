@@ -66,6 +66,18 @@ int orc_run(const sbe_config *cfg, const sbe_model *model, int64_t k_begin,
             double *j_intra, double *rho_bar, double *band_pop,
             double *rho_final, orc_info *info);
 
+/* Adaptive Cash-Karp 5(4) restatement (see sbe_oracle.c): blocks of `block`
+ * consecutive k points (<= 0: 64) choose their own steps; windows of
+ * `window` samples (<= 0: all) end on their sample time. 2-band models, T1
+ * off. accepted / rejected [nblocks] may be NULL. */
+int orc_run_adaptive(const sbe_config *cfg, const sbe_model *model, int64_t k_begin,
+                     int64_t k_end, int64_t block, int64_t window, int nthreads,
+                     double *j_inter, double *j_intra, double *rho_bar, double *band_pop,
+                     double *rho_final, int64_t *accepted, int64_t *rejected,
+                     orc_info *info);
+/* The Cash-Karp coefficients and dense-output weights the restatement uses. */
+int orc_cash_karp(double c[6], double a[6][5], double b[6], double bhat[6], double beta[7][4]);
+
 /* Last error message of the calling thread. */
 const char *orc_last_error(void);
 

@@ -11,6 +11,7 @@ from pathlib import Path
 import pytest
 
 from antelope_amd import _abi as A
+from antelope_amd.models import synthetic_wannier
 from antelope_amd import HaldaneModel, Pulse, RunConfig, SBESolver, laser_eval, query_footprint
 from antelope_amd.solver import SBEError, library_path, load_library
 from tests.cases import haldane_cfg1_model, small_config
@@ -43,8 +44,11 @@ def test_struct_layout_matches_header(tmp_path):
         "sbe_config": [f for f, _ in A.sbe_config._fields_],
         "sbe_model": [f for f, _ in A.sbe_model._fields_],
         "sbe_info": [f for f, _ in A.sbe_info._fields_],
+        "sbe_request_status": [f for f, _ in A.sbe_request_status._fields_],
+        "sbe_step_stats": [f for f, _ in A.sbe_step_stats._fields_],
     }
-    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "antelope/sbe_c.h"', "int main(void){"]
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "antelope/sbe_c.h"',
+             '#include "antelope/sbe_server.h"', "int main(void){"]
     for s, fs in fields.items():
         lines.append(f'printf("{s} %zu\\n", sizeof({s}));')
         for f in fs:
@@ -79,6 +83,10 @@ def test_invalid_inputs_return_status_codes():
     _expect(A.SBE_ERR_MODEL, lambda: SBESolver(cfg, dataclasses.replace(m, a0=-1.0)))
     _expect(A.SBE_ERR_RANGE, lambda: SBESolver(cfg, m, 0, 10_000))
     _expect(A.SBE_ERR_RANGE, lambda: SBESolver(cfg, m, 50, 40))
+    ck = dataclasses.replace(cfg, rk_order=45)
+    _expect(A.SBE_ERR_UNSUPPORTED, lambda: SBESolver(dataclasses.replace(ck, T1=100.0), m))
+    _expect(A.SBE_ERR_CONFIG, lambda: SBESolver(dataclasses.replace(ck, rtol=0.0), m))
+    _expect(A.SBE_ERR_UNSUPPORTED, lambda: SBESolver(ck, synthetic_wannier(4, nmax=(1, 1, 0))))
 
 
 def test_laser_eval_and_footprint_without_gpu():
@@ -89,3 +97,31 @@ def test_laser_eval_and_footprint_without_gpu():
     assert Ev[0, 0] == pytest.approx(0.0045 / (1.25 ** 0.5))
     b = query_footprint(cfg, m)
     assert 90_000 * 32 < b < 512 * 2 ** 20
+
+
+def test_server_bookkeeping_without_gpu():
+    """Submission validates and prices a request on the host; nothing runs until a round."""
+    import math
+
+    from antelope_amd import HaldaneModel, Pulse, RunConfig
+    from antelope_amd.serve import SBEServer
+    from antelope_amd.solver import query_footprint
+
+    t1 = 0.075
+    model = HaldaneModel(a0=2.683, t1=t1, t2=t1 / 3, phi0=0.3 * math.pi, M0=2.54 * t1 / 3)
+    cfg = RunConfig(npoints=[40, 40], pulses=[Pulse(E0=0.0045, w0=0.014, ncycles=2.0)], dt=0.25, rk_order=4)
+    fp = query_footprint(cfg, model)
+    srv = SBEServer(hbm_budget_bytes=3 * fp, max_active=2, slice_steps=100)
+    a, b = srv.submit(cfg, model), srv.submit(cfg, model)
+    assert b == a + 1
+    st = srv.poll(a)
+    assert st.state == A.SBE_REQ_QUEUED and st.footprint == fp and st.t_submit > 0
+    assert srv.counts() == (2, 0, 0, 3 * fp)
+    _expect(A.SBE_ERR_STATE, lambda: srv.release(a))        # still queued
+    _expect(A.SBE_ERR_RANGE, lambda: srv.poll(12345))
+    small = SBEServer(hbm_budget_bytes=fp - 1)
+    _expect(A.SBE_ERR_OOM, lambda: small.submit(cfg, model))  # can never fit
+    bad = RunConfig(npoints=[40, 40], pulses=[Pulse(E0=0.0045, w0=0.014, ncycles=2.0)], dt=-1.0)
+    _expect(A.SBE_ERR_CONFIG, lambda: srv.submit(bad, model))
+    small.close()
+    srv.close()
