@@ -12,7 +12,8 @@ from .params import HaldaneModel, Pulse, RunConfig
 
 
 def haldane_batch(n: int = 64, grid: int = 200, seed: int = 7, dt: float = 0.25, rk_order=4,
-                  precision: str = "fp64", T2: float = 220.0, cycles=(6, 16)):
+                  precision: str = "fp64", T2: float = 220.0, cycles=(6, 16), rtol: float = 1e-8,
+                  atol: float = 1e-8):
     rng = SplitMix64(seed)
     cfgs, models = [], []
     t1 = 0.075
@@ -26,5 +27,6 @@ def haldane_batch(n: int = 64, grid: int = 200, seed: int = 7, dt: float = 0.25,
         models.append(HaldaneModel(a0=2.683, t1=t1, t2=t1 / 3, phi0=phi0, M0=mt2 * t1 / 3))
         cfgs.append(RunConfig(npoints=[grid, grid], pulses=[Pulse(E0=E0, w0=0.014, ncycles=float(ncyc), ellip=ellip,
                                                                    cep=cep, envelope="sin2")],
-                              dt=dt, T2=T2, rk_order=rk_order, inter_intra=True, precision=precision))
+                              dt=dt, T2=T2, rk_order=rk_order, inter_intra=True, precision=precision,
+                              rtol=rtol, atol=atol))
     return cfgs, models

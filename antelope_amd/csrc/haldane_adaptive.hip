@@ -20,13 +20,17 @@
 //    the same pulse formulas the host uses (laser_core.hpp), for all six
 //    stage times of a step at once across the lanes of the wave (see
 //    step_rows). Sample times use the host-built table row.
-//  * Dense output. Cash-Karp's six stages plus f(t+h, y5) (which is the next
-//    step's first stage, so it costs nothing extra) admit a C1 continuous
-//    extension of order 4, y(t + th h) = y + h sum_i b_i(th) f_i with the
-//    rational b_i(th) below (the order-4 conditions sum b_i(th) Phi_i(tree) =
-//    th^r(tree)/gamma(tree) hold for every th, b(1) = the 5th-order weights,
-//    b'(0) = e_1, b'(1) = e_7). Every sample time inside an accepted step is
-//    evaluated from it.
+//  * Samples. The k-summed observables (inter/intra currents, populations,
+//    rho sums) are needed every dt, while a step covers several dt. At every
+//    step end the kernel evaluates the observables of each k point together
+//    with their exact time derivatives (ds/dt is the stage f(t+h, y5), which
+//    is also the next step's first stage; d/dt of the k + A(t) dependence is
+//    the k gradient along dA/dt), reduces both over the wave, and writes each
+//    sample inside the step as the cubic Hermite interpolant of the two wave
+//    sums at its ends. A sample then costs a few FMAs per wave instead of a
+//    full per-k evaluation; its error is the Hermite error of the observable,
+//    the same order as interpolating the state, which was measured to give
+//    the same accuracy.
 //  * A launch covers the samples [out0, out1) and ends with the state exactly
 //    at t_out1 (the last step is shortened to land there), so consecutive
 //    launches continue seamlessly and the partial-sum buffer stays bounded.
@@ -52,15 +56,6 @@ struct CashKarp {
     // b - bhat (5th- minus 4th-order weights)
     static constexpr double e[6] = {37. / 378. - 2825. / 27648., 0, 250. / 621. - 18575. / 48384.,
                                     125. / 594. - 13525. / 55296., -277. / 14336., 512. / 1771. - 0.25};
-    // dense output b_i(th) = sum_m beta[i][m] th^(m+1), stages 1..6 then f(t+h, y5)
-    static constexpr double beta[7][4] = {
-        {1.0, -65. / 21., 677. / 189., -25. / 18.},
-        {0, 0, 0, 0},
-        {0, 2500. / 483., -38000. / 4347., 250. / 63.},
-        {0, -125. / 44., 3875. / 594., -125. / 36.},
-        {0, -45. / 28., 45. / 14., -45. / 28.},
-        {0, 1536. / 1771., -1024. / 1771., 0},
-        {0, 1.5, -4.0, 2.5}};
 };
 
 __device__ __forceinline__ double uniform(double x)
@@ -80,14 +75,17 @@ __device__ __forceinline__ double wave_max(double x)
     return uniform(x);
 }
 
-// Phase rows of the six stage times of one step (stages 2..6 of Cash-Karp and
-// t + h), in the layout of the sample table: t1 e^{iA.a_j} (j = 0..2) then
-// e^{iA.b_j}, as (re, im) pairs. All six are evaluated at once across the
-// wave: lane s*np + p evaluates pulse p at time s, then lane 6s + j the
-// phase j of time s (36 lanes); the rows go to this wave's LDS slice, from
-// which every lane reads them as broadcasts.
+constexpr int kRow = 14;  // 12 phase values, then dA/dt (x, y)
+
+// Laser rows of six times (the stages 2..6 of a Cash-Karp step and its end
+// t + h, or six copies of one time) in the layout of the sample table:
+// t1 e^{iA.a_j} (j = 0..2) then e^{iA.b_j}, as (re, im) pairs, followed by
+// dA/dt. All six are evaluated at once across the wave: lane s*np + p
+// evaluates pulse p at time s, then lane 6s + j the phase j of time s (36
+// lanes); the rows go to this wave's LDS slice, from which every lane reads
+// them as broadcasts.
 template <typename R>
-__device__ __forceinline__ void step_rows(const AdaptiveArgs &A, double t, double hh, double tn, R *rows /*[6][12]*/)
+__device__ __forceinline__ void laser_rows(const AdaptiveArgs &A, double t, double hh, double tn, R *rows /*[6][kRow]*/)
 {
     const int lane = __lane_id();
     const int np = A.npulses;
@@ -98,19 +96,23 @@ __device__ __forceinline__ void step_rows(const AdaptiveArgs &A, double t, doubl
             if (s == i) tau = t + CashKarp::c[i + 1] * hh;
         return tau;
     };
-    double ax = 0.0, ay = 0.0;
+    double ax = 0.0, ay = 0.0, ex = 0.0, ey = 0.0;
     if (lane < 6 * np) {
         double pa[3], pe[3];
-        pulse_field(A.pulses[lane % np], time_of(lane / np), false, pa, pe);
+        pulse_field(A.pulses[lane % np], time_of(lane / np), true, pa, pe);  // consistent E = -dA/dt
         ax = pa[0];
         ay = pa[1];
+        ex = pe[0];
+        ey = pe[1];
     }
     const int s = lane / 6, j = lane - 6 * s;  // phase j of time s (lanes < 36)
     const int base = (s < 6 ? s : 0) * np;
-    double sx = 0.0, sy = 0.0;  // summed in pulse order, then the value at atmin removed (as the host)
+    double sx = 0.0, sy = 0.0, gx = 0.0, gy = 0.0;  // summed in pulse order, then the value at atmin removed
     for (int q = 0; q < np; ++q) {
         sx += __shfl(ax, base + q, 64);
         sy += __shfl(ay, base + q, 64);
+        gx -= __shfl(ex, base + q, 64);
+        gy -= __shfl(ey, base + q, 64);
     }
     const double Ax = sx - A.A_at_min[0], Ay = sy - A.A_at_min[1];
     const HaldaneArgs &a = A.h;
@@ -124,14 +126,116 @@ __device__ __forceinline__ void step_rows(const AdaptiveArgs &A, double t, doubl
     sincos(Ax * vx + Ay * vy, &sn, &cs);
     const double scale = j < 3 ? a.t1 : 1.0;
     if (lane < 36) {
-        rows[s * 12 + 2 * j] = static_cast<R>(cs * scale);
-        rows[s * 12 + 2 * j + 1] = static_cast<R>(sn * scale);
+        rows[s * kRow + 2 * j] = static_cast<R>(cs * scale);
+        rows[s * kRow + 2 * j + 1] = static_cast<R>(sn * scale);
+        if (j == 0) {
+            rows[s * kRow + 12] = static_cast<R>(gx);
+            rows[s * kRow + 13] = static_cast<R>(gy);
+        }
     }
     // the writes of other lanes of this wave are read next: LDS operations of
     // one wave complete in order; keep the compiler from moving reads above
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Weighted observables of one k point at a step end and their time
+// derivatives: o = w x (inter_x, inter_y, intra_x, intra_y, s_x, s_y, s_z,
+// s.dhat) exactly as the fixed-step sample (antelope.cpp:307-333,
+// SBEs.h:694-760 in Bloch form), and do/dt along the trajectory, with
+// ds/dt = f and d/dt g(k + A(t)) = grad_k g . dA/dt (the k Hessian of d enters
+// through the currents). Stage-precision arithmetic; FP64 accumulation.
+template <typename R>
+__device__ __forceinline__ void observe(const KPoint<R> &p, R nn, const Vec3<R> &s, const Vec3<R> &f, const R *row,
+                                        const HaldaneArgs &a, R ii, double o[8], double od[8])
+{
+    R ar[3], ai[3], br[3], bi[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        cmul(p.er[j], p.ei[j], row[2 * j], row[2 * j + 1], ar[j], ai[j]);  // t1 e^{i(k+A).a_j}
+        cmul(p.er[j + 3], p.ei[j + 3], row[6 + 2 * j], row[7 + 2 * j], br[j], bi[j]);
+    }
+    const R c3 = static_cast<R>(a.c3), c0 = static_cast<R>(a.c0);
+    const R ux = row[12], uy = row[13];
+    const Vec3<R> d{ar[0] + ar[1] + ar[2], ai[0] + ai[1] + ai[2], static_cast<R>(a.M0) - c3 * (bi[0] + bi[1] + bi[2])};
+    // d_a of (d_x, d_y, d_z, d0) for a = x, y; dd/dt; d/dt of the d_a's
+    R xx = 0, xy = 0, yx = 0, yy = 0, zx = 0, zy = 0, ox = 0, oy = 0;
+    R tdx = 0, tdy = 0, tdz = 0;
+    R gxx = 0, gxy = 0, gyx = 0, gyy = 0, gzx = 0, gzy = 0, gox = 0, goy = 0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const R ax = static_cast<R>(a.va[j][0]), ay = static_cast<R>(a.va[j][1]);
+        const R bx = static_cast<R>(a.vb[j][0]), by = static_cast<R>(a.vb[j][1]);
+        const R al = ax * ux + ay * uy, be = bx * ux + by * uy;  // a_j . dA/dt, b_j . dA/dt
+        xx -= ax * ai[j];
+        xy -= ay * ai[j];
+        yx += ax * ar[j];
+        yy += ay * ar[j];
+        zx += bx * br[j];
+        zy += by * br[j];
+        ox += bx * bi[j];
+        oy += by * bi[j];
+        tdx -= al * ai[j];
+        tdy += al * ar[j];
+        tdz += be * br[j];
+        gxx -= ax * al * ar[j];
+        gxy -= ay * al * ar[j];
+        gyx -= ax * al * ai[j];
+        gyy -= ay * al * ai[j];
+        gzx += bx * be * bi[j];
+        gzy += by * be * bi[j];
+        gox += bx * be * br[j];
+        goy += by * be * br[j];
+    }
+    zx *= -c3;
+    zy *= -c3;
+    ox *= -c0;
+    oy *= -c0;
+    tdz *= -c3;
+    gzx *= c3;
+    gzy *= c3;
+    gox *= -c0;
+    goy *= -c0;
+    const Vec3<R> td{tdx, tdy, tdz};
+    const R dd = d.x * d.x + d.y * d.y + d.z * d.z;
+    const R inv = dd > R(0) ? R(1) / dd : R(0);
+    const R ddt = d.x * td.x + d.y * td.y + d.z * td.z;  // (d.dd/dt) = (d|d|^2/dt)/2
+    const R inv_t = R(-2) * inv * inv * ddt;
+    const R sd = s.x * d.x + s.y * d.y + s.z * d.z;
+    const R sd_t = f.x * d.x + f.y * d.y + f.z * d.z + s.x * td.x + s.y * td.y + s.z * td.z;
+    // per axis: t = s.d_a d, q = d.d_a d, p = sd q / |d|^2, and their rates
+    auto axis = [&](R gx, R gy, R gz, R hx, R hy, R hz, R &t, R &t_t, R &pp, R &pp_t) {
+        t = s.x * gx + s.y * gy + s.z * gz;
+        t_t = f.x * gx + f.y * gy + f.z * gz + s.x * hx + s.y * hy + s.z * hz;
+        const R q = d.x * gx + d.y * gy + d.z * gz;
+        const R q_t = td.x * gx + td.y * gy + td.z * gz + d.x * hx + d.y * hy + d.z * hz;
+        pp = sd * q * inv;
+        pp_t = (sd_t * q + sd * q_t) * inv + sd * q * inv_t;
+    };
+    R tx, tx_t, px, px_t, ty, ty_t, py, py_t;
+    axis(xx, yx, zx, gxx, gyx, gzx, tx, tx_t, px, px_t);
+    axis(xy, yy, zy, gxy, gyy, gzy, ty, ty_t, py, py_t);
+    const R ni = R(1) - ii;
+    const double w = p.w;
+    o[0] -= static_cast<double>(tx - ii * px + ni * nn * ox) * w;
+    o[1] -= static_cast<double>(ty - ii * py + ni * nn * oy) * w;
+    o[2] -= static_cast<double>(ii * (nn * ox + px)) * w;
+    o[3] -= static_cast<double>(ii * (nn * oy + py)) * w;
+    o[4] += static_cast<double>(s.x) * w;
+    o[5] += static_cast<double>(s.y) * w;
+    o[6] += static_cast<double>(s.z) * w;
+    od[0] -= static_cast<double>(tx_t - ii * px_t + ni * nn * gox) * w;
+    od[1] -= static_cast<double>(ty_t - ii * py_t + ni * nn * goy) * w;
+    od[2] -= static_cast<double>(ii * (nn * gox + px_t)) * w;
+    od[3] -= static_cast<double>(ii * (nn * goy + py_t)) * w;
+    od[4] += static_cast<double>(f.x) * w;
+    od[5] += static_cast<double>(f.y) * w;
+    od[6] += static_cast<double>(f.z) * w;
+    const bool deg = !(dd > R(0));  // degenerate point: eigenbasis = orbital basis
+    const R rinv = deg ? R(0) : rsqrt(dd);
+    o[7] += static_cast<double>(deg ? -s.z : sd * rinv) * w;
+    od[7] += static_cast<double>(deg ? -f.z : sd_t * rinv - sd * rinv * inv * ddt) * w;
 }
 
 // d(k + A) from the k phases and a stage row (same arithmetic as the fixed-step kernel)
@@ -176,79 +280,94 @@ __device__ __forceinline__ void kahan(R &sum, R &c, R inc)
     sum = t;
 }
 
-template <typename R, bool DEPH>
-__device__ void adaptive_body(const AdaptiveArgs &A, int64_t blk)
+// One wave owns the 64 * KPT consecutive k points [64 KPT w, 64 KPT (w+1))
+// of its solve (lane l: points 64 q + l, q < KPT); the wave is the unit of
+// step control. KPT = 2 gives each lane two independent dependency chains
+// and halves the per-wave work (laser, controller, reductions) per k point.
+template <typename R, bool DEPH, int KPT>
+__device__ void adaptive_body(const AdaptiveArgs &A, int64_t blk, const int64_t out0, const int64_t out1)
 {
     using CK = CashKarp;
     constexpr bool kComp = StateT<R>::kComp != 0;
     const HaldaneArgs &a = A.h;
     const int64_t ks = a.nk_stride;
-    const int64_t k = blk * kBlock + threadIdx.x;
-    const int wave = static_cast<int>(k >> 6);
     const int lane = __lane_id();
+    const int wave = static_cast<int>(blk * (kBlock / 64) + (threadIdx.x >> 6));
+    const int64_t kbase = static_cast<int64_t>(wave) * 64 * KPT + lane;
     R *st = static_cast<R *>(a.state);
-    const R *samp = StateT<R>::kComp ? reinterpret_cast<const R *>(a.phase_tab32)
-                                     : reinterpret_cast<const R *>(a.phase_tab);
 
-    KPoint<R> p;
-    {
+    KPoint<R> p[KPT];
+#pragma unroll
+    for (int q = 0; q < KPT; ++q) {
+        const int64_t k = kbase + 64 * q;
         const double kx = a.kpts[k], ky = a.kpts[ks + k];
-        p.w = a.weight[k];
+        p[q].w = a.weight[k];
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
             double sa, ca, sb, cb;
             sincos(kx * a.va[j][0] + ky * a.va[j][1], &sa, &ca);
             sincos(kx * a.vb[j][0] + ky * a.vb[j][1], &sb, &cb);
-            p.er[j] = static_cast<R>(ca);
-            p.ei[j] = static_cast<R>(sa);
-            p.er[j + 3] = static_cast<R>(cb);
-            p.ei[j + 3] = static_cast<R>(sb);
+            p[q].er[j] = static_cast<R>(ca);
+            p[q].ei[j] = static_cast<R>(sa);
+            p[q].er[j + 3] = static_cast<R>(cb);
+            p[q].ei[j + 3] = static_cast<R>(sb);
         }
-        p.n = st[k];
-        p.sx = st[ks + k];
-        p.sy = st[2 * ks + k];
-        p.sz = st[3 * ks + k];
-        p.cn = p.cx = p.cy = p.cz = R(0);
+        p[q].n = st[k];
+        p[q].sx = st[ks + k];
+        p[q].sy = st[2 * ks + k];
+        p[q].sz = st[3 * ks + k];
+        p[q].cn = p[q].cx = p[q].cy = p[q].cz = R(0);
         if (kComp) {
-            p.cn = st[4 * ks + k];
-            p.cx = st[5 * ks + k];
-            p.cy = st[6 * ks + k];
-            p.cz = st[7 * ks + k];
+            p[q].cn = st[4 * ks + k];
+            p[q].cx = st[5 * ks + k];
+            p[q].cy = st[6 * ks + k];
+            p[q].cz = st[7 * ks + k];
         }
     }
     const R c3 = static_cast<R>(a.c3), M0 = static_cast<R>(a.M0);
     // T1 is off on this path (the host rejects it): g1 = 0
     const R gh2 = static_cast<R>(0.5 * a.gam2), gh12 = static_cast<R>(-0.5 * a.gam2);
     const R ii = a.inter_intra ? R(1) : R(0);
-    const int64_t nw = a.nwaves;
-    auto emit = [&](int64_t m, double v[8]) {
-        const double r = wave_reduce8(v);
-        a.partials[((m - A.out0) * nw + wave) * kHaldaneNV + (lane >> 3)] = r;
-    };
+    // this lane writes value (lane >> 3) of the wave's partial row of each sample
+    double *part = a.partials + static_cast<int64_t>(wave) * kHaldaneNV + (lane >> 3);
+    const int64_t row_stride = static_cast<int64_t>(a.nwaves) * kHaldaneNV;
 
-    int64_t m = A.out0;
+    __shared__ R lds_rows[kBlock / 64][6 * kRow];
+    R *rows = lds_rows[threadIdx.x >> 6];
+
+    int64_t m = out0;
     double t = A.atmin + A.dt_out * static_cast<double>(m);
-    const double t_end = A.atmin + A.dt_out * static_cast<double>(A.out1);
+    const double t_end = A.atmin + A.dt_out * static_cast<double>(out1);
+    const int64_t max_iter = 64 * (out1 - out0) + 1024;  // step attempts allowed in this launch
     double h = uniform(A.hstep[wave]);
     int64_t nacc = 0, nrej = 0;
     bool failed = false;
 
-    // sample at t_out0 from the state itself; its d(k+A) gives the first stage
-    Vec3<R> k1;
+    // Step ends carry the wave sums of the observables and of their time
+    // derivatives (this lane: value lane >> 3); samples in between are the
+    // cubic Hermite interpolant of the two ends. At the window start: the
+    // sample itself, and the first stage.
+    Vec3<R> k1[KPT];
+    double O0, D0;
     {
-        double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        const Vec3<R> d = time_sample<R>(p, samp + m * 12, a, ii, v);
-        emit(m, v);
+        laser_rows<R>(A, t, 0.0, t, rows);
+        const R *row = rows + 5 * kRow;
+        double o[8] = {0, 0, 0, 0, 0, 0, 0, 0}, od[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int q = 0; q < KPT; ++q) {
+            k1[q] = half_f<R, DEPH>(stage_d(p[q], row, c3, M0), Vec3<R>{p[q].sx, p[q].sy, p[q].sz}, gh2, gh12);
+            observe<R>(p[q], p[q].n - p[q].cn, Vec3<R>{p[q].sx - p[q].cx, p[q].sy - p[q].cy, p[q].sz - p[q].cz},
+                       Vec3<R>{2 * k1[q].x, 2 * k1[q].y, 2 * k1[q].z}, row, a, ii, o, od);
+        }
+        O0 = wave_reduce8(o);
+        D0 = wave_reduce8(od);
+        part[(m - out0) * row_stride] = O0;
         ++m;
-        k1 = half_f<R, DEPH>(d, Vec3<R>{p.sx, p.sy, p.sz}, gh2, gh12);
     }
 
-    __shared__ R lds_rows[kBlock / 64][6 * 12];
-    R *rows = lds_rows[threadIdx.x >> 6];
-
-    int iter = 0;
+    int64_t iter = 0;
     while (t < t_end) {
-        if (++iter > A.max_iter || !(h >= A.hmin)) {
+        if (++iter > max_iter || !(h >= A.hmin)) {
             failed = true;
             break;
         }
@@ -256,87 +375,88 @@ __device__ void adaptive_body(const AdaptiveArgs &A, int64_t blk)
         const bool last = t + 1.0001 * hh >= t_end;
         if (last) hh = t_end - t;
         const double tn = last ? t_end : t + hh;
-        step_rows<R>(A, t, hh, tn, rows);
+        laser_rows<R>(A, t, hh, tn, rows);
 
-        // stages 2..6 (stage 1 is k1)
-        Vec3<R> kv[6];
-        kv[0] = k1;
-        const Vec3<R> y0{p.sx, p.sy, p.sz};
+        Vec3<R> inc[KPT], y5[KPT], kv[KPT][6];
+        double err_k = 0.0;
 #pragma unroll
-        for (int i = 1; i < 6; ++i) {
-            Vec3<R> y = y0;
+        for (int q = 0; q < KPT; ++q) {
+            // stages 2..6 (stage 1 is k1)
+            kv[q][0] = k1[q];
+            const Vec3<R> y0{p[q].sx, p[q].sy, p[q].sz};
 #pragma unroll
-            for (int j = 0; j < i; ++j) {
-                if (CK::a[i][j] != 0.0) {
-                    const R aij = static_cast<R>(2.0 * CK::a[i][j] * hh);
-                    y.x += aij * kv[j].x;
-                    y.y += aij * kv[j].y;
-                    y.z += aij * kv[j].z;
+            for (int i = 1; i < 6; ++i) {
+                Vec3<R> y = y0;
+#pragma unroll
+                for (int j = 0; j < i; ++j) {
+                    if (CK::a[i][j] != 0.0) {
+                        const R aij = static_cast<R>(2.0 * CK::a[i][j] * hh);
+                        y.x += aij * kv[q][j].x;
+                        y.y += aij * kv[q][j].y;
+                        y.z += aij * kv[q][j].z;
+                    }
+                }
+                kv[q][i] = half_f<R, DEPH>(stage_d(p[q], rows + (i - 1) * kRow, c3, M0), y, gh2, gh12);
+            }
+            // 5th-order increment (state precision) and the error estimate (FP64)
+            inc[q] = Vec3<R>{0, 0, 0};
+            double ex = 0, ey = 0, ez = 0;
+#pragma unroll
+            for (int i = 0; i < 6; ++i) {
+                if (CK::b[i] != 0.0) {
+                    const R bi = static_cast<R>(2.0 * CK::b[i] * hh);
+                    inc[q].x += bi * kv[q][i].x;
+                    inc[q].y += bi * kv[q][i].y;
+                    inc[q].z += bi * kv[q][i].z;
+                }
+                if (CK::e[i] != 0.0) {
+                    const double ei = 2.0 * CK::e[i] * hh;
+                    ex += ei * static_cast<double>(kv[q][i].x);
+                    ey += ei * static_cast<double>(kv[q][i].y);
+                    ez += ei * static_cast<double>(kv[q][i].z);
                 }
             }
-            kv[i] = half_f<R, DEPH>(stage_d(p, rows + (i - 1) * 12, c3, M0), y, gh2, gh12);
+            y5[q] = Vec3<R>{y0.x + inc[q].x, y0.y + inc[q].y, y0.z + inc[q].z};
+            auto ratio = [&](double e, R y, R z) {  // 0 when the component and its error vanish (atol = 0)
+                const double sc = A.atol + A.rtol * fmax(fabs(static_cast<double>(y)), fabs(static_cast<double>(z)));
+                return e == 0.0 ? 0.0 : fabs(e) / sc;
+            };
+            err_k = nanmax(err_k, nanmax(ratio(ex, y0.x, y5[q].x),
+                                         nanmax(ratio(ey, y0.y, y5[q].y), ratio(ez, y0.z, y5[q].z))));
         }
-        // 5th-order increment (state precision) and the error estimate (FP64)
-        Vec3<R> inc{0, 0, 0};
-        double ex = 0, ey = 0, ez = 0;
-#pragma unroll
-        for (int i = 0; i < 6; ++i) {
-            if (CK::b[i] != 0.0) {
-                const R bi = static_cast<R>(2.0 * CK::b[i] * hh);
-                inc.x += bi * kv[i].x;
-                inc.y += bi * kv[i].y;
-                inc.z += bi * kv[i].z;
-            }
-            if (CK::e[i] != 0.0) {
-                const double ei = 2.0 * CK::e[i] * hh;
-                ex += ei * static_cast<double>(kv[i].x);
-                ey += ei * static_cast<double>(kv[i].y);
-                ez += ei * static_cast<double>(kv[i].z);
-            }
-        }
-        const Vec3<R> y5{y0.x + inc.x, y0.y + inc.y, y0.z + inc.z};
-        auto ratio = [&](double e, R y, R z) {  // 0 when the component and its error vanish (atol = 0)
-            const double sc = A.atol + A.rtol * fmax(fabs(static_cast<double>(y)), fabs(static_cast<double>(z)));
-            return e == 0.0 ? 0.0 : fabs(e) / sc;
-        };
-        const double err_k = nanmax(ratio(ex, y0.x, y5.x), nanmax(ratio(ey, y0.y, y5.y), ratio(ez, y0.z, y5.z)));
         const double err = wave_max(err_k);
 
         if (err <= 1.0) {
-            const Vec3<R> k7 = half_f<R, DEPH>(stage_d(p, rows + 5 * 12, c3, M0), y5, gh2, gh12);
+            const R *row7 = rows + 5 * kRow;
+            double o[8] = {0, 0, 0, 0, 0, 0, 0, 0}, od[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+            for (int q = 0; q < KPT; ++q) {
+                k1[q] = half_f<R, DEPH>(stage_d(p[q], row7, c3, M0), y5[q], gh2, gh12);  // f(t+h, y5)
+                if (kComp) {
+                    kahan(p[q].sx, p[q].cx, inc[q].x);
+                    kahan(p[q].sy, p[q].cy, inc[q].y);
+                    kahan(p[q].sz, p[q].cz, inc[q].z);
+                } else {
+                    p[q].sx += inc[q].x;
+                    p[q].sy += inc[q].y;
+                    p[q].sz += inc[q].z;
+                }
+                observe<R>(p[q], p[q].n - p[q].cn, Vec3<R>{p[q].sx - p[q].cx, p[q].sy - p[q].cy, p[q].sz - p[q].cz},
+                           Vec3<R>{2 * k1[q].x, 2 * k1[q].y, 2 * k1[q].z}, row7, a, ii, o, od);
+            }
+            const double O1 = wave_reduce8(o), D1 = wave_reduce8(od);
             // samples inside (t, tn]
-            const R nn = p.n - p.cn;
-            const Vec3<R> yt{p.sx - p.cx, p.sy - p.cy, p.sz - p.cz};
-            while (m < A.out1) {
+            while (m < out1) {
                 const double tm = A.atmin + A.dt_out * static_cast<double>(m);
                 if (tm > tn) break;
-                const double th = (tm - t) / hh;
-                Vec3<R> yo = yt;
-#pragma unroll
-                for (int i = 0; i < 7; ++i) {
-                    if (i == 1) continue;  // b_2(th) = 0
-                    const double bt = th * (CK::beta[i][0] + th * (CK::beta[i][1] + th * (CK::beta[i][2] + th * CK::beta[i][3])));
-                    const Vec3<R> &kk = i < 6 ? kv[i] : k7;
-                    const R w = static_cast<R>(2.0 * hh * bt);
-                    yo.x += w * kk.x;
-                    yo.y += w * kk.y;
-                    yo.z += w * kk.z;
-                }
-                double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-                (void)sample_state<R>(p, nn, yo.x, yo.y, yo.z, samp + m * 12, a, ii, v);
-                emit(m, v);
+                const double th = (tm - t) / hh, th2 = th * th, th3 = th2 * th;
+                const double h00 = 2.0 * th3 - 3.0 * th2 + 1.0, h01 = 3.0 * th2 - 2.0 * th3;
+                const double h10 = (th3 - 2.0 * th2 + th) * hh, h11 = (th3 - th2) * hh;
+                part[(m - out0) * row_stride] = h00 * O0 + h10 * D0 + h01 * O1 + h11 * D1;
                 ++m;
             }
-            if (kComp) {
-                kahan(p.sx, p.cx, inc.x);
-                kahan(p.sy, p.cy, inc.y);
-                kahan(p.sz, p.cz, inc.z);
-            } else {
-                p.sx += inc.x;
-                p.sy += inc.y;
-                p.sz += inc.z;
-            }
-            k1 = k7;
+            O0 = O1;
+            D0 = D1;
             t = tn;
             ++nacc;
             const double fac = err > 0.0 ? fmin(5.0, fmax(0.2, 0.9 * pow(err, -0.2))) : 5.0;
@@ -347,15 +467,19 @@ __device__ void adaptive_body(const AdaptiveArgs &A, int64_t blk)
         }
     }
 
-    st[k] = p.n;
-    st[ks + k] = p.sx;
-    st[2 * ks + k] = p.sy;
-    st[3 * ks + k] = p.sz;
-    if (kComp) {
-        st[4 * ks + k] = p.cn;
-        st[5 * ks + k] = p.cx;
-        st[6 * ks + k] = p.cy;
-        st[7 * ks + k] = p.cz;
+#pragma unroll
+    for (int q = 0; q < KPT; ++q) {
+        const int64_t k = kbase + 64 * q;
+        st[k] = p[q].n;
+        st[ks + k] = p[q].sx;
+        st[2 * ks + k] = p[q].sy;
+        st[3 * ks + k] = p[q].sz;
+        if (kComp) {
+            st[4 * ks + k] = p[q].cn;
+            st[5 * ks + k] = p[q].cx;
+            st[6 * ks + k] = p[q].cy;
+            st[7 * ks + k] = p[q].cz;
+        }
     }
     if (lane == 0) {
         A.hstep[wave] = h;
@@ -365,57 +489,64 @@ __device__ void adaptive_body(const AdaptiveArgs &A, int64_t blk)
     }
 }
 
-// Many solves in one launch: block b works for solve block_inst[b] on that
-// solve's block b - block_first[solve]; solves with no samples left leave at
-// once.
-template <typename R, bool DEPH>
+// Block b works for solve block_inst[b] on that solve's block
+// b - block_first[solve] (a single solve: block_inst null, solve 0). The
+// window is the next nwin samples after the solve's cursor; solves with none
+// left leave at once. One kernel serves both cases, so a solve gives the
+// same bits alone and in a batch.
+template <typename R, bool DEPH, int KPT>
 __global__ __launch_bounds__(kBlock) void haldane_adaptive_kernel(const AdaptiveArgs *__restrict__ insts,
                                                                   const int32_t *__restrict__ block_inst,
-                                                                  const int32_t *__restrict__ block_first)
+                                                                  const int32_t *__restrict__ block_first,
+                                                                  int64_t nwin)
 {
-    const int inst = block_inst[blockIdx.x];
+    const int inst = block_inst ? block_inst[blockIdx.x] : 0;
     const AdaptiveArgs &A = insts[inst];
-    if (A.out1 <= A.out0) return;  // whole block
-    adaptive_body<R, DEPH>(A, static_cast<int64_t>(blockIdx.x) - block_first[inst]);
+    const int64_t out0 = *A.cursor;
+    const int64_t out1 = min(out0 + nwin, A.nt);
+    if (out1 <= out0) return;  // whole block
+    const int64_t blk = static_cast<int64_t>(blockIdx.x) - (block_inst ? block_first[inst] : 0);
+    adaptive_body<R, DEPH, KPT>(A, blk, out0, out1);
 }
 
-// One solve, arguments by value (no upload per launch).
-template <typename R, bool DEPH>
-__global__ __launch_bounds__(kBlock) void haldane_adaptive_single_kernel(const AdaptiveArgs A)
+__global__ void adaptive_advance_kernel(const AdaptiveArgs *__restrict__ insts, int n, int64_t nwin)
 {
-    adaptive_body<R, DEPH>(A, blockIdx.x);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t c = *insts[i].cursor;
+    *insts[i].cursor = min(c + nwin, insts[i].nt);
 }
 
-template <typename R>
-void launch_adaptive(const AdaptiveArgs *d, const int32_t *bi, const int32_t *bf, int nblocks, bool deph, hipStream_t s)
+template <typename R, int KPT>
+void launch_adaptive(const AdaptiveArgs *d, const int32_t *bi, const int32_t *bf, int nblocks, int64_t nwin, bool deph,
+                     hipStream_t s)
 {
-    if (deph) haldane_adaptive_kernel<R, true><<<nblocks, kBlock, 0, s>>>(d, bi, bf);
-    else haldane_adaptive_kernel<R, false><<<nblocks, kBlock, 0, s>>>(d, bi, bf);
+    if (deph) haldane_adaptive_kernel<R, true, KPT><<<nblocks, kBlock, 0, s>>>(d, bi, bf, nwin);
+    else haldane_adaptive_kernel<R, false, KPT><<<nblocks, kBlock, 0, s>>>(d, bi, bf, nwin);
 }
 
 }  // namespace
 
-hipError_t launch_haldane_adaptive(const AdaptiveArgs &a, hipStream_t s)
+hipError_t launch_haldane_adaptive(const AdaptiveArgs *d_insts, const int32_t *d_block_inst,
+                                   const int32_t *d_block_first, int nblocks, int64_t nwin, int precision,
+                                   bool deph, int kpt, hipStream_t s)
 {
-    const dim3 grid(static_cast<unsigned>(a.h.nk_stride / kBlock));
-    const bool deph = a.h.gam2 != 0.0;
-    if (a.h.precision == SBE_PREC_FP32) {
-        if (deph) haldane_adaptive_single_kernel<float, true><<<grid, kBlock, 0, s>>>(a);
-        else haldane_adaptive_single_kernel<float, false><<<grid, kBlock, 0, s>>>(a);
+    if (nblocks <= 0 || nwin <= 0) return hipSuccess;
+    const bool f32 = precision == SBE_PREC_FP32;
+    if (kpt == 2) {
+        if (f32) launch_adaptive<float, 2>(d_insts, d_block_inst, d_block_first, nblocks, nwin, deph, s);
+        else launch_adaptive<double, 2>(d_insts, d_block_inst, d_block_first, nblocks, nwin, deph, s);
     } else {
-        if (deph) haldane_adaptive_single_kernel<double, true><<<grid, kBlock, 0, s>>>(a);
-        else haldane_adaptive_single_kernel<double, false><<<grid, kBlock, 0, s>>>(a);
+        if (f32) launch_adaptive<float, 1>(d_insts, d_block_inst, d_block_first, nblocks, nwin, deph, s);
+        else launch_adaptive<double, 1>(d_insts, d_block_inst, d_block_first, nblocks, nwin, deph, s);
     }
     return hipGetLastError();
 }
 
-hipError_t launch_haldane_adaptive_batch(const AdaptiveArgs *d_insts, const int32_t *d_block_inst,
-                                         const int32_t *d_block_first, int nblocks, int precision, bool deph,
-                                         hipStream_t s)
+hipError_t launch_adaptive_advance(const AdaptiveArgs *d_insts, int n, int64_t nwin, hipStream_t s)
 {
-    if (nblocks <= 0) return hipSuccess;
-    if (precision == SBE_PREC_FP32) launch_adaptive<float>(d_insts, d_block_inst, d_block_first, nblocks, deph, s);
-    else launch_adaptive<double>(d_insts, d_block_inst, d_block_first, nblocks, deph, s);
+    if (n <= 0) return hipSuccess;
+    adaptive_advance_kernel<<<(n + 63) / 64, 64, 0, s>>>(d_insts, n, nwin);
     return hipGetLastError();
 }
 

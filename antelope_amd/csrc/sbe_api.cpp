@@ -42,7 +42,12 @@ int hip_check(sbe_ctx *ctx, hipError_t e, const char *what)
 // of a batch use a smaller one).
 constexpr size_t kPartialBytes = size_t(128) << 20;
 constexpr size_t kBatchPartialBytes = size_t(24) << 20;
-constexpr size_t kAdaptiveBatchPartialBytes = size_t(96) << 20;  // longer windows: fewer shortened steps
+// The adaptive stepper ends every launch exactly on a sample time, so launch
+// boundaries shape its step sequence. Its windows are therefore a fixed
+// property of the method, [0, W), [W, 2W), ..., for single solves, batches and
+// served requests alike; a run split at multiples of W gives the same bits as
+// one run.
+constexpr int64_t kAdaptiveWindow = 2048;
 thread_local size_t t_partial_bytes = kPartialBytes;
 
 struct Plan {
@@ -110,7 +115,12 @@ int make_plan(const sbe_config &cfg, const sbe_model &m, int64_t nk, int64_t nt,
     // dependency chains, which pays when there are few waves per SIMD
     p.kpt = 1;
     if (const char *e = std::getenv("ANTELOPE_HALDANE_KPT")) p.kpt = std::atoi(e) == 2 ? 2 : 1;
-    if (cfg.rk_order == SBE_RK_CASH_KARP) p.kpt = 1;  // the adaptive kernel takes one k point per lane
+    if (cfg.rk_order == SBE_RK_CASH_KARP) {  // adaptive: 64 kpt consecutive k points per wave (step-control block)
+        // two chains per lane pay in FP64 (1.4x on the batched workload); the FP32 kernel
+        // loses its second wave per SIMD to registers with two, so it keeps one
+        p.kpt = cfg.precision == SBE_PREC_FP32 ? 1 : 2;
+        if (const char *e = std::getenv("ANTELOPE_ADAPTIVE_KPT")) p.kpt = std::atoi(e) == 1 ? 1 : 2;
+    }
     const int64_t unit = static_cast<int64_t>(kBlock) * p.kpt;
     p.nk_stride = (nk + unit - 1) / unit * unit;
     p.nwaves = static_cast<int32_t>(p.nk_stride / (64 * p.kpt));
@@ -119,6 +129,7 @@ int make_plan(const sbe_config &cfg, const sbe_model &m, int64_t nk, int64_t nt,
     p.nstage = nstage;
     const size_t row = static_cast<size_t>(p.nwaves) * p.nv * sizeof(double);
     p.chunk = std::max<int64_t>(1, std::min<int64_t>(nt, static_cast<int64_t>(t_partial_bytes / row)));
+    if (cfg.rk_order == SBE_RK_CASH_KARP) p.chunk = std::min<int64_t>(nt, kAdaptiveWindow);
     const size_t real = cfg.precision == SBE_PREC_FP32 ? sizeof(float) : sizeof(double);
     const int comps = cfg.precision == SBE_PREC_FP32 ? 8 : 4;
     p.bytes_state = static_cast<size_t>(p.nk_stride) * comps * real;
@@ -361,6 +372,8 @@ int sbe_create(const sbe_config *cfg, const sbe_model *model, int64_t k_begin, i
         ctx->d_counts = dalloc<int64_t>(ctx, 2 * sizeof(int64_t) * ctx->nwaves, rc);
         ctx->d_fail = dalloc<int32_t>(ctx, sizeof(int32_t), rc);
         ctx->d_pulses = dalloc<sbe::PulseConst>(ctx, sizeof(sbe::PulseConst) * sbe::kMaxDevicePulses, rc);
+        ctx->d_aargs = dalloc<sbe::AdaptiveArgs>(ctx, sizeof(sbe::AdaptiveArgs), rc);
+        ctx->d_cursor = dalloc<int64_t>(ctx, sizeof(int64_t), rc);
     }
     if (rc) {
         set_error(nullptr, rc, ctx->err);
@@ -440,6 +453,8 @@ int sbe_create(const sbe_config *cfg, const sbe_model *model, int64_t k_begin, i
         A.h = h;
         A.atmin = ctx->laser.atmin;
         A.dt_out = ctx->laser.dt;
+        A.nt = ctx->laser.nt;
+        A.cursor = ctx->d_cursor;
         A.rtol = ctx->cfg.rtol;
         A.atol = ctx->cfg.atol;
         A.hmax = 32.0 * ctx->laser.dt;
@@ -459,6 +474,9 @@ int sbe_create(const sbe_config *cfg, const sbe_model *model, int64_t k_begin, i
                            "upload pulses");
         if (!rc) rc = hip_check(ctx, hipMemsetAsync(ctx->d_counts, 0, 2 * sizeof(int64_t) * ctx->nwaves, ctx->stream), "clear counts");
         if (!rc) rc = hip_check(ctx, hipMemsetAsync(ctx->d_fail, 0, sizeof(int32_t), ctx->stream), "clear flag");
+        if (!rc) rc = hip_check(ctx, hipMemsetAsync(ctx->d_cursor, 0, sizeof(int64_t), ctx->stream), "clear cursor");
+        if (!rc)
+            rc = hip_check(ctx, hipMemcpy(ctx->d_aargs, &A, sizeof A, hipMemcpyHostToDevice), "upload adaptive arguments");
     }
     if (!rc) rc = hip_check(ctx, sbe::launch_haldane_init(h, ctx->cfg.init_type, ctx->cfg.fermi_energy, ctx->stream), "init kernel");
     if (!rc) rc = hip_check(ctx, hipStreamSynchronize(ctx->stream), "init sync");
@@ -484,13 +502,16 @@ int sbe_run(sbe_ctx *ctx, int64_t step_begin, int64_t step_end)
         return set_error(ctx, SBE_ERR_RANGE, "step_end outside [step_begin, Nt]");
     if (ctx->path == sbe::PathKind::NBand) return sbe::nb_run(ctx, step_begin, step_end);
     if (ctx->adaptive) {
+        const bool deph = ctx->hargs.gam2 != 0.0;
+        const int nblocks = static_cast<int>(ctx->nk_stride / (kBlock * ctx->hargs.kpt));
         for (int64_t s = step_begin; s < step_end; s += ctx->chunk_steps) {
             const int64_t n = std::min<int64_t>(ctx->chunk_steps, step_end - s);
-            sbe::AdaptiveArgs A = ctx->aargs;
-            A.out0 = s;
-            A.out1 = s + n;
-            A.max_iter = static_cast<int32_t>(std::min<int64_t>(INT32_MAX, 64 * n + 1024));
-            int rc = timed(ctx, 0, [&] { return sbe::launch_haldane_adaptive(A, ctx->stream); }, "adaptive step kernel");
+            // the kernel reads the window start from the device cursor (== s)
+            int rc = timed(ctx, 0, [&] {
+                return sbe::launch_haldane_adaptive(ctx->d_aargs, nullptr, nullptr, nblocks, n, ctx->cfg.precision,
+                                                    deph, ctx->hargs.kpt, ctx->stream);
+            }, "adaptive step kernel");
+            if (!rc) rc = hip_check(ctx, sbe::launch_adaptive_advance(ctx->d_aargs, 1, n, ctx->stream), "advance kernel");
             if (!rc)
                 rc = timed(ctx, 1, [&] {
                     return sbe::launch_series_reduce(ctx->d_partials, n, ctx->nwaves, ctx->nv,
@@ -531,7 +552,7 @@ int sbe_get_step_stats(sbe_ctx *ctx, sbe_step_stats *out)
         return SBE_OK;
     }
     out->blocks = ctx->nwaves;
-    out->k_per_block = 64;
+    out->k_per_block = 64 * (ctx->adaptive ? ctx->hargs.kpt : 1);
     if (!ctx->adaptive) {
         out->accepted = ctx->steps_done * ctx->nwaves;
         out->max_accepted = out->min_accepted = ctx->steps_done;
@@ -680,6 +701,7 @@ struct sbe_batch {
     int max_blocks = 0;
     int rk = 0, prec = 0;
     bool adaptive = false;
+    int kpt = 1;
     sbe::AdaptiveArgs *d_ainsts = nullptr;
     bool t1 = false, deph = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
@@ -728,7 +750,7 @@ int sbe_batch_create(int32_t n, const sbe_config *cfgs, const sbe_model *models,
     sbe_batch *b = new (std::nothrow) sbe_batch();
     if (!b) return batch_fail(nullptr, SBE_ERR_OOM, "host allocation failed");
     b->adaptive = cfgs[0].rk_order == SBE_RK_CASH_KARP;
-    t_partial_bytes = b->adaptive ? kAdaptiveBatchPartialBytes : kBatchPartialBytes;
+    t_partial_bytes = kBatchPartialBytes;
     for (int i = 0; i < n; ++i) {
         sbe_ctx *c = nullptr;
         const int rc = sbe_create(&cfgs[i], &models[i], -1, -1, &c);
@@ -739,7 +761,8 @@ int sbe_batch_create(int32_t n, const sbe_config *cfgs, const sbe_model *models,
             return batch_fail(nullptr, rc, m);
         }
         b->ctx.push_back(c);
-        b->max_blocks += static_cast<int>(c->nk_stride / kBlock);
+        b->max_blocks += static_cast<int>(c->nk_stride / (kBlock * (b->adaptive ? c->hargs.kpt : 1)));
+        b->kpt = c->hargs.kpt;
         b->t1 = b->t1 || c->hargs.gam1 != 0.0;
         b->deph = b->deph || c->hargs.gam1 != 0.0 || c->hargs.gam2 != 0.0;
     }
@@ -748,8 +771,22 @@ int sbe_batch_create(int32_t n, const sbe_config *cfgs, const sbe_model *models,
     b->prec = cfgs[0].precision;
     hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(&b->d_insts, sizeof(sbe::HaldaneArgs) * n);
-    if (e == hipSuccess && b->adaptive) e = hipMalloc(&b->d_ainsts, sizeof(sbe::AdaptiveArgs) * n);
     if (e == hipSuccess) e = hipMalloc(&b->d_map, sizeof(int32_t) * (b->max_blocks + n));
+    if (e == hipSuccess && b->adaptive) {
+        // constant for the batch's lifetime: every solve's arguments and the block -> solve map
+        // over all solves (solves with no samples left return at once)
+        e = hipMalloc(&b->d_ainsts, sizeof(sbe::AdaptiveArgs) * n);
+        std::vector<sbe::AdaptiveArgs> ai(n);
+        std::vector<int32_t> map(b->max_blocks + n);
+        int nb = 0;
+        for (int i = 0; i < n; ++i) {
+            ai[i] = b->ctx[i]->aargs;
+            map[b->max_blocks + i] = nb;
+            for (int64_t q = 0; q < b->ctx[i]->nk_stride / (kBlock * b->kpt); ++q) map[nb++] = i;
+        }
+        if (e == hipSuccess) e = hipMemcpy(b->d_ainsts, ai.data(), sizeof(sbe::AdaptiveArgs) * n, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(b->d_map, map.data(), sizeof(int32_t) * map.size(), hipMemcpyHostToDevice);
+    }
     if (e != hipSuccess) {
         const std::string m = std::string("batch set-up: ") + hipGetErrorString(e);
         sbe_batch_destroy(b);
@@ -765,13 +802,13 @@ int sbe_batch_run(sbe_batch *b, int64_t max_steps)
     const int n = static_cast<int>(b->ctx.size());
     int64_t budget = max_steps > 0 ? max_steps : INT64_MAX;
     std::vector<sbe::HaldaneArgs> insts(n);
-    std::vector<sbe::AdaptiveArgs> ainsts(b->adaptive ? n : 0);
     std::vector<int32_t> map(b->max_blocks + n);
     while (budget > 0) {
         int64_t launch = INT64_MAX;
         for (sbe_ctx *c : b->ctx)
             if (c->steps_done < c->laser.nt) launch = std::min<int64_t>(launch, c->chunk_steps);
         if (launch == INT64_MAX) break;  // all done
+        if (b->adaptive) launch = kAdaptiveWindow;  // every solve on the fixed window grid
         launch = std::min(launch, budget);
         int nb = 0;
         for (int i = 0; i < n; ++i) {
@@ -779,31 +816,31 @@ int sbe_batch_run(sbe_batch *b, int64_t max_steps)
             insts[i] = c->hargs;
             insts[i].step0 = c->steps_done;
             insts[i].nsteps = static_cast<int32_t>(std::min<int64_t>(launch, c->laser.nt - c->steps_done));
-            if (b->adaptive) {
-                ainsts[i] = c->aargs;
-                ainsts[i].out0 = c->steps_done;
-                ainsts[i].out1 = c->steps_done + insts[i].nsteps;
-                ainsts[i].max_iter = static_cast<int32_t>(64 * static_cast<int64_t>(insts[i].nsteps) + 1024);
-            }
+            if (b->adaptive) continue;  // its block map is static (set up at creation)
             map[b->max_blocks + i] = nb;
             if (insts[i].nsteps > 0)
                 for (int64_t q = 0; q < c->nk_stride / kBlock; ++q) map[nb++] = i;
         }
         // solve -> first block table sits after the block -> solve table
         int32_t *d_first = b->d_map + b->max_blocks;
-        hipError_t e = b->adaptive
-                           ? hipMemcpyAsync(b->d_ainsts, ainsts.data(), sizeof(sbe::AdaptiveArgs) * n, hipMemcpyHostToDevice, b->stream)
-                           : hipMemcpyAsync(b->d_insts, insts.data(), sizeof(sbe::HaldaneArgs) * n, hipMemcpyHostToDevice, b->stream);
-        if (e == hipSuccess) e = hipMemcpyAsync(b->d_map, map.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, b->stream);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(d_first, map.data() + b->max_blocks, sizeof(int32_t) * n, hipMemcpyHostToDevice, b->stream);
+        hipError_t e = hipSuccess;
+        if (!b->adaptive) {  // the adaptive batch keeps its arguments and block map on the device (set up once)
+            e = hipMemcpyAsync(b->d_insts, insts.data(), sizeof(sbe::HaldaneArgs) * n, hipMemcpyHostToDevice, b->stream);
+            if (e == hipSuccess) e = hipMemcpyAsync(b->d_map, map.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, b->stream);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(d_first, map.data() + b->max_blocks, sizeof(int32_t) * n, hipMemcpyHostToDevice, b->stream);
+        }
         hipEvent_t ea = nullptr, eb = nullptr;
         if (e == hipSuccess) e = hipEventCreate(&ea);
         if (e == hipSuccess) e = hipEventCreate(&eb);
         if (e == hipSuccess) e = hipEventRecord(ea, b->stream);
-        if (e == hipSuccess)
-            e = b->adaptive ? sbe::launch_haldane_adaptive_batch(b->d_ainsts, b->d_map, d_first, nb, b->prec, b->deph, b->stream)
-                            : sbe::launch_haldane_batch(b->d_insts, b->d_map, d_first, nb, b->rk, b->prec, b->t1, b->deph, b->stream);
+        if (e == hipSuccess && b->adaptive) {
+            e = sbe::launch_haldane_adaptive(b->d_ainsts, b->d_map, d_first, b->max_blocks, launch, b->prec, b->deph,
+                                             b->kpt, b->stream);
+            if (e == hipSuccess) e = sbe::launch_adaptive_advance(b->d_ainsts, n, launch, b->stream);
+        } else if (e == hipSuccess) {
+            e = sbe::launch_haldane_batch(b->d_insts, b->d_map, d_first, nb, b->rk, b->prec, b->t1, b->deph, b->stream);
+        }
         if (e == hipSuccess) e = hipEventRecord(eb, b->stream);
         if (ea && eb) b->ev.push_back({ea, eb});
         for (int i = 0; i < n && e == hipSuccess; ++i) {

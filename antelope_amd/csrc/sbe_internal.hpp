@@ -88,18 +88,22 @@ struct HaldaneArgs {
 // fixed times t_n = atmin + n dt_out by the stepper's dense output. The
 // laser is evaluated on the device at the stage times.
 constexpr int kMaxDevicePulses = 8;
+// The struct lives in device memory and does not change after set-up; a
+// launch covers the samples [*cursor, min(*cursor + nwin, nt)) and a second
+// one-thread-per-solve kernel then advances the cursor, so nothing is copied
+// per launch and one kernel serves single solves and batches alike.
 struct AdaptiveArgs {
     HaldaneArgs h;             // model, grid, state, partials (phase_tab: rows at t_n only)
     double atmin, dt_out;
-    int64_t out0, out1;        // samples [out0, out1) of this launch; the state ends at t_out1
+    int64_t nt;
+    int64_t *cursor;           // next sample; the state is at t_cursor
     double rtol, atol, hmax, hmin;
-    int32_t max_iter;          // step attempts allowed per wave and launch
     int32_t npulses;
     double A_at_min[3];
     const PulseConst *pulses;  // [npulses] in device memory
     double *hstep;             // [nwaves] proposed step size carried between launches
     int64_t *counts;           // [nwaves][2] accepted and rejected step attempts
-    int32_t *fail;             // set when a wave exhausts max_iter or its step falls below hmin
+    int32_t *fail;             // set when a wave exhausts its attempts or its step falls below hmin
 };
 
 // ------------------------------------------------------------------ ctx --
@@ -145,6 +149,8 @@ struct sbe_ctx {
     bool adaptive = false;
     sbe::AdaptiveArgs aargs{};
     sbe::PulseConst *d_pulses = nullptr;
+    sbe::AdaptiveArgs *d_aargs = nullptr;  // device copy of aargs (constant)
+    int64_t *d_cursor = nullptr;
     double *d_hstep = nullptr;
     int64_t *d_counts = nullptr;
     int32_t *d_fail = nullptr;
@@ -174,10 +180,12 @@ hipError_t launch_series_reduce(const double *partials, int64_t nsteps, int32_t 
 hipError_t launch_haldane_export(const HaldaneArgs &a, double *out, hipStream_t s);
 hipError_t launch_haldane_batch(const HaldaneArgs *d_insts, const int32_t *d_block_inst, const int32_t *d_block_first,
                                 int nblocks, int rk_order, int precision, bool t1_on, bool deph, hipStream_t s);
-hipError_t launch_haldane_adaptive(const AdaptiveArgs &a, hipStream_t s);
-hipError_t launch_haldane_adaptive_batch(const AdaptiveArgs *d_insts, const int32_t *d_block_inst,
-                                         const int32_t *d_block_first, int nblocks, int precision, bool deph,
-                                         hipStream_t s);
+// Adaptive stepper: d_block_inst / d_block_first may be null for a single
+// solve (d_insts[0], grid = its blocks). Then advance every solve's cursor.
+hipError_t launch_haldane_adaptive(const AdaptiveArgs *d_insts, const int32_t *d_block_inst,
+                                   const int32_t *d_block_first, int nblocks, int64_t nwin, int precision,
+                                   bool deph, int kpt, hipStream_t s);
+hipError_t launch_adaptive_advance(const AdaptiveArgs *d_insts, int n, int64_t nwin, hipStream_t s);
 int haldane_waves(int64_t nk);
 // N-band host side (nband_host.cpp)
 int nb_validate(const sbe_model &m, std::string &err);

@@ -250,7 +250,13 @@ int sbe_server_step(sbe_server *s)
         r.nt = inf.nt;
         r.nband = inf.nband;
         if (r.ctx->path == sbe::PathKind::NBand) {
-            r.slice = s->slice;
+            // every N-band step is a dozen small launches: short slices keep the hardware
+            // queue this request shares with others from filling with thousands of them
+            r.slice = std::max<int64_t>(1, s->slice / 32);
+        } else if (r.ctx->adaptive) {
+            // whole windows of the adaptive stepper, so the result equals a run on its own
+            const int64_t w = r.ctx->chunk_steps;
+            r.slice = std::max<int64_t>(1, (s->slice + w - 1) / w) * w;
         } else {
             const double by_work = kSliceKSteps / static_cast<double>(std::max<int64_t>(1, inf.nk_total));
             r.slice = std::max<int64_t>(s->slice, static_cast<int64_t>(by_work));

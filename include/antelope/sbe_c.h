@@ -103,8 +103,12 @@ enum { SBE_PREC_FP64 = 0, SBE_PREC_FP32 = 1 };
  * embedded 5(4) pair of Cash & Karp (no reference counterpart; 2-band
  * models with T1 off): every block of 64 k points chooses its own steps
  * under rtol/atol, and the series is still sampled at t_n = atmin + n dt,
- * from a 4th-order dense output, so it has the layout of a fixed-step run.
- * dt then only sets the sample spacing (and the first step size). */
+ * from the stepper's dense output, so it has the layout of a fixed-step run.
+ * dt then only sets the sample spacing (and the first step size). The
+ * stepper lands exactly on the sample time at the end of every window of
+ * 2048 samples and at every sbe_run step_end; runs split at multiples of
+ * 2048 samples give the same bits as one run, other splits agree to the
+ * tolerance. */
 enum { SBE_RK4 = 4, SBE_RK5_FEHLBERG = 5, SBE_RK_CASH_KARP = 45 };
 
 /* Electric field convention. REFERENCE reproduces laser.h:217-224, whose

@@ -38,7 +38,9 @@ _SIGS = {
     "orc_run_adaptive": ([C.POINTER(A.sbe_config), C.POINTER(A.sbe_model), C.c_int64, C.c_int64, C.c_int64,
                           C.c_int64, C.c_int, _D, _D, _D, _D, _D, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                           C.POINTER(orc_info)], C.c_int),
-    "orc_cash_karp": ([_D, _D, _D, _D, _D], C.c_int),
+    "orc_cash_karp": ([_D, _D, _D, _D], C.c_int),
+    "orc_observe": ([C.POINTER(A.sbe_config), C.POINTER(A.sbe_model), _D, C.c_double, _D, _D, _D, _D], C.c_int),
+    "orc_currents": ([C.POINTER(A.sbe_config), C.POINTER(A.sbe_model), _D, C.c_double, _D, _D, _D, _D], C.c_int),
     "orc_last_error": ([], C.c_char_p),
 }
 
@@ -175,7 +177,27 @@ def run_adaptive(config, model, k_begin=-1, k_end=-1, block=64, window=-1, nthre
 
 
 def cash_karp():
-    """(c, a, b, bhat, beta) exactly as the restatement uses them."""
-    c, a, b, bh, beta = np.zeros(6), np.zeros((6, 5)), np.zeros(6), np.zeros(6), np.zeros((7, 4))
-    _chk(lib().orc_cash_karp(_p(c), _p(a), _p(b), _p(bh), _p(beta)))
-    return c, a, b, bh, beta
+    """(c, a, b, bhat) exactly as the restatement uses them."""
+    c, a, b, bh = np.zeros(6), np.zeros((6, 5)), np.zeros(6), np.zeros(6)
+    _chk(lib().orc_cash_karp(_p(c), _p(a), _p(b), _p(bh)))
+    return c, a, b, bh
+
+
+def observe(config, model, k, t, rho, drho):
+    """2-band observables (inter x,y, intra x,y, s, s.dhat) and their time derivatives."""
+    cfg, k1 = config.to_c()
+    mdl, k2 = model.to_c()
+    o, od = np.zeros(8), np.zeros(8)
+    kk = np.ascontiguousarray(k, dtype=np.float64)
+    _chk(lib().orc_observe(C.byref(cfg), C.byref(mdl), _p(kk), t, _p(_c(rho)), _p(_c(drho)), _p(o), _p(od)))
+    return o, od
+
+
+def currents(config, model, k, t, rho):
+    """Generic-path currents (inter[3], intra[3]) and band populations."""
+    cfg, k1 = config.to_c()
+    mdl, k2 = model.to_c()
+    inter, intra, pop = np.zeros(3), np.zeros(3), np.zeros(model.nband)
+    kk = np.ascontiguousarray(k, dtype=np.float64)
+    _chk(lib().orc_currents(C.byref(cfg), C.byref(mdl), _p(kk), t, _p(_c(rho)), _p(inter), _p(intra), _p(pop)))
+    return inter, intra, pop

@@ -947,7 +947,7 @@ int orc_run(const sbe_config *cfg, const sbe_model *model, int64_t k_begin, int6
 /* ------------------------------------------------------ adaptive stepper */
 /* Restatement of the build's adaptive Cash-Karp 5(4) stepper (no reference
  * counterpart; the reference steps with a fixed dt, SBEs.h:596-624). Same
- * model, right-hand side and samples as orc_run; the step sequence is:
+ * model, right-hand side and observables as orc_run; the step sequence is:
  *  - blocks of `block` consecutive k points share one step size; a step is
  *    accepted when err = max over the block's k points and the Bloch
  *    components s_x = 2 Re rho01, s_y = -2 Im rho01, s_z = rho00 - rho11 of
@@ -957,13 +957,11 @@ int orc_run(const sbe_config *cfg, const sbe_model *model, int64_t k_begin, int6
  *    within 1e-4 of it before a window end is stretched to end there; each
  *    window of `window` samples (the last one shorter) ends exactly on its
  *    sample time, and a shortened last step leaves the proposal unchanged;
- *  - samples t_n = atmin + n dt inside an accepted step come from the 4th-
- *    order continuous extension y + h sum_i b_i(th) f_i over the six stages
- *    and f(t + h, y5).
- * Cash-Karp coefficients: J.R. Cash, A.H. Karp, ACM TOMS 16 (1990) 201.
- * The dense-output weights b_i(th) were derived for this build (order-4
- * conditions for all th, b(1) = b, b'(0) = e_1, b'(1) = e_7); tests check the
- * order conditions from these numbers directly. */
+ *  - at every step end the block sums of the eight observables (inter and
+ *    intra current x, y; the Bloch vector; s.dhat) and of their time
+ *    derivatives are formed; a sample t_n = atmin + n dt inside the step is
+ *    the cubic Hermite interpolant of the two ends.
+ * Cash-Karp coefficients: J.R. Cash, A.H. Karp, ACM TOMS 16 (1990) 201. */
 static const double ck_c[6] = {0.0, 0.2, 0.3, 0.6, 1.0, 0.875};
 static const double ck_a[6][5] = {
     {0, 0, 0, 0, 0},
@@ -974,22 +972,13 @@ static const double ck_a[6][5] = {
     {1631. / 55296., 175. / 512., 575. / 13824., 44275. / 110592., 253. / 4096.}};
 static const double ck_b[6] = {37. / 378., 0, 250. / 621., 125. / 594., 0, 512. / 1771.};
 static const double ck_bhat[6] = {2825. / 27648., 0, 18575. / 48384., 13525. / 55296., 277. / 14336., 0.25};
-static const double ck_beta[7][4] = {
-    {1.0, -65. / 21., 677. / 189., -25. / 18.},
-    {0, 0, 0, 0},
-    {0, 2500. / 483., -38000. / 4347., 250. / 63.},
-    {0, -125. / 44., 3875. / 594., -125. / 36.},
-    {0, -45. / 28., 45. / 14., -45. / 28.},
-    {0, 1536. / 1771., -1024. / 1771., 0},
-    {0, 1.5, -4.0, 2.5}};
 
-int orc_cash_karp(double c[6], double a[6][5], double b[6], double bhat[6], double beta[7][4])
+int orc_cash_karp(double c[6], double a[6][5], double b[6], double bhat[6])
 {
     if (c) memcpy(c, ck_c, sizeof ck_c);
     if (a) memcpy(a, ck_a, sizeof ck_a);
     if (b) memcpy(b, ck_b, sizeof ck_b);
     if (bhat) memcpy(bhat, ck_bhat, sizeof ck_bhat);
-    if (beta) memcpy(beta, ck_beta, sizeof ck_beta);
     return SBE_OK;
 }
 
@@ -1000,15 +989,167 @@ static void bloch(const cplx *r, double s[3])
     s[2] = creal(r[0]) - creal(r[3]);
 }
 
+/* Haldane B-components of Haldane.h:86-92 (v[0] = B0 = d0, v[1..3] = d) at q
+ * with their first (g[a]) and second (hs[a][b]) derivatives in k_x, k_y. */
+static void haldane_d2(const omodel *om, const double q[3], double v[4], double g[2][4], double hs[2][2][4])
+{
+    const sbe_model *m = om->m;
+    const double c0 = 2. * m->t2 * cos(m->phi0), c3 = 2. * m->t2 * sin(m->phi0);
+    memset(v, 0, 4 * sizeof(double));
+    memset(g, 0, 8 * sizeof(double));
+    memset(hs, 0, 16 * sizeof(double));
+    v[3] = m->M0;
+    for (int i = 0; i < 3; ++i) {
+        const double *va = om->va[i], *vb = om->vb[i];
+        const double aa = q[0] * va[0] + q[1] * va[1], ab = q[0] * vb[0] + q[1] * vb[1];
+        const double ca = cos(aa), sa = sin(aa), cb = cos(ab), sb = sin(ab);
+        v[0] += c0 * cb;
+        v[1] += m->t1 * ca;
+        v[2] += m->t1 * sa;
+        v[3] -= c3 * sb;
+        for (int x = 0; x < 2; ++x) {
+            g[x][0] -= c0 * vb[x] * sb;
+            g[x][1] -= m->t1 * va[x] * sa;
+            g[x][2] += m->t1 * va[x] * ca;
+            g[x][3] -= c3 * vb[x] * cb;
+            for (int y = 0; y < 2; ++y) {
+                hs[x][y][0] -= c0 * vb[x] * vb[y] * cb;
+                hs[x][y][1] -= m->t1 * va[x] * va[y] * ca;
+                hs[x][y][2] -= m->t1 * va[x] * va[y] * sa;
+                hs[x][y][3] += c3 * vb[x] * vb[y] * sb;
+            }
+        }
+    }
+}
+
+/* dA/dt at t: the derivative of the summed pulses (the offset at atmin is a
+ * constant), i.e. minus the consistent field of laser.h's formulas. */
+static void laser_rate(const olaser *L, double t, double u[3])
+{
+    u[0] = u[1] = u[2] = 0.0;
+    for (int p = 0; p < L->np; ++p) {
+        double A[3], E[3];
+        pulse_eval(&L->p[p], t, 1, A, E);
+        for (int i = 0; i < 3; ++i) u[i] -= E[i];
+    }
+}
+
+/* Observables of one k point with Bloch vector s (n = trace) and their time
+ * derivatives along ds/dt = f: o = (inter_x, inter_y, intra_x, intra_y, s_x,
+ * s_y, s_z, s.dhat) in the GenInterIntraCurrent / GenCurrent conventions of
+ * SBEs.h:626-760 (inter carries the total when InterIntra is off). */
+static void bloch_observe(const osolver *S, const double k[3], double t, double n, const double s[3],
+                          const double f[3], double o[8], double od[8])
+{
+    double A[3], E[3], u[3];
+    laser_eval(&S->L, t, A, E);
+    laser_rate(&S->L, t, u);
+    const double q[3] = {k[0] + A[0], k[1] + A[1], k[2] + A[2]};
+    double v[4], g[2][4], hs[2][2][4];
+    haldane_d2(&S->om, q, v, g, hs);
+    const double *d = v + 1;
+    double td[4], gt[2][4];
+    for (int c = 0; c < 4; ++c) {
+        td[c] = g[0][c] * u[0] + g[1][c] * u[1];
+        for (int x = 0; x < 2; ++x) gt[x][c] = hs[x][0][c] * u[0] + hs[x][1][c] * u[1];
+    }
+    const double dd = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    const double inv = dd > 0 ? 1.0 / dd : 0.0;
+    const double ddt = d[0] * td[1] + d[1] * td[2] + d[2] * td[3];
+    const double inv_t = -2.0 * inv * inv * ddt;
+    const double sd = s[0] * d[0] + s[1] * d[1] + s[2] * d[2];
+    const double sd_t = f[0] * d[0] + f[1] * d[1] + f[2] * d[2] + s[0] * td[1] + s[1] * td[2] + s[2] * td[3];
+    const int ii = S->cfg->inter_intra != 0;
+    for (int x = 0; x < 2; ++x) {
+        const double *gd = g[x] + 1, *hd = gt[x] + 1;
+        const double tt = s[0] * gd[0] + s[1] * gd[1] + s[2] * gd[2];
+        const double tt_t = f[0] * gd[0] + f[1] * gd[1] + f[2] * gd[2] + s[0] * hd[0] + s[1] * hd[1] + s[2] * hd[2];
+        const double qq = d[0] * gd[0] + d[1] * gd[1] + d[2] * gd[2];
+        const double qq_t = td[1] * gd[0] + td[2] * gd[1] + td[3] * gd[2] + d[0] * hd[0] + d[1] * hd[1] + d[2] * hd[2];
+        const double pp = sd * qq * inv, pp_t = (sd_t * qq + sd * qq_t) * inv + sd * qq * inv_t;
+        /* Tr(rho J_x) = -(n d_x d0 + s.d_x d); intra = -(n d_x d0 + (s.dh)(dh.d_x d)) */
+        if (ii) {
+            o[x] = -(tt - pp);
+            od[x] = -(tt_t - pp_t);
+            o[2 + x] = -(n * g[x][0] + pp);
+            od[2 + x] = -(n * gt[x][0] + pp_t);
+        } else {
+            o[x] = -(n * g[x][0] + tt);
+            od[x] = -(n * gt[x][0] + tt_t);
+            o[2 + x] = od[2 + x] = 0.0;
+        }
+    }
+    for (int c = 0; c < 3; ++c) {
+        o[4 + c] = s[c];
+        od[4 + c] = f[c];
+    }
+    if (dd > 0) {
+        const double r = 1.0 / sqrt(dd);
+        o[7] = sd * r;
+        od[7] = sd_t * r - sd * r * inv * ddt;
+    } else {
+        o[7] = -s[2];
+        od[7] = -f[2];
+    }
+}
+
+int orc_observe(const sbe_config *cfg, const sbe_model *model, const double k[3], double t, const double *rho,
+                const double *drho, double *o, double *od)
+{
+    osolver S;
+    int rc = solver_init(cfg, model, &S);
+    if (rc) return rc;
+    if (S.N != 2) { solver_free(&S); return fail(SBE_ERR_UNSUPPORTED, "orc_observe is for N = 2"); }
+    double s[3], f[3];
+    bloch((const cplx *)rho, s);
+    bloch((const cplx *)drho, f);
+    const cplx *r = (const cplx *)rho;
+    bloch_observe(&S, k, t, creal(r[0]) + creal(r[3]), s, f, o, od);
+    solver_free(&S);
+    return SBE_OK;
+}
+
+int orc_currents(const sbe_config *cfg, const sbe_model *model, const double k[3], double t, const double *rho,
+                 double *inter, double *intra, double *pop)
+{
+    osolver S;
+    int rc = solver_init(cfg, model, &S);
+    if (rc) return rc;
+    oscratch w;
+    scratch_alloc(&w, S.N);
+    currents(&S, k, t, (const cplx *)rho, inter, intra, pop, &w);
+    scratch_free(&w);
+    solver_free(&S);
+    return SBE_OK;
+}
+
 typedef struct {
     const osolver *S;
     const double *kp, *wk;
     int64_t b0, b1, nk, block, nsteps, window;
-    double *ji, *ja, *rb, *bp;
+    double *ob; /* [nt][8] block-summed observables (this thread) */
+    double *nsum;
     cplx *rho;
     int64_t *acc, *rej;
     int failed;
 } oathread;
+
+static void block_observe(const oathread *T, int64_t k0, int64_t k1, const cplx *rho, const cplx *drho, double t,
+                          double O[8], double D[8])
+{
+    for (int i = 0; i < 8; ++i) O[i] = D[i] = 0.0;
+    for (int64_t k = k0; k < k1; ++k) {
+        const cplx *r = rho + (size_t)(k - k0) * 4, *dr = drho + (size_t)(k - k0) * 4;
+        double s[3], f[3], o[8], od[8];
+        bloch(r, s);
+        bloch(dr, f);
+        bloch_observe(T->S, T->kp + 3 * k, t, creal(r[0]) + creal(r[3]), s, f, o, od);
+        for (int i = 0; i < 8; ++i) {
+            O[i] += T->wk[k] * o[i];
+            D[i] += T->wk[k] * od[i];
+        }
+    }
+}
 
 static void *adaptive_main(void *arg)
 {
@@ -1020,39 +1161,31 @@ static void *adaptive_main(void *arg)
     oscratch w;
     scratch_alloc(&w, N);
     const int64_t B = T->block;
-    cplx *st = calloc((size_t)B * 7 * nn, sizeof(cplx)); /* stages f_1..f_7 per k */
+    cplx *st = calloc((size_t)B * 7 * nn, sizeof(cplx)); /* stages f_1..f_6 and f(t+h, y5) per k */
     cplx *y5 = calloc((size_t)B * nn, sizeof(cplx)), *tmp = calloc((size_t)nn, sizeof(cplx));
-    double inter[3], intra[3], pop[2];
+    cplx *cur = calloc((size_t)B * nn, sizeof(cplx)), *der = calloc((size_t)B * nn, sizeof(cplx));
     for (int64_t blk = T->b0; blk < T->b1; ++blk) {
-        const int64_t k0 = blk * B, k1 = k0 + B < T->nk ? k0 + B : T->nk, nb = k1 - k0;
+        const int64_t k0 = blk * B, k1 = k0 + B < T->nk ? k0 + B : T->nk;
         double h = dt;
         int64_t nacc = 0, nrej = 0;
-        /* the currents of sample n from state r (antelope.cpp:307-333) */
-#define ORC_SAMPLE(n, kk, r)                                                             \
-        do {                                                                             \
-            const double *kq = T->kp + 3 * (kk);                                         \
-            const double wq = T->wk[kk];                                                 \
-            currents(S, kq, atmin + dt * (double)(n), (r), inter, intra, pop, &w);      \
-            for (int ax = 0; ax < 3; ++ax) {                                             \
-                T->ja[3 * (n) + ax] += intra[ax] * wq;                                   \
-                T->ji[3 * (n) + ax] += inter[ax] * wq;                                   \
-            }                                                                            \
-            for (int i = 0; i < nn; ++i) {                                               \
-                T->rb[((n) * nn + i) * 2 + 0] += creal((r)[i]) * wq;                     \
-                T->rb[((n) * nn + i) * 2 + 1] += cimag((r)[i]) * wq;                     \
-            }                                                                            \
-            for (int q = 0; q < N; ++q) T->bp[(n) * N + q] += pop[q] * wq;              \
-        } while (0)
+        for (int64_t k = k0; k < k1; ++k) {
+            const cplx *r = T->rho + (size_t)k * nn;
+            *T->nsum += T->wk[k] * (creal(r[0]) + creal(r[3]));
+        }
         for (int64_t w0 = 0; w0 < T->nsteps && !T->failed; w0 += T->window) {
             const int64_t w1 = w0 + T->window < T->nsteps ? w0 + T->window : T->nsteps;
             double t = atmin + dt * (double)w0;
             const double t_end = atmin + dt * (double)w1;
             int64_t m = w0;
+            double O0[8], D0[8], O1[8], D1[8];
             for (int64_t k = k0; k < k1; ++k) {
                 const cplx *r = T->rho + (size_t)k * nn;
-                ORC_SAMPLE(m, k, r);
                 rhs(S, T->kp + 3 * k, t, r, st + ((k - k0) * 7 + 0) * nn, &w);
+                memcpy(cur + (k - k0) * nn, r, sizeof(cplx) * nn);
+                memcpy(der + (k - k0) * nn, st + (k - k0) * 7 * nn, sizeof(cplx) * nn);
             }
+            block_observe(T, k0, k1, cur, der, t, O0, D0);
+            for (int i = 0; i < 8; ++i) T->ob[m * 8 + i] += O0[i];
             ++m;
             int64_t iter = 0;
             while (t < t_end) {
@@ -1095,28 +1228,25 @@ static void *adaptive_main(void *arg)
                     for (int64_t k = k0; k < k1; ++k) {
                         cplx *f = st + (k - k0) * 7 * nn;
                         rhs(S, T->kp + 3 * k, tn, y5 + (k - k0) * nn, f + 6 * nn, &w);
+                        memcpy(der + (k - k0) * nn, f + 6 * nn, sizeof(cplx) * nn);
                     }
+                    block_observe(T, k0, k1, y5, der, tn, O1, D1);
                     for (; m < w1; ++m) {
                         const double tm = atmin + dt * (double)m;
                         if (tm > tn) break;
-                        const double th = (tm - t) / hh;
-                        double bt[7];
-                        for (int i = 0; i < 7; ++i)
-                            bt[i] = th * (ck_beta[i][0] + th * (ck_beta[i][1] + th * (ck_beta[i][2] + th * ck_beta[i][3])));
-                        for (int64_t k = k0; k < k1; ++k) {
-                            const cplx *r = T->rho + (size_t)k * nn, *f = st + (k - k0) * 7 * nn;
-                            for (int e = 0; e < nn; ++e) {
-                                tmp[e] = r[e];
-                                for (int i = 0; i < 7; ++i) tmp[e] += hh * bt[i] * f[i * nn + e];
-                            }
-                            ORC_SAMPLE(m, k, tmp);
-                        }
+                        const double th = (tm - t) / hh, th2 = th * th, th3 = th2 * th;
+                        const double h00 = 2.0 * th3 - 3.0 * th2 + 1.0, h01 = 3.0 * th2 - 2.0 * th3;
+                        const double h10 = (th3 - 2.0 * th2 + th) * hh, h11 = (th3 - th2) * hh;
+                        for (int i = 0; i < 8; ++i)
+                            T->ob[m * 8 + i] += h00 * O0[i] + h10 * D0[i] + h01 * O1[i] + h11 * D1[i];
                     }
                     for (int64_t k = k0; k < k1; ++k) {
                         memcpy(T->rho + (size_t)k * nn, y5 + (k - k0) * nn, sizeof(cplx) * nn);
                         cplx *f = st + (k - k0) * 7 * nn;
                         memcpy(f, f + 6 * nn, sizeof(cplx) * nn); /* f(t+h, y5) is the next first stage */
                     }
+                    memcpy(O0, O1, sizeof O0);
+                    memcpy(D0, D1, sizeof D0);
                     t = tn;
                     ++nacc;
                     const double fac = err > 0.0 ? fmin(5.0, fmax(0.2, 0.9 * pow(err, -0.2))) : 5.0;
@@ -1127,12 +1257,10 @@ static void *adaptive_main(void *arg)
                 }
             }
         }
-#undef ORC_SAMPLE
-        (void)nb;
         if (T->acc) T->acc[blk] = nacc;
         if (T->rej) T->rej[blk] = nrej;
     }
-    free(st); free(y5); free(tmp);
+    free(st); free(y5); free(tmp); free(cur); free(der);
     scratch_free(&w);
     return NULL;
 }
@@ -1156,7 +1284,8 @@ int orc_run_adaptive(const sbe_config *cfg, const sbe_model *model, int64_t k_be
     if (nk <= 0 || k_end > ntot) { solver_free(&S); return fail(SBE_ERR_RANGE, "bad k range"); }
     if (block < 1) block = 64;
     const int64_t nt = S.L.nt;
-    if (window < 1 || window > nt) window = nt;
+    if (window < 1) window = 2048; /* the product's fixed window grid */
+    if (window > nt) window = nt;
     double *kp = malloc(sizeof(double) * 3 * nk), *wk = malloc(sizeof(double) * nk);
     double dV, tw;
     rc = orc_kgrid(cfg, model, k_begin, k_end, kp, wk, &dV, &tw);
@@ -1173,16 +1302,15 @@ int orc_run_adaptive(const sbe_config *cfg, const sbe_model *model, int64_t k_be
     if (nthreads > nblk) nthreads = (int)nblk;
     oathread *th = calloc((size_t)nthreads, sizeof(oathread));
     pthread_t *tid = calloc((size_t)nthreads, sizeof(pthread_t));
+    double *nsums = calloc((size_t)nthreads, sizeof(double));
     for (int r = 0; r < nthreads; ++r) {
         int64_t w1 = nblk / nthreads, w2 = nblk % nthreads;
         int64_t s = w1 * r + (r < w2 ? r : w2), e = s + w1 + (w2 > r ? 1 : 0);
         th[r].S = &S; th[r].kp = kp; th[r].wk = wk; th[r].b0 = s; th[r].b1 = e; th[r].nk = nk;
         th[r].block = block; th[r].nsteps = nt; th[r].window = window; th[r].rho = rho;
         th[r].acc = accepted; th[r].rej = rejected;
-        th[r].ji = calloc((size_t)nt * 3, sizeof(double));
-        th[r].ja = calloc((size_t)nt * 3, sizeof(double));
-        th[r].rb = calloc((size_t)nt * nn * 2, sizeof(double));
-        th[r].bp = calloc((size_t)nt * N, sizeof(double));
+        th[r].ob = calloc((size_t)nt * 8, sizeof(double));
+        th[r].nsum = nsums + r;
     }
     struct timespec t0, t1;
     clock_gettime(CLOCK_MONOTONIC, &t0);
@@ -1191,26 +1319,22 @@ int orc_run_adaptive(const sbe_config *cfg, const sbe_model *model, int64_t k_be
     for (int r = 1; r < nthreads; ++r) pthread_join(tid[r], NULL);
     clock_gettime(CLOCK_MONOTONIC, &t1);
     int failed = 0;
-    for (int r = 0; r < nthreads; ++r) failed |= th[r].failed;
+    double nsum = 0.0;
+    for (int r = 0; r < nthreads; ++r) { failed |= th[r].failed; nsum += nsums[r]; }
     for (int64_t n = 0; n < nt; ++n) {
-        for (int ax = 0; ax < 3; ++ax) {
-            double si = 0, sa = 0;
-            for (int r = 0; r < nthreads; ++r) { si += th[r].ji[3 * n + ax]; sa += th[r].ja[3 * n + ax]; }
-            if (j_inter) j_inter[3 * n + ax] = si;
-            if (j_intra) j_intra[3 * n + ax] = sa;
+        double O[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int r = 0; r < nthreads; ++r)
+            for (int i = 0; i < 8; ++i) O[i] += th[r].ob[n * 8 + i];
+        if (j_inter) { j_inter[3 * n] = O[0]; j_inter[3 * n + 1] = O[1]; j_inter[3 * n + 2] = 0.0; }
+        if (j_intra) { j_intra[3 * n] = O[2]; j_intra[3 * n + 1] = O[3]; j_intra[3 * n + 2] = 0.0; }
+        if (rho_bar) { /* rho = (n + s.sigma)/2 */
+            double *o = rho_bar + n * 8;
+            o[0] = 0.5 * (nsum + O[6]); o[1] = 0.0;
+            o[2] = 0.5 * O[4];          o[3] = -0.5 * O[5];
+            o[4] = 0.5 * O[4];          o[5] = 0.5 * O[5];
+            o[6] = 0.5 * (nsum - O[6]); o[7] = 0.0;
         }
-        if (rho_bar)
-            for (int i = 0; i < nn * 2; ++i) {
-                double s = 0;
-                for (int r = 0; r < nthreads; ++r) s += th[r].rb[n * nn * 2 + i];
-                rho_bar[n * nn * 2 + i] = s;
-            }
-        if (band_pop)
-            for (int m = 0; m < N; ++m) {
-                double s = 0;
-                for (int r = 0; r < nthreads; ++r) s += th[r].bp[n * N + m];
-                band_pop[n * N + m] = s;
-            }
+        if (band_pop) { band_pop[2 * n] = 0.5 * (nsum - O[7]); band_pop[2 * n + 1] = 0.5 * (nsum + O[7]); }
     }
     if (rho_final) memcpy(rho_final, rho, sizeof(cplx) * (size_t)nk * nn);
     if (info) {
@@ -1219,8 +1343,8 @@ int orc_run_adaptive(const sbe_config *cfg, const sbe_model *model, int64_t k_be
         info->dV = dV; info->total_weight = tw;
         info->seconds = (t1.tv_sec - t0.tv_sec) + 1e-9 * (t1.tv_nsec - t0.tv_nsec);
     }
-    for (int r = 0; r < nthreads; ++r) { free(th[r].ji); free(th[r].ja); free(th[r].rb); free(th[r].bp); }
-    free(th); free(tid); free(rho); free(kp); free(wk);
+    for (int r = 0; r < nthreads; ++r) free(th[r].ob);
+    free(nsums); free(th); free(tid); free(rho); free(kp); free(wk);
     solver_free(&S);
     return failed ? fail(SBE_ERR_NAN, "adaptive step size underflow") : SBE_OK;
 }

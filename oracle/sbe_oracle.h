@@ -68,15 +68,22 @@ int orc_run(const sbe_config *cfg, const sbe_model *model, int64_t k_begin,
 
 /* Adaptive Cash-Karp 5(4) restatement (see sbe_oracle.c): blocks of `block`
  * consecutive k points (<= 0: 64) choose their own steps; windows of
- * `window` samples (<= 0: all) end on their sample time. 2-band models, T1
- * off. accepted / rejected [nblocks] may be NULL. */
+ * `window` samples (<= 0: 2048, as the product) end on their sample time.
+ * 2-band models, T1 off. accepted / rejected [nblocks] may be NULL. */
 int orc_run_adaptive(const sbe_config *cfg, const sbe_model *model, int64_t k_begin,
                      int64_t k_end, int64_t block, int64_t window, int nthreads,
                      double *j_inter, double *j_intra, double *rho_bar, double *band_pop,
                      double *rho_final, int64_t *accepted, int64_t *rejected,
                      orc_info *info);
-/* The Cash-Karp coefficients and dense-output weights the restatement uses. */
-int orc_cash_karp(double c[6], double a[6][5], double b[6], double bhat[6], double beta[7][4]);
+/* The Cash-Karp coefficients the restatement uses. */
+int orc_cash_karp(double c[6], double a[6][5], double b[6], double bhat[6]);
+/* 2-band observables o[8] = (inter_x, inter_y, intra_x, intra_y, s_x, s_y,
+ * s_z, s.dhat) of rho at (k, t) and their time derivatives along drho/dt. */
+int orc_observe(const sbe_config *cfg, const sbe_model *model, const double k[3], double t,
+                const double *rho, const double *drho, double *o, double *od);
+/* Currents and band populations of the generic path (SBEs.h:626-760). */
+int orc_currents(const sbe_config *cfg, const sbe_model *model, const double k[3], double t,
+                 const double *rho, double *inter, double *intra, double *pop);
 
 /* Last error message of the calling thread. */
 const char *orc_last_error(void);

@@ -16,8 +16,8 @@ pytestmark = pytest.mark.gpu
 MODEL = HaldaneModel(a0=2.683, t1=0.075, t2=0.025, phi0=0.3 * math.pi, M0=2.54 * 0.025)
 
 
-def _cfg(precision="fp64", G=16, rtol=1e-8, ncycles=1.0, E0=0.006, ellip=0.5, cep=0.0):
-    return RunConfig(npoints=[G, G], pulses=[Pulse(E0=E0, w0=0.014, ncycles=ncycles, ellip=ellip, cep=cep,
+def _cfg(precision="fp64", G=16, rtol=1e-8, ncycles=1.0, E0=0.006, ellip=0.5, cep=0.0, G2=None):
+    return RunConfig(npoints=[G, G2 or G], pulses=[Pulse(E0=E0, w0=0.014, ncycles=ncycles, ellip=ellip, cep=cep,
                                                    envelope="sin2")],
                      dt=0.25, T2=220.0, rk_order=45, inter_intra=True, precision=precision,
                      rtol=rtol, atol=1e-3 * rtol)
@@ -38,14 +38,14 @@ def _gpu(cfg, model=MODEL, slices=None):
 
 
 def test_fp64_matches_oracle_restatement():
-    cfg = _cfg()
-    print("case: 16x16 grid, 1 cycle, fp64, rtol 1e-8", flush=True)
+    cfg = _cfg(G=32, G2=16)   # 512 k points: whole step-control blocks, no padding
+    print("case: 32x16 grid, 1 cycle, fp64, rtol 1e-8", flush=True)
     g, st = _gpu(cfg)
-    r = pyoracle.run_adaptive(cfg, MODEL, nthreads=8)
+    r = pyoracle.run_adaptive(cfg, MODEL, block=st.k_per_block, nthreads=8)
     for key in ("j_inter", "j_intra", "band_pop"):
         assert _rel(g[key], r[key]) < 1e-10, key
     assert _rel(g["rho_bar"], r["rho_bar"]) < 1e-10
-    assert st.blocks == 4 and st.k_per_block == 64
+    assert st.blocks * st.k_per_block == 512 and st.blocks == len(r["accepted"])
     assert st.accepted == r["accepted"].sum() and st.rejected == r["rejected"].sum()
     assert st.max_accepted == r["accepted"].max() and st.min_accepted == r["accepted"].min()
 
@@ -53,8 +53,8 @@ def test_fp64_matches_oracle_restatement():
 def test_sliced_runs_match_oracle_windows():
     cfg = _cfg(ncycles=1.5, cep=1.0)
     print("case: sbe_run in slices of 97 samples", flush=True)
-    g, _ = _gpu(cfg, slices=97)
-    r = pyoracle.run_adaptive(cfg, MODEL, window=97, nthreads=8)
+    g, st = _gpu(cfg, slices=97)
+    r = pyoracle.run_adaptive(cfg, MODEL, block=st.k_per_block, window=97, nthreads=8)
     assert _rel(g["j_inter"] + g["j_intra"], r["j_inter"] + r["j_intra"]) < 1e-10
     assert _rel(g["band_pop"], r["band_pop"]) < 1e-10
 
@@ -62,8 +62,8 @@ def test_sliced_runs_match_oracle_windows():
 def test_fp32_state_error_vs_fp64_oracle():
     """FP32 state (Kahan-compensated) with FP64 error control and k sums."""
     print("case: fp32 state, 24x24 grid", flush=True)
-    r = pyoracle.run_adaptive(_cfg(G=24), MODEL, nthreads=8)
     g, st = _gpu(_cfg("fp32", G=24))
+    r = pyoracle.run_adaptive(_cfg(G=24), MODEL, block=st.k_per_block, nthreads=8)
     ej = _rel(g["j_inter"] + g["j_intra"], r["j_inter"] + r["j_intra"])
     ep = _rel(g["band_pop"], r["band_pop"])
     print(f"fp32 adaptive: j {ej:.2e}, populations {ep:.2e}, accepted/block {st.accepted / st.blocks:.0f}")

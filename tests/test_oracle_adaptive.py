@@ -1,10 +1,11 @@
 """Pins of the oracle's adaptive Cash-Karp restatement (CPU only).
 
 The coefficients are checked against the Runge-Kutta order conditions
-(Butcher trees up to order 5), written out here independently of how they
-were derived; the stepper is checked against the oracle's fixed-step RK4 at a
-step size small enough to be converged, which pins the controller, the
-dense output and the window handling together.
+(Butcher trees up to order 5); the 2-band observables against the generic
+eigen-decomposition currents, and their time derivatives against central
+differences; the stepper against the oracle's fixed-step RK4 at a step size
+small enough to be converged, which pins the controller, the sample
+interpolation and the window handling together.
 """
 import math
 
@@ -31,14 +32,14 @@ def _order_conditions(A, c, b):
 
 
 def _tableau():
-    c, a, b, bh, beta = pyoracle.cash_karp()
+    c, a, b, bh = pyoracle.cash_karp()
     A = np.zeros((6, 6))
     A[:, :5] = a
-    return c, A, b, bh, beta
+    return c, A, b, bh
 
 
 def test_cash_karp_pair_orders():
-    c, A, b, bh, _ = _tableau()
+    c, A, b, bh = _tableau()
     np.testing.assert_allclose(A.sum(1), c, atol=1e-15)
     conds5 = _order_conditions(A, c, b)
     assert all(abs(v - x) < 1e-14 for v, x in conds5)            # b: order 5
@@ -47,44 +48,51 @@ def test_cash_karp_pair_orders():
     assert max(abs(v - x) for v, x in conds4[8:]) > 1e-4          # ... and not 5
 
 
-def test_dense_output_weights():
-    """b_i(th) over the six stages and f(t+h, y5): order 4 for every th, C1 at both ends."""
-    c, A6, b, _, beta = _tableau()
-    A = np.zeros((7, 7))
-    A[:6, :6] = A6
-    A[6, :6] = b          # the seventh stage evaluates f at y5 = y + h sum b_i f_i
-    c7 = np.append(c, 1.0)
-    bt = lambda th: np.array([sum(beta[i][m] * th ** (m + 1) for m in range(4)) for i in range(7)])
-    dbt = lambda th: np.array([sum((m + 1) * beta[i][m] * th ** m for m in range(4)) for i in range(7)])
-    for th in (0.0, 0.13, 0.5, 0.77, 1.0):
-        w = bt(th)
-        Ac = A @ c7
-        got = [w.sum(), w @ c7, w @ c7 ** 2, w @ Ac, w @ c7 ** 3, w @ (c7 * Ac), w @ (A @ c7 ** 2), w @ (A @ Ac)]
-        want = [th, th ** 2 / 2, th ** 3 / 3, th ** 3 / 6, th ** 4 / 4, th ** 4 / 8, th ** 4 / 12, th ** 4 / 24]
-        np.testing.assert_allclose(got, want, atol=1e-14)
-    np.testing.assert_allclose(bt(1.0), np.append(b, 0.0), atol=1e-14)
-    np.testing.assert_allclose(dbt(0.0), np.eye(7)[0], atol=1e-14)
-    np.testing.assert_allclose(dbt(1.0), np.eye(7)[6], atol=1e-14)
+def _random_rho(rng):
+    """A 2x2 density matrix (Hermitian, trace 1, |s| < 1)."""
+    s = rng.normal(size=3)
+    s *= 0.9 / np.linalg.norm(s)
+    return 0.5 * np.array([[1 + s[2], s[0] - 1j * s[1]], [s[0] + 1j * s[1], 1 - s[2]]])
 
 
-def test_dense_output_local_order():
-    """On y' = i w y one step of Cash-Karp + the continuous extension has interior
-    error O(h^5): halving h divides it by about 32."""
-    c, A6, b, _, beta = _tableau()
-    w = 1.3
+@pytest.mark.parametrize("ii", [True, False])
+def test_observables_match_generic_currents(ii):
+    """The Bloch-form observables equal the eigen-decomposition currents and band
+    populations of the generic path (SBEs.h:626-760)."""
+    cfg = _cfg(45)
+    cfg.inter_intra = ii
+    rng = np.random.default_rng(4)
+    _, _, atmin, _, nt = pyoracle.laser_eval(cfg, np.zeros(1))
+    for _ in range(6):
+        k = np.append(rng.uniform(-1, 1, 2), 0.0)
+        t = atmin + rng.uniform(0, nt * 0.25)
+        rho = _random_rho(rng)
+        o, _ = pyoracle.observe(cfg, _MODEL, k, t, rho, np.zeros((2, 2)))
+        inter, intra, pop = pyoracle.currents(cfg, _MODEL, k, t, rho)
+        np.testing.assert_allclose(o[:2], inter[:2], rtol=1e-10, atol=1e-14)
+        np.testing.assert_allclose(o[2:4], intra[:2], rtol=1e-10, atol=1e-14)
+        assert abs(o[7] - (pop[1] - pop[0])) < 1e-12
+        np.testing.assert_allclose(o[4:7], [2 * rho[0, 1].real, -2 * rho[0, 1].imag, (rho[0, 0] - rho[1, 1]).real],
+                                   atol=1e-15)
 
-    def interp_err(h, th=0.4):
-        f = lambda y: 1j * w * y
-        k = []
-        for i in range(6):
-            k.append(f(1.0 + h * sum(A6[i, j] * k[j] for j in range(i))))
-        y5 = 1.0 + h * sum(b[i] * k[i] for i in range(6))
-        k.append(f(y5))
-        bt = [sum(beta[i][m] * th ** (m + 1) for m in range(4)) for i in range(7)]
-        return abs(1.0 + h * sum(bt[i] * k[i] for i in range(7)) - np.exp(1j * w * th * h))
 
-    r = interp_err(0.2) / interp_err(0.1)
-    assert 25 < r < 40, r
+def test_observable_rates_match_finite_differences():
+    """d/dt of the observables along the SBE trajectory (state through rho' = RHS,
+    k + A(t) through the laser) against central differences."""
+    cfg = _cfg(45)
+    rng = np.random.default_rng(5)
+    _, _, atmin, _, nt = pyoracle.laser_eval(cfg, np.zeros(1))
+    for _ in range(5):
+        k = np.append(rng.uniform(-1, 1, 2), 0.0)
+        t = atmin + rng.uniform(0.3, 0.7) * nt * 0.25      # inside the pulse
+        rho = _random_rho(rng)
+        f = pyoracle.rhs(cfg, _MODEL, k, t, rho)
+        _, od = pyoracle.observe(cfg, _MODEL, k, t, rho, f)
+        eps = 1e-4
+        op, _ = pyoracle.observe(cfg, _MODEL, k, t + eps, rho + eps * f, f)
+        om, _ = pyoracle.observe(cfg, _MODEL, k, t - eps, rho - eps * f, f)
+        # rho +- eps f is the trajectory to first order; its O(eps^2) error cancels in the central difference
+        np.testing.assert_allclose(od, (op - om) / (2 * eps), rtol=1e-5, atol=1e-9)
 
 
 def _cfg(rk, dt=0.25, rtol=1e-8, G=10, E0=0.006):
