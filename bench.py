@@ -158,7 +158,7 @@ def bench_serving(args) -> int:
     k-point RK steps/s over the whole stream and p50/p99 submit-to-done latency."""
     # one hardware queue per in-flight request (HIP's default is 4 per process): requests on
     # a shared queue wait behind each other's kernels; must be set before the first HIP call
-    os.environ.setdefault("GPU_MAX_HW_QUEUES", str(min(16, args.max_active)))
+    os.environ["GPU_MAX_HW_QUEUES"] = str(min(16, args.max_active))
     from antelope_amd.serve import SBEServer, run_stream, serving_stream
 
     n = args.requests
