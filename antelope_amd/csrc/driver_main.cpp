@@ -1,7 +1,8 @@
 // antelope_mi355x - the reference's executable (antelope.cpp:50-593) on one
 // MI355X: reads inputParam.cfg from the working directory (or the path
 // given), propagates the whole k grid on the GPU and writes the reference's
-// output files into the working directory (or --outdir).
+// output files into the working directory (or --outdir). A config whose name
+// ends in .atl is read in the .atl format of src/inputFile.atl.
 //
 //   antelope_mi355x [config] [--outdir DIR] [--quiet]
 #include <cstdio>
@@ -22,8 +23,9 @@ int main(int argc, char **argv)
         else path = argv[i];
     }
     sbe_spec *spec = nullptr;
-    if (sbe_spec_parse_file(path.c_str(), &spec) != SBE_OK) {
-        std::fprintf(stderr, "%s\n", sbe_frontend_last_error());
+    const bool atl = path.size() > 4 && path.compare(path.size() - 4, 4, ".atl") == 0;
+    if ((atl ? sbe_spec_parse_atl_file(path.c_str(), &spec) : sbe_spec_parse_file(path.c_str(), &spec)) != SBE_OK) {
+        std::fprintf(stderr, "%s\n", atl ? sbe_atl_last_error() : sbe_frontend_last_error());
         return 1;
     }
     if (*sbe_spec_warnings(spec)) std::fprintf(stderr, "Warning: %s", sbe_spec_warnings(spec));

@@ -25,6 +25,7 @@
 
 #include "antelope/sbe_c.h"
 #include "antelope/sbe_frontend.h"
+#include "frontend_internal.hpp"
 #include "sbe_internal.hpp"
 
 namespace {
@@ -249,13 +250,11 @@ constexpr double kAuAngstrom = 0.529177210903;  // constant.h:34
 
 }  // namespace
 
-struct sbe_spec {
-    sbe_config cfg{};
-    sbe_model model{};
-    std::vector<sbe_pulse> pulses;
-    std::string target;
-    std::string warnings;
-};
+
+namespace {
+constexpr double kAuEV = 27.211386245988;  // constant.h:35
+thread_local std::string t_base_dir;      // directory of the config file being parsed
+}  // namespace
 
 extern "C" {
 
@@ -393,10 +392,26 @@ int sbe_spec_parse_string(const char *text, sbe_spec **out)
         m.nband = 2;
         m.nval = 1;
         m.is_dipole_zero = 1;
+    } else if (mg && mg->child("dataName")) {  // Wannier90.h:107-125
+        std::string path;
+        if (!get_string(mg, "dataName", path)) return fe_fail(SBE_ERR_CONFIG, "dataName must be a string");
+        if (!path.empty() && path[0] != '/' && !t_base_dir.empty()) path = t_base_dir + "/" + path;
+        long long nval = 0;
+        double fermi_ev;
+        if (get_double(mg, "FermiE", fermi_ev)) {  // if FermiE exists, the valence band number is ignored
+            c.init_type = SBE_INIT_FERMI_DIRAC;
+            c.fermi_energy = fermi_ev / kAuEV;
+        } else if (!get_int(mg, "Nval", nval)) {
+            return fe_fail(SBE_ERR_MODEL, "Wannier90 target needs Nval or FermiE");
+        }
+        int rc = sbe_wannier_read_tb(path.c_str(), &spec->tb);
+        if (rc) return fe_fail(rc, sbe_wannier_last_error());
+        rc = sbe_wannier_tb_model(spec->tb, static_cast<int32_t>(nval), &m);
+        if (rc) return fe_fail(rc, sbe_wannier_last_error());
     } else {
         return fe_fail(SBE_ERR_UNSUPPORTED, "target '" + spec->target +
-                                                "' is not available as a config-file model (Haldane only; "
-                                                "Wannier models are passed as arrays)");
+                                                "' is not available as a config-file model (Haldane, or a "
+                                                "Wannier90 group with dataName)");
     }
     c.npulses = static_cast<int32_t>(spec->pulses.size());
     c.pulses = spec->pulses.data();
@@ -411,7 +426,12 @@ int sbe_spec_parse_file(const char *path, sbe_spec **out)
     if (!f) return fe_fail(SBE_ERR_CONFIG, std::string("I/O error while reading file ") + path);
     std::stringstream ss;
     ss << f.rdbuf();
-    return sbe_spec_parse_string(ss.str().c_str(), out);
+    const std::string p(path);
+    const size_t slash = p.find_last_of('/');
+    t_base_dir = slash == std::string::npos ? std::string() : p.substr(0, slash);
+    const int rc = sbe_spec_parse_string(ss.str().c_str(), out);
+    t_base_dir.clear();
+    return rc;
 }
 
 int sbe_spec_get(const sbe_spec *spec, const sbe_config **cfg, const sbe_model **model)

@@ -23,6 +23,14 @@ _SIGS = {
     "sbe_frontend_last_error": ([], C.c_char_p),
     "sbe_write_outputs": ([C.c_char_p, C.POINTER(A.sbe_config), C.POINTER(A.sbe_model), C.POINTER(A.sbe_info),
                            A.c_double_p, A.c_double_p, A.c_double_p, A.c_double_p], C.c_int),
+    "sbe_wannier_read_tb": ([C.c_char_p, C.POINTER(C.c_void_p)], C.c_int),
+    "sbe_wannier_tb_model": ([C.c_void_p, C.c_int32, C.POINTER(A.sbe_model)], C.c_int),
+    "sbe_wannier_tb_free": ([C.c_void_p], None),
+    "sbe_wannier_write_tb": ([C.c_char_p, C.POINTER(A.sbe_model), C.c_char_p], C.c_int),
+    "sbe_wannier_last_error": ([], C.c_char_p),
+    "sbe_spec_parse_atl_string": ([C.c_char_p, C.POINTER(C.c_void_p)], C.c_int),
+    "sbe_spec_parse_atl_file": ([C.c_char_p, C.POINTER(C.c_void_p)], C.c_int),
+    "sbe_atl_last_error": ([], C.c_char_p),
 }
 _bound = False
 
@@ -54,6 +62,14 @@ class Spec:
         self.model = A.sbe_model.from_buffer_copy(pm.contents)
         self.warnings = lib.sbe_spec_warnings(self._h).decode()
 
+    def solver_model(self):
+        """The model as a params object (HaldaneModel / WannierModel, arrays copied)."""
+        from .params import HaldaneModel
+        m = self.model
+        if m.kind == A.SBE_MODEL_HALDANE:
+            return HaldaneModel(a0=m.a0, t1=m.t1, t2=m.t2, phi0=m.phi0, M0=m.M0, nval=m.nval)
+        return _wannier_from_c(m)
+
     def to_c(self):
         return self.config, self
 
@@ -76,6 +92,59 @@ class Spec:
 def _check(rc):
     if rc != A.SBE_OK:
         raise SBEError(rc, _lib().sbe_frontend_last_error().decode())
+
+
+def parse_atl(text: str | None = None, path=None) -> Spec:
+    """Read a .atl input (the format of the reference's src/inputFile.atl)."""
+    lib = _lib()
+    h = C.c_void_p()
+    if path is not None:
+        rc = lib.sbe_spec_parse_atl_file(str(path).encode(), C.byref(h))
+    else:
+        rc = lib.sbe_spec_parse_atl_string(text.encode(), C.byref(h))
+    if rc != A.SBE_OK:
+        raise SBEError(rc, lib.sbe_atl_last_error().decode())
+    return Spec(h)
+
+
+def _wannier_from_c(m):
+    from .params import WannierModel
+    N, nr = m.nband, m.nrpts
+    rvec = np.ctypeslib.as_array(m.rvec, shape=(nr, 3)).astype(np.int32)
+    rweight = np.ctypeslib.as_array(m.rweight, shape=(nr,)).copy()
+    h = np.ctypeslib.as_array(m.ham, shape=(nr, N, N, 2))
+    pos = None
+    if m.pos:
+        p = np.ctypeslib.as_array(m.pos, shape=(nr, 3, N, N, 2))
+        pos = p[..., 0] + 1j * p[..., 1]
+    return WannierModel(lattice=np.array(list(m.lattice)).reshape(3, 3), rvec=rvec, rweight=rweight,
+                        ham=h[..., 0] + 1j * h[..., 1], pos=pos, nval=m.nval, is_dipole_zero=bool(m.is_dipole_zero))
+
+
+def read_tb(path, nval: int = 0):
+    """Wannier90 seedname_tb.dat -> WannierModel (Hartree, bohr)."""
+    lib = _lib()
+    h = C.c_void_p()
+    rc = lib.sbe_wannier_read_tb(str(path).encode(), C.byref(h))
+    if rc != A.SBE_OK:
+        raise SBEError(rc, lib.sbe_wannier_last_error().decode())
+    try:
+        m = A.sbe_model()
+        rc = lib.sbe_wannier_tb_model(h, nval, C.byref(m))
+        if rc != A.SBE_OK:
+            raise SBEError(rc, lib.sbe_wannier_last_error().decode())
+        return _wannier_from_c(m)
+    finally:
+        lib.sbe_wannier_tb_free(h)
+
+
+def write_tb(path, model, header: str = "written by antelope_amd") -> None:
+    """WannierModel -> Wannier90 seedname_tb.dat (eV, Angstrom)."""
+    lib = _lib()
+    m, keep = model.to_c()
+    rc = lib.sbe_wannier_write_tb(str(path).encode(), C.byref(m), header.encode())
+    if rc != A.SBE_OK:
+        raise SBEError(rc, lib.sbe_wannier_last_error().decode())
 
 
 def parse_config(text: str | None = None, path: str | Path | None = None) -> Spec:

@@ -5,7 +5,8 @@
  *   sbe_spec_parse_file / _string  read a libconfig-format inputParam.cfg
  *       the way antelope.cpp:90-138, SBEs.h:125-215, SBEs.h:432-561 and
  *       Materials/Haldane.h:46-84 read it (defaults, degree and Angstrom
- *       conversions, required keys) into sbe_config + sbe_model.
+ *       conversions, required keys) into sbe_config + sbe_model; a
+ *       Wannier90 target reads its _tb.dat file (Wannier90.h:107-218).
  *   sbe_write_outputs  writes the reference's files (antelope.cpp:208-235,
  *       514-548): outlaserdata.dat, laserParameters.dat (empty, as the
  *       reference leaves it), interband_dipole_full_evol.dat,
@@ -34,6 +35,37 @@ int sbe_spec_get(const sbe_spec *spec, const sbe_config **cfg, const sbe_model *
 const char *sbe_spec_warnings(const sbe_spec *spec);
 void sbe_spec_free(sbe_spec *spec);
 const char *sbe_frontend_last_error(void);
+
+/* The .atl format of src/inputFile.atl (which the reference ships but does
+ * not read): BEGIN/END sections of `name = expression [unit]` lines.
+ * GLOBAL_VARIABLES are arithmetic expressions (units au, rad, ang, bohr,
+ * nm, microm, fs, as, eV, deg); SYSTEM E(k).model = MODEL_HALDANE_BANDS(la0,
+ * t1, t2, phi0, M0) and U_f(t).model = LASER_SIN2 | LASER_COS4 |
+ * LASER_GAUSS(E0, omega, ncycles [, ellip [, cep [, t0]]]); SETUP tgrid.step,
+ * kgrid.size (or kgrid.n1/n2/n3), T1, T2, InterIntra, offset.before/after;
+ * SOLVER tsolver = RUNGE_KUTTA_4() | RUNGE_KUTTA_5() | CASH_KARP(rtol, atol).
+ * Directives of solvers this library does not have (LASER_TOPHAT_SIN, ksolver
+ * = CRANK_NICOLSON(), kbox.*) return SBE_ERR_UNSUPPORTED naming them. */
+int sbe_spec_parse_atl_string(const char *text, sbe_spec **out);
+int sbe_spec_parse_atl_file(const char *path, sbe_spec **out);
+const char *sbe_atl_last_error(void);
+
+/* Wannier90 seedname_tb.dat (Materials/Wannier90.h:107-218): lattice,
+ * degeneracies, <0m|H|Rn> and <0m|r|Rn>, converted to Hartree and bohr; the
+ * position elements are read as re + i*im (the reference has re * i*im,
+ * Wannier90.h:173). A file without the position part gives
+ * is_dipole_zero = 1. sbe_wannier_tb_model points a model into the reader's
+ * arrays (valid until sbe_wannier_tb_free). In an inputParam.cfg, a target
+ * group with `dataName` (path; relative to the config file's directory when
+ * parsed from a file) and `Nval`, or `FermiE` in eV (Fermi-Dirac initial
+ * state, Nval ignored), selects such a model. sbe_wannier_write_tb writes a
+ * Wannier model in the same format (eV, Angstrom). */
+typedef struct sbe_wannier_tb sbe_wannier_tb;
+int sbe_wannier_read_tb(const char *path, sbe_wannier_tb **out);
+int sbe_wannier_tb_model(const sbe_wannier_tb *tb, int32_t nval, sbe_model *m);
+void sbe_wannier_tb_free(sbe_wannier_tb *tb);
+int sbe_wannier_write_tb(const char *path, const sbe_model *m, const char *header);
+const char *sbe_wannier_last_error(void);
 
 /* Write the reference's output files into `dir` from summed series (the
  * layout of sbe_get_series; several ranks' series added together). */

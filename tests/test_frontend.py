@@ -147,3 +147,65 @@ def test_output_files(tmp_path):
     assert laser[0, 0] == pytest.approx(atmin, rel=1e-6)
     line = (tmp_path / "outlaserdata.dat").read_text().splitlines()[1]
     assert re.fullmatch(r"-?\d\.\d{6}e[+-]\d{2}    (-?\d\.\d{6}e[+-]\d{2}      ){6}", line), line
+
+
+def _w90_model():
+    from antelope_amd.models import synthetic_wannier
+    return synthetic_wannier(4, nmax=(1, 1, 0), seed=77)
+
+
+def test_wannier_tb_round_trip(tmp_path):
+    """seedname_tb.dat as Wannier90 writes it (Wannier90.h:107-218): eV / Angstrom in the
+    file, Hartree / bohr in the model, weights 1/degeneracy, position elements re + i im."""
+    from antelope_amd.frontend import read_tb, write_tb
+    m = _w90_model()
+    m.rweight = np.where(np.arange(len(m.rweight)) % 3 == 0, 0.5, 1.0)   # some degeneracy 2
+    f = tmp_path / "wannier90_tb.dat"
+    write_tb(f, m, header="synthetic N=4")
+    lines = f.read_text().splitlines()
+    assert lines[0] == "synthetic N=4" and int(lines[4]) == 4 and int(lines[5]) == len(m.rweight)
+    np.testing.assert_allclose(float(lines[1].split()[0]), m.lattice[0, 0] * 0.529177210903, rtol=1e-15)
+    # the first Hamiltonian line is element (1, 1) of the first R in eV
+    first = next(l for l in lines[8:] if len(l.split()) == 4)
+    mm, nn, re, im = first.split()
+    assert (mm, nn) == ("1", "1")
+    np.testing.assert_allclose(float(re), m.ham[0, 0, 0].real * 27.211386245988, rtol=1e-15)
+    back = read_tb(f, nval=2)
+    np.testing.assert_array_equal(back.rvec, m.rvec)
+    np.testing.assert_allclose(back.rweight, m.rweight, rtol=1e-15)
+    np.testing.assert_allclose(back.lattice, m.lattice, rtol=1e-14)
+    np.testing.assert_allclose(back.ham, m.ham, rtol=1e-14, atol=1e-18)
+    np.testing.assert_allclose(back.pos, m.pos, rtol=1e-14, atol=1e-18)
+    assert back.nval == 2 and not back.is_dipole_zero
+
+
+def test_wannier_target_in_config(tmp_path):
+    from antelope_amd.frontend import write_tb
+    write_tb(tmp_path / "model_tb.dat", _w90_model())
+    cfg = BASE.replace('target = "Haldane"', 'target = "W90"') % "" + '\nW90 = { dataName = "model_tb.dat"; Nval = 2; };\n'
+    (tmp_path / "inputParam.cfg").write_text(cfg)
+    s = parse_config(path=tmp_path / "inputParam.cfg")
+    assert s.model.kind == A.SBE_MODEL_WANNIER and s.model.nband == 4 and s.model.nval == 2
+    wm = s.solver_model()
+    np.testing.assert_allclose(wm.ham, _w90_model().ham, rtol=1e-14, atol=1e-18)
+    fd = parse_config(path=tmp_path / "inputParam.cfg").config  # Nval path
+    assert fd.init_type == A.SBE_INIT_UNIFORM_VALENCE
+    (tmp_path / "fd.cfg").write_text(cfg.replace("Nval = 2;", "FermiE = 0.5;"))
+    c = parse_config(path=tmp_path / "fd.cfg").config
+    assert c.init_type == A.SBE_INIT_FERMI_DIRAC and c.fermi_energy == pytest.approx(0.5 / 27.211386245988)
+    with pytest.raises(SBEError) as e:
+        parse_config(cfg.replace('"model_tb.dat"', '"missing_tb.dat"'))
+    assert e.value.code == A.SBE_ERR_CONFIG
+
+
+def test_wannier_tb_errors(tmp_path):
+    from antelope_amd.frontend import read_tb
+    f = tmp_path / "bad_tb.dat"
+    f.write_text("header\n1 0 0\n0 1 0\n0 0 1\n2\n1\n1\n\n0 0 0\n1 1 0.1 0\n1 2 0 0\n2 1 0 0\n")
+    with pytest.raises(SBEError) as e:     # one Hamiltonian element short
+        read_tb(f)
+    assert e.value.code == A.SBE_ERR_MODEL
+    f.write_text("header\n1 0 0\n0 1 0\n0 0 1\n2\n1\n1\n\n0 0 0\n1 1 0.1 0\n1 2 0 0\n2 1 0 0\n2 2 -0.1 0\n")
+    m = read_tb(f, nval=1)                 # no position part: dipoles zero
+    assert m.is_dipole_zero and m.pos is None
+    np.testing.assert_allclose(m.ham[0].real, np.diag([0.1, -0.1]) / 27.211386245988, rtol=1e-15)
