@@ -40,7 +40,7 @@ ENVELOPES = {
 SBE_GAUGE_LENGTH_WANNIER, SBE_GAUGE_LENGTH_HAMILTONIAN, SBE_GAUGE_VELOCITY_HAMILTONIAN = range(3)
 SBE_GRID_REFERENCE, SBE_GRID_MP = 0, 1
 SBE_INIT_UNIFORM_VALENCE, SBE_INIT_FERMI_DIRAC = 0, 1
-SBE_PREC_FP64, SBE_PREC_FP32 = 0, 1
+SBE_PREC_FP64, SBE_PREC_FP32, SBE_PREC_SPLIT16 = 0, 1, 2
 SBE_RK4, SBE_RK5_FEHLBERG, SBE_RK_CASH_KARP = 4, 5, 45
 SBE_EFIELD_REFERENCE, SBE_EFIELD_CONSISTENT = 0, 1
 SBE_MODEL_HALDANE, SBE_MODEL_WANNIER = 1, 2

@@ -325,7 +325,8 @@ int sbe_spec_parse_string(const char *text, sbe_spec **out)
     }
     std::string s;
     if (get_string(calc, "grid", s)) c.grid = (s == "mp" || s == "MP") ? SBE_GRID_MP : SBE_GRID_REFERENCE;
-    if (get_string(calc, "precision", s)) c.precision = s == "fp32" ? SBE_PREC_FP32 : SBE_PREC_FP64;
+    if (get_string(calc, "precision", s))
+        c.precision = s == "fp32" ? SBE_PREC_FP32 : s == "split16" ? SBE_PREC_SPLIT16 : SBE_PREC_FP64;
     c.rtol = 1e-8;
     c.atol = 1e-12;
     get_double(calc, "rtol", c.rtol);

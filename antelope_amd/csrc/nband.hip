@@ -15,6 +15,8 @@
 //  * nb_rhs_kernel / nb_sample_kernel: one wave per k point; matrices live in
 //    LDS; the Hermitian eigenproblem is solved by a parallel-order complex
 //    Jacobi method (N/2 disjoint rotations per round).
+#include <cmath>
+
 #include "nband.hpp"
 
 namespace sbe {
@@ -183,6 +185,156 @@ __global__ __launch_bounds__(256) void nb_fourier_kernel(const NbArgs a, const N
                 if (k >= a.nk_stride || c >= ncols) continue;
                 const int mat = static_cast<int>(c / nn), ij = static_cast<int>(c % nn);
                 out[(static_cast<int64_t>(mat) * a.nk_stride + k) * nn + ij] = cmk(accr[i][j][q], acci[i][j][q]);
+            }
+}
+
+// ---------------------------------------- FP16-split Fourier GEMM (SPLIT16) --
+// Each real operand x (|x| <= 1 after a power-of-two scale) is carried as
+// hi = fp16(x), lo = fp16(x - hi), about 22 significant bits; a product is
+// hi*hi + hi*lo + lo*hi on v_mfma_f32_16x16x32_f16 with FP32 accumulation
+// (the lo*lo term is below FP32's resolution of the sum). The complex
+// product re += ar br - ai bi, im += ar bi + ai br is 12 MFMAs per 16 x 16
+// tile and K chunk of 32. Operand layout (probed on gfx950): A row = lane %
+// 16, B column = lane % 16, both with the same 8 k values per lane (group
+// lane / 16); D element r of lane l is row 4 (l / 16) + r, column l % 16.
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+constexpr int kSplitK = 32;
+
+struct NbScale {
+    double s[5];  // power-of-two scale of each matrix's B (|B s| <= 1)
+};
+
+__device__ __forceinline__ void split16(double x, uint16_t &hi, uint16_t &lo)
+{
+    const _Float16 h = static_cast<_Float16>(static_cast<float>(x));
+    const _Float16 l = static_cast<_Float16>(static_cast<float>(x - static_cast<double>(h)));
+    hi = __builtin_bit_cast(uint16_t, h);
+    lo = __builtin_bit_cast(uint16_t, l);
+}
+
+__global__ void nb_psplit_kernel(const NbArgs a, const cd *__restrict__ P, uint16_t *__restrict__ p16)
+{
+    const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t plane = a.nk_stride * a.nr_pad;
+    if (idx >= plane) return;
+    const int64_t k = idx / a.nr_pad;
+    const int r = static_cast<int>(idx % a.nr_pad);
+    const cd v = r < a.nr ? P[k * a.nr + r] : cmk(0.0, 0.0);
+    uint16_t h, l;
+    split16(v.x, h, l);
+    p16[idx] = h;
+    p16[plane + idx] = l;
+    split16(v.y, h, l);
+    p16[2 * plane + idx] = h;
+    p16[3 * plane + idx] = l;
+}
+
+__global__ void nb_bsplit_kernel(const NbArgs a, const NbEval ev, const NbScale sc, uint16_t *__restrict__ b16)
+{
+    const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t plane = a.bcols * a.nr_pad;
+    if (idx >= plane) return;
+    const int64_t c = idx / a.nr_pad;
+    const int r = static_cast<int>(idx % a.nr_pad);
+    const int nn = a.N * a.N;
+    cd v = cmk(0.0, 0.0);
+    if (r < a.nr && c < static_cast<int64_t>(ev.nmat) * nn) {
+        const double *R = a.rcart + 3 * r;
+        double sn, cs;
+        sincos(ev.A[0] * R[0] + ev.A[1] * R[1] + ev.A[2] * R[2], &sn, &cs);
+        v = cscale(b_element(a, ev, r, c, cmk(cs, sn)), sc.s[c / nn]);
+    }
+    uint16_t h, l;
+    split16(v.x, h, l);
+    b16[idx] = h;
+    b16[plane + idx] = l;
+    split16(v.y, h, l);
+    b16[2 * plane + idx] = h;
+    b16[3 * plane + idx] = l;
+}
+
+__device__ __forceinline__ h8 neg8(h8 v)
+{
+    typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+    return __builtin_bit_cast(h8, __builtin_bit_cast(u4, v) ^ u4{0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u});
+}
+
+// 64 k x 64 column tile per block of 4 waves; wave w owns the 32 x 32
+// sub-tile (w / 2, w % 2) as 2 x 2 MFMA tiles of 16 x 16. Operands are read
+// straight from the plane arrays (16-byte loads of 8 consecutive R).
+__global__ __launch_bounds__(256) void nb_fourier16_kernel(const NbArgs a, const NbEval ev, const NbScale sc,
+                                                           const uint16_t *__restrict__ p16,
+                                                           const uint16_t *__restrict__ b16, cd *__restrict__ out)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int64_t k0 = static_cast<int64_t>(blockIdx.x) * 64 + 32 * wm;
+    const int64_t c0 = static_cast<int64_t>(blockIdx.y) * 64 + 32 * wn;
+    const int nn = a.N * a.N;
+    const int64_t ncols = static_cast<int64_t>(ev.nmat) * nn;
+    const int64_t PS = a.nk_stride * a.nr_pad, BS = a.bcols * a.nr_pad;
+    f4 accr[2][2], acci[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            accr[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+            acci[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+        }
+    for (int r0 = 0; r0 < a.nr_pad; r0 += kSplitK) {
+        const int kk = r0 + 8 * (lane >> 4);
+        h8 A[2][4], B[2][4];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int64_t row = k0 + 16 * i + (lane & 15);
+#pragma unroll
+            for (int pl = 0; pl < 4; ++pl)
+                A[i][pl] = *reinterpret_cast<const h8 *>(p16 + pl * PS + row * a.nr_pad + kk);
+        }
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int64_t col = c0 + 16 * j + (lane & 15);
+#pragma unroll
+            for (int pl = 0; pl < 4; ++pl)
+                B[j][pl] = *reinterpret_cast<const h8 *>(b16 + pl * BS + col * a.nr_pad + kk);
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const h8 nih = neg8(A[i][2]), nil = neg8(A[i][3]);
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                f4 re = accr[i][j], im = acci[i][j];
+                re = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[i][0], B[j][0], re, 0, 0, 0);
+                re = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[i][0], B[j][1], re, 0, 0, 0);
+                re = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[i][1], B[j][0], re, 0, 0, 0);
+                re = __builtin_amdgcn_mfma_f32_16x16x32_f16(nih, B[j][2], re, 0, 0, 0);
+                re = __builtin_amdgcn_mfma_f32_16x16x32_f16(nih, B[j][3], re, 0, 0, 0);
+                re = __builtin_amdgcn_mfma_f32_16x16x32_f16(nil, B[j][2], re, 0, 0, 0);
+                im = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[i][0], B[j][2], im, 0, 0, 0);
+                im = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[i][0], B[j][3], im, 0, 0, 0);
+                im = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[i][1], B[j][2], im, 0, 0, 0);
+                im = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[i][2], B[j][0], im, 0, 0, 0);
+                im = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[i][2], B[j][1], im, 0, 0, 0);
+                im = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[i][3], B[j][0], im, 0, 0, 0);
+                accr[i][j] = re;
+                acci[i][j] = im;
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int64_t k = k0 + 16 * i + 4 * (lane >> 4) + q;
+                const int64_t c = c0 + 16 * j + (lane & 15);
+                if (c >= ncols) continue;
+                const int mat = static_cast<int>(c / nn), ij = static_cast<int>(c % nn);
+                const double inv = 1.0 / sc.s[mat];
+                out[(static_cast<int64_t>(mat) * a.nk_stride + k) * nn + ij] =
+                    cmk(static_cast<double>(accr[i][j][q]) * inv, static_cast<double>(acci[i][j][q]) * inv);
             }
 }
 
@@ -539,9 +691,31 @@ hipError_t nb_launch_phase(const NbArgs &a, double2 *P, hipStream_t s)
     return hipGetLastError();
 }
 
+hipError_t nb_launch_psplit(const NbArgs &a, const double2 *P, uint16_t *p16, hipStream_t s)
+{
+    const int64_t total = a.nk_stride * a.nr_pad;
+    nb_psplit_kernel<<<dim3(static_cast<unsigned>((total + 255) / 256)), 256, 0, s>>>(a, P, p16);
+    return hipGetLastError();
+}
+
 hipError_t nb_launch_fourier(const NbArgs &a, const NbEval &ev, const double2 *P, double2 *out, hipStream_t s)
 {
     const int64_t ncols = static_cast<int64_t>(ev.nmat) * a.N * a.N;
+    if (a.split16) {
+        NbScale sc{};
+        const double emag = std::fabs(ev.E[0]) + std::fabs(ev.E[1]) + std::fabs(ev.E[2]);
+        for (int m = 0; m < 5; ++m) {
+            double bound = a.hmax;  // H0
+            if (m < ev.nmat && ev.mats[m] == kMatHtilde) bound = a.hmax + emag * a.pmax;
+            if (m < ev.nmat && ev.mats[m] >= kMatJx) bound = a.rmax * a.hmax;
+            sc.s[m] = bound > 0 ? std::ldexp(1.0, -static_cast<int>(std::ceil(std::log2(bound)))) : 1.0;
+        }
+        const int64_t total = a.bcols * a.nr_pad;
+        nb_bsplit_kernel<<<dim3(static_cast<unsigned>((total + 255) / 256)), 256, 0, s>>>(a, ev, sc, a.b16);
+        const dim3 grid(static_cast<unsigned>(a.nk_stride / 64), static_cast<unsigned>((ncols + 63) / 64));
+        nb_fourier16_kernel<<<grid, 256, 0, s>>>(a, ev, sc, a.p16, a.b16, out);
+        return hipGetLastError();
+    }
     const dim3 grid(static_cast<unsigned>((a.nk_stride + GM - 1) / GM), static_cast<unsigned>((ncols + GN - 1) / GN));
     nb_fourier_kernel<<<grid, 256, 0, s>>>(a, ev, P, out);
     return hipGetLastError();

@@ -30,6 +30,13 @@ struct NbArgs {
     const double *rweight; // [nr] 1/degeneracy
     const double2 *ham;    // [nr][N][N]
     const double2 *pos;    // [nr][3][N][N]
+    // FP16-split Fourier sums (SBE_PREC_SPLIT16): halves as raw 16-bit words
+    int split16;
+    int nr_pad;            // nr rounded up to the MFMA K (32)
+    int64_t bcols;         // columns of the B planes (4 N^2 rounded up to 64)
+    const uint16_t *p16;   // e^{ik.R}: planes [re_hi, re_lo, im_hi, im_lo][nk_stride][nr_pad]
+    uint16_t *b16;         // B of one evaluation: planes [re_hi, re_lo, im_hi, im_lo][bcols][nr_pad]
+    double hmax, pmax, rmax;  // max |H(R)|, max |r(R)|, max |R_a| (scales of B)
 };
 
 // One evaluation of the Fourier sums at k + A(t) with field E(t).
@@ -49,6 +56,8 @@ struct NbStage {
 
 size_t nb_lds_bytes(int N);
 hipError_t nb_launch_phase(const NbArgs &a, double2 *P, hipStream_t s);
+// SPLIT16: e^{ik.R} split into FP16 hi/lo planes once per context
+hipError_t nb_launch_psplit(const NbArgs &a, const double2 *P, uint16_t *p16, hipStream_t s);
 hipError_t nb_launch_fourier(const NbArgs &a, const NbEval &ev, const double2 *P, double2 *out, hipStream_t s);
 hipError_t nb_launch_init(const NbArgs &a, const double2 *H0, double2 *rho, double2 *U, hipStream_t s);
 hipError_t nb_launch_sample(const NbArgs &a, const double2 *mats, const double2 *rho, double *partials,

@@ -44,10 +44,12 @@ struct NbPlan {
     int64_t nk_stride;
     int sample_waves, nv;
     int64_t chunk;
-    size_t bytes_P, bytes_rho, bytes_mats, bytes_k, bytes_tab, bytes_partials, bytes_series;
+    int nr_pad;
+    int64_t bcols;
+    size_t bytes_P, bytes_rho, bytes_mats, bytes_k, bytes_tab, bytes_partials, bytes_series, bytes_split;
 };
 
-NbPlan nb_plan(int N, int nr, int64_t nk, int64_t nt, int nstage)
+NbPlan nb_plan(int N, int nr, int64_t nk, int64_t nt, int nstage, bool split16)
 {
     NbPlan p{};
     p.nk_stride = (nk + 63) / 64 * 64;
@@ -64,15 +66,22 @@ NbPlan nb_plan(int N, int nr, int64_t nk, int64_t nt, int nstage)
                   static_cast<size_t>(nr) * 4 * nn * sizeof(double2);
     p.bytes_partials = static_cast<size_t>(p.chunk) * row;
     p.bytes_series = static_cast<size_t>(nt) * p.nv * sizeof(double);
+    p.nr_pad = (nr + 31) / 32 * 32;
+    p.bcols = (4 * static_cast<int64_t>(nn) + 63) / 64 * 64;
+    // FP16 hi/lo planes of e^{ik.R} (per context) and of B (per evaluation)
+    p.bytes_split = split16 ? 4 * sizeof(uint16_t) * static_cast<size_t>(p.nr_pad) *
+                                  static_cast<size_t>(p.nk_stride + p.bcols)
+                            : 0;
     return p;
 }
 
 }  // namespace
 
-size_t nb_footprint(const sbe_model &m, int64_t nk, int64_t nt, int nstage)
+size_t nb_footprint(const sbe_model &m, int64_t nk, int64_t nt, int nstage, int precision)
 {
-    const NbPlan p = nb_plan(m.nband, m.nrpts, nk, nt, nstage);
-    return p.bytes_P + p.bytes_rho + p.bytes_mats + p.bytes_k + p.bytes_tab + p.bytes_partials + p.bytes_series;
+    const NbPlan p = nb_plan(m.nband, m.nrpts, nk, nt, nstage, precision == SBE_PREC_SPLIT16);
+    return p.bytes_P + p.bytes_rho + p.bytes_mats + p.bytes_k + p.bytes_tab + p.bytes_partials + p.bytes_series +
+           p.bytes_split;
 }
 
 int nb_validate(const sbe_model &m, std::string &err)
@@ -105,7 +114,8 @@ int nb_setup(sbe_ctx *ctx)
     const sbe_model &m = ctx->model;
     const int N = m.nband, nr = m.nrpts;
     const int64_t nk = ctx->grid.k_end - ctx->grid.k_begin;
-    const NbPlan p = nb_plan(N, nr, nk, ctx->laser.nt, ctx->tab.nstage);
+    const bool split16 = ctx->cfg.precision == SBE_PREC_SPLIT16;
+    const NbPlan p = nb_plan(N, nr, nk, ctx->laser.nt, ctx->tab.nstage, split16);
     const size_t nn = static_cast<size_t>(N) * N;
     ctx->nk = nk;
     ctx->nk_stride = p.nk_stride;
@@ -120,6 +130,7 @@ int nb_setup(sbe_ctx *ctx)
     char *tab = nalloc<char>(ctx, p.bytes_tab, rc);
     ctx->d_partials = nalloc<double>(ctx, p.bytes_partials, rc);
     ctx->d_series = nalloc<double>(ctx, p.bytes_series, rc);
+    uint16_t *split = split16 ? nalloc<uint16_t>(ctx, p.bytes_split, rc) : nullptr;
     if (rc) return rc;
 
     // host copies of the tables: k [nk_stride][3], w, R cartesian, weights, H, r
@@ -166,9 +177,21 @@ int nb_setup(sbe_ctx *ctx)
     a.rweight = dr + 3 * nr;
     a.ham = dham;
     a.pos = dpos;
+    a.split16 = split16 ? 1 : 0;
+    a.nr_pad = p.nr_pad;
+    a.bcols = p.bcols;
+    a.p16 = split;
+    a.b16 = split ? split + 4 * static_cast<size_t>(p.nr_pad) * p.nk_stride : nullptr;
+    a.hmax = a.pmax = a.rmax = 0.0;
+    for (size_t e = 0; e < static_cast<size_t>(nr) * nn; ++e)
+        a.hmax = std::max(a.hmax, std::hypot(m.ham[2 * e], m.ham[2 * e + 1]));
+    for (size_t e = 0; e < static_cast<size_t>(nr) * 3 * nn; ++e)
+        a.pmax = std::max(a.pmax, std::hypot(m.pos[2 * e], m.pos[2 * e + 1]));
+    for (int e = 0; e < 3 * nr; ++e) a.rmax = std::max(a.rmax, std::fabs(hr[e]));
 
     // e^{ik.R}, then rho(k) from H(k) (A(atmin) = 0)
     rc = hipc(ctx, nb_launch_phase(a, ctx->nb_P, ctx->stream), "phase kernel");
+    if (!rc && split16) rc = hipc(ctx, nb_launch_psplit(a, ctx->nb_P, split, ctx->stream), "phase split kernel");
     NbEval ev{};
     ev.nmat = 1;
     ev.mats[0] = kMatH0;

@@ -83,9 +83,17 @@ int validate(const sbe_config *cfg, const sbe_model *m, std::string &err)
         err = "only the LengthWannier gauge is built";
         return SBE_ERR_UNSUPPORTED;
     }
-    if (cfg->precision != SBE_PREC_FP64 && cfg->precision != SBE_PREC_FP32) {
+    if (cfg->precision != SBE_PREC_FP64 && cfg->precision != SBE_PREC_FP32 && cfg->precision != SBE_PREC_SPLIT16) {
         err = "unknown precision";
         return SBE_ERR_CONFIG;
+    }
+    if (cfg->precision == SBE_PREC_SPLIT16 && m->kind != SBE_MODEL_WANNIER) {
+        err = "SPLIT16 precision applies to the Wannier Fourier sums";
+        return SBE_ERR_UNSUPPORTED;
+    }
+    if (cfg->precision == SBE_PREC_FP32 && m->kind == SBE_MODEL_WANNIER) {
+        err = "the Wannier path runs FP64 or SPLIT16 (FP16-split MFMA Fourier sums)";
+        return SBE_ERR_UNSUPPORTED;
     }
     if (m->kind == SBE_MODEL_HALDANE) {
         if (m->nband != 2) {
@@ -283,7 +291,7 @@ int sbe_query_footprint(const sbe_config *cfg, const sbe_model *model, int64_t k
     }
     if (k_begin < 0 || k_end > ntot || k_begin >= k_end) return set_error(nullptr, SBE_ERR_RANGE, "bad k range");
     if (model->kind == SBE_MODEL_WANNIER) {
-        if (bytes) *bytes = sbe::nb_footprint(*model, k_end - k_begin, L.nt, tab.nstage);
+        if (bytes) *bytes = sbe::nb_footprint(*model, k_end - k_begin, L.nt, tab.nstage, cfg->precision);
         return SBE_OK;
     }
     Plan p;

@@ -104,7 +104,7 @@ class RunConfig:
             org = list(self.bz_origin) + [0.0] * (3 - len(self.bz_origin))
             c.bz_origin = (C.c_double * 3)(*org)
         c.init_type = {"uniform": A.SBE_INIT_UNIFORM_VALENCE, "fermi": A.SBE_INIT_FERMI_DIRAC}[self.init]
-        c.precision = {"fp64": A.SBE_PREC_FP64, "fp32": A.SBE_PREC_FP32}[self.precision]
+        c.precision = {"fp64": A.SBE_PREC_FP64, "fp32": A.SBE_PREC_FP32, "split16": A.SBE_PREC_SPLIT16}[self.precision]
         c.fermi_energy = self.fermi_energy
         c.npulses = len(self.pulses)
         c.efield_mode = {"reference": A.SBE_EFIELD_REFERENCE, "consistent": A.SBE_EFIELD_CONSISTENT}[self.efield]

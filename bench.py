@@ -129,13 +129,28 @@ def cpu_baseline(cfg, model, budget_s: float):
 
 
 def bench_wannier(args) -> int:
-    """N-band path on the Wannier workload: times `steps` steps (default 50)."""
-    cfg, model = wannier_workload("fp64")
-    steps = args.steps if args.steps > 0 else 50
-    r = timed_solve(cfg, model, steps, min(args.warmup, 5))
-    ks = r["nk"] * r["steps"] / r["wall_s"]
-    # algorithmic flops per k-point RK4 step: 4 stages x (H + 3 D
-    # Fourier sums, 8 N^2 N_R each, + H rho 8 N^3) + the sample's 3 J + H sums
+    """N-band path on the Wannier workload: times `steps` steps (default 200) of
+    the FP64 form (Fourier sums on FP64 MFMA) and of the SPLIT16 form (Fourier
+    sums on FP16 MFMA with hi/lo operands, FP32 accumulation), and reports the
+    SPLIT16 series' deviation from the FP64 one over those steps."""
+    steps = args.steps if args.steps > 0 else 200
+    forms, res = {}, {}
+    for prec in ("fp64", "split16"):
+        cfg, model = wannier_workload(prec)
+        r = timed_solve(cfg, model, steps, min(args.warmup, 5))
+        res[prec] = r
+        forms[prec] = {"k_steps_per_s": r["nk"] * r["steps"] / r["wall_s"], "wall_s": r["wall_s"],
+                       "ms_per_step": r["wall_s"] * 1e3 / r["steps"]}
+        print(f"[bench] wannier16 {prec}: {forms[prec]['ms_per_step']:.2f} ms/step", file=sys.stderr, flush=True)
+    a, b = res["fp64"]["series"], res["split16"]["series"]
+    n = res["fp64"]["steps"]
+    forms["split16"]["err_j_rel_l2_vs_fp64"] = rel_l2((b["j_inter"] + b["j_intra"])[:n], (a["j_inter"] + a["j_intra"])[:n])
+    forms["split16"]["err_pop_rel_l2_vs_fp64"] = rel_l2(b["band_pop"][:n], a["band_pop"][:n])
+    r = res["fp64"]
+    ks = forms["fp64"]["k_steps_per_s"]
+    # algorithmic flops per k-point RK4 step: 4 stages x (H + 3 D Fourier sums, 8 N^2 N_R each,
+    # + H rho 8 N^3) + the sample's 3 J + H sums and its 8 N^3 products
+    model = wannier_workload("fp64")[1]
     N, nr = 16, int(model.rvec.shape[0])
     fsum = 8 * N * N * nr
     flops = 4 * (4 * fsum + 8 * N ** 3) + 4 * fsum + 8 * 8 * N ** 3
@@ -145,52 +160,15 @@ def bench_wannier(args) -> int:
             "data": "synthetic (seeded SplitMix64 Wannier N=16 model, 147 R)",
             "config": {"workload": f"Wannier N=16, 147 R, 48x48x8 k grid, sin^2 8-cycle pulse, RK4 dt=0.1, "
                                    f"T2=220, Nt={r['info'].nt} (first {r['steps']} steps timed)", "nk": r["nk"]},
+            "forms": forms,
             "algorithmic_flops_per_kstep": flops, "achieved_tflops": ks * flops / 1e12,
+            "roofline": {"bound": "mfma", "achieved": ks * flops / 1e12, "peak": PEAK_TFLOPS["fp64"],
+                         "unit": "TFLOP/s", "frac": ks * flops / 1e12 / PEAK_TFLOPS["fp64"], "traffic": None,
+                         "note": "algorithmic flops of the whole step over the step time; the Fourier sums "
+                                 "are 94% of them, the eigensolver flops are not counted"},
             "timing": r["timing"]}
     print(json.dumps(line), flush=True)
     return 0
-
-
-def bench_serving(args) -> int:
-    """Continuous-batching request loop on a seeded Poisson stream (20 requests/s):
-    Haldane solves on 100^2..400^2 grids with 4..20-cycle pulses, 10% synthetic
-    Wannier solves (N in {8, 16, 32}) at reduced grid sizes. Reports aggregate
-    k-point RK steps/s over the whole stream and p50/p99 submit-to-done latency."""
-    # one hardware queue per in-flight request (HIP's default is 4 per process): requests on
-    # a shared queue wait behind each other's kernels; must be set before the first HIP call
-    os.environ["GPU_MAX_HW_QUEUES"] = str(min(16, args.max_active))
-    from antelope_amd.serve import SBEServer, run_stream, serving_stream
-
-    n = args.requests
-    reqs = serving_stream(n, rate=20.0, seed=5, wannier_share=0.1)
-    # warm the kernels (first-launch code object loading) outside the stream
-    warm = serving_stream(2, rate=1e3, seed=11, wannier_share=0.5, grid_range=(16, 16), cycles=(1, 1),
-                          wannier_cycles=(1, 1))
-    with SBEServer(max_active=args.max_active, slice_steps=args.slice_steps) as srv:
-        run_stream(srv, warm, realtime=False)
-        out = run_stream(srv, reqs, realtime=True)
-    kinds = {}
-    for r in out["records"]:
-        kinds.setdefault(r["kind"], []).append(r["latency_s"])
-    print(f"[bench] serving: {out['completed']} done, {out['failed']} failed in {out['wall_s']:.2f} s, "
-          f"max {out['max_active']} active; " +
-          ", ".join(f"{k}: {len(v)} (p50 {np.percentile(v, 50):.3f} s)" for k, v in sorted(kinds.items())),
-          file=sys.stderr, flush=True)
-    line = {"metric": METRIC, "value": out["k_steps_per_s"], "unit": "k-point·RK-steps/s", "n_gpus": 1,
-            "steps": n, "warmup": len(warm), "ms_per_step": out["wall_s"] * 1e3 / n,
-            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-            "data": "synthetic (seeded SplitMix64 request stream, models and pulses)",
-            "config": {"workload": f"serving: {n} requests, Poisson 20/s, Haldane 100^2..400^2 grids, "
-                                   f"4..20 cycles, E0 log-uniform [0.001, 0.006]; 10% Wannier N in {{8,16,32}} "
-                                   f"on 8^2..16^2 grids, 2..4 cycles", "max_active": args.max_active,
-                       "slice_steps": args.slice_steps, "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES")},
-            "latency_p50_s": out["p50_s"], "latency_p99_s": out["p99_s"],
-            "latency_by_kind": {k: {"n": len(v), "p50_s": float(np.percentile(v, 50)),
-                                    "p99_s": float(np.percentile(v, 99))} for k, v in sorted(kinds.items())},
-            "completed": out["completed"], "failed": out["failed"], "max_active_seen": out["max_active"],
-            "wall_s": out["wall_s"]}
-    print(json.dumps(line), flush=True)
-    return 0 if out["failed"] == 0 else 1
 
 
 def _timed_batch(cfgs, models):

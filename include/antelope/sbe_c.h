@@ -95,8 +95,11 @@ enum { SBE_GRID_REFERENCE = 0, SBE_GRID_MP = 1 };
 enum { SBE_INIT_UNIFORM_VALENCE = 0, SBE_INIT_FERMI_DIRAC = 1 };
 
 /* Arithmetic of the propagated state. The k-sums of currents and
- * populations and the laser tables are FP64 in every mode. */
-enum { SBE_PREC_FP64 = 0, SBE_PREC_FP32 = 1 };
+ * populations and the laser tables are FP64 in every mode. FP32: 2-band
+ * models, FP32 state (Kahan-compensated). SPLIT16: Wannier models, the
+ * Fourier sums H(k+A), D, J on FP16 MFMA with split (hi + lo) operands and
+ * FP32 accumulation; state, eigensolver and currents stay FP64. */
+enum { SBE_PREC_FP64 = 0, SBE_PREC_FP32 = 1, SBE_PREC_SPLIT16 = 2 };
 
 /* Time stepper: RKorder = 4 (classical RK4) or 5 (fixed-step 6-stage
  * Fehlberg weights) as in SBEs.h:395-416. SBE_RK_CASH_KARP is the adaptive
