@@ -342,20 +342,38 @@ __global__ __launch_bounds__(256) void nb_fourier16_kernel(const NbArgs a, const
 // All matrices are N x N complex, row-major, in LDS; one wave (64 lanes).
 __device__ __forceinline__ void wave_sync() { __syncthreads(); }
 
-// C = op(A) op(B); adjA/adjB select the conjugate transpose.
+// C = op(A) op(B); adjA/adjB select the conjugate transpose. On FP64 MFMA:
+// 16 x 16 output tiles (ceil(N/16)^2 of them), K in steps of 4, the complex
+// product as four real v_mfma_f64_16x16x4f64 (layout as in the Fourier GEMM:
+// A row = lane % 16, k = lane / 16; B k = lane / 16, column = lane % 16; D
+// element r of lane l is row l / 16 + 4 r, column l % 16). Operands read
+// from LDS once per k step instead of once per output element; entries
+// beyond N read as zero. C must not alias A or B.
 __device__ void wmm(cd *C, const cd *A, const cd *B, int N, bool adjA, bool adjB)
 {
-    const int nn = N * N;
-    for (int e = threadIdx.x; e < nn; e += 64) {
-        const int i = e / N, j = e % N;
-        cd s = cmk(0.0, 0.0);
-        for (int l = 0; l < N; ++l) {
-            const cd x = adjA ? cconj(A[l * N + i]) : A[i * N + l];
-            const cd y = adjB ? cconj(B[j * N + l]) : B[l * N + j];
-            cfma(s, x, y);
+    const int lane = threadIdx.x;
+    const int r16 = lane & 15, g = lane >> 4;
+    const int T = (N + 15) >> 4, K4 = (N + 3) >> 2;
+    for (int ti = 0; ti < T; ++ti)
+        for (int tj = 0; tj < T; ++tj) {
+            d4 re = d4{0.0, 0.0, 0.0, 0.0}, im = d4{0.0, 0.0, 0.0, 0.0};
+            const int i = 16 * ti + r16, j = 16 * tj + r16;
+            for (int kb = 0; kb < K4; ++kb) {
+                const int l = 4 * kb + g;
+                cd x = cmk(0.0, 0.0), y = cmk(0.0, 0.0);
+                if (i < N && l < N) x = adjA ? cconj(A[l * N + i]) : A[i * N + l];
+                if (l < N && j < N) y = adjB ? cconj(B[j * N + l]) : B[l * N + j];
+                re = __builtin_amdgcn_mfma_f64_16x16x4f64(x.x, y.x, re, 0, 0, 0);
+                re = __builtin_amdgcn_mfma_f64_16x16x4f64(-x.y, y.y, re, 0, 0, 0);
+                im = __builtin_amdgcn_mfma_f64_16x16x4f64(x.x, y.y, im, 0, 0, 0);
+                im = __builtin_amdgcn_mfma_f64_16x16x4f64(x.y, y.x, im, 0, 0, 0);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = 16 * ti + g + 4 * r, col = 16 * tj + r16;
+                if (row < N && col < N) C[row * N + col] = cmk(re[r], im[r]);
+            }
         }
-        C[e] = s;
-    }
     wave_sync();
 }
 
