@@ -8,6 +8,25 @@
 namespace sbe {
 namespace dev {
 
+// ------------------------------------------- RK tableaus (SBEs.h:395-416)
+template <int RK> struct Coef;
+template <> struct Coef<4> {
+    static constexpr int S = 4;
+    static constexpr double a[4][4] = {{0, 0, 0, 0}, {0.5, 0, 0, 0}, {0, 0.5, 0, 0}, {0, 0, 1.0, 0}};
+    static constexpr double b[4] = {1.0 / 6.0, 2.0 / 6.0, 2.0 / 6.0, 1.0 / 6.0};
+};
+template <> struct Coef<5> {
+    static constexpr int S = 6;
+    static constexpr double a[6][6] = {
+        {0, 0, 0, 0, 0, 0},
+        {0.25, 0, 0, 0, 0, 0},
+        {3. / 32., 9. / 32., 0, 0, 0, 0},
+        {1932. / 2197., -7200. / 2197., 7296. / 2197., 0, 0, 0},
+        {439. / 216., -8., 3680. / 513., -845. / 4104., 0, 0},
+        {-8. / 27., 2., -3544. / 2565., 1859. / 4104., -11. / 40., 0}};
+    static constexpr double b[6] = {16. / 135., 0, 6656. / 12825., 28561. / 56430., -9. / 50., 2. / 55.};
+};
+
 // --------------------------------------------------- wave-level reduction
 template <int CTRL>
 __device__ __forceinline__ double dpp_f64(double x)

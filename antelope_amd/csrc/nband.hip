@@ -694,6 +694,209 @@ __global__ __launch_bounds__(64) void nb_rhs_kernel(const NbArgs a, const NbStag
     }
 }
 
+// ------------------------------------------------- velocity gauge ----
+// Set-up of the VelocityHamiltonian gauge (SBEs.h:284-373): per k,
+// U = eig(H(k)) with the column phases fixed (column j times
+// conj(U[p][j]) / |U[p][j]|, p the first row with |U[p][j]|^2 >= 1/(2N)),
+// E(k), pMatrix_a = U^dag J_a(k) U, occupations, rho = diag(occupations).
+__global__ __launch_bounds__(64) void nb_vel_init_kernel(const NbArgs a, const cd *__restrict__ mats,
+                                                         cd *__restrict__ P, double *__restrict__ Eocc,
+                                                         cd *__restrict__ Ust, cd *__restrict__ rho)
+{
+    extern __shared__ double4 smem4[];
+    const int N = a.N, nn = N * N;
+    cd *base = reinterpret_cast<cd *>(smem4);
+    cd *A = mat_slot(base, 0, nn), *U = mat_slot(base, 1, nn), *W = mat_slot(base, 2, nn);
+    cd *T = mat_slot(base, 3, nn), *X = mat_slot(base, 4, nn);
+    double *ev = reinterpret_cast<double *>(mat_slot(base, 6, nn));
+    double4 *rot = reinterpret_cast<double4 *>(ev + 64);
+    const int64_t k = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int64_t ks = a.nk_stride * nn;
+    if (k >= a.nk) {  // padding: zero everything the step kernel could read
+        for (int e = lane; e < nn; e += 64) {
+            rho[k * nn + e] = cmk(0.0, 0.0);
+            Ust[k * nn + e] = cmk(0.0, 0.0);
+            for (int ax = 0; ax < 3; ++ax) P[ax * ks + k * nn + e] = cmk(0.0, 0.0);
+        }
+        if (lane < 2 * N) Eocc[k * 2 * N + lane] = 0.0;
+        return;
+    }
+    wcopy(A, mats + k * nn, nn);
+    weigh(A, U, ev, W, N, rot, false);
+    if (lane < N) {
+        const int j = lane;
+        int p = 0;
+        while (p < N - 1 && (U[p * N + j].x * U[p * N + j].x + U[p * N + j].y * U[p * N + j].y) * (2.0 * N) < 1.0) ++p;
+        const cd u = U[p * N + j];
+        const double au = sqrt(u.x * u.x + u.y * u.y);
+        if (au > 0.0) {
+            const cd ph = cmk(u.x / au, -u.y / au);
+            for (int r = 0; r < N; ++r) U[r * N + j] = cmul(U[r * N + j], ph);
+            U[p * N + j] = cmk(au, 0.0);
+        }
+    }
+    wave_sync();
+    for (int e = lane; e < nn; e += 64) Ust[k * nn + e] = U[e];
+    for (int ax = 0; ax < 3; ++ax) {
+        wcopy(X, mats + (1 + ax) * ks + k * nn, nn);
+        wmm(T, X, U, N, false, false);
+        wmm(W, U, T, N, true, false);  // U^dag J_a U
+        for (int e = lane; e < nn; e += 64) P[ax * ks + k * nn + e] = W[e];
+    }
+    __shared__ double occ[64];
+    if (lane == 0) {
+        double sum = 0.0;
+        for (int m = 0; m < N; ++m) {
+            double f;
+            if (a.fermi) f = 1.0 / (exp((ev[m] - a.fermi_energy) / (300.0 / 3.15775024804e5)) + 1.0);
+            else f = m < a.nval ? 1.0 : 0.0;
+            occ[m] = f;
+            sum += f;
+        }
+        if (a.fermi)
+            for (int m = 0; m < N; ++m) occ[m] /= sum;
+    }
+    wave_sync();
+    if (lane < N) {
+        Eocc[k * 2 * N + lane] = ev[lane];
+        Eocc[k * 2 * N + N + lane] = occ[lane];
+    }
+    for (int e = lane; e < nn; e += 64) {
+        const int i = e / N, j = e % N;
+        rho[k * nn + e] = cmk(i == j ? occ[i] : 0.0, 0.0);
+    }
+}
+
+// One time step of the velocity gauge for every k: the sample at t_n
+// (SBEs.h:694-760 with _tkp = k: rho is band-basis and U^dag J U is pMatrix;
+// the Nval A(t) term is added by the host), then the RK step with
+//   f(y) = -i((H y) - (H y)^dag) - (y - rho0) o Gamma,  H = sum_a A_a P_a + diag(E)
+// (SBEs.h:596-624, 787-805, 814-829, 862-871), in the reference's order of
+// operations. Each wave walks k = wave, wave + G, ...: rho of its k point in
+// registers (EPL elements per lane), the stage input, H, H y and the stage
+// derivatives in LDS, products on FP64 MFMA (wmm). One row of partial sums
+// per wave: [inter(3) intra(3) pop(N) rho(2 N^2)].
+template <int EPL>
+__global__ __launch_bounds__(64) void nb_vel_step_kernel(const NbArgs a, const NbVelStep v, const cd *__restrict__ P,
+                                                         const double *__restrict__ Eocc, cd *__restrict__ rho,
+                                                         double *__restrict__ partials, int64_t step_local, int p_lds)
+{
+    extern __shared__ double4 smem4[];
+    const int N = a.N, nn = N * N, S = v.nstage;
+    const int lane = threadIdx.x;
+    const int64_t ks = a.nk_stride * nn;
+    cd *base = reinterpret_cast<cd *>(smem4);
+    cd *Y = mat_slot(base, 0, nn), *M = mat_slot(base, 1, nn), *T = mat_slot(base, 2, nn);
+    cd *K = mat_slot(base, 3, nn);       // S stage derivatives
+    cd *PL = mat_slot(base, 3 + S, nn);  // pMatrix_{x,y,z} when p_lds
+    double *EL = reinterpret_cast<double *>(mat_slot(base, 3 + S + (p_lds ? 3 : 0), nn));  // E[N], occ[N]
+    double racc[2 * EPL];
+#pragma unroll
+    for (int t = 0; t < 2 * EPL; ++t) racc[t] = 0.0;
+    double cur[6] = {0, 0, 0, 0, 0, 0};
+    double pop = 0.0;
+    for (int64_t k = blockIdx.x; k < a.nk; k += gridDim.x) {
+        const double w = a.weight[k];
+        cd r[EPL];
+#pragma unroll
+        for (int t = 0; t < EPL; ++t) {
+            const int e = lane + 64 * t;
+            r[t] = e < nn ? rho[k * nn + e] : cmk(0.0, 0.0);
+            if (e < nn) Y[e] = r[t];
+        }
+        if (lane < 2 * N) EL[lane] = Eocc[k * 2 * N + lane];
+        const cd *Pk[3];
+        for (int ax = 0; ax < 3; ++ax) {
+            const cd *g = P + ax * ks + k * nn;
+            if (p_lds) {
+                for (int e = lane; e < nn; e += 64) PL[ax * nn + e] = g[e];
+                Pk[ax] = PL + ax * nn;
+            } else {
+                Pk[ax] = g;
+            }
+        }
+        wave_sync();
+        // ---- sample at t_n
+        if (lane < N) pop += Y[lane * N + lane].x * w;
+#pragma unroll
+        for (int t = 0; t < EPL; ++t) {
+            racc[2 * t] += r[t].x * w;
+            racc[2 * t + 1] += r[t].y * w;
+        }
+        for (int ax = 0; ax < 3; ++ax) {
+            double inter = 0.0, intra = 0.0;
+            for (int e = lane; e < nn; e += 64) {
+                const int i = e / N, j = e % N;
+                const cd y = Y[e], p = Pk[ax][j * N + i];
+                const double val = y.x * p.x - y.y * p.y;
+                if (i == j && a.inter_intra) intra += val;
+                else inter += val;
+            }
+            cur[ax] += wave_sum(inter) * w;
+            cur[3 + ax] += wave_sum(intra) * w;
+        }
+        // ---- RK stages
+        for (int s = 0; s < S; ++s) {
+            if (s > 0) {
+#pragma unroll
+                for (int t = 0; t < EPL; ++t) {
+                    const int e = lane + 64 * t;
+                    if (e >= nn) continue;
+                    cd y = r[t];
+                    for (int j = 0; j < s; ++j)
+                        if (v.a[s][j] != 0.0) y = cadd(y, cscale(K[j * nn + e], v.a[s][j]));
+                    Y[e] = y;
+                }
+            }
+            const double A0 = v.A[s][0], A1 = v.A[s][1], A2 = v.A[s][2];
+            for (int e = lane; e < nn; e += 64) {
+                cd m = cscale(Pk[0][e], A0);
+                m = cadd(m, cscale(Pk[1][e], A1));
+                m = cadd(m, cscale(Pk[2][e], A2));
+                const int i = e / N, j = e % N;
+                if (i == j) m.x += EL[i];
+                M[e] = m;
+            }
+            wave_sync();
+            wmm(T, M, Y, N, false, false);  // H y
+            for (int e = lane; e < nn; e += 64) {
+                const int i = e / N, j = e % N;
+                const cd d = csub(T[i * N + j], cconj(T[j * N + i]));
+                cd x = cmk(d.y, -d.x);  // -i (H y - (H y)^dag)
+                if (a.deph) {
+                    const double g = i == j ? a.gam1 : a.gam2;
+                    const cd dy = i == j ? cmk(Y[e].x - EL[N + i], Y[e].y) : Y[e];
+                    x = csub(x, cscale(dy, g));
+                }
+                K[s * nn + e] = x;
+            }
+            wave_sync();
+        }
+#pragma unroll
+        for (int t = 0; t < EPL; ++t) {
+            const int e = lane + 64 * t;
+            if (e >= nn) continue;
+            cd y = r[t];
+            for (int j = 0; j < S; ++j)
+                if (v.b[j] != 0.0) y = cadd(y, cscale(K[j * nn + e], v.b[j]));
+            rho[k * nn + e] = y;
+        }
+        wave_sync();
+    }
+    double *row = partials + (step_local * gridDim.x + blockIdx.x) * static_cast<int64_t>(a.nv);
+    if (lane < 6) row[lane] = cur[lane];
+    if (lane < N) row[6 + lane] = pop;
+#pragma unroll
+    for (int t = 0; t < EPL; ++t) {
+        const int e = lane + 64 * t;
+        if (e < nn) {
+            row[6 + N + 2 * e] = racc[2 * t];
+            row[6 + N + 2 * e + 1] = racc[2 * t + 1];
+        }
+    }
+}
+
 }  // namespace
 
 size_t nb_lds_bytes(int N)
@@ -757,6 +960,33 @@ hipError_t nb_launch_rhs(const NbArgs &a, const NbStage &sg, const double2 *mats
                          double2 *U, hipStream_t s)
 {
     nb_rhs_kernel<<<dim3(static_cast<unsigned>(a.nk)), 64, nb_lds_bytes(a.N), s>>>(a, sg, mats, rho, kbuf, U);
+    return hipGetLastError();
+}
+
+size_t nb_vel_lds_bytes(int N, int nstage, bool p_lds)
+{
+    const size_t nn = static_cast<size_t>(N) * N;
+    return (3 + nstage + (p_lds ? 3 : 0)) * nn * sizeof(double2) + 2 * N * sizeof(double);
+}
+
+hipError_t nb_launch_vel_init(const NbArgs &a, const double2 *mats, double2 *P, double *Eocc, double2 *U,
+                              double2 *rho, hipStream_t s)
+{
+    nb_vel_init_kernel<<<dim3(static_cast<unsigned>(a.nk_stride)), 64, nb_lds_bytes(a.N), s>>>(a, mats, P, Eocc, U, rho);
+    return hipGetLastError();
+}
+
+hipError_t nb_launch_vel_step(const NbArgs &a, const NbVelStep &v, const double2 *P, const double *Eocc, double2 *rho,
+                              double *partials, int64_t step_local, bool p_lds, hipStream_t s)
+{
+    const dim3 grid(static_cast<unsigned>(a.sample_waves));
+    const size_t lds = nb_vel_lds_bytes(a.N, v.nstage, p_lds);
+    const int nn = a.N * a.N;
+    const int pl = p_lds ? 1 : 0;
+    if (nn <= 64) nb_vel_step_kernel<1><<<grid, 64, lds, s>>>(a, v, P, Eocc, rho, partials, step_local, pl);
+    else if (nn <= 256) nb_vel_step_kernel<4><<<grid, 64, lds, s>>>(a, v, P, Eocc, rho, partials, step_local, pl);
+    else if (nn <= 1024) nb_vel_step_kernel<16><<<grid, 64, lds, s>>>(a, v, P, Eocc, rho, partials, step_local, pl);
+    else return hipErrorInvalidValue;
     return hipGetLastError();
 }
 

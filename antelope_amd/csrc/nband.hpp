@@ -37,6 +37,7 @@ struct NbArgs {
     const uint16_t *p16;   // e^{ik.R}: planes [re_hi, re_lo, im_hi, im_lo][nk_stride][nr_pad]
     uint16_t *b16;         // B of one evaluation: planes [re_hi, re_lo, im_hi, im_lo][bcols][nr_pad]
     double hmax, pmax, rmax;  // max |H(R)|, max |r(R)|, max |R_a| (scales of B)
+    int velocity;          // VelocityHamiltonian gauge (band basis at k)
 };
 
 // One evaluation of the Fourier sums at k + A(t) with field E(t).
@@ -54,7 +55,25 @@ struct NbStage {
     double b[8];  // dt b_j (used when last)
 };
 
+// One time step of the VelocityHamiltonian gauge: A at the stage times and
+// the tableau (coefficients already multiplied by dt).
+struct NbVelStep {
+    int nstage;
+    double A[7][3];  // A(t_n + c_i dt), i < nstage
+    double a[7][7];  // dt a_ij
+    double b[7];     // dt b_i
+};
+
 size_t nb_lds_bytes(int N);
+// LDS of the velocity step kernel; p_lds keeps the k point's pMatrix in LDS
+size_t nb_vel_lds_bytes(int N, int nstage, bool p_lds);
+// per k: U, E and pMatrix_a = U^dag J_a U from mats {H, J_x, J_y, J_z} at k;
+// rho = diag(occupations). P [3][nk_stride][N][N], Eocc [nk_stride][2N] (E, occ)
+hipError_t nb_launch_vel_init(const NbArgs &a, const double2 *mats, double2 *P, double *Eocc, double2 *U,
+                              double2 *rho, hipStream_t s);
+// sample at t_n, then one RK step, for every k (partials row step_local)
+hipError_t nb_launch_vel_step(const NbArgs &a, const NbVelStep &v, const double2 *P, const double *Eocc, double2 *rho,
+                              double *partials, int64_t step_local, bool p_lds, hipStream_t s);
 hipError_t nb_launch_phase(const NbArgs &a, double2 *P, hipStream_t s);
 // SPLIT16: e^{ik.R} split into FP16 hi/lo planes once per context
 hipError_t nb_launch_psplit(const NbArgs &a, const double2 *P, uint16_t *p16, hipStream_t s);

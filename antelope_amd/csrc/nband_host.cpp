@@ -189,12 +189,29 @@ int nb_setup(sbe_ctx *ctx)
         a.pmax = std::max(a.pmax, std::hypot(m.pos[2 * e], m.pos[2 * e + 1]));
     for (int e = 0; e < 3 * nr; ++e) a.rmax = std::max(a.rmax, std::fabs(hr[e]));
 
+    a.velocity = ctx->velocity ? 1 : 0;
     // e^{ik.R}, then rho(k) from H(k) (A(atmin) = 0)
     rc = hipc(ctx, nb_launch_phase(a, ctx->nb_P, ctx->stream), "phase kernel");
     if (!rc && split16) rc = hipc(ctx, nb_launch_psplit(a, ctx->nb_P, split, ctx->stream), "phase split kernel");
     NbEval ev{};
     ev.nmat = 1;
     ev.mats[0] = kMatH0;
+    if (ctx->velocity) {
+        // VelocityHamiltonian: U, E, pMatrix at k once (SBEs.h:284-373); the
+        // stage derivatives live in LDS, so nb_k holds P [3] and (E, occ) instead
+        ev.nmat = 4;
+        ev.mats[1] = kMatJx;
+        ev.mats[2] = kMatJy;
+        ev.mats[3] = kMatJz;
+        double2 *Pv = ctx->nb_k;
+        double *Eocc = reinterpret_cast<double *>(ctx->nb_k + 3 * static_cast<size_t>(p.nk_stride) * nn);
+        if (!rc) rc = hipc(ctx, nb_launch_fourier(a, ev, ctx->nb_P, ctx->nb_mats, ctx->stream), "fourier kernel");
+        if (!rc) rc = hipc(ctx, nb_launch_vel_init(a, ctx->nb_mats, Pv, Eocc, ctx->nb_U, ctx->nb_rho, ctx->stream), "velocity init kernel");
+        const size_t lds_p = nb_vel_lds_bytes(N, ctx->tab.nstage, true);
+        ctx->nb_vel_p_lds = lds_p <= (size_t(64) << 10);
+        if (!rc) rc = hipc(ctx, hipStreamSynchronize(ctx->stream), "init sync");
+        return rc;
+    }
     if (!rc) rc = hipc(ctx, nb_launch_fourier(a, ev, ctx->nb_P, ctx->nb_mats, ctx->stream), "fourier kernel");
     if (!rc) rc = hipc(ctx, nb_launch_init(a, ctx->nb_mats, ctx->nb_rho, ctx->nb_U, ctx->stream), "init kernel");
     ctx->nb_u_time = ctx->laser.atmin;  // U diagonalises H(k) = H(k + A(atmin))
@@ -203,8 +220,48 @@ int nb_setup(sbe_ctx *ctx)
     return rc;
 }
 
+namespace {
+
+// VelocityHamiltonian gauge: one fused sample + RK-step launch per time step
+int nb_run_velocity(sbe_ctx *ctx, int64_t step_begin, int64_t step_end)
+{
+    const NbArgs &a = ctx->nargs;
+    const Tableau &tb = ctx->tab;
+    const double dt = ctx->laser.dt;
+    const size_t nn = static_cast<size_t>(a.N) * a.N;
+    const double2 *Pv = ctx->nb_k;
+    const double *Eocc = reinterpret_cast<const double *>(ctx->nb_k + 3 * static_cast<size_t>(a.nk_stride) * nn);
+    NbVelStep v{};
+    v.nstage = tb.nstage;
+    for (int i = 0; i < tb.nstage; ++i) {
+        for (int j = 0; j < i; ++j) v.a[i][j] = dt * tb.a[i][j];
+        v.b[i] = dt * tb.b[i];
+    }
+    int rc = SBE_OK;
+    for (int64_t c0 = step_begin; c0 < step_end && !rc; c0 += ctx->chunk_steps) {
+        const int64_t n = std::min<int64_t>(ctx->chunk_steps, step_end - c0);
+        for (int64_t s = 0; s < n && !rc; ++s) {
+            const double tn = ctx->laser.atmin + dt * static_cast<double>(c0 + s);  // antelope.cpp:299
+            for (int i = 0; i < tb.nstage; ++i) {
+                double E[3];
+                ctx->laser.eval(tn + tb.c[i] * dt, v.A[i], E);  // SBEs.h:612
+            }
+            rc = hipc(ctx, nb_launch_vel_step(a, v, Pv, Eocc, ctx->nb_rho, ctx->d_partials, s, ctx->nb_vel_p_lds, ctx->stream),
+                      "velocity step kernel");
+        }
+        if (!rc)
+            rc = hipc(ctx, launch_series_reduce(ctx->d_partials, n, a.sample_waves, a.nv, ctx->d_series + c0 * a.nv, ctx->stream),
+                      "series kernel");
+        if (!rc) ctx->steps_done = c0 + n;
+    }
+    return rc;
+}
+
+}  // namespace
+
 int nb_run(sbe_ctx *ctx, int64_t step_begin, int64_t step_end)
 {
+    if (ctx->velocity) return nb_run_velocity(ctx, step_begin, step_end);
     const NbArgs &a = ctx->nargs;
     const Tableau &tb = ctx->tab;
     const double dt = ctx->laser.dt;
@@ -283,9 +340,15 @@ int nb_series(sbe_ctx *ctx, double *j_inter, double *j_intra, double *rho_bar, d
         const double *r = s.data() + n * nv;
         if (n < ctx->steps_done)
             for (int i = 0; i < nv; ++i) finite = finite && std::isfinite(r[i]);
+        double dia[3] = {0.0, 0.0, 0.0};  // velocity gauge: sum_k w_k Nval A(t_n) (SBEs.h:683-690,750-757)
+        if (ctx->velocity && n < ctx->steps_done) {
+            double A[3], E[3];
+            ctx->laser.eval(ctx->laser.atmin + ctx->laser.dt * static_cast<double>(n), A, E);
+            for (int ax = 0; ax < 3; ++ax) dia[ax] = ctx->model.nval * A[ax] * ctx->grid.total_weight;
+        }
         for (int ax = 0; ax < 3; ++ax) {
-            if (j_inter) j_inter[3 * n + ax] = r[ax];
-            if (j_intra) j_intra[3 * n + ax] = ctx->cfg.inter_intra ? r[3 + ax] : 0.0;
+            if (j_inter) j_inter[3 * n + ax] = r[ax] + (ctx->cfg.inter_intra ? 0.0 : dia[ax]);
+            if (j_intra) j_intra[3 * n + ax] = ctx->cfg.inter_intra ? r[3 + ax] + dia[ax] : 0.0;
         }
         if (band_pop)
             for (int m = 0; m < N; ++m) band_pop[n * N + m] = r[6 + m];
@@ -300,14 +363,57 @@ int nb_series(sbe_ctx *ctx, double *j_inter, double *j_intra, double *rho_bar, d
 
 int nb_state(sbe_ctx *ctx, int basis, double *rho)
 {
-    if (basis != SBE_BASIS_ORBITAL) {
-        ctx->err = "band-basis export is available for the 2-band path only";
+    if (basis != SBE_BASIS_ORBITAL && basis != SBE_BASIS_BAND) {
+        ctx->err = "unknown basis";
+        return SBE_ERR_CONFIG;
+    }
+    // LengthWannier propagates the orbital basis, VelocityHamiltonian the band basis at k
+    const int native = ctx->velocity ? SBE_BASIS_BAND : SBE_BASIS_ORBITAL;
+    if (basis != native && !ctx->velocity) {
+        ctx->err = "band-basis export of the LengthWannier N-band state is available for the 2-band path only";
         return SBE_ERR_UNSUPPORTED;
     }
-    const size_t bytes = static_cast<size_t>(ctx->nk) * ctx->N * ctx->N * sizeof(double2);
+    const int N = ctx->N;
+    const size_t nn = static_cast<size_t>(N) * N;
+    const size_t bytes = static_cast<size_t>(ctx->nk) * nn * sizeof(double2);
     int rc = hipc(ctx, hipMemcpyAsync(rho, ctx->nb_rho, bytes, hipMemcpyDeviceToHost, ctx->stream), "copy state");
+    std::vector<double2> U;
+    if (!rc && basis != native) {  // velocity gauge, orbital basis: U rho U^dag with the stored U(k)
+        U.resize(static_cast<size_t>(ctx->nk) * nn);
+        rc = hipc(ctx, hipMemcpyAsync(U.data(), ctx->nb_U, bytes, hipMemcpyDeviceToHost, ctx->stream), "copy U");
+    }
     if (!rc) rc = hipc(ctx, hipStreamSynchronize(ctx->stream), "sync state");
-    return rc;
+    if (rc || basis == native) return rc;
+    std::vector<double> T(2 * nn);
+    for (int64_t k = 0; k < ctx->nk; ++k) {
+        const double2 *u = U.data() + k * nn;
+        double *r = rho + 2 * k * nn;
+        for (int i = 0; i < N; ++i)  // T = rho U^dag
+            for (int j = 0; j < N; ++j) {
+                double re = 0, im = 0;
+                for (int l = 0; l < N; ++l) {
+                    const double ar = r[2 * (i * N + l)], ai = r[2 * (i * N + l) + 1];
+                    const double br = u[j * N + l].x, bi = -u[j * N + l].y;
+                    re += ar * br - ai * bi;
+                    im += ar * bi + ai * br;
+                }
+                T[2 * (i * N + j)] = re;
+                T[2 * (i * N + j) + 1] = im;
+            }
+        for (int i = 0; i < N; ++i)  // rho = U T
+            for (int j = 0; j < N; ++j) {
+                double re = 0, im = 0;
+                for (int l = 0; l < N; ++l) {
+                    const double ar = u[i * N + l].x, ai = u[i * N + l].y;
+                    const double br = T[2 * (l * N + j)], bi = T[2 * (l * N + j) + 1];
+                    re += ar * br - ai * bi;
+                    im += ar * bi + ai * br;
+                }
+                r[2 * (i * N + j)] = re;
+                r[2 * (i * N + j) + 1] = im;
+            }
+    }
+    return SBE_OK;
 }
 
 }  // namespace sbe

@@ -16,6 +16,7 @@
 #include <string>
 #include <vector>
 
+#include "band_frame.hpp"
 #include "sbe_internal.hpp"
 
 using sbe::kBlock;
@@ -79,13 +80,21 @@ int validate(const sbe_config *cfg, const sbe_model *m, std::string &err)
         err = "LengthHamiltonian gauge is not implemented (as in the reference, SBEs.h:806-810)";
         return SBE_ERR_UNSUPPORTED;
     }
-    if (cfg->gauge != SBE_GAUGE_LENGTH_WANNIER) {
-        err = "only the LengthWannier gauge is built";
+    if (cfg->gauge != SBE_GAUGE_LENGTH_WANNIER && cfg->gauge != SBE_GAUGE_VELOCITY_HAMILTONIAN) {
+        err = "unknown gauge";
+        return SBE_ERR_CONFIG;
+    }
+    if (cfg->gauge == SBE_GAUGE_VELOCITY_HAMILTONIAN && cfg->rk_order == SBE_RK_CASH_KARP) {
+        err = "the adaptive Cash-Karp stepper is built for the LengthWannier gauge";
         return SBE_ERR_UNSUPPORTED;
     }
     if (cfg->precision != SBE_PREC_FP64 && cfg->precision != SBE_PREC_FP32 && cfg->precision != SBE_PREC_SPLIT16) {
         err = "unknown precision";
         return SBE_ERR_CONFIG;
+    }
+    if (cfg->precision == SBE_PREC_SPLIT16 && cfg->gauge == SBE_GAUGE_VELOCITY_HAMILTONIAN) {
+        err = "SPLIT16 precision applies to the per-stage Fourier sums, which the velocity gauge does not have";
+        return SBE_ERR_UNSUPPORTED;
     }
     if (cfg->precision == SBE_PREC_SPLIT16 && m->kind != SBE_MODEL_WANNIER) {
         err = "SPLIT16 precision applies to the Wannier Fourier sums";
@@ -315,6 +324,7 @@ int sbe_create(const sbe_config *cfg, const sbe_model *model, int64_t k_begin, i
     ctx->cfg.pulses = ctx->pulses.data();
     ctx->model = *model;
     ctx->N = model->nband;
+    ctx->velocity = cfg->gauge == SBE_GAUGE_VELOCITY_HAMILTONIAN;
 
     if ((rc = ctx->laser.init(ctx->cfg, err)) || (rc = ctx->tab.init(ctx->cfg.rk_order, err)) ||
         (rc = ctx->grid.build(ctx->cfg, ctx->model, k_begin, k_end, err))) {
@@ -421,6 +431,8 @@ int sbe_create(const sbe_config *cfg, const sbe_model *model, int64_t k_begin, i
     h.inter_intra = ctx->cfg.inter_intra ? 1 : 0;
     h.precision = ctx->cfg.precision;
     h.nval = model->nval;
+    h.fermi = ctx->cfg.init_type == SBE_INIT_FERMI_DIRAC ? 1 : 0;
+    h.fermi_energy = ctx->cfg.fermi_energy;
     h.kpt = p.kpt;
     h.kpts = ctx->d_kpts;
     h.weight = ctx->d_weight;
@@ -437,6 +449,12 @@ int sbe_create(const sbe_config *cfg, const sbe_model *model, int64_t k_begin, i
             for (int s = 0; s < S; ++s) {
                 double A[3], E[3];
                 ctx->laser.eval(tn + ctx->tab.c[s] * ctx->laser.dt, A, E);  // SBEs.h:612
+                if (ctx->velocity) {  // A at the stage times, [nt][S][2]
+                    double *row = tab.data() + (static_cast<size_t>(n) * S + s) * 2;
+                    row[0] = A[0];
+                    row[1] = A[1];
+                    continue;
+                }
                 double *row = tab.data() + (static_cast<size_t>(n) * S + s) * 12;
                 for (int j = 0; j < 3; ++j) {
                     const double pa = A[0] * va[j][0] + A[1] * va[j][1];
@@ -486,7 +504,8 @@ int sbe_create(const sbe_config *cfg, const sbe_model *model, int64_t k_begin, i
         if (!rc)
             rc = hip_check(ctx, hipMemcpy(ctx->d_aargs, &A, sizeof A, hipMemcpyHostToDevice), "upload adaptive arguments");
     }
-    if (!rc) rc = hip_check(ctx, sbe::launch_haldane_init(h, ctx->cfg.init_type, ctx->cfg.fermi_energy, ctx->stream), "init kernel");
+    if (!rc && ctx->velocity) rc = hip_check(ctx, sbe::launch_haldane_vel_init(h, ctx->stream), "init kernel");
+    else if (!rc) rc = hip_check(ctx, sbe::launch_haldane_init(h, ctx->cfg.init_type, ctx->cfg.fermi_energy, ctx->stream), "init kernel");
     if (!rc) rc = hip_check(ctx, hipStreamSynchronize(ctx->stream), "init sync");
     if (rc) {
         set_error(nullptr, rc, ctx->err);
@@ -536,8 +555,10 @@ int sbe_run(sbe_ctx *ctx, int64_t step_begin, int64_t step_end)
         sbe::HaldaneArgs a = ctx->hargs;
         a.step0 = s;
         a.nsteps = static_cast<int32_t>(n);
-        int rc = timed(ctx, 0, [&] { return sbe::launch_haldane_steps(a, ctx->cfg.rk_order, t1_on, ctx->stream); },
-                       "step kernel");
+        int rc = timed(ctx, 0, [&] {
+            return ctx->velocity ? sbe::launch_haldane_vel_steps(a, ctx->cfg.rk_order, t1_on, ctx->stream)
+                                 : sbe::launch_haldane_steps(a, ctx->cfg.rk_order, t1_on, ctx->stream);
+        }, "step kernel");
         if (!rc)
             rc = timed(ctx, 1, [&] {
                 return sbe::launch_series_reduce(ctx->d_partials, n, ctx->nwaves, ctx->nv, ctx->d_series + s * ctx->nv,
@@ -607,8 +628,10 @@ int sbe_get_series(sbe_ctx *ctx, double *j_inter, double *j_intra, double *rho_b
     if (failed)
         return set_error(ctx, SBE_ERR_NAN, "adaptive stepper: a step size fell below 1e-6 dt or a window needed more "
                                            "than 64 attempts per sample (non-finite state or rtol too tight)");
-    // n(t_m) = n(0) R^m for every k when T1 is on (R: RK factor of dn/dt = -n/T1)
-    const double R = ctx->hargs.gam1 != 0.0 ? rk_linear_factor(ctx->tab, -ctx->hargs.gam1 * ctx->laser.dt) : 1.0;
+    // n(t_m) = n(0) R^m for every k when T1 is on (R: RK factor of dn/dt = -n/T1);
+    // in the velocity gauge the diagonal relaxes toward rho0 and n stays n(0)
+    const double R = ctx->hargs.gam1 != 0.0 && !ctx->velocity
+                         ? rk_linear_factor(ctx->tab, -ctx->hargs.gam1 * ctx->laser.dt) : 1.0;
     double nsum = ctx->n_sum;
     bool finite = true;
     for (int64_t n = 0; n < nt; ++n, nsum *= R) {
@@ -616,8 +639,17 @@ int sbe_get_series(sbe_ctx *ctx, double *j_inter, double *j_intra, double *rho_b
         const bool done = n < ctx->steps_done;
         for (int i = 0; i < ctx->nv && done; ++i) finite = finite && std::isfinite(r[i]);
         const double ns = done ? nsum : 0.0;
+        double dia[3] = {0.0, 0.0, 0.0};  // velocity gauge: sum_k w_k Nval A(t_n) (SBEs.h:683-690,750-757)
+        if (ctx->velocity && done) {
+            double A[3], E[3];
+            ctx->laser.eval(ctx->laser.atmin + ctx->laser.dt * static_cast<double>(n), A, E);
+            for (int ax = 0; ax < 3; ++ax) dia[ax] = ctx->model.nval * A[ax] * ctx->grid.total_weight;
+        }
+        double *jd = ctx->cfg.inter_intra ? j_intra : j_inter;
         if (j_inter) { j_inter[3 * n] = r[0]; j_inter[3 * n + 1] = r[1]; j_inter[3 * n + 2] = 0.0; }
         if (j_intra) { j_intra[3 * n] = r[2]; j_intra[3 * n + 1] = r[3]; j_intra[3 * n + 2] = 0.0; }
+        if (jd)
+            for (int ax = 0; ax < 3; ++ax) jd[3 * n + ax] += dia[ax];
         if (rho_bar) {
             double *o = rho_bar + n * 8;  // [2][2][re,im], rho = (n + s.sigma)/2
             o[0] = 0.5 * (ns + r[6]); o[1] = 0.0;
@@ -647,6 +679,33 @@ int sbe_get_state(sbe_ctx *ctx, int32_t basis, double *rho)
     if (!rc) rc = hip_check(ctx, hipStreamSynchronize(ctx->stream), "sync state");
     if (d_out) (void)hipFree(d_out);
     if (rc) return rc;
+    if (ctx->velocity) {
+        // the state is band-basis at k (phases of band_frame.hpp); orbital: s_orb = sum_i s_i f_i
+        if (basis == SBE_BASIS_ORBITAL) {
+            const sbe::HaldaneArgs &h = ctx->hargs;
+            for (int64_t k = 0; k < ctx->nk; ++k) {
+                const double kx = ctx->grid.k[3 * k], ky = ctx->grid.k[3 * k + 1];
+                double ca = 0, sa = 0, sb = 0;
+                for (int j = 0; j < 3; ++j) {
+                    ca += std::cos(kx * h.va[j][0] + ky * h.va[j][1]);
+                    sa += std::sin(kx * h.va[j][0] + ky * h.va[j][1]);
+                    sb += std::sin(kx * h.vb[j][0] + ky * h.vb[j][1]);
+                }
+                const sbe::BandFrame F = sbe::band_frame(h.t1 * ca, h.t1 * sa, h.M0 - h.c3 * sb);
+                double *o = rho + 8 * k;
+                const double n = o[0] + o[6], b[3] = {2 * o[2], 2 * o[5], o[0] - o[6]};
+                double s[3];
+                for (int c = 0; c < 3; ++c) s[c] = b[0] * F.fx[c] + b[1] * F.fy[c] + b[2] * F.fz[c];
+                o[0] = 0.5 * (n + s[2]); o[1] = 0.0;
+                o[2] = 0.5 * s[0];       o[3] = -0.5 * s[1];
+                o[4] = 0.5 * s[0];       o[5] = 0.5 * s[1];
+                o[6] = 0.5 * (n - s[2]); o[7] = 0.0;
+            }
+        } else if (basis != SBE_BASIS_BAND) {
+            return set_error(ctx, SBE_ERR_CONFIG, "unknown basis");
+        }
+        return SBE_OK;
+    }
     if (basis == SBE_BASIS_BAND) {
         // U^dag rho U at k + A(t_now) with the eigenvectors of d.sigma
         const double t = ctx->laser.atmin + ctx->laser.dt * static_cast<double>(ctx->steps_done);
@@ -754,6 +813,8 @@ int sbe_batch_create(int32_t n, const sbe_config *cfgs, const sbe_model *models,
             return batch_fail(nullptr, SBE_ERR_UNSUPPORTED, "batched solves take 2-band (Haldane) models");
         if (cfgs[i].rk_order != cfgs[0].rk_order || cfgs[i].precision != cfgs[0].precision)
             return batch_fail(nullptr, SBE_ERR_CONFIG, "all solves of a batch share rk_order and precision");
+        if (cfgs[i].gauge != SBE_GAUGE_LENGTH_WANNIER)
+            return batch_fail(nullptr, SBE_ERR_UNSUPPORTED, "batched solves run the LengthWannier gauge");
     }
     sbe_batch *b = new (std::nothrow) sbe_batch();
     if (!b) return batch_fail(nullptr, SBE_ERR_OOM, "host allocation failed");

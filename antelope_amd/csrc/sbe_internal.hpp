@@ -76,9 +76,12 @@ struct HaldaneArgs {
     const double *kpts;     // [3][nk_stride]
     const double *weight;   // [nk_stride] (0 on padding)
     const double *phase_tab;  // [nt][nstage][12]: t1 e^{i A.a_j} (j<3), e^{i A.b_j}, at t_n + c_s dt
+                              // (velocity gauge: [nt][nstage][2] = A_x, A_y at t_n + c_s dt)
     const float *phase_tab32; // same table rounded to FP32 (FP32 state only)
     int32_t nval;             // occupied bands of the initial state
     int32_t kpt;              // k points per lane in the step kernel (1 or 2)
+    int32_t fermi;            // Fermi-Dirac initial occupations (velocity gauge kernels)
+    double fermi_energy;
     void *state;            // SoA [4][nk_stride] (fp64) or [8][nk_stride] (fp32 + compensation)
     double *partials;       // [nsteps][nwaves][kHaldaneNV]
 };
@@ -124,6 +127,7 @@ struct sbe_ctx {
     sbe::Grid grid;
     sbe::Tableau tab;
     sbe::PathKind path = sbe::PathKind::Haldane2;
+    bool velocity = false;  // VelocityHamiltonian gauge: the state is band-basis at k
     int N = 2;
     int64_t nk = 0, nk_stride = 0;
     int64_t steps_done = 0;
@@ -164,6 +168,7 @@ struct sbe_ctx {
     double nb_u_time = 0;        // time whose H the stored eigenvectors diagonalise
     double nb_ht_A[3] = {0, 0, 0}, nb_ht_E[3] = {0, 0, 0};  // (A, E) of the H~ in mats slot 0
     bool nb_ht_valid = false;
+    bool nb_vel_p_lds = false;   // velocity gauge: pMatrix of the k point staged in LDS
     std::string err;
 
     // optional kernel timing: (start, stop) event pairs per kernel class
@@ -178,6 +183,9 @@ hipError_t launch_haldane_steps(const HaldaneArgs &a, int rk_order, bool t1_on, 
 hipError_t launch_series_reduce(const double *partials, int64_t nsteps, int32_t nrows, int nv,
                                 double *series, hipStream_t s);
 hipError_t launch_haldane_export(const HaldaneArgs &a, double *out, hipStream_t s);
+// VelocityHamiltonian gauge (haldane_velocity.hip): band-basis state
+hipError_t launch_haldane_vel_init(const HaldaneArgs &a, hipStream_t s);
+hipError_t launch_haldane_vel_steps(const HaldaneArgs &a, int rk_order, bool t1_on, hipStream_t s);
 hipError_t launch_haldane_batch(const HaldaneArgs *d_insts, const int32_t *d_block_inst, const int32_t *d_block_first,
                                 int nblocks, int rk_order, int precision, bool t1_on, bool deph, hipStream_t s);
 // Adaptive stepper: d_block_inst / d_block_first may be null for a single

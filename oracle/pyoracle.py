@@ -33,6 +33,7 @@ _SIGS = {
     "orc_dipole": ([C.POINTER(A.sbe_model), _D, _D], C.c_int),
     "orc_eigh": ([C.c_int, _D, _D, _D], C.c_int),
     "orc_rhs": ([C.POINTER(A.sbe_config), C.POINTER(A.sbe_model), _D, C.c_double, _D, _D], C.c_int),
+    "orc_velocity_setup": ([C.POINTER(A.sbe_config), C.POINTER(A.sbe_model), _D, _D, _D, _D], C.c_int),
     "orc_run": ([C.POINTER(A.sbe_config), C.POINTER(A.sbe_model), C.c_int64, C.c_int64, C.c_int64, C.c_int,
                  _D, _D, _D, _D, _D, C.POINTER(orc_info)], C.c_int),
     "orc_run_adaptive": ([C.POINTER(A.sbe_config), C.POINTER(A.sbe_model), C.c_int64, C.c_int64, C.c_int64,
@@ -129,6 +130,17 @@ def rhs(config, model, k, t, rho):
     kk = np.ascontiguousarray(k, dtype=np.float64)
     _chk(lib().orc_rhs(C.byref(cfg), C.byref(mdl), _p(kk), t, _p(_c(rho)), _p(out)))
     return out[..., 0] + 1j * out[..., 1]
+
+
+def velocity_setup(config, model, k):
+    """VelocityHamiltonian per-k constants: E[N], P[3][N][N] = U^dag J U, rho0."""
+    cfg, k1 = config.to_c()
+    mdl, k2 = model.to_c()
+    N = model.nband
+    E, P, r0 = np.zeros(N), np.zeros((3, N, N, 2)), np.zeros((N, N, 2))
+    kk = np.ascontiguousarray(k, dtype=np.float64)
+    _chk(lib().orc_velocity_setup(C.byref(cfg), C.byref(mdl), _p(kk), _p(E), _p(P), _p(r0)))
+    return E, P[..., 0] + 1j * P[..., 1], r0[..., 0] + 1j * r0[..., 1]
 
 
 def run(config, model, k_begin=-1, k_end=-1, nsteps=-1, nthreads=1, want_state=False):

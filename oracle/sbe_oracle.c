@@ -616,7 +616,16 @@ typedef struct {
     int N, nstage;
     double c[6], b[6], a[6][6];
     double gam_intra, gam_inter;
+    int velocity; /* GaugeType::VelocityHamiltonian */
 } osolver;
+
+/* Per-k constants of the VelocityHamiltonian gauge, built once at set-up as
+ * SBEs.h:284-373 does: edispersion E[N], pMatrix P[3][N][N] = U^dag J(k) U
+ * and initMatrix rho0 = diag(occupations) (band basis, no rotation). */
+typedef struct {
+    double *E;
+    cplx *P, *rho0;
+} ovel;
 
 /* SBEs::InitializeGeneral post-processing (SBEs.h:484-508) and the RK
  * tableaus of SBEs.h:395-416 */
@@ -625,8 +634,9 @@ static int solver_init(const sbe_config *cfg, const sbe_model *model, osolver *S
     memset(S, 0, sizeof *S);
     S->cfg = cfg;
     if (cfg->abi_version != SBE_ABI_VERSION) return fail(SBE_ERR_CONFIG, "abi_version mismatch");
-    if (cfg->gauge != SBE_GAUGE_LENGTH_WANNIER)
-        return fail(SBE_ERR_UNSUPPORTED, "only the LengthWannier gauge is restated");
+    if (cfg->gauge != SBE_GAUGE_LENGTH_WANNIER && cfg->gauge != SBE_GAUGE_VELOCITY_HAMILTONIAN)
+        return fail(SBE_ERR_UNSUPPORTED, "the LengthWannier and VelocityHamiltonian gauges are restated");
+    S->velocity = cfg->gauge == SBE_GAUGE_VELOCITY_HAMILTONIAN;
     int rc = model_init(model, &S->om);
     if (rc) return rc;
     rc = laser_init(cfg, &S->L);
@@ -697,10 +707,94 @@ static void gen_u(const osolver *S, const double kp[3], oscratch *w)
     adjoint(w->Ud, w->U, S->N);
 }
 
+/* Eigenvector phases of the velocity gauge. The reference keeps whatever
+ * phases zheevr returns (SBEs.h:284-316); LAPACK does not specify them, and
+ * the off-diagonal band-basis elements of rho depend on them (currents and
+ * populations do not). Fixed here: column j is multiplied by
+ * conj(U[p][j]) / |U[p][j]| with p the first row where |U[p][j]|^2 >= 1/(2N)
+ * (one exists, the column being a unit vector). */
+static void phase_fix(int N, cplx *U)
+{
+    for (int j = 0; j < N; ++j) {
+        int p = 0;
+        while (p < N - 1 && creal(U[p * N + j] * conj(U[p * N + j])) * (2.0 * N) < 1.0) ++p;
+        const double a = cabs(U[p * N + j]);
+        if (a == 0.0) continue;
+        const cplx ph = conj(U[p * N + j]) / a;
+        for (int r = 0; r < N; ++r) U[r * N + j] *= ph;
+        U[p * N + j] = a;
+    }
+}
+
+/* Initial occupations of the bands (SBEs.h:318-340): the first Nval bands,
+ * or Fermi-Dirac at kT = 300 K / 3.15775024804e5 normalised to sum 1. */
+static void occupations(const osolver *S, const double *ev, double *occ)
+{
+    int N = S->N;
+    if (S->cfg->init_type == SBE_INIT_FERMI_DIRAC) {
+        double kT = 300. / 3.15775024804e5, sum = 0;
+        for (int m = 0; m < N; ++m) { occ[m] = 1.0 / (exp((ev[m] - S->cfg->fermi_energy) / kT) + 1.0); sum += occ[m]; }
+        for (int m = 0; m < N; ++m) occ[m] /= sum;
+    } else {
+        for (int m = 0; m < N; ++m) occ[m] = m < S->om.m->nval ? 1.0 : 0.0;
+    }
+}
+
+/* Velocity-gauge constants at one k (SBEs.h:284-373): E(k), U(k) (phases
+ * fixed as above), pMatrix_a = U^dag J_a(k) U with J = GenJMatrix, and
+ * rho0 = diag(occupations). */
+static void vel_setup(const osolver *S, const double k[3], double *E, cplx *P, cplx *rho0, oscratch *w)
+{
+    int N = S->N, nn = N * N;
+    double occ[256];
+    model_H(&S->om, k, w->Hd);
+    eigh(N, w->Hd, E, w->U, w->work);
+    phase_fix(N, w->U);
+    adjoint(w->Ud, w->U, N);
+    model_J(&S->om, k, w->J);
+    for (int ax = 0; ax < 3; ++ax) {
+        matmul(w->t1, w->J + ax * nn, w->U, N);
+        matmul(P + ax * nn, w->Ud, w->t1, N);
+    }
+    occupations(S, E, occ);
+    for (int i = 0; i < nn; ++i) rho0[i] = 0;
+    for (int m = 0; m < N; ++m) rho0[m * N + m] = occ[m];
+}
+
+/* SBEs::GenDifferentialDM, VelocityHamiltonian branch, SBEs.h:787-805,
+ * 814-829 and the relaxation toward initMatrix of SBEs.h:862-871:
+ * out = -i((H rho) - (H rho)^dag) - (rho - rho0) o Gamma with
+ * H = sum_a A_a(t) pMatrix_a + diag(E). */
+static void rhs_velocity(const osolver *S, const ovel *V, double t, const cplx *rho, cplx *out, oscratch *w)
+{
+    int N = S->N, nn = N * N;
+    double A[3], E[3];
+    laser_eval(&S->L, t, A, E);
+    for (int i = 0; i < nn; ++i) {
+        w->H[i] = 0;
+        for (int ax = 0; ax < 3; ++ax) w->H[i] += A[ax] * V->P[ax * nn + i];
+    }
+    for (int m = 0; m < N; ++m) w->H[m * N + m] += V->E[m];
+    matmul(out, w->H, rho, N);
+    for (int i = 0; i < nn; ++i) w->T[i] = out[i];
+    for (int m = 0; m < N; ++m)
+        for (int n = 0; n < N; ++n) {
+            out[m * N + n] -= conj(w->T[n * N + m]);
+            out[m * N + n] *= -I;
+        }
+    for (int m = 0; m < N; ++m)
+        for (int n = 0; n < N; ++n)
+            out[m * N + n] -= (rho[m * N + n] - V->rho0[m * N + n]) * ((m == n) ? S->gam_intra : S->gam_inter);
+}
+
 /* SBEs::GenDifferentialDM, LengthWannier branch, SBEs.h:762-786,814-861 */
-static void rhs(const osolver *S, const double k[3], double t, const cplx *rho, cplx *out,
+static void rhs(const osolver *S, const double k[3], const ovel *V, double t, const cplx *rho, cplx *out,
                 oscratch *w)
 {
+    if (S->velocity) {
+        rhs_velocity(S, V, t, rho, out, w);
+        return;
+    }
     int N = S->N;
     double A[3], E[3];
     laser_eval(&S->L, t, A, E);
@@ -728,6 +822,24 @@ static void rhs(const osolver *S, const double k[3], double t, const cplx *rho, 
     for (int i = 0; i < N * N; ++i) out[i] -= w->t2[i];
 }
 
+/* velocity-gauge constants of one k in freshly allocated arrays */
+static void vel_alloc(ovel *V, int N, size_t nk)
+{
+    size_t nn = (size_t)N * N;
+    V->E = calloc(nk * N, sizeof(double));
+    V->P = calloc(nk * 3 * nn, sizeof(cplx));
+    V->rho0 = calloc(nk * nn, sizeof(cplx));
+}
+
+static void vel_free(ovel *V) { free(V->E); free(V->P); free(V->rho0); }
+
+static ovel vel_at(const ovel *V, int N, int64_t k)
+{
+    size_t nn = (size_t)N * N;
+    ovel v = {V->E + (size_t)k * N, V->P + (size_t)k * 3 * nn, V->rho0 + (size_t)k * nn};
+    return v;
+}
+
 int orc_rhs(const sbe_config *cfg, const sbe_model *model, const double k[3], double t,
             const double *rho, double *out)
 {
@@ -736,25 +848,45 @@ int orc_rhs(const sbe_config *cfg, const sbe_model *model, const double k[3], do
     if (rc) return rc;
     oscratch w;
     scratch_alloc(&w, S.N);
-    rhs(&S, k, t, (const cplx *)rho, (cplx *)out, &w);
+    ovel V = {0};
+    if (S.velocity) {
+        vel_alloc(&V, S.N, 1);
+        vel_setup(&S, k, V.E, V.P, V.rho0, &w);
+    }
+    rhs(&S, k, &V, t, (const cplx *)rho, (cplx *)out, &w);
+    if (S.velocity) vel_free(&V);
+    scratch_free(&w);
+    solver_free(&S);
+    return SBE_OK;
+}
+
+int orc_velocity_setup(const sbe_config *cfg, const sbe_model *model, const double k[3], double *E, double *P,
+                       double *rho0)
+{
+    osolver S;
+    int rc = solver_init(cfg, model, &S);
+    if (rc) return rc;
+    oscratch w;
+    scratch_alloc(&w, S.N);
+    vel_setup(&S, k, E, (cplx *)P, (cplx *)rho0, &w);
     scratch_free(&w);
     solver_free(&S);
     return SBE_OK;
 }
 
 /* SBEs::RunSBEs, SBEs.h:596-624 */
-static void rk_step(const osolver *S, const double k[3], double t, cplx *rho, cplx **stg,
+static void rk_step(const osolver *S, const double k[3], const ovel *V, double t, cplx *rho, cplx **stg,
                     cplx *tmp, oscratch *w)
 {
     int N = S->N, nn = N * N;
     double dt = S->L.dt;
-    rhs(S, k, t, rho, stg[0], w);
+    rhs(S, k, V, t, rho, stg[0], w);
     for (int irk = 0; irk < S->nstage - 1; ++irk) {
         for (int i = 0; i < nn; ++i) {
             tmp[i] = rho[i];
             for (int j = 0; j < irk + 1; ++j) tmp[i] += dt * S->a[irk][j] * stg[j][i];
         }
-        rhs(S, k, t + S->c[irk] * dt, tmp, stg[irk + 1], w);
+        rhs(S, k, V, t + S->c[irk] * dt, tmp, stg[irk + 1], w);
     }
     for (int i = 0; i < nn; ++i)
         for (int irk = 0; irk < S->nstage; ++irk) rho[i] += dt * S->b[irk] * stg[irk][i];
@@ -763,12 +895,30 @@ static void rk_step(const osolver *S, const double k[3], double t, cplx *rho, cp
 /* SBEs::GenInterIntraCurrent SBEs.h:694-760 / SBEs::GenCurrent SBEs.h:626-657,
  * plus the band populations diag(U^dag rho U) that the reference builds as
  * newdMatrix but does not output. */
-static void currents(const osolver *S, const double k[3], double t, const cplx *rho,
+static void currents(const osolver *S, const double k[3], const ovel *V, double t, const cplx *rho,
                      double inter[3], double intra[3], double *pop, oscratch *w)
 {
     int N = S->N;
     double A[3], E[3];
     laser_eval(&S->L, t, A, E);
+    if (S->velocity) {
+        /* _tkp = k: rho is already band-basis and U^dag J U is pMatrix; then
+         * j = -(p + A(t)) adds Nval A(t) (SBEs.h:640-642,676-690,746-757) */
+        for (int m = 0; m < N; ++m) pop[m] = creal(rho[m * N + m]);
+        for (int ax = 0; ax < 3; ++ax) {
+            const cplx *P = V->P + ax * N * N;
+            inter[ax] = 0; intra[ax] = 0;
+            for (int i = 0; i < N; ++i)
+                for (int j = 0; j < N; ++j) {
+                    double v = creal(rho[N * i + j] * P[N * j + i]);
+                    if (i == j && S->cfg->inter_intra) intra[ax] += v;
+                    else inter[ax] += v;
+                }
+            if (S->cfg->inter_intra) intra[ax] += S->om.m->nval * A[ax];
+            else inter[ax] += S->om.m->nval * A[ax];
+        }
+        return;
+    }
     double kp[3] = {k[0] + A[0], k[1] + A[1], k[2] + A[2]};
     model_J(&S->om, kp, w->J);
     gen_u(S, kp, w);
@@ -799,19 +949,13 @@ static void currents(const osolver *S, const double k[3], double t, const cplx *
 static void init_state(const osolver *S, const double k[3], cplx *rho, oscratch *w)
 {
     int N = S->N;
+    double occ[256];
     model_H(&S->om, k, w->Hd);
     eigh(N, w->Hd, w->ev, w->U, w->work);
     adjoint(w->Ud, w->U, N);
     for (int i = 0; i < N * N; ++i) rho[i] = 0;
-    if (S->cfg->init_type == SBE_INIT_FERMI_DIRAC) {
-        double kT = 300. / 3.15775024804e5, sum = 0, *f = w->ev;
-        double tmp[256];
-        for (int m = 0; m < N; ++m) { tmp[m] = 1.0 / (exp((f[m] - S->cfg->fermi_energy) / kT) + 1.0); sum += tmp[m]; }
-        for (int m = 0; m < N; ++m) rho[m * N + m] = tmp[m] / sum;
-    } else {
-        for (int m = 0; m < N; ++m)
-            if (m < S->om.m->nval) rho[m * N + m] = 1.0;
-    }
+    occupations(S, w->ev, occ);
+    for (int m = 0; m < N; ++m) rho[m * N + m] = occ[m];
     matmul(w->t1, rho, w->Ud, N);
     matmul(rho, w->U, w->t1, N);
 }
@@ -822,6 +966,7 @@ typedef struct {
     int64_t k0, k1, nsteps;
     double *ji, *ja, *rb, *bp; /* per-thread series */
     cplx *rho;                 /* [nk][N*N] slice */
+    const ovel *V;             /* velocity gauge: per-k constants [nk] */
 } othread;
 
 /* antelope.cpp:297-338 for one contiguous k block */
@@ -841,7 +986,9 @@ static void *thread_main(void *arg)
             const double *kk = T->kp + 3 * k;
             double wk = T->wk[k];
             cplx *rho = T->rho + (size_t)k * nn;
-            currents(S, kk, t, rho, inter, intra, pop, &w);
+            ovel v = {0};
+            if (S->velocity) v = vel_at(T->V, N, k);
+            currents(S, kk, &v, t, rho, inter, intra, pop, &w);
             for (int ax = 0; ax < 3; ++ax) {
                 T->ja[3 * n + ax] += intra[ax] * wk;
                 T->ji[3 * n + ax] += inter[ax] * wk;
@@ -851,7 +998,7 @@ static void *thread_main(void *arg)
                 T->rb[(n * nn + i) * 2 + 1] += cimag(rho[i]) * wk;
             }
             for (int m = 0; m < N; ++m) T->bp[n * N + m] += pop[m] * wk;
-            rk_step(S, kk, t, rho, stg, tmp, &w);
+            rk_step(S, kk, &v, t, rho, stg, tmp, &w);
         }
     }
     for (int i = 0; i < 6; ++i) free(stg[i]);
@@ -883,10 +1030,20 @@ int orc_run(const sbe_config *cfg, const sbe_model *model, int64_t k_begin, int6
     rc = orc_kgrid(cfg, model, k_begin, k_end, kp, wk, &dV, &tw);
     if (rc) { free(kp); free(wk); solver_free(&S); return rc; }
     cplx *rho = calloc((size_t)nk * nn, sizeof(cplx));
+    ovel V = {0};
     {
         oscratch w;
         scratch_alloc(&w, N);
-        for (int64_t k = 0; k < nk; ++k) init_state(&S, kp + 3 * k, rho + (size_t)k * nn, &w);
+        if (S.velocity) {
+            vel_alloc(&V, N, (size_t)nk);
+            for (int64_t k = 0; k < nk; ++k) {
+                ovel v = vel_at(&V, N, k);
+                vel_setup(&S, kp + 3 * k, v.E, v.P, v.rho0, &w);
+                memcpy(rho + (size_t)k * nn, v.rho0, sizeof(cplx) * nn); /* dmatrix = initMatrix */
+            }
+        } else {
+            for (int64_t k = 0; k < nk; ++k) init_state(&S, kp + 3 * k, rho + (size_t)k * nn, &w);
+        }
         scratch_free(&w);
     }
     if (nthreads < 1) nthreads = 1;
@@ -899,7 +1056,7 @@ int orc_run(const sbe_config *cfg, const sbe_model *model, int64_t k_begin, int6
         int64_t w1 = nk / nthreads, w2 = nk % nthreads;
         int64_t s = w1 * r + (r < w2 ? r : w2), e = s + w1 + (w2 > r ? 1 : 0);
         th[r].S = &S; th[r].kp = kp; th[r].wk = wk; th[r].k0 = s; th[r].k1 = e;
-        th[r].nsteps = nsteps; th[r].rho = rho;
+        th[r].nsteps = nsteps; th[r].rho = rho; th[r].V = &V;
         th[r].ji = calloc((size_t)ns * 3, sizeof(double));
         th[r].ja = calloc((size_t)ns * 3, sizeof(double));
         th[r].rb = calloc((size_t)ns * nn * 2, sizeof(double));
@@ -940,6 +1097,7 @@ int orc_run(const sbe_config *cfg, const sbe_model *model, int64_t k_begin, int6
     }
     for (int r = 0; r < nthreads; ++r) { free(th[r].ji); free(th[r].ja); free(th[r].rb); free(th[r].bp); }
     free(th); free(tid); free(rho); free(kp); free(wk);
+    if (S.velocity) vel_free(&V);
     solver_free(&S);
     return SBE_OK;
 }
@@ -1099,7 +1257,7 @@ int orc_observe(const sbe_config *cfg, const sbe_model *model, const double k[3]
     osolver S;
     int rc = solver_init(cfg, model, &S);
     if (rc) return rc;
-    if (S.N != 2) { solver_free(&S); return fail(SBE_ERR_UNSUPPORTED, "orc_observe is for N = 2"); }
+    if (S.N != 2 || S.velocity) { solver_free(&S); return fail(SBE_ERR_UNSUPPORTED, "orc_observe is for N = 2, LengthWannier"); }
     double s[3], f[3];
     bloch((const cplx *)rho, s);
     bloch((const cplx *)drho, f);
@@ -1117,7 +1275,13 @@ int orc_currents(const sbe_config *cfg, const sbe_model *model, const double k[3
     if (rc) return rc;
     oscratch w;
     scratch_alloc(&w, S.N);
-    currents(&S, k, t, (const cplx *)rho, inter, intra, pop, &w);
+    ovel V = {0};
+    if (S.velocity) {
+        vel_alloc(&V, S.N, 1);
+        vel_setup(&S, k, V.E, V.P, V.rho0, &w);
+    }
+    currents(&S, k, &V, t, (const cplx *)rho, inter, intra, pop, &w);
+    if (S.velocity) vel_free(&V);
     scratch_free(&w);
     solver_free(&S);
     return SBE_OK;
@@ -1180,7 +1344,7 @@ static void *adaptive_main(void *arg)
             double O0[8], D0[8], O1[8], D1[8];
             for (int64_t k = k0; k < k1; ++k) {
                 const cplx *r = T->rho + (size_t)k * nn;
-                rhs(S, T->kp + 3 * k, t, r, st + ((k - k0) * 7 + 0) * nn, &w);
+                rhs(S, T->kp + 3 * k, NULL, t, r, st + ((k - k0) * 7 + 0) * nn, &w);
                 memcpy(cur + (k - k0) * nn, r, sizeof(cplx) * nn);
                 memcpy(der + (k - k0) * nn, st + (k - k0) * 7 * nn, sizeof(cplx) * nn);
             }
@@ -1202,7 +1366,7 @@ static void *adaptive_main(void *arg)
                             tmp[e] = r[e];
                             for (int j = 0; j < i; ++j) tmp[e] += hh * ck_a[i][j] * f[j * nn + e];
                         }
-                        rhs(S, T->kp + 3 * k, t + ck_c[i] * hh, tmp, f + i * nn, &w);
+                        rhs(S, T->kp + 3 * k, NULL, t + ck_c[i] * hh, tmp, f + i * nn, &w);
                     }
                     cplx *o = y5 + (k - k0) * nn, ev[4];
                     for (int e = 0; e < nn; ++e) {
@@ -1227,7 +1391,7 @@ static void *adaptive_main(void *arg)
                     const double tn = last ? t_end : t + hh;
                     for (int64_t k = k0; k < k1; ++k) {
                         cplx *f = st + (k - k0) * 7 * nn;
-                        rhs(S, T->kp + 3 * k, tn, y5 + (k - k0) * nn, f + 6 * nn, &w);
+                        rhs(S, T->kp + 3 * k, NULL, tn, y5 + (k - k0) * nn, f + 6 * nn, &w);
                         memcpy(der + (k - k0) * nn, f + 6 * nn, sizeof(cplx) * nn);
                     }
                     block_observe(T, k0, k1, y5, der, tn, O1, D1);
@@ -1273,6 +1437,8 @@ int orc_run_adaptive(const sbe_config *cfg, const sbe_model *model, int64_t k_be
     if (cfg->rk_order != SBE_RK_CASH_KARP) return fail(SBE_ERR_CONFIG, "orc_run_adaptive needs SBE_RK_CASH_KARP");
     if (!(cfg->rtol > 0.0) || !(cfg->atol >= 0.0)) return fail(SBE_ERR_CONFIG, "rtol must be > 0, atol >= 0");
     if (cfg->T1 > 1e-16) return fail(SBE_ERR_UNSUPPORTED, "T1 is not restated for the adaptive stepper");
+    if (cfg->gauge != SBE_GAUGE_LENGTH_WANNIER)
+        return fail(SBE_ERR_UNSUPPORTED, "the adaptive stepper is restated for the LengthWannier gauge");
     osolver S;
     int rc = solver_init(cfg, model, &S);
     if (rc) return rc;
@@ -1291,10 +1457,20 @@ int orc_run_adaptive(const sbe_config *cfg, const sbe_model *model, int64_t k_be
     rc = orc_kgrid(cfg, model, k_begin, k_end, kp, wk, &dV, &tw);
     if (rc) { free(kp); free(wk); solver_free(&S); return rc; }
     cplx *rho = calloc((size_t)nk * nn, sizeof(cplx));
+    ovel V = {0};
     {
         oscratch w;
         scratch_alloc(&w, N);
-        for (int64_t k = 0; k < nk; ++k) init_state(&S, kp + 3 * k, rho + (size_t)k * nn, &w);
+        if (S.velocity) {
+            vel_alloc(&V, N, (size_t)nk);
+            for (int64_t k = 0; k < nk; ++k) {
+                ovel v = vel_at(&V, N, k);
+                vel_setup(&S, kp + 3 * k, v.E, v.P, v.rho0, &w);
+                memcpy(rho + (size_t)k * nn, v.rho0, sizeof(cplx) * nn); /* dmatrix = initMatrix */
+            }
+        } else {
+            for (int64_t k = 0; k < nk; ++k) init_state(&S, kp + 3 * k, rho + (size_t)k * nn, &w);
+        }
         scratch_free(&w);
     }
     const int64_t nblk = (nk + block - 1) / block;

@@ -52,9 +52,17 @@ int orc_dipole(const sbe_model *m, const double k[3], double *D /*[3][N][N][2]*/
  * of SBEs.h:874-898). */
 int orc_eigh(int n, const double *H, double *evals, double *U);
 
-/* Right-hand side of the LengthWannier SBE at one k (SBEs.h:762-861). */
+/* Right-hand side of the SBE at one k: LengthWannier (SBEs.h:762-861) or
+ * VelocityHamiltonian (SBEs.h:787-805,814-829,862-871, with the per-k
+ * constants of orc_velocity_setup; rho is then band-basis). */
 int orc_rhs(const sbe_config *cfg, const sbe_model *model, const double k[3],
             double t, const double *rho, double *out);
+
+/* VelocityHamiltonian per-k constants (SBEs.h:284-373): E[N] ascending,
+ * P[3][N][N][2] = U^dag J_a(k) U and rho0[N][N][2] = diag(occupations),
+ * with the eigenvector phases fixed as sbe_oracle.c's phase_fix states. */
+int orc_velocity_setup(const sbe_config *cfg, const sbe_model *model, const double k[3],
+                       double *E, double *P, double *rho0);
 
 /* Full propagation of block [k_begin,k_end) over steps [0, nsteps)
  * (nsteps < 0: all Nt), antelope.cpp:297-338. Series as sbe_get_series;

@@ -12,7 +12,7 @@ import pytest
 
 from antelope_amd import _abi as A
 from antelope_amd.models import synthetic_wannier
-from antelope_amd import HaldaneModel, Pulse, RunConfig, SBESolver, laser_eval, query_footprint
+from antelope_amd import HaldaneModel, Pulse, RunConfig, SBEBatch, SBESolver, laser_eval, query_footprint
 from antelope_amd.solver import SBEError, library_path, load_library
 from tests.cases import haldane_cfg1_model, small_config
 
@@ -87,6 +87,13 @@ def test_invalid_inputs_return_status_codes():
     _expect(A.SBE_ERR_UNSUPPORTED, lambda: SBESolver(dataclasses.replace(ck, T1=100.0), m))
     _expect(A.SBE_ERR_CONFIG, lambda: SBESolver(dataclasses.replace(ck, rtol=0.0), m))
     _expect(A.SBE_ERR_UNSUPPORTED, lambda: SBESolver(ck, synthetic_wannier(4, nmax=(1, 1, 0))))
+    # the velocity gauge runs the fixed-step tableaus, FP64 or FP32, not batched
+    vel = dataclasses.replace(cfg, gauge="VelocityHamiltonian")
+    _expect(A.SBE_ERR_UNSUPPORTED, lambda: SBESolver(dataclasses.replace(vel, rk_order=45), m))
+    _expect(A.SBE_ERR_UNSUPPORTED,
+            lambda: SBESolver(dataclasses.replace(vel, precision="split16"), synthetic_wannier(4, nmax=(1, 1, 0))))
+    _expect(A.SBE_ERR_UNSUPPORTED, lambda: SBEBatch([vel, vel], [m, m]))
+    assert query_footprint(vel, m) == query_footprint(cfg, m)
 
 
 def test_laser_eval_and_footprint_without_gpu():
