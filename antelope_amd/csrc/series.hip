@@ -21,23 +21,21 @@ __global__ __launch_bounds__(kReduceThreads) void series_reduce_kernel(const dou
     __shared__ double acc[kReduceThreads];
     const int64_t step = blockIdx.x;
     const double *p = partials + step * static_cast<int64_t>(nrows) * nv;
-    const int vl = nv < 64 ? nv : 64;          // values per pass
+    const int vl = nv < 64 ? nv : 64;          // values per block
     const int rl = kReduceThreads / vl;        // row lanes
     const int tid = threadIdx.x;
     const int vlane = tid % vl, rlane = tid / vl;
-    for (int vb = 0; vb < nv; vb += vl) {
-        const int v = vb + vlane;
-        double s = 0.0;
-        if (rlane < rl && v < nv)
-            for (int r = rlane; r < nrows; r += rl) s += p[static_cast<int64_t>(r) * nv + v];
-        acc[tid] = s;
-        __syncthreads();
-        if (tid < vl && vb + tid < nv) {
-            double t = 0.0;
-            for (int r = 0; r < rl; ++r) t += acc[r * vl + tid];
-            series[step * nv + vb + tid] = t;
-        }
-        __syncthreads();
+    const int vb = blockIdx.y * vl;            // this block's values [vb, vb + vl)
+    const int v = vb + vlane;
+    double s = 0.0;
+    if (rlane < rl && v < nv)
+        for (int r = rlane; r < nrows; r += rl) s += p[static_cast<int64_t>(r) * nv + v];
+    acc[tid] = s;
+    __syncthreads();
+    if (tid < vl && vb + tid < nv) {
+        double t = 0.0;
+        for (int r = 0; r < rl; ++r) t += acc[r * vl + tid];
+        series[step * nv + vb + tid] = t;
     }
 }
 
@@ -47,8 +45,11 @@ hipError_t launch_series_reduce(const double *partials, int64_t nsteps, int32_t 
                                 hipStream_t s)
 {
     if (nsteps <= 0) return hipSuccess;
-    series_reduce_kernel<<<dim3(static_cast<unsigned>(nsteps)), kReduceThreads, 0, s>>>(partials, nrows, nv,
-                                                                                        series);
+    // one block per (step, block of 64 values): the summation order per value
+    // (row lanes, then the lanes in order) does not depend on the grid
+    const int vl = nv < 64 ? nv : 64;
+    const dim3 grid(static_cast<unsigned>(nsteps), static_cast<unsigned>((nv + vl - 1) / vl));
+    series_reduce_kernel<<<grid, kReduceThreads, 0, s>>>(partials, nrows, nv, series);
     return hipGetLastError();
 }
 

@@ -44,6 +44,9 @@ FLOPS_PER_KSTEP = {"fp64": 505.0, "fp32": 522.0}
 KERNEL_NAME = {"fp64": "haldane_steps_kernel<double, 4, false, true, 1>",
                "fp32": "haldane_steps_kernel<float, 4, false, true, 1>"}
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
+# Flops per k-point RK4 step of the velocity-gauge step kernel (hand count:
+# sample 36, four stages of input 6 + h 10 + h x y 12 + dephasing 4, update 27)
+VEL_FLOPS_PER_KSTEP = {"fp64": 190.0, "fp32": 190.0}
 
 
 def wannier_workload(precision: str = "fp64"):
@@ -99,7 +102,8 @@ def timed_solve(cfg, model, steps: int, warmup: int):
     return {"wall_s": wall, "steps": steps, "nk": nk, "timing": tim, "series": series, "info": info}
 
 
-def cpu_baseline(cfg, model, budget_s: float):
+def cpu_baseline(cfg, model, budget_s: float,
+                 method: str = "per-stage Jacobi eigensolve, generic N^3 products"):
     """FP64 CPU oracle (faithful-cost restatement of the reference per-k loop)
     on a bounded sample: a contiguous k block of the same workload for the
     first steps of the pulse, all host threads (capped at 16)."""
@@ -124,7 +128,7 @@ def cpu_baseline(cfg, model, budget_s: float):
         "cores": threads,
         "kind": "port",
         "sample": f"first {nsteps} steps of k points [0,{nk}) of the same workload, FP64 oracle "
-                  f"(per-stage Jacobi eigensolve, generic N^3 products), {sec:.1f} s wall",
+                  f"({method}), {sec:.1f} s wall",
     }
 
 
@@ -167,6 +171,69 @@ def bench_wannier(args) -> int:
                          "note": "algorithmic flops of the whole step over the step time; the Fourier sums "
                                  "are 94% of them, the eigensolver flops are not counted"},
             "timing": r["timing"]}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def bench_velocity(args) -> int:
+    """VelocityHamiltonian gauge (SBEs.h:787-871): the Haldane workload above
+    with gauge = Velocity, FP64 and FP32 state over the whole pulse (the FP32
+    error measured against the FP64 run), and the Wannier N = 16 workload
+    (48 x 48 x 8) in the same gauge for `steps` steps (default 2000). Per-k
+    E(k) and pMatrix = U^dag J U are built once; a step has no eigensolve and
+    no Fourier sum."""
+    import dataclasses
+    forms, res = {}, {}
+    for p in ("fp64", "fp32"):
+        cfg, model = workload(args.grid, p)
+        cfg = dataclasses.replace(cfg, gauge="VelocityHamiltonian")
+        r = timed_solve(cfg, model, args.steps if args.steps > 0 else 10 ** 9, args.warmup)
+        res[p] = (r, cfg, model)
+        forms[f"haldane_{p}"] = {"k_steps_per_s": r["nk"] * r["steps"] / r["wall_s"], "wall_s": r["wall_s"],
+                                 "steps": r["steps"], "step_kernel_ms": r["timing"]["step_ms"],
+                                 "step_launches": r["timing"]["step_launches"],
+                                 "reduce_kernel_ms": r["timing"]["reduce_ms"]}
+        print(f"[bench] velocity haldane {p}: {r['steps']} steps in {r['wall_s']:.3f} s", file=sys.stderr, flush=True)
+    a, b = res["fp64"][0]["series"], res["fp32"][0]["series"]
+    n = min(res["fp64"][0]["steps"], res["fp32"][0]["steps"])
+    forms["haldane_fp32"]["err_j_rel_l2_vs_fp64"] = rel_l2((b["j_inter"] + b["j_intra"])[:n], (a["j_inter"] + a["j_intra"])[:n])
+    forms["haldane_fp32"]["err_pop_rel_l2_vs_fp64"] = rel_l2(b["band_pop"][:n], a["band_pop"][:n])
+    wsteps = args.steps if args.steps > 0 else 2000
+    wcfg, wmodel = wannier_workload("fp64")
+    wcfg = dataclasses.replace(wcfg, gauge="VelocityHamiltonian")
+    w = timed_solve(wcfg, wmodel, wsteps, min(args.warmup, 5))
+    forms["wannier16_fp64"] = {"k_steps_per_s": w["nk"] * w["steps"] / w["wall_s"], "wall_s": w["wall_s"],
+                               "steps": w["steps"], "nk": w["nk"], "ms_per_step": w["wall_s"] * 1e3 / w["steps"]}
+    print(f"[bench] velocity wannier16: {forms['wannier16_fp64']['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
+    best = "fp64"
+    if forms["haldane_fp32"]["err_j_rel_l2_vs_fp64"] <= 1e-6 and \
+            forms["haldane_fp32"]["k_steps_per_s"] > forms["haldane_fp64"]["k_steps_per_s"]:
+        best = "fp32"
+    r, cfg, model = res[best]
+    kst = r["nk"] * r["steps"]
+    sec = r["timing"]["step_ms"] * 1e-3
+    fl = kst * VEL_FLOPS_PER_KSTEP[best] / sec / 1e12
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(res["fp64"][1], res["fp64"][2], args.cpu_seconds,
+                           "velocity gauge: per-k E, U, pMatrix once, generic N^3 products per stage")
+    line = {"metric": METRIC, "value": forms[f"haldane_{best}"]["k_steps_per_s"], "unit": "k-point·RK-steps/s",
+            "n_gpus": 1, "steps": r["steps"], "warmup": args.warmup, "ms_per_step": r["wall_s"] * 1e3 / r["steps"],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f64" if best == "fp64" else "f32 state, f64 sums",
+            "data": "synthetic (seeded analytic Haldane model and laser pulse; seeded Wannier N=16 model)",
+            "config": {"workload": f"VelocityHamiltonian gauge: Haldane 2-band {args.grid}x{args.grid}, elliptic 0.5 "
+                                   f"sin^2 12-cycle pulse, T2=220, RK4 dt=0.25, Nt={r['info'].nt}; Wannier N=16 "
+                                   f"48x48x8 dt=0.1 (first {w['steps']} steps)", "nk": r["nk"], "form": best},
+            "forms": forms,
+            "roofline": {"bound": "valu", "kernel": f"haldane_vel_kernel<{'double' if best == 'fp64' else 'float'}, 4, "
+                                                    f"false, true, 1>",
+                         "achieved": fl, "peak": PEAK_TFLOPS["fp64"], "unit": "TFLOP/s", "frac": fl / PEAK_TFLOPS["fp64"],
+                         "traffic": None, "flops_per_kstep": VEL_FLOPS_PER_KSTEP[best],
+                         "avg_launch_ms": r["timing"]["step_ms"] / max(r["timing"]["step_launches"], 1),
+                         "note": "state and per-k constants stay in registers for a launch; HBM sees the state "
+                                 "once per launch and one row of partial sums per wave and step"},
+            "cpu_baseline": cpu}
     print(json.dumps(line), flush=True)
     return 0
 
@@ -281,7 +348,7 @@ def main(argv=None) -> int:
     ap.add_argument("--steps", type=int, default=0, help="timed steps (default: the whole pulse)")
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--grid", type=int, default=300)
-    ap.add_argument("--workload", choices=["haldane", "wannier16", "serving", "batch"], default="haldane")
+    ap.add_argument("--workload", choices=["haldane", "wannier16", "serving", "batch", "velocity"], default="haldane")
     ap.add_argument("--precision", choices=["both", "fp64", "fp32"], default="both")
     ap.add_argument("--requests", type=int, default=100, help="serving: requests in the stream")
     ap.add_argument("--instances", type=int, default=64, help="batch: solves in the batch")
@@ -316,6 +383,8 @@ def main(argv=None) -> int:
         return bench_serving(args)
     if args.workload == "batch":
         return bench_batch(args)
+    if args.workload == "velocity":
+        return bench_velocity(args)
     forms = ["fp64", "fp32"] if args.precision == "both" else [args.precision]
     res = {}
     for p in forms:
