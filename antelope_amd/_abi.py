@@ -44,6 +44,8 @@ SBE_PREC_FP64, SBE_PREC_FP32, SBE_PREC_SPLIT16 = 0, 1, 2
 SBE_RK4, SBE_RK5_FEHLBERG, SBE_RK_CASH_KARP = 4, 5, 45
 SBE_EFIELD_REFERENCE, SBE_EFIELD_CONSISTENT = 0, 1
 SBE_MODEL_HALDANE, SBE_MODEL_WANNIER = 1, 2
+SBE_MODEL_WILSON_MASS, SBE_MODEL_SQUARE_SBAND, SBE_MODEL_KANE_MELE, SBE_MODEL_XYH_LATTICE, SBE_MODEL_TMDC = 3, 4, 5, 6, 7
+SBE_MODEL_NPARAMS = 24
 SBE_BASIS_ORBITAL, SBE_BASIS_BAND = 0, 1
 
 c_double_p = C.POINTER(C.c_double)
@@ -79,6 +81,7 @@ class sbe_model(C.Structure):
         ("t2", C.c_double), ("phi0", C.c_double), ("M0", C.c_double), ("nrpts", C.c_int32),
         ("reserved", C.c_int32), ("lattice", C.c_double * 9), ("rvec", c_int32_p),
         ("rweight", c_double_p), ("ham", c_double_p), ("pos", c_double_p),
+        ("params", C.c_double * 24),
     ]
 
 
@@ -144,6 +147,7 @@ SIGNATURES = {
     "sbe_server_destroy": ([C.c_void_p], None),
     "sbe_destroy": ([C.c_void_p], None),
     "sbe_last_error": ([C.c_void_p], C.c_char_p),
+    "sbe_tmdc_params": ([C.c_char_p, C.c_int32, C.POINTER(sbe_model)], C.c_int),
 }
 
 

@@ -254,11 +254,102 @@ constexpr double kAuAngstrom = 0.529177210903;  // constant.h:34
 namespace {
 constexpr double kAuEV = 27.211386245988;  // constant.h:35
 thread_local std::string t_base_dir;      // directory of the config file being parsed
+
+// GGA parameter sets of the three-band TMDC model (G.-B. Liu et al., Phys.
+// Rev. B 88, 085433 (2013)) as TMDCs.h:69-115 hard-codes them: lattice
+// constant (Angstrom), spin-orbit coupling and {e1, e2, t0, t1, t2, t11, t12,
+// t22, r0, r1, r2, r11, r12, u0, u1, u2, u11, u12, u22} (eV).
+struct TmdcRow {
+    const char *name;
+    double a_ang, soc_ev;
+    double v[19];
+};
+const TmdcRow kTmdc[6] = {
+    {"MoS2", 3.190, 0.073, {0.683, 1.707, -0.146, -0.114, 0.506, 0.085, 0.162, 0.073, 0.060, -0.236,
+                            0.067, 0.016, 0.087, -0.038, 0.046, 0.001, 0.266, -0.176, -0.150}},
+    {"WS2", 3.191, 0.211, {0.717, 1.916, -0.152, -0.097, 0.590, 0.047, 0.178, 0.016, 0.069, -0.261,
+                           0.107, -0.003, 0.109, -0.054, 0.045, 0.002, 0.325, -0.206, -0.163}},
+    {"MoSe2", 3.326, 0.091, {0.684, 1.546, -0.146, -0.130, 0.432, 0.144, 0.117, 0.075, 0.039, -0.209,
+                             0.069, 0.052, 0.060, -0.042, 0.036, 0.008, 0.272, -0.172, -0.150}},
+    {"WSe2", 3.325, 0.228, {0.728, 1.655, -0.146, -0.124, 0.507, 0.117, 0.127, 0.015, 0.036, -0.234,
+                            0.107, 0.044, 0.075, -0.061, 0.032, 0.007, 0.329, -0.202, -0.164}},
+    {"MoTe2", 3.557, 0.107, {0.588, 1.303, -0.226, -0.234, 0.036, 0.400, 0.098, 0.017, 0.003, -0.025,
+                             -0.169, 0.082, 0.051, 0.057, 0.103, 0.187, -0.045, -0.141, 0.087}},
+    {"WTe2", 3.560, 0.237, {0.697, 1.380, -0.109, -0.164, 0.368, 0.204, 0.093, 0.038, -0.015, -0.209,
+                            0.107, 0.115, 0.009, -0.066, 0.011, -0.013, 0.312, -0.177, -0.132}},
+};
+
+// The analytic classes' constructors (keys, Angstrom conversion, band counts):
+// WilsonMass.h:40-63, SquareSBand.h:39-59, KaneMele.h:47-75, XYHlattice.h:49-73,
+// TMDCs.h:48-149.
+int analytic_target(const std::string &name, const Setting *g, sbe_model &m)
+{
+    auto need = [&](const char *key, double &v) { return get_double(g, key, v); };
+    m.is_dipole_zero = 1;
+    double *p = m.params;
+    if (name == "WilsonMass") {
+        if (!(need("t", p[0]) && need("delta", p[1]) && need("mu", p[2]) && need("a0", p[3])))
+            return fe_fail(SBE_ERR_MODEL, "Some WilsonMass paramters are missing");
+        if (!need("a1", p[4])) p[4] = p[3];  // a.u., no Angstrom conversion in this class
+        m.kind = SBE_MODEL_WILSON_MASS;
+        m.nband = 2;
+        m.nval = 1;
+    } else if (name == "SquareSBand") {
+        if (!(need("t1", p[0]) && need("M0", p[1]) && need("a0", p[2])))
+            return fe_fail(SBE_ERR_MODEL, "Some SquareSBand paramters are missing");
+        p[2] /= kAuAngstrom;
+        m.kind = SBE_MODEL_SQUARE_SBAND;
+        m.nband = 2;
+        m.nval = 1;
+    } else if (name == "KaneMele") {
+        double mt2;
+        if (!(need("t1", p[0]) && need("t2", p[1]) && need("tR", p[2]) && need("Mt2", mt2) && need("phi0", p[4]) &&
+              need("a0", p[5])))
+            return fe_fail(SBE_ERR_MODEL, "Some KaneMele paramters are missing");
+        p[3] = mt2 * p[1];
+        p[5] /= kAuAngstrom;
+        m.kind = SBE_MODEL_KANE_MELE;
+        m.nband = 4;
+        m.nval = 2;
+    } else if (name == "XYHlattice") {
+        if (!(need("tl", p[0]) && need("tr", p[1]) && need("lambda", p[2]) && need("a0", p[3])))
+            return fe_fail(SBE_ERR_MODEL, "Some XYHlattice paramters are missing");
+        p[3] /= kAuAngstrom;
+        m.kind = SBE_MODEL_XYH_LATTICE;
+        m.nband = 4;
+        m.nval = 2;
+    } else {  // TMDCs
+        bool soc = false;
+        get_bool(g, "isSOCEnabled", soc);
+        std::string mat;
+        get_string(g, "Material", mat);
+        if (sbe_tmdc_params(mat.c_str(), soc ? 1 : 0, &m) != SBE_OK)
+            return fe_fail(SBE_ERR_MODEL, "Unknowm material: " + mat);
+    }
+    return SBE_OK;
+}
 }  // namespace
 
 extern "C" {
 
 const char *sbe_frontend_last_error(void) { return g_fe_error.c_str(); }
+
+int sbe_tmdc_params(const char *material, int32_t soc, sbe_model *m)
+{
+    if (!material || !m) return SBE_ERR_CONFIG;
+    for (const TmdcRow &r : kTmdc) {
+        if (std::string(material) != r.name) continue;
+        m->kind = SBE_MODEL_TMDC;
+        m->nband = soc ? 6 : 3;
+        m->nval = soc ? 2 : 1;
+        m->is_dipole_zero = 1;
+        m->params[0] = r.a_ang / kAuAngstrom;
+        m->params[1] = r.soc_ev / kAuEV;
+        for (int i = 0; i < 19; ++i) m->params[2 + i] = r.v[i] / kAuEV;
+        return SBE_OK;
+    }
+    return SBE_ERR_MODEL;
+}
 
 int sbe_spec_parse_string(const char *text, sbe_spec **out)
 {
@@ -393,6 +484,10 @@ int sbe_spec_parse_string(const char *text, sbe_spec **out)
         m.nband = 2;
         m.nval = 1;
         m.is_dipole_zero = 1;
+    } else if (spec->target == "WilsonMass" || spec->target == "SquareSBand" || spec->target == "KaneMele" ||
+               spec->target == "XYHlattice" || spec->target == "TMDCs") {
+        int rc = analytic_target(spec->target, mg, m);
+        if (rc) return rc;
     } else if (mg && mg->child("dataName")) {  // Wannier90.h:107-125
         std::string path;
         if (!get_string(mg, "dataName", path)) return fe_fail(SBE_ERR_CONFIG, "dataName must be a string");
@@ -411,8 +506,9 @@ int sbe_spec_parse_string(const char *text, sbe_spec **out)
         if (rc) return fe_fail(rc, sbe_wannier_last_error());
     } else {
         return fe_fail(SBE_ERR_UNSUPPORTED, "target '" + spec->target +
-                                                "' is not available as a config-file model (Haldane, or a "
-                                                "Wannier90 group with dataName)");
+                                                "' is not available as a config-file model (Haldane, WilsonMass, "
+                                                "SquareSBand, KaneMele, XYHlattice, TMDCs, or a Wannier90 group "
+                                                "with dataName)");
     }
     c.npulses = static_cast<int32_t>(spec->pulses.size());
     c.pulses = spec->pulses.data();

@@ -146,6 +146,22 @@ int Grid::build(const sbe_config &cfg, const sbe_model &m, int64_t kb, int64_t k
     if (m.kind == SBE_MODEL_HALDANE) {
         axes[0] = 2.0 * kPi / std::sqrt(3.0) / m.a0;
         axes[4] = 2.0 * (2.0 * kPi / 3.0 / m.a0);
+    } else if (m.kind != SBE_MODEL_WANNIER) {
+        // rectangles 2 kxMax x 2 kyMax centred at 0 of the analytic classes:
+        // WilsonMass.h:56-60, SquareSBand.h:54-58, KaneMele.h:68-72,
+        // XYHlattice.h:65-69, TMDCs.h:143-148
+        const double *p = m.params, r3 = std::sqrt(3.0);
+        double kx = 0, ky = 0;
+        switch (m.kind) {
+        case SBE_MODEL_WILSON_MASS: kx = kPi / p[3]; ky = kPi / p[4]; break;
+        case SBE_MODEL_SQUARE_SBAND: kx = ky = kPi / p[2]; break;
+        case SBE_MODEL_KANE_MELE: kx = kPi / r3 / p[5]; ky = 2.0 * kPi / 3.0 / p[5]; break;
+        case SBE_MODEL_XYH_LATTICE: kx = kPi / p[3]; ky = 2.0 * kPi / r3 / p[3]; break;
+        case SBE_MODEL_TMDC: kx = kPi / p[0]; ky = 2.0 * kPi / r3 / p[0]; break;
+        default: err = "unknown model kind"; return SBE_ERR_MODEL;
+        }
+        axes[0] = 2.0 * kx;
+        axes[4] = 2.0 * ky;
     } else {
         const double *L = m.lattice;
         const double vol = L[0] * (L[4] * L[8] - L[5] * L[7]) + L[1] * (L[5] * L[6] - L[3] * L[8]) +

@@ -38,6 +38,10 @@ struct NbArgs {
     uint16_t *b16;         // B of one evaluation: planes [re_hi, re_lo, im_hi, im_lo][bcols][nr_pad]
     double hmax, pmax, rmax;  // max |H(R)|, max |r(R)|, max |R_a| (scales of B)
     int velocity;          // VelocityHamiltonian gauge (band basis at k)
+    // analytic tight-binding class (SBE_MODEL_WILSON_MASS ... SBE_MODEL_TMDC):
+    // H and J come from analytic.hpp instead of the Fourier sums; 0 = Wannier
+    int model_kind;
+    double mparams[24];
 };
 
 // One evaluation of the Fourier sums at k + A(t) with field E(t).
@@ -78,6 +82,8 @@ hipError_t nb_launch_phase(const NbArgs &a, double2 *P, hipStream_t s);
 // SPLIT16: e^{ik.R} split into FP16 hi/lo planes once per context
 hipError_t nb_launch_psplit(const NbArgs &a, const double2 *P, uint16_t *p16, hipStream_t s);
 hipError_t nb_launch_fourier(const NbArgs &a, const NbEval &ev, const double2 *P, double2 *out, hipStream_t s);
+// analytic classes: the same matrices [mat][nk_stride][N][N] at k + A (analytic.hip)
+hipError_t nb_launch_model(const NbArgs &a, const NbEval &ev, double2 *out, hipStream_t s);
 hipError_t nb_launch_init(const NbArgs &a, const double2 *H0, double2 *rho, double2 *U, hipStream_t s);
 hipError_t nb_launch_sample(const NbArgs &a, const double2 *mats, const double2 *rho, double *partials,
                             int64_t step_local, double2 *U, int ureuse, hipStream_t s);

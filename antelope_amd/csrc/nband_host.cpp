@@ -9,6 +9,7 @@
 #include <string>
 #include <vector>
 
+#include "analytic.hpp"
 #include "nband.hpp"
 #include "sbe_internal.hpp"
 
@@ -39,6 +40,12 @@ T *nalloc(sbe_ctx *ctx, size_t bytes, int &rc)
 }
 
 constexpr size_t kNbPartialBytes = size_t(128) << 20;
+
+// the matrices {H~, H, J} at k + A: Fourier sums (Wannier) or the analytic classes
+hipError_t eval_mats(const NbArgs &a, const NbEval &ev, const double2 *P, double2 *out, hipStream_t s)
+{
+    return a.model_kind ? nb_launch_model(a, ev, out, s) : nb_launch_fourier(a, ev, P, out, s);
+}
 
 struct NbPlan {
     int64_t nk_stride;
@@ -86,6 +93,31 @@ size_t nb_footprint(const sbe_model &m, int64_t nk, int64_t nt, int nstage, int 
 
 int nb_validate(const sbe_model &m, std::string &err)
 {
+    if (am::is_analytic(m.kind)) {
+        int want = 2;
+        if (m.kind == SBE_MODEL_KANE_MELE || m.kind == SBE_MODEL_XYH_LATTICE) want = 4;
+        if (m.kind == SBE_MODEL_TMDC) want = m.nband == 6 ? 6 : 3;
+        if (m.nband != want) {
+            err = "this model class has " + std::to_string(want) + " bands (TMDC: 3, or 6 with spin-orbit)";
+            return SBE_ERR_MODEL;
+        }
+        for (double v : m.params)
+            if (!std::isfinite(v)) {
+                err = "model parameters must be finite";
+                return SBE_ERR_MODEL;
+            }
+        const int ia0 = m.kind == SBE_MODEL_WILSON_MASS ? 3 : m.kind == SBE_MODEL_SQUARE_SBAND ? 2
+                      : m.kind == SBE_MODEL_KANE_MELE ? 5 : m.kind == SBE_MODEL_XYH_LATTICE ? 3 : 0;
+        if (!(m.params[ia0] > 0.0) || (m.kind == SBE_MODEL_WILSON_MASS && !(m.params[4] > 0.0))) {
+            err = "lattice constants must be > 0";
+            return SBE_ERR_MODEL;
+        }
+        if (m.nval < 0 || m.nval > m.nband) {
+            err = "nval must be within 0..nband";
+            return SBE_ERR_MODEL;
+        }
+        return SBE_OK;
+    }
     if (m.nband < 1 || m.nband > kNbMaxBands) {
         err = "Wannier path supports 1..32 bands";
         return SBE_ERR_UNSUPPORTED;
@@ -112,7 +144,8 @@ int nb_validate(const sbe_model &m, std::string &err)
 int nb_setup(sbe_ctx *ctx)
 {
     const sbe_model &m = ctx->model;
-    const int N = m.nband, nr = m.nrpts;
+    const bool analytic = am::is_analytic(m.kind);
+    const int N = m.nband, nr = analytic ? 0 : m.nrpts;
     const int64_t nk = ctx->grid.k_end - ctx->grid.k_begin;
     const bool split16 = ctx->cfg.precision == SBE_PREC_SPLIT16;
     const NbPlan p = nb_plan(N, nr, nk, ctx->laser.nt, ctx->tab.nstage, split16);
@@ -152,9 +185,9 @@ int nb_setup(sbe_ctx *ctx)
     double2 *dham = reinterpret_cast<double2 *>(tab + bytes_k + bytes_r);
     double2 *dpos = dham + static_cast<size_t>(nr) * nn;
     rc = hipc(ctx, hipMemcpy(dk, hk.data(), bytes_k, hipMemcpyHostToDevice), "upload k");
-    if (!rc) rc = hipc(ctx, hipMemcpy(dr, hr.data(), bytes_r, hipMemcpyHostToDevice), "upload R");
-    if (!rc) rc = hipc(ctx, hipMemcpy(dham, m.ham, static_cast<size_t>(nr) * nn * sizeof(double2), hipMemcpyHostToDevice), "upload H");
-    if (!rc) rc = hipc(ctx, hipMemcpy(dpos, m.pos, static_cast<size_t>(nr) * 3 * nn * sizeof(double2), hipMemcpyHostToDevice), "upload r");
+    if (!rc && nr) rc = hipc(ctx, hipMemcpy(dr, hr.data(), bytes_r, hipMemcpyHostToDevice), "upload R");
+    if (!rc && nr) rc = hipc(ctx, hipMemcpy(dham, m.ham, static_cast<size_t>(nr) * nn * sizeof(double2), hipMemcpyHostToDevice), "upload H");
+    if (!rc && nr) rc = hipc(ctx, hipMemcpy(dpos, m.pos, static_cast<size_t>(nr) * 3 * nn * sizeof(double2), hipMemcpyHostToDevice), "upload r");
     if (rc) return rc;
 
     NbArgs &a = ctx->nargs;
@@ -188,10 +221,12 @@ int nb_setup(sbe_ctx *ctx)
     for (size_t e = 0; e < static_cast<size_t>(nr) * 3 * nn; ++e)
         a.pmax = std::max(a.pmax, std::hypot(m.pos[2 * e], m.pos[2 * e + 1]));
     for (int e = 0; e < 3 * nr; ++e) a.rmax = std::max(a.rmax, std::fabs(hr[e]));
+    a.model_kind = analytic ? m.kind : 0;
+    for (int i = 0; i < 24; ++i) a.mparams[i] = analytic ? m.params[i] : 0.0;
 
     a.velocity = ctx->velocity ? 1 : 0;
     // e^{ik.R}, then rho(k) from H(k) (A(atmin) = 0)
-    rc = hipc(ctx, nb_launch_phase(a, ctx->nb_P, ctx->stream), "phase kernel");
+    if (!analytic) rc = hipc(ctx, nb_launch_phase(a, ctx->nb_P, ctx->stream), "phase kernel");
     if (!rc && split16) rc = hipc(ctx, nb_launch_psplit(a, ctx->nb_P, split, ctx->stream), "phase split kernel");
     NbEval ev{};
     ev.nmat = 1;
@@ -205,14 +240,14 @@ int nb_setup(sbe_ctx *ctx)
         ev.mats[3] = kMatJz;
         double2 *Pv = ctx->nb_k;
         double *Eocc = reinterpret_cast<double *>(ctx->nb_k + 3 * static_cast<size_t>(p.nk_stride) * nn);
-        if (!rc) rc = hipc(ctx, nb_launch_fourier(a, ev, ctx->nb_P, ctx->nb_mats, ctx->stream), "fourier kernel");
+        if (!rc) rc = hipc(ctx, eval_mats(a, ev, ctx->nb_P, ctx->nb_mats, ctx->stream), "fourier kernel");
         if (!rc) rc = hipc(ctx, nb_launch_vel_init(a, ctx->nb_mats, Pv, Eocc, ctx->nb_U, ctx->nb_rho, ctx->stream), "velocity init kernel");
         const size_t lds_p = nb_vel_lds_bytes(N, ctx->tab.nstage, true);
         ctx->nb_vel_p_lds = lds_p <= (size_t(64) << 10);
         if (!rc) rc = hipc(ctx, hipStreamSynchronize(ctx->stream), "init sync");
         return rc;
     }
-    if (!rc) rc = hipc(ctx, nb_launch_fourier(a, ev, ctx->nb_P, ctx->nb_mats, ctx->stream), "fourier kernel");
+    if (!rc) rc = hipc(ctx, eval_mats(a, ev, ctx->nb_P, ctx->nb_mats, ctx->stream), "fourier kernel");
     if (!rc) rc = hipc(ctx, nb_launch_init(a, ctx->nb_mats, ctx->nb_rho, ctx->nb_U, ctx->stream), "init kernel");
     ctx->nb_u_time = ctx->laser.atmin;  // U diagonalises H(k) = H(k + A(atmin))
     ctx->nb_ht_valid = false;
@@ -282,7 +317,7 @@ int nb_run(sbe_ctx *ctx, int64_t step_begin, int64_t step_end)
             // eigenvectors: reused when they already belong to H(k + A(t_n)),
             // otherwise warm-started from the stored ones
             const int ureuse = ctx->nb_u_time == tn;
-            rc = hipc(ctx, nb_launch_fourier(a, ev, ctx->nb_P, ctx->nb_mats, ctx->stream), "fourier kernel");
+            rc = hipc(ctx, eval_mats(a, ev, ctx->nb_P, ctx->nb_mats, ctx->stream), "fourier kernel");
             if (!rc)
                 rc = hipc(ctx, nb_launch_sample(a, ctx->nb_mats, ctx->nb_rho, ctx->d_partials, s, ctx->nb_U, ureuse, ctx->stream),
                           "sample kernel");
@@ -308,7 +343,7 @@ int nb_run(sbe_ctx *ctx, int64_t step_begin, int64_t step_end)
                     se.nmat = need_h ? 2 : 1;
                     se.mats[0] = kMatHtilde;
                     se.mats[1] = kMatH0;
-                    rc = hipc(ctx, nb_launch_fourier(a, se, ctx->nb_P, ctx->nb_mats, ctx->stream), "fourier kernel");
+                    rc = hipc(ctx, eval_mats(a, se, ctx->nb_P, ctx->nb_mats, ctx->stream), "fourier kernel");
                     std::memcpy(ctx->nb_ht_A, se.A, sizeof se.A);
                     std::memcpy(ctx->nb_ht_E, se.E, sizeof se.E);
                     ctx->nb_ht_valid = true;

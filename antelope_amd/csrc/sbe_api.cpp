@@ -16,6 +16,7 @@
 #include <string>
 #include <vector>
 
+#include "analytic.hpp"
 #include "band_frame.hpp"
 #include "sbe_internal.hpp"
 
@@ -100,8 +101,8 @@ int validate(const sbe_config *cfg, const sbe_model *m, std::string &err)
         err = "SPLIT16 precision applies to the Wannier Fourier sums";
         return SBE_ERR_UNSUPPORTED;
     }
-    if (cfg->precision == SBE_PREC_FP32 && m->kind == SBE_MODEL_WANNIER) {
-        err = "the Wannier path runs FP64 or SPLIT16 (FP16-split MFMA Fourier sums)";
+    if (cfg->precision == SBE_PREC_FP32 && m->kind != SBE_MODEL_HALDANE) {
+        err = "the N-band path runs FP64 (or SPLIT16 Fourier sums for Wannier models)";
         return SBE_ERR_UNSUPPORTED;
     }
     if (m->kind == SBE_MODEL_HALDANE) {
@@ -120,7 +121,7 @@ int validate(const sbe_config *cfg, const sbe_model *m, std::string &err)
         }
         return SBE_OK;
     }
-    if (m->kind == SBE_MODEL_WANNIER) return sbe::nb_validate(*m, err);
+    if (m->kind == SBE_MODEL_WANNIER || sbe::am::is_analytic(m->kind)) return sbe::nb_validate(*m, err);
     err = "unknown model kind";
     return SBE_ERR_MODEL;
 }
@@ -299,7 +300,7 @@ int sbe_query_footprint(const sbe_config *cfg, const sbe_model *model, int64_t k
         k_end = ntot;
     }
     if (k_begin < 0 || k_end > ntot || k_begin >= k_end) return set_error(nullptr, SBE_ERR_RANGE, "bad k range");
-    if (model->kind == SBE_MODEL_WANNIER) {
+    if (model->kind != SBE_MODEL_HALDANE) {
         if (bytes) *bytes = sbe::nb_footprint(*model, k_end - k_begin, L.nt, tab.nstage, cfg->precision);
         return SBE_OK;
     }
@@ -358,7 +359,7 @@ int sbe_create(const sbe_config *cfg, const sbe_model *model, int64_t k_begin, i
     }
     (void)hipGetDevice(&ctx->device);
 
-    if (model->kind == SBE_MODEL_WANNIER) {
+    if (model->kind != SBE_MODEL_HALDANE) {
         ctx->path = sbe::PathKind::NBand;
         rc = hip_check(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
         if (!rc) rc = sbe::nb_setup(ctx);

@@ -64,10 +64,13 @@ class Spec:
 
     def solver_model(self):
         """The model as a params object (HaldaneModel / WannierModel, arrays copied)."""
-        from .params import HaldaneModel
+        from .params import _ANALYTIC, AnalyticModel, HaldaneModel
         m = self.model
         if m.kind == A.SBE_MODEL_HALDANE:
             return HaldaneModel(a0=m.a0, t1=m.t1, t2=m.t2, phi0=m.phi0, M0=m.M0, nval=m.nval)
+        for name, (kind, _, _, names) in _ANALYTIC.items():
+            if m.kind == kind:
+                return AnalyticModel(name, list(m.params)[:len(names)], nband=m.nband, nval=m.nval)
         return _wannier_from_c(m)
 
     def to_c(self):

@@ -177,3 +177,72 @@ class WannierModel:
             keep.append(ps)
             m.pos = ps.ctypes.data_as(A.c_double_p)
         return m, keep
+
+
+_ANALYTIC = {  # name -> (kind, bands, valence bands, parameter names)
+    "WilsonMass": (A.SBE_MODEL_WILSON_MASS, 2, 1, ("t", "delta", "mu", "a0", "a1")),
+    "SquareSBand": (A.SBE_MODEL_SQUARE_SBAND, 2, 1, ("t1", "M0", "a0")),
+    "KaneMele": (A.SBE_MODEL_KANE_MELE, 4, 2, ("t1", "t2", "tR", "M0", "phi0", "a0")),
+    "XYHlattice": (A.SBE_MODEL_XYH_LATTICE, 4, 2, ("tl", "tr", "lambda", "a0")),
+    "TMDCs": (A.SBE_MODEL_TMDC, 3, 1, ("a0", "soc", "e1", "e2", "t0", "t1", "t2", "t11", "t12", "t22", "r0", "r1",
+                                       "r2", "r11", "r12", "u0", "u1", "u2", "u11", "u12", "u22")),
+}
+
+
+@dataclass
+class AnalyticModel:
+    """One of the analytic tight-binding classes of Materials/ (WilsonMass,
+    SquareSBand, KaneMele, XYHlattice, TMDCs) with its parameters in atomic
+    units (lengths in bohr; the `from_reference` constructors take the
+    classes' config keys and convert Angstrom where the class does)."""
+    name: str
+    params: Sequence[float]
+    nband: int = 0
+    nval: int = -1
+
+    def __post_init__(self):
+        kind, nb, nv, names = _ANALYTIC[self.name]
+        if self.nband == 0:
+            self.nband = nb
+        if self.nval < 0:
+            self.nval = nv if self.nband == nb else 2
+
+    @property
+    def kind(self) -> int:
+        return _ANALYTIC[self.name][0]
+
+    @classmethod
+    def from_reference(cls, name: str, **keys) -> "AnalyticModel":
+        """Parameters as the class constructor reads them from its config group."""
+        if name == "WilsonMass":  # WilsonMass.h:40-63 (a.u.)
+            return cls(name, [keys["t"], keys["delta"], keys["mu"], keys["a0"], keys.get("a1", keys["a0"])])
+        if name == "SquareSBand":  # SquareSBand.h:39-59
+            return cls(name, [keys["t1"], keys["M0"], keys["a0"] / AU_ANGSTROM])
+        if name == "KaneMele":  # KaneMele.h:47-75: M0 = Mt2 t2
+            return cls(name, [keys["t1"], keys["t2"], keys["tR"], keys["Mt2"] * keys["t2"], keys["phi0"],
+                              keys["a0"] / AU_ANGSTROM])
+        if name == "XYHlattice":  # XYHlattice.h:49-73
+            return cls(name, [keys["tl"], keys["tr"], keys["lambda"], keys["a0"] / AU_ANGSTROM])
+        if name == "TMDCs":  # TMDCs.h:48-149: parameter table by material, 6 bands with spin-orbit
+            return tmdc(keys["Material"], bool(keys.get("isSOCEnabled", False)))
+        raise KeyError(name)
+
+    def to_c(self):
+        m = A.sbe_model()
+        m.kind = self.kind
+        m.nband, m.nval, m.is_dipole_zero = self.nband, self.nval, 1
+        p = list(self.params) + [0.0] * (A.SBE_MODEL_NPARAMS - len(self.params))
+        m.params = (C.c_double * A.SBE_MODEL_NPARAMS)(*p)
+        return m, []
+
+
+def tmdc(material: str, soc: bool = False) -> AnalyticModel:
+    """TMDC three-band model of `material` (MoS2, WS2, MoSe2, WSe2, MoTe2,
+    WTe2) from the library's parameter table (sbe_tmdc_params)."""
+    from .solver import load_library
+    m = A.sbe_model()
+    rc = load_library().sbe_tmdc_params(material.encode(), int(soc), C.byref(m))
+    if rc != 0:
+        raise ValueError(f"unknown TMDC material {material!r}")
+    return AnalyticModel("TMDCs", list(m.params)[:21], nband=m.nband, nval=m.nval)
+

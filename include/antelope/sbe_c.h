@@ -119,8 +119,23 @@ enum { SBE_RK4 = 4, SBE_RK5_FEHLBERG = 5, SBE_RK_CASH_KARP = 45 };
  * E = -dA/dt. Only matters when dipoles are non-zero and for the laser file. */
 enum { SBE_EFIELD_REFERENCE = 0, SBE_EFIELD_CONSISTENT = 1 };
 
-/* Models (material classes of Materials/ that this build carries as data). */
-enum { SBE_MODEL_HALDANE = 1, SBE_MODEL_WANNIER = 2 };
+/* Models (material classes of Materials/ that this build carries as data).
+ * The analytic tight-binding classes take their parameters in
+ * sbe_model.params, already in Hartree / bohr / rad (the config readers
+ * convert Angstrom as the classes do), and run on the N-band path:
+ *   WILSON_MASS   Materials/WilsonMass.h   N=2 Nval=1  {t, delta, mu, a0, a1}
+ *   SQUARE_SBAND  Materials/SquareSBand.h  N=2 Nval=1  {t1, M0, a0}
+ *   KANE_MELE     Materials/KaneMele.h     N=4 Nval=2  {t1, t2, tR, M0, phi0, a0}
+ *   XYH_LATTICE   Materials/XYHlattice.h   N=4 Nval=2  {tl, tr, lambda, a0}
+ *   TMDC          Materials/TMDCs.h        N=3 Nval=1 (N=6 Nval=2 with spin-orbit)
+ *                 {a0, soc, e1, e2, t0, t1, t2, t11, t12, t22, r0, r1, r2, r11,
+ *                  r12, u0, u1, u2, u11, u12, u22}; sbe_tmdc_params
+ *                 (sbe_frontend.h) fills them from the class's GGA table. */
+enum {
+    SBE_MODEL_HALDANE = 1, SBE_MODEL_WANNIER = 2, SBE_MODEL_WILSON_MASS = 3, SBE_MODEL_SQUARE_SBAND = 4,
+    SBE_MODEL_KANE_MELE = 5, SBE_MODEL_XYH_LATTICE = 6, SBE_MODEL_TMDC = 7
+};
+#define SBE_MODEL_NPARAMS 24
 
 /* Basis for sbe_get_state. */
 enum { SBE_BASIS_ORBITAL = 0, SBE_BASIS_BAND = 1 };
@@ -187,6 +202,8 @@ typedef struct sbe_model {
     const double *rweight;  /* [nrpts] 1/degeneracy                        */
     const double *ham;      /* [nrpts][N][N][re,im] <0m|H|Rn> (Hartree)    */
     const double *pos;      /* [nrpts][3][N][N][re,im] <0m|r_i|Rn> (bohr)  */
+    /* analytic tight-binding classes (layouts above) */
+    double params[SBE_MODEL_NPARAMS];
 } sbe_model;
 
 /* Read-only facts about a context. */

@@ -25,7 +25,9 @@ extern "C" {
 
 typedef struct sbe_spec sbe_spec;
 
-/* Parse inputParam.cfg text / file. Extensions beyond the reference's keys:
+/* Parse inputParam.cfg text / file. Targets: Haldane, WilsonMass, SquareSBand,
+ * KaneMele, XYHlattice, TMDCs (the classes' own keys and unit conversions)
+ * and Wannier90 groups with dataName. Extensions beyond the reference's keys:
  * calc.grid = "mp", calc.precision = "fp32", calc.method = "cash_karp",
  * calc.rtol/atol, laser.efield = "consistent", env_name = "sin2". */
 int sbe_spec_parse_string(const char *text, sbe_spec **out);
@@ -66,6 +68,13 @@ int sbe_wannier_tb_model(const sbe_wannier_tb *tb, int32_t nval, sbe_model *m);
 void sbe_wannier_tb_free(sbe_wannier_tb *tb);
 int sbe_wannier_write_tb(const char *path, const sbe_model *m, const char *header);
 const char *sbe_wannier_last_error(void);
+
+/* Fill a TMDC model (SBE_MODEL_TMDC, sbe_c.h) for material "MoS2", "WS2",
+ * "MoSe2", "WSe2", "MoTe2" or "WTe2" from the GGA parameter table the
+ * reference's TMDCs class hard-codes (TMDCs.h:48-149): lattice constant,
+ * spin-orbit coupling and the 19 hoppings, converted to bohr and Hartree;
+ * soc != 0 gives the 6-band model. SBE_ERR_MODEL for an unknown material. */
+int sbe_tmdc_params(const char *material, int32_t soc, sbe_model *m);
 
 /* Write the reference's output files into `dir` from summed series (the
  * layout of sbe_get_series; several ranks' series added together). */

@@ -259,6 +259,13 @@ static int model_init(const sbe_model *m, omodel *om)
                     om->tau[3 * n + i] = m->pos[((((size_t)r0 * 3 + i) * N + n) * N + n) * 2];
         return SBE_OK;
     }
+    if (m->kind >= SBE_MODEL_WILSON_MASS && m->kind <= SBE_MODEL_TMDC) {
+        int want = (m->kind == SBE_MODEL_KANE_MELE || m->kind == SBE_MODEL_XYH_LATTICE) ? 4
+                   : m->kind == SBE_MODEL_TMDC ? (m->nband == 6 ? 6 : 3) : 2;
+        if (m->nband != want) return fail(SBE_ERR_MODEL, "model class has %d bands", want);
+        om->N = want;
+        return SBE_OK;
+    }
     return fail(SBE_ERR_MODEL, "unknown model kind %d", m->kind);
 }
 
@@ -390,23 +397,253 @@ static void wannier_J(const omodel *om, const double k[3], cplx *J)
             }
 }
 
+/* ---------------------------------------------- analytic model classes */
+/* 2x2 from Pauli components (b0, b1, b2, b3): [[b0+b3, b1-ib2], [b1+ib2, b0-b3]]
+ * at rows/cols (o, o+1) of an N x N matrix */
+static void pauli_block(cplx *M, int N, int o, double b0, double b1, double b2, double b3)
+{
+    M[o * N + o] = b0 + b3;
+    M[o * N + o + 1] = b1 - I * b2;
+    M[(o + 1) * N + o] = b1 + I * b2;
+    M[(o + 1) * N + o + 1] = b0 - b3;
+}
+
+/* WilsonMass::GenHamiltonian / GenJMatrix, WilsonMass.h:65-92,110-116 */
+static void wilson_mass_HJ(const double *p, const double k[3], cplx *H, cplx *J)
+{
+    double t = p[0], dl = p[1], mu = p[2], a0 = p[3], a1 = p[4];
+    double B[4] = {0., dl * sin(a0 * k[0]), dl * sin(a1 * k[1]), -2. * t * (cos(a0 * k[0]) + cos(a1 * k[1])) - mu};
+    if (H) pauli_block(H, 2, 0, B[0], B[1], B[2], B[3]);
+    if (J) {
+        double X[4] = {0., a0 * dl * cos(a0 * k[0]), 0., 2. * t * a0 * sin(a0 * k[0])};
+        double Y[4] = {0., 0., a1 * dl * cos(a1 * k[1]), 2. * t * a1 * sin(a1 * k[1])};
+        pauli_block(J, 2, 0, -X[0], -X[1], -X[2], -X[3]);
+        pauli_block(J + 4, 2, 0, -Y[0], -Y[1], -Y[2], -Y[3]);
+    }
+}
+
+/* SquareSBand.h:61-78,95-101: J has only the (0,1) element */
+static void square_sband_HJ(const double *p, const double k[3], cplx *H, cplx *J)
+{
+    double t1 = p[0], M0 = p[1], a0 = p[2];
+    if (H) pauli_block(H, 2, 0, 0., 2 * t1 * (cos(a0 * k[0]) + cos(a0 * k[1])), 0., M0);
+    if (J) {
+        J[1] = 2 * t1 * a0 * sin(a0 * k[0]);
+        J[4 + 1] = 2 * t1 * a0 * sin(a0 * k[1]);
+    }
+}
+
+static void honeycomb_vectors(double a0, double va[3][2], double vb[3][2])
+{
+    va[0][0] = 0.;                 va[0][1] = a0;
+    va[1][0] = -sqrt(3.) / 2 * a0; va[1][1] = -0.5 * a0;
+    va[2][0] = sqrt(3.) / 2 * a0;  va[2][1] = -0.5 * a0;
+    vb[0][0] = sqrt(3.) * a0;      vb[0][1] = 0.;
+    vb[1][0] = -sqrt(3.) / 2 * a0; vb[1][1] = 3. / 2. * a0;
+    vb[2][0] = -sqrt(3.) / 2 * a0; vb[2][1] = -3. / 2. * a0;
+}
+
+/* KaneMele::GenHamiltonian / GenJMatrix / GenBcomp / GenRcomp, KaneMele.h:77-248 */
+static void kane_mele_HJ(const double *p, const double k[3], cplx *H, cplx *J)
+{
+    double t1 = p[0], t2 = p[1], tR = p[2], M0 = p[3], phi0 = p[4], a0 = p[5];
+    double va[3][2], vb[3][2];
+    honeycomb_vectors(a0, va, vb);
+    const double cR[3][2] = {{1., 0.}, {-0.5, sqrt(3.) / 2}, {-0.5, -sqrt(3.) / 2}};
+    double aa[3], ab[3];
+    for (int i = 0; i < 3; ++i) {
+        aa[i] = k[0] * va[i][0] + k[1] * va[i][1];
+        ab[i] = k[0] * vb[i][0] + k[1] * vb[i][1];
+    }
+    if (H) {
+        for (int blk = 0; blk < 2; ++blk) {
+            double ph = blk ? -phi0 : phi0, B[4] = {0, 0, 0, 0};
+            for (int i = 0; i < 3; ++i) {
+                B[0] += cos(ab[i]); B[1] += cos(aa[i]); B[2] += sin(aa[i]); B[3] += sin(ab[i]);
+            }
+            B[0] *= 2. * t2 * cos(ph); B[1] *= t1; B[2] *= t1; B[3] = M0 - 2. * t2 * sin(ph) * B[3];
+            pauli_block(H, 4, 2 * blk, B[0], B[1], B[2], B[3]);
+        }
+        cplx R0 = 0, R1 = 0;
+        for (int i = 0; i < 3; ++i) {
+            R0 += (cR[i][0] - I * cR[i][1]) * cexp(I * aa[i]);
+            R1 += (cR[i][0] - I * cR[i][1]) * cexp(-I * aa[i]);
+        }
+        R0 *= I * tR; R1 *= I * tR;
+        H[3] = R0; H[6] = R1; H[9] = conj(R1); H[12] = conj(R0);
+    }
+    if (J) {
+        for (int ax = 0; ax < 2; ++ax) {
+            cplx *Ja = J + 16 * ax;
+            double d[4] = {0, 0, 0, 0};
+            for (int i = 0; i < 3; ++i) {
+                d[0] += vb[i][ax] * sin(ab[i]); d[1] += va[i][ax] * sin(aa[i]);
+                d[2] += va[i][ax] * cos(aa[i]); d[3] += vb[i][ax] * cos(ab[i]);
+            }
+            d[0] *= -2. * t2 * cos(phi0); d[1] *= -t1; d[2] *= t1; d[3] *= -2. * t2 * sin(phi0);
+            pauli_block(Ja, 4, 0, -d[0], -d[1], -d[2], -d[3]);
+            pauli_block(Ja, 4, 2, -d[0], -d[1], -d[2], d[3]); /* -phi0: sin(phi0) -> -sin(phi0) */
+            cplx R[4] = {0, 0, 0, 0};
+            for (int i = 0; i < 3; ++i) {
+                cplx cm = cR[i][0] - I * cR[i][1], cp = cR[i][0] + I * cR[i][1];
+                R[0] += cm * cexp(I * aa[i]) * I * va[i][ax];
+                R[1] += cm * cexp(-I * aa[i]) * -I * va[i][ax];
+                R[2] -= cp * cexp(I * aa[i]) * I * va[i][ax];
+                R[3] -= cp * cexp(-I * aa[i]) * -I * va[i][ax];
+            }
+            for (int i = 0; i < 3; ++i) R[i] *= I * tR; /* not R[3] (KaneMele.h:155,174) */
+            Ja[3] = R[0]; Ja[6] = R[1]; Ja[9] = R[2]; Ja[12] = R[3];
+        }
+    }
+}
+
+/* XYHlattice::GenHamiltonian / GenJMatrix / GenBcomp, XYHlattice.h:75-183 */
+static void xyh_fill(cplx *M, double tl, double tr, cplx p0, cplx p1, cplx p2)
+{
+    cplx ap = p0 + p1, am = p0 - p1;
+    M[2] = (3. / 4 * tl + 1. / 4 * tr) * ap + tr * p2;
+    M[3] = sqrt(3.) / 4 * (tl - tr) * am;
+    M[6] = M[3];
+    M[7] = (1. / 4 * tl + 3. / 4 * tr) * ap + tl * p2;
+    M[8] = conj(M[2]); M[9] = conj(M[6]); M[12] = conj(M[3]); M[13] = conj(M[7]);
+}
+
+static void xyh_lattice_HJ(const double *p, const double k[3], cplx *H, cplx *J)
+{
+    double tl = p[0], tr = p[1], lam = p[2], a0 = p[3];
+    double va[3][2] = {{0., a0 / sqrt(3.)}, {-0.5 * a0, -0.5 * a0 / sqrt(3.)}, {0.5 * a0, -0.5 * a0 / sqrt(3.)}};
+    cplx ph[3];
+    for (int i = 0; i < 3; ++i) ph[i] = cexp(-I * (k[0] * va[i][0] + k[1] * va[i][1]));
+    if (H) {
+        H[0] = lam; H[5] = lam; H[10] = -lam; H[15] = -lam;
+        xyh_fill(H, tl, tr, ph[0], ph[1], ph[2]);
+    }
+    if (J)
+        for (int ax = 0; ax < 2; ++ax)
+            xyh_fill(J + 16 * ax, tl, tr, -I * va[0][ax] * ph[0], -I * va[1][ax] * ph[1], -I * va[2][ax] * ph[2]);
+}
+
+/* TMDCs::GenHamiltonian / GenJMatrix, TMDCs.h:151-283 */
+static void tmdc_fill(cplx *M, int N, int o, double v0, cplx v1, cplx v2, double v11, cplx v12, double v22, double s)
+{
+    M[(o + 0) * N + o + 0] = v0;       M[(o + 0) * N + o + 1] = v1;            M[(o + 0) * N + o + 2] = v2;
+    M[(o + 1) * N + o + 0] = conj(v1); M[(o + 1) * N + o + 1] = v11;           M[(o + 1) * N + o + 2] = v12 + I * s;
+    M[(o + 2) * N + o + 0] = conj(v2); M[(o + 2) * N + o + 1] = conj(v12) - I * s; M[(o + 2) * N + o + 2] = v22;
+}
+
+static void tmdc_HJ(const double *p, int N, const double k[3], cplx *H, cplx *J)
+{
+    double a0 = p[0], soc = p[1], e1 = p[2], e2 = p[3], t0 = p[4], t1 = p[5], t2 = p[6], t11 = p[7], t12 = p[8],
+           t22 = p[9], r0 = p[10], r1 = p[11], r2 = p[12], r11 = p[13], r12 = p[14], u0 = p[15], u1 = p[16],
+           u2 = p[17], u11 = p[18], u12 = p[19], u22 = p[20];
+    double r3 = sqrt(3.), al = k[0] * a0 / 2, be = k[1] * a0 * r3 / 2;
+    double ca[4], sa[4], cb[4], sb[4], dca[4], dsa[4], dcb[4], dsb[4];
+    for (int i = 0; i < 4; ++i) {
+        ca[i] = cos((i + 1) * al); cb[i] = cos((i + 1) * be);
+        sa[i] = sin((i + 1) * al); sb[i] = sin((i + 1) * be);
+        dca[i] = -(i + 1) * a0 / 2 * sin((i + 1) * al); dcb[i] = -(i + 1) * a0 * r3 / 2 * sin((i + 1) * be);
+        dsa[i] = (i + 1) * a0 / 2 * cos((i + 1) * al);  dsb[i] = (i + 1) * a0 * r3 / 2 * cos((i + 1) * be);
+    }
+    int soc_on = N == 6;
+    if (H) {
+        double V0 = e1 + 2 * t0 * (2 * ca[0] * cb[0] + ca[1]) + 2 * r0 * (2 * ca[2] * cb[0] + cb[1]) +
+                    2 * u0 * (2 * ca[1] * cb[1] + ca[3]);
+        cplx V1 = (-2 * r3 * t2 * sa[0] * sb[0] + 2 * (r1 + r2) * sa[2] * sb[0] - 2 * r3 * u2 * sa[1] * sb[1]) +
+                  I * (2 * t1 * sa[0] * (2 * ca[0] + cb[0]) + 2 * (r1 - r2) * sa[2] * cb[0] +
+                       2 * u1 * sa[1] * (2 * ca[1] + cb[1]));
+        cplx V2 = (2 * t2 * (ca[1] - ca[0] * cb[0]) - 2 / r3 * (r1 + r2) * (ca[2] * cb[0] - cb[1]) +
+                   2 * u2 * (ca[3] - ca[1] * cb[1])) +
+                  I * (2 * r3 * t1 * ca[0] * sb[0] + 2 / r3 * sb[0] * (r1 - r2) * (ca[2] + 2 * cb[0]) +
+                       2 * r3 * u1 * ca[1] * sb[1]);
+        double V11 = e2 + (t11 + 3 * t22) * ca[0] * cb[0] + 2 * t11 * ca[1] + 4 * r11 * ca[2] * cb[0] +
+                     2 * (r11 + r3 * r12) * cb[1] + (u11 + 3 * u22) * ca[1] * cb[1] + 2 * u11 * ca[3];
+        cplx V12 = (r3 * (t22 - t11) * sa[0] * sb[0] + 4 * r12 * sa[2] * sb[0] + r3 * (u22 - u11) * sa[1] * sb[1]) +
+                   I * (4 * t12 * sa[0] * (ca[0] - cb[0]) + 4 * u12 * sa[1] * (ca[1] - cb[1]));
+        double V22 = e2 + (3 * t11 + t22) * ca[0] * cb[0] + 2 * t22 * ca[1] + 2 * r11 * (2 * ca[2] * cb[0] + cb[1]) +
+                     2 / r3 * r12 * (4 * ca[2] * cb[0] - cb[1]) + (3 * u11 + u22) * ca[1] * cb[1] + 2 * u22 * ca[3];
+        tmdc_fill(H, N, 0, V0, V1, V2, V11, V12, V22, soc_on ? soc : 0.);
+        if (soc_on) tmdc_fill(H, N, 3, V0, V1, V2, V11, V12, V22, -soc);
+    }
+    if (J) {
+        double dxV0 = 2 * t0 * (2 * dca[0] * cb[0] + dca[1]) + 2 * r0 * (2 * dca[2] * cb[0]) +
+                      2 * u0 * (2 * dca[1] * cb[1] + dca[3]);
+        cplx dxV1 = (-2 * r3 * t2 * dsa[0] * sb[0] + 2 * (r1 + r2) * dsa[2] * sb[0] - 2 * r3 * u2 * dsa[1] * sb[1]) +
+                    I * (2 * t1 * dsa[0] * (2 * ca[0] + cb[0]) + 2 * t1 * sa[0] * (2 * dca[0]) +
+                         2 * (r1 - r2) * dsa[2] * cb[0] + 2 * u1 * dsa[1] * (2 * ca[1] + cb[1]) +
+                         2 * u1 * sa[1] * (2 * dca[1]));
+        cplx dxV2 = (2 * t2 * (dca[1] - dca[0] * cb[0]) - 2 / r3 * (r1 + r2) * (dca[2] * cb[0]) +
+                     2 * u2 * (dca[3] - dca[1] * cb[1])) +
+                    I * (2 * r3 * t1 * dca[0] * sb[0] + 2 / r3 * sb[0] * (r1 - r2) * (dca[2]) +
+                         2 * r3 * u1 * dca[1] * sb[1]);
+        double dxV11 = (t11 + 3 * t22) * dca[0] * cb[0] + 2 * t11 * dca[1] + 4 * r11 * dca[2] * cb[0] +
+                       (u11 + 3 * u22) * dca[1] * cb[1] + 2 * u11 * dca[3];
+        cplx dxV12 = (r3 * (t22 - t11) * dsa[0] * sb[0] + 4 * r12 * dsa[2] * sb[0] + r3 * (u22 - u11) * dsa[1] * sb[1]) +
+                     I * (4 * t12 * dsa[0] * (ca[0] - cb[0]) + 4 * t12 * sa[0] * (dca[0]) +
+                          4 * u12 * dsa[1] * (ca[1] - cb[1]) + 4 * u12 * sa[1] * (dca[1]));
+        double dxV22 = (3 * t11 + t22) * dca[0] * cb[0] + 2 * t22 * dca[1] + 2 * r11 * (2 * dca[2] * cb[0]) +
+                       2 / r3 * r12 * (4 * dca[2] * cb[0]) + (3 * u11 + u22) * dca[1] * cb[1] + 2 * u22 * dca[3];
+        double dyV0 = 2 * t0 * (2 * ca[0] * dcb[0]) + 2 * r0 * (2 * ca[2] * dcb[0] + dcb[1]) +
+                      2 * u0 * (2 * ca[1] * dcb[1]);
+        cplx dyV1 = (-2 * r3 * t2 * sa[0] * dsb[0] + 2 * (r1 + r2) * sa[2] * dsb[0] - 2 * r3 * u2 * sa[1] * dsb[1]) +
+                    I * (2 * t1 * sa[0] * (dcb[0]) + 2 * (r1 - r2) * sa[2] * dcb[0] + 2 * u1 * sa[1] * (dcb[1]));
+        cplx dyV2 = (2 * t2 * (-ca[0] * dcb[0]) - 2 / r3 * (r1 + r2) * (ca[2] * dcb[0] - dcb[1]) +
+                     2 * u2 * (-ca[1] * dcb[1])) +
+                    I * (2 * r3 * t1 * ca[0] * dsb[0] + 2 / r3 * dsb[0] * (r1 - r2) * (ca[2] + 2 * cb[0]) +
+                         2 / r3 * sb[0] * (r1 - r2) * (2 * dcb[0]) + 2 * r3 * u1 * ca[1] * dsb[1]);
+        double dyV11 = (t11 + 3 * t22) * ca[0] * dcb[0] + 4 * r11 * ca[2] * dcb[0] + 2 * (r11 + r3 * r12) * dcb[1] +
+                       (u11 + 3 * u22) * ca[1] * dcb[1];
+        cplx dyV12 = (r3 * (t22 - t11) * sa[0] * dsb[0] + 4 * r12 * sa[2] * dsb[0] + r3 * (u22 - u11) * sa[1] * dsb[1]) +
+                     I * (4 * t12 * sa[0] * (-dcb[0]) + 4 * u12 * sa[1] * (-dcb[1]));
+        double dyV22 = (3 * t11 + t22) * ca[0] * dcb[0] + 2 * r11 * (2 * ca[2] * dcb[0] + dcb[1]) +
+                       2 / r3 * r12 * (4 * ca[2] * dcb[0] - dcb[1]) + (3 * u11 + u22) * ca[1] * dcb[1];
+        int nn = N * N;
+        tmdc_fill(J, N, 0, dxV0, dxV1, dxV2, dxV11, dxV12, dxV22, 0.);
+        tmdc_fill(J + nn, N, 0, dyV0, dyV1, dyV2, dyV11, dyV12, dyV22, 0.);
+        if (soc_on) {
+            tmdc_fill(J, N, 3, dxV0, dxV1, dxV2, dxV11, dxV12, dxV22, 0.);
+            tmdc_fill(J + nn, N, 3, dyV0, dyV1, dyV2, dyV11, dyV12, dyV22, 0.);
+        }
+    }
+}
+
+static int is_analytic(int kind) { return kind >= SBE_MODEL_WILSON_MASS && kind <= SBE_MODEL_TMDC; }
+
+/* H (if H) and J[3] (if J) of an analytic class; J_z stays zero */
+static void analytic_HJ(const omodel *om, const double k[3], cplx *H, cplx *J)
+{
+    const sbe_model *m = om->m;
+    int N = om->N;
+    if (H) memset(H, 0, sizeof(cplx) * N * N);
+    if (J) memset(J, 0, sizeof(cplx) * 3 * N * N);
+    switch (m->kind) {
+    case SBE_MODEL_WILSON_MASS: wilson_mass_HJ(m->params, k, H, J); break;
+    case SBE_MODEL_SQUARE_SBAND: square_sband_HJ(m->params, k, H, J); break;
+    case SBE_MODEL_KANE_MELE: kane_mele_HJ(m->params, k, H, J); break;
+    case SBE_MODEL_XYH_LATTICE: xyh_lattice_HJ(m->params, k, H, J); break;
+    case SBE_MODEL_TMDC: tmdc_HJ(m->params, N, k, H, J); break;
+    default: break;
+    }
+}
+
 static void model_H(const omodel *om, const double k[3], cplx *H)
 {
     if (om->m->kind == SBE_MODEL_HALDANE) haldane_H(om, k, H);
+    else if (is_analytic(om->m->kind)) analytic_HJ(om, k, H, NULL);
     else wannier_H(om, k, H);
 }
 
 static void model_J(const omodel *om, const double k[3], cplx *J)
 {
     if (om->m->kind == SBE_MODEL_HALDANE) haldane_J(om, k, J);
+    else if (is_analytic(om->m->kind)) analytic_HJ(om, k, NULL, J);
     else wannier_J(om, k, J);
 }
 
 static void model_D(const omodel *om, const double k[3], cplx *D)
 {
     int N = om->N;
-    if (om->m->kind == SBE_MODEL_HALDANE || om->m->is_dipole_zero)
-        memset(D, 0, sizeof(cplx) * 3 * N * N); /* Haldane::GenDipole does nothing */
+    if (om->m->kind != SBE_MODEL_WANNIER || om->m->is_dipole_zero)
+        memset(D, 0, sizeof(cplx) * 3 * N * N); /* the analytic classes' GenDipole do nothing */
     else wannier_D(om, k, D);
 }
 
@@ -537,6 +774,16 @@ static void bz_select(const sbe_config *cfg, const sbe_model *m, double ax[9], d
     memset(org, 0, 3 * sizeof(double));
     if (m->kind == SBE_MODEL_HALDANE) {
         double kx = PI / sqrt(3.) / m->a0, ky = 2. * PI / 3. / m->a0;
+        ax[0] = 2 * kx; ax[4] = 2 * ky;
+    } else if (m->kind != SBE_MODEL_WANNIER) {
+        /* GenBrillouinzone of the classes: rectangles 2 kxMax x 2 kyMax at 0 */
+        const double *p = m->params;
+        double kx = 0, ky = 0;
+        if (m->kind == SBE_MODEL_WILSON_MASS) { kx = PI / p[3]; ky = PI / p[4]; }            /* WilsonMass.h:56 */
+        if (m->kind == SBE_MODEL_SQUARE_SBAND) { kx = PI / p[2]; ky = PI / p[2]; }           /* SquareSBand.h:54 */
+        if (m->kind == SBE_MODEL_KANE_MELE) { kx = PI / sqrt(3.) / p[5]; ky = 2. * PI / 3. / p[5]; } /* KaneMele.h:68 */
+        if (m->kind == SBE_MODEL_XYH_LATTICE) { kx = PI / p[3]; ky = 2. * PI / sqrt(3.) / p[3]; }    /* XYHlattice.h:65 */
+        if (m->kind == SBE_MODEL_TMDC) { kx = PI / p[0]; ky = 2. * PI / sqrt(3.) / p[0]; }          /* TMDCs.h:143 */
         ax[0] = 2 * kx; ax[4] = 2 * ky;
     } else {
         const double *a = m->lattice;

@@ -83,7 +83,8 @@ def test_libconfig_syntax_features():
     (BASE.replace('env_name = "cos4"', 'env_name = "cos2"') % "", A.SBE_ERR_CONFIG),
     (BASE.replace("ellip = 0.5;", "") % "", A.SBE_ERR_CONFIG),
     (BASE.replace("M0 = 0.01;", "") % "", A.SBE_ERR_MODEL),
-    (BASE.replace('target = "Haldane"', 'target = "KaneMele"') % "", A.SBE_ERR_UNSUPPORTED),
+    (BASE.replace('target = "Haldane"', 'target = "Haldane2L"') % "", A.SBE_ERR_UNSUPPORTED),
+    (BASE.replace('target = "Haldane"', 'target = "KaneMele"') % "", A.SBE_ERR_MODEL),
     ("calc = { Npoints = [4, 4] ", A.SBE_ERR_CONFIG),
     ("calc = { Npoints = [4, 4]; dt = 1.2.3; };", A.SBE_ERR_CONFIG),
 ])
@@ -209,3 +210,38 @@ def test_wannier_tb_errors(tmp_path):
     m = read_tb(f, nval=1)                 # no position part: dipoles zero
     assert m.is_dipole_zero and m.pos is None
     np.testing.assert_allclose(m.ham[0].real, np.diag([0.1, -0.1]) / 27.211386245988, rtol=1e-15)
+
+
+ANALYTIC_CFG = """
+target = "%s";
+gauge = "%s";
+laser = { pulses = ( { E0 = 0.003; w0 = 0.014; ncycles = 2; ellip = 0.0; cep = 0.0; t0 = 0.0; phix = 0.0;
+                       env_name = "sin2"; } ); };
+calc = { Npoints = [6, 6]; dt = 0.25; T2 = 150.; ksfactor = [1.0, 1.0, 1.0]; RKorder = 4; InterIntra = true; };
+%s
+"""
+
+
+@pytest.mark.parametrize("target,group,keys", [
+    ("WilsonMass", "WilsonMass = { t = 0.02; delta = 0.03; mu = 0.01; a0 = 4.0; };",
+     dict(t=0.02, delta=0.03, mu=0.01, a0=4.0)),
+    ("SquareSBand", "SquareSBand = { t1 = 0.03; M0 = 0.05; a0 = 2.5; };", dict(t1=0.03, M0=0.05, a0=2.5)),
+    ("KaneMele", "KaneMele = { t1 = 0.075; t2 = 0.025; tR = 0.01; Mt2 = 1.5; phi0 = 0.9; a0 = 1.42; };",
+     dict(t1=0.075, t2=0.025, tR=0.01, Mt2=1.5, phi0=0.9, a0=1.42)),
+    ("XYHlattice", "XYHlattice = { tl = 0.05; tr = 0.03; lambda = 0.01; a0 = 2.46; };",
+     {"tl": 0.05, "tr": 0.03, "lambda": 0.01, "a0": 2.46}),
+    ("TMDCs", 'TMDCs = { Material = "WSe2"; isSOCEnabled = true; };', dict(Material="WSe2", isSOCEnabled=True)),
+])
+def test_analytic_targets(target, group, keys):
+    """The analytic classes' config groups (their constructors' keys and
+    Angstrom conversions) give the same model as the Python mirror."""
+    from antelope_amd import AnalyticModel
+    s = parse_config(ANALYTIC_CFG % (target, "Velocity", group))
+    assert s.config.gauge == A.SBE_GAUGE_VELOCITY_HAMILTONIAN
+    got = s.solver_model()
+    want = AnalyticModel.from_reference(target, **keys)
+    assert got.name == want.name and got.nband == want.nband and got.nval == want.nval
+    np.testing.assert_allclose(got.params, want.params, rtol=1e-15)
+    with pytest.raises(SBEError) as e:
+        parse_config(ANALYTIC_CFG % (target, "Wannier", f"{target} = {{ }};"))
+    assert "missing" in str(e.value) or "Unknowm" in str(e.value)
