@@ -45,6 +45,7 @@ SBE_RK4, SBE_RK5_FEHLBERG, SBE_RK_CASH_KARP = 4, 5, 45
 SBE_EFIELD_REFERENCE, SBE_EFIELD_CONSISTENT = 0, 1
 SBE_MODEL_HALDANE, SBE_MODEL_WANNIER = 1, 2
 SBE_MODEL_WILSON_MASS, SBE_MODEL_SQUARE_SBAND, SBE_MODEL_KANE_MELE, SBE_MODEL_XYH_LATTICE, SBE_MODEL_TMDC = 3, 4, 5, 6, 7
+SBE_MODEL_HALDANE_2L, SBE_MODEL_BI2SE3_SURF, SBE_MODEL_MULTI_WSM, SBE_MODEL_WDSM_TAAS = 8, 9, 10, 11
 SBE_MODEL_NPARAMS = 24
 SBE_BASIS_ORBITAL, SBE_BASIS_BAND = 0, 1
 

@@ -28,7 +28,7 @@ __global__ __launch_bounds__(64) void nb_model_kernel(const NbArgs a, const NbEv
             J[t - kMatJx] = M;
         }
     }
-    am::analytic_eval(a.model_kind, N, a.mparams, kk, H, J[0], J[1]);
+    am::analytic_eval(a.model_kind, N, a.mparams, kk, H, J[0], J[1], J[2]);
     if (hcopy >= 0) {
         double2 *M = out + (static_cast<int64_t>(hcopy) * a.nk_stride + k) * nn;
         for (int e = 0; e < nn; ++e) M[e] = H[e];

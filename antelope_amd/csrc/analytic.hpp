@@ -314,9 +314,245 @@ __host__ __device__ inline void tmdc(const double *p, int N, const double *k, cd
     }
 }
 
-// Dispatch: H and (Jx, Jy) of the analytic class `kind` (J_z is zero for all
-// of them). Any of H / Jx may be null; Jy is written when Jx is.
-__host__ __device__ inline void analytic_eval(int kind, int N, const double *p, const double *k, cd *H, cd *Jx, cd *Jy)
+// Materials/Haldane2L.h:52-311: two Haldane layers (A top, B top, A bottom,
+// B bottom) with real interlayer hoppings h (polytype), B2 signs s_l (+-1,
+// the complex-conjugate layer of the primed stackings) and per-layer
+// parameters. p = {t1, t2, M0, phi0 (top), t1, t2, M0, phi0 (bottom), a0,
+// same, s0, s1, h0, h1, h2, h3}; same = 1 when no bottom parameter was
+// given. With different layers the class forms the bottom B3 from the
+// top's finished B3 (Haldane2L.h:307): M0_b - 2 t2_b sin(phi0_b) B3_top,
+// restated as written; J is the intended -dH/dk of each layer.
+__host__ __device__ inline void haldane2l(const double *p, const double *k, cd *H, cd *Jx, cd *Jy)
+{
+    double va[3][2], vb[3][2];
+    honeycomb(p[8], va, vb);
+    double tha[3], thb[3];
+    for (int i = 0; i < 3; ++i) {
+        tha[i] = k[0] * va[i][0] + k[1] * va[i][1];
+        thb[i] = k[0] * vb[i][0] + k[1] * vb[i][1];
+    }
+    const bool same = p[9] != 0.0;
+    if (H) {
+        double sca = 0, ssa = 0, scb = 0, ssb = 0;
+        for (int i = 0; i < 3; ++i) {
+            sca += cos(tha[i]);
+            ssa += sin(tha[i]);
+            scb += cos(thb[i]);
+            ssb += sin(thb[i]);
+        }
+        double B[2][4];
+        for (int l = 0; l < 2; ++l) {
+            const double *q = p + 4 * l;  // t1, t2, M0, phi0 of layer l
+            B[l][0] = 2.0 * q[1] * cos(q[3]) * scb;
+            B[l][1] = q[0] * sca;
+            B[l][2] = q[0] * ssa;
+            B[l][3] = q[2] - 2.0 * q[1] * sin(q[3]) * ssb;
+        }
+        if (same) {
+            for (int c = 0; c < 4; ++c) B[1][c] = B[0][c];
+        } else {
+            B[1][3] = p[6] - 2.0 * p[5] * sin(p[7]) * B[0][3];
+        }
+        for (int l = 0; l < 2; ++l) {
+            const int o = 2 * l;
+            const double sg = p[10 + l];
+            H[o * 4 + o] = mk(B[l][0] + B[l][3], 0.0);
+            H[o * 4 + o + 1] = mk(B[l][1], -sg * B[l][2]);
+            H[(o + 1) * 4 + o] = mk(B[l][1], sg * B[l][2]);
+            H[(o + 1) * 4 + o + 1] = mk(B[l][0] - B[l][3], 0.0);
+        }
+        H[2] = mk(p[12], 0.0);
+        H[3] = mk(p[13], 0.0);
+        H[6] = mk(p[14], 0.0);
+        H[7] = mk(p[15], 0.0);
+        H[8] = mk(p[12], 0.0);
+        H[9] = mk(p[14], 0.0);
+        H[12] = mk(p[13], 0.0);
+        H[13] = mk(p[15], 0.0);
+    }
+    if (Jx) {
+        for (int ax = 0; ax < 2; ++ax) {
+            cd *J = ax == 0 ? Jx : Jy;
+            for (int l = 0; l < 2; ++l) {
+                const double *q = p + 4 * l;
+                double x0 = 0, x1 = 0, x2 = 0, x3 = 0;
+                for (int i = 0; i < 3; ++i) {
+                    x0 += vb[i][ax] * sin(thb[i]);
+                    x1 += va[i][ax] * sin(tha[i]);
+                    x2 += va[i][ax] * cos(tha[i]);
+                    x3 += vb[i][ax] * cos(thb[i]);
+                }
+                x0 *= -2.0 * q[1] * cos(q[3]);
+                x1 *= -q[0];
+                x2 *= q[0];
+                x3 *= -2.0 * q[1] * sin(q[3]);
+                const int o = 2 * l;
+                const double sg = p[10 + l];
+                J[o * 4 + o] = mk(-(x0 + x3), 0.0);
+                J[o * 4 + o + 1] = mk(-x1, sg * x2);
+                J[(o + 1) * 4 + o] = mk(-x1, -sg * x2);
+                J[(o + 1) * 4 + o + 1] = mk(-(x0 - x3), 0.0);
+            }
+        }
+    }
+}
+
+// Materials/Bi2Se3surf.h (class BieSe3surf): surface bands of Bi2Se3 on a
+// triangular lattice, parameters hard-coded (eV, a = 4.14 Angstrom). The
+// class's GenJMatrix writes its sixth derivative component past the end of
+// five-element arrays (Bi2Se3surf.h:104,159,176): undefined behaviour in the
+// reference; here that component is the derivative of h5, as the formulas
+// intend, giving J = -dH/dk.
+__host__ __device__ inline void bi2se3_consts(double c[10])
+{
+    const double ev = 27.211386245988;
+    c[0] = -0.0255 / ev; c[1] = 0.1937 / ev; c[2] = 0.2240 / ev; c[3] = 0.0551 / ev;  // A0, A11, A12, A14
+    c[4] = 0.0164 / ev;  c[5] = 0.1203 / ev; c[6] = 0.3264 / ev; c[7] = 0.0;          // B0, B11, B12, B14
+    c[8] = -1.6978 / ev;                                                                // m11
+    c[9] = 4.14 / 0.529177210903;                                                       // a (bohr)
+}
+
+__host__ __device__ inline void bi2se3_surf(const double *k, cd *H, cd *Jx, cd *Jy)
+{
+    double c[10];
+    bi2se3_consts(c);
+    const double A0 = c[0], A11 = c[1], A12 = c[2], A14 = c[3], B0 = c[4], B11 = c[5], B12 = c[6], B14 = c[7];
+    const double m11 = c[8], a = c[9], r3 = sqrt(3.0);
+    const double va[3][2] = {{a, 0.0}, {-a / 2, r3 / 2 * a}, {-a / 2, -r3 / 2 * a}};
+    const double vb[3][2] = {{0.0, r3 * a / 3}, {-a / 2, -r3 / 6 * a}, {a / 2, -r3 / 6 * a}};
+    double tha[3], thb[3];
+    for (int i = 0; i < 3; ++i) {
+        tha[i] = k[0] * va[i][0] + k[1] * va[i][1];
+        thb[i] = k[0] * vb[i][0] + k[1] * vb[i][1];
+    }
+    const double w = -2.0 * 3.14159265358979323846 / 3.0, sw = sin(w), cw = cos(w);
+    const double bc = sqrt(1.0 - B0 * B0 / (B11 * B11));
+    const double g5 = 6.0 * A11 + 6.0 * B11 + m11;  // h5 at Gamma
+    // h0..h5 (or their derivative along axis ax when ax >= 0)
+    auto comps = [&](int ax, double h[6]) {
+        double ca = 0, sa = 0, cb = 0, sb = 0;
+        double fa[3], ga[3], fb[3], gb[3];  // sin/cos of the phases or their derivatives
+        for (int i = 0; i < 3; ++i) {
+            const double da = ax < 0 ? 1.0 : va[i][ax], db = ax < 0 ? 1.0 : vb[i][ax];
+            fa[i] = ax < 0 ? sin(tha[i]) : da * cos(tha[i]);   // (sin tha)'
+            ga[i] = ax < 0 ? cos(tha[i]) : -da * sin(tha[i]);  // (cos tha)'
+            fb[i] = ax < 0 ? sin(thb[i]) : db * cos(thb[i]);
+            gb[i] = ax < 0 ? cos(thb[i]) : -db * sin(thb[i]);
+            ca += ga[i];
+            sa += fa[i];
+            cb += gb[i];
+            sb += fb[i];
+        }
+        h[0] = 2.0 * A0 * ca + 2.0 * B0 * cb;
+        h[1] = -2.0 * A14 * sw * (fa[1] - fa[2]) + 2.0 * B14 * (fb[0] + cw * (fb[1] - fb[2]));
+        h[2] = -2.0 * B14 * sw * (fb[1] - fb[2]) - 2.0 * A14 * (fa[0] + cw * (fa[1] + fa[2]));
+        h[3] = 2.0 * A12 * sa;
+        h[4] = -2.0 * B12 * sb;
+        h[5] = 2.0 * A11 * ca + 2.0 * B11 * cb + (ax < 0 ? m11 : 0.0);
+    };
+    if (H) {
+        double h[6];
+        comps(-1, h);
+        const double b0 = h[0] + B0 / B11 * (-h[5] + g5), b1 = bc * h[1], b2 = bc * h[2], b3 = bc * h[3];
+        H[0] = mk(b0 + b3, 0.0);
+        H[1] = mk(b1, -b2);
+        H[2] = mk(b1, b2);
+        H[3] = mk(b0 - b3, 0.0);
+    }
+    if (Jx) {
+        for (int ax = 0; ax < 2; ++ax) {
+            double h[6];
+            comps(ax, h);
+            const double d0 = h[0] - B0 / B11 * h[5], d1 = bc * h[1], d2 = bc * h[2], d3 = bc * h[3];
+            cd *J = ax == 0 ? Jx : Jy;
+            J[0] = mk(-(d0 + d3), 0.0);
+            J[1] = mk(-d1, d2);
+            J[2] = mk(-d1, -d2);
+            J[3] = mk(-(d0 - d3), 0.0);
+        }
+    }
+}
+
+// Materials/MultiWSM.h:47-200: multi-Weyl k.p model, p = {n, v, alphan, phi0, a0};
+// B1 + i B2 = alphan |k_perp|^n e^{i n atan2(ky, kx)}, B3 = v kz (phi0 and a0
+// enter only the Brillouin zone). J = -dH/dk in the class's closed form,
+// which divides by |k_perp|^2 (non-finite on the kz axis, as in the reference).
+__host__ __device__ inline void multi_wsm(const double *p, const double *k, cd *H, cd *Jx, cd *Jy, cd *Jz)
+{
+    const double n = p[0], v = p[1], al = p[2];
+    const double ph = atan2(k[1], k[0]), ps = k[0] * k[0] + k[1] * k[1];
+    if (H) {
+        const double r = al * pow(ps, n / 2.0);
+        const double b1 = r * cos(n * ph), b2 = r * sin(n * ph), b3 = v * k[2];
+        H[0] = mk(b3, 0.0);
+        H[1] = mk(b1, -b2);
+        H[2] = mk(b1, b2);
+        H[3] = mk(-b3, 0.0);
+    }
+    if (Jx) {
+        const double c = cos(n * ph), s = sin(n * ph), pm = pow(ps, n / 2.0 - 1.0), pn = pow(ps, n / 2.0);
+        const double x1 = n * al * k[0] * c * pm + n * al * s * pn * k[1] / ps;
+        const double y1 = n * al * k[1] * c * pm - n * al * s * pn * k[0] / ps;
+        const double x2 = n * al * k[0] * s * pm - n * al * k[1] * c * pn / ps;
+        const double y2 = n * al * k[1] * s * pm + n * al * k[0] * c * pn / ps;
+        Jx[1] = mk(-x1, x2);
+        Jx[2] = mk(-x1, -x2);
+        Jy[1] = mk(-y1, y2);
+        Jy[2] = mk(-y1, -y2);
+        if (Jz) {
+            Jz[0] = mk(-v, 0.0);
+            Jz[3] = mk(v, 0.0);
+        }
+    }
+}
+
+// Materials/WDSM_TaAs.h:41-255: four-band Weyl/Dirac semimetal model, p = {t,
+// delta, my, mz, a0}. J as the class writes it: the x component is
+// -d/dkx without the factor a0, the y slot is overwritten by the z
+// component (-d/dkz, again without a0; WDSM_TaAs.h:170-185), the z slot
+// stays zero.
+__host__ __device__ inline void wdsm_taas(const double *p, const double *k, cd *H, cd *Jx, cd *Jy)
+{
+    const double t = p[0], dl = p[1], my = p[2], mz = p[3], a0 = p[4];
+    const double sx = sin(k[0] * a0), cx = cos(k[0] * a0), sy = sin(k[1] * a0), cy = cos(k[1] * a0);
+    const double sz = sin(k[2] * a0), cz = cos(k[2] * a0);
+    if (H) {
+        const double m = t * cx + t * my * (1.0 - cy) + t * mz * (1.0 - cz);
+        H[1] = mk(t * sz, 0.0);
+        H[4] = mk(t * sz, 0.0);
+        H[11] = mk(-t * sz, 0.0);
+        H[14] = mk(-t * sz, 0.0);
+        H[2] = mk(m, -t * sy);
+        H[7] = mk(m, -t * sy);
+        H[8] = mk(m, t * sy);
+        H[13] = mk(m, t * sy);
+        H[3] = mk(0.0, -t * cy * dl);
+        H[6] = mk(0.0, -t * cy * dl);
+        H[9] = mk(0.0, t * cy * dl);
+        H[12] = mk(0.0, t * cy * dl);
+    }
+    if (Jx) {
+        const double dx = -t * sx;
+        Jx[2] = mk(-dx, 0.0);
+        Jx[7] = mk(-dx, 0.0);
+        Jx[8] = mk(-dx, 0.0);
+        Jx[13] = mk(-dx, 0.0);
+        const double z1 = t * mz * sz, z2 = t * cz;  // z derivatives in the y slot
+        Jy[2] = mk(-z1, 0.0);
+        Jy[7] = mk(-z1, 0.0);
+        Jy[8] = mk(-z1, 0.0);
+        Jy[13] = mk(-z1, 0.0);
+        Jy[1] = mk(-z2, 0.0);
+        Jy[4] = mk(-z2, 0.0);
+        Jy[11] = mk(z2, 0.0);
+        Jy[14] = mk(z2, 0.0);
+    }
+}
+
+// Dispatch: H and (Jx, Jy, Jz) of the analytic class `kind`. H may be null;
+// Jx, Jy are written together (Jz may be null: only MultiWSM has one).
+__host__ __device__ inline void analytic_eval(int kind, int N, const double *p, const double *k, cd *H, cd *Jx, cd *Jy,
+                                              cd *Jz)
 {
     switch (kind) {
     case SBE_MODEL_WILSON_MASS: wilson_mass(p, k, H, Jx, Jy); break;
@@ -324,11 +560,15 @@ __host__ __device__ inline void analytic_eval(int kind, int N, const double *p, 
     case SBE_MODEL_KANE_MELE: kane_mele(p, k, H, Jx, Jy); break;
     case SBE_MODEL_XYH_LATTICE: xyh_lattice(p, k, H, Jx, Jy); break;
     case SBE_MODEL_TMDC: tmdc(p, N, k, H, Jx, Jy); break;
+    case SBE_MODEL_HALDANE_2L: haldane2l(p, k, H, Jx, Jy); break;
+    case SBE_MODEL_BI2SE3_SURF: bi2se3_surf(k, H, Jx, Jy); break;
+    case SBE_MODEL_MULTI_WSM: multi_wsm(p, k, H, Jx, Jy, Jz); break;
+    case SBE_MODEL_WDSM_TAAS: wdsm_taas(p, k, H, Jx, Jy); break;
     default: break;
     }
 }
 
-__host__ __device__ inline bool is_analytic(int kind) { return kind >= SBE_MODEL_WILSON_MASS && kind <= SBE_MODEL_TMDC; }
+__host__ __device__ inline bool is_analytic(int kind) { return kind >= SBE_MODEL_WILSON_MASS && kind <= SBE_MODEL_WDSM_TAAS; }
 
 }  // namespace am
 }  // namespace sbe

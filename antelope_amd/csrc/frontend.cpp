@@ -281,7 +281,7 @@ const TmdcRow kTmdc[6] = {
 
 // The analytic classes' constructors (keys, Angstrom conversion, band counts):
 // WilsonMass.h:40-63, SquareSBand.h:39-59, KaneMele.h:47-75, XYHlattice.h:49-73,
-// TMDCs.h:48-149.
+// TMDCs.h:48-149 and the ones cited below.
 int analytic_target(const std::string &name, const Setting *g, sbe_model &m)
 {
     auto need = [&](const char *key, double &v) { return get_double(g, key, v); };
@@ -318,6 +318,61 @@ int analytic_target(const std::string &name, const Setting *g, sbe_model &m)
         m.kind = SBE_MODEL_XYH_LATTICE;
         m.nband = 4;
         m.nval = 2;
+    } else if (name == "Haldane2L") {  // Haldane2L.h:52-148
+        double *q = p;  // t1, t2, M0, phi0 (top), the same (bottom), a0, same, s0, s1, h0..h3
+        double t11;
+        std::string poly;
+        if (!(need("t1", q[0]) && need("t2", q[1]) && need("phi0", q[3]) && need("a0", q[8]) && need("t11", t11) &&
+              get_string(g, "polytype", poly)))
+            return fe_fail(SBE_ERR_MODEL, "Some Haldane2L paramters are missing");
+        q[8] /= kAuAngstrom;
+        if (!need("M0", q[2])) {
+            if (!need("Mt2", q[2])) return fe_fail(SBE_ERR_MODEL, "M0 param missing");
+            q[2] *= q[1];
+        }
+        q[4] = q[0]; q[5] = q[1]; q[6] = q[2]; q[7] = q[3];
+        bool same = true;
+        // the class reads the first of these keys that exists and skips the rest
+        // (a short-circuit ||, Haldane2L.h:83-88)
+        if (need("t1_bottom", q[4]) || need("t2_bottom", q[5]) || need("phi0_bottom", q[7])) same = false;
+        if (need("M0_bottom", q[6])) {
+            same = false;
+        } else if (need("Mt2_bottom", q[6])) {
+            q[6] *= q[5];
+            same = false;
+        }
+        q[9] = same ? 1.0 : 0.0;
+        q[10] = q[11] = 1.0;
+        double h[4] = {0, 0, 0, 0};
+        if (poly == "AA") { h[0] = t11; h[3] = t11; }  // stackings of Phys. Rev. B 83, 235312
+        else if (poly == "AB" || poly == "AB1") { h[1] = t11; }
+        else if (poly == "AB2" || poly == "3R") { h[2] = t11; }
+        else if (poly == "AA'" || poly == "2H") { h[1] = t11; h[2] = t11; q[11] = -1.0; }
+        else if (poly == "A'B") { h[3] = t11; q[10] = -1.0; }
+        else if (poly == "AB'") { h[0] = t11; q[11] = -1.0; }
+        else return fe_fail(SBE_ERR_MODEL, "Undefined polytype");
+        for (int i = 0; i < 4; ++i) q[12 + i] = h[i];
+        m.kind = SBE_MODEL_HALDANE_2L;
+        m.nband = 4;
+        m.nval = 2;
+    } else if (name == "Bi2Se3surf") {  // Bi2Se3surf.h:51-82: no parameters
+        m.kind = SBE_MODEL_BI2SE3_SURF;
+        m.nband = 2;
+        m.nval = 1;
+    } else if (name == "MultiWSM") {  // MultiWSM.h:47-74
+        if (!(need("n", p[0]) && need("v", p[1]) && need("alphan", p[2]) && need("phi0", p[3]) && need("a0", p[4])))
+            return fe_fail(SBE_ERR_MODEL, "Some MultiWSM paramters are missing");
+        p[4] /= kAuAngstrom;
+        m.kind = SBE_MODEL_MULTI_WSM;
+        m.nband = 2;
+        m.nval = 1;
+    } else if (name == "WDSM_TaAs") {  // WDSM_TaAs.h:41-67
+        if (!(need("my", p[2]) && need("mz", p[3]) && need("t", p[0]) && need("delta", p[1]) && need("a0", p[4])))
+            return fe_fail(SBE_ERR_MODEL, "Some WDSM paramters are missing");
+        p[4] /= kAuAngstrom;
+        m.kind = SBE_MODEL_WDSM_TAAS;
+        m.nband = 4;
+        m.nval = 1;
     } else {  // TMDCs
         bool soc = false;
         get_bool(g, "isSOCEnabled", soc);
@@ -485,7 +540,8 @@ int sbe_spec_parse_string(const char *text, sbe_spec **out)
         m.nval = 1;
         m.is_dipole_zero = 1;
     } else if (spec->target == "WilsonMass" || spec->target == "SquareSBand" || spec->target == "KaneMele" ||
-               spec->target == "XYHlattice" || spec->target == "TMDCs") {
+               spec->target == "XYHlattice" || spec->target == "TMDCs" || spec->target == "Haldane2L" ||
+               spec->target == "Bi2Se3surf" || spec->target == "MultiWSM" || spec->target == "WDSM_TaAs") {
         int rc = analytic_target(spec->target, mg, m);
         if (rc) return rc;
     } else if (mg && mg->child("dataName")) {  // Wannier90.h:107-125
@@ -506,9 +562,9 @@ int sbe_spec_parse_string(const char *text, sbe_spec **out)
         if (rc) return fe_fail(rc, sbe_wannier_last_error());
     } else {
         return fe_fail(SBE_ERR_UNSUPPORTED, "target '" + spec->target +
-                                                "' is not available as a config-file model (Haldane, WilsonMass, "
-                                                "SquareSBand, KaneMele, XYHlattice, TMDCs, or a Wannier90 group "
-                                                "with dataName)");
+                                                "' is not a model class (Haldane, Haldane2L, KaneMele, WilsonMass, "
+                                                "Bi2Se3surf, TMDCs, XYHlattice, SquareSBand, WDSM_TaAs, MultiWSM, "
+                                                "or a Wannier90 group with dataName)");
     }
     c.npulses = static_cast<int32_t>(spec->pulses.size());
     c.pulses = spec->pulses.data();

@@ -147,9 +147,9 @@ int Grid::build(const sbe_config &cfg, const sbe_model &m, int64_t kb, int64_t k
         axes[0] = 2.0 * kPi / std::sqrt(3.0) / m.a0;
         axes[4] = 2.0 * (2.0 * kPi / 3.0 / m.a0);
     } else if (m.kind != SBE_MODEL_WANNIER) {
-        // rectangles 2 kxMax x 2 kyMax centred at 0 of the analytic classes:
-        // WilsonMass.h:56-60, SquareSBand.h:54-58, KaneMele.h:68-72,
-        // XYHlattice.h:65-69, TMDCs.h:143-148
+        // boxes 2 kxMax x 2 kyMax (x 2 kzMax) centred at 0 of the analytic
+        // classes: WilsonMass.h:56-60, SquareSBand.h:54-58, KaneMele.h:68-72,
+        // XYHlattice.h:65-69, TMDCs.h:143-148 and below
         const double *p = m.params, r3 = std::sqrt(3.0);
         double kx = 0, ky = 0;
         switch (m.kind) {
@@ -158,6 +158,15 @@ int Grid::build(const sbe_config &cfg, const sbe_model &m, int64_t kb, int64_t k
         case SBE_MODEL_KANE_MELE: kx = kPi / r3 / p[5]; ky = 2.0 * kPi / 3.0 / p[5]; break;
         case SBE_MODEL_XYH_LATTICE: kx = kPi / p[3]; ky = 2.0 * kPi / r3 / p[3]; break;
         case SBE_MODEL_TMDC: kx = kPi / p[0]; ky = 2.0 * kPi / r3 / p[0]; break;
+        case SBE_MODEL_HALDANE_2L: kx = kPi / r3 / p[8]; ky = 2.0 * kPi / 3.0 / p[8]; break;  // Haldane2L.h:151-154
+        case SBE_MODEL_BI2SE3_SURF: {  // Bi2Se3surf.h:71-77, a = 4.14 Angstrom
+            const double a = 4.14 / 0.529177210903;
+            kx = kPi / a;
+            ky = 2.0 * kPi / r3 / a;
+            break;
+        }
+        case SBE_MODEL_MULTI_WSM:  // MultiWSM.h:65-71, WDSM_TaAs.h:61-65: cubes
+        case SBE_MODEL_WDSM_TAAS: kx = ky = kPi / p[4]; axes[8] = 2.0 * kPi / p[4]; break;
         default: err = "unknown model kind"; return SBE_ERR_MODEL;
         }
         axes[0] = 2.0 * kx;

@@ -95,7 +95,9 @@ int nb_validate(const sbe_model &m, std::string &err)
 {
     if (am::is_analytic(m.kind)) {
         int want = 2;
-        if (m.kind == SBE_MODEL_KANE_MELE || m.kind == SBE_MODEL_XYH_LATTICE) want = 4;
+        if (m.kind == SBE_MODEL_KANE_MELE || m.kind == SBE_MODEL_XYH_LATTICE || m.kind == SBE_MODEL_HALDANE_2L ||
+            m.kind == SBE_MODEL_WDSM_TAAS)
+            want = 4;
         if (m.kind == SBE_MODEL_TMDC) want = m.nband == 6 ? 6 : 3;
         if (m.nband != want) {
             err = "this model class has " + std::to_string(want) + " bands (TMDC: 3, or 6 with spin-orbit)";
@@ -106,9 +108,19 @@ int nb_validate(const sbe_model &m, std::string &err)
                 err = "model parameters must be finite";
                 return SBE_ERR_MODEL;
             }
-        const int ia0 = m.kind == SBE_MODEL_WILSON_MASS ? 3 : m.kind == SBE_MODEL_SQUARE_SBAND ? 2
-                      : m.kind == SBE_MODEL_KANE_MELE ? 5 : m.kind == SBE_MODEL_XYH_LATTICE ? 3 : 0;
-        if (!(m.params[ia0] > 0.0) || (m.kind == SBE_MODEL_WILSON_MASS && !(m.params[4] > 0.0))) {
+        int ia0 = 0;  // index of the lattice constant in params (-1: none)
+        switch (m.kind) {
+        case SBE_MODEL_WILSON_MASS: ia0 = 3; break;
+        case SBE_MODEL_SQUARE_SBAND: ia0 = 2; break;
+        case SBE_MODEL_KANE_MELE: ia0 = 5; break;
+        case SBE_MODEL_XYH_LATTICE: ia0 = 3; break;
+        case SBE_MODEL_HALDANE_2L: ia0 = 8; break;
+        case SBE_MODEL_BI2SE3_SURF: ia0 = -1; break;
+        case SBE_MODEL_MULTI_WSM: ia0 = 4; break;
+        case SBE_MODEL_WDSM_TAAS: ia0 = 4; break;
+        default: ia0 = 0; break;
+        }
+        if ((ia0 >= 0 && !(m.params[ia0] > 0.0)) || (m.kind == SBE_MODEL_WILSON_MASS && !(m.params[4] > 0.0))) {
             err = "lattice constants must be > 0";
             return SBE_ERR_MODEL;
         }

@@ -186,13 +186,24 @@ _ANALYTIC = {  # name -> (kind, bands, valence bands, parameter names)
     "XYHlattice": (A.SBE_MODEL_XYH_LATTICE, 4, 2, ("tl", "tr", "lambda", "a0")),
     "TMDCs": (A.SBE_MODEL_TMDC, 3, 1, ("a0", "soc", "e1", "e2", "t0", "t1", "t2", "t11", "t12", "t22", "r0", "r1",
                                        "r2", "r11", "r12", "u0", "u1", "u2", "u11", "u12", "u22")),
+    "Haldane2L": (A.SBE_MODEL_HALDANE_2L, 4, 2, ("t1", "t2", "M0", "phi0", "t1_b", "t2_b", "M0_b", "phi0_b", "a0",
+                                                 "same", "s0", "s1", "h0", "h1", "h2", "h3")),
+    "Bi2Se3surf": (A.SBE_MODEL_BI2SE3_SURF, 2, 1, ()),
+    "MultiWSM": (A.SBE_MODEL_MULTI_WSM, 2, 1, ("n", "v", "alphan", "phi0", "a0")),
+    "WDSM_TaAs": (A.SBE_MODEL_WDSM_TAAS, 4, 1, ("t", "delta", "my", "mz", "a0")),
 }
+
+# Haldane2L interlayer blocks and B2 signs per polytype (Haldane2L.h:107-147)
+_H2L_POLY = {"AA": ((1, 0, 0, 1), (1, 1)), "AB": ((0, 1, 0, 0), (1, 1)), "AB1": ((0, 1, 0, 0), (1, 1)),
+             "AB2": ((0, 0, 1, 0), (1, 1)), "3R": ((0, 0, 1, 0), (1, 1)), "AA'": ((0, 1, 1, 0), (1, -1)),
+             "2H": ((0, 1, 1, 0), (1, -1)), "A'B": ((0, 0, 0, 1), (-1, 1)), "AB'": ((1, 0, 0, 0), (1, -1))}
 
 
 @dataclass
 class AnalyticModel:
-    """One of the analytic tight-binding classes of Materials/ (WilsonMass,
-    SquareSBand, KaneMele, XYHlattice, TMDCs) with its parameters in atomic
+    """One of the analytic model classes of Materials/ (WilsonMass,
+    SquareSBand, KaneMele, XYHlattice, TMDCs, Haldane2L, Bi2Se3surf, MultiWSM,
+    WDSM_TaAs) with its parameters in atomic
     units (lengths in bohr; the `from_reference` constructors take the
     classes' config keys and convert Angstrom where the class does)."""
     name: str
@@ -225,6 +236,29 @@ class AnalyticModel:
             return cls(name, [keys["tl"], keys["tr"], keys["lambda"], keys["a0"] / AU_ANGSTROM])
         if name == "TMDCs":  # TMDCs.h:48-149: parameter table by material, 6 bands with spin-orbit
             return tmdc(keys["Material"], bool(keys.get("isSOCEnabled", False)))
+        if name == "Haldane2L":  # Haldane2L.h:52-148
+            t1, t2, phi = keys["t1"], keys["t2"], keys["phi0"]
+            M0 = keys["M0"] if "M0" in keys else keys["Mt2"] * t2
+            b = [t1, t2, M0, phi]
+            same = True
+            for i, key in ((0, "t1_bottom"), (1, "t2_bottom"), (3, "phi0_bottom")):  # first present key only
+                if key in keys:
+                    b[i], same = keys[key], False
+                    break
+            if "M0_bottom" in keys:
+                b[2], same = keys["M0_bottom"], False
+            elif "Mt2_bottom" in keys:
+                b[2], same = keys["Mt2_bottom"] * b[1], False
+            h, sg = _H2L_POLY[keys["polytype"]]
+            t11 = keys["t11"]
+            return cls(name, [t1, t2, M0, phi, *b, keys["a0"] / AU_ANGSTROM, 1.0 if same else 0.0, *sg,
+                              *[x * t11 for x in h]])
+        if name == "Bi2Se3surf":  # Bi2Se3surf.h:51-82: hard-coded
+            return cls(name, [])
+        if name == "MultiWSM":  # MultiWSM.h:47-74
+            return cls(name, [keys["n"], keys["v"], keys["alphan"], keys["phi0"], keys["a0"] / AU_ANGSTROM])
+        if name == "WDSM_TaAs":  # WDSM_TaAs.h:41-67
+            return cls(name, [keys["t"], keys["delta"], keys["my"], keys["mz"], keys["a0"] / AU_ANGSTROM])
         raise KeyError(name)
 
     def to_c(self):

@@ -130,10 +130,18 @@ enum { SBE_EFIELD_REFERENCE = 0, SBE_EFIELD_CONSISTENT = 1 };
  *   TMDC          Materials/TMDCs.h        N=3 Nval=1 (N=6 Nval=2 with spin-orbit)
  *                 {a0, soc, e1, e2, t0, t1, t2, t11, t12, t22, r0, r1, r2, r11,
  *                  r12, u0, u1, u2, u11, u12, u22}; sbe_tmdc_params
- *                 (sbe_frontend.h) fills them from the class's GGA table. */
+ *                 (sbe_frontend.h) fills them from the class's GGA table.
+ *   HALDANE_2L    Materials/Haldane2L.h    N=4 Nval=2  {t1, t2, M0, phi0 (top),
+ *                 t1, t2, M0, phi0 (bottom), a0, same, s0, s1, h0, h1, h2, h3}:
+ *                 same = 1 when no bottom key was given, s_l = +-1 the B2 sign
+ *                 of layer l, h the interlayer block of the polytype
+ *   BI2SE3_SURF   Materials/Bi2Se3surf.h   N=2 Nval=1  {} (hard-coded)
+ *   MULTI_WSM     Materials/MultiWSM.h     N=2 Nval=1  {n, v, alphan, phi0, a0}
+ *   WDSM_TAAS     Materials/WDSM_TaAs.h    N=4 Nval=1  {t, delta, my, mz, a0} */
 enum {
     SBE_MODEL_HALDANE = 1, SBE_MODEL_WANNIER = 2, SBE_MODEL_WILSON_MASS = 3, SBE_MODEL_SQUARE_SBAND = 4,
-    SBE_MODEL_KANE_MELE = 5, SBE_MODEL_XYH_LATTICE = 6, SBE_MODEL_TMDC = 7
+    SBE_MODEL_KANE_MELE = 5, SBE_MODEL_XYH_LATTICE = 6, SBE_MODEL_TMDC = 7, SBE_MODEL_HALDANE_2L = 8,
+    SBE_MODEL_BI2SE3_SURF = 9, SBE_MODEL_MULTI_WSM = 10, SBE_MODEL_WDSM_TAAS = 11
 };
 #define SBE_MODEL_NPARAMS 24
 

@@ -259,8 +259,9 @@ static int model_init(const sbe_model *m, omodel *om)
                     om->tau[3 * n + i] = m->pos[((((size_t)r0 * 3 + i) * N + n) * N + n) * 2];
         return SBE_OK;
     }
-    if (m->kind >= SBE_MODEL_WILSON_MASS && m->kind <= SBE_MODEL_TMDC) {
-        int want = (m->kind == SBE_MODEL_KANE_MELE || m->kind == SBE_MODEL_XYH_LATTICE) ? 4
+    if (m->kind >= SBE_MODEL_WILSON_MASS && m->kind <= SBE_MODEL_WDSM_TAAS) {
+        int want = (m->kind == SBE_MODEL_KANE_MELE || m->kind == SBE_MODEL_XYH_LATTICE ||
+                    m->kind == SBE_MODEL_HALDANE_2L || m->kind == SBE_MODEL_WDSM_TAAS) ? 4
                    : m->kind == SBE_MODEL_TMDC ? (m->nband == 6 ? 6 : 3) : 2;
         if (m->nband != want) return fail(SBE_ERR_MODEL, "model class has %d bands", want);
         om->N = want;
@@ -606,7 +607,164 @@ static void tmdc_HJ(const double *p, int N, const double k[3], cplx *H, cplx *J)
     }
 }
 
-static int is_analytic(int kind) { return kind >= SBE_MODEL_WILSON_MASS && kind <= SBE_MODEL_TMDC; }
+/* Haldane2L::GenHamiltonian / GenJMatrix / GenBcomp, Haldane2L.h:166-311 */
+static void haldane2l_HJ(const double *p, const double k[3], cplx *H, cplx *J)
+{
+    double va[3][2], vb[3][2];
+    honeycomb_vectors(p[8], va, vb);
+    int same = p[9] != 0.;
+    double sgn[2] = {p[10], p[11]};
+    if (H) {
+        double Bt[4] = {0, 0, 0, 0}, Bb[4];
+        for (int i = 0; i < 3; ++i) {
+            double aa = k[0] * va[i][0] + k[1] * va[i][1], ab = k[0] * vb[i][0] + k[1] * vb[i][1];
+            Bt[0] += cos(ab); Bt[1] += cos(aa); Bt[2] += sin(aa); Bt[3] += sin(ab);
+        }
+        for (int c = 0; c < 4; ++c) Bb[c] = Bt[c]; /* raw sums for the bottom layer */
+        Bt[0] *= 2. * p[1] * cos(p[3]); Bt[1] *= p[0]; Bt[2] *= p[0]; Bt[3] = p[2] - 2. * p[1] * sin(p[3]) * Bt[3];
+        if (same) {
+            for (int c = 0; c < 4; ++c) Bb[c] = Bt[c];
+        } else {
+            Bb[0] *= 2. * p[5] * cos(p[7]); Bb[1] *= p[4]; Bb[2] *= p[4];
+            Bb[3] = p[6] - 2. * p[5] * sin(p[7]) * Bt[3]; /* the top's B3, Haldane2L.h:307 */
+        }
+        H[0] = Bt[0] + Bt[3];                 H[1] = Bt[1] - sgn[0] * I * Bt[2];
+        H[4] = Bt[1] + sgn[0] * I * Bt[2];    H[5] = Bt[0] - Bt[3];
+        H[10] = Bb[0] + Bb[3];                H[11] = Bb[1] - sgn[1] * I * Bb[2];
+        H[14] = Bb[1] + sgn[1] * I * Bb[2];   H[15] = Bb[0] - Bb[3];
+        H[2] = p[12]; H[3] = p[13]; H[6] = p[14]; H[7] = p[15];
+        H[8] = p[12]; H[9] = p[14]; H[12] = p[13]; H[13] = p[15];
+    }
+    if (J) {
+        for (int l = 0; l < 2; ++l) {
+            const double *q = p + 4 * l;
+            double X[4] = {0, 0, 0, 0}, Y[4] = {0, 0, 0, 0};
+            for (int i = 0; i < 3; ++i) {
+                double aa = k[0] * va[i][0] + k[1] * va[i][1], ab = k[0] * vb[i][0] + k[1] * vb[i][1];
+                X[0] += vb[i][0] * sin(ab); Y[0] += vb[i][1] * sin(ab);
+                X[1] += va[i][0] * sin(aa); Y[1] += va[i][1] * sin(aa);
+                X[2] += va[i][0] * cos(aa); Y[2] += va[i][1] * cos(aa);
+                X[3] += vb[i][0] * cos(ab); Y[3] += vb[i][1] * cos(ab);
+            }
+            double f[4] = {-2. * q[1] * cos(q[3]), -q[0], q[0], -2. * q[1] * sin(q[3])};
+            for (int c = 0; c < 4; ++c) { X[c] *= f[c]; Y[c] *= f[c]; }
+            int o = 10 * l; /* 0 or 10: first element of the layer block */
+            for (int ax = 0; ax < 2; ++ax) {
+                double *D = ax ? Y : X;
+                cplx *Ja = J + 16 * ax;
+                Ja[o + 0] = -(D[0] + D[3]);
+                Ja[o + 1] = -(D[1] - sgn[l] * I * D[2]);
+                Ja[o + 4] = -(D[1] + sgn[l] * I * D[2]);
+                Ja[o + 5] = -(D[0] - D[3]);
+            }
+        }
+    }
+}
+
+/* BieSe3surf::GenHamiltonian / GenJMatrix / GenBcomp, Bi2Se3surf.h:51-277,
+ * with the sixth derivative component kept in bounds (the class writes it
+ * past five-element arrays) */
+static void bi2se3_HJ(const double k[3], cplx *H, cplx *J)
+{
+    const double ev = 27.211386245988;
+    double A0 = -0.0255 / ev, A11 = 0.1937 / ev, A12 = 0.2240 / ev, A14 = 0.0551 / ev;
+    double B0 = 0.0164 / ev, B11 = 0.1203 / ev, B12 = 0.3264 / ev, B14 = 0.0 / ev, m11 = -1.6978 / ev;
+    (void)B12; /* enters h4, which the class computes and never uses */
+    double a = 4.14 / 0.529177210903;
+    double va[3][2] = {{a, 0.}, {-a / 2, sqrt(3.) / 2 * a}, {-a / 2, -sqrt(3.) / 2 * a}};
+    double vb[3][2] = {{0., sqrt(3.) * a / 3}, {-a / 2, -sqrt(3.) / 6 * a}, {a / 2, -sqrt(3.) / 6 * a}};
+    double aa[3], ab[3];
+    for (int i = 0; i < 3; ++i) {
+        aa[i] = k[0] * va[i][0] + k[1] * va[i][1];
+        ab[i] = k[0] * vb[i][0] + k[1] * vb[i][1];
+    }
+    double w = -2. * PI / 3.;
+    double h5g = 6. * A11 + 6. * B11 + m11, bco = sqrt(1. - B0 * B0 / (B11 * B11));
+    if (H) {
+        double cA = 0, sA = 0, cB = 0, sB = 0;
+        for (int i = 0; i < 3; ++i) { cA += cos(aa[i]); sA += sin(aa[i]); cB += cos(ab[i]); sB += sin(ab[i]); }
+        double h0 = 2. * A0 * cA + 2. * B0 * cB;
+        double h1 = -2. * A14 * sin(w) * (sin(aa[1]) - sin(aa[2])) +
+                    2. * B14 * (sin(ab[0]) + cos(w) * (sin(ab[1]) - sin(ab[2])));
+        double h2 = -2. * B14 * sin(w) * (sin(ab[1]) - sin(ab[2])) -
+                    2. * A14 * (sin(aa[0]) + cos(w) * (sin(aa[1]) + sin(aa[2])));
+        double h3 = 2. * A12 * sA;
+        double h5 = 2. * A11 * cA + 2. * B11 * cB + m11;
+        pauli_block(H, 2, 0, h0 + B0 / B11 * (-h5 + h5g), bco * h1, bco * h2, bco * h3);
+    }
+    if (J) {
+        for (int ax = 0; ax < 2; ++ax) {
+            double dcA = 0, dsA = 0, dcB = 0, dsB = 0;
+            for (int i = 0; i < 3; ++i) {
+                dcA += -va[i][ax] * sin(aa[i]); dsA += va[i][ax] * cos(aa[i]);
+                dcB += -vb[i][ax] * sin(ab[i]); dsB += vb[i][ax] * cos(ab[i]);
+            }
+            double d0 = 2. * A0 * dcA + 2. * B0 * dcB;
+            double d1 = -2. * A14 * sin(w) * (va[1][ax] * cos(aa[1]) - va[2][ax] * cos(aa[2])) +
+                        2. * B14 * (vb[0][ax] * cos(ab[0]) + cos(w) * (vb[1][ax] * cos(ab[1]) + vb[2][ax] * cos(ab[2])));
+            double d2 = -2. * B14 * sin(w) * (vb[1][ax] * cos(ab[1]) - vb[2][ax] * cos(ab[2])) -
+                        2. * A14 * (va[0][ax] * cos(aa[0]) + cos(w) * (va[1][ax] * cos(aa[1]) + va[2][ax] * cos(aa[2])));
+            double d3 = 2. * A12 * dsA;
+            double d5 = 2. * A11 * dcA + 2. * B11 * dcB;
+            pauli_block(J + 4 * ax, 2, 0, -(d0 - B0 / B11 * d5), -bco * d1, -bco * d2, -bco * d3);
+        }
+    }
+}
+
+/* MultiWSM::GenHamiltonian / GenJMatrix / GenBcomp, MultiWSM.h:77-200 */
+static void multi_wsm_HJ(const double *p, const double k[3], cplx *H, cplx *J)
+{
+    double n = p[0], v = p[1], al = p[2];
+    double phip = atan2(k[1], k[0]), ps = k[0] * k[0] + k[1] * k[1];
+    if (H) {
+        double b1 = al * pow(ps, n / 2.) * cos(n * phip), b2 = al * pow(ps, n / 2.) * sin(n * phip);
+        pauli_block(H, 2, 0, 0., b1, b2, v * k[2]);
+    }
+    if (J) {
+        double X[4] = {0, 0, 0, 0}, Y[4] = {0, 0, 0, 0}, Z[4] = {0, 0, 0, v};
+        X[1] = n * al * k[0] * cos(n * phip) * pow(ps, n / 2. - 1) + n * al * sin(n * phip) * pow(ps, n / 2.) * k[1] / ps;
+        Y[1] = n * al * k[1] * cos(n * phip) * pow(ps, n / 2. - 1) - n * al * sin(n * phip) * pow(ps, n / 2.) * k[0] / ps;
+        X[2] = n * al * k[0] * sin(n * phip) * pow(ps, n / 2. - 1) - n * al * k[1] * cos(n * phip) * pow(ps, n / 2.) / ps;
+        Y[2] = n * al * k[1] * sin(n * phip) * pow(ps, n / 2. - 1) + n * al * k[0] * cos(n * phip) * pow(ps, n / 2.) / ps;
+        pauli_block(J, 2, 0, -X[0], -X[1], -X[2], -X[3]);
+        pauli_block(J + 4, 2, 0, -Y[0], -Y[1], -Y[2], -Y[3]);
+        pauli_block(J + 8, 2, 0, -Z[0], -Z[1], -Z[2], -Z[3]);
+    }
+}
+
+/* WDSM_TaAs::GenHamiltonian / GenJMatrix / GenBcomp, WDSM_TaAs.h:69-255:
+ * J's y slot ends up holding -d/dkz (written last), its z slot stays zero */
+static void wdsm_HJ(const double *p, const double k[3], cplx *H, cplx *J)
+{
+    double t = p[0], dl = p[1], my = p[2], mz = p[3], a0 = p[4];
+    if (H) {
+        double B[16] = {0}, Bc[16] = {0};
+        B[1] += t * sin(k[2] * a0); B[4] += t * sin(k[2] * a0);
+        B[11] += -t * sin(k[2] * a0); B[14] += -t * sin(k[2] * a0);
+        double m = t * cos(k[0] * a0) + t * my * (1 - cos(k[1] * a0)) + t * mz * (1 - cos(k[2] * a0));
+        B[2] += m; B[7] += m; B[8] += m; B[13] += m;
+        Bc[2] -= t * sin(k[1] * a0); Bc[3] -= t * cos(k[1] * a0) * dl;
+        Bc[6] -= t * cos(k[1] * a0) * dl; Bc[7] -= t * sin(k[1] * a0);
+        Bc[8] += t * sin(k[1] * a0); Bc[9] += t * cos(k[1] * a0) * dl;
+        Bc[12] += t * cos(k[1] * a0) * dl; Bc[13] += t * sin(k[1] * a0);
+        for (int i = 0; i < 16; ++i) H[i] = B[i] + I * Bc[i];
+    }
+    if (J) {
+        double X[16] = {0}, Z[16] = {0};
+        X[2] += -t * sin(k[0] * a0); X[7] += -t * sin(k[0] * a0);
+        X[8] += -t * sin(k[0] * a0); X[13] += -t * sin(k[0] * a0);
+        Z[2] += t * mz * sin(k[2] * a0); Z[7] += t * mz * sin(k[2] * a0);
+        Z[8] += t * mz * sin(k[2] * a0); Z[13] += t * mz * sin(k[2] * a0);
+        Z[1] += t * cos(k[2] * a0); Z[4] += t * cos(k[2] * a0);
+        Z[11] += -t * cos(k[2] * a0); Z[14] += -t * cos(k[2] * a0);
+        for (int i = 0; i < 16; ++i) {
+            J[i] = -X[i];
+            J[16 + i] = -Z[i];
+        }
+    }
+}
+
+static int is_analytic(int kind) { return kind >= SBE_MODEL_WILSON_MASS && kind <= SBE_MODEL_WDSM_TAAS; }
 
 /* H (if H) and J[3] (if J) of an analytic class; J_z stays zero */
 static void analytic_HJ(const omodel *om, const double k[3], cplx *H, cplx *J)
@@ -621,6 +779,10 @@ static void analytic_HJ(const omodel *om, const double k[3], cplx *H, cplx *J)
     case SBE_MODEL_KANE_MELE: kane_mele_HJ(m->params, k, H, J); break;
     case SBE_MODEL_XYH_LATTICE: xyh_lattice_HJ(m->params, k, H, J); break;
     case SBE_MODEL_TMDC: tmdc_HJ(m->params, N, k, H, J); break;
+    case SBE_MODEL_HALDANE_2L: haldane2l_HJ(m->params, k, H, J); break;
+    case SBE_MODEL_BI2SE3_SURF: bi2se3_HJ(k, H, J); break;
+    case SBE_MODEL_MULTI_WSM: multi_wsm_HJ(m->params, k, H, J); break;
+    case SBE_MODEL_WDSM_TAAS: wdsm_HJ(m->params, k, H, J); break;
     default: break;
     }
 }
@@ -784,6 +946,15 @@ static void bz_select(const sbe_config *cfg, const sbe_model *m, double ax[9], d
         if (m->kind == SBE_MODEL_KANE_MELE) { kx = PI / sqrt(3.) / p[5]; ky = 2. * PI / 3. / p[5]; } /* KaneMele.h:68 */
         if (m->kind == SBE_MODEL_XYH_LATTICE) { kx = PI / p[3]; ky = 2. * PI / sqrt(3.) / p[3]; }    /* XYHlattice.h:65 */
         if (m->kind == SBE_MODEL_TMDC) { kx = PI / p[0]; ky = 2. * PI / sqrt(3.) / p[0]; }          /* TMDCs.h:143 */
+        if (m->kind == SBE_MODEL_HALDANE_2L) { kx = PI / sqrt(3.) / p[8]; ky = 2. * PI / 3. / p[8]; } /* Haldane2L.h:151 */
+        if (m->kind == SBE_MODEL_BI2SE3_SURF) {                                                       /* Bi2Se3surf.h:71 */
+            double a = 4.14 / 0.529177210903;
+            kx = PI / a; ky = 2. * PI / sqrt(3.) / a;
+        }
+        if (m->kind == SBE_MODEL_MULTI_WSM || m->kind == SBE_MODEL_WDSM_TAAS) { /* MultiWSM.h:65, WDSM_TaAs.h:61 */
+            kx = ky = PI / p[4];
+            ax[8] = 2 * PI / p[4];
+        }
         ax[0] = 2 * kx; ax[4] = 2 * ky;
     } else {
         const double *a = m->lattice;

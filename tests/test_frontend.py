@@ -83,7 +83,7 @@ def test_libconfig_syntax_features():
     (BASE.replace('env_name = "cos4"', 'env_name = "cos2"') % "", A.SBE_ERR_CONFIG),
     (BASE.replace("ellip = 0.5;", "") % "", A.SBE_ERR_CONFIG),
     (BASE.replace("M0 = 0.01;", "") % "", A.SBE_ERR_MODEL),
-    (BASE.replace('target = "Haldane"', 'target = "Haldane2L"') % "", A.SBE_ERR_UNSUPPORTED),
+    (BASE.replace('target = "Haldane"', 'target = "Graphene"') % "", A.SBE_ERR_UNSUPPORTED),
     (BASE.replace('target = "Haldane"', 'target = "KaneMele"') % "", A.SBE_ERR_MODEL),
     ("calc = { Npoints = [4, 4] ", A.SBE_ERR_CONFIG),
     ("calc = { Npoints = [4, 4]; dt = 1.2.3; };", A.SBE_ERR_CONFIG),
@@ -102,11 +102,13 @@ def test_syntax_error_reports_line():
 
 @pytest.mark.skipif(not REF_CFG.exists(), reason="reference checkout not present")
 def test_reference_inputparam_cfg():
-    """The reference's own config parses; its Haldane2L target is not a built model."""
+    """The reference's own config parses, its Haldane2L target included: t1_bottom
+    is present, so t2_bottom and phi0_bottom are never read (Haldane2L.h:83-88)
+    and the bottom layer keeps the top's t2 and phi0; Mt2_bottom scales that t2."""
     text = REF_CFG.read_text()
-    with pytest.raises(SBEError) as e:
-        parse_config(text)
-    assert e.value.code == A.SBE_ERR_UNSUPPORTED and "Haldane2L" in str(e.value)
+    p = parse_config(text).solver_model().params
+    assert p[4:8] == [0.075, 0.025, -2.54 * 0.025, 1.16] and p[9] == 0.0
+    assert p[10:16] == [1.0, -1.0, 0.0, 0.01, 0.01, 0.0]      # polytype AA': h01 = h10 = t11, bottom conjugated
     s = parse_config(text.replace('target = "Haldane2L"', 'target = "Haldane"'))
     c, m = s.config, s.model
     assert list(c.npoints) == [201, 201, 1] and c.dt == 0.1 and c.T2 == 220.0 and c.inter_intra == 1
@@ -231,6 +233,14 @@ calc = { Npoints = [6, 6]; dt = 0.25; T2 = 150.; ksfactor = [1.0, 1.0, 1.0]; RKo
     ("XYHlattice", "XYHlattice = { tl = 0.05; tr = 0.03; lambda = 0.01; a0 = 2.46; };",
      {"tl": 0.05, "tr": 0.03, "lambda": 0.01, "a0": 2.46}),
     ("TMDCs", 'TMDCs = { Material = "WSe2"; isSOCEnabled = true; };', dict(Material="WSe2", isSOCEnabled=True)),
+    ("Haldane2L", 'Haldane2L = { t1 = 0.075; t2 = 0.025; phi0 = 0.9; a0 = 1.42; t11 = 0.01; polytype = "2H"; '
+                  'Mt2 = 1.0; t1_bottom = 0.07; t2_bottom = 0.02; Mt2_bottom = 2.0; };',
+     dict(t1=0.075, t2=0.025, phi0=0.9, a0=1.42, t11=0.01, polytype="2H", Mt2=1.0, t1_bottom=0.07, t2_bottom=0.02,
+          Mt2_bottom=2.0)),
+    ("MultiWSM", "MultiWSM = { n = 2.0; v = 0.05; alphan = 0.3; phi0 = 0.0; a0 = 3.0; };",
+     dict(n=2.0, v=0.05, alphan=0.3, phi0=0.0, a0=3.0)),
+    ("WDSM_TaAs", "WDSM_TaAs = { t = 0.02; delta = 0.5; my = 0.8; mz = 0.9; a0 = 3.4; };",
+     dict(t=0.02, delta=0.5, my=0.8, mz=0.9, a0=3.4)),
 ])
 def test_analytic_targets(target, group, keys):
     """The analytic classes' config groups (their constructors' keys and
@@ -245,3 +255,17 @@ def test_analytic_targets(target, group, keys):
     with pytest.raises(SBEError) as e:
         parse_config(ANALYTIC_CFG % (target, "Wannier", f"{target} = {{ }};"))
     assert "missing" in str(e.value) or "Unknowm" in str(e.value)
+
+
+def test_bi2se3_target_and_haldane2l_bottom_keys():
+    """Bi2Se3surf reads no group (its parameters are hard-coded); Haldane2L
+    reads only the first present of t1_bottom / t2_bottom / phi0_bottom, as
+    the short-circuit test of Haldane2L.h:83-88 does."""
+    from antelope_amd import AnalyticModel
+    s = parse_config(ANALYTIC_CFG % ("Bi2Se3surf", "Wannier", ""))
+    assert s.solver_model().name == "Bi2Se3surf" and s.model.nband == 2
+    g = ('Haldane2L = { t1 = 0.075; t2 = 0.025; phi0 = 0.9; a0 = 1.42; t11 = 0.01; polytype = "AB"; M0 = 0.02; '
+         't1_bottom = 0.07; t2_bottom = 0.02; };')
+    p = parse_config(ANALYTIC_CFG % ("Haldane2L", "Wannier", g)).solver_model().params
+    assert p[4] == 0.07 and p[5] == 0.025 and p[9] == 0.0   # t2_bottom skipped
+

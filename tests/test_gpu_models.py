@@ -24,6 +24,13 @@ def _models():
         "xyh": AnalyticModel.from_reference("XYHlattice", tl=0.05, tr=0.03, **{"lambda": 0.01}, a0=2.46),
         "mos2": tmdc("MoS2"),
         "wse2_soc": tmdc("WSe2", soc=True),
+        "h2l_2h": AnalyticModel.from_reference("Haldane2L", t1=0.075, t2=0.025, phi0=0.9, a0=1.42, t11=0.01,
+                                               polytype="2H", Mt2=1.0),
+        "h2l_ab_bottom": AnalyticModel.from_reference("Haldane2L", t1=0.075, t2=0.025, phi0=0.9, a0=1.42, t11=0.012,
+                                                      polytype="AB", M0=0.02, t2_bottom=0.02),
+        "bi2se3": AnalyticModel.from_reference("Bi2Se3surf"),
+        "multiwsm": AnalyticModel.from_reference("MultiWSM", n=2.0, v=0.05, alphan=0.3, phi0=0.0, a0=3.0),
+        "wdsm": AnalyticModel.from_reference("WDSM_TaAs", t=0.02, delta=0.5, my=0.8, mz=0.9, a0=3.4),
     }
 
 
@@ -44,7 +51,10 @@ def test_model_parity(oracle, name, gauge):
     # Monkhorst-Pack points: the endpoint-inclusive grid puts k on the zone edge, where the
     # spin-orbit TMDC's spin bands are exactly degenerate; there the band basis (and so the
     # velocity gauge's band-resolved outputs, not its currents) is the eigensolver's choice
-    cfg = small_config(n=(6, 5), ncycles=1.0, ellip=0.3, T2=150.0, gauge=gauge, grid="mp")
+    # WDSM_TaAs's bands are pairwise degenerate along lines of the kz = 0 plane, where the
+    # eigensolver's basis choice (LAPACK's in the reference) decides the state: use kz != 0
+    n = (4, 4, 2) if name == "wdsm" else (6, 5)
+    cfg = small_config(n=n, ncycles=1.0, ellip=0.3, T2=150.0, gauge=gauge, grid="mp")
     print(name, gauge, flush=True)
     ref = oracle.run(cfg, m, nthreads=8, want_state=True)
     with SBESolver(cfg, m) as s:

@@ -31,6 +31,13 @@ def models():
         "xyh": AnalyticModel.from_reference("XYHlattice", tl=0.05, tr=0.03, **{"lambda": 0.01}, a0=2.46),
         "mos2": tmdc("MoS2"),
         "wse2_soc": tmdc("WSe2", soc=True),
+        "h2l_2h": AnalyticModel.from_reference("Haldane2L", t1=0.075, t2=0.025, phi0=0.9, a0=1.42, t11=0.01,
+                                               polytype="2H", Mt2=1.0),
+        "h2l_ab_bottom": AnalyticModel.from_reference("Haldane2L", t1=0.075, t2=0.025, phi0=0.9, a0=1.42, t11=0.012,
+                                                      polytype="AB", M0=0.02, t2_bottom=0.02, phi0_bottom=0.3),
+        "bi2se3": AnalyticModel.from_reference("Bi2Se3surf"),
+        "multiwsm": AnalyticModel.from_reference("MultiWSM", n=2.0, v=0.05, alphan=0.3, phi0=0.0, a0=3.0),
+        "wdsm": AnalyticModel.from_reference("WDSM_TaAs", t=0.02, delta=0.5, my=0.8, mz=0.9, a0=3.4),
     }
 
 
@@ -45,7 +52,8 @@ def test_hamiltonian_hermitian_and_symmetric(oracle, name):
     rng = np.random.default_rng(7)
     rot = {"wilson": math.pi / 2, "square": math.pi / 2, "kanemele": 2 * math.pi / 3,
            "kanemele_rashba": 2 * math.pi / 3, "xyh": 2 * math.pi / 3, "mos2": 2 * math.pi / 3,
-           "wse2_soc": 2 * math.pi / 3}.get(name)
+           "wse2_soc": 2 * math.pi / 3, "h2l_2h": 2 * math.pi / 3, "h2l_ab_bottom": 2 * math.pi / 3,
+           "bi2se3": 2 * math.pi / 3, "multiwsm": 0.37}.get(name)
     for _ in range(5):
         k = np.array([*rng.uniform(-0.6, 0.6, 2), 0.0])
         H = oracle.hamiltonian(m, k)
@@ -53,7 +61,7 @@ def test_hamiltonian_hermitian_and_symmetric(oracle, name):
         e = np.linalg.eigvalsh(H)
         if rot is not None:  # point-group rotation of the lattice leaves the spectrum unchanged
             np.testing.assert_allclose(np.linalg.eigvalsh(oracle.hamiltonian(m, _rot(k, rot))), e, atol=1e-13)
-        if name in ("wilson", "wilson_rect", "square", "mos2", "wse2_soc", "kanemele_rashba", "xyh"):
+        if name in ("wilson", "wilson_rect", "square", "mos2", "wse2_soc", "kanemele_rashba", "xyh", "bi2se3"):
             # time reversal (spinless: H(-k) = H(k)*; with spin: Kramers pairs) -> E(-k) = E(k)
             np.testing.assert_allclose(np.linalg.eigvalsh(oracle.hamiltonian(m, -k)), e, atol=1e-13)
 
@@ -67,13 +75,20 @@ def _fd(oracle, m, k, h=1e-5):
     return np.array(out)
 
 
+def _fd3(oracle, m, k, h=1e-5):
+    return _fd(oracle, m, k, h)
+
+
 @pytest.mark.parametrize("name", list(models()))
 def test_current_matrix_against_finite_differences(oracle, name):
     m = models()[name]
     k = np.array([0.137, -0.211, 0.0])
     J = oracle.jmatrix(m, k)
     dH = _fd(oracle, m, k)
-    np.testing.assert_allclose(J[2], 0.0, atol=0)
+    if name == "multiwsm":
+        np.testing.assert_allclose(J[2], np.diag([-m.params[1], m.params[1]]), atol=0)
+    else:
+        np.testing.assert_allclose(J[2], 0.0, atol=0)
     if name.startswith("wilson"):
         np.testing.assert_allclose(J, -dH, atol=1e-9)            # J = -dH/dk
     elif name in ("xyh", "mos2", "wse2_soc"):
@@ -81,6 +96,23 @@ def test_current_matrix_against_finite_differences(oracle, name):
     elif name == "square":                                       # SquareSBand.h:69-78: only (0,1)
         np.testing.assert_allclose(J[:2, 0, 1], -dH[:2, 0, 1], atol=1e-9)
         assert np.all(J[:2, 1, 0] == 0) and np.all(J[:2, 0, 0] == 0) and np.all(J[:2, 1, 1] == 0)
+    elif name in ("h2l_2h", "bi2se3"):
+        np.testing.assert_allclose(J, -dH, atol=1e-9)            # Haldane2L.h, Bi2Se3surf.h: -dH/dk
+    elif name == "h2l_ab_bottom":                                # bottom H uses the top's B3 (Haldane2L.h:307)
+        np.testing.assert_allclose(J[:, :2, :2], -dH[:, :2, :2], atol=1e-9)
+        assert np.all(J[:, :2, 2:] == 0) and np.all(J[:, 2:, :2] == 0)
+        m2 = AnalyticModel("Haldane2L", [*m.params[4:8], *m.params[4:8], *m.params[8:]])  # bottom as a top layer
+        np.testing.assert_allclose(J[:, 2:, 2:], oracle.jmatrix(m2, k)[:, :2, :2], atol=1e-15)
+        assert np.abs(J[:, 2:, 2:] + dH[:, 2:, 2:]).max() > 1e-3   # not -dH/dk of the written H
+    elif name == "multiwsm":                                     # MultiWSM.h: J_z = -v sz as well
+        dHz = _fd(oracle, m, k)
+        np.testing.assert_allclose(J[:2], -dHz[:2], atol=1e-9)
+        return
+    elif name == "wdsm":                                         # WDSM_TaAs.h:126-185
+        a0 = m.params[4]
+        np.testing.assert_allclose(J[0], -dH[0] / a0, atol=1e-9)
+        np.testing.assert_allclose(J[1], -_fd3(oracle, m, k)[2].real / a0, atol=1e-9)
+        return
     elif name.startswith("kanemele"):                            # KaneMele.h:98-176
         blk = np.zeros((4, 4), bool)
         blk[:2, :2] = blk[2:, 2:] = True
@@ -126,14 +158,16 @@ def test_tmdc_table_matches_reference_header():
         assert f'a0 = {a:.3f}/au_angstrom' in text and f'socparam = {soc:.3f}/au_eV' in text
 
 
-@pytest.mark.parametrize("name", ["wilson_rect", "square", "kanemele_rashba", "xyh", "mos2", "wse2_soc"])
+@pytest.mark.parametrize("name", ["wilson_rect", "square", "kanemele_rashba", "xyh", "mos2", "wse2_soc", "h2l_2h",
+                                  "h2l_ab_bottom", "bi2se3", "multiwsm", "wdsm"])
 def test_model_runs_keep_invariants(oracle, name):
     """Short oracle runs of each class: trace of the k-summed rho conserved,
     inter + intra = the InterIntra = false total, lower bands full before the pulse."""
     m = models()[name]
-    cfg = small_config(n=5, ncycles=1.0, ellip=0.3, T2=150.0)
+    n = 6 if name == "multiwsm" else 5  # MultiWSM's J divides by |k_perp|^2: keep k = 0 off the grid
+    cfg = small_config(n=n, ncycles=1.0, ellip=0.3, T2=150.0)
     r1 = oracle.run(cfg, m, nthreads=4)
-    r0 = oracle.run(small_config(n=5, ncycles=1.0, ellip=0.3, T2=150.0, inter_intra=False), m, nthreads=4)
+    r0 = oracle.run(small_config(n=n, ncycles=1.0, ellip=0.3, T2=150.0, inter_intra=False), m, nthreads=4)
     tw = r1["info"].total_weight
     np.testing.assert_allclose(np.trace(r1["rho_bar"], axis1=1, axis2=2).real, m.nval * tw, rtol=1e-12)
     np.testing.assert_allclose(r1["j_inter"] + r1["j_intra"], r0["j_inter"], atol=1e-13)
