@@ -1,8 +1,9 @@
 // antelope_mi355x - the reference's executable (antelope.cpp:50-593) on one
 // MI355X: reads inputParam.cfg from the working directory (or the path
 // given), propagates the whole k grid on the GPU and writes the reference's
-// output files into the working directory (or --outdir). A config whose name
-// ends in .atl is read in the .atl format of src/inputFile.atl.
+// output files into the working directory (or --outdir), snapshots included
+// when calc.shotNumber > 0. A config whose name ends in .atl is read in the
+// .atl format of src/inputFile.atl.
 //
 //   antelope_mi355x [config] [--outdir DIR] [--quiet]
 #include <cstdio>
@@ -44,15 +45,30 @@ int main(int argc, char **argv)
         std::printf("Nk = %lld, Nt = %lld, dt = %g, atmin = %g, dV = %.6e\n", (long long)info.nk_total,
                     (long long)info.nt, info.dt, info.atmin, info.dV);
     const int64_t chunk = info.nt / 10 > 0 ? info.nt / 10 : 1;
-    for (int64_t s = 0; s < info.nt; s += chunk) {
-        const int64_t e = s + chunk < info.nt ? s + chunk : info.nt;
+    // snapshots every `shot` steps, taken before the step (antelope.cpp:340-421)
+    const int64_t shot = sbe_snapshot_interval(cfg, sbe_spec_shot_number(spec));
+    std::vector<double> band;
+    if (shot > 0) band.resize(static_cast<size_t>(info.nk_total) * info.nband * info.nband * 2);
+    for (int64_t s = 0; s < info.nt;) {
+        if (shot > 0 && s % shot == 0) {
+            if (sbe_get_state(ctx, SBE_BASIS_BAND, band.data()) != SBE_OK ||
+                sbe_write_snapshot(outdir.c_str(), cfg, model, band.data(), s > 0) != SBE_OK) {
+                std::fprintf(stderr, "snapshot: %s%s\n", sbe_last_error(ctx), sbe_frontend_last_error());
+                sbe_destroy(ctx);
+                sbe_spec_free(spec);
+                return 1;
+            }
+        }
+        int64_t e = s + chunk < info.nt ? s + chunk : info.nt;
+        if (shot > 0 && (s / shot + 1) * shot < e) e = (s / shot + 1) * shot;
         if (sbe_run(ctx, s, e) != SBE_OK || sbe_synchronize(ctx) != SBE_OK) {
             std::fprintf(stderr, "%s\n", sbe_last_error(ctx));
             sbe_destroy(ctx);
             sbe_spec_free(spec);
             return 1;
         }
-        if (!quiet) std::printf("%lld / %lld\n", (long long)e, (long long)info.nt);
+        if (!quiet && (e / chunk != s / chunk || e == info.nt)) std::printf("%lld / %lld\n", (long long)e, (long long)info.nt);
+        s = e;
     }
     const int N = info.nband;
     std::vector<double> ji(3 * info.nt), ja(3 * info.nt), rb(2 * N * N * info.nt), bp(N * info.nt);

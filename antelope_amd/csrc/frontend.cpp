@@ -389,6 +389,61 @@ extern "C" {
 
 const char *sbe_frontend_last_error(void) { return g_fe_error.c_str(); }
 
+int32_t sbe_spec_shot_number(const sbe_spec *spec) { return spec ? spec->shot_number : 0; }
+
+int64_t sbe_snapshot_interval(const sbe_config *cfg, int32_t shot_number)
+{
+    // antelope.cpp:273-278: int(period0 / shotNumber / dt) with period0 = 2 pi / w0 of
+    // pulse 0; no snapshots (Nt + 100) when shotNumber <= 0
+    if (!cfg || shot_number <= 0 || cfg->npulses < 1 || !(cfg->pulses[0].w0 > 0) || !(cfg->dt > 0)) return 0;
+    const double period0 = 2.0 * kPi / cfg->pulses[0].w0;
+    return static_cast<int64_t>(period0 / shot_number / cfg->dt);
+}
+
+int sbe_write_snapshot(const char *dir, const sbe_config *cfg, const sbe_model *model, const double *rho_band,
+                       int32_t append)
+{
+    if (!dir || !cfg || !model || !rho_band) return fe_fail(SBE_ERR_CONFIG, "null argument");
+    sbe::Grid g;
+    std::string err;
+    int rc = g.build(*cfg, *model, -1, -1, err);
+    if (rc) return fe_fail(rc, err);
+    const int N = model->nband, nv = model->nval;
+    if (nv < 1 || nv >= N) return fe_fail(SBE_ERR_MODEL, "snapshots need 1 <= Nval < N (conduction and coherence elements)");
+    const std::string d(dir);
+    const char *mode = append ? "a" : "w";
+    const char *bmode = append ? "ab" : "wb";
+    FILE *f1 = std::fopen((d + "/occupation_1d_integration.dat").c_str(), mode);
+    FILE *fn = std::fopen((d + "/fisrt_conduction.dat").c_str(), bmode);
+    FILE *fp = std::fopen((d + "/fisrt_coherence.dat").c_str(), bmode);
+    FILE *fl = std::fopen((d + "/snapshot_log.dat").c_str(), mode);  // opened, never written (antelope.cpp:291)
+    const bool ok = f1 && fn && fp && fl;
+    if (ok) {
+        const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
+        const size_t nn = static_cast<size_t>(N) * N;
+        std::vector<double> occ1d(2 * static_cast<size_t>(n0), 0.0), nc(g.ntotal);
+        std::vector<double> pi(2 * static_cast<size_t>(g.ntotal));
+        for (int64_t k = 0; k < g.ntotal; ++k) {
+            const double *r = rho_band + 2 * nn * k;
+            const double *cc = r + 2 * (static_cast<size_t>(nv) * N + nv);      // rho_H[Nval][Nval]
+            const double *co = r + 2 * (static_cast<size_t>(nv) * N + nv - 1);  // rho_H[Nval][Nval-1]
+            const int64_t i0 = k / (static_cast<int64_t>(n1) * n2);             // momaxis.h:87-91
+            occ1d[2 * i0] += cc[0] * g.w[k];
+            occ1d[2 * i0 + 1] += cc[1] * g.w[k];
+            nc[k] = cc[0];
+            pi[2 * k] = co[0];
+            pi[2 * k + 1] = co[1];
+        }
+        for (int m = 0; m < n0; ++m) std::fprintf(f1, "%16e    ", occ1d[2 * m] / static_cast<double>(n1));
+        std::fprintf(f1, "\n");
+        std::fwrite(nc.data(), sizeof(double), nc.size(), fn);
+        std::fwrite(pi.data(), sizeof(double), pi.size(), fp);
+    }
+    for (FILE *f : {f1, fn, fp, fl})
+        if (f) std::fclose(f);
+    return ok ? SBE_OK : fe_fail(SBE_ERR_CONFIG, "cannot write snapshot files in " + d);
+}
+
 int sbe_tmdc_params(const char *material, int32_t soc, sbe_model *m)
 {
     if (!material || !m) return SBE_ERR_CONFIG;
@@ -475,6 +530,10 @@ int sbe_spec_parse_string(const char *text, sbe_spec **out)
         c.precision = s == "fp32" ? SBE_PREC_FP32 : s == "split16" ? SBE_PREC_SPLIT16 : SBE_PREC_FP64;
     c.rtol = 1e-8;
     c.atol = 1e-12;
+    {
+        long long shots = 0;  // SBEs.h:441,461
+        if (get_int(calc, "shotNumber", shots)) spec->shot_number = static_cast<int32_t>(shots);
+    }
     get_double(calc, "rtol", c.rtol);
     get_double(calc, "atol", c.atol);
 

@@ -14,5 +14,6 @@ struct sbe_spec {
     std::string target;
     std::string warnings;
     sbe_wannier_tb *tb = nullptr;  // owns the Wannier arrays the model points into
+    int32_t shot_number = 0;       // calc.shotNumber: snapshots per period of pulse 0
     ~sbe_spec() { sbe_wannier_tb_free(tb); }
 };

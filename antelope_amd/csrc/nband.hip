@@ -768,6 +768,43 @@ __global__ __launch_bounds__(64) void nb_vel_init_kernel(const NbArgs a, const c
     }
 }
 
+// Band-basis export of the LengthWannier state (SBEs.h:927-946,
+// WannierToHamiltonian): U^dag rho U with U = eig(H(k + A)) from mats slot 0
+// and the column phase rule of nb_vel_init_kernel.
+__global__ __launch_bounds__(64) void nb_band_export_kernel(const NbArgs a, const cd *__restrict__ H,
+                                                            const cd *__restrict__ rho, cd *__restrict__ out)
+{
+    extern __shared__ double4 smem4[];
+    const int N = a.N, nn = N * N;
+    cd *base = reinterpret_cast<cd *>(smem4);
+    cd *A = mat_slot(base, 0, nn), *U = mat_slot(base, 1, nn), *W = mat_slot(base, 2, nn);
+    cd *T = mat_slot(base, 3, nn), *R = mat_slot(base, 4, nn);
+    double *ev = reinterpret_cast<double *>(mat_slot(base, 6, nn));
+    double4 *rot = reinterpret_cast<double4 *>(ev + 64);
+    const int64_t k = blockIdx.x;
+    const int lane = threadIdx.x;
+    if (k >= a.nk) return;
+    wcopy(A, H + k * nn, nn);
+    weigh(A, U, ev, W, N, rot, false);
+    if (lane < N) {
+        const int j = lane;
+        int p = 0;
+        while (p < N - 1 && (U[p * N + j].x * U[p * N + j].x + U[p * N + j].y * U[p * N + j].y) * (2.0 * N) < 1.0) ++p;
+        const cd u = U[p * N + j];
+        const double au = sqrt(u.x * u.x + u.y * u.y);
+        if (au > 0.0) {
+            const cd ph = cmk(u.x / au, -u.y / au);
+            for (int r = 0; r < N; ++r) U[r * N + j] = cmul(U[r * N + j], ph);
+            U[p * N + j] = cmk(au, 0.0);
+        }
+    }
+    wave_sync();
+    wcopy(R, rho + k * nn, nn);
+    wmm(T, R, U, N, false, false);
+    wmm(W, U, T, N, true, false);
+    for (int e = lane; e < nn; e += 64) out[k * nn + e] = W[e];
+}
+
 // One time step of the velocity gauge for every k: the sample at t_n
 // (SBEs.h:694-760 with _tkp = k: rho is band-basis and U^dag J U is pMatrix;
 // the Nval A(t) term is added by the host), then the RK step with
@@ -987,6 +1024,12 @@ hipError_t nb_launch_vel_step(const NbArgs &a, const NbVelStep &v, const double2
     else if (nn <= 256) nb_vel_step_kernel<4><<<grid, 64, lds, s>>>(a, v, P, Eocc, rho, partials, step_local, pl);
     else if (nn <= 1024) nb_vel_step_kernel<16><<<grid, 64, lds, s>>>(a, v, P, Eocc, rho, partials, step_local, pl);
     else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t nb_launch_band_export(const NbArgs &a, const double2 *H, const double2 *rho, double2 *out, hipStream_t s)
+{
+    nb_band_export_kernel<<<dim3(static_cast<unsigned>(a.nk_stride)), 64, nb_lds_bytes(a.N), s>>>(a, H, rho, out);
     return hipGetLastError();
 }
 

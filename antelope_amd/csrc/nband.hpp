@@ -75,6 +75,8 @@ size_t nb_vel_lds_bytes(int N, int nstage, bool p_lds);
 // rho = diag(occupations). P [3][nk_stride][N][N], Eocc [nk_stride][2N] (E, occ)
 hipError_t nb_launch_vel_init(const NbArgs &a, const double2 *mats, double2 *P, double *Eocc, double2 *U,
                               double2 *rho, hipStream_t s);
+// U^dag rho U per k with U = eig(H) (H: [nk_stride][N][N]), the phase rule of the velocity set-up
+hipError_t nb_launch_band_export(const NbArgs &a, const double2 *H, const double2 *rho, double2 *out, hipStream_t s);
 // sample at t_n, then one RK step, for every k (partials row step_local)
 hipError_t nb_launch_vel_step(const NbArgs &a, const NbVelStep &v, const double2 *P, const double *Eocc, double2 *rho,
                               double *partials, int64_t step_local, bool p_lds, hipStream_t s);

@@ -416,13 +416,26 @@ int nb_state(sbe_ctx *ctx, int basis, double *rho)
     }
     // LengthWannier propagates the orbital basis, VelocityHamiltonian the band basis at k
     const int native = ctx->velocity ? SBE_BASIS_BAND : SBE_BASIS_ORBITAL;
-    if (basis != native && !ctx->velocity) {
-        ctx->err = "band-basis export of the LengthWannier N-band state is available for the 2-band path only";
-        return SBE_ERR_UNSUPPORTED;
-    }
     const int N = ctx->N;
     const size_t nn = static_cast<size_t>(N) * N;
     const size_t bytes = static_cast<size_t>(ctx->nk) * nn * sizeof(double2);
+    if (basis != native && !ctx->velocity) {
+        // U^dag rho U at k + A(t) for the next sample time t (WannierToHamiltonian, SBEs.h:927-946):
+        // H into mats slot 0, the rotated state into slot 1 (slot 0 then no longer holds H~)
+        const NbArgs &a = ctx->nargs;
+        NbEval ev{};
+        double E[3];
+        ctx->laser.eval(ctx->laser.atmin + ctx->laser.dt * static_cast<double>(ctx->steps_done), ev.A, E);
+        ev.nmat = 1;
+        ev.mats[0] = kMatH0;
+        double2 *out = ctx->nb_mats + static_cast<size_t>(a.nk_stride) * nn;
+        ctx->nb_ht_valid = false;
+        int rc = hipc(ctx, eval_mats(a, ev, ctx->nb_P, ctx->nb_mats, ctx->stream), "fourier kernel");
+        if (!rc) rc = hipc(ctx, nb_launch_band_export(a, ctx->nb_mats, ctx->nb_rho, out, ctx->stream), "band export kernel");
+        if (!rc) rc = hipc(ctx, hipMemcpyAsync(rho, out, bytes, hipMemcpyDeviceToHost, ctx->stream), "copy state");
+        if (!rc) rc = hipc(ctx, hipStreamSynchronize(ctx->stream), "sync state");
+        return rc;
+    }
     int rc = hipc(ctx, hipMemcpyAsync(rho, ctx->nb_rho, bytes, hipMemcpyDeviceToHost, ctx->stream), "copy state");
     std::vector<double2> U;
     if (!rc && basis != native) {  // velocity gauge, orbital basis: U rho U^dag with the stored U(k)

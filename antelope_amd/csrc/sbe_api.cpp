@@ -708,7 +708,7 @@ int sbe_get_state(sbe_ctx *ctx, int32_t basis, double *rho)
         return SBE_OK;
     }
     if (basis == SBE_BASIS_BAND) {
-        // U^dag rho U at k + A(t_now) with the eigenvectors of d.sigma
+        // U^dag rho U at k + A(t_now) with the eigenvectors of d.sigma (t_now: the next sample time)
         const double t = ctx->laser.atmin + ctx->laser.dt * static_cast<double>(ctx->steps_done);
         double A[3], E[3];
         ctx->laser.eval(t, A, E);
@@ -721,18 +721,20 @@ int sbe_get_state(sbe_ctx *ctx, int32_t basis, double *rho)
                 sa += std::sin(kx * h.va[j][0] + ky * h.va[j][1]);
                 sb += std::sin(kx * h.vb[j][0] + ky * h.vb[j][1]);
             }
-            const double d[3] = {h.t1 * ca, h.t1 * sa, h.M0 - h.c3 * sb};
-            const double dn = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+            // eigenvectors with the phase rule of band_frame.hpp: s_band,i = s . f_i
+            const sbe::BandFrame F = sbe::band_frame(h.t1 * ca, h.t1 * sa, h.M0 - h.c3 * sb);
             double *o = rho + 8 * k;
-            const double n = o[0] + o[6], sx = 2 * o[2], sy = 2 * o[5], sz = o[0] - o[6];
-            double sd = dn > 0 ? (sx * d[0] + sy * d[1] + sz * d[2]) / dn : -sz;
-            // perpendicular part magnitude goes to the coherence, real by phase choice
-            const double s2 = sx * sx + sy * sy + sz * sz;
-            const double perp = std::sqrt(std::max(0.0, s2 - sd * sd));
-            o[0] = 0.5 * (n - sd); o[1] = 0.0;
-            o[2] = 0.5 * perp;     o[3] = 0.0;
-            o[4] = 0.5 * perp;     o[5] = 0.0;
-            o[6] = 0.5 * (n + sd); o[7] = 0.0;
+            const double n = o[0] + o[6], sv[3] = {2 * o[2], 2 * o[5], o[0] - o[6]};
+            double b[3] = {0, 0, 0};
+            for (int c = 0; c < 3; ++c) {
+                b[0] += sv[c] * F.fx[c];
+                b[1] += sv[c] * F.fy[c];
+                b[2] += sv[c] * F.fz[c];
+            }
+            o[0] = 0.5 * (n + b[2]); o[1] = 0.0;
+            o[2] = 0.5 * b[0];       o[3] = -0.5 * b[1];
+            o[4] = 0.5 * b[0];       o[5] = 0.5 * b[1];
+            o[6] = 0.5 * (n - b[2]); o[7] = 0.0;
         }
     } else if (basis != SBE_BASIS_ORBITAL) {
         return set_error(ctx, SBE_ERR_CONFIG, "unknown basis");

@@ -31,6 +31,10 @@ _SIGS = {
     "sbe_spec_parse_atl_string": ([C.c_char_p, C.POINTER(C.c_void_p)], C.c_int),
     "sbe_spec_parse_atl_file": ([C.c_char_p, C.POINTER(C.c_void_p)], C.c_int),
     "sbe_atl_last_error": ([], C.c_char_p),
+    "sbe_spec_shot_number": ([C.c_void_p], C.c_int32),
+    "sbe_snapshot_interval": ([C.POINTER(A.sbe_config), C.c_int32], C.c_int64),
+    "sbe_write_snapshot": ([C.c_char_p, C.POINTER(A.sbe_config), C.POINTER(A.sbe_model), A.c_double_p, C.c_int32],
+                           C.c_int),
 }
 _bound = False
 
@@ -61,6 +65,7 @@ class Spec:
         self.config._keep = self._pulses
         self.model = A.sbe_model.from_buffer_copy(pm.contents)
         self.warnings = lib.sbe_spec_warnings(self._h).decode()
+        self.shot_number = lib.sbe_spec_shot_number(self._h)
 
     def solver_model(self):
         """The model as a params object (HaldaneModel / WannierModel, arrays copied)."""
@@ -170,3 +175,19 @@ def write_outputs(outdir, config_c, model_c, info, series: dict) -> None:
     p = lambda a: a.ctypes.data_as(A.c_double_p)  # noqa: E731
     _check(lib.sbe_write_outputs(str(outdir).encode(), C.byref(config_c), C.byref(model_c), C.byref(info),
                                  p(ji), p(ja), p(rb), p(bp)))
+
+
+def snapshot_interval(config, shot_number: int) -> int:
+    """Steps between snapshots, int(2 pi / w0 / shotNumber / dt) (antelope.cpp:273-278); 0 = none."""
+    cfg, keep = config.to_c()
+    return int(_lib().sbe_snapshot_interval(C.byref(cfg), int(shot_number)))
+
+
+def write_snapshot(outdir, config, model, rho_band: np.ndarray, append: bool) -> None:
+    """Append one snapshot (sbe_write_snapshot) from the band-basis state [Ntotal][N][N]."""
+    cfg, k1 = config.to_c()
+    mdl, k2 = model.to_c()
+    buf = np.ascontiguousarray(np.stack([rho_band.real, rho_band.imag], -1), dtype=np.float64)
+    _check(_lib().sbe_write_snapshot(str(outdir).encode(), C.byref(cfg), C.byref(mdl),
+                                     buf.ctypes.data_as(A.c_double_p), int(append)))
+

@@ -76,6 +76,23 @@ const char *sbe_wannier_last_error(void);
  * soc != 0 gives the 6-band model. SBE_ERR_MODEL for an unknown material. */
 int sbe_tmdc_params(const char *material, int32_t soc, sbe_model *m);
 
+/* Snapshots (antelope.cpp:267-292,340-421). calc.shotNumber of the config;
+ * the step interval int(2 pi / w0 / shotNumber / dt) of pulse 0 (0: none).
+ * sbe_write_snapshot appends one snapshot of the whole grid from the
+ * band-basis state (sbe_get_state(ctx, SBE_BASIS_BAND, ...) of a context
+ * holding all k points): a row of occupation_1d_integration.dat (per first k
+ * index, sum w Re rho_H[Nval][Nval] / N2, "%16e    "), Ntotal doubles
+ * Re rho_H[Nval][Nval] to fisrt_conduction.dat and Ntotal complex
+ * rho_H[Nval][Nval-1] to fisrt_coherence.dat (the reference's file names);
+ * snapshot_log.dat is created empty as the reference leaves it. append = 0
+ * truncates the files. The reference fills the per-k arrays with a stale
+ * index (one element, or the last k point's values everywhere); these are
+ * the values it intends. */
+int32_t sbe_spec_shot_number(const sbe_spec *spec);
+int64_t sbe_snapshot_interval(const sbe_config *cfg, int32_t shot_number);
+int sbe_write_snapshot(const char *dir, const sbe_config *cfg, const sbe_model *model, const double *rho_band,
+                       int32_t append);
+
 /* Write the reference's output files into `dir` from summed series (the
  * layout of sbe_get_series; several ranks' series added together). */
 int sbe_write_outputs(const char *dir, const sbe_config *cfg, const sbe_model *model, const sbe_info *info,

@@ -269,3 +269,32 @@ def test_bi2se3_target_and_haldane2l_bottom_keys():
     p = parse_config(ANALYTIC_CFG % ("Haldane2L", "Wannier", g)).solver_model().params
     assert p[4] == 0.07 and p[5] == 0.025 and p[9] == 0.0   # t2_bottom skipped
 
+
+
+def test_snapshot_files(tmp_path):
+    """sbe_write_snapshot (antelope.cpp:340-421): one row of the 1-D integrated
+    conduction occupation per snapshot (sum over the other k indices of
+    w Re rho_H[Nval][Nval], / N2, "%16e    "), Ntotal doubles of Re
+    rho_H[Nval][Nval] and Ntotal complex rho_H[Nval][Nval-1] per snapshot."""
+    from antelope_amd import AnalyticModel
+    from antelope_amd.frontend import snapshot_interval, write_snapshot
+    from oracle import pyoracle
+    m = AnalyticModel.from_reference("KaneMele", t1=0.075, t2=0.025, tR=0.01, Mt2=1.5, phi0=0.9, a0=1.42)
+    from tests.cases import small_config
+    cfg = small_config(n=(4, 3), ncycles=2.0)
+    assert snapshot_interval(cfg, 4) == int(2 * math.pi / 0.014 / 4 / 0.25) and snapshot_interval(cfg, 0) == 0
+    rng = np.random.default_rng(1)
+    states = [rng.normal(size=(12, 4, 4)) + 1j * rng.normal(size=(12, 4, 4)) for _ in range(2)]
+    for i, st in enumerate(states):
+        write_snapshot(tmp_path, cfg, m, st, append=i > 0)
+    k, w, dV, tw = pyoracle.kgrid(cfg, m)
+    occ = np.loadtxt(tmp_path / "occupation_1d_integration.dat")
+    assert occ.shape == (2, 4)
+    nc = np.fromfile(tmp_path / "fisrt_conduction.dat").reshape(2, 12)
+    pi = np.fromfile(tmp_path / "fisrt_coherence.dat").reshape(2, 12, 2)
+    for i, st in enumerate(states):
+        want = np.array([np.sum((w * st[:, 2, 2].real)[j * 3:(j + 1) * 3]) for j in range(4)]) / 3
+        np.testing.assert_allclose(occ[i], want, rtol=1e-6)      # %16e keeps 7 digits
+        np.testing.assert_array_equal(nc[i], st[:, 2, 2].real)
+        np.testing.assert_array_equal(pi[i, :, 0] + 1j * pi[i, :, 1], st[:, 2, 1])
+    assert (tmp_path / "snapshot_log.dat").read_text() == ""
