@@ -31,6 +31,20 @@
 //    full per-k evaluation; its error is the Hermite error of the observable,
 //    the same order as interpolating the state, which was measured to give
 //    the same accuracy.
+//  * FP32 state. Near the ground state s = -d/|d| + delta with delta small,
+//    so 2 d x s is a difference of O(|d|) products that FP32 resolves only
+//    to ~1e-7 |d| absolute. The FP32 kernel therefore propagates delta =
+//    s + dh with the reference vector dh = d / sqrt(|d|^2 + eps^2), parallel
+//    to d(k + A(t)): d x dh = 0 and the dephasing acts on the part normal to
+//    d, so d(delta)/dt = f(delta) + d(dh)/dt with f the right-hand side of
+//    s itself (2 d x delta less the dephasing of the part of delta normal to
+//    d): the same ODE with no cancelling terms (d(dh)/dt from the k gradient
+//    of d along dA/dt). eps (2 % of a bound on |d|) keeps d(dh)/dt bounded
+//    where the gap closes; for |d| << eps the form reverts to s itself, whose
+//    FP32 error is small there (it scales with |d|). s <-> delta are
+//    converted in FP64 at the window ends, and the observables are formed in
+//    FP64 from s = delta - dh with FP64 k phases. The FP64 kernel propagates
+//    s as the oracle restates it.
 //  * A launch covers the samples [out0, out1) and ends with the state exactly
 //    at t_out1 (the last step is shortened to land there), so consecutive
 //    launches continue seamlessly and the partial-sum buffer stays bounded.
@@ -85,7 +99,8 @@ constexpr int kRow = 14;  // 12 phase values, then dA/dt (x, y)
 // lanes); the rows go to this wave's LDS slice, from which every lane reads
 // them as broadcasts.
 template <typename R>
-__device__ __forceinline__ void laser_rows(const AdaptiveArgs &A, double t, double hh, double tn, R *rows /*[6][kRow]*/)
+__device__ __forceinline__ void laser_rows(const AdaptiveArgs &A, double t, double hh, double tn, R *rows /*[6][kRow]*/,
+                                           double *row64 = nullptr /*[6][kRow]: FP64 copies of the rows*/)
 {
     const int lane = __lane_id();
     const int np = A.npulses;
@@ -131,6 +146,14 @@ __device__ __forceinline__ void laser_rows(const AdaptiveArgs &A, double t, doub
         if (j == 0) {
             rows[s * kRow + 12] = static_cast<R>(gx);
             rows[s * kRow + 13] = static_cast<R>(gy);
+        }
+        if (row64) {
+            row64[s * kRow + 2 * j] = cs * scale;
+            row64[s * kRow + 2 * j + 1] = sn * scale;
+            if (j == 0) {
+                row64[s * kRow + 12] = gx;
+                row64[s * kRow + 13] = gy;
+            }
         }
     }
     // the writes of other lanes of this wave are read next: LDS operations of
@@ -252,6 +275,44 @@ __device__ __forceinline__ Vec3<R> stage_d(const KPoint<R> &p, const R *row, R c
     return {dx, dy, M0 - c3 * sb};
 }
 
+// d(k + A) and dd/dt = grad_k d . dA/dt from k phases (er, ei) and a row
+// (with dA/dt in row[12..13]); any arithmetic type T.
+template <typename T, typename P>
+__device__ __forceinline__ void d_and_rate(const P *er, const P *ei, const T *row, const HaldaneArgs &a, Vec3<T> &d,
+                                           Vec3<T> &td)
+{
+    const T c3 = static_cast<T>(a.c3), ux = row[12], uy = row[13];
+    T dx = 0, dy = 0, sb = 0, tx = 0, ty = 0, tz = 0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        T ar, ai, br, bi;
+        cmul(static_cast<T>(er[j]), static_cast<T>(ei[j]), row[2 * j], row[2 * j + 1], ar, ai);
+        cmul(static_cast<T>(er[j + 3]), static_cast<T>(ei[j + 3]), row[6 + 2 * j], row[7 + 2 * j], br, bi);
+        const T al = static_cast<T>(a.va[j][0]) * ux + static_cast<T>(a.va[j][1]) * uy;
+        const T be = static_cast<T>(a.vb[j][0]) * ux + static_cast<T>(a.vb[j][1]) * uy;
+        dx += ar;
+        dy += ai;
+        sb += bi;
+        tx -= al * ai;
+        ty += al * ar;
+        tz += be * br;
+    }
+    d = {dx, dy, static_cast<T>(a.M0) - c3 * sb};
+    td = {tx, ty, -c3 * tz};
+}
+
+// Reference vector dh = d / sqrt(|d|^2 + eps^2) (the unit vector d/|d| where
+// |d| >> eps, going smoothly to 0 at a gap closing) and its rate
+// (dd/dt - dh (dh . dd/dt)) / sqrt(|d|^2 + eps^2).
+template <typename T>
+__device__ __forceinline__ void unit_and_rate(const Vec3<T> &d, const Vec3<T> &td, T eps2, Vec3<T> &dh, Vec3<T> &dht)
+{
+    const T r = T(1) / sqrt(d.x * d.x + d.y * d.y + d.z * d.z + eps2);
+    dh = {d.x * r, d.y * r, d.z * r};
+    const T p = dh.x * td.x + dh.y * td.y + dh.z * td.z;
+    dht = {(td.x - dh.x * p) * r, (td.y - dh.y * p) * r, (td.z - dh.z * p) * r};
+}
+
 // f/2 = d x y - (g2/2) y + ((g2 - g1)/2) (y.d / |d|^2) d  (T1 off: g1 = 0)
 template <typename R, bool DEPH>
 __device__ __forceinline__ Vec3<R> half_f(const Vec3<R> &d, const Vec3<R> &y, R gh2, R gh12)
@@ -296,7 +357,17 @@ __device__ void adaptive_body(const AdaptiveArgs &A, int64_t blk, const int64_t 
     const int64_t kbase = static_cast<int64_t>(wave) * 64 * KPT + lane;
     R *st = static_cast<R *>(a.state);
 
+    // FP32 state: the propagated variable is delta = s + dh(k + A(t)), the
+    // deviation from the instantaneous ground state -dh (see the FP32 notes
+    // above); FP64 copies of the k phases give dh and the observables in FP64.
+    constexpr bool kDelta = kComp;
     KPoint<R> p[KPT];
+    // delta form: d(k + A) in FP64 from FP64 phases and laser rows, rounded to
+    // FP32 per stage. FP32 phases or rows would carry a rounding error that is
+    // fixed per k, or shared by all k at one time, and so shift the dynamical
+    // phases coherently; rounding d itself leaves errors that average out.
+    // ph: cos a_0..a_2, cos b_0..b_2, then the sines
+    double ph[kDelta ? KPT : 1][12];
 #pragma unroll
     for (int q = 0; q < KPT; ++q) {
         const int64_t k = kbase + 64 * q;
@@ -311,6 +382,12 @@ __device__ void adaptive_body(const AdaptiveArgs &A, int64_t blk, const int64_t 
             p[q].ei[j] = static_cast<R>(sa);
             p[q].er[j + 3] = static_cast<R>(cb);
             p[q].ei[j + 3] = static_cast<R>(sb);
+            if constexpr (kDelta) {
+                ph[q][j] = ca;
+                ph[q][6 + j] = sa;
+                ph[q][j + 3] = cb;
+                ph[q][6 + j + 3] = sb;
+            }
         }
         p[q].n = st[k];
         p[q].sx = st[ks + k];
@@ -334,6 +411,54 @@ __device__ void adaptive_body(const AdaptiveArgs &A, int64_t blk, const int64_t 
 
     __shared__ R lds_rows[kBlock / 64][6 * kRow];
     R *rows = lds_rows[threadIdx.x >> 6];
+    // eps of the reference vector: 2 % of a bound on |d| over the zone, so
+    // that its rate stays below |dd/dt| / eps where the gap closes
+    const double dmax = 3.0 * fabs(a.t1) + fabs(a.M0) + 3.0 * fabs(a.c3);
+    const double eps2 = (0.02 * dmax) * (0.02 * dmax);
+    __shared__ double lds_row64[kBlock / 64][kDelta ? 6 * kRow : 1];
+    double *row64 = kDelta ? lds_row64[threadIdx.x >> 6] : nullptr;
+    const double *end64 = row64 + 5 * kRow;  // the step end (window start: its time)
+    // FP64 dh and its rate at the step end
+    auto dh64 = [&](int q, Vec3<double> &dh, Vec3<double> &dht) {
+        Vec3<double> d, td;
+        d_and_rate<double, double>(ph[q], ph[q] + 6, end64, a, d, td);
+        unit_and_rate(d, td, eps2, dh, dht);
+    };
+    // FP64 observables of the delta state at the step end: s = delta - dh,
+    // ds/dt = 2 hf (the stage without the dh rate)
+    auto observe64 = [&](int q, const Vec3<R> &hf, double o[8], double od[8]) {
+        Vec3<double> dh, dht;
+        dh64(q, dh, dht);
+        KPoint<double> k64;
+        for (int j = 0; j < 6; ++j) {
+            k64.er[j] = ph[q][j];
+            k64.ei[j] = ph[q][6 + j];
+        }
+        k64.w = p[q].w;
+        const Vec3<double> s64{static_cast<double>(p[q].sx) - static_cast<double>(p[q].cx) - dh.x,
+                               static_cast<double>(p[q].sy) - static_cast<double>(p[q].cy) - dh.y,
+                               static_cast<double>(p[q].sz) - static_cast<double>(p[q].cz) - dh.z};
+        const Vec3<double> f64{2.0 * hf.x, 2.0 * hf.y, 2.0 * hf.z};
+        observe<double>(k64, static_cast<double>(p[q].n) - static_cast<double>(p[q].cn), s64, f64, end64, a,
+                        a.inter_intra ? 1.0 : 0.0, o, od);
+    };
+    // stage of the delta form at the FP64 row r64: hf = d x y - dephasing
+    // (f/2 of s), k = hf + dh_t / 2
+    auto stage_delta = [&](int q, const double *r64, const Vec3<R> &y, Vec3<R> &hf, Vec3<R> &dh) {
+        Vec3<double> d64, td64;
+        d_and_rate<double, double>(ph[q], ph[q] + 6, r64, a, d64, td64);
+        const Vec3<R> d{static_cast<R>(d64.x), static_cast<R>(d64.y), static_cast<R>(d64.z)};
+        const Vec3<R> td{static_cast<R>(td64.x), static_cast<R>(td64.y), static_cast<R>(td64.z)};
+        Vec3<R> dht;
+        unit_and_rate(d, td, static_cast<R>(eps2), dh, dht);
+        hf = half_f<R, DEPH>(d, y, gh2, gh12);
+        return Vec3<R>{hf.x + R(0.5) * dht.x, hf.y + R(0.5) * dht.y, hf.z + R(0.5) * dht.z};
+    };
+    // Kahan pair (hi, c) holding v = hi - c from an FP64 value
+    auto split = [](double v, R &hi, R &c) {
+        hi = static_cast<R>(v);
+        c = static_cast<R>(static_cast<double>(hi) - v);
+    };
 
     int64_t m = out0;
     double t = A.atmin + A.dt_out * static_cast<double>(m);
@@ -348,16 +473,28 @@ __device__ void adaptive_body(const AdaptiveArgs &A, int64_t blk, const int64_t 
     // cubic Hermite interpolant of the two ends. At the window start: the
     // sample itself, and the first stage.
     Vec3<R> k1[KPT];
+    Vec3<R> dh0[KPT];  // delta form: dh at the current step start (error scale)
     double O0, D0;
     {
-        laser_rows<R>(A, t, 0.0, t, rows);
+        laser_rows<R>(A, t, 0.0, t, rows, row64);
         const R *row = rows + 5 * kRow;
         double o[8] = {0, 0, 0, 0, 0, 0, 0, 0}, od[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
         for (int q = 0; q < KPT; ++q) {
-            k1[q] = half_f<R, DEPH>(stage_d(p[q], row, c3, M0), Vec3<R>{p[q].sx, p[q].sy, p[q].sz}, gh2, gh12);
-            observe<R>(p[q], p[q].n - p[q].cn, Vec3<R>{p[q].sx - p[q].cx, p[q].sy - p[q].cy, p[q].sz - p[q].cz},
-                       Vec3<R>{2 * k1[q].x, 2 * k1[q].y, 2 * k1[q].z}, row, a, ii, o, od);
+            if constexpr (kDelta) {
+                Vec3<double> dh, dht;  // s -> delta = s + dh at the window start, in FP64
+                dh64(q, dh, dht);
+                split(static_cast<double>(p[q].sx) - static_cast<double>(p[q].cx) + dh.x, p[q].sx, p[q].cx);
+                split(static_cast<double>(p[q].sy) - static_cast<double>(p[q].cy) + dh.y, p[q].sy, p[q].cy);
+                split(static_cast<double>(p[q].sz) - static_cast<double>(p[q].cz) + dh.z, p[q].sz, p[q].cz);
+                Vec3<R> hf;
+                k1[q] = stage_delta(q, end64, Vec3<R>{p[q].sx, p[q].sy, p[q].sz}, hf, dh0[q]);
+                observe64(q, hf, o, od);
+            } else {
+                k1[q] = half_f<R, DEPH>(stage_d(p[q], row, c3, M0), Vec3<R>{p[q].sx, p[q].sy, p[q].sz}, gh2, gh12);
+                observe<R>(p[q], p[q].n - p[q].cn, Vec3<R>{p[q].sx - p[q].cx, p[q].sy - p[q].cy, p[q].sz - p[q].cz},
+                           Vec3<R>{2 * k1[q].x, 2 * k1[q].y, 2 * k1[q].z}, row, a, ii, o, od);
+            }
         }
         O0 = wave_reduce8(o);
         D0 = wave_reduce8(od);
@@ -375,9 +512,9 @@ __device__ void adaptive_body(const AdaptiveArgs &A, int64_t blk, const int64_t 
         const bool last = t + 1.0001 * hh >= t_end;
         if (last) hh = t_end - t;
         const double tn = last ? t_end : t + hh;
-        laser_rows<R>(A, t, hh, tn, rows);
+        laser_rows<R>(A, t, hh, tn, rows, row64);
 
-        Vec3<R> inc[KPT], y5[KPT], kv[KPT][6];
+        Vec3<R> inc[KPT], y5[KPT], kv[KPT][6], dh1[KPT];
         double err_k = 0.0;
 #pragma unroll
         for (int q = 0; q < KPT; ++q) {
@@ -396,7 +533,13 @@ __device__ void adaptive_body(const AdaptiveArgs &A, int64_t blk, const int64_t 
                         y.z += aij * kv[q][j].z;
                     }
                 }
-                kv[q][i] = half_f<R, DEPH>(stage_d(p[q], rows + (i - 1) * kRow, c3, M0), y, gh2, gh12);
+                if constexpr (kDelta) {
+                    Vec3<R> hf, dh;
+                    kv[q][i] = stage_delta(q, row64 + (i - 1) * kRow, y, hf, dh);
+                    if (i == 4) dh1[q] = dh;  // c = 1: the step end
+                } else {
+                    kv[q][i] = half_f<R, DEPH>(stage_d(p[q], rows + (i - 1) * kRow, c3, M0), y, gh2, gh12);
+                }
             }
             // 5th-order increment (state precision) and the error estimate (FP64)
             inc[q] = Vec3<R>{0, 0, 0};
@@ -421,8 +564,14 @@ __device__ void adaptive_body(const AdaptiveArgs &A, int64_t blk, const int64_t 
                 const double sc = A.atol + A.rtol * fmax(fabs(static_cast<double>(y)), fabs(static_cast<double>(z)));
                 return e == 0.0 ? 0.0 : fabs(e) / sc;
             };
-            err_k = nanmax(err_k, nanmax(ratio(ex, y0.x, y5[q].x),
-                                         nanmax(ratio(ey, y0.y, y5[q].y), ratio(ez, y0.z, y5[q].z))));
+            if constexpr (kDelta) {  // scales on s = delta - dh at both ends
+                const Vec3<R> s0{y0.x - dh0[q].x, y0.y - dh0[q].y, y0.z - dh0[q].z};
+                const Vec3<R> s1{y5[q].x - dh1[q].x, y5[q].y - dh1[q].y, y5[q].z - dh1[q].z};
+                err_k = nanmax(err_k, nanmax(ratio(ex, s0.x, s1.x), nanmax(ratio(ey, s0.y, s1.y), ratio(ez, s0.z, s1.z))));
+            } else {
+                err_k = nanmax(err_k, nanmax(ratio(ex, y0.x, y5[q].x),
+                                             nanmax(ratio(ey, y0.y, y5[q].y), ratio(ez, y0.z, y5[q].z))));
+            }
         }
         const double err = wave_max(err_k);
 
@@ -431,7 +580,9 @@ __device__ void adaptive_body(const AdaptiveArgs &A, int64_t blk, const int64_t 
             double o[8] = {0, 0, 0, 0, 0, 0, 0, 0}, od[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
             for (int q = 0; q < KPT; ++q) {
-                k1[q] = half_f<R, DEPH>(stage_d(p[q], row7, c3, M0), y5[q], gh2, gh12);  // f(t+h, y5)
+                Vec3<R> hf;
+                if constexpr (kDelta) k1[q] = stage_delta(q, end64, y5[q], hf, dh0[q]);  // f(t+h, y5)
+                else k1[q] = half_f<R, DEPH>(stage_d(p[q], row7, c3, M0), y5[q], gh2, gh12);
                 if (kComp) {
                     kahan(p[q].sx, p[q].cx, inc[q].x);
                     kahan(p[q].sy, p[q].cy, inc[q].y);
@@ -441,8 +592,9 @@ __device__ void adaptive_body(const AdaptiveArgs &A, int64_t blk, const int64_t 
                     p[q].sy += inc[q].y;
                     p[q].sz += inc[q].z;
                 }
-                observe<R>(p[q], p[q].n - p[q].cn, Vec3<R>{p[q].sx - p[q].cx, p[q].sy - p[q].cy, p[q].sz - p[q].cz},
-                           Vec3<R>{2 * k1[q].x, 2 * k1[q].y, 2 * k1[q].z}, row7, a, ii, o, od);
+                if constexpr (kDelta) observe64(q, hf, o, od);
+                else observe<R>(p[q], p[q].n - p[q].cn, Vec3<R>{p[q].sx - p[q].cx, p[q].sy - p[q].cy, p[q].sz - p[q].cz},
+                                Vec3<R>{2 * k1[q].x, 2 * k1[q].y, 2 * k1[q].z}, row7, a, ii, o, od);
             }
             const double O1 = wave_reduce8(o), D1 = wave_reduce8(od);
             // samples inside (t, tn]
@@ -467,6 +619,16 @@ __device__ void adaptive_body(const AdaptiveArgs &A, int64_t blk, const int64_t 
         }
     }
 
+    if constexpr (kDelta) {  // delta -> s at the window end (row64 holds the last step end, t_end)
+#pragma unroll
+        for (int q = 0; q < KPT; ++q) {
+            Vec3<double> dh, dht;
+            dh64(q, dh, dht);
+            split(static_cast<double>(p[q].sx) - static_cast<double>(p[q].cx) - dh.x, p[q].sx, p[q].cx);
+            split(static_cast<double>(p[q].sy) - static_cast<double>(p[q].cy) - dh.y, p[q].sy, p[q].cy);
+            split(static_cast<double>(p[q].sz) - static_cast<double>(p[q].cz) - dh.z, p[q].sz, p[q].cz);
+        }
+    }
 #pragma unroll
     for (int q = 0; q < KPT; ++q) {
         const int64_t k = kbase + 64 * q;
@@ -495,7 +657,8 @@ __device__ void adaptive_body(const AdaptiveArgs &A, int64_t blk, const int64_t 
 // left leave at once. One kernel serves both cases, so a solve gives the
 // same bits alone and in a batch.
 template <typename R, bool DEPH, int KPT>
-__global__ __launch_bounds__(kBlock) void haldane_adaptive_kernel(const AdaptiveArgs *__restrict__ insts,
+__global__ __launch_bounds__(kBlock)
+__attribute__((amdgpu_waves_per_eu(KPT == 1 && sizeof(R) == 8 ? 2 : 1))) void haldane_adaptive_kernel(const AdaptiveArgs *__restrict__ insts,
                                                                   const int32_t *__restrict__ block_inst,
                                                                   const int32_t *__restrict__ block_first,
                                                                   int64_t nwin)

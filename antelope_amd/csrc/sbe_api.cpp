@@ -134,9 +134,10 @@ int make_plan(const sbe_config &cfg, const sbe_model &m, int64_t nk, int64_t nt,
     p.kpt = 1;
     if (const char *e = std::getenv("ANTELOPE_HALDANE_KPT")) p.kpt = std::atoi(e) == 2 ? 2 : 1;
     if (cfg.rk_order == SBE_RK_CASH_KARP) {  // adaptive: 64 kpt consecutive k points per wave (step-control block)
-        // two chains per lane pay in FP64 (1.4x on the batched workload); the FP32 kernel
-        // loses its second wave per SIMD to registers with two, so it keeps one
-        p.kpt = cfg.precision == SBE_PREC_FP32 ? 1 : 2;
+        // one k point per lane: the FP64 kernel then fits two waves per SIMD (1.1x over
+        // two chains per lane at one wave on the batched workload) and the 64-point
+        // step-control blocks take longer steps than 128-point ones
+        p.kpt = 1;
         if (const char *e = std::getenv("ANTELOPE_ADAPTIVE_KPT")) p.kpt = std::atoi(e) == 1 ? 1 : 2;
     }
     const int64_t unit = static_cast<int64_t>(kBlock) * p.kpt;
