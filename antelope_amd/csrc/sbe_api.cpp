@@ -156,6 +156,7 @@ int make_plan(const sbe_config &cfg, const sbe_model &m, int64_t nk, int64_t nt,
     // the adaptive stepper reads table rows at the sample times only
     const bool adaptive = cfg.rk_order == SBE_RK_CASH_KARP;
     p.bytes_tab = static_cast<size_t>(nt) * (adaptive ? 1 : nstage) * 12 * (sizeof(double) + sizeof(float));
+    if (adaptive) p.bytes_tab += static_cast<size_t>(std::max<int64_t>(nt, 2)) * 4 * sizeof(double);  // A, dA/dt at t_n
     p.bytes_adaptive = adaptive ? static_cast<size_t>(p.nwaves) * (sizeof(double) + 2 * sizeof(int64_t)) + 256 +
                                       sizeof(sbe::PulseConst) * sbe::kMaxDevicePulses
                                 : 0;
@@ -474,6 +475,29 @@ int sbe_create(const sbe_config *cfg, const sbe_model *model, int64_t k_begin, i
         if (!rc) rc = hip_check(ctx, hipMemcpy(d32, tab32.data(), n64 * sizeof(float), hipMemcpyHostToDevice), "upload table");
         h.phase_tab = ctx->d_phase_tab;
         h.phase_tab32 = d32;
+        if (ctx->adaptive && !rc) {  // A - A(t_min) and the consistent dA/dt = -E at t_n, summed in pulse order
+            const int64_t na = std::max<int64_t>(nt, 2);  // an interval to interpolate in
+            std::vector<double> at(static_cast<size_t>(na) * 4);
+            for (int64_t n = 0; n < na; ++n) {
+                const double tn = ctx->laser.atmin + ctx->laser.dt * static_cast<double>(n);
+                double sx = 0.0, sy = 0.0, gx = 0.0, gy = 0.0;
+                for (const sbe::PulseConst &pc : ctx->laser.pulses) {
+                    double pa[3], pe[3];
+                    sbe::pulse_field(pc, tn, true, pa, pe);
+                    sx += pa[0];
+                    sy += pa[1];
+                    gx -= pe[0];
+                    gy -= pe[1];
+                }
+                at[4 * n] = sx - ctx->laser.A_at_min[0];
+                at[4 * n + 1] = sy - ctx->laser.A_at_min[1];
+                at[4 * n + 2] = gx;
+                at[4 * n + 3] = gy;
+            }
+            double *d_at = reinterpret_cast<double *>(d32 + n64);
+            rc = hip_check(ctx, hipMemcpy(d_at, at.data(), at.size() * sizeof(double), hipMemcpyHostToDevice), "upload table");
+            ctx->aargs.a_tab = d_at;
+        }
     }
     if (!rc) rc = hip_check(ctx, hipMemsetAsync(ctx->d_series, 0, p.bytes_series, ctx->stream), "clear series");
     if (ctx->adaptive && !rc) {

@@ -104,6 +104,7 @@ struct AdaptiveArgs {
     int32_t npulses;
     double A_at_min[3];
     const PulseConst *pulses;  // [npulses] in device memory
+    const double *a_tab;       // [nt][4]: A - A(t_min) (x, y) and dA/dt (x, y) at t_n
     double *hstep;             // [nwaves] proposed step size carried between launches
     int64_t *counts;           // [nwaves][2] accepted and rejected step attempts
     int32_t *fail;             // set when a wave exhausts its attempts or its step falls below hmin
