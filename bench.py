@@ -47,6 +47,11 @@ PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
 # Flops per k-point RK4 step of the velocity-gauge step kernel (hand count:
 # sample 36, four stages of input 6 + h 10 + h x y 12 + dephasing 4, update 27)
 VEL_FLOPS_PER_KSTEP = {"fp64": 190.0, "fp32": 190.0}
+# Executed FP64 flops per accepted k-point step of the adaptive Cash-Karp FP64
+# kernel (one k point per lane), from SQ_INSTS_VALU_{FMA,MUL,ADD}_F64 (FMA = 2)
+# on a batch of this workload's solves; includes rejected attempts and the
+# per-wave laser, controller and reduction work counted on all 64 lanes
+ADAPT_FLOPS_PER_KSTEP = {"fp64": 1697.0}
 
 
 def wannier_workload(precision: str = "fp64"):
@@ -315,6 +320,18 @@ def bench_batch(args) -> int:
     kt = (f["step_kernel_ms"] if f else plain["timing"]["step_ms"]) * 1e-3
     abytes = ALGO_BYTES_PER_KSTEP["fp32" if form == "adaptive_fp32" else "fp64"]
     ach = eq * abytes / kt / 1e9
+    compute = None
+    if form == "plain_fp64_rk4":
+        fl = eq * FLOPS_PER_KSTEP["fp64"] / kt / 1e12
+        compute = {"bound": "valu", "kernel": "haldane_batch_kernel<double, 4, false, true>", "achieved": fl,
+                   "peak": PEAK_TFLOPS["fp64"], "unit": "TFLOP/s", "frac": fl / PEAK_TFLOPS["fp64"],
+                   "flops_per_kstep": FLOPS_PER_KSTEP["fp64"]}
+    elif form == "adaptive_fp64":
+        ksa = f["accepted_block_steps"] * f["k_per_block"]
+        fl = ksa * ADAPT_FLOPS_PER_KSTEP["fp64"] / kt / 1e12
+        compute = {"bound": "valu", "kernel": "haldane_adaptive_kernel<double, true, 1>", "achieved": fl,
+                   "peak": PEAK_TFLOPS["fp64"], "unit": "TFLOP/s", "frac": fl / PEAK_TFLOPS["fp64"],
+                   "flops_per_accepted_kstep": ADAPT_FLOPS_PER_KSTEP["fp64"], "accepted_ksteps": ksa}
     line = {"metric": METRIC, "value": value, "unit": "k-point·RK-steps/s (oracle-equivalent, dt=0.25)",
             "n_gpus": 1, "steps": int(max(inf.nt for inf in plain["info"])), "warmup": 1,
             "ms_per_step": eq / value * 1e3 / (n * grid * grid),
@@ -331,7 +348,9 @@ def bench_batch(args) -> int:
                          "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": ach / PEAK_HBM_GBS,
                          "traffic": None, "algo_bytes_per_kstep": abytes,
                          "note": "oracle-equivalent k-steps x algorithmic state bytes over the step-kernel time; the "
-                                 "state stays in registers for a window, the kernels are VALU/latency-bound"},
+                                 "state stays in registers for a window, the kernels are VALU/latency-bound "
+                                 "(compute_roofline)"},
+            "compute_roofline": compute,
             "cpu_baseline": None}
     print(json.dumps(line), flush=True)
     return 0
