@@ -988,6 +988,7 @@ hipError_t nb_launch_init(const NbArgs &a, const double2 *H0, double2 *rho, doub
 hipError_t nb_launch_sample(const NbArgs &a, const double2 *mats, const double2 *rho, double *partials,
                             int64_t step_local, double2 *U, int ureuse, hipStream_t s)
 {
+    if (a.tile16) return nb16_launch_sample(a, mats, rho, partials, step_local, U, ureuse, s);
     nb_sample_kernel<<<dim3(static_cast<unsigned>(a.sample_waves)), 64, nb_lds_bytes(a.N), s>>>(a, mats, rho, partials,
                                                                                                step_local, U, ureuse);
     return hipGetLastError();
@@ -996,6 +997,7 @@ hipError_t nb_launch_sample(const NbArgs &a, const double2 *mats, const double2 
 hipError_t nb_launch_rhs(const NbArgs &a, const NbStage &sg, const double2 *mats, double2 *rho, double2 *kbuf,
                          double2 *U, hipStream_t s)
 {
+    if (a.tile16) return nb16_launch_rhs(a, sg, mats, rho, kbuf, U, s);
     nb_rhs_kernel<<<dim3(static_cast<unsigned>(a.nk)), 64, nb_lds_bytes(a.N), s>>>(a, sg, mats, rho, kbuf, U);
     return hipGetLastError();
 }

@@ -42,6 +42,7 @@ struct NbArgs {
     // H and J come from analytic.hpp instead of the Fourier sums; 0 = Wannier
     int model_kind;
     double mparams[24];
+    int tile16;            // N <= 16: stage and sample kernels with the matrices in MFMA registers
 };
 
 // One evaluation of the Fourier sums at k + A(t) with field E(t).
@@ -91,5 +92,10 @@ hipError_t nb_launch_sample(const NbArgs &a, const double2 *mats, const double2 
                             int64_t step_local, double2 *U, int ureuse, hipStream_t s);
 hipError_t nb_launch_rhs(const NbArgs &a, const NbStage &sg, const double2 *mats, double2 *rho, double2 *kbuf,
                          double2 *U, hipStream_t s);
+// N <= 16 forms of the two above (nband16.hip)
+hipError_t nb16_launch_rhs(const NbArgs &a, const NbStage &sg, const double2 *mats, double2 *rho, double2 *kbuf,
+                           double2 *U, hipStream_t s);
+hipError_t nb16_launch_sample(const NbArgs &a, const double2 *mats, const double2 *rho, double *partials,
+                              int64_t step_local, double2 *U, int ureuse, hipStream_t s);
 
 }  // namespace sbe

@@ -1,0 +1,438 @@
+// nband16.hip - N-band (Wannier) stage and sample kernels for N <= 16 with
+// every matrix in MFMA registers.
+//
+// Same mathematics as nb_rhs_kernel / nb_sample_kernel (SBEs.h:762-861 RHS,
+// SBEs.h:694-760 currents), restructured for one 16x16 FP64 MFMA tile per
+// matrix and one wave per k point:
+//  * A 16x16 complex matrix is a Tile: lane l = c + 16 g holds the elements
+//    (g + 4r, c), r = 0..3 - the D layout of v_mfma_f64_16x16x4f64. Bands
+//    beyond N are zero padding (identity in the eigenvectors).
+//  * C = A B runs as four real MFMAs per K block of 4. The B operand of
+//    block kb is register kb of B's tile; the A operand is register kb of
+//    A^T's tile. For Hermitian A (H, H~, rho) and for A = X^dag that is
+//    the conjugate of A's own tile, so products of that form need no data
+//    movement; a transpose (U^T, T^dag) goes through a padded 16x17 LDS
+//    buffer of the wave, conflict-free.
+//  * Eigenvectors U(k + A(t)) are refined from the stored ones (warm start)
+//    by simultaneous Jacobi rotations written as matrix products: with
+//    W = U^dag H U nearly diagonal, K_ij = W_ij / (D/2 + s sqrt(D^2/4 +
+//    |W_ij|^2)), D = W_jj - W_ii (the 2x2 Jacobi angle; W_ij / D for small
+//    W_ij, bounded for near-degenerate pairs), K anti-Hermitian;
+//    U <- U (I + K + K^2/2), W <- V^dag W V until max|K| < 1e-8, then a last
+//    U (I + K) and one Newton-Schulz step U (3I - U^dag U)/2 to keep U
+//    unitary to rounding (after a round with max|K| > 1e-3, whose V is
+//    unitary only to |K|^4, U is restored at once and W recomputed from H).
+//    Convergence is quadratic; between RK stages two or three rounds reach
+//    machine precision. The columns keep the order of
+//    the warm start (bands are followed continuously); the sample kernel
+//    ranks them by eigenvalue, as the reference's ascending zheevr order.
+//  * The dephasing U (Gamma o (U^dag y U)) U^dag is g2 y + (g1 - g2)
+//    U diag(U^dag y U) U^dag (Gamma = g1 on the diagonal, g2 off it): one
+//    product y U, a column reduction and one product U diag(d) U^dag.
+//  * Currents: Tr(rho J_a) elementwise; the intraband part sum_i
+//    (U^dag rho U)_ii (U^dag J_a U)_ii from the products rho U and J_a U.
+#include <cmath>
+
+#include "nband.hpp"
+
+namespace sbe {
+namespace {
+
+using cd = double2;
+using d4 = double __attribute__((ext_vector_type(4)));
+
+struct Tile {
+    double re[4], im[4];
+};
+
+constexpr int kTpad = 17;  // LDS row pitch (complex) of the transpose buffer
+
+__device__ __forceinline__ int lane_c() { return threadIdx.x & 15; }
+__device__ __forceinline__ int lane_g() { return (threadIdx.x >> 4) & 3; }
+
+// C = A B. Aop holds A^T's tile (register kb = A[c][4 kb + g]); conjA and
+// conjB conjugate the operands as they are read.
+__device__ __forceinline__ Tile cmm(const Tile &Aop, bool conjA, const Tile &B, bool conjB = false)
+{
+    d4 re = d4{0.0, 0.0, 0.0, 0.0}, im = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) {
+        const double ar = Aop.re[kb], ai = conjA ? -Aop.im[kb] : Aop.im[kb];
+        const double br = B.re[kb], bi = conjB ? -B.im[kb] : B.im[kb];
+        re = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, re, 0, 0, 0);
+        re = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, bi, re, 0, 0, 0);
+        im = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bi, im, 0, 0, 0);
+        im = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, br, im, 0, 0, 0);
+    }
+    Tile C;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        C.re[r] = re[r];
+        C.im[r] = im[r];
+    }
+    return C;
+}
+
+__device__ __forceinline__ void wave_lds_order()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// X^T through the wave's LDS buffer (16 x kTpad complex)
+__device__ __forceinline__ Tile transpose(const Tile &X, cd *buf)
+{
+    const int c = lane_c(), g = lane_g();
+#pragma unroll
+    for (int r = 0; r < 4; ++r) buf[(g + 4 * r) * kTpad + c] = make_double2(X.re[r], X.im[r]);
+    wave_lds_order();
+    Tile T;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const cd v = buf[c * kTpad + g + 4 * r];
+        T.re[r] = v.x;
+        T.im[r] = v.y;
+    }
+    wave_lds_order();  // the next transpose overwrites the buffer
+    return T;
+}
+
+// N x N row-major matrix -> tile (zero padding; pad_one: identity padding)
+__device__ __forceinline__ Tile load_tile(const cd *M, int N, bool pad_one = false)
+{
+    const int c = lane_c(), g = lane_g();
+    Tile T;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int i = g + 4 * r;
+        cd v = make_double2(pad_one && i == c ? 1.0 : 0.0, 0.0);
+        if (i < N && c < N) v = M[i * N + c];
+        T.re[r] = v.x;
+        T.im[r] = v.y;
+    }
+    return T;
+}
+
+// the transpose of an N x N row-major matrix -> tile
+__device__ __forceinline__ Tile load_tile_t(const cd *M, int N, bool pad_one = false)
+{
+    const int c = lane_c(), g = lane_g();
+    Tile T;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int i = g + 4 * r;
+        cd v = make_double2(pad_one && i == c ? 1.0 : 0.0, 0.0);
+        if (i < N && c < N) v = M[c * N + i];
+        T.re[r] = v.x;
+        T.im[r] = v.y;
+    }
+    return T;
+}
+
+__device__ __forceinline__ void store_tile(cd *M, const Tile &T, int N)
+{
+    const int c = lane_c(), g = lane_g();
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int i = g + 4 * r;
+        if (i < N && c < N) M[i * N + c] = make_double2(T.re[r], T.im[r]);
+    }
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ double wave_max(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Re sum_i conj(X_ic) Y_ic for this lane's column c (the diagonal of X^dag Y)
+__device__ __forceinline__ double col_dot(const Tile &X, const Tile &Y)
+{
+    double s = 0.0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) s += X.re[r] * Y.re[r] + X.im[r] * Y.im[r];
+    s += __shfl_xor(s, 16, 64);
+    s += __shfl_xor(s, 32, 64);
+    return s;
+}
+
+// The diagonal of a tile, for this lane's rows (lam_r) and column (lam_c).
+__device__ __forceinline__ void diag_bcast(const Tile &W, double lam_r[4], double &lam_c)
+{
+    const int c = lane_c(), g = lane_g();
+    double dv = 0.0;  // the diagonal element this lane holds, if any: (c, c) with c = g + 4r
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+        if (g + 4 * r == c) dv = W.re[r];
+    lam_c = __shfl(dv, c + 16 * (c & 3), 64);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int i = g + 4 * r;
+        lam_r[r] = __shfl(dv, i + 16 * (i & 3), 64);
+    }
+}
+
+// U <- U (3I - U^dag U) / 2 (Newton-Schulz: unitarity error e -> ~e^2)
+__device__ __forceinline__ void orthonormalize(Tile &U, Tile &Ut, cd *buf)
+{
+    const int c = lane_c(), g = lane_g();
+    const Tile G = cmm(U, true, U);
+    Tile S;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        S.re[r] = (g + 4 * r == c ? 1.5 : 0.0) - 0.5 * G.re[r];
+        S.im[r] = -0.5 * G.im[r];
+    }
+    U = cmm(Ut, false, S);
+    Ut = transpose(U, buf);
+}
+
+// Warm-started eigenvectors of the Hermitian H: U (and its transpose Ut) in,
+// refined U, Ut out.
+__device__ void eig_refine(const Tile &H, Tile &U, Tile &Ut, int N, cd *buf)
+{
+    const int c = lane_c(), g = lane_g();
+    Tile W = cmm(U, true, cmm(H, true, U));  // U^dag (H U)
+    for (int it = 0; it < 12; ++it) {
+        double lam_r[4], lam_c;
+        diag_bcast(W, lam_r, lam_c);
+        Tile K;
+        double kmax = 0.0;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int i = g + 4 * r;
+            K.re[r] = 0.0;
+            K.im[r] = 0.0;
+            const double w2 = W.re[r] * W.re[r] + W.im[r] * W.im[r];
+            if (i != c && i < N && c < N && w2 > 0.0) {
+                const double D = lam_c - lam_r[r];
+                const double s = (D > 0.0 || (D == 0.0 && i < c)) ? 1.0 : -1.0;
+                const double den = 0.5 * D + s * sqrt(0.25 * D * D + w2);
+                K.re[r] = W.re[r] / den;
+                K.im[r] = W.im[r] / den;
+                kmax = fmax(kmax, sqrt(K.re[r] * K.re[r] + K.im[r] * K.im[r]));
+            }
+        }
+        kmax = wave_max(kmax);
+        if (kmax == 0.0) break;
+        Tile V;
+        const bool small = kmax < 1e-8;
+        if (small) {  // I + K: the neglected K^2 / 2 is below 1e-16
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                V.re[r] = K.re[r] + (g + 4 * r == c ? 1.0 : 0.0);
+                V.im[r] = K.im[r];
+            }
+        } else {  // I + K + K^2/2, unitary to |K|^4 / 4; K^T's tile is -conj(K)
+            Tile Kn;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                Kn.re[r] = -K.re[r];
+                Kn.im[r] = -K.im[r];
+            }
+            const Tile K2 = cmm(Kn, true, K);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                V.re[r] = K.re[r] + 0.5 * K2.re[r] + (g + 4 * r == c ? 1.0 : 0.0);
+                V.im[r] = K.im[r] + 0.5 * K2.im[r];
+            }
+        }
+        U = cmm(Ut, false, V);  // U V
+        Ut = transpose(U, buf);
+        if (small) break;
+        if (kmax > 1e-3) {  // V far from unitary: restore U, then W from H itself
+            orthonormalize(U, Ut, buf);
+            W = cmm(U, true, cmm(H, true, U));
+        } else {
+            W = cmm(V, true, cmm(W, true, V));  // V^dag (W V)
+        }
+    }
+    orthonormalize(U, Ut, buf);
+}
+
+// ------------------------------------------------------------- RK stage --
+// Stage i: y = rho + sum_j dt a_ij k_j; k_i = -i(H~ y - (H~ y)^dag)
+// - U (Gamma o (U^dag y U)) U^dag; after the last stage rho += sum_i dt b_i k_i.
+__global__ __launch_bounds__(64) void nb16_rhs_kernel(const NbArgs a, const NbStage sg, const cd *__restrict__ mats,
+                                                      cd *__restrict__ rho, cd *__restrict__ kbuf,
+                                                      cd *__restrict__ Ust)
+{
+    __shared__ cd buf[16 * kTpad];
+    const int64_t k = blockIdx.x;
+    if (k >= a.nk) return;
+    const int N = a.N, nn = N * N;
+    const int c = lane_c(), g = lane_g();
+    const int64_t kstride = a.nk_stride * nn;
+    cd *rk = rho + k * nn;
+    Tile Y = load_tile(rk, N);
+    for (int j = 0; j < sg.stage; ++j) {
+        if (sg.a[j] == 0.0) continue;
+        const Tile Kj = load_tile(kbuf + j * kstride + k * nn, N);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            Y.re[r] += sg.a[j] * Kj.re[r];
+            Y.im[r] += sg.a[j] * Kj.im[r];
+        }
+    }
+    const Tile Ht = load_tile(mats + k * nn, N);  // mat 0: H~ = H + E.D
+    const Tile T = cmm(Ht, true, Y);              // H~ y
+    const Tile Tt = transpose(T, buf);
+    Tile X;  // -i (T - T^dag)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const double dr = T.re[r] - Tt.re[r], di = T.im[r] + Tt.im[r];
+        X.re[r] = di;
+        X.im[r] = -dr;
+    }
+    if (a.deph) {
+        Tile U = load_tile(Ust + k * nn, N, true);
+        Tile Ut;
+        if (!sg.ureuse) {  // eigenvectors at this stage time, warm-started
+            const Tile H = load_tile(mats + a.nk_stride * nn + k * nn, N);  // mat 1: H
+            Ut = load_tile_t(Ust + k * nn, N, true);
+            eig_refine(H, U, Ut, N, buf);
+            store_tile(Ust + k * nn, U, N);
+        } else {
+            Ut = transpose(U, buf);
+        }
+        const Tile G = cmm(Y, true, U);  // y U
+        const double d = col_dot(U, G);  // (U^dag y U)_cc
+        Tile Ud;  // (U diag(d))^T's tile: U^T[4kb + g][c] d_{4kb+g}
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const double dr = __shfl(d, 4 * r + g, 64);
+            Ud.re[r] = Ut.re[r] * dr;
+            Ud.im[r] = Ut.im[r] * dr;
+        }
+        const Tile M = cmm(Ud, false, Ut, true);  // U diag(d) U^dag (B = U^dag: tile conj(U^T))
+        const double g2 = a.gam2, g12 = a.gam1 - a.gam2;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            X.re[r] -= g2 * Y.re[r] + g12 * M.re[r];
+            X.im[r] -= g2 * Y.im[r] + g12 * M.im[r];
+        }
+    }
+    store_tile(kbuf + sg.stage * kstride + k * nn, X, N);
+    if (sg.last) {
+        Tile R = load_tile(rk, N);
+        for (int j = 0; j < sg.nstage; ++j) {
+            if (sg.b[j] == 0.0) continue;
+            const Tile Kj = j == sg.stage ? X : load_tile(kbuf + j * kstride + k * nn, N);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                R.re[r] += sg.b[j] * Kj.re[r];
+                R.im[r] += sg.b[j] * Kj.im[r];
+            }
+        }
+        store_tile(rk, R, N);
+    }
+    (void)c;
+    (void)g;
+}
+
+// ------------------------------------------------------------ sample --
+// Observables at t_n (mats: H, J_x, J_y, J_z at k + A(t_n)); each wave walks
+// k = wave, wave + G, ... and writes one row of partial sums
+// [inter(3) intra(3) pop(N) rho(2 N^2)], as nb_sample_kernel.
+__global__ __launch_bounds__(64) void nb16_sample_kernel(const NbArgs a, const cd *__restrict__ mats,
+                                                         const cd *__restrict__ rho, double *__restrict__ partials,
+                                                         int64_t step_local, cd *__restrict__ Ust, int ureuse)
+{
+    __shared__ cd buf[16 * kTpad];
+    const int N = a.N, nn = N * N;
+    const int c = lane_c(), g = lane_g();
+    const int lane = threadIdx.x;
+    double racc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double cur[6] = {0, 0, 0, 0, 0, 0};
+    double pop = 0.0;  // lane m < N: population of the m-th lowest band
+    for (int64_t k = blockIdx.x; k < a.nk; k += gridDim.x) {
+        const double w = a.weight[k];
+        const Tile R = load_tile(rho + k * nn, N);
+        const Tile H = load_tile(mats + k * nn, N);  // mat 0: H
+        Tile U = load_tile(Ust + k * nn, N, true);
+        if (!ureuse) {  // eigenvectors at k + A(t_n), warm-started from the last ones
+            Tile Ut = load_tile_t(Ust + k * nn, N, true);
+            eig_refine(H, U, Ut, N, buf);
+            store_tile(Ust + k * nn, U, N);
+        }
+        // band energies (for the ascending order) and populations of U's columns
+        const double lam = col_dot(U, cmm(H, true, U));
+        const double p = col_dot(U, cmm(R, true, U));
+        int rank = 0;  // rank of column c among the N energies, ties by index
+        for (int j = 0; j < N; ++j) {
+            const double lj = __shfl(lam, j, 64);
+            rank += (lj < lam) || (lj == lam && j < c);
+        }
+        for (int j = 0; j < N; ++j) {
+            const int rj = __shfl(rank, j, 64);
+            const double pj = __shfl(p, j, 64);
+            if (rj == lane) pop += pj * w;
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            racc[2 * r] += R.re[r] * w;
+            racc[2 * r + 1] += R.im[r] * w;
+        }
+        for (int ax = 0; ax < 3; ++ax) {
+            const Tile J = load_tile(mats + (1 + ax) * a.nk_stride * nn + k * nn, N);  // mats 1..3: J_a
+            double tot = 0.0;  // Re Tr(rho J) = sum_ij Re(rho_ij conj(J_ij)), J Hermitian
+#pragma unroll
+            for (int r = 0; r < 4; ++r) tot += R.re[r] * J.re[r] + R.im[r] * J.im[r];
+            tot = wave_sum(tot);
+            double intra = 0.0;
+            if (a.inter_intra) {
+                const double jd = col_dot(U, cmm(J, true, U));  // (U^dag J U)_cc
+                intra = wave_sum(lane < 16 && c < N ? p * jd : 0.0);
+            }
+            cur[ax] += (tot - intra) * w;
+            cur[3 + ax] += intra * w;
+        }
+    }
+    double *row = partials + (step_local * gridDim.x + blockIdx.x) * static_cast<int64_t>(a.nv);
+    if (lane < 6) {
+        double v = 0.0;
+#pragma unroll
+        for (int q = 0; q < 6; ++q)
+            if (lane == q) v = cur[q];
+        row[lane] = v;
+    }
+    if (lane < N) row[6 + lane] = pop;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int i = g + 4 * r;
+        if (i < N && c < N) {
+            row[6 + N + 2 * (i * N + c)] = racc[2 * r];
+            row[6 + N + 2 * (i * N + c) + 1] = racc[2 * r + 1];
+        }
+    }
+}
+
+}  // namespace
+
+hipError_t nb16_launch_rhs(const NbArgs &a, const NbStage &sg, const double2 *mats, double2 *rho, double2 *kbuf,
+                           double2 *U, hipStream_t s)
+{
+    if (a.N > 16) return hipErrorInvalidValue;
+    nb16_rhs_kernel<<<dim3(static_cast<unsigned>(a.nk)), 64, 0, s>>>(a, sg, mats, rho, kbuf, U);
+    return hipGetLastError();
+}
+
+hipError_t nb16_launch_sample(const NbArgs &a, const double2 *mats, const double2 *rho, double *partials,
+                              int64_t step_local, double2 *U, int ureuse, hipStream_t s)
+{
+    if (a.N > 16) return hipErrorInvalidValue;
+    nb16_sample_kernel<<<dim3(static_cast<unsigned>(a.sample_waves)), 64, 0, s>>>(a, mats, rho, partials, step_local,
+                                                                                 U, ureuse);
+    return hipGetLastError();
+}
+
+}  // namespace sbe

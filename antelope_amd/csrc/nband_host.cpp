@@ -5,6 +5,7 @@
 // (antelope.cpp:297-338 with SBEs.h:596-624).
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -216,6 +217,9 @@ int nb_setup(sbe_ctx *ctx)
     a.nk_stride = p.nk_stride;
     a.nv = p.nv;
     a.sample_waves = p.sample_waves;
+    // N <= 16: stage and sample kernels on MFMA register tiles (ANTELOPE_NB_LDS=1 keeps the
+    // LDS kernels, for comparison)
+    a.tile16 = N <= 16 && !(std::getenv("ANTELOPE_NB_LDS") && std::atoi(std::getenv("ANTELOPE_NB_LDS")) == 1);
     a.kpts = dk;
     a.weight = dk + 3 * p.nk_stride;
     a.rcart = dr;
