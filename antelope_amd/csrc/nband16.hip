@@ -114,22 +114,6 @@ __device__ __forceinline__ Tile load_tile(const cd *M, int N, bool pad_one = fal
     return T;
 }
 
-// the transpose of an N x N row-major matrix -> tile
-__device__ __forceinline__ Tile load_tile_t(const cd *M, int N, bool pad_one = false)
-{
-    const int c = lane_c(), g = lane_g();
-    Tile T;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const int i = g + 4 * r;
-        cd v = make_double2(pad_one && i == c ? 1.0 : 0.0, 0.0);
-        if (i < N && c < N) v = M[c * N + i];
-        T.re[r] = v.x;
-        T.im[r] = v.y;
-    }
-    return T;
-}
-
 __device__ __forceinline__ void store_tile(cd *M, const Tile &T, int N)
 {
     const int c = lane_c(), g = lane_g();
@@ -262,7 +246,7 @@ __device__ void eig_refine(const Tile &H, Tile &U, Tile &Ut, int N, cd *buf)
 // ------------------------------------------------------------- RK stage --
 // Stage i: y = rho + sum_j dt a_ij k_j; k_i = -i(H~ y - (H~ y)^dag)
 // - U (Gamma o (U^dag y U)) U^dag; after the last stage rho += sum_i dt b_i k_i.
-__global__ __launch_bounds__(64) void nb16_rhs_kernel(const NbArgs a, const NbStage sg, const cd *__restrict__ mats,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void nb16_rhs_kernel(const NbArgs a, const NbStage sg, const cd *__restrict__ mats,
                                                       cd *__restrict__ rho, cd *__restrict__ kbuf,
                                                       cd *__restrict__ Ust)
 {
@@ -295,14 +279,11 @@ __global__ __launch_bounds__(64) void nb16_rhs_kernel(const NbArgs a, const NbSt
     }
     if (a.deph) {
         Tile U = load_tile(Ust + k * nn, N, true);
-        Tile Ut;
+        Tile Ut = transpose(U, buf);
         if (!sg.ureuse) {  // eigenvectors at this stage time, warm-started
             const Tile H = load_tile(mats + a.nk_stride * nn + k * nn, N);  // mat 1: H
-            Ut = load_tile_t(Ust + k * nn, N, true);
             eig_refine(H, U, Ut, N, buf);
             store_tile(Ust + k * nn, U, N);
-        } else {
-            Ut = transpose(U, buf);
         }
         const Tile G = cmm(Y, true, U);  // y U
         const double d = col_dot(U, G);  // (U^dag y U)_cc
@@ -343,7 +324,7 @@ __global__ __launch_bounds__(64) void nb16_rhs_kernel(const NbArgs a, const NbSt
 // Observables at t_n (mats: H, J_x, J_y, J_z at k + A(t_n)); each wave walks
 // k = wave, wave + G, ... and writes one row of partial sums
 // [inter(3) intra(3) pop(N) rho(2 N^2)], as nb_sample_kernel.
-__global__ __launch_bounds__(64) void nb16_sample_kernel(const NbArgs a, const cd *__restrict__ mats,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void nb16_sample_kernel(const NbArgs a, const cd *__restrict__ mats,
                                                          const cd *__restrict__ rho, double *__restrict__ partials,
                                                          int64_t step_local, cd *__restrict__ Ust, int ureuse)
 {
@@ -360,7 +341,7 @@ __global__ __launch_bounds__(64) void nb16_sample_kernel(const NbArgs a, const c
         const Tile H = load_tile(mats + k * nn, N);  // mat 0: H
         Tile U = load_tile(Ust + k * nn, N, true);
         if (!ureuse) {  // eigenvectors at k + A(t_n), warm-started from the last ones
-            Tile Ut = load_tile_t(Ust + k * nn, N, true);
+            Tile Ut = transpose(U, buf);
             eig_refine(H, U, Ut, N, buf);
             store_tile(Ust + k * nn, U, N);
         }
