@@ -1018,6 +1018,8 @@ hipError_t nb_launch_vel_init(const NbArgs &a, const double2 *mats, double2 *P, 
 hipError_t nb_launch_vel_step(const NbArgs &a, const NbVelStep &v, const double2 *P, const double *Eocc, double2 *rho,
                               double *partials, int64_t step_local, bool p_lds, hipStream_t s)
 {
+    if (a.tile16 && (v.nstage == 4 || v.nstage == 6))
+        return nb16_launch_vel_step(a, v, P, Eocc, rho, partials, step_local, s);
     const dim3 grid(static_cast<unsigned>(a.sample_waves));
     const size_t lds = nb_vel_lds_bytes(a.N, v.nstage, p_lds);
     const int nn = a.N * a.N;

@@ -31,6 +31,11 @@
 //    product y U, a column reduction and one product U diag(d) U^dag.
 //  * Currents: Tr(rho J_a) elementwise; the intraband part sum_i
 //    (U^dag rho U)_ii (U^dag J_a U)_ii from the products rho U and J_a U.
+//    J_a (and pMatrix_a below) is read transposed and used as a general
+//    matrix: some model classes' current matrices are not Hermitian.
+//  * VelocityHamiltonian gauge (band basis): one product H y per stage with
+//    H = sum_a A_a P_a + diag(E) formed elementwise, the stage derivatives
+//    kept in registers.
 #include <cmath>
 
 #include "nband.hpp"
@@ -108,6 +113,22 @@ __device__ __forceinline__ Tile load_tile(const cd *M, int N, bool pad_one = fal
         const int i = g + 4 * r;
         cd v = make_double2(pad_one && i == c ? 1.0 : 0.0, 0.0);
         if (i < N && c < N) v = M[i * N + c];
+        T.re[r] = v.x;
+        T.im[r] = v.y;
+    }
+    return T;
+}
+
+// the transpose of an N x N row-major matrix -> tile (zero padding)
+__device__ __forceinline__ Tile load_tile_t(const cd *M, int N)
+{
+    const int c = lane_c(), g = lane_g();
+    Tile T;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int i = g + 4 * r;
+        cd v = make_double2(0.0, 0.0);
+        if (i < N && c < N) v = M[c * N + i];
         T.re[r] = v.x;
         T.im[r] = v.y;
     }
@@ -364,14 +385,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void nb
             racc[2 * r + 1] += R.im[r] * w;
         }
         for (int ax = 0; ax < 3; ++ax) {
-            const Tile J = load_tile(mats + (1 + ax) * a.nk_stride * nn + k * nn, N);  // mats 1..3: J_a
-            double tot = 0.0;  // Re Tr(rho J) = sum_ij Re(rho_ij conj(J_ij)), J Hermitian
+            // mats 1..3: J_a, read transposed: some model classes' J is not Hermitian
+            const Tile Jt = load_tile_t(mats + (1 + ax) * a.nk_stride * nn + k * nn, N);
+            double tot = 0.0;  // Re Tr(rho J) = sum_ij Re(rho_ij J_ji)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) tot += R.re[r] * J.re[r] + R.im[r] * J.im[r];
+            for (int r = 0; r < 4; ++r) tot += R.re[r] * Jt.re[r] - R.im[r] * Jt.im[r];
             tot = wave_sum(tot);
             double intra = 0.0;
             if (a.inter_intra) {
-                const double jd = col_dot(U, cmm(J, true, U));  // (U^dag J U)_cc
+                const double jd = col_dot(U, cmm(Jt, false, U));  // Re (U^dag J U)_cc
                 intra = wave_sum(lane < 16 && c < N ? p * jd : 0.0);
             }
             cur[ax] += (tot - intra) * w;
@@ -397,7 +419,135 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void nb
     }
 }
 
+// ------------------------------------------------- velocity gauge ----
+// One time step of the VelocityHamiltonian gauge (SBEs.h:762-861 with the
+// band-basis H = sum_a A_a(t) P_a + diag(E)), as nb_vel_step_kernel: the
+// sample at t_n, S stages with the derivatives in registers, the update.
+// Each wave walks k = wave, wave + G, ... and writes one row of partial sums.
+template <int S>
+__global__ __launch_bounds__(64) void nb16_vel_step_kernel(const NbArgs a, const NbVelStep v,
+                                                           const cd *__restrict__ P, const double *__restrict__ Eocc,
+                                                           cd *__restrict__ rho, double *__restrict__ partials,
+                                                           int64_t step_local)
+{
+    __shared__ cd buf[16 * kTpad];
+    const int N = a.N, nn = N * N;
+    const int c = lane_c(), g = lane_g();
+    const int lane = threadIdx.x;
+    const int64_t ks = a.nk_stride * nn;
+    double racc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double cur[6] = {0, 0, 0, 0, 0, 0};
+    double pop = 0.0;
+    for (int64_t k = blockIdx.x; k < a.nk; k += gridDim.x) {
+        const double w = a.weight[k];
+        const Tile R = load_tile(rho + k * nn, N);
+        // pMatrix_a read transposed: not Hermitian for the model classes whose J is not
+        const Tile Px = load_tile_t(P + k * nn, N), Py = load_tile_t(P + ks + k * nn, N),
+                   Pz = load_tile_t(P + 2 * ks + k * nn, N);
+        const double Ec = c < N ? Eocc[k * 2 * N + c] : 0.0;       // E of band c
+        const double oc = c < N ? Eocc[k * 2 * N + N + c] : 0.0;   // occupation of band c
+        // ---- sample at t_n: band populations, currents Tr(rho P_a) split
+        // into the diagonal (intraband) and off-diagonal parts, rho sums
+        double lr[4], lc;
+        diag_bcast(R, lr, lc);
+        if (lane < N) pop += lc * w;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            racc[2 * r] += R.re[r] * w;
+            racc[2 * r + 1] += R.im[r] * w;
+        }
+        for (int ax = 0; ax < 3; ++ax) {
+            const Tile &Pa = ax == 0 ? Px : (ax == 1 ? Py : Pz);
+            double inter = 0.0, intra = 0.0;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const double val = R.re[r] * Pa.re[r] - R.im[r] * Pa.im[r];  // Re(rho_ij P_ji)
+                if (g + 4 * r == c && a.inter_intra) intra += val;
+                else inter += val;
+            }
+            cur[ax] += wave_sum(inter) * w;
+            cur[3 + ax] += wave_sum(intra) * w;
+        }
+        // ---- RK stages
+        Tile Kd[S];
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            Tile Y = R;
+#pragma unroll
+            for (int j = 0; j < s; ++j) {
+                if (v.a[s][j] == 0.0) continue;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    Y.re[r] += v.a[s][j] * Kd[j].re[r];
+                    Y.im[r] += v.a[s][j] * Kd[j].im[r];
+                }
+            }
+            const double A0 = v.A[s][0], A1 = v.A[s][1], A2 = v.A[s][2];
+            Tile M;  // H^T = sum_a A_a P_a^T + diag(E)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                M.re[r] = A0 * Px.re[r] + A1 * Py.re[r] + A2 * Pz.re[r] + (g + 4 * r == c ? Ec : 0.0);
+                M.im[r] = A0 * Px.im[r] + A1 * Py.im[r] + A2 * Pz.im[r];
+            }
+            const Tile T = cmm(M, false, Y);  // H y
+            const Tile Tt = transpose(T, buf);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const double dr = T.re[r] - Tt.re[r], di = T.im[r] + Tt.im[r];
+                double xr = di, xi = -dr;  // -i (H y - (H y)^dag)
+                if (a.deph) {
+                    const bool dg = g + 4 * r == c;
+                    const double gm = dg ? a.gam1 : a.gam2;
+                    xr -= gm * (Y.re[r] - (dg ? oc : 0.0));
+                    xi -= gm * Y.im[r];
+                }
+                Kd[s].re[r] = xr;
+                Kd[s].im[r] = xi;
+            }
+        }
+        Tile Rn = R;
+#pragma unroll
+        for (int j = 0; j < S; ++j) {
+            if (v.b[j] == 0.0) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                Rn.re[r] += v.b[j] * Kd[j].re[r];
+                Rn.im[r] += v.b[j] * Kd[j].im[r];
+            }
+        }
+        store_tile(rho + k * nn, Rn, N);
+    }
+    double *row = partials + (step_local * gridDim.x + blockIdx.x) * static_cast<int64_t>(a.nv);
+    if (lane < 6) {
+        double vv = 0.0;
+#pragma unroll
+        for (int q = 0; q < 6; ++q)
+            if (lane == q) vv = cur[q];
+        row[lane] = vv;
+    }
+    if (lane < N) row[6 + lane] = pop;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int i = g + 4 * r;
+        if (i < N && c < N) {
+            row[6 + N + 2 * (i * N + c)] = racc[2 * r];
+            row[6 + N + 2 * (i * N + c) + 1] = racc[2 * r + 1];
+        }
+    }
+}
+
 }  // namespace
+
+hipError_t nb16_launch_vel_step(const NbArgs &a, const NbVelStep &v, const double2 *P, const double *Eocc,
+                                double2 *rho, double *partials, int64_t step_local, hipStream_t s)
+{
+    if (a.N > 16) return hipErrorInvalidValue;
+    const dim3 grid(static_cast<unsigned>(a.sample_waves));
+    if (v.nstage == 4) nb16_vel_step_kernel<4><<<grid, 64, 0, s>>>(a, v, P, Eocc, rho, partials, step_local);
+    else if (v.nstage == 6) nb16_vel_step_kernel<6><<<grid, 64, 0, s>>>(a, v, P, Eocc, rho, partials, step_local);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
 
 hipError_t nb16_launch_rhs(const NbArgs &a, const NbStage &sg, const double2 *mats, double2 *rho, double2 *kbuf,
                            double2 *U, hipStream_t s)
