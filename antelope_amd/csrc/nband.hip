@@ -12,9 +12,12 @@
 //    and B = w_R e^{iA(t).R} X_R (X = H + E.r, H, or -i R_a H), formed on
 //    the fly while staging through LDS. t and E(t) come from a device table
 //    indexed by (step, stage), so launch arguments do not carry time.
-//  * nb_rhs_kernel / nb_sample_kernel: one wave per k point; matrices live in
-//    LDS; the Hermitian eigenproblem is solved by a parallel-order complex
-//    Jacobi method (N/2 disjoint rotations per round).
+//  * nb_rhs_kernel / nb_sample_kernel (N > 16; nband16.hip has the N <= 16
+//    forms): one wave per k point; matrices live in LDS; the eigenvectors are
+//    refined from the stored ones by simultaneous rotations written as MFMA
+//    products (weigh_warm). Cold starts (initial state, velocity-gauge set-up,
+//    band export) use a parallel-order complex Jacobi method (N/2 disjoint
+//    rotations per round).
 #include <cmath>
 
 #include "nband.hpp"
@@ -501,6 +504,95 @@ __device__ void weigh(cd *A, cd *U, double *ev, cd *work, int N, double4 *rot, b
     wcopy(U, work, nn);
 }
 
+// Warm-started eigenvectors for the LDS kernels: the simultaneous-rotation
+// refinement of nband16.hip's eig_refine on LDS matrices and wmm products.
+// In: A = H (overwritten), U = eigenvectors of a nearby H. Out: U refined,
+// columns in the warm start's order. W, T, X: scratch. Pointers are swapped
+// among the five slots; Hsrc reloads H on the rare path that needs it.
+__device__ void weigh_warm(cd *&A, cd *&U, cd *&W, cd *&T, cd *&X, int N, double *lam, const cd *Hsrc)
+{
+    const int lane = threadIdx.x;
+    const int nn = N * N;
+    wmm(T, A, U, N, false, false);
+    wmm(W, U, T, N, true, false);  // W = U^dag H U
+    auto newton_schulz = [&]() {  // U <- U (3I - U^dag U) / 2
+        wmm(T, U, U, N, true, false);
+        for (int e = lane; e < nn; e += 64) {
+            const int i = e / N, j = e % N;
+            T[e] = cmk((i == j ? 1.5 : 0.0) - 0.5 * T[e].x, -0.5 * T[e].y);
+        }
+        wave_sync();
+        wmm(X, U, T, N, false, false);
+        cd *t = U;
+        U = X;
+        X = t;
+    };
+    for (int it = 0; it < 12; ++it) {
+        if (lane < N) lam[lane] = W[lane * N + lane].x;
+        wave_sync();
+        double kmax = 0.0;  // K = rotation generators, into A
+        for (int e = lane; e < nn; e += 64) {
+            const int i = e / N, j = e % N;
+            const cd w = W[e];
+            const double w2 = w.x * w.x + w.y * w.y;
+            cd kk = cmk(0.0, 0.0);
+            if (i != j && w2 > 0.0) {
+                const double D = lam[j] - lam[i];
+                const double s = (D > 0.0 || (D == 0.0 && i < j)) ? 1.0 : -1.0;
+                const double den = 0.5 * D + s * sqrt(0.25 * D * D + w2);
+                kk = cmk(w.x / den, w.y / den);
+                kmax = fmax(kmax, sqrt(kk.x * kk.x + kk.y * kk.y));
+            }
+            A[e] = kk;
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) kmax = fmax(kmax, __shfl_xor(kmax, o, 64));
+        wave_sync();
+        if (kmax == 0.0) break;
+        const bool small = kmax < 1e-8;
+        if (!small) wmm(T, A, A, N, false, false);  // K^2
+        for (int e = lane; e < nn; e += 64) {  // V = I + K (+ K^2 / 2)
+            const int i = e / N, j = e % N;
+            cd v = A[e];
+            if (!small) v = cadd(v, cscale(T[e], 0.5));
+            if (i == j) v.x += 1.0;
+            A[e] = v;
+        }
+        wave_sync();
+        wmm(T, U, A, N, false, false);  // U V
+        {
+            cd *t = U;
+            U = T;
+            T = t;
+        }
+        if (small) break;
+        if (kmax > 1e-3) {  // V far from unitary: restore U, then W from H itself
+            newton_schulz();
+            for (int e = lane; e < nn; e += 64) A[e] = Hsrc[e];
+            wave_sync();
+            wmm(T, A, U, N, false, false);
+            wmm(W, U, T, N, true, false);
+        } else {
+            wmm(T, W, A, N, false, false);  // W V
+            wmm(X, A, T, N, true, false);   // V^dag W V
+            cd *t = W;
+            W = X;
+            X = t;
+        }
+    }
+    newton_schulz();
+}
+
+// d[c] = Re (U^dag M)_cc = Re sum_r conj(U_rc) M_rc for c < N (lanes c)
+__device__ __forceinline__ double col_dot_lds(const cd *U, const cd *M, int N)
+{
+    const int c = threadIdx.x;
+    double s = 0.0;
+    if (c < N)
+        for (int r = 0; r < N; ++r) s += U[r * N + c].x * M[r * N + c].x + U[r * N + c].y * M[r * N + c].y;
+    return s;
+}
+
 // LDS layout of one wave: kMats matrices of N*N complex, N eigenvalues,
 // 32 rotation slots + 32 int pair slots.
 __device__ __forceinline__ cd *mat_slot(cd *base, int i, int nn) { return base + static_cast<int64_t>(i) * nn; }
@@ -564,7 +656,6 @@ __global__ __launch_bounds__(64) void nb_sample_kernel(const NbArgs a, const cd 
     cd *A = mat_slot(base, 0, nn), *U = mat_slot(base, 1, nn), *W = mat_slot(base, 2, nn);
     cd *R = mat_slot(base, 3, nn), *T = mat_slot(base, 4, nn), *X = mat_slot(base, 5, nn);
     double *ev = reinterpret_cast<double *>(mat_slot(base, 6, nn));
-    double4 *rot = reinterpret_cast<double4 *>(ev + 64);
     const int lane = threadIdx.x;
     const int nv = a.nv;
     // per-lane accumulators: rho elements e = lane + 64 t
@@ -572,48 +663,56 @@ __global__ __launch_bounds__(64) void nb_sample_kernel(const NbArgs a, const cd 
     double racc[2 * kMaxPer];
     for (int t = 0; t < 2 * kMaxPer; ++t) racc[t] = 0.0;
     double cur[6] = {0, 0, 0, 0, 0, 0};
-    double pop = 0.0;  // lane m < N: population of band m
+    double pop = 0.0;  // lane m < N: population of the m-th lowest band
     const cd *H0 = mats;                              // mat 0: H0
     const cd *J = mats + a.nk_stride * nn;            // mats 1..3: J_x, J_y, J_z
     for (int64_t k = blockIdx.x; k < a.nk; k += gridDim.x) {
         const double w = a.weight[k];
         wcopy(R, rho + k * nn, nn);
         wcopy(U, Ust + k * nn, nn);
+        wcopy(A, H0 + k * nn, nn);
         if (!ureuse) {  // eigenvectors at k + A(t_n), warm-started from the last ones
-            wcopy(A, H0 + k * nn, nn);
-            weigh(A, U, ev, W, N, rot, true);
+            weigh_warm(A, U, W, T, X, N, ev, H0 + k * nn);
             for (int e = lane; e < nn; e += 64) Ust[k * nn + e] = U[e];
+            wcopy(A, H0 + k * nn, nn);
         }
+        // band energies (ascending order) and populations of U's columns
+        wmm(T, A, U, N, false, false);
+        const double lam = col_dot_lds(U, T, N);
         wmm(T, R, U, N, false, false);
-        wmm(X, U, T, N, true, false);  // rho_H
-        if (lane < N) pop += X[lane * N + lane].x * w;
+        const double p = col_dot_lds(U, T, N);
+        int rank = 0;
+        for (int j = 0; j < N; ++j) {
+            const double lj = __shfl(lam, j, 64);
+            rank += (lj < lam) || (lj == lam && j < lane);
+        }
+        for (int j = 0; j < N; ++j) {
+            const int rj = __shfl(rank, j, 64);
+            const double pj = __shfl(p, j, 64);
+            if (rj == lane) pop += pj * w;
+        }
         for (int e = lane, t = 0; e < nn; e += 64, ++t) {
             racc[2 * t] += R[e].x * w;
             racc[2 * t + 1] += R[e].y * w;
         }
         for (int ax = 0; ax < 3; ++ax) {
             const cd *Ja = J + static_cast<int64_t>(ax) * a.nk_stride * nn + k * nn;
-            double inter = 0.0, intra = 0.0;
-            if (a.inter_intra) {
+            double tot = 0.0;  // Re Tr(rho J) = sum_ij Re(rho_ij J_ji)
+            for (int e = lane; e < nn; e += 64) {
+                const int i = e / N, j = e % N;
+                const cd r = R[e], jj = Ja[j * N + i];
+                tot += r.x * jj.x - r.y * jj.y;
+            }
+            tot = wave_sum(tot);
+            double intra = 0.0;
+            if (a.inter_intra) {  // sum_c (U^dag rho U)_cc Re (U^dag J U)_cc
                 wcopy(A, Ja, nn);
                 wmm(T, A, U, N, false, false);
-                wmm(W, U, T, N, true, false);  // J_H
-                for (int e = lane; e < nn; e += 64) {
-                    const int i = e / N, j = e % N;
-                    const cd r = X[i * N + j], jj = W[j * N + i];
-                    const double v = r.x * jj.x - r.y * jj.y;
-                    if (i == j) intra += v;
-                    else inter += v;
-                }
-            } else {
-                for (int e = lane; e < nn; e += 64) {
-                    const int i = e / N, j = e % N;
-                    const cd r = R[i * N + j], jj = Ja[j * N + i];
-                    inter += r.x * jj.x - r.y * jj.y;
-                }
+                const double jd = col_dot_lds(U, T, N);
+                intra = wave_sum(lane < N ? p * jd : 0.0);
             }
-            cur[ax] += wave_sum(inter) * w;
-            cur[3 + ax] += wave_sum(intra) * w;
+            cur[ax] += (tot - intra) * w;
+            cur[3 + ax] += intra * w;
             wave_sync();
         }
     }
@@ -629,7 +728,8 @@ __global__ __launch_bounds__(64) void nb_sample_kernel(const NbArgs a, const cd 
 // ------------------------------------------------------------- RK stage --
 // Stage i of the step: y = rho + dt sum_j a_ij k_j; k_i = f(y) with
 // f(y) = -i(H~ y - (H~ y)^dag) - U (Gamma o (U^dag y U)) U^dag; after the
-// last stage rho += dt sum_i b_i k_i (SBEs.h:596-624, 762-861).
+// last stage rho += dt sum_i b_i k_i (SBEs.h:596-624, 762-861). The
+// dephasing is formed as g2 y + (g1 - g2) U diag(U^dag y U) U^dag.
 __global__ __launch_bounds__(64) void nb_rhs_kernel(const NbArgs a, const NbStage sg, const cd *__restrict__ mats,
                                                     cd *__restrict__ rho, cd *__restrict__ kbuf, cd *__restrict__ Ust)
 {
@@ -639,7 +739,6 @@ __global__ __launch_bounds__(64) void nb_rhs_kernel(const NbArgs a, const NbStag
     cd *A = mat_slot(base, 0, nn), *U = mat_slot(base, 1, nn), *W = mat_slot(base, 2, nn);
     cd *Y = mat_slot(base, 3, nn), *T = mat_slot(base, 4, nn), *X = mat_slot(base, 5, nn);
     double *ev = reinterpret_cast<double *>(mat_slot(base, 6, nn));
-    double4 *rot = reinterpret_cast<double4 *>(ev + 64);
     const int64_t k = blockIdx.x;
     if (k >= a.nk) return;
     const int lane = threadIdx.x;
@@ -652,37 +751,39 @@ __global__ __launch_bounds__(64) void nb_rhs_kernel(const NbArgs a, const NbStag
             if (sg.a[j] != 0.0) y = cadd(y, cscale(kbuf[j * kstride + k * nn + e], sg.a[j]));
         Y[e] = y;
     }
-    const cd *Ht = mats + k * nn;                 // mat 0: H~ = H + E.D
-    wcopy(A, Ht, nn);
-    wmm(T, A, Y, N, false, false);                // T = H~ y
-    cd *ko = kbuf + sg.stage * kstride + k * nn;
-    // -i (T - T^dag): (-i)(t_ij - conj(t_ji))
-    for (int e = lane; e < nn; e += 64) {
-        const int i = e / N, j = e % N;
-        const cd d = csub(T[i * N + j], cconj(T[j * N + i]));
-        X[e] = cmk(d.y, -d.x);
-    }
     wave_sync();
     if (a.deph) {
         wcopy(U, Ust + k * nn, nn);
         if (!sg.ureuse) {  // eigenvectors at this stage time, warm-started
             const cd *H0 = mats + a.nk_stride * nn + k * nn;  // mat 1: H
             wcopy(A, H0, nn);
-            weigh(A, U, ev, W, N, rot, true);
+            weigh_warm(A, U, W, T, X, N, ev, H0);
             for (int e = lane; e < nn; e += 64) Ust[k * nn + e] = U[e];
         }
-        wmm(T, Y, U, N, false, false);
-        wmm(W, U, T, N, true, false);             // U^dag y U
-        for (int e = lane; e < nn; e += 64) {
-            const int i = e / N, j = e % N;
-            W[e] = cscale(W[e], i == j ? a.gam1 : a.gam2);
-        }
+        wmm(T, Y, U, N, false, false);  // y U
+        const double d = col_dot_lds(U, T, N);
+        if (lane < N) ev[lane] = d;  // (U^dag y U)_cc
         wave_sync();
-        wmm(T, U, W, N, false, false);
-        wmm(A, T, U, N, false, true);             // U (G o ...) U^dag
-        for (int e = lane; e < nn; e += 64) X[e] = csub(X[e], A[e]);
+        for (int e = lane; e < nn; e += 64) T[e] = cscale(U[e], ev[e % N]);  // U diag(d)
+        wave_sync();
+        wmm(A, T, U, N, false, true);  // U diag(d) U^dag
+        for (int e = lane; e < nn; e += 64)
+            A[e] = cadd(cscale(Y[e], a.gam2), cscale(A[e], a.gam1 - a.gam2));  // the dephasing term
         wave_sync();
     }
+    const cd *Ht = mats + k * nn;                 // mat 0: H~ = H + E.D
+    wcopy(W, Ht, nn);
+    wmm(T, W, Y, N, false, false);                // T = H~ y
+    cd *ko = kbuf + sg.stage * kstride + k * nn;
+    // -i (T - T^dag): (-i)(t_ij - conj(t_ji))
+    for (int e = lane; e < nn; e += 64) {
+        const int i = e / N, j = e % N;
+        const cd d = csub(T[i * N + j], cconj(T[j * N + i]));
+        cd x = cmk(d.y, -d.x);
+        if (a.deph) x = csub(x, A[e]);
+        X[e] = x;
+    }
+    wave_sync();
     for (int e = lane; e < nn; e += 64) ko[e] = X[e];
     if (sg.last) {
         for (int e = lane; e < nn; e += 64) {

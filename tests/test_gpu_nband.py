@@ -43,6 +43,7 @@ def _compare(gpu, ref, label):
     ("n6_odd_no_dephasing", 6, dict(T2=-1.0)),
     ("n8_total_current", 8, dict(inter_intra=False)),
     ("n16_short", 16, dict(npts=(2, 2, 2), ncycles=0.25)),
+    ("n32_short", 32, dict(npts=(2, 2, 1), ncycles=0.25)),
 ])
 def test_wannier_parity(oracle, label, N, kw):
     model = synthetic_wannier(N, nmax=(1, 1, 1), seed=7 + N)
