@@ -73,7 +73,7 @@ __global__ void nb_phase_kernel(const NbArgs a, cd *__restrict__ P)
 }
 
 // --------------------------------------------------------- Fourier GEMM --
-constexpr int GM = 64, GN = 64, GK = 8;
+constexpr int GM = 64, GN = 64, GK = 32;
 
 // B element (row r, column c) of the evaluation: c = mat * N^2 + ij
 __device__ __forceinline__ cd b_element(const NbArgs &a, const NbEval &ev, int r, int64_t c, cd phase)
@@ -101,19 +101,41 @@ __device__ __forceinline__ cd b_element(const NbArgs &a, const NbEval &ev, int r
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
+// B of one evaluation, formed once (instead of once per k tile):
+// b64[r][c] = w_R e^{iA(t).R} X_R[c] for r < nr, c < ncols; zero padding to
+// nr_pad rows and bcols columns.
+__global__ void nb_bform_kernel(const NbArgs a, const NbEval ev, cd *__restrict__ b64)
+{
+    const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= static_cast<int64_t>(a.nr_pad) * a.bcols) return;
+    const int r = static_cast<int>(idx / a.bcols);
+    const int64_t c = idx % a.bcols;
+    const int64_t ncols = static_cast<int64_t>(ev.nmat) * a.N * a.N;
+    cd v = cmk(0.0, 0.0);
+    if (r < a.nr && c < ncols) {
+        const double *R = a.rcart + 3 * r;
+        double sn, cs;
+        sincos(ev.A[0] * R[0] + ev.A[1] * R[1] + ev.A[2] * R[2], &sn, &cs);
+        v = b_element(a, ev, r, c, cmk(cs, sn));
+    }
+    b64[idx] = v;
+}
+
 // 64 k x 64 column tile per block of 4 waves; wave w owns the 32 x 32
-// sub-tile (w / 2, w % 2) as 2 x 2 MFMA tiles of 16 x 16. The complex
-// product is four real v_mfma_f64_16x16x4f64 per tile and K step:
+// sub-tile (w / 2, w % 2) as 2 x 2 MFMA tiles of 16 x 16. K (lattice
+// vectors) in slices of 32 staged through LDS: e^{ik.R} from P (rows of 32
+// consecutive R per k, row pitch padded against bank conflicts) and B from
+// the evaluation's b64. The complex product is four real
+// v_mfma_f64_16x16x4f64 per tile and K step:
 //   re += a_re b_re - a_im b_im,  im += a_re b_im + a_im b_re.
 // MFMA operand layout (16x16x4, FP64): A row = lane % 16, k = lane / 16;
 // B k = lane / 16, column = lane % 16; D element r of lane l is
 // row (l / 16) + 4 r, column l % 16.
 __global__ __launch_bounds__(256) void nb_fourier_kernel(const NbArgs a, const NbEval ev, const cd *__restrict__ P,
-                                                         cd *__restrict__ out)
+                                                         const cd *__restrict__ b64, cd *__restrict__ out)
 {
-    __shared__ cd As[GK][GM];
+    __shared__ cd As[GK][GM + 1];
     __shared__ cd Bs[GK][GN];
-    __shared__ cd ph[GK];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave >> 1, wn = wave & 1;
     const int64_t k0 = static_cast<int64_t>(blockIdx.x) * GM;
@@ -129,19 +151,7 @@ __global__ __launch_bounds__(256) void nb_fourier_kernel(const NbArgs a, const N
             acci[i][j] = d4{0.0, 0.0, 0.0, 0.0};
         }
 
-    for (int r0 = 0; r0 < a.nr; r0 += GK) {
-        // e^{i A(t).R} for this slice
-        if (threadIdx.x < GK) {
-            const int r = r0 + threadIdx.x;
-            cd v = cmk(0.0, 0.0);
-            if (r < a.nr) {
-                const double *R = a.rcart + 3 * r;
-                double s, c;
-                sincos(ev.A[0] * R[0] + ev.A[1] * R[1] + ev.A[2] * R[2], &s, &c);
-                v = cmk(c, s);
-            }
-            ph[threadIdx.x] = v;
-        }
+    for (int r0 = 0; r0 < a.nr_pad; r0 += GK) {
         // A tile: P[k0 + m][r0 + kk]
         for (int t = threadIdx.x; t < GM * GK; t += 256) {
             const int m = t / GK, kk = t % GK;
@@ -149,12 +159,10 @@ __global__ __launch_bounds__(256) void nb_fourier_kernel(const NbArgs a, const N
             const int r = r0 + kk;
             As[kk][m] = (k < a.nk_stride && r < a.nr) ? P[k * a.nr + r] : cmk(0.0, 0.0);
         }
-        __syncthreads();
+        // B tile: b64[r0 + kk][c0 + n]
         for (int t = threadIdx.x; t < GK * GN; t += 256) {
             const int kk = t / GN, n = t % GN;
-            const int r = r0 + kk;
-            const int64_t c = c0 + n;
-            Bs[kk][n] = (r < a.nr && c < ncols) ? b_element(a, ev, r, c, ph[kk]) : cmk(0.0, 0.0);
+            Bs[kk][n] = b64[static_cast<int64_t>(r0 + kk) * a.bcols + c0 + n];
         }
         __syncthreads();
 #pragma unroll
@@ -1075,8 +1083,10 @@ hipError_t nb_launch_fourier(const NbArgs &a, const NbEval &ev, const double2 *P
         nb_fourier16_kernel<<<grid, 256, 0, s>>>(a, ev, sc, a.p16, a.b16, out);
         return hipGetLastError();
     }
+    const int64_t nb = static_cast<int64_t>(a.nr_pad) * a.bcols;
+    nb_bform_kernel<<<dim3(static_cast<unsigned>((nb + 255) / 256)), 256, 0, s>>>(a, ev, a.b64);
     const dim3 grid(static_cast<unsigned>((a.nk_stride + GM - 1) / GM), static_cast<unsigned>((ncols + GN - 1) / GN));
-    nb_fourier_kernel<<<grid, 256, 0, s>>>(a, ev, P, out);
+    nb_fourier_kernel<<<grid, 256, 0, s>>>(a, ev, P, a.b64, out);
     return hipGetLastError();
 }
 

@@ -36,6 +36,7 @@ struct NbArgs {
     int64_t bcols;         // columns of the B planes (4 N^2 rounded up to 64)
     const uint16_t *p16;   // e^{ik.R}: planes [re_hi, re_lo, im_hi, im_lo][nk_stride][nr_pad]
     uint16_t *b16;         // B of one evaluation: planes [re_hi, re_lo, im_hi, im_lo][bcols][nr_pad]
+    double2 *b64;          // FP64 form: B of one evaluation [nr_pad][bcols]
     double hmax, pmax, rmax;  // max |H(R)|, max |r(R)|, max |R_a| (scales of B)
     int velocity;          // VelocityHamiltonian gauge (band basis at k)
     // analytic tight-binding class (SBE_MODEL_WILSON_MASS ... SBE_MODEL_TMDC):

@@ -54,7 +54,7 @@ struct NbPlan {
     int64_t chunk;
     int nr_pad;
     int64_t bcols;
-    size_t bytes_P, bytes_rho, bytes_mats, bytes_k, bytes_tab, bytes_partials, bytes_series, bytes_split;
+    size_t bytes_P, bytes_rho, bytes_mats, bytes_k, bytes_tab, bytes_partials, bytes_series, bytes_split, bytes_b64;
 };
 
 NbPlan nb_plan(int N, int nr, int64_t nk, int64_t nt, int nstage, bool split16)
@@ -80,6 +80,8 @@ NbPlan nb_plan(int N, int nr, int64_t nk, int64_t nt, int nstage, bool split16)
     p.bytes_split = split16 ? 4 * sizeof(uint16_t) * static_cast<size_t>(p.nr_pad) *
                                   static_cast<size_t>(p.nk_stride + p.bcols)
                             : 0;
+    // FP64 form: B of one evaluation, formed once per evaluation (Wannier models only)
+    p.bytes_b64 = !split16 && nr > 0 ? sizeof(double2) * static_cast<size_t>(p.nr_pad) * p.bcols : 0;
     return p;
 }
 
@@ -89,7 +91,7 @@ size_t nb_footprint(const sbe_model &m, int64_t nk, int64_t nt, int nstage, int 
 {
     const NbPlan p = nb_plan(m.nband, m.nrpts, nk, nt, nstage, precision == SBE_PREC_SPLIT16);
     return p.bytes_P + p.bytes_rho + p.bytes_mats + p.bytes_k + p.bytes_tab + p.bytes_partials + p.bytes_series +
-           p.bytes_split;
+           p.bytes_split + p.bytes_b64;
 }
 
 int nb_validate(const sbe_model &m, std::string &err)
@@ -177,6 +179,7 @@ int nb_setup(sbe_ctx *ctx)
     ctx->d_partials = nalloc<double>(ctx, p.bytes_partials, rc);
     ctx->d_series = nalloc<double>(ctx, p.bytes_series, rc);
     uint16_t *split = split16 ? nalloc<uint16_t>(ctx, p.bytes_split, rc) : nullptr;
+    double2 *b64 = p.bytes_b64 ? nalloc<double2>(ctx, p.bytes_b64, rc) : nullptr;
     if (rc) return rc;
 
     // host copies of the tables: k [nk_stride][3], w, R cartesian, weights, H, r
@@ -231,6 +234,7 @@ int nb_setup(sbe_ctx *ctx)
     a.bcols = p.bcols;
     a.p16 = split;
     a.b16 = split ? split + 4 * static_cast<size_t>(p.nr_pad) * p.nk_stride : nullptr;
+    a.b64 = b64;
     a.hmax = a.pmax = a.rmax = 0.0;
     for (size_t e = 0; e < static_cast<size_t>(nr) * nn; ++e)
         a.hmax = std::max(a.hmax, std::hypot(m.ham[2 * e], m.ham[2 * e + 1]));
