@@ -514,13 +514,27 @@ __device__ void weigh(cd *A, cd *U, double *ev, cd *work, int N, double4 *rot, b
 
 // Warm-started eigenvectors for the LDS kernels: the simultaneous-rotation
 // refinement of nband16.hip's eig_refine on LDS matrices and wmm products.
-// In: A = H (overwritten), U = eigenvectors of a nearby H. Out: U refined,
+// In: A = H (overwritten; its Hermitian part is used), U = eigenvectors of a
+// nearby H. Out: U refined,
 // columns in the warm start's order. W, T, X: scratch. Pointers are swapped
 // among the five slots; Hsrc reloads H on the rare path that needs it.
 __device__ void weigh_warm(cd *&A, cd *&U, cd *&W, cd *&T, cd *&X, int N, double *lam, const cd *Hsrc)
 {
     const int lane = threadIdx.x;
     const int nn = N * N;
+    // the Hermitian part of H (as zheevr, which reads one triangle)
+    auto hermitize = [&]() {
+        for (int e = lane; e < nn; e += 64) {
+            const int i = e / N, j = e % N;
+            const cd x = A[e], y = A[j * N + i];
+            X[e] = cmk(0.5 * (x.x + y.x), 0.5 * (x.y - y.y));
+        }
+        wave_sync();
+        cd *t = A;
+        A = X;
+        X = t;
+    };
+    hermitize();
     wmm(T, A, U, N, false, false);
     wmm(W, U, T, N, true, false);  // W = U^dag H U
     auto newton_schulz = [&]() {  // U <- U (3I - U^dag U) / 2
@@ -578,6 +592,7 @@ __device__ void weigh_warm(cd *&A, cd *&U, cd *&W, cd *&T, cd *&X, int N, double
             newton_schulz();
             for (int e = lane; e < nn; e += 64) A[e] = Hsrc[e];
             wave_sync();
+            hermitize();
             wmm(T, A, U, N, false, false);
             wmm(W, U, T, N, true, false);
         } else {

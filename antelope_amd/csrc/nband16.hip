@@ -201,11 +201,21 @@ __device__ __forceinline__ void orthonormalize(Tile &U, Tile &Ut, cd *buf)
     Ut = transpose(U, buf);
 }
 
-// Warm-started eigenvectors of the Hermitian H: U (and its transpose Ut) in,
-// refined U, Ut out.
-__device__ void eig_refine(const Tile &H, Tile &U, Tile &Ut, int N, cd *buf)
+// Warm-started eigenvectors of the Hermitian part of H (as zheevr, which
+// reads one triangle; H from FP16-split sums is Hermitian only to their
+// error): U (and its transpose Ut) in, refined U, Ut out.
+__device__ void eig_refine(const Tile &Hin, Tile &U, Tile &Ut, int N, cd *buf)
 {
     const int c = lane_c(), g = lane_g();
+    Tile H;
+    {
+        const Tile T = transpose(Hin, buf);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            H.re[r] = 0.5 * (Hin.re[r] + T.re[r]);
+            H.im[r] = 0.5 * (Hin.im[r] - T.im[r]);
+        }
+    }
     Tile W = cmm(U, true, cmm(H, true, U));  // U^dag (H U)
     for (int it = 0; it < 12; ++it) {
         double lam_r[4], lam_c;
@@ -288,8 +298,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void nb
             Y.im[r] += sg.a[j] * Kj.im[r];
         }
     }
-    const Tile Ht = load_tile(mats + k * nn, N);  // mat 0: H~ = H + E.D
-    const Tile T = cmm(Ht, true, Y);              // H~ y
+    const Tile HtT = load_tile_t(mats + k * nn, N);  // mat 0: H~ = H + E.D, transposed (A operand)
+    const Tile T = cmm(HtT, false, Y);               // H~ y
     const Tile Tt = transpose(T, buf);
     Tile X;  // -i (T - T^dag)
 #pragma unroll
