@@ -687,8 +687,8 @@ __global__ __launch_bounds__(64) void nb_sample_kernel(const NbArgs a, const cd 
     for (int t = 0; t < 2 * kMaxPer; ++t) racc[t] = 0.0;
     double cur[6] = {0, 0, 0, 0, 0, 0};
     double pop = 0.0;  // lane m < N: population of the m-th lowest band
-    const cd *H0 = mats;                              // mat 0: H0
-    const cd *J = mats + a.nk_stride * nn;            // mats 1..3: J_x, J_y, J_z
+    const cd *H0 = mats + a.nk_stride * nn;           // mat 1: H0
+    const cd *J = mats + 2 * a.nk_stride * nn;        // mats 2..4: J_x, J_y, J_z
     for (int64_t k = blockIdx.x; k < a.nk; k += gridDim.x) {
         const double w = a.weight[k];
         wcopy(R, rho + k * nn, nn);

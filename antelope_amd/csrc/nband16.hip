@@ -352,7 +352,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void nb
 }
 
 // ------------------------------------------------------------ sample --
-// Observables at t_n (mats: H, J_x, J_y, J_z at k + A(t_n)); each wave walks
+// Observables at t_n (mats slots 1..4: H, J_x, J_y, J_z at k + A(t_n)); each wave walks
 // k = wave, wave + G, ... and writes one row of partial sums
 // [inter(3) intra(3) pop(N) rho(2 N^2)], as nb_sample_kernel.
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void nb16_sample_kernel(const NbArgs a, const cd *__restrict__ mats,
@@ -369,7 +369,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void nb
     for (int64_t k = blockIdx.x; k < a.nk; k += gridDim.x) {
         const double w = a.weight[k];
         const Tile R = load_tile(rho + k * nn, N);
-        const Tile H = load_tile(mats + k * nn, N);  // mat 0: H
+        const Tile H = load_tile(mats + a.nk_stride * nn + k * nn, N);  // mat 1: H
         Tile U = load_tile(Ust + k * nn, N, true);
         if (!ureuse) {  // eigenvectors at k + A(t_n), warm-started from the last ones
             Tile Ut = transpose(U, buf);
@@ -395,8 +395,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void nb
             racc[2 * r + 1] += R.im[r] * w;
         }
         for (int ax = 0; ax < 3; ++ax) {
-            // mats 1..3: J_a, read transposed: some model classes' J is not Hermitian
-            const Tile Jt = load_tile_t(mats + (1 + ax) * a.nk_stride * nn + k * nn, N);
+            // mats 2..4: J_a, read transposed: some model classes' J is not Hermitian
+            const Tile Jt = load_tile_t(mats + (2 + ax) * a.nk_stride * nn + k * nn, N);
             double tot = 0.0;  // Re Tr(rho J) = sum_ij Re(rho_ij J_ji)
 #pragma unroll
             for (int r = 0; r < 4; ++r) tot += R.re[r] * Jt.re[r] - R.im[r] * Jt.im[r];

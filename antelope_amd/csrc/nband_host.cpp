@@ -43,6 +43,9 @@ T *nalloc(sbe_ctx *ctx, size_t bytes, int &rc)
 constexpr size_t kNbPartialBytes = size_t(128) << 20;
 
 // the matrices {H~, H, J} at k + A: Fourier sums (Wannier) or the analytic classes
+// two sample/stage times that differ only by rounding (t_n + dt against t_n+1)
+bool same_time(double x, double y, double dt) { return std::fabs(x - y) <= 1e-9 * dt; }
+
 hipError_t eval_mats(const NbArgs &a, const NbEval &ev, const double2 *P, double2 *out, hipStream_t s)
 {
     return a.model_kind ? nb_launch_model(a, ev, out, s) : nb_launch_fourier(a, ev, P, out, s);
@@ -68,7 +71,7 @@ NbPlan nb_plan(int N, int nr, int64_t nk, int64_t nt, int nstage, bool split16)
     p.chunk = std::max<int64_t>(1, std::min<int64_t>(nt, static_cast<int64_t>(kNbPartialBytes / row)));
     p.bytes_P = static_cast<size_t>(p.nk_stride) * nr * sizeof(double2);
     p.bytes_rho = static_cast<size_t>(p.nk_stride) * nn * sizeof(double2);
-    p.bytes_mats = 4 * p.bytes_rho;
+    p.bytes_mats = 5 * p.bytes_rho;  // H~, H, J_x, J_y, J_z
     p.bytes_k = static_cast<size_t>(nstage + 1) * p.bytes_rho;  // stage derivatives + eigenvectors
     p.bytes_tab = static_cast<size_t>(p.nk_stride) * 4 * sizeof(double) + static_cast<size_t>(nr) * 4 * sizeof(double) +
                   static_cast<size_t>(nr) * 4 * nn * sizeof(double2);
@@ -267,9 +270,13 @@ int nb_setup(sbe_ctx *ctx)
         if (!rc) rc = hipc(ctx, hipStreamSynchronize(ctx->stream), "init sync");
         return rc;
     }
-    if (!rc) rc = hipc(ctx, eval_mats(a, ev, ctx->nb_P, ctx->nb_mats, ctx->stream), "fourier kernel");
-    if (!rc) rc = hipc(ctx, nb_launch_init(a, ctx->nb_mats, ctx->nb_rho, ctx->nb_U, ctx->stream), "init kernel");
+    // H(k) = H(k + A(atmin)) into slot 1, where the first sample finds it
+    double2 *h_slot = ctx->nb_mats + static_cast<size_t>(p.nk_stride) * nn;
+    if (!rc) rc = hipc(ctx, eval_mats(a, ev, ctx->nb_P, h_slot, ctx->stream), "fourier kernel");
+    if (!rc) rc = hipc(ctx, nb_launch_init(a, h_slot, ctx->nb_rho, ctx->nb_U, ctx->stream), "init kernel");
     ctx->nb_u_time = ctx->laser.atmin;  // U diagonalises H(k) = H(k + A(atmin))
+    ctx->nb_h_time = ctx->laser.atmin;
+    ctx->nb_h_valid = true;
     ctx->nb_ht_valid = false;
     if (!rc) rc = hipc(ctx, hipStreamSynchronize(ctx->stream), "init sync");
     return rc;
@@ -326,23 +333,29 @@ int nb_run(sbe_ctx *ctx, int64_t step_begin, int64_t step_end)
         for (int64_t s = 0; s < n && !rc; ++s) {
             const int64_t step = c0 + s;
             const double tn = ctx->laser.atmin + dt * static_cast<double>(step);  // antelope.cpp:299
+            // matrix slots: 0 H~, 1 H, 2..4 J. The last stage of a step is at the next sample's
+            // time, so its H (and H~, and the eigenvectors) serve the sample and the first
+            // stage of the next step; times match to rounding (t_n + dt against t_n+1)
+            const size_t slot = static_cast<size_t>(a.nk_stride) * a.N * a.N;
             NbEval ev{};
             double Ev[3];
             ctx->laser.eval(tn, ev.A, Ev);
-            ev.nmat = 4;
-            ev.mats[0] = kMatH0;
-            ev.mats[1] = kMatJx;
-            ev.mats[2] = kMatJy;
-            ev.mats[3] = kMatJz;
+            const bool h_ok = ctx->nb_h_valid && same_time(ctx->nb_h_time, tn, dt);
+            ev.nmat = 0;
+            if (!h_ok) ev.mats[ev.nmat++] = kMatH0;
+            ev.mats[ev.nmat++] = kMatJx;
+            ev.mats[ev.nmat++] = kMatJy;
+            ev.mats[ev.nmat++] = kMatJz;
             // eigenvectors: reused when they already belong to H(k + A(t_n)),
             // otherwise warm-started from the stored ones
-            const int ureuse = ctx->nb_u_time == tn;
-            rc = hipc(ctx, eval_mats(a, ev, ctx->nb_P, ctx->nb_mats, ctx->stream), "fourier kernel");
+            const int ureuse = same_time(ctx->nb_u_time, tn, dt);
+            rc = hipc(ctx, eval_mats(a, ev, ctx->nb_P, ctx->nb_mats + (h_ok ? 2 : 1) * slot, ctx->stream), "fourier kernel");
             if (!rc)
                 rc = hipc(ctx, nb_launch_sample(a, ctx->nb_mats, ctx->nb_rho, ctx->d_partials, s, ctx->nb_U, ureuse, ctx->stream),
                           "sample kernel");
             ctx->nb_u_time = tn;
-            ctx->nb_ht_valid = false;  // mats slot 0 now holds H, not H~
+            ctx->nb_h_time = tn;
+            ctx->nb_h_valid = true;
             for (int i = 0; i < tb.nstage && !rc; ++i) {
                 NbEval se{};
                 const double ts = tn + tb.c[i] * dt;
@@ -351,22 +364,28 @@ int nb_run(sbe_ctx *ctx, int64_t step_begin, int64_t step_end)
                 sg.stage = i;
                 sg.nstage = tb.nstage;
                 sg.last = i == tb.nstage - 1;
-                sg.ureuse = ctx->nb_u_time == ts;
+                sg.ureuse = same_time(ctx->nb_u_time, ts, dt);
                 for (int j = 0; j < i; ++j) sg.a[j] = dt * tb.a[i][j];
                 for (int j = 0; j < tb.nstage; ++j) sg.b[j] = dt * tb.b[j];
-                // H~ (and H when new eigenvectors are needed) unless slot 0 already
-                // holds H~ for the same (A, E) (RK4 stages 1 and 2 share a time)
-                const bool same_ht = ctx->nb_ht_valid && std::memcmp(ctx->nb_ht_A, se.A, sizeof se.A) == 0 &&
-                                     std::memcmp(ctx->nb_ht_E, se.E, sizeof se.E) == 0;
-                const bool need_h = a.deph && !sg.ureuse;
-                if (!same_ht || need_h) {
-                    se.nmat = need_h ? 2 : 1;
-                    se.mats[0] = kMatHtilde;
-                    se.mats[1] = kMatH0;
-                    rc = hipc(ctx, eval_mats(a, se, ctx->nb_P, ctx->nb_mats, ctx->stream), "fourier kernel");
-                    std::memcpy(ctx->nb_ht_A, se.A, sizeof se.A);
-                    std::memcpy(ctx->nb_ht_E, se.E, sizeof se.E);
-                    ctx->nb_ht_valid = true;
+                // H~ unless slot 0 already holds it for this time (RK4 stages 1 and 2 share a
+                // time; the last stage's serves the next step's first), and H when new
+                // eigenvectors are needed and slot 1 does not hold it
+                const bool need_ht = !(ctx->nb_ht_valid && same_time(ctx->nb_ht_time, ts, dt));
+                const bool need_h = a.deph && !sg.ureuse && !(ctx->nb_h_valid && same_time(ctx->nb_h_time, ts, dt));
+                if (need_ht || need_h) {
+                    se.nmat = 0;
+                    if (need_ht) se.mats[se.nmat++] = kMatHtilde;
+                    if (need_h) se.mats[se.nmat++] = kMatH0;
+                    rc = hipc(ctx, eval_mats(a, se, ctx->nb_P, ctx->nb_mats + (need_ht ? 0 : slot), ctx->stream),
+                              "fourier kernel");
+                    if (need_ht) {
+                        ctx->nb_ht_time = ts;
+                        ctx->nb_ht_valid = true;
+                    }
+                    if (need_h) {
+                        ctx->nb_h_time = ts;
+                        ctx->nb_h_valid = true;
+                    }
                 }
                 if (!rc)
                     rc = hipc(ctx, nb_launch_rhs(a, sg, ctx->nb_mats, ctx->nb_rho, ctx->nb_k, ctx->nb_U, ctx->stream),
@@ -438,6 +457,7 @@ int nb_state(sbe_ctx *ctx, int basis, double *rho)
         ev.mats[0] = kMatH0;
         double2 *out = ctx->nb_mats + static_cast<size_t>(a.nk_stride) * nn;
         ctx->nb_ht_valid = false;
+        ctx->nb_h_valid = false;
         int rc = hipc(ctx, eval_mats(a, ev, ctx->nb_P, ctx->nb_mats, ctx->stream), "fourier kernel");
         if (!rc) rc = hipc(ctx, nb_launch_band_export(a, ctx->nb_mats, ctx->nb_rho, out, ctx->stream), "band export kernel");
         if (!rc) rc = hipc(ctx, hipMemcpyAsync(rho, out, bytes, hipMemcpyDeviceToHost, ctx->stream), "copy state");

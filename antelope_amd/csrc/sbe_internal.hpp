@@ -163,12 +163,14 @@ struct sbe_ctx {
     sbe::NbArgs nargs{};
     double2 *nb_P = nullptr;     // e^{ik.R} [nk_stride][nr]
     double2 *nb_rho = nullptr;   // [nk_stride][N][N]
-    double2 *nb_mats = nullptr;  // Fourier-sum outputs [4][nk_stride][N][N]
+    double2 *nb_mats = nullptr;  // Fourier-sum outputs [5][nk_stride][N][N]: H~, H, J_x, J_y, J_z
     double2 *nb_k = nullptr;     // RK stage derivatives [nstage][nk_stride][N][N]
     double2 *nb_U = nullptr;     // last eigenvectors per k [nk_stride][N][N]
     double nb_u_time = 0;        // time whose H the stored eigenvectors diagonalise
-    double nb_ht_A[3] = {0, 0, 0}, nb_ht_E[3] = {0, 0, 0};  // (A, E) of the H~ in mats slot 0
+    double nb_ht_time = 0;       // time of the H~ in mats slot 0
     bool nb_ht_valid = false;
+    double nb_h_time = 0;        // time of the H in mats slot 1
+    bool nb_h_valid = false;
     bool nb_vel_p_lds = false;   // velocity gauge: pMatrix of the k point staged in LDS
     std::string err;
 
