@@ -183,7 +183,7 @@ namespace sbe {
 // Launchers (haldane.hip, series.hip).
 hipError_t launch_haldane_init(const HaldaneArgs &a, int init_type, double ef, hipStream_t s);
 hipError_t launch_haldane_steps(const HaldaneArgs &a, int rk_order, bool t1_on, hipStream_t s);
-hipError_t launch_series_reduce(const double *partials, int64_t nsteps, int32_t nrows, int nv,
+hipError_t launch_series_reduce(double *partials, int64_t nsteps, int32_t nrows, int nv,
                                 double *series, hipStream_t s);
 hipError_t launch_haldane_export(const HaldaneArgs &a, double *out, hipStream_t s);
 // VelocityHamiltonian gauge (haldane_velocity.hip): band-basis state

@@ -7,20 +7,26 @@
 // series no matter what else shares the GPU. It replaces the MPI_Reduce with
 // the custom complex-sum op of antelope.cpp:482-512,598-606 for the k points
 // owned by one context.
+//
+// Steps with many rows (large grids: 2^18 waves at 4096^2 k points) are
+// summed in two passes, so that one step is not left to a single block: the
+// rows split into groups of kGroupRows, each group's sum is written over the
+// group's first row, and the second pass adds the group sums in order. The
+// order still depends only on the row count.
 #include "sbe_internal.hpp"
 
 namespace sbe {
 namespace {
 
 constexpr int kReduceThreads = 256;
+constexpr int32_t kGroupRows = 4096;
 
-__global__ __launch_bounds__(kReduceThreads) void series_reduce_kernel(const double *__restrict__ partials,
-                                                                       int32_t nrows, int nv,
-                                                                       double *__restrict__ series)
+// Sum over the rows r0 + r rs, r < nr, of one step's block p, for this
+// block's values [vb, vb + vl), into out[0 .. vl). Thread (rlane, vlane) sums
+// rows rlane, rlane + rl, ... in order; the row lanes are then added in order.
+__device__ __forceinline__ void reduce_rows(const double *p, int64_t r0, int64_t rs, int64_t nr, int nv, double *out)
 {
     __shared__ double acc[kReduceThreads];
-    const int64_t step = blockIdx.x;
-    const double *p = partials + step * static_cast<int64_t>(nrows) * nv;
     const int vl = nv < 64 ? nv : 64;          // values per block
     const int rl = kReduceThreads / vl;        // row lanes
     const int tid = threadIdx.x;
@@ -29,27 +35,63 @@ __global__ __launch_bounds__(kReduceThreads) void series_reduce_kernel(const dou
     const int v = vb + vlane;
     double s = 0.0;
     if (rlane < rl && v < nv)
-        for (int r = rlane; r < nrows; r += rl) s += p[static_cast<int64_t>(r) * nv + v];
+        for (int64_t r = rlane; r < nr; r += rl) s += p[(r0 + r * rs) * nv + v];
     acc[tid] = s;
     __syncthreads();
     if (tid < vl && vb + tid < nv) {
         double t = 0.0;
         for (int r = 0; r < rl; ++r) t += acc[r * vl + tid];
-        series[step * nv + vb + tid] = t;
+        out[tid] = t;
     }
+}
+
+// one block per (step, block of 64 values): the rows 0, rs, 2 rs, ... (nr)
+__global__ __launch_bounds__(kReduceThreads) void series_reduce_kernel(const double *__restrict__ partials,
+                                                                       int32_t nrows, int32_t nr, int32_t rs,
+                                                                       int nv, double *__restrict__ series)
+{
+    const int64_t step = blockIdx.x;
+    const int vl = nv < 64 ? nv : 64;
+    reduce_rows(partials + step * static_cast<int64_t>(nrows) * nv, 0, rs, nr, nv, series + step * nv + blockIdx.y * vl);
+}
+
+// first pass: group g = blockIdx.z of a step sums its rows [g G, (g + 1) G)
+// and writes the sum over row g G. Only this block reads or writes those rows;
+// its reads are done (barrier in reduce_rows) before the write.
+__global__ __launch_bounds__(kReduceThreads) void series_group_kernel(double *__restrict__ partials, int32_t nrows,
+                                                                      int nv)
+{
+    const int64_t step = blockIdx.x;
+    const int64_t g0 = static_cast<int64_t>(blockIdx.z) * kGroupRows;
+    const int64_t nr = nrows - g0 < kGroupRows ? nrows - g0 : kGroupRows;
+    double *p = partials + step * static_cast<int64_t>(nrows) * nv;
+    __shared__ double sums[64];
+    reduce_rows(p, g0, 1, nr, nv, sums);
+    __syncthreads();
+    const int vl = nv < 64 ? nv : 64, vb = blockIdx.y * vl;
+    if (threadIdx.x < vl && vb + static_cast<int>(threadIdx.x) < nv) p[g0 * nv + vb + threadIdx.x] = sums[threadIdx.x];
 }
 
 }  // namespace
 
-hipError_t launch_series_reduce(const double *partials, int64_t nsteps, int32_t nrows, int nv, double *series,
+hipError_t launch_series_reduce(double *partials, int64_t nsteps, int32_t nrows, int nv, double *series,
                                 hipStream_t s)
 {
     if (nsteps <= 0) return hipSuccess;
     // one block per (step, block of 64 values): the summation order per value
     // (row lanes, then the lanes in order) does not depend on the grid
     const int vl = nv < 64 ? nv : 64;
-    const dim3 grid(static_cast<unsigned>(nsteps), static_cast<unsigned>((nv + vl - 1) / vl));
-    series_reduce_kernel<<<grid, kReduceThreads, 0, s>>>(partials, nrows, nv, series);
+    const unsigned vblocks = static_cast<unsigned>((nv + vl - 1) / vl);
+    if (nrows > 2 * kGroupRows) {
+        const int32_t groups = (nrows + kGroupRows - 1) / kGroupRows;
+        series_group_kernel<<<dim3(static_cast<unsigned>(nsteps), vblocks, static_cast<unsigned>(groups)),
+                              kReduceThreads, 0, s>>>(partials, nrows, nv);
+        series_reduce_kernel<<<dim3(static_cast<unsigned>(nsteps), vblocks), kReduceThreads, 0, s>>>(
+            partials, nrows, groups, kGroupRows, nv, series);
+    } else {
+        series_reduce_kernel<<<dim3(static_cast<unsigned>(nsteps), vblocks), kReduceThreads, 0, s>>>(
+            partials, nrows, nrows, 1, nv, series);
+    }
     return hipGetLastError();
 }
 
