@@ -362,11 +362,13 @@ __device__ __forceinline__ void wave_sync() { __syncthreads(); }
 // beyond N read as zero. C must not alias A or B.
 __device__ void wmm(cd *C, const cd *A, const cd *B, int N, bool adjA, bool adjB)
 {
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63;
     const int r16 = lane & 15, g = lane >> 4;
     const int T = (N + 15) >> 4, K4 = (N + 3) >> 2;
-    for (int ti = 0; ti < T; ++ti)
-        for (int tj = 0; tj < T; ++tj) {
+    // output tiles are dealt to the block's waves
+    for (int tt = static_cast<int>(threadIdx.x >> 6); tt < T * T; tt += static_cast<int>(blockDim.x >> 6)) {
+        const int ti = tt / T, tj = tt % T;
+        {
             d4 re = d4{0.0, 0.0, 0.0, 0.0}, im = d4{0.0, 0.0, 0.0, 0.0};
             const int i = 16 * ti + r16, j = 16 * tj + r16;
             for (int kb = 0; kb < K4; ++kb) {
@@ -385,13 +387,40 @@ __device__ void wmm(cd *C, const cd *A, const cd *B, int N, bool adjA, bool adjB
                 if (row < N && col < N) C[row * N + col] = cmk(re[r], im[r]);
             }
         }
+    }
     wave_sync();
 }
 
 __device__ void wcopy(cd *D, const cd *S, int nn)
 {
-    for (int e = threadIdx.x; e < nn; e += 64) D[e] = S[e];
+    for (int e = threadIdx.x; e < nn; e += blockDim.x) D[e] = S[e];
     wave_sync();
+}
+
+// sum / max over the block, in a fixed order (each wave's lanes, then the waves)
+__device__ __forceinline__ double block_sum(double v)
+{
+    __shared__ double red[16];
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int i = 0; i < static_cast<int>(blockDim.x >> 6); ++i) t += red[i];
+    __syncthreads();
+    return t;
+}
+
+__device__ __forceinline__ double block_max(double v)
+{
+    __shared__ double red[16];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int i = 0; i < static_cast<int>(blockDim.x >> 6); ++i) t = fmax(t, red[i]);
+    __syncthreads();
+    return t;
 }
 
 // Hermitian eigensolver: A (destroyed) -> ev ascending, U columns = vectors.
@@ -524,7 +553,7 @@ __device__ void weigh_warm(cd *&A, cd *&U, cd *&W, cd *&T, cd *&X, int N, double
     const int nn = N * N;
     // the Hermitian part of H (as zheevr, which reads one triangle)
     auto hermitize = [&]() {
-        for (int e = lane; e < nn; e += 64) {
+        for (int e = lane; e < nn; e += blockDim.x) {
             const int i = e / N, j = e % N;
             const cd x = A[e], y = A[j * N + i];
             X[e] = cmk(0.5 * (x.x + y.x), 0.5 * (x.y - y.y));
@@ -539,7 +568,7 @@ __device__ void weigh_warm(cd *&A, cd *&U, cd *&W, cd *&T, cd *&X, int N, double
     wmm(W, U, T, N, true, false);  // W = U^dag H U
     auto newton_schulz = [&]() {  // U <- U (3I - U^dag U) / 2
         wmm(T, U, U, N, true, false);
-        for (int e = lane; e < nn; e += 64) {
+        for (int e = lane; e < nn; e += blockDim.x) {
             const int i = e / N, j = e % N;
             T[e] = cmk((i == j ? 1.5 : 0.0) - 0.5 * T[e].x, -0.5 * T[e].y);
         }
@@ -553,7 +582,7 @@ __device__ void weigh_warm(cd *&A, cd *&U, cd *&W, cd *&T, cd *&X, int N, double
         if (lane < N) lam[lane] = W[lane * N + lane].x;
         wave_sync();
         double kmax = 0.0;  // K = rotation generators, into A
-        for (int e = lane; e < nn; e += 64) {
+        for (int e = lane; e < nn; e += blockDim.x) {
             const int i = e / N, j = e % N;
             const cd w = W[e];
             const double w2 = w.x * w.x + w.y * w.y;
@@ -567,13 +596,11 @@ __device__ void weigh_warm(cd *&A, cd *&U, cd *&W, cd *&T, cd *&X, int N, double
             }
             A[e] = kk;
         }
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) kmax = fmax(kmax, __shfl_xor(kmax, o, 64));
-        wave_sync();
+        kmax = block_max(kmax);
         if (kmax == 0.0) break;
         const bool small = kmax < 1e-8;
         if (!small) wmm(T, A, A, N, false, false);  // K^2
-        for (int e = lane; e < nn; e += 64) {  // V = I + K (+ K^2 / 2)
+        for (int e = lane; e < nn; e += blockDim.x) {  // V = I + K (+ K^2 / 2)
             const int i = e / N, j = e % N;
             cd v = A[e];
             if (!small) v = cadd(v, cscale(T[e], 0.5));
@@ -590,7 +617,7 @@ __device__ void weigh_warm(cd *&A, cd *&U, cd *&W, cd *&T, cd *&X, int N, double
         if (small) break;
         if (kmax > 1e-3) {  // V far from unitary: restore U, then W from H itself
             newton_schulz();
-            for (int e = lane; e < nn; e += 64) A[e] = Hsrc[e];
+            for (int e = lane; e < nn; e += blockDim.x) A[e] = Hsrc[e];
             wave_sync();
             hermitize();
             wmm(T, A, U, N, false, false);
@@ -606,13 +633,15 @@ __device__ void weigh_warm(cd *&A, cd *&U, cd *&W, cd *&T, cd *&X, int N, double
     newton_schulz();
 }
 
-// d[c] = Re (U^dag M)_cc = Re sum_r conj(U_rc) M_rc for c < N (lanes c)
+// d[c] = Re (U^dag M)_cc = Re sum_r conj(U_rc) M_rc for c < N (threads c);
+// ends with a block barrier, so M may be overwritten next
 __device__ __forceinline__ double col_dot_lds(const cd *U, const cd *M, int N)
 {
     const int c = threadIdx.x;
     double s = 0.0;
     if (c < N)
         for (int r = 0; r < N; ++r) s += U[r * N + c].x * M[r * N + c].x + U[r * N + c].y * M[r * N + c].y;
+    wave_sync();
     return s;
 }
 
@@ -669,7 +698,7 @@ __global__ __launch_bounds__(64) void nb_init_kernel(const NbArgs a, const cd *_
 // Re Tr(rho J) when inter_intra == 0); band populations diag(rho_H); and
 // w rho for the orbital-basis sum. Each wave walks k = wave, wave + G, ...
 // and writes one row of partial sums: [inter(3) intra(3) pop(N) rho(2N^2)].
-__global__ __launch_bounds__(64) void nb_sample_kernel(const NbArgs a, const cd *__restrict__ mats,
+__global__ __launch_bounds__(256) void nb_sample_kernel(const NbArgs a, const cd *__restrict__ mats,
                                                        const cd *__restrict__ rho, double *__restrict__ partials,
                                                        int64_t step_local, cd *__restrict__ Ust, int ureuse)
 {
@@ -696,7 +725,7 @@ __global__ __launch_bounds__(64) void nb_sample_kernel(const NbArgs a, const cd 
         wcopy(A, H0 + k * nn, nn);
         if (!ureuse) {  // eigenvectors at k + A(t_n), warm-started from the last ones
             weigh_warm(A, U, W, T, X, N, ev, H0 + k * nn);
-            for (int e = lane; e < nn; e += 64) Ust[k * nn + e] = U[e];
+            for (int e = lane; e < nn; e += blockDim.x) Ust[k * nn + e] = U[e];
             wcopy(A, H0 + k * nn, nn);
         }
         // band energies (ascending order) and populations of U's columns
@@ -714,25 +743,25 @@ __global__ __launch_bounds__(64) void nb_sample_kernel(const NbArgs a, const cd 
             const double pj = __shfl(p, j, 64);
             if (rj == lane) pop += pj * w;
         }
-        for (int e = lane, t = 0; e < nn; e += 64, ++t) {
+        for (int e = lane, t = 0; e < nn; e += blockDim.x, ++t) {
             racc[2 * t] += R[e].x * w;
             racc[2 * t + 1] += R[e].y * w;
         }
         for (int ax = 0; ax < 3; ++ax) {
             const cd *Ja = J + static_cast<int64_t>(ax) * a.nk_stride * nn + k * nn;
             double tot = 0.0;  // Re Tr(rho J) = sum_ij Re(rho_ij J_ji)
-            for (int e = lane; e < nn; e += 64) {
+            for (int e = lane; e < nn; e += blockDim.x) {
                 const int i = e / N, j = e % N;
                 const cd r = R[e], jj = Ja[j * N + i];
                 tot += r.x * jj.x - r.y * jj.y;
             }
-            tot = wave_sum(tot);
+            tot = block_sum(tot);
             double intra = 0.0;
             if (a.inter_intra) {  // sum_c (U^dag rho U)_cc Re (U^dag J U)_cc
                 wcopy(A, Ja, nn);
                 wmm(T, A, U, N, false, false);
                 const double jd = col_dot_lds(U, T, N);
-                intra = wave_sum(lane < N ? p * jd : 0.0);
+                intra = block_sum(lane < N ? p * jd : 0.0);
             }
             cur[ax] += (tot - intra) * w;
             cur[3 + ax] += intra * w;
@@ -742,7 +771,7 @@ __global__ __launch_bounds__(64) void nb_sample_kernel(const NbArgs a, const cd 
     double *row = partials + (step_local * gridDim.x + blockIdx.x) * static_cast<int64_t>(nv);
     if (lane < 6) row[lane] = cur[lane];
     if (lane < N) row[6 + lane] = pop;
-    for (int e = lane, t = 0; e < nn; e += 64, ++t) {
+    for (int e = lane, t = 0; e < nn; e += blockDim.x, ++t) {
         row[6 + N + 2 * e] = racc[2 * t];
         row[6 + N + 2 * e + 1] = racc[2 * t + 1];
     }
@@ -753,7 +782,7 @@ __global__ __launch_bounds__(64) void nb_sample_kernel(const NbArgs a, const cd 
 // f(y) = -i(H~ y - (H~ y)^dag) - U (Gamma o (U^dag y U)) U^dag; after the
 // last stage rho += dt sum_i b_i k_i (SBEs.h:596-624, 762-861). The
 // dephasing is formed as g2 y + (g1 - g2) U diag(U^dag y U) U^dag.
-__global__ __launch_bounds__(64) void nb_rhs_kernel(const NbArgs a, const NbStage sg, const cd *__restrict__ mats,
+__global__ __launch_bounds__(256) void nb_rhs_kernel(const NbArgs a, const NbStage sg, const cd *__restrict__ mats,
                                                     cd *__restrict__ rho, cd *__restrict__ kbuf, cd *__restrict__ Ust)
 {
     extern __shared__ double4 smem4[];
@@ -768,7 +797,7 @@ __global__ __launch_bounds__(64) void nb_rhs_kernel(const NbArgs a, const NbStag
     const int64_t kstride = a.nk_stride * nn;
     cd *rk = rho + k * nn;
     // stage input
-    for (int e = lane; e < nn; e += 64) {
+    for (int e = lane; e < nn; e += blockDim.x) {
         cd y = rk[e];
         for (int j = 0; j < sg.stage; ++j)
             if (sg.a[j] != 0.0) y = cadd(y, cscale(kbuf[j * kstride + k * nn + e], sg.a[j]));
@@ -781,16 +810,16 @@ __global__ __launch_bounds__(64) void nb_rhs_kernel(const NbArgs a, const NbStag
             const cd *H0 = mats + a.nk_stride * nn + k * nn;  // mat 1: H
             wcopy(A, H0, nn);
             weigh_warm(A, U, W, T, X, N, ev, H0);
-            for (int e = lane; e < nn; e += 64) Ust[k * nn + e] = U[e];
+            for (int e = lane; e < nn; e += blockDim.x) Ust[k * nn + e] = U[e];
         }
         wmm(T, Y, U, N, false, false);  // y U
         const double d = col_dot_lds(U, T, N);
         if (lane < N) ev[lane] = d;  // (U^dag y U)_cc
         wave_sync();
-        for (int e = lane; e < nn; e += 64) T[e] = cscale(U[e], ev[e % N]);  // U diag(d)
+        for (int e = lane; e < nn; e += blockDim.x) T[e] = cscale(U[e], ev[e % N]);  // U diag(d)
         wave_sync();
         wmm(A, T, U, N, false, true);  // U diag(d) U^dag
-        for (int e = lane; e < nn; e += 64)
+        for (int e = lane; e < nn; e += blockDim.x)
             A[e] = cadd(cscale(Y[e], a.gam2), cscale(A[e], a.gam1 - a.gam2));  // the dephasing term
         wave_sync();
     }
@@ -799,7 +828,7 @@ __global__ __launch_bounds__(64) void nb_rhs_kernel(const NbArgs a, const NbStag
     wmm(T, W, Y, N, false, false);                // T = H~ y
     cd *ko = kbuf + sg.stage * kstride + k * nn;
     // -i (T - T^dag): (-i)(t_ij - conj(t_ji))
-    for (int e = lane; e < nn; e += 64) {
+    for (int e = lane; e < nn; e += blockDim.x) {
         const int i = e / N, j = e % N;
         const cd d = csub(T[i * N + j], cconj(T[j * N + i]));
         cd x = cmk(d.y, -d.x);
@@ -807,9 +836,9 @@ __global__ __launch_bounds__(64) void nb_rhs_kernel(const NbArgs a, const NbStag
         X[e] = x;
     }
     wave_sync();
-    for (int e = lane; e < nn; e += 64) ko[e] = X[e];
+    for (int e = lane; e < nn; e += blockDim.x) ko[e] = X[e];
     if (sg.last) {
-        for (int e = lane; e < nn; e += 64) {
+        for (int e = lane; e < nn; e += blockDim.x) {
             cd r = rk[e];
             for (int j = 0; j < sg.nstage; ++j)
                 if (sg.b[j] != 0.0) r = cadd(r, cscale(j == sg.stage ? X[e] : kbuf[j * kstride + k * nn + e], sg.b[j]));
@@ -1115,7 +1144,10 @@ hipError_t nb_launch_sample(const NbArgs &a, const double2 *mats, const double2 
                             int64_t step_local, double2 *U, int ureuse, hipStream_t s)
 {
     if (a.tile16) return nb16_launch_sample(a, mats, rho, partials, step_local, U, ureuse, s);
-    nb_sample_kernel<<<dim3(static_cast<unsigned>(a.sample_waves)), 64, nb_lds_bytes(a.N), s>>>(a, mats, rho, partials,
+    // N > 16: four waves per k point (the products' output tiles and the element loops split
+    // among them)
+    const unsigned threads = a.N > 16 ? 256u : 64u;
+    nb_sample_kernel<<<dim3(static_cast<unsigned>(a.sample_waves)), threads, nb_lds_bytes(a.N), s>>>(a, mats, rho, partials,
                                                                                                step_local, U, ureuse);
     return hipGetLastError();
 }
@@ -1124,7 +1156,8 @@ hipError_t nb_launch_rhs(const NbArgs &a, const NbStage &sg, const double2 *mats
                          double2 *U, hipStream_t s)
 {
     if (a.tile16) return nb16_launch_rhs(a, sg, mats, rho, kbuf, U, s);
-    nb_rhs_kernel<<<dim3(static_cast<unsigned>(a.nk)), 64, nb_lds_bytes(a.N), s>>>(a, sg, mats, rho, kbuf, U);
+    const unsigned threads = a.N > 16 ? 256u : 64u;
+    nb_rhs_kernel<<<dim3(static_cast<unsigned>(a.nk)), threads, nb_lds_bytes(a.N), s>>>(a, sg, mats, rho, kbuf, U);
     return hipGetLastError();
 }
 
