@@ -544,14 +544,14 @@ __device__ void weigh(cd *A, cd *U, double *ev, cd *work, int N, double4 *rot, b
 // Warm-started eigenvectors for the LDS kernels: the simultaneous-rotation
 // refinement of nband16.hip's eig_refine on LDS matrices and wmm products.
 // In: A = H (overwritten; its Hermitian part is used), U = eigenvectors of a
-// nearby H. Out: U refined,
+// nearby H; Hsrc = H in memory (reloaded when W is rebuilt). Out: U refined,
 // columns in the warm start's order. W, T, X: scratch. Pointers are swapped
-// among the five slots; Hsrc reloads H on the rare path that needs it.
+// among the five slots.
 __device__ void weigh_warm(cd *&A, cd *&U, cd *&W, cd *&T, cd *&X, int N, double *lam, const cd *Hsrc)
 {
     const int lane = threadIdx.x;
     const int nn = N * N;
-    // the Hermitian part of H (as zheevr, which reads one triangle)
+    // A <- the Hermitian part of H (as zheevr, which reads one triangle)
     auto hermitize = [&]() {
         for (int e = lane; e < nn; e += blockDim.x) {
             const int i = e / N, j = e % N;
@@ -563,22 +563,27 @@ __device__ void weigh_warm(cd *&A, cd *&U, cd *&W, cd *&T, cd *&X, int N, double
         A = X;
         X = t;
     };
-    hermitize();
-    wmm(T, A, U, N, false, false);
-    wmm(W, U, T, N, true, false);  // W = U^dag H U
-    auto newton_schulz = [&]() {  // U <- U (3I - U^dag U) / 2
+    // U <- U (3I - U^dag U) / 2; returns max |U^dag U - I| before the step
+    auto newton_schulz = [&]() {
         wmm(T, U, U, N, true, false);
-        for (int e = lane; e < nn; e += blockDim.x) {
-            const int i = e / N, j = e % N;
-            T[e] = cmk((i == j ? 1.5 : 0.0) - 0.5 * T[e].x, -0.5 * T[e].y);
+        double e = 0.0;
+        for (int q = lane; q < nn; q += blockDim.x) {
+            const int i = q / N, j = q % N;
+            const double dr = T[q].x - (i == j ? 1.0 : 0.0);
+            e = fmax(e, fmax(fabs(dr), fabs(T[q].y)));
+            T[q] = cmk((i == j ? 1.5 : 0.0) - 0.5 * T[q].x, -0.5 * T[q].y);
         }
-        wave_sync();
+        e = block_max(e);
         wmm(X, U, T, N, false, false);
         cd *t = U;
         U = X;
         X = t;
+        return e;
     };
-    for (int it = 0; it < 12; ++it) {
+    hermitize();
+    wmm(T, A, U, N, false, false);
+    wmm(W, U, T, N, true, false);  // W = U^dag H U
+    for (int it = 0; it < 30; ++it) {
         if (lane < N) lam[lane] = W[lane * N + lane].x;
         wave_sync();
         double kmax = 0.0;  // K = rotation generators, into A
@@ -599,6 +604,11 @@ __device__ void weigh_warm(cd *&A, cd *&U, cd *&W, cd *&T, cd *&X, int N, double
         kmax = block_max(kmax);
         if (kmax == 0.0) break;
         const bool small = kmax < 1e-8;
+        const double damp = kmax > 0.5 ? 0.5 / kmax : 1.0;  // cold-like start: damped rotations
+        if (damp != 1.0) {
+            for (int e = lane; e < nn; e += blockDim.x) A[e] = cscale(A[e], damp);
+            wave_sync();
+        }
         if (!small) wmm(T, A, A, N, false, false);  // K^2
         for (int e = lane; e < nn; e += blockDim.x) {  // V = I + K (+ K^2 / 2)
             const int i = e / N, j = e % N;
@@ -615,8 +625,9 @@ __device__ void weigh_warm(cd *&A, cd *&U, cd *&W, cd *&T, cd *&X, int N, double
             T = t;
         }
         if (small) break;
-        if (kmax > 1e-3) {  // V far from unitary: restore U, then W from H itself
-            newton_schulz();
+        if (kmax > 1e-3) {  // V far from unitary: restore U fully, then W from H itself
+            for (int q = 0; q < 8; ++q)
+                if (newton_schulz() < 1e-8) break;
             for (int e = lane; e < nn; e += blockDim.x) A[e] = Hsrc[e];
             wave_sync();
             hermitize();

@@ -20,8 +20,10 @@
 //    W_ij, bounded for near-degenerate pairs), K anti-Hermitian;
 //    U <- U (I + K + K^2/2), W <- V^dag W V until max|K| < 1e-8, then a last
 //    U (I + K) and one Newton-Schulz step U (3I - U^dag U)/2 to keep U
-//    unitary to rounding (after a round with max|K| > 1e-3, whose V is
-//    unitary only to |K|^4, U is restored at once and W recomputed from H).
+//    unitary to rounding. After a round with max|K| > 1e-3 (V unitary only to
+//    |K|^4 / 4) U is made unitary again by Newton-Schulz steps until
+//    |U^dag U - I| < 1e-8 and W is rebuilt from H; rounds with max|K| > 0.5
+//    are damped to 0.5 (a cold-like start converges in ~10 rounds).
 //    Convergence is quadratic; between RK stages two or three rounds reach
 //    machine precision. The columns keep the order of
 //    the warm start (bands are followed continuously); the sample kernel
@@ -186,19 +188,24 @@ __device__ __forceinline__ void diag_bcast(const Tile &W, double lam_r[4], doubl
     }
 }
 
-// U <- U (3I - U^dag U) / 2 (Newton-Schulz: unitarity error e -> ~e^2)
-__device__ __forceinline__ void orthonormalize(Tile &U, Tile &Ut, cd *buf)
+// U <- U (3I - U^dag U) / 2 (Newton-Schulz: unitarity error e -> ~e^2);
+// returns e = max |U^dag U - I| before the step (over the wave)
+__device__ __forceinline__ double orthonormalize(Tile &U, Tile &Ut, cd *buf)
 {
     const int c = lane_c(), g = lane_g();
     const Tile G = cmm(U, true, U);
     Tile S;
+    double e = 0.0;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
+        const double dr = G.re[r] - (g + 4 * r == c ? 1.0 : 0.0);
+        e = fmax(e, fmax(fabs(dr), fabs(G.im[r])));
         S.re[r] = (g + 4 * r == c ? 1.5 : 0.0) - 0.5 * G.re[r];
         S.im[r] = -0.5 * G.im[r];
     }
     U = cmm(Ut, false, S);
     Ut = transpose(U, buf);
+    return wave_max(e);
 }
 
 // Warm-started eigenvectors of the Hermitian part of H (as zheevr, which
@@ -217,7 +224,7 @@ __device__ void eig_refine(const Tile &Hin, Tile &U, Tile &Ut, int N, cd *buf)
         }
     }
     Tile W = cmm(U, true, cmm(H, true, U));  // U^dag (H U)
-    for (int it = 0; it < 12; ++it) {
+    for (int it = 0; it < 30; ++it) {
         double lam_r[4], lam_c;
         diag_bcast(W, lam_r, lam_c);
         Tile K;
@@ -239,6 +246,14 @@ __device__ void eig_refine(const Tile &Hin, Tile &U, Tile &Ut, int N, cd *buf)
         }
         kmax = wave_max(kmax);
         if (kmax == 0.0) break;
+        if (kmax > 0.5) {  // far from converged (a cold-like start): damped rotations
+            const double f = 0.5 / kmax;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                K.re[r] *= f;
+                K.im[r] *= f;
+            }
+        }
         Tile V;
         const bool small = kmax < 1e-8;
         if (small) {  // I + K: the neglected K^2 / 2 is below 1e-16
@@ -264,8 +279,9 @@ __device__ void eig_refine(const Tile &Hin, Tile &U, Tile &Ut, int N, cd *buf)
         U = cmm(Ut, false, V);  // U V
         Ut = transpose(U, buf);
         if (small) break;
-        if (kmax > 1e-3) {  // V far from unitary: restore U, then W from H itself
-            orthonormalize(U, Ut, buf);
+        if (kmax > 1e-3) {  // V far from unitary: restore U fully, then W from H itself
+            for (int q = 0; q < 8; ++q)
+                if (orthonormalize(U, Ut, buf) < 1e-8) break;
             W = cmm(U, true, cmm(H, true, U));
         } else {
             W = cmm(V, true, cmm(W, true, V));  // V^dag (W V)

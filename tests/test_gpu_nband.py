@@ -118,3 +118,25 @@ def test_register_tile_kernels_match_lds_kernels(monkeypatch, N, gauge):
     errs["state"] = rel_l2(sa, sb)
     print(N, gauge, {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
     assert all(v <= 1e-12 for v in errs.values()), errs
+
+
+@pytest.mark.parametrize("lds", [False, True])
+def test_strong_field_large_rotations(oracle, monkeypatch, lds):
+    """A strong, fast field with a coarse step, so the eigenvectors turn far
+    between stage times and the warm-started refinement takes its large-angle
+    branch (rotation generators above 1e-3: unitarity restored and W rebuilt
+    from H). Same 1e-10 bar against the oracle, whose eigensolver starts cold
+    at every stage; both the register-tile and the LDS kernels."""
+    if lds:
+        monkeypatch.setenv("ANTELOPE_NB_LDS", "1")
+    model = synthetic_wannier(8, nmax=(1, 1, 1), seed=21)
+    cfg = _cfg(npts=(3, 2, 2), ncycles=0.5, E0=0.3, dt=1.0, T1=60.0, T2=40.0)
+    print("case: N=8 E0=0.3 dt=1.0 lds=", lds, flush=True)
+    ref = oracle.run(cfg, model, nthreads=8, want_state=True)
+    with SBESolver(cfg, model) as s:
+        s.run(0, 9)
+        s.run(9, s.nt)
+        gpu = s.series()
+        st = s.state()
+    _compare(gpu, ref, f"strong_field_lds{int(lds)}")
+    assert rel_l2(st, ref["state"]) < TOL64
