@@ -130,8 +130,11 @@ int make_plan(const sbe_config &cfg, const sbe_model &m, int64_t nk, int64_t nt,
 {
     p.nk = nk;
     // k points per lane of the step kernel: 2 gives each lane two independent
-    // dependency chains, which pays when there are few waves per SIMD
-    p.kpt = 1;
+    // dependency chains. Measured: FP64 gains 6-10 % with two once there are a
+    // thousand waves or more (300^2: 36.9 against 41.0 ms; 1024^2: 37.6 against
+    // 39.9 ms for 3000 steps) and loses below that (200^2: 5.1 against 4.0 ms);
+    // FP32 does best with one throughout
+    p.kpt = cfg.precision == SBE_PREC_FP64 && nk >= 65536 ? 2 : 1;
     if (const char *e = std::getenv("ANTELOPE_HALDANE_KPT")) p.kpt = std::atoi(e) == 2 ? 2 : 1;
     if (cfg.rk_order == SBE_RK_CASH_KARP) {  // adaptive: 64 kpt consecutive k points per wave (step-control block)
         // one k point per lane: the FP64 kernel then fits two waves per SIMD (1.1x over
