@@ -44,6 +44,9 @@ def _compare(gpu, ref, label):
     ("n8_total_current", 8, dict(inter_intra=False)),
     ("n16_short", 16, dict(npts=(2, 2, 2), ncycles=0.25)),
     ("n32_short", 32, dict(npts=(2, 2, 1), ncycles=0.25)),
+    ("n5_odd_t1t2", 5, dict(T1=120.0)),
+    ("n13_odd_rk5", 13, dict(npts=(2, 2, 2), ncycles=0.25, rk=5)),
+    ("n17_four_waves", 17, dict(npts=(2, 2, 1), ncycles=0.25)),
 ])
 def test_wannier_parity(oracle, label, N, kw):
     model = synthetic_wannier(N, nmax=(1, 1, 1), seed=7 + N)
