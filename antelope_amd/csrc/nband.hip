@@ -40,11 +40,6 @@ __device__ __forceinline__ void cfma(cd &acc, cd a, cd b)
     acc.y += a.x * b.y + a.y * b.x;
 }
 // acc += conj(a) * b
-__device__ __forceinline__ void cfmac(cd &acc, cd a, cd b)
-{
-    acc.x += a.x * b.x + a.y * b.y;
-    acc.y += a.x * b.y - a.y * b.x;
-}
 
 __device__ __forceinline__ double wave_sum(double v)
 {

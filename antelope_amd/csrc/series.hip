@@ -69,7 +69,8 @@ __global__ __launch_bounds__(kReduceThreads) void series_group_kernel(double *__
     reduce_rows(p, g0, 1, nr, nv, sums);
     __syncthreads();
     const int vl = nv < 64 ? nv : 64, vb = blockIdx.y * vl;
-    if (threadIdx.x < vl && vb + static_cast<int>(threadIdx.x) < nv) p[g0 * nv + vb + threadIdx.x] = sums[threadIdx.x];
+    const int tid = static_cast<int>(threadIdx.x);
+    if (tid < vl && vb + tid < nv) p[g0 * nv + vb + tid] = sums[tid];
 }
 
 }  // namespace
