@@ -98,6 +98,9 @@ hipError_t nb16_launch_rhs(const NbArgs &a, const NbStage &sg, const double2 *ma
                            double2 *U, hipStream_t s);
 hipError_t nb16_launch_sample(const NbArgs &a, const double2 *mats, const double2 *rho, double *partials,
                               int64_t step_local, double2 *U, int ureuse, hipStream_t s);
+// two consecutive stages at the same time in one launch (RK4's c = 1/2, 1/2)
+hipError_t nb16_launch_rhs_pair(const NbArgs &a, const NbStage &sg, const NbStage &sg2, const double2 *mats,
+                                double2 *rho, double2 *kbuf, double2 *U, hipStream_t s);
 hipError_t nb16_launch_vel_step(const NbArgs &a, const NbVelStep &v, const double2 *P, const double *Eocc,
                                 double2 *rho, double *partials, int64_t step_local, hipStream_t s);
 

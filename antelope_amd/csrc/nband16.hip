@@ -291,31 +291,13 @@ __device__ void eig_refine(const Tile &Hin, Tile &U, Tile &Ut, int N, cd *buf)
 }
 
 // ------------------------------------------------------------- RK stage --
-// Stage i: y = rho + sum_j dt a_ij k_j; k_i = -i(H~ y - (H~ y)^dag)
-// - U (Gamma o (U^dag y U)) U^dag; after the last stage rho += sum_i dt b_i k_i.
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void nb16_rhs_kernel(const NbArgs a, const NbStage sg, const cd *__restrict__ mats,
-                                                      cd *__restrict__ rho, cd *__restrict__ kbuf,
-                                                      cd *__restrict__ Ust)
+// The stage derivative at the stage input y, given H~^T's tile and (when
+// dephasing) U, U^T: -i(H~ y - (H~ y)^dag) - g2 y - (g1 - g2) U diag(U^dag y U) U^dag.
+__device__ __forceinline__ Tile stage_derivative(const NbArgs &a, const Tile &Y, const Tile &HtT, const Tile &U,
+                                                 const Tile &Ut, cd *buf)
 {
-    __shared__ cd buf[16 * kTpad];
-    const int64_t k = blockIdx.x;
-    if (k >= a.nk) return;
-    const int N = a.N, nn = N * N;
-    const int c = lane_c(), g = lane_g();
-    const int64_t kstride = a.nk_stride * nn;
-    cd *rk = rho + k * nn;
-    Tile Y = load_tile(rk, N);
-    for (int j = 0; j < sg.stage; ++j) {
-        if (sg.a[j] == 0.0) continue;
-        const Tile Kj = load_tile(kbuf + j * kstride + k * nn, N);
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            Y.re[r] += sg.a[j] * Kj.re[r];
-            Y.im[r] += sg.a[j] * Kj.im[r];
-        }
-    }
-    const Tile HtT = load_tile_t(mats + k * nn, N);  // mat 0: H~ = H + E.D, transposed (A operand)
-    const Tile T = cmm(HtT, false, Y);               // H~ y
+    const int g = lane_g();
+    const Tile T = cmm(HtT, false, Y);  // H~ y
     const Tile Tt = transpose(T, buf);
     Tile X;  // -i (T - T^dag)
 #pragma unroll
@@ -325,13 +307,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void nb
         X.im[r] = -dr;
     }
     if (a.deph) {
-        Tile U = load_tile(Ust + k * nn, N, true);
-        Tile Ut = transpose(U, buf);
-        if (!sg.ureuse) {  // eigenvectors at this stage time, warm-started
-            const Tile H = load_tile(mats + a.nk_stride * nn + k * nn, N);  // mat 1: H
-            eig_refine(H, U, Ut, N, buf);
-            store_tile(Ust + k * nn, U, N);
-        }
         const Tile G = cmm(Y, true, U);  // y U
         const double d = col_dot(U, G);  // (U^dag y U)_cc
         Tile Ud;  // (U diag(d))^T's tile: U^T[4kb + g][c] d_{4kb+g}
@@ -349,22 +324,76 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void nb
             X.im[r] -= g2 * Y.im[r] + g12 * M.im[r];
         }
     }
+    return X;
+}
+
+// Stage i: y = rho + sum_j dt a_ij k_j; k_i = f(y); after the last stage
+// rho += sum_i dt b_i k_i. PAIR: stage sg and the next stage sg2 at the same
+// time (RK4's c = 1/2, 1/2) in one pass, sharing H~, U and rho in registers;
+// sg2's input takes sg's derivative from registers.
+template <bool PAIR>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PAIR ? 2 : 3))) void nb16_rhs_kernel(
+    const NbArgs a, const NbStage sg, const NbStage sg2, const cd *__restrict__ mats, cd *__restrict__ rho,
+    cd *__restrict__ kbuf, cd *__restrict__ Ust)
+{
+    __shared__ cd buf[16 * kTpad];
+    const int64_t k = blockIdx.x;
+    if (k >= a.nk) return;
+    const int N = a.N, nn = N * N;
+    const int64_t kstride = a.nk_stride * nn;
+    cd *rk = rho + k * nn;
+    // rho is read again where needed rather than held across the stages (registers)
+    Tile Y = load_tile(rk, N);
+    for (int j = 0; j < sg.stage; ++j) {
+        if (sg.a[j] == 0.0) continue;
+        const Tile Kj = load_tile(kbuf + j * kstride + k * nn, N);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            Y.re[r] += sg.a[j] * Kj.re[r];
+            Y.im[r] += sg.a[j] * Kj.im[r];
+        }
+    }
+    const Tile HtT = load_tile_t(mats + k * nn, N);  // mat 0: H~ = H + E.D, transposed (A operand)
+    Tile U, Ut;
+    if (a.deph) {
+        U = load_tile(Ust + k * nn, N, true);
+        Ut = transpose(U, buf);
+        if (!sg.ureuse) {  // eigenvectors at this stage time, warm-started
+            const Tile H = load_tile(mats + a.nk_stride * nn + k * nn, N);  // mat 1: H
+            eig_refine(H, U, Ut, N, buf);
+            store_tile(Ust + k * nn, U, N);
+        }
+    }
+    Tile X = stage_derivative(a, Y, HtT, U, Ut, buf);
     store_tile(kbuf + sg.stage * kstride + k * nn, X, N);
-    if (sg.last) {
-        Tile R = load_tile(rk, N);
-        for (int j = 0; j < sg.nstage; ++j) {
-            if (sg.b[j] == 0.0) continue;
+    const NbStage &fin = PAIR ? sg2 : sg;  // the stage whose derivative X holds at the end
+    if (PAIR) {
+        Tile Y2 = load_tile(rk, N);
+        for (int j = 0; j < sg2.stage; ++j) {
+            if (sg2.a[j] == 0.0) continue;
             const Tile Kj = j == sg.stage ? X : load_tile(kbuf + j * kstride + k * nn, N);
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                R.re[r] += sg.b[j] * Kj.re[r];
-                R.im[r] += sg.b[j] * Kj.im[r];
+                Y2.re[r] += sg2.a[j] * Kj.re[r];
+                Y2.im[r] += sg2.a[j] * Kj.im[r];
             }
         }
-        store_tile(rk, R, N);
+        X = stage_derivative(a, Y2, HtT, U, Ut, buf);
+        store_tile(kbuf + sg2.stage * kstride + k * nn, X, N);
     }
-    (void)c;
-    (void)g;
+    if (fin.last) {
+        Tile Rn = load_tile(rk, N);
+        for (int j = 0; j < fin.nstage; ++j) {
+            if (fin.b[j] == 0.0) continue;
+            const Tile Kj = j == fin.stage ? X : load_tile(kbuf + j * kstride + k * nn, N);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                Rn.re[r] += fin.b[j] * Kj.re[r];
+                Rn.im[r] += fin.b[j] * Kj.im[r];
+            }
+        }
+        store_tile(rk, Rn, N);
+    }
 }
 
 // ------------------------------------------------------------ sample --
@@ -579,7 +608,15 @@ hipError_t nb16_launch_rhs(const NbArgs &a, const NbStage &sg, const double2 *ma
                            double2 *U, hipStream_t s)
 {
     if (a.N > 16) return hipErrorInvalidValue;
-    nb16_rhs_kernel<<<dim3(static_cast<unsigned>(a.nk)), 64, 0, s>>>(a, sg, mats, rho, kbuf, U);
+    nb16_rhs_kernel<false><<<dim3(static_cast<unsigned>(a.nk)), 64, 0, s>>>(a, sg, sg, mats, rho, kbuf, U);
+    return hipGetLastError();
+}
+
+hipError_t nb16_launch_rhs_pair(const NbArgs &a, const NbStage &sg, const NbStage &sg2, const double2 *mats,
+                                double2 *rho, double2 *kbuf, double2 *U, hipStream_t s)
+{
+    if (a.N > 16) return hipErrorInvalidValue;
+    nb16_rhs_kernel<true><<<dim3(static_cast<unsigned>(a.nk)), 64, 0, s>>>(a, sg, sg2, mats, rho, kbuf, U);
     return hipGetLastError();
 }
 

@@ -387,9 +387,25 @@ int nb_run(sbe_ctx *ctx, int64_t step_begin, int64_t step_end)
                         ctx->nb_h_valid = true;
                     }
                 }
-                if (!rc)
+                // the next stage at the same time (RK4's c = 1/2, 1/2) runs in the same pass on
+                // the register-tile kernels, sharing H~, U and rho
+                const bool pair = a.tile16 && i + 1 < tb.nstage && tb.c[i + 1] == tb.c[i];
+                if (!rc && pair) {
+                    NbStage sg2{};
+                    sg2.stage = i + 1;
+                    sg2.nstage = tb.nstage;
+                    sg2.last = i + 1 == tb.nstage - 1;
+                    sg2.ureuse = 1;
+                    for (int j = 0; j <= i; ++j) sg2.a[j] = dt * tb.a[i + 1][j];
+                    for (int j = 0; j < tb.nstage; ++j) sg2.b[j] = dt * tb.b[j];
+                    rc = hipc(ctx, nb16_launch_rhs_pair(a, sg, sg2, ctx->nb_mats, ctx->nb_rho, ctx->nb_k, ctx->nb_U,
+                                                        ctx->stream),
+                              "rhs kernel");
+                    ++i;
+                } else if (!rc) {
                     rc = hipc(ctx, nb_launch_rhs(a, sg, ctx->nb_mats, ctx->nb_rho, ctx->nb_k, ctx->nb_U, ctx->stream),
                               "rhs kernel");
+                }
                 if (a.deph) ctx->nb_u_time = ts;
             }
         }
