@@ -155,6 +155,14 @@ def bench_wannier(args) -> int:
     n = res["fp64"]["steps"]
     forms["split16"]["err_j_rel_l2_vs_fp64"] = rel_l2((b["j_inter"] + b["j_intra"])[:n], (a["j_inter"] + a["j_intra"])[:n])
     forms["split16"]["err_pop_rel_l2_vs_fp64"] = rel_l2(b["band_pop"][:n], a["band_pop"][:n])
+    forms["split16"]["err_rho_bar_rel_l2_vs_fp64"] = rel_l2(b["rho_bar"][:n], a["rho_bar"][:n])
+    inf = res["fp64"]["info"]
+    if n == inf.nt:  # the whole pulse: harmonic intensities too
+        o, S64 = hhg_spectrum(a["j_inter"] * inf.dV, a["j_intra"] * inf.dV, inf.dt, 0.014)
+        _, S16 = hhg_spectrum(b["j_inter"] * inf.dV, b["j_intra"] * inf.dV, inf.dt, 0.014)
+        db, hcut = harmonic_error_db(o, S16, S64)
+        forms["split16"]["err_hhg_db_vs_fp64"] = db
+        forms["split16"]["hhg_cutoff_harmonic"] = hcut
     r = res["fp64"]
     ks = forms["fp64"]["k_steps_per_s"]
     # flops per k-point RK4 step. Executed: 7 Fourier matrix sums (8 N^2 N_R each: the sample's
