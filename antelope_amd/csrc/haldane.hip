@@ -41,6 +41,13 @@ constexpr int kTile = 32;  // time steps per LDS tile of the phase table
 //   f/2 = d x s - (g2/2)(s - s_par) - (g1/2) s_par,  dn/2 = -(g1/2) n,
 // with the factor 2 folded into the RK coefficients (dt a_ij, dt b_i).
 // The NN part of the phase table carries the factor t1.
+//
+// d(k+A) is formed once per distinct stage time: stages at an equal c share
+// it (RK4: c = 1/2 twice), and the stage at c = 1 is the next step's sample
+// time. The table row of that stage equals the next step's first row bit for
+// bit (the host evaluates both at t_{n+1}), so with kCarry its phases (and
+// d, 1/|d|^2) serve the next sample and stage 0, and a run split into any
+// launches gives the same bits as one launch.
 // blk: block index within the k block of this solve.
 template <typename R, int RK, bool T1ON, bool DEPH, int KPT>
 __device__ __forceinline__ void steps_body(const HaldaneArgs &a, const R *__restrict__ tabr, R *__restrict__ st,
@@ -49,6 +56,8 @@ __device__ __forceinline__ void steps_body(const HaldaneArgs &a, const R *__rest
     using C = Coef<RK>;
     constexpr int S = C::S;
     constexpr bool kComp = StateT<R>::kComp != 0;
+    constexpr int kOne = stage_at_one<RK>();
+    constexpr bool kCarry = kOne > 0;
     const int64_t ks = a.nk_stride;
     const int64_t lane0 = blk * (kBlock * KPT) + threadIdx.x;
     const int wave = static_cast<int>((blk * kBlock + threadIdx.x) >> 6);
@@ -117,18 +126,27 @@ __device__ __forceinline__ void steps_body(const HaldaneArgs &a, const R *__rest
     stash(0);
     __syncthreads();
 
+    Phases<R> ph[KPT];  // at this step's sample time t_n
+    if (kCarry) {
+#pragma unroll
+        for (int q = 0; q < KPT; ++q) ph[q] = phases_at(p[q], &lds_tab[0][0], c3, M0);
+    }
+
     for (int tile = 0; tile < ntiles; ++tile) {
     if (tile + 1 < ntiles) fetch(tile + 1);  // in flight during this tile
     const int s_end = min(kTile, a.nsteps - tile * kTile);
     for (int st_ = 0; st_ < s_end; ++st_) {
         const int step = tile * kTile + st_;
         const R *tr = &lds_tab[tile & 1][st_ * ROW];
-        Vec3<R> dstage0[KPT];
+        if (!kCarry) {
+#pragma unroll
+            for (int q = 0; q < KPT; ++q) ph[q] = phases_at(p[q], tr, c3, M0);
+        }
 
         // ---- time sample at t_n from the state before the step
         double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
-        for (int q = 0; q < KPT; ++q) dstage0[q] = time_sample<R>(p[q], tr, a, ii, v);
+        for (int q = 0; q < KPT; ++q) time_sample<R>(p[q], ph[q], a, ii, v);
         {
             const double r = wave_reduce8(v);
             // lanes 8q..8q+7 all hold value q: duplicate stores of equal data keep the step branch-free
@@ -140,6 +158,8 @@ __device__ __forceinline__ void steps_body(const HaldaneArgs &a, const R *__rest
         for (int q = 0; q < KPT; ++q) {
             R kn[S];
             Vec3<R> kv[S];
+            Vec3<R> ds_[S];  // d at each stage time
+            R inv_[S];       // 1/|d|^2 at each stage time
 #pragma unroll
             for (int i = 0; i < S; ++i) {
                 R yn = p[q].n;
@@ -155,8 +175,18 @@ __device__ __forceinline__ void steps_body(const HaldaneArgs &a, const R *__rest
                     }
                 }
                 Vec3<R> d;
+                R inv;
+                const int same = same_time_stage<RK>(i);
                 if (i == 0) {
-                    d = dstage0[q];
+                    d = ph[q].d;
+                    inv = ph[q].inv;
+                } else if (same > 0) {
+                    d = ds_[same];
+                    inv = inv_[same];
+                } else if (kCarry && i == kOne) {
+                    ph[q] = phases_at(p[q], tr + i * 12, c3, M0);  // the next step's t_n
+                    d = ph[q].d;
+                    inv = ph[q].inv;
                 } else {
                     const R *row = tr + i * 12;
                     R dx = 0, dy = 0, sb = 0;
@@ -167,18 +197,17 @@ __device__ __forceinline__ void steps_body(const HaldaneArgs &a, const R *__rest
                         sb += p[q].er[j + 3] * row[7 + 2 * j] + p[q].ei[j + 3] * row[6 + 2 * j];
                     }
                     d = {dx, dy, M0 - c3 * sb};
+                    const R dd = d.x * d.x + d.y * d.y + d.z * d.z;
+                    inv = dd > R(0) ? rcp_nr(dd) : R(0);
                 }
+                ds_[i] = d;
+                inv_[i] = inv;
                 // f/2 = d x y - (g2/2) y + ((g2-g1)/2) (y.d / |d|^2) d
                 Vec3<R> f;
                 f.x = d.y * y.z - d.z * y.y;
                 f.y = d.z * y.x - d.x * y.z;
                 f.z = d.x * y.y - d.y * y.x;
                 if (DEPH) {
-                    const R dd = d.x * d.x + d.y * d.y + d.z * d.z;
-                    R inv = __builtin_amdgcn_rcp(dd);  // reciprocal with two Newton steps
-                    inv = inv + inv * (R(1) - dd * inv);
-                    inv = inv + inv * (R(1) - dd * inv);
-                    inv = dd > R(0) ? inv : R(0);
                     const R c = -gh12 * (y.x * d.x + y.y * d.y + y.z * d.z) * inv;
                     f.x += c * d.x - gh2 * y.x;
                     f.y += c * d.y - gh2 * y.y;

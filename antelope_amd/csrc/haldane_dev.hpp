@@ -14,6 +14,7 @@ template <> struct Coef<4> {
     static constexpr int S = 4;
     static constexpr double a[4][4] = {{0, 0, 0, 0}, {0.5, 0, 0, 0}, {0, 0.5, 0, 0}, {0, 0, 1.0, 0}};
     static constexpr double b[4] = {1.0 / 6.0, 2.0 / 6.0, 2.0 / 6.0, 1.0 / 6.0};
+    static constexpr double c[4] = {0.0, 0.5, 0.5, 1.0};
 };
 template <> struct Coef<5> {
     static constexpr int S = 6;
@@ -25,7 +26,25 @@ template <> struct Coef<5> {
         {439. / 216., -8., 3680. / 513., -845. / 4104., 0, 0},
         {-8. / 27., 2., -3544. / 2565., 1859. / 4104., -11. / 40., 0}};
     static constexpr double b[6] = {16. / 135., 0, 6656. / 12825., 28561. / 56430., -9. / 50., 2. / 55.};
+    static constexpr double c[6] = {0.0, 0.25, 3. / 8., 12. / 13., 1.0, 0.5};
 };
+
+// Stage bookkeeping of a tableau: the stage at c = 1 (the next step's sample
+// time), or -1; for each stage, an earlier stage (>= 1) at the same c, or -1.
+template <int RK>
+__host__ __device__ constexpr int stage_at_one()
+{
+    for (int i = 1; i < Coef<RK>::S; ++i)
+        if (Coef<RK>::c[i] == 1.0) return i;
+    return -1;
+}
+template <int RK>
+__host__ __device__ constexpr int same_time_stage(int i)
+{
+    for (int j = 1; j < i; ++j)
+        if (Coef<RK>::c[j] == Coef<RK>::c[i]) return j;
+    return -1;
+}
 
 // --------------------------------------------------- wave-level reduction
 template <int CTRL>
@@ -112,6 +131,27 @@ template <> struct StateT<float> {
     static constexpr int kComp = 1;
 };
 
+// explicit fused multiply-add: the same inputs give the same bits wherever
+// it is inlined (no contraction left to the compiler)
+__device__ __forceinline__ float fmar(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmar(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// 1/x from the hardware reciprocal with Newton steps (two in FP64, one in FP32)
+__device__ __forceinline__ double rcp_nr(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    r = fmar(r, fmar(-x, r, 1.0), r);
+    return fmar(r, fmar(-x, r, 1.0), r);
+}
+__device__ __forceinline__ float rcp_nr(float x)
+{
+    const float r = __builtin_amdgcn_rcpf(x);
+    return fmar(r, fmar(-x, r, 1.0f), r);
+}
+// 1/sqrt(x): the FP32 hardware rsqrt (1 ulp), the library one in FP64
+__device__ __forceinline__ double rsqrt_r(double x) { return rsqrt(x); }
+__device__ __forceinline__ float rsqrt_r(float x) { return __builtin_amdgcn_rsqf(x); }
+
 // complex product (a + ib)(c + id)
 template <typename R>
 __device__ __forceinline__ void cmul(R a, R b, R c, R d, R &re, R &im)
@@ -129,45 +169,68 @@ struct KPoint {
     double w;
 };
 
-// Time sample of one k point at t_n, from the given state (antelope.cpp:307-333 with
-// SBEs.h:694-760): adds w x (inter_x, inter_y, intra_x, intra_y, s_x, s_y,
-// s_z, s.dhat) to v (FP64) and returns d(k + A(t_n)) for RK stage 0.
+// Everything a k point needs at one time t: the phases e^{i(k+A(t)).v_j},
+// d(k+A), |d|^2 and 1/|d|^2 (0 at a gap closing). One function computes it
+// wherever it is needed, with explicit FMAs, so equal inputs give equal bits.
+template <typename R>
+struct Phases {
+    R ar[3], ai[3];  // t1 e^{i(k+A).a_j}
+    R br[3], bi[3];  // e^{i(k+A).b_j}
+    Vec3<R> d;
+    R dd, inv;
+};
+
+template <typename R>
+__device__ __forceinline__ Phases<R> phases_at(const KPoint<R> &p, const R *row, R c3, R M0)
+{
+    Phases<R> e;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        e.ar[j] = fmar(p.er[j], row[2 * j], -(p.ei[j] * row[2 * j + 1]));
+        e.ai[j] = fmar(p.er[j], row[2 * j + 1], p.ei[j] * row[2 * j]);
+        e.br[j] = fmar(p.er[j + 3], row[6 + 2 * j], -(p.ei[j + 3] * row[7 + 2 * j]));
+        e.bi[j] = fmar(p.er[j + 3], row[7 + 2 * j], p.ei[j + 3] * row[6 + 2 * j]);
+    }
+    e.d.x = e.ar[0] + e.ar[1] + e.ar[2];
+    e.d.y = e.ai[0] + e.ai[1] + e.ai[2];
+    e.d.z = fmar(-c3, e.bi[0] + e.bi[1] + e.bi[2], M0);
+    e.dd = fmar(e.d.x, e.d.x, fmar(e.d.y, e.d.y, e.d.z * e.d.z));
+    e.inv = e.dd > R(0) ? rcp_nr(e.dd) : R(0);
+    return e;
+}
+
+// Time sample of one k point at t_n (antelope.cpp:307-333 with
+// SBEs.h:694-760), from the state nn, s and the phases at t_n: adds
+// w x (inter_x, inter_y, intra_x, intra_y, s_x, s_y, s_z, s.dhat) to v (FP64).
 // Evaluated in the state precision R; the k sum is always FP64. Branch-free
 // so the scheduler can interleave it with the RK chain.
 template <typename R>
-__device__ __forceinline__ Vec3<R> sample_state(const KPoint<R> &p, R nn, R s_x, R s_y, R s_z, const R *row,
-                                                const HaldaneArgs &a, R ii, double v[8])
+__device__ __forceinline__ void sample_state(const Phases<R> &e, double w, R nn, R s_x, R s_y, R s_z,
+                                             const HaldaneArgs &a, R ii, double v[8])
 {
-    R ar[3], ai[3], br[3], bi[3];
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-        cmul(p.er[j], p.ei[j], row[2 * j], row[2 * j + 1], ar[j], ai[j]);  // t1 e^{i(k+A).a_j}
-        cmul(p.er[j + 3], p.ei[j + 3], row[6 + 2 * j], row[7 + 2 * j], br[j], bi[j]);
-    }
     const R c3 = static_cast<R>(a.c3);
-    Vec3<R> d{ar[0] + ar[1] + ar[2], ai[0] + ai[1] + ai[2], static_cast<R>(a.M0) - c3 * (bi[0] + bi[1] + bi[2])};
+    const Vec3<R> &d = e.d;
     // derivatives d/dk_x, d/dk_y of (d0, dx, dy, dz)
     R xx = 0, xy = 0, yx = 0, yy = 0, zx = 0, zy = 0, ox = 0, oy = 0;
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
         const R ax = static_cast<R>(a.va[j][0]), ay = static_cast<R>(a.va[j][1]);
         const R bx = static_cast<R>(a.vb[j][0]), by = static_cast<R>(a.vb[j][1]);
-        xx -= ax * ai[j];
-        xy -= ay * ai[j];
-        yx += ax * ar[j];
-        yy += ay * ar[j];
-        zx += bx * br[j];
-        zy += by * br[j];
-        ox += bx * bi[j];
-        oy += by * bi[j];
+        xx -= ax * e.ai[j];
+        xy -= ay * e.ai[j];
+        yx += ax * e.ar[j];
+        yy += ay * e.ar[j];
+        zx += bx * e.br[j];
+        zy += by * e.br[j];
+        ox += bx * e.bi[j];
+        oy += by * e.bi[j];
     }
     zx *= -c3;
     zy *= -c3;
     const R c0 = static_cast<R>(a.c0);
     ox *= -c0;
     oy *= -c0;
-    const R dd = d.x * d.x + d.y * d.y + d.z * d.z;
-    const R inv = dd > R(0) ? R(1) / dd : R(0);
+    const R inv = e.inv;
     const R sd = s_x * d.x + s_y * d.y + s_z * d.z;
     const R tx = s_x * xx + s_y * yx + s_z * zx;  // s . d_x d
     const R ty = s_x * xy + s_y * yy + s_z * zy;
@@ -176,7 +239,6 @@ __device__ __forceinline__ Vec3<R> sample_state(const KPoint<R> &p, R nn, R s_x,
     // InterIntra (ii = 1): inter = -(s.d_a d - px), intra = -(n d_a d0 + px)
     // GenCurrent  (ii = 0): inter = -(n d_a d0 + s.d_a d), intra = 0
     const R ni = R(1) - ii;
-    const double w = p.w;
     v[0] -= static_cast<double>(tx - ii * px + ni * nn * ox) * w;
     v[1] -= static_cast<double>(ty - ii * py + ni * nn * oy) * w;
     v[2] -= static_cast<double>(ii * (nn * ox + px)) * w;
@@ -184,15 +246,14 @@ __device__ __forceinline__ Vec3<R> sample_state(const KPoint<R> &p, R nn, R s_x,
     v[4] += static_cast<double>(s_x) * w;
     v[5] += static_cast<double>(s_y) * w;
     v[6] += static_cast<double>(s_z) * w;
-    v[7] += static_cast<double>(dd > R(0) ? sd * rsqrt(dd) : -s_z) * w;
-    return d;
+    v[7] += static_cast<double>(e.dd > R(0) ? sd * rsqrt_r(e.dd) : -s_z) * w;
 }
 
 template <typename R>
-__device__ __forceinline__ Vec3<R> time_sample(const KPoint<R> &p, const R *row, const HaldaneArgs &a, R ii,
-                                               double v[8])
+__device__ __forceinline__ void time_sample(const KPoint<R> &p, const Phases<R> &e, const HaldaneArgs &a, R ii,
+                                            double v[8])
 {
-    return sample_state(p, p.n - p.cn, p.sx - p.cx, p.sy - p.cy, p.sz - p.cz, row, a, ii, v);
+    sample_state(e, p.w, p.n - p.cn, p.sx - p.cx, p.sy - p.cy, p.sz - p.cz, a, ii, v);
 }
 
 }  // namespace dev

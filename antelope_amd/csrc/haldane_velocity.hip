@@ -204,6 +204,7 @@ __device__ __forceinline__ void vel_body(const HaldaneArgs &a, const R *__restri
 #pragma unroll
         for (int q = 0; q < KPT; ++q) {
             Vec3<R> kv[S];
+            Vec3<R> hs[S];  // h at each stage time (stages at an equal c share it)
 #pragma unroll
             for (int i = 0; i < S; ++i) {
                 Vec3<R> y{p[q].sx, p[q].sy, p[q].sz};
@@ -216,9 +217,16 @@ __device__ __forceinline__ void vel_body(const HaldaneArgs &a, const R *__restri
                         y.z += aij * kv[j].z;
                     }
                 }
-                const R Ax = tr[2 * i], Ay = tr[2 * i + 1];
-                const Vec3<R> h{-(Ax * p[q].g[0][0] + Ay * p[q].g[1][0]), -(Ax * p[q].g[0][1] + Ay * p[q].g[1][1]),
-                                -p[q].r - (Ax * p[q].g[0][2] + Ay * p[q].g[1][2])};
+                const int same = same_time_stage<RK>(i);
+                Vec3<R> h;
+                if (same > 0) {
+                    h = hs[same];
+                } else {
+                    const R Ax = tr[2 * i], Ay = tr[2 * i + 1];
+                    h = {-(Ax * p[q].g[0][0] + Ay * p[q].g[1][0]), -(Ax * p[q].g[0][1] + Ay * p[q].g[1][1]),
+                         -p[q].r - (Ax * p[q].g[0][2] + Ay * p[q].g[1][2])};
+                }
+                hs[i] = h;
                 Vec3<R> f;
                 f.x = h.y * y.z - h.z * y.y;
                 f.y = h.z * y.x - h.x * y.z;

@@ -452,9 +452,12 @@ int sbe_create(const sbe_config *cfg, const sbe_model *model, int64_t k_begin, i
         std::vector<float> tab32(n64);
         for (int64_t n = 0; n < nt; ++n) {
             const double tn = ctx->laser.atmin + ctx->laser.dt * static_cast<double>(n);  // antelope.cpp:299
+            const double tn1 = ctx->laser.atmin + ctx->laser.dt * static_cast<double>(n + 1);
             for (int s = 0; s < S; ++s) {
                 double A[3], E[3];
-                ctx->laser.eval(tn + ctx->tab.c[s] * ctx->laser.dt, A, E);  // SBEs.h:612
+                // SBEs.h:612; the stage at c = 1 is taken at t_{n+1} itself, so its row is the
+                // next step's first row bit for bit (the step kernels carry it across steps)
+                ctx->laser.eval(ctx->tab.c[s] == 1.0 ? tn1 : tn + ctx->tab.c[s] * ctx->laser.dt, A, E);
                 if (ctx->velocity) {  // A at the stage times, [nt][S][2]
                     double *row = tab.data() + (static_cast<size_t>(n) * S + s) * 2;
                     row[0] = A[0];

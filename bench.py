@@ -38,15 +38,16 @@ PEAK_HBM_GBS = 8000.0
 # complex density matrix): 128 B in FP64, 64 B with FP32 state.
 ALGO_BYTES_PER_KSTEP = {"fp64": 128.0, "fp32": 64.0}
 # Executed flops per k-point RK4 step of the step kernel, from the rocprofv3
-# counters SQ_INSTS_VALU_{FMA,MUL,ADD}_F64 (FMA = 2) on this workload (the
-# hand count of the algorithm gives the same figure).
-FLOPS_PER_KSTEP = {"fp64": 500.0, "fp32": 522.0}
+# counters SQ_INSTS_VALU_{FMA,MUL,ADD}_{F64,F32} (FMA = 2) on this workload:
+# FP64 385; FP32 state 365 FP32 + 26 FP64 (the k sums).
+FLOPS_PER_KSTEP = {"fp64": 385.0, "fp32": 391.0}
 KERNEL_NAME = {"fp64": "haldane_steps_kernel<double, 4, false, true, 2>",
                "fp32": "haldane_steps_kernel<float, 4, false, true, 1>"}
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
 # Flops per k-point RK4 step of the velocity-gauge step kernel (hand count:
-# sample 36, four stages of input 6 + h 10 + h x y 12 + dephasing 4, update 27)
-VEL_FLOPS_PER_KSTEP = {"fp64": 190.0, "fp32": 190.0}
+# sample 36, four stages of input 6 + h x y 12 + dephasing 4, h 10 at the three
+# distinct stage times, update 27)
+VEL_FLOPS_PER_KSTEP = {"fp64": 181.0, "fp32": 181.0}
 # Executed FP64 flops per accepted k-point step of the adaptive Cash-Karp FP64
 # kernel (one k point per lane), from SQ_INSTS_VALU_{FMA,MUL,ADD}_F64 (FMA = 2)
 # on a batch of this workload's solves; includes rejected attempts and the
