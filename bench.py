@@ -322,10 +322,11 @@ def bench_batch(args) -> int:
     """The batched configuration: 64 seeded Haldane solves on 200^2 grids with
     uneven pulses (6..16 cycles), one launch per window for the whole batch.
     Timed on the same batch: the plain FP64 fixed-step RK4 (dt = 0.25, the
-    reference's method) and the adaptive Cash-Karp forms, FP64 (rtol 1e-8,
+    reference's method), the same RK4 with FP32 state (FP64 sums) and the
+    adaptive Cash-Karp forms, FP64 (rtol 1e-8,
     atol --atol, default 1e-9) and FP32 state with FP64 error control and
     sums (rtol 1e-8, atol --atol32, default 1e-9). Throughput counts oracle-equivalent steps,
-    sum_i nk_i Nt_i, for every form; each adaptive form's error is measured
+    sum_i nk_i Nt_i, for every form; each other form's error is measured
     per solve against the FP64 plain run, and `value` is the fastest form
     whose worst solve stays within 1e-6 relative L2 in j(t)."""
     from antelope_amd.batch_workloads import haldane_batch
@@ -341,8 +342,12 @@ def bench_batch(args) -> int:
     v_plain = eq / plain["wall_s"]
     print(f"[bench] batch plain fp64 RK4: {plain['wall_s']:.3f} s", file=sys.stderr, flush=True)
     forms = {}
-    for name, prec, atol in (("adaptive_fp64", "fp64", args.atol), ("adaptive_fp32", "fp32", args.atol32)):
-        cfgs, _ = haldane_batch(n=n, grid=grid, rk_order=45, precision=prec, rtol=args.rtol, atol=atol)
+    for name, prec, atol in (("plain_fp32_rk4", "fp32", None), ("adaptive_fp64", "fp64", args.atol),
+                             ("adaptive_fp32", "fp32", args.atol32)):
+        if atol is None:
+            cfgs, _ = haldane_batch(n=n, grid=grid, rk_order=4, precision=prec)
+        else:
+            cfgs, _ = haldane_batch(n=n, grid=grid, rk_order=45, precision=prec, rtol=args.rtol, atol=atol)
         r = _timed_batch(cfgs, models)
         errs, perrs, dbs = [], [], []
         for i in range(n):
@@ -360,7 +365,8 @@ def bench_batch(args) -> int:
         blocks = sum(st.blocks for st in r["stats"])
         kpb = r["stats"][0].k_per_block
         forms[name] = {"k_steps_per_s": eq / r["wall_s"], "wall_s": r["wall_s"], "speedup_vs_plain_fp64": (eq / r["wall_s"]) / v_plain,
-                       "step_kernel_ms": r["timing"]["step_ms"], "rtol": args.rtol, "atol": atol,
+                       "step_kernel_ms": r["timing"]["step_ms"],
+                       "rtol": args.rtol if atol is not None else None, "atol": atol,
                        "accepted_block_steps": acc, "rejected_block_steps": sum(st.rejected for st in r["stats"]),
                        "blocks": blocks, "k_per_block": kpb, "mean_step_over_dt": eq / max(acc * kpb, 1),
                        "err_j_rel_l2_vs_fp64_plain": {"max": max(errs), "median": float(np.median(errs)),
@@ -376,14 +382,17 @@ def bench_batch(args) -> int:
     form = best if best and forms[best]["k_steps_per_s"] > v_plain else "plain_fp64_rk4"
     f = forms.get(form)
     kt = (f["step_kernel_ms"] if f else plain["timing"]["step_ms"]) * 1e-3
-    abytes = ALGO_BYTES_PER_KSTEP["fp32" if form == "adaptive_fp32" else "fp64"]
+    abytes = ALGO_BYTES_PER_KSTEP["fp32" if form in ("adaptive_fp32", "plain_fp32_rk4") else "fp64"]
     ach = eq * abytes / kt / 1e9
     compute = None
-    if form == "plain_fp64_rk4":
-        fl = eq * FLOPS_PER_KSTEP["fp64"] / kt / 1e12
-        compute = {"bound": "valu", "kernel": "haldane_batch_kernel<double, 4, false, true>", "achieved": fl,
+    kname = {"plain_fp64_rk4": "haldane_batch_kernel<double, 4, false, true>",
+             "plain_fp32_rk4": "haldane_batch_kernel<float, 4, false, true>"}.get(form, "haldane_adaptive_kernel")
+    if form in ("plain_fp64_rk4", "plain_fp32_rk4"):
+        fpk = FLOPS_PER_KSTEP["fp64" if form == "plain_fp64_rk4" else "fp32"]
+        fl = eq * fpk / kt / 1e12
+        compute = {"bound": "valu", "kernel": kname, "achieved": fl,
                    "peak": PEAK_TFLOPS["fp64"], "unit": "TFLOP/s", "frac": fl / PEAK_TFLOPS["fp64"],
-                   "flops_per_kstep": FLOPS_PER_KSTEP["fp64"]}
+                   "flops_per_kstep": fpk}
     elif form == "adaptive_fp64":
         ksa = f["accepted_block_steps"] * f["k_per_block"]
         fl = ksa * ADAPT_FLOPS_PER_KSTEP["fp64"] / kt / 1e12
@@ -394,7 +403,8 @@ def bench_batch(args) -> int:
             "n_gpus": 1, "steps": int(max(inf.nt for inf in plain["info"])), "warmup": 1,
             "ms_per_step": eq / value * 1e3 / (n * grid * grid),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-            "dtype": "f32 state, f64 error control and sums" if form == "adaptive_fp32" else "f64",
+            "dtype": {"adaptive_fp32": "f32 state, f64 error control and sums",
+                      "plain_fp32_rk4": "f32 state, f64 sums"}.get(form, "f64"),
             "data": "synthetic (seeded SplitMix64 batch of Haldane models and pulses)",
             "config": {"workload": f"batched: {n} Haldane solves on {grid}x{grid} grids, per solve E0 log-U[0.001,0.006], "
                                    f"ellipticity U[0,1], CEP U[0,2pi), phi0 U[0,pi/2], M/t2 U[0,4], 6..16 cycles",
@@ -402,7 +412,7 @@ def bench_batch(args) -> int:
             "plain_fp64_rk4": {"k_steps_per_s": v_plain, "wall_s": plain["wall_s"],
                                "step_kernel_ms": plain["timing"]["step_ms"]},
             "forms": forms,
-            "roofline": {"bound": "hbm", "kernel": "haldane_adaptive_kernel" if f else "haldane_batch_kernel",
+            "roofline": {"bound": "hbm", "kernel": kname,
                          "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": ach / PEAK_HBM_GBS,
                          "traffic": None, "algo_bytes_per_kstep": abytes,
                          "note": "oracle-equivalent k-steps x algorithmic state bytes over the step-kernel time; the "
