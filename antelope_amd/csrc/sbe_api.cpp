@@ -44,6 +44,12 @@ int hip_check(sbe_ctx *ctx, hipError_t e, const char *what)
 // of a batch use a smaller one).
 constexpr size_t kPartialBytes = size_t(128) << 20;
 constexpr size_t kBatchPartialBytes = size_t(24) << 20;
+// Large grids: every fixed-step launch reads the state and the k points once
+// and recomputes e^{ik.v} (six sincos per k point), so a launch keeps at least
+// kMinChunkSteps steps while the buffer stays under kPartialBytesMax (a 4096^2
+// grid in FP32: 64 steps, 1.07 GB, against 8 steps at the 128 MiB bound).
+constexpr int64_t kMinChunkSteps = 64;
+constexpr size_t kPartialBytesMax = size_t(2) << 30;
 // The adaptive stepper ends every launch exactly on a sample time, so launch
 // boundaries shape its step sequence. Its windows are therefore a fixed
 // property of the method, [0, W), [W, 2W), ..., for single solves, batches and
@@ -150,7 +156,9 @@ int make_plan(const sbe_config &cfg, const sbe_model &m, int64_t nk, int64_t nt,
     p.nt = nt;
     p.nstage = nstage;
     const size_t row = static_cast<size_t>(p.nwaves) * p.nv * sizeof(double);
-    p.chunk = std::max<int64_t>(1, std::min<int64_t>(nt, static_cast<int64_t>(t_partial_bytes / row)));
+    int64_t by_bytes = static_cast<int64_t>(t_partial_bytes / row);
+    if (by_bytes < kMinChunkSteps) by_bytes = std::min<int64_t>(kMinChunkSteps, static_cast<int64_t>(kPartialBytesMax / row));
+    p.chunk = std::max<int64_t>(1, std::min<int64_t>(nt, by_bytes));
     if (cfg.rk_order == SBE_RK_CASH_KARP) p.chunk = std::min<int64_t>(nt, kAdaptiveWindow);
     const size_t real = cfg.precision == SBE_PREC_FP32 ? sizeof(float) : sizeof(double);
     const int comps = cfg.precision == SBE_PREC_FP32 ? 8 : 4;
