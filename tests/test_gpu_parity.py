@@ -104,6 +104,34 @@ def test_split_runs_equal_single_run():
     assert np.array_equal(sa, sb)
 
 
+def test_large_grid_long_launches_split_equal():
+    """2,359,296 k points with FP32 state: one step's partial rows (2.4 MB) fit
+    fewer than 64 times in 128 MiB, so a launch covers 64 steps in a larger
+    buffer, and the series take the two-pass reduction (36,864 rows per step).
+    A run split at odd steps gives the same bits as one run, and the single run
+    of 130 steps is three launches (64 + 64 + 2)."""
+    cfg = small_config(n=1536, ncycles=0.5, precision="fp32")
+    model = haldane_cfg1_model()
+
+    def run(bounds):
+        with SBESolver(cfg, model) as s:
+            s.set_timing(True)
+            b = 0
+            for e in bounds:
+                s.run(b, e)
+                b = e
+            s.synchronize()
+            return s.series(), s.state(), s.timing()
+
+    a, sa, ta = run([130])
+    assert ta["step_launches"] == 3, ta
+    b, sb, _ = run([1, 64, 65, 130])
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+    assert np.array_equal(sa, sb)
+    assert np.abs(a["j_inter"][1:130]).max() > 0  # the pulse has started
+
+
 def test_single_k_point(oracle):
     cfg = small_config(n=(1, 1))
     model = haldane_cfg1_model()
