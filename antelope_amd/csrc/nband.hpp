@@ -44,6 +44,16 @@ struct NbArgs {
     int model_kind;
     double mparams[24];
     int tile16;            // N <= 16: stage and sample kernels with the matrices in MFMA registers
+    // Separable Fourier sums (FP64 form, product k grids): R = n1 a1 + n2 a2 + n3 a3 on the
+    // lattice box lo_i <= n_i < lo_i + M_i, theta_i = k.a_i taking K_i values per axis
+    int box;
+    int boxM[3], boxK[3];
+    const int32_t *box_r;     // [M1][M2][M3] index into the R list, -1 where the box has no R
+    const double2 *boxE[3];   // [K_i][M_i] e^{i n theta}, n = lo_i + m
+    const int32_t *box_kinv;  // [K1][K2][K3] k index of this block, -1 where absent
+    double2 *box_b;           // B on the box [M1 M2 M3][bcols]
+    double2 *box_t1;          // first pass  [M1][M2][K3][bcols]
+    double2 *box_t2;          // second pass [M1][K2][K3][bcols]
 };
 
 // One evaluation of the Fourier sums at k + A(t) with field E(t).

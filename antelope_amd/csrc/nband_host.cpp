@@ -51,6 +51,48 @@ hipError_t eval_mats(const NbArgs &a, const NbEval &ev, const double2 *P, double
     return a.model_kind ? nb_launch_model(a, ev, out, s) : nb_launch_fourier(a, ev, P, out, s);
 }
 
+// Shape of the separable Fourier sums: the lattice box of the R list and the
+// k grid's sides (distinct k.a_i per axis are at most npoints[i]).
+struct BoxShape {
+    int ok = 0;
+    int lo[3] = {0, 0, 0}, M[3] = {1, 1, 1}, K[3] = {1, 1, 1};
+    int64_t cells() const { return static_cast<int64_t>(M[0]) * M[1] * M[2]; }
+    int64_t kcells() const { return static_cast<int64_t>(K[0]) * K[1] * K[2]; }
+};
+
+constexpr int kBoxMaxSide = 32;
+
+// The box form is taken when the model is a Wannier model in the FP64 form, its
+// R fill a lattice box of sides <= 32 at least a quarter full, and the three
+// passes cost under half the dense sums (nk x nr multiply-adds per column).
+// ANTELOPE_NB_DENSE=1 keeps the dense GEMM.
+BoxShape box_shape(const sbe_model &m, const int32_t npoints[3], int64_t nk, bool split16)
+{
+    BoxShape b;
+    if (am::is_analytic(m.kind) || split16 || m.nrpts < 1 || !m.rvec) return b;
+    if (const char *e = std::getenv("ANTELOPE_NB_DENSE"))
+        if (std::atoi(e) == 1) return b;
+    int hi[3];
+    for (int i = 0; i < 3; ++i) b.lo[i] = hi[i] = m.rvec[i];
+    for (int r = 0; r < m.nrpts; ++r)
+        for (int i = 0; i < 3; ++i) {
+            b.lo[i] = std::min(b.lo[i], m.rvec[3 * r + i]);
+            hi[i] = std::max(hi[i], m.rvec[3 * r + i]);
+        }
+    for (int i = 0; i < 3; ++i) {
+        b.M[i] = hi[i] - b.lo[i] + 1;
+        b.K[i] = npoints[i] > 0 ? npoints[i] : 1;
+        if (b.M[i] > kBoxMaxSide) return b;
+    }
+    if (b.cells() > 4 * static_cast<int64_t>(m.nrpts)) return b;
+    const double box = static_cast<double>(b.M[0]) * b.M[1] * b.K[2] * b.M[2] +
+                       static_cast<double>(b.M[0]) * b.K[1] * b.K[2] * b.M[1] +
+                       static_cast<double>(b.K[0]) * b.K[1] * b.K[2] * b.M[0];
+    if (2.0 * box > static_cast<double>(nk) * m.nrpts) return b;
+    b.ok = 1;
+    return b;
+}
+
 struct NbPlan {
     int64_t nk_stride;
     int sample_waves, nv;
@@ -58,9 +100,14 @@ struct NbPlan {
     int nr_pad;
     int64_t bcols;
     size_t bytes_P, bytes_rho, bytes_mats, bytes_k, bytes_tab, bytes_partials, bytes_series, bytes_split, bytes_b64;
+    size_t bytes_box;
+    // offsets inside the box buffer: B, T1, T2, E_1..E_3, R map, k map
+    size_t off_t1, off_t2, off_e[3], off_r, off_kinv;
 };
 
-NbPlan nb_plan(int N, int nr, int64_t nk, int64_t nt, int nstage, bool split16)
+size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+
+NbPlan nb_plan(int N, int nr, int64_t nk, int64_t nt, int nstage, bool split16, const BoxShape &bx)
 {
     NbPlan p{};
     p.nk_stride = (nk + 63) / 64 * 64;
@@ -85,16 +132,90 @@ NbPlan nb_plan(int N, int nr, int64_t nk, int64_t nt, int nstage, bool split16)
                             : 0;
     // FP64 form: B of one evaluation, formed once per evaluation (Wannier models only)
     p.bytes_b64 = !split16 && nr > 0 ? sizeof(double2) * static_cast<size_t>(p.nr_pad) * p.bcols : 0;
+    p.bytes_box = 0;
+    if (bx.ok) {
+        const size_t col = sizeof(double2) * static_cast<size_t>(p.bcols);
+        size_t o = align256(static_cast<size_t>(bx.cells()) * col);
+        p.off_t1 = o;
+        o += align256(static_cast<size_t>(bx.M[0]) * bx.M[1] * bx.K[2] * col);
+        p.off_t2 = o;
+        o += align256(static_cast<size_t>(bx.M[0]) * bx.K[1] * bx.K[2] * col);
+        for (int i = 0; i < 3; ++i) {
+            p.off_e[i] = o;
+            o += align256(static_cast<size_t>(bx.K[i]) * bx.M[i] * sizeof(double2));
+        }
+        p.off_r = o;
+        o += align256(static_cast<size_t>(bx.cells()) * sizeof(int32_t));
+        p.off_kinv = o;
+        o += align256(static_cast<size_t>(bx.kcells()) * sizeof(int32_t));
+        p.bytes_box = o;
+    }
     return p;
+}
+
+// theta_i = k.a_i of this block's k points grouped per axis (values within
+// 1e-9 relative are one grid line); fills E_i, the R map and the k map and
+// returns false when the k points do not form a product grid within the
+// planned sides (the dense sums are used then).
+bool box_tables(const sbe_model &m, const std::vector<double> &hk, int64_t nk, const BoxShape &bx, int Kact[3],
+                std::vector<double> &E, size_t e_off[3], std::vector<int32_t> &rmap, std::vector<int32_t> &kinv)
+{
+    std::vector<int32_t> gi[3];
+    std::vector<double> rep[3];
+    for (int i = 0; i < 3; ++i) {
+        const double *av = m.lattice + 3 * i;
+        std::vector<double> th(static_cast<size_t>(nk));
+        double big = 0.0;
+        for (int64_t k = 0; k < nk; ++k) {
+            th[k] = hk[3 * k] * av[0] + hk[3 * k + 1] * av[1] + hk[3 * k + 2] * av[2];
+            big = std::max(big, std::fabs(th[k]));
+        }
+        const double tol = 1e-9 * (1.0 + big);
+        std::vector<int64_t> ord(static_cast<size_t>(nk));
+        for (int64_t k = 0; k < nk; ++k) ord[k] = k;
+        std::sort(ord.begin(), ord.end(), [&](int64_t x, int64_t y) { return th[x] < th[y]; });
+        gi[i].assign(static_cast<size_t>(nk), 0);
+        for (int64_t q = 0; q < nk; ++q) {
+            const int64_t k = ord[q];
+            if (rep[i].empty() || th[k] - rep[i].back() > tol) rep[i].push_back(th[k]);
+            gi[i][k] = static_cast<int32_t>(rep[i].size() - 1);
+        }
+        Kact[i] = static_cast<int>(rep[i].size());
+        if (Kact[i] > bx.K[i]) return false;
+    }
+    // E_i[g][m] = e^{i (lo_i + m) theta_g}, stored at the planned offsets (in units of double)
+    for (int i = 0; i < 3; ++i)
+        for (int g = 0; g < Kact[i]; ++g)
+            for (int mm = 0; mm < bx.M[i]; ++mm) {
+                const double ph = static_cast<double>(bx.lo[i] + mm) * rep[i][g];
+                E[e_off[i] + 2 * (static_cast<size_t>(g) * bx.M[i] + mm)] = std::cos(ph);
+                E[e_off[i] + 2 * (static_cast<size_t>(g) * bx.M[i] + mm) + 1] = std::sin(ph);
+            }
+    rmap.assign(static_cast<size_t>(bx.cells()), -1);
+    for (int r = 0; r < m.nrpts; ++r) {
+        const int32_t *n = m.rvec + 3 * r;
+        const int64_t cell = (static_cast<int64_t>(n[0] - bx.lo[0]) * bx.M[1] + (n[1] - bx.lo[1])) * bx.M[2] + (n[2] - bx.lo[2]);
+        if (rmap[cell] >= 0) return false;  // the same R twice
+        rmap[cell] = r;
+    }
+    kinv.assign(static_cast<size_t>(bx.kcells()), -1);
+    for (int64_t k = 0; k < nk; ++k) {
+        const int64_t cell = (static_cast<int64_t>(gi[0][k]) * Kact[1] + gi[1][k]) * Kact[2] + gi[2][k];
+        if (kinv[cell] >= 0) return false;  // two k points on one grid node
+        kinv[cell] = static_cast<int32_t>(k);
+    }
+    return true;
 }
 
 }  // namespace
 
-size_t nb_footprint(const sbe_model &m, int64_t nk, int64_t nt, int nstage, int precision)
+size_t nb_footprint(const sbe_model &m, const int32_t npoints[3], int64_t nk, int64_t nt, int nstage, int precision)
 {
-    const NbPlan p = nb_plan(m.nband, m.nrpts, nk, nt, nstage, precision == SBE_PREC_SPLIT16);
+    const bool split16 = precision == SBE_PREC_SPLIT16;
+    const BoxShape bx = box_shape(m, npoints, nk, split16);
+    const NbPlan p = nb_plan(m.nband, am::is_analytic(m.kind) ? 0 : m.nrpts, nk, nt, nstage, split16, bx);
     return p.bytes_P + p.bytes_rho + p.bytes_mats + p.bytes_k + p.bytes_tab + p.bytes_partials + p.bytes_series +
-           p.bytes_split + p.bytes_b64;
+           p.bytes_split + p.bytes_b64 + p.bytes_box;
 }
 
 int nb_validate(const sbe_model &m, std::string &err)
@@ -166,7 +287,8 @@ int nb_setup(sbe_ctx *ctx)
     const int N = m.nband, nr = analytic ? 0 : m.nrpts;
     const int64_t nk = ctx->grid.k_end - ctx->grid.k_begin;
     const bool split16 = ctx->cfg.precision == SBE_PREC_SPLIT16;
-    const NbPlan p = nb_plan(N, nr, nk, ctx->laser.nt, ctx->tab.nstage, split16);
+    const BoxShape bx = box_shape(m, ctx->cfg.npoints, nk, split16);
+    const NbPlan p = nb_plan(N, nr, nk, ctx->laser.nt, ctx->tab.nstage, split16, bx);
     const size_t nn = static_cast<size_t>(N) * N;
     ctx->nk = nk;
     ctx->nk_stride = p.nk_stride;
@@ -183,6 +305,7 @@ int nb_setup(sbe_ctx *ctx)
     ctx->d_series = nalloc<double>(ctx, p.bytes_series, rc);
     uint16_t *split = split16 ? nalloc<uint16_t>(ctx, p.bytes_split, rc) : nullptr;
     double2 *b64 = p.bytes_b64 ? nalloc<double2>(ctx, p.bytes_b64, rc) : nullptr;
+    char *boxbuf = p.bytes_box ? nalloc<char>(ctx, p.bytes_box, rc) : nullptr;
     if (rc) return rc;
 
     // host copies of the tables: k [nk_stride][3], w, R cartesian, weights, H, r
@@ -246,6 +369,32 @@ int nb_setup(sbe_ctx *ctx)
     for (int e = 0; e < 3 * nr; ++e) a.rmax = std::max(a.rmax, std::fabs(hr[e]));
     a.model_kind = analytic ? m.kind : 0;
     for (int i = 0; i < 24; ++i) a.mparams[i] = analytic ? m.params[i] : 0.0;
+    a.box = 0;
+    if (boxbuf) {  // separable Fourier sums when this block's k points form a product grid
+        int Kact[3];
+        std::vector<double> E(p.off_r / sizeof(double), 0.0);  // covers E_1..E_3 at their offsets
+        size_t e_off[3];
+        for (int i = 0; i < 3; ++i) e_off[i] = p.off_e[i] / sizeof(double);
+        std::vector<int32_t> rmap, kinv;
+        if (box_tables(m, hk, nk, bx, Kact, E, e_off, rmap, kinv)) {
+            const size_t e_bytes = p.off_r - p.off_e[0];
+            rc = hipc(ctx, hipMemcpy(boxbuf + p.off_e[0], E.data() + e_off[0], e_bytes, hipMemcpyHostToDevice), "upload box");
+            if (!rc) rc = hipc(ctx, hipMemcpy(boxbuf + p.off_r, rmap.data(), rmap.size() * sizeof(int32_t), hipMemcpyHostToDevice), "upload box");
+            if (!rc) rc = hipc(ctx, hipMemcpy(boxbuf + p.off_kinv, kinv.data(), kinv.size() * sizeof(int32_t), hipMemcpyHostToDevice), "upload box");
+            if (rc) return rc;
+            a.box = 1;
+            for (int i = 0; i < 3; ++i) {
+                a.boxM[i] = bx.M[i];
+                a.boxK[i] = Kact[i];
+                a.boxE[i] = reinterpret_cast<const double2 *>(boxbuf + p.off_e[i]);
+            }
+            a.box_r = reinterpret_cast<const int32_t *>(boxbuf + p.off_r);
+            a.box_kinv = reinterpret_cast<const int32_t *>(boxbuf + p.off_kinv);
+            a.box_b = reinterpret_cast<double2 *>(boxbuf);
+            a.box_t1 = reinterpret_cast<double2 *>(boxbuf + p.off_t1);
+            a.box_t2 = reinterpret_cast<double2 *>(boxbuf + p.off_t2);
+        }
+    }
 
     a.velocity = ctx->velocity ? 1 : 0;
     // e^{ik.R}, then rho(k) from H(k) (A(atmin) = 0)
