@@ -57,6 +57,9 @@ constexpr size_t kPartialBytesMax = size_t(2) << 30;
 // one run.
 constexpr int64_t kAdaptiveWindow = 2048;
 thread_local size_t t_partial_bytes = kPartialBytes;
+// set while a batch creates its members: the batch step kernel runs one k point
+// per lane for every member, so a member's rows and block count must be planned so
+thread_local bool t_batch_member = false;
 
 struct Plan {
     int kpt = 1;
@@ -142,6 +145,7 @@ int make_plan(const sbe_config &cfg, const sbe_model &m, int64_t nk, int64_t nt,
     // FP32 does best with one throughout
     p.kpt = cfg.precision == SBE_PREC_FP64 && nk >= 65536 ? 2 : 1;
     if (const char *e = std::getenv("ANTELOPE_HALDANE_KPT")) p.kpt = std::atoi(e) == 2 ? 2 : 1;
+    if (t_batch_member) p.kpt = 1;
     if (cfg.rk_order == SBE_RK_CASH_KARP) {  // adaptive: 64 kpt consecutive k points per wave (step-control block)
         // one k point per lane: the FP64 kernel then fits two waves per SIMD (1.1x over
         // two chains per lane at one wave on the batched workload) and the 64-point
@@ -862,11 +866,13 @@ int sbe_batch_create(int32_t n, const sbe_config *cfgs, const sbe_model *models,
     if (!b) return batch_fail(nullptr, SBE_ERR_OOM, "host allocation failed");
     b->adaptive = cfgs[0].rk_order == SBE_RK_CASH_KARP;
     t_partial_bytes = kBatchPartialBytes;
+    t_batch_member = !b->adaptive;  // the adaptive batch takes each member's kpt
     for (int i = 0; i < n; ++i) {
         sbe_ctx *c = nullptr;
         const int rc = sbe_create(&cfgs[i], &models[i], -1, -1, &c);
         if (rc) {
             t_partial_bytes = kPartialBytes;
+            t_batch_member = false;
             const std::string m = "solve " + std::to_string(i) + ": " + g_thread_error;
             sbe_batch_destroy(b);
             return batch_fail(nullptr, rc, m);
@@ -878,6 +884,7 @@ int sbe_batch_create(int32_t n, const sbe_config *cfgs, const sbe_model *models,
         b->deph = b->deph || c->hargs.gam1 != 0.0 || c->hargs.gam2 != 0.0;
     }
     t_partial_bytes = kPartialBytes;
+    t_batch_member = false;
     b->rk = cfgs[0].rk_order;
     b->prec = cfgs[0].precision;
     hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);

@@ -25,3 +25,26 @@ def test_batch_equals_alone(precision):
             alone = s.series()
         for k in alone:
             assert np.array_equal(got[i][k], alone[k]), (i, k)
+
+
+def test_batch_member_large_fp64_grid():
+    """A FP64 member of 65,536 k points (alone it takes two k points per lane) in a
+    plain batch: the batch kernel runs one k point per lane for every member, so
+    the member is planned that way; its series match its solve alone to rounding
+    (the wave sums group k points differently) and the small member's bit for bit."""
+    cfgs, models = haldane_batch(n=2, grid=20, cycles=(1, 1), precision="fp64", seed=5)
+    cfgs[0].npoints = [256, 256]
+    with SBEBatch(cfgs, models) as b:
+        b.run()
+        got = [b.series(i) for i in range(2)]
+    for i in range(2):
+        with SBESolver(cfgs[i], models[i]) as s:
+            s.run()
+            alone = s.series()
+        for k in alone:
+            if i == 1:
+                assert np.array_equal(got[i][k], alone[k]), (i, k)
+            else:
+                den = np.linalg.norm(alone[k])
+                err = np.linalg.norm(got[i][k] - alone[k]) / den if den > 0 else np.abs(got[i][k]).max()
+                assert err < 1e-12, (k, err)
