@@ -18,6 +18,7 @@
 //    products (weigh_warm). Cold starts (initial state, velocity-gauge set-up,
 //    band export) use a parallel-order complex Jacobi method (N/2 disjoint
 //    rotations per round).
+#include <algorithm>
 #include <cmath>
 
 #include "nband.hpp"
@@ -223,14 +224,12 @@ __global__ void nb_box_bform_kernel(const NbArgs a, const NbEval ev, int64_t nce
 }
 
 // One pass: Y[o][k][i][c] = sum_{m < M} E[k][m] X[o][m][i][c] for k < K; block x
-// = (o, i), block y = 256 columns, block z = kKChunk consecutive k. The M inputs
-// of a thread stay in registers (MM >= M); E is read with uniform loads. With
-// out set (the last pass, o = 0) the result goes to out[mat][kinv[k][i]][ij]
-// instead of Y.
-constexpr int kKChunk = 8;
+// = (o, i), block y = 256 columns, block z = kc consecutive k. The M inputs of a
+// thread stay in registers (MM >= M); E is read with uniform loads. With out set
+// (the last pass, o = 0) the result goes to out[mat][kinv[k][i]][ij] instead of Y.
 template <int MM>
 __global__ __launch_bounds__(256) void nb_box_pass_kernel(const cd *__restrict__ X, cd *__restrict__ Y,
-                                                          const cd *__restrict__ E, int M, int K, int64_t inner,
+                                                          const cd *__restrict__ E, int M, int K, int kc, int64_t inner,
                                                           int64_t ncols, int64_t pitch, const int32_t *__restrict__ kinv,
                                                           cd *__restrict__ out, int64_t nk_stride, int nn)
 {
@@ -242,7 +241,7 @@ __global__ __launch_bounds__(256) void nb_box_pass_kernel(const cd *__restrict__
 #pragma unroll
     for (int m = 0; m < MM; ++m) x[m] = m < M ? X[((o * M + m) * inner + i) * pitch + c] : cmk(0.0, 0.0);
     const int mat = static_cast<int>(c / nn), ij = static_cast<int>(c - static_cast<int64_t>(mat) * nn);
-    const int k0 = static_cast<int>(blockIdx.z) * kKChunk, k1 = min(K, k0 + kKChunk);
+    const int k0 = static_cast<int>(blockIdx.z) * kc, k1 = min(K, k0 + kc);
     for (int k = k0; k < k1; ++k) {
         const cd *e = E + static_cast<int64_t>(k) * M;
         double re = 0.0, im = 0.0;
@@ -279,9 +278,13 @@ template <int MM>
 static void box_pass(const cd *X, cd *Y, const cd *E, int M, int K, int64_t outer, int64_t inner, int64_t ncols,
                      int64_t pitch, const int32_t *kinv, cd *out, int64_t nk_stride, int nn, hipStream_t s)
 {
+    // k per block: about 2048 blocks in all (each block re-reads its M inputs, so
+    // fewer, longer blocks where the (o, i, column) grid alone is large)
+    const int64_t xy = outer * inner * ((ncols + 255) / 256);
+    const int kc = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(K, K * xy / 2048)));
     const dim3 grid(static_cast<unsigned>(outer * inner), static_cast<unsigned>((ncols + 255) / 256),
-                    static_cast<unsigned>((K + kKChunk - 1) / kKChunk));
-    nb_box_pass_kernel<MM><<<grid, 256, 0, s>>>(X, Y, E, M, K, inner, ncols, pitch, kinv, out, nk_stride, nn);
+                    static_cast<unsigned>((K + kc - 1) / kc));
+    nb_box_pass_kernel<MM><<<grid, 256, 0, s>>>(X, Y, E, M, K, kc, inner, ncols, pitch, kinv, out, nk_stride, nn);
 }
 
 static void box_pass_m(const cd *X, cd *Y, const cd *E, int M, int K, int64_t outer, int64_t inner, int64_t ncols,
