@@ -106,6 +106,46 @@ def test_laser_eval_and_footprint_without_gpu():
     assert 90_000 * 32 < b < 512 * 2 ** 20
 
 
+def _footprint_env(cfg, m, dense):
+    import os
+    old = os.environ.get("ANTELOPE_NB_DENSE")
+    os.environ["ANTELOPE_NB_DENSE"] = "1" if dense else "0"
+    try:
+        return query_footprint(cfg, m)
+    finally:
+        if old is None:
+            del os.environ["ANTELOPE_NB_DENSE"]
+        else:
+            os.environ["ANTELOPE_NB_DENSE"] = old
+
+
+def test_separable_sums_footprint_without_gpu():
+    """A Wannier model with R on a 5 x 5 x 3 box on a 6 x 5 x 2 grid takes the separable
+    Fourier sums; the footprint adds exactly their buffers: B on the box, the two pass
+    outputs (columns padded to 4 N^2 rounded to 64, complex FP64), the axis factors
+    [K_i][M_i], the R map and the k map, each 256-byte aligned. Where the passes would
+    not cost under half the dense sum (2 x 2 x 1 grid) nothing is added."""
+    from antelope_amd import Pulse, RunConfig
+    from antelope_amd.models import synthetic_wannier
+
+    def cfg_for(npts):
+        return RunConfig(npoints=list(npts), pulses=[Pulse(E0=0.01, w0=0.05, ncycles=0.25, envelope="sin2")],
+                         dt=0.5, T2=80.0, rk_order=4, inter_intra=True)
+
+    N, M, K = 8, (5, 5, 3), (6, 5, 2)
+    m = synthetic_wannier(N, nmax=(2, 2, 1), seed=21)
+    cfg = cfg_for(K)
+    al = lambda x: (x + 255) // 256 * 256
+    col = 16 * ((4 * N * N + 63) // 64 * 64)
+    cells = M[0] * M[1] * M[2]
+    want = (al(cells * col) + al(M[0] * M[1] * K[2] * col) + al(M[0] * K[1] * K[2] * col)
+            + sum(al(K[i] * M[i] * 16) for i in range(3)) + al(cells * 4) + al(K[0] * K[1] * K[2] * 4))
+    assert _footprint_env(cfg, m, dense=False) - _footprint_env(cfg, m, dense=True) == want
+    small = cfg_for((2, 2, 1))
+    m1 = synthetic_wannier(N, nmax=(1, 1, 1), seed=21)
+    assert _footprint_env(small, m1, dense=False) == _footprint_env(small, m1, dense=True)
+
+
 def test_server_bookkeeping_without_gpu():
     """Submission validates and prices a request on the host; nothing runs until a round."""
     import math
