@@ -92,18 +92,24 @@ def _run_fourier_form(cfg, model, dense, kb=-1, ke=-1):
             os.environ["ANTELOPE_NB_DENSE"] = old
 
 
-@pytest.mark.parametrize("label,kb,ke,gauge", [
-    ("full_grid", -1, -1, "LengthWannier"),
-    ("k_block", 7, 41, "LengthWannier"),
-    ("velocity", -1, -1, "VelocityHamiltonian"),
+@pytest.mark.parametrize("label,kb,ke,gauge,grid,ks,origin", [
+    ("full_grid", -1, -1, "LengthWannier", "reference", (1.0, 1.0, 1.0), None),
+    ("k_block", 7, 41, "LengthWannier", "reference", (1.0, 1.0, 1.0), None),
+    ("velocity", -1, -1, "VelocityHamiltonian", "reference", (1.0, 1.0, 1.0), None),
+    ("mp_grid", -1, -1, "LengthWannier", "mp", (1.0, 1.0, 1.0), None),
+    ("ksfactor_origin", -1, -1, "LengthWannier", "reference", (0.5, 1.5, 1.0), (0.1, -0.2, 0.05)),
 ])
-def test_box_fourier_sums_match_dense(label, kb, ke, gauge):
+def test_box_fourier_sums_match_dense(label, kb, ke, gauge, grid, ks, origin):
     """On a product k grid the Fourier sums run as three passes along the lattice
-    axes (R on a 5 x 5 x 3 box, 6 x 5 x 2 grid, a k block of it, a ragged side);
-    the result equals the dense GEMM's to rounding."""
+    axes (R on a 5 x 5 x 3 box, 6 x 5 x 2 grid, a k block of it, a ragged side,
+    the Monkhorst-Pack grid, scaled sides and a shifted origin); the result equals
+    the dense GEMM's to rounding."""
     model = synthetic_wannier(8, nmax=(2, 2, 1), seed=21)
     cfg = _cfg(npts=(6, 5, 2), ncycles=0.25)
     cfg.gauge = gauge
+    cfg.grid = grid
+    cfg.ksfactor = ks
+    cfg.bz_origin = origin
     a, sa, bytes_box = _run_fourier_form(cfg, model, dense=False, kb=kb, ke=ke)
     b, sb, bytes_dense = _run_fourier_form(cfg, model, dense=True, kb=kb, ke=ke)
     assert bytes_box > bytes_dense  # the box buffers were allocated: the separable path ran
