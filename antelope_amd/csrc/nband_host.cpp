@@ -116,7 +116,8 @@ NbPlan nb_plan(int N, int nr, int64_t nk, int64_t nt, int nstage, bool split16, 
     const size_t nn = static_cast<size_t>(N) * N;
     const size_t row = static_cast<size_t>(p.sample_waves) * p.nv * sizeof(double);
     p.chunk = std::max<int64_t>(1, std::min<int64_t>(nt, static_cast<int64_t>(kNbPartialBytes / row)));
-    p.bytes_P = static_cast<size_t>(p.nk_stride) * nr * sizeof(double2);
+    // e^{ik.R} for the dense sums; the separable form does not read it
+    p.bytes_P = bx.ok ? 0 : static_cast<size_t>(p.nk_stride) * nr * sizeof(double2);
     p.bytes_rho = static_cast<size_t>(p.nk_stride) * nn * sizeof(double2);
     p.bytes_mats = 5 * p.bytes_rho;  // H~, H, J_x, J_y, J_z
     p.bytes_k = static_cast<size_t>(nstage + 1) * p.bytes_rho;  // stage derivatives + eigenvectors
@@ -295,7 +296,7 @@ int nb_setup(sbe_ctx *ctx)
     ctx->nv = p.nv;
     ctx->chunk_steps = p.chunk;
     int rc = SBE_OK;
-    ctx->nb_P = nalloc<double2>(ctx, p.bytes_P, rc);
+    ctx->nb_P = p.bytes_P ? nalloc<double2>(ctx, p.bytes_P, rc) : nullptr;
     ctx->nb_rho = nalloc<double2>(ctx, p.bytes_rho, rc);
     ctx->nb_mats = nalloc<double2>(ctx, p.bytes_mats, rc);
     ctx->nb_k = nalloc<double2>(ctx, p.bytes_k, rc);
@@ -398,7 +399,13 @@ int nb_setup(sbe_ctx *ctx)
 
     a.velocity = ctx->velocity ? 1 : 0;
     // e^{ik.R}, then rho(k) from H(k) (A(atmin) = 0)
-    if (!analytic) rc = hipc(ctx, nb_launch_phase(a, ctx->nb_P, ctx->stream), "phase kernel");
+    if (!analytic && !a.box && !ctx->nb_P) {
+        // planned for the separable sums, but the k points did not group into a product
+        // grid: the dense sums need e^{ik.R} after all (beyond the planned footprint)
+        ctx->nb_P = nalloc<double2>(ctx, static_cast<size_t>(p.nk_stride) * nr * sizeof(double2), rc);
+        if (rc) return rc;
+    }
+    if (!analytic && !a.box) rc = hipc(ctx, nb_launch_phase(a, ctx->nb_P, ctx->stream), "phase kernel");
     if (!rc && split16) rc = hipc(ctx, nb_launch_psplit(a, ctx->nb_P, split, ctx->stream), "phase split kernel");
     NbEval ev{};
     ev.nmat = 1;

@@ -123,7 +123,8 @@ def test_separable_sums_footprint_without_gpu():
     """A Wannier model with R on a 5 x 5 x 3 box on a 6 x 5 x 2 grid takes the separable
     Fourier sums; the footprint adds exactly their buffers: B on the box, the two pass
     outputs (columns padded to 4 N^2 rounded to 64, complex FP64), the axis factors
-    [K_i][M_i], the R map and the k map, each 256-byte aligned. Where the passes would
+    [K_i][M_i], the R map and the k map, each 256-byte aligned, and drops the dense sums'
+    e^{ik.R} table [nk_stride][N_R]. Where the passes would
     not cost under half the dense sum (2 x 2 x 1 grid) nothing is added."""
     from antelope_amd import Pulse, RunConfig
     from antelope_amd.models import synthetic_wannier
@@ -140,6 +141,8 @@ def test_separable_sums_footprint_without_gpu():
     cells = M[0] * M[1] * M[2]
     want = (al(cells * col) + al(M[0] * M[1] * K[2] * col) + al(M[0] * K[1] * K[2] * col)
             + sum(al(K[i] * M[i] * 16) for i in range(3)) + al(cells * 4) + al(K[0] * K[1] * K[2] * 4))
+    nk_stride, nr = (K[0] * K[1] * K[2] + 63) // 64 * 64, cells  # the dense e^{ik.R} table is not kept
+    want -= nk_stride * nr * 16
     assert _footprint_env(cfg, m, dense=False) - _footprint_env(cfg, m, dense=True) == want
     small = cfg_for((2, 2, 1))
     m1 = synthetic_wannier(N, nmax=(1, 1, 1), seed=21)
