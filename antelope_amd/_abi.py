@@ -149,6 +149,11 @@ SIGNATURES = {
     "sbe_destroy": ([C.c_void_p], None),
     "sbe_last_error": ([C.c_void_p], C.c_char_p),
     "sbe_tmdc_params": ([C.c_char_p, C.c_int32, C.POINTER(sbe_model)], C.c_int),
+    "sbe_hhg_points": ([C.c_int64], C.c_int64),
+    "sbe_hhg_spectrum": ([C.c_int64, c_double_p, c_double_p, C.c_double, C.c_double, C.c_int32, c_double_p,
+                          c_double_p, c_double_p, c_double_p, c_double_p], C.c_int),
+    "sbe_write_hhg": ([C.c_char_p, C.c_int64, c_double_p, c_double_p, C.c_double, C.c_double, C.c_int32], C.c_int),
+    "sbe_spectrum_last_error": ([], C.c_char_p),
 }
 
 

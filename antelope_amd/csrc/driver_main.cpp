@@ -3,9 +3,11 @@
 // given), propagates the whole k grid on the GPU and writes the reference's
 // output files into the working directory (or --outdir), snapshots included
 // when calc.shotNumber > 0. A config whose name ends in .atl is read in the
-// .atl format of src/inputFile.atl.
+// .atl format of src/inputFile.atl. --hhg FILE also writes the HHG spectrum
+// of the total current (pulse 0's w0, x and y axes) in the layout of the
+// reference's post-processing script (AnalysisReader_PyDir.py:367-382).
 //
-//   antelope_mi355x [config] [--outdir DIR] [--quiet]
+//   antelope_mi355x [config] [--outdir DIR] [--hhg FILE] [--quiet]
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -16,10 +18,11 @@
 
 int main(int argc, char **argv)
 {
-    std::string path = "inputParam.cfg", outdir = ".";  // antelope.cpp:93
+    std::string path = "inputParam.cfg", outdir = ".", hhg;  // antelope.cpp:93
     bool quiet = false;
     for (int i = 1; i < argc; ++i) {
         if (!std::strcmp(argv[i], "--outdir") && i + 1 < argc) outdir = argv[++i];
+        else if (!std::strcmp(argv[i], "--hhg") && i + 1 < argc) hhg = argv[++i];
         else if (!std::strcmp(argv[i], "--quiet")) quiet = true;
         else path = argv[i];
     }
@@ -79,6 +82,14 @@ int main(int argc, char **argv)
         std::printf("\n\nWriting output-data\n\n");
         rc = sbe_write_outputs(outdir.c_str(), cfg, model, &info, ji.data(), ja.data(), rb.data(), bp.data());
         if (rc != SBE_OK) std::fprintf(stderr, "%s\n", sbe_frontend_last_error());
+    }
+    if (rc == SBE_OK && !hhg.empty()) {
+        // the script reads the .dat files, which hold the sums x dV
+        for (double &v : ji) v *= info.dV;
+        for (double &v : ja) v *= info.dV;
+        rc = cfg->npulses > 0 ? sbe_write_hhg(hhg.c_str(), info.nt, ji.data(), ja.data(), info.dt, cfg->pulses[0].w0, 2)
+                              : SBE_ERR_CONFIG;
+        if (rc != SBE_OK) std::fprintf(stderr, "%s\n", sbe_spectrum_last_error());
     }
     sbe_spec_free(spec);
     return rc == SBE_OK ? 0 : 1;

@@ -40,6 +40,17 @@ T *nalloc(sbe_ctx *ctx, size_t bytes, int &rc)
     return static_cast<T *>(p);
 }
 
+void nfree(sbe_ctx *ctx, void *ptr)
+{
+    for (size_t i = 0; i < ctx->owned.size(); ++i)
+        if (ctx->owned[i].ptr == ptr) {
+            (void)hipFree(ptr);
+            ctx->device_bytes -= static_cast<int64_t>(ctx->owned[i].bytes);
+            ctx->owned.erase(ctx->owned.begin() + static_cast<std::ptrdiff_t>(i));
+            return;
+        }
+}
+
 constexpr size_t kNbPartialBytes = size_t(128) << 20;
 
 // the matrices {H~, H, J} at k + A: Fourier sums (Wannier) or the analytic classes
@@ -62,16 +73,42 @@ struct BoxShape {
 
 constexpr int kBoxMaxSide = 32;
 
-// The box form is taken when the model is a Wannier model in the FP64 form, its
-// R fill a lattice box of sides <= 32 at least a quarter full, and the three
-// passes cost under half the dense sums (nk x nr multiply-adds per column).
-// ANTELOPE_NB_DENSE=1 keeps the dense GEMM.
-BoxShape box_shape(const sbe_model &m, const int32_t npoints[3], int64_t nk, bool split16)
+// theta_j = k.a_j of a grid point k = origin + sum_i f_i b_i depends on f_j alone
+// when every BZ axis b_i (of an axis with more than one point) is normal to the
+// lattice vectors a_j, j != i: then any block of the grid is a product grid in
+// theta. The model's own axes (reciprocal vectors, scaled per row by ksfactor)
+// are; an explicit calc.BZaxes is checked.
+bool axes_separable(const sbe_config &cfg, const sbe_model &m)
+{
+    if (!cfg.has_bz_axes) return true;
+    for (int i = 0; i < 3; ++i) {
+        if (cfg.npoints[i] <= 1) continue;
+        const double *b = cfg.bz_axes + 3 * i;
+        const double nb = std::sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+        for (int j = 0; j < 3; ++j) {
+            if (j == i) continue;
+            const double *av = m.lattice + 3 * j;
+            const double na = std::sqrt(av[0] * av[0] + av[1] * av[1] + av[2] * av[2]);
+            if (std::fabs(b[0] * av[0] + b[1] * av[1] + b[2] * av[2]) > 1e-12 * nb * na) return false;
+        }
+    }
+    return true;
+}
+
+// The box form is taken when the model is a Wannier model in the FP64 form, the
+// grid is a product grid in theta (axes_separable), its R are distinct and fill a
+// lattice box of sides <= 32 at least a quarter full, and the three passes cost
+// under half the dense sums (nk x nr multiply-adds per column). The footprint
+// query and the set-up take the same decision. ANTELOPE_NB_DENSE=1 keeps the
+// dense GEMM.
+BoxShape box_shape(const sbe_config &cfg, const sbe_model &m, int64_t nk, bool split16)
 {
     BoxShape b;
+    const int32_t *npoints = cfg.npoints;
     if (am::is_analytic(m.kind) || split16 || m.nrpts < 1 || !m.rvec) return b;
     if (const char *e = std::getenv("ANTELOPE_NB_DENSE"))
         if (std::atoi(e) == 1) return b;
+    if (!axes_separable(cfg, m)) return b;
     int hi[3];
     for (int i = 0; i < 3; ++i) b.lo[i] = hi[i] = m.rvec[i];
     for (int r = 0; r < m.nrpts; ++r)
@@ -85,6 +122,13 @@ BoxShape box_shape(const sbe_model &m, const int32_t npoints[3], int64_t nk, boo
         if (b.M[i] > kBoxMaxSide) return b;
     }
     if (b.cells() > 4 * static_cast<int64_t>(m.nrpts)) return b;
+    std::vector<char> seen(static_cast<size_t>(b.cells()), 0);
+    for (int r = 0; r < m.nrpts; ++r) {  // the same R twice: the dense sums add both
+        const int32_t *n = m.rvec + 3 * r;
+        const int64_t cell = (static_cast<int64_t>(n[0] - b.lo[0]) * b.M[1] + (n[1] - b.lo[1])) * b.M[2] + (n[2] - b.lo[2]);
+        if (seen[cell]) return b;
+        seen[cell] = 1;
+    }
     const double box = static_cast<double>(b.M[0]) * b.M[1] * b.K[2] * b.M[2] +
                        static_cast<double>(b.M[0]) * b.K[1] * b.K[2] * b.M[1] +
                        static_cast<double>(b.K[0]) * b.K[1] * b.K[2] * b.M[0];
@@ -131,8 +175,8 @@ NbPlan nb_plan(int N, int nr, int64_t nk, int64_t nt, int nstage, bool split16, 
     p.bytes_split = split16 ? 4 * sizeof(uint16_t) * static_cast<size_t>(p.nr_pad) *
                                   static_cast<size_t>(p.nk_stride + p.bcols)
                             : 0;
-    // FP64 form: B of one evaluation, formed once per evaluation (Wannier models only)
-    p.bytes_b64 = !split16 && nr > 0 ? sizeof(double2) * static_cast<size_t>(p.nr_pad) * p.bcols : 0;
+    // FP64 dense form: B of one evaluation, formed once per evaluation (Wannier models only)
+    p.bytes_b64 = !split16 && nr > 0 && !bx.ok ? sizeof(double2) * static_cast<size_t>(p.nr_pad) * p.bcols : 0;
     p.bytes_box = 0;
     if (bx.ok) {
         const size_t col = sizeof(double2) * static_cast<size_t>(p.bcols);
@@ -210,10 +254,10 @@ bool box_tables(const sbe_model &m, const std::vector<double> &hk, int64_t nk, c
 
 }  // namespace
 
-size_t nb_footprint(const sbe_model &m, const int32_t npoints[3], int64_t nk, int64_t nt, int nstage, int precision)
+size_t nb_footprint(const sbe_config &cfg, const sbe_model &m, int64_t nk, int64_t nt, int nstage)
 {
-    const bool split16 = precision == SBE_PREC_SPLIT16;
-    const BoxShape bx = box_shape(m, npoints, nk, split16);
+    const bool split16 = cfg.precision == SBE_PREC_SPLIT16;
+    const BoxShape bx = box_shape(cfg, m, nk, split16);
     const NbPlan p = nb_plan(m.nband, am::is_analytic(m.kind) ? 0 : m.nrpts, nk, nt, nstage, split16, bx);
     return p.bytes_P + p.bytes_rho + p.bytes_mats + p.bytes_k + p.bytes_tab + p.bytes_partials + p.bytes_series +
            p.bytes_split + p.bytes_b64 + p.bytes_box;
@@ -288,7 +332,7 @@ int nb_setup(sbe_ctx *ctx)
     const int N = m.nband, nr = analytic ? 0 : m.nrpts;
     const int64_t nk = ctx->grid.k_end - ctx->grid.k_begin;
     const bool split16 = ctx->cfg.precision == SBE_PREC_SPLIT16;
-    const BoxShape bx = box_shape(m, ctx->cfg.npoints, nk, split16);
+    const BoxShape bx = box_shape(ctx->cfg, m, nk, split16);
     const NbPlan p = nb_plan(N, nr, nk, ctx->laser.nt, ctx->tab.nstage, split16, bx);
     const size_t nn = static_cast<size_t>(N) * N;
     ctx->nk = nk;
@@ -399,10 +443,14 @@ int nb_setup(sbe_ctx *ctx)
 
     a.velocity = ctx->velocity ? 1 : 0;
     // e^{ik.R}, then rho(k) from H(k) (A(atmin) = 0)
-    if (!analytic && !a.box && !ctx->nb_P) {
+    if (!analytic && !a.box && boxbuf) {
         // planned for the separable sums, but the k points did not group into a product
-        // grid: the dense sums need e^{ik.R} after all (beyond the planned footprint)
+        // grid (box_shape rules this out for the grids this library builds): the dense
+        // sums' e^{ik.R} table and B buffer replace the box buffer, as the dense plan has them
+        nfree(ctx, boxbuf);
+        boxbuf = nullptr;
         ctx->nb_P = nalloc<double2>(ctx, static_cast<size_t>(p.nk_stride) * nr * sizeof(double2), rc);
+        a.b64 = nalloc<double2>(ctx, sizeof(double2) * static_cast<size_t>(p.nr_pad) * p.bcols, rc);
         if (rc) return rc;
     }
     if (!analytic && !a.box) rc = hipc(ctx, nb_launch_phase(a, ctx->nb_P, ctx->stream), "phase kernel");

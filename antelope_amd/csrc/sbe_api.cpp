@@ -318,7 +318,7 @@ int sbe_query_footprint(const sbe_config *cfg, const sbe_model *model, int64_t k
     }
     if (k_begin < 0 || k_end > ntot || k_begin >= k_end) return set_error(nullptr, SBE_ERR_RANGE, "bad k range");
     if (model->kind != SBE_MODEL_HALDANE) {
-        if (bytes) *bytes = sbe::nb_footprint(*model, cfg->npoints, k_end - k_begin, L.nt, tab.nstage, cfg->precision);
+        if (bytes) *bytes = sbe::nb_footprint(*cfg, *model, k_end - k_begin, L.nt, tab.nstage);
         return SBE_OK;
     }
     Plan p;

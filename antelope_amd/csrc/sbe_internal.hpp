@@ -200,7 +200,7 @@ hipError_t launch_adaptive_advance(const AdaptiveArgs *d_insts, int n, int64_t n
 int haldane_waves(int64_t nk);
 // N-band host side (nband_host.cpp)
 int nb_validate(const sbe_model &m, std::string &err);
-size_t nb_footprint(const sbe_model &m, const int32_t npoints[3], int64_t nk, int64_t nt, int nstage, int precision);
+size_t nb_footprint(const sbe_config &cfg, const sbe_model &m, int64_t nk, int64_t nt, int nstage);
 int nb_setup(sbe_ctx *ctx);
 int nb_run(sbe_ctx *ctx, int64_t step_begin, int64_t step_end);
 int nb_series(sbe_ctx *ctx, double *j_inter, double *j_intra, double *rho_bar, double *band_pop);

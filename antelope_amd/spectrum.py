@@ -1,39 +1,64 @@
 """HHG spectrum of the current, as the reference's post-processing defines it.
 
-Restates src/GeneralModel/AlsisPythonShort/AnalysisReader_PyDir.py:
+`hhg_spectrum` calls the library's C++ implementation (sbe_hhg_spectrum,
+include/antelope/sbe_frontend.h) of
+src/GeneralModel/AlsisPythonShort/AnalysisReader_PyDir.py:
   :163-167  w = linspace(-pi/dt, pi/dt - dw, Nt), dw = 2 pi / (Nt dt)
   :249-253  Blackman window over the whole series
   :323-330  F_a = -i w fftshift(fft(window j_a) dt), per axis, inter and intra
   :354      S = |F_x|^2 + |F_y|^2 + hzero (hzero = 0.98e-25, :118), Dim = 2
   :367-372  output half w[Nt//2 : Nt-1] / w0 (harmonic order)
-and the error measure used to compare two solves: the largest |10 log10(S /
-S_ref)| over integer harmonics 1..H_cut, where H_cut is the last harmonic whose
-reference intensity lies within `window_db` of the spectrum's maximum.
+`harmonic_error_db` is the error measure used to compare two solves: the
+largest |10 log10(S / S_ref)| over integer harmonics 1..H_cut, where H_cut is
+the last harmonic whose reference intensity lies within `window_db` of the
+spectrum's maximum.
 """
 from __future__ import annotations
 
 import numpy as np
 
+from . import _abi as A
+from .solver import load_library
+
 HZERO = 0.98e-25
 
 
-def hhg_spectrum(j_inter: np.ndarray, j_intra: np.ndarray, dt: float, w0: float, dim: int = 2):
-    """j_*: [Nt][3] current series as written to the .dat files (x dV).
-    Returns (harmonic order, S) over the non-negative half of the axis."""
-    j_inter = np.asarray(j_inter, dtype=np.float64)
-    j_intra = np.asarray(j_intra, dtype=np.float64)
-    nt = j_inter.shape[0]
-    wmax = np.pi / dt
-    dw = 2.0 * wmax / float(nt)
-    w = np.linspace(-wmax, wmax - dw, num=nt)
-    win = np.blackman(nt)
-    S = np.full(nt, HZERO)
-    for a in range(dim):
-        fi = -1j * w * np.fft.fftshift(np.fft.fft(win * j_inter[:, a]) * dt)
-        fa = -1j * w * np.fft.fftshift(np.fft.fft(win * j_intra[:, a]) * dt)
-        S += np.abs(fi + fa) ** 2
-    half = nt // 2
-    return w[half:nt - 1] / w0, S[half:nt - 1]
+def _series(j, nt):
+    a = np.zeros((nt, 3))
+    j = np.asarray(j, dtype=np.float64)
+    a[:, :j.shape[1]] = j
+    return np.ascontiguousarray(a)
+
+
+def hhg_spectrum(j_inter: np.ndarray, j_intra: np.ndarray, dt: float, w0: float, dim: int = 2,
+                 parts: bool = False):
+    """j_*: [Nt][>=dim] current series as written to the .dat files (x dV).
+    Returns (harmonic order, S) over the script's half of the axis; with
+    parts=True also (F [n][dim] complex, S_inter, S_intra)."""
+    lib = load_library()
+    nt = int(np.asarray(j_inter).shape[0])
+    ji, ja = _series(j_inter, nt), _series(j_intra, nt)
+    n = int(lib.sbe_hhg_points(nt))
+    order, S, Si, Sa = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(n)
+    F = np.zeros((n, dim, 2))
+    p = lambda a: a.ctypes.data_as(A.c_double_p)  # noqa: E731
+    rc = lib.sbe_hhg_spectrum(nt, p(ji), p(ja), float(dt), float(w0), int(dim), p(order), p(F), p(S), p(Si), p(Sa))
+    if rc != A.SBE_OK:
+        raise ValueError(lib.sbe_spectrum_last_error().decode())
+    if parts:
+        return order, S, F[..., 0] + 1j * F[..., 1], Si, Sa
+    return order, S
+
+
+def write_hhg(path, j_inter, j_intra, dt: float, w0: float, dim: int = 2) -> None:
+    """The script's HHG__*.dat file: w/w0, Re F_a, Im F_a per axis, S (%1.16e)."""
+    lib = load_library()
+    nt = int(np.asarray(j_inter).shape[0])
+    ji, ja = _series(j_inter, nt), _series(j_intra, nt)
+    rc = lib.sbe_write_hhg(str(path).encode(), nt, ji.ctypes.data_as(A.c_double_p), ja.ctypes.data_as(A.c_double_p),
+                           float(dt), float(w0), int(dim))
+    if rc != A.SBE_OK:
+        raise ValueError(lib.sbe_spectrum_last_error().decode())
 
 
 def harmonic_intensities(order: np.ndarray, S: np.ndarray, hmax: int, halfwidth: float = 0.25):

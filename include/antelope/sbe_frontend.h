@@ -12,6 +12,8 @@
  *       reference leaves it), interband_dipole_full_evol.dat,
  *       intraband_current_full_evol.dat, occupation__full__evol.dat,
  *       coherence__full__evol.dat, plus band_population__full__evol.dat.
+ *   sbe_hhg_spectrum / sbe_write_hhg  the HHG spectrum of the currents, as
+ *       AnalysisReader_PyDir.py:163-167,249-382 computes it.
  * Host-only: none of these touch the GPU.
  */
 #ifndef ANTELOPE_SBE_FRONTEND_H
@@ -92,6 +94,27 @@ int32_t sbe_spec_shot_number(const sbe_spec *spec);
 int64_t sbe_snapshot_interval(const sbe_config *cfg, int32_t shot_number);
 int sbe_write_snapshot(const char *dir, const sbe_config *cfg, const sbe_model *model, const double *rho_band,
                        int32_t append);
+
+/* HHG spectrum of a current series, as the reference's post-processing
+ * computes it (src/GeneralModel/AlsisPythonShort/AnalysisReader_PyDir.py:
+ * 163-167, 249-382, which does it in numpy): with w = linspace(-pi/dt,
+ * pi/dt - dw, nt), dw = 2 pi / (nt dt), and a Blackman window over the whole
+ * series, F_a(w) = -i w fftshift(fft(window j_a) dt) for the inter and intra
+ * series separately, their sum per axis a < dim, and S = sum_a |F_a|^2 +
+ * 0.98e-25. j_inter, j_intra are [nt][3] as written to the .dat files (x dV).
+ * Outputs cover the script's half w[nt/2 : nt-1], sbe_hhg_points(nt) =
+ * nt - nt/2 - 1 points; each may be NULL:
+ *   order [n]          w / w0 (harmonic order)
+ *   F     [n][dim][2]  inter + intra F_a (re, im)
+ *   S, S_inter, S_intra [n]  total, inter-only and intra-only spectra
+ * sbe_write_hhg writes the script's HHG__*.dat layout to `path`: per point
+ * w/w0, Re F_a, Im F_a for a < dim, then S, in %1.16e. Host-only. */
+int64_t sbe_hhg_points(int64_t nt);
+int sbe_hhg_spectrum(int64_t nt, const double *j_inter, const double *j_intra, double dt, double w0, int32_t dim,
+                     double *order, double *F, double *S, double *S_inter, double *S_intra);
+int sbe_write_hhg(const char *path, int64_t nt, const double *j_inter, const double *j_intra, double dt, double w0,
+                  int32_t dim);
+const char *sbe_spectrum_last_error(void);
 
 /* Write the reference's output files into `dir` from summed series (the
  * layout of sbe_get_series; several ranks' series added together). */

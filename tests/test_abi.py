@@ -8,6 +8,7 @@ import re
 import subprocess
 from pathlib import Path
 
+import numpy as np
 import pytest
 
 from antelope_amd import _abi as A
@@ -124,8 +125,8 @@ def test_separable_sums_footprint_without_gpu():
     Fourier sums; the footprint adds exactly their buffers: B on the box, the two pass
     outputs (columns padded to 4 N^2 rounded to 64, complex FP64), the axis factors
     [K_i][M_i], the R map and the k map, each 256-byte aligned, and drops the dense sums'
-    e^{ik.R} table [nk_stride][N_R]. Where the passes would
-    not cost under half the dense sum (2 x 2 x 1 grid) nothing is added."""
+    e^{ik.R} table [nk_stride][N_R] and their B buffer [N_R padded to 32][columns]. Where
+    the passes would not cost under half the dense sum (2 x 2 x 1 grid) nothing is added."""
     from antelope_amd import Pulse, RunConfig
     from antelope_amd.models import synthetic_wannier
 
@@ -142,11 +143,39 @@ def test_separable_sums_footprint_without_gpu():
     want = (al(cells * col) + al(M[0] * M[1] * K[2] * col) + al(M[0] * K[1] * K[2] * col)
             + sum(al(K[i] * M[i] * 16) for i in range(3)) + al(cells * 4) + al(K[0] * K[1] * K[2] * 4))
     nk_stride, nr = (K[0] * K[1] * K[2] + 63) // 64 * 64, cells  # the dense e^{ik.R} table is not kept
-    want -= nk_stride * nr * 16
+    want -= nk_stride * nr * 16 + (nr + 31) // 32 * 32 * col
     assert _footprint_env(cfg, m, dense=False) - _footprint_env(cfg, m, dense=True) == want
     small = cfg_for((2, 2, 1))
     m1 = synthetic_wannier(N, nmax=(1, 1, 1), seed=21)
     assert _footprint_env(small, m1, dense=False) == _footprint_env(small, m1, dense=True)
+
+
+def test_separable_sums_eligibility_without_gpu():
+    """The separable form needs theta_j = k.a_j to depend on grid index j alone. On a
+    hexagonal lattice, calc.BZaxes given as Cartesian axes break that (the footprint is
+    the dense one); the reciprocal vectors given explicitly keep it; an R listed twice
+    (the dense sums add both) also keeps the dense form."""
+    import dataclasses
+    import math
+
+    from antelope_amd import Pulse, RunConfig
+    from antelope_amd.models import synthetic_wannier
+
+    a = 5.0
+    hexl = np.array([[a, 0, 0], [a / 2, a * math.sqrt(3) / 2, 0], [0, 0, 10.0]])
+    m = synthetic_wannier(8, nmax=(2, 2, 1), seed=21, lattice=hexl)
+    base = RunConfig(npoints=[6, 5, 2], pulses=[Pulse(E0=0.01, w0=0.05, ncycles=0.25, envelope="sin2")],
+                     dt=0.5, T2=80.0, rk_order=4, inter_intra=True)
+    box, dense = _footprint_env(base, m, dense=False), _footprint_env(base, m, dense=True)
+    assert box != dense
+    cart = dataclasses.replace(base, bz_axes=[1.2, 0, 0, 0, 1.2, 0, 0, 0, 0.6])
+    assert _footprint_env(cart, m, dense=False) == _footprint_env(cart, m, dense=True)
+    recip = 2 * math.pi * np.linalg.inv(hexl).T  # rows b_i with b_i . a_j = 2 pi delta_ij
+    rec = dataclasses.replace(base, bz_axes=list(recip.ravel()))
+    assert _footprint_env(rec, m, dense=False) == box
+    dup = dataclasses.replace(m, rvec=np.concatenate([m.rvec, m.rvec[:1]]), rweight=np.concatenate([m.rweight, [1.0]]),
+                              ham=np.concatenate([m.ham, m.ham[:1]]), pos=np.concatenate([m.pos, m.pos[:1]]))
+    assert _footprint_env(base, dup, dense=False) == _footprint_env(base, dup, dense=True)
 
 
 def test_server_bookkeeping_without_gpu():

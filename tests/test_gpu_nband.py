@@ -76,14 +76,18 @@ def test_wannier_k_blocks_and_split_runs(oracle):
 
 
 def _run_fourier_form(cfg, model, dense, kb=-1, ke=-1):
-    """One solve with the separable (box) Fourier sums or, with dense, the dense GEMM."""
+    """One solve with the separable (box) Fourier sums or, with dense, the dense GEMM.
+    The context holds exactly the device bytes the footprint query prices."""
     import os
+
+    from antelope_amd import query_footprint
     old = os.environ.get("ANTELOPE_NB_DENSE")
     os.environ["ANTELOPE_NB_DENSE"] = "1" if dense else "0"
     try:
         with SBESolver(cfg, model, kb, ke) as s:
             s.run()
             s.synchronize()
+            assert s.info.device_bytes == query_footprint(cfg, model, kb, ke)
             return s.series(), s.state(), s.info.device_bytes
     finally:
         if old is None:
@@ -112,10 +116,36 @@ def test_box_fourier_sums_match_dense(label, kb, ke, gauge, grid, ks, origin):
     cfg.bz_origin = origin
     a, sa, bytes_box = _run_fourier_form(cfg, model, dense=False, kb=kb, ke=ke)
     b, sb, bytes_dense = _run_fourier_form(cfg, model, dense=True, kb=kb, ke=ke)
-    assert bytes_box > bytes_dense  # the box buffers were allocated: the separable path ran
+    assert bytes_box != bytes_dense  # the box plan's buffers (no e^{ik.R} table): the separable path ran
     errs = {k: rel_l2(a[k], b[k]) for k in ("j_inter", "j_intra", "band_pop", "rho_bar")}
     errs["state"] = rel_l2(sa, sb)
     print(label, {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    assert all(v < 1e-12 for v in errs.values()), errs
+
+
+@pytest.mark.parametrize("axes", ["cartesian", "reciprocal"])
+def test_hexagonal_lattice_bz_axes(axes):
+    """A hexagonal Wannier lattice with calc.BZaxes given. Cartesian axes mix the
+    lattice directions, so theta_j = k.a_j is no product grid: the dense GEMM runs,
+    and the context holds the dense footprint (nothing of the separable plan). The
+    reciprocal vectors given explicitly keep the separable form. Either way the
+    series equal the dense GEMM's to rounding."""
+    import math
+    a0 = 5.0
+    hexl = np.array([[a0, 0, 0], [a0 / 2, a0 * math.sqrt(3) / 2, 0], [0, 0, 10.0]])
+    model = synthetic_wannier(8, nmax=(2, 2, 1), seed=23, lattice=hexl)
+    cfg = _cfg(npts=(6, 5, 2), ncycles=0.25)
+    if axes == "cartesian":
+        cfg.bz_axes = [1.2, 0, 0, 0, 1.2, 0, 0, 0, 0.6]
+    else:
+        cfg.bz_axes = list((2 * math.pi * np.linalg.inv(hexl).T).ravel())
+    print("case: hexagonal", axes, flush=True)
+    a, sa, bytes_a = _run_fourier_form(cfg, model, dense=False)
+    b, sb, bytes_b = _run_fourier_form(cfg, model, dense=True)
+    assert (bytes_a == bytes_b) == (axes == "cartesian")
+    errs = {k: rel_l2(a[k], b[k]) for k in ("j_inter", "j_intra", "band_pop", "rho_bar")}
+    errs["state"] = rel_l2(sa, sb)
+    print(axes, {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
     assert all(v < 1e-12 for v in errs.values()), errs
 
 
