@@ -48,6 +48,8 @@ SBE_MODEL_WILSON_MASS, SBE_MODEL_SQUARE_SBAND, SBE_MODEL_KANE_MELE, SBE_MODEL_XY
 SBE_MODEL_HALDANE_2L, SBE_MODEL_BI2SE3_SURF, SBE_MODEL_MULTI_WSM, SBE_MODEL_WDSM_TAAS = 8, 9, 10, 11
 SBE_MODEL_NPARAMS = 24
 SBE_BASIS_ORBITAL, SBE_BASIS_BAND = 0, 1
+SBE_NOALLOC_SCRATCH_BYTES = 16 << 20
+SBE_NOALLOC_K_ALIGN = 512
 
 c_double_p = C.POINTER(C.c_double)
 c_int32_p = C.POINTER(C.c_int32)
@@ -130,6 +132,7 @@ SIGNATURES = {
     "sbe_batch_create": ([C.c_int32, C.POINTER(sbe_config), C.POINTER(sbe_model), C.POINTER(C.c_void_p)], C.c_int),
     "sbe_batch_run": ([C.c_void_p, C.c_int64], C.c_int),
     "sbe_batch_synchronize": ([C.c_void_p], C.c_int),
+    "sbe_batch_reset": ([C.c_void_p], C.c_int),
     "sbe_batch_get_info": ([C.c_void_p, C.c_int32, C.POINTER(sbe_info)], C.c_int),
     "sbe_batch_get_series": ([C.c_void_p, C.c_int32, c_double_p, c_double_p, c_double_p, c_double_p], C.c_int),
     "sbe_batch_get_timing": ([C.c_void_p, c_double_p, C.POINTER(C.c_int64)], C.c_int),
@@ -147,6 +150,8 @@ SIGNATURES = {
     "sbe_server_last_error": ([C.c_void_p], C.c_char_p),
     "sbe_server_destroy": ([C.c_void_p], None),
     "sbe_destroy": ([C.c_void_p], None),
+    "sbe_run_haldane_noalloc": ([C.POINTER(sbe_config), C.POINTER(sbe_model), C.c_void_p, C.c_void_p, C.c_int64,
+                                 C.c_int64, C.c_int64, C.c_int64, C.c_int32, C.c_void_p, C.c_void_p], C.c_int),
     "sbe_last_error": ([C.c_void_p], C.c_char_p),
     "sbe_tmdc_params": ([C.c_char_p, C.c_int32, C.POINTER(sbe_model)], C.c_int),
     "sbe_hhg_points": ([C.c_int64], C.c_int64),

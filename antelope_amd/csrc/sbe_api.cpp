@@ -241,7 +241,43 @@ int timed(sbe_ctx *ctx, int cls, F &&fn, const char *what)
     return rc;
 }
 
+// Back to the initial state of a 2-band context, in stream order on s: the
+// state of sbe_create (SBEs.h:284-364), an empty series, step 0, and for the
+// adaptive stepper the cursor, the first step proposals (dt), the counters
+// and the failure flag. Used by sbe_create and sbe_batch_reset.
+int reset_2band(sbe_ctx *ctx, hipStream_t s)
+{
+    ctx->steps_done = 0;
+    const size_t series_bytes = static_cast<size_t>(ctx->laser.nt) * ctx->nv * sizeof(double);
+    int rc = hip_check(ctx, hipMemsetAsync(ctx->d_series, 0, series_bytes, s), "clear series");
+    if (ctx->adaptive && !rc) {
+        const std::vector<double> h0(static_cast<size_t>(ctx->nwaves), ctx->laser.dt);  // first proposal: dt
+        rc = hip_check(ctx, hipMemcpyAsync(ctx->d_hstep, h0.data(), h0.size() * sizeof(double), hipMemcpyHostToDevice, s),
+                       "upload step sizes");
+        if (!rc) rc = hip_check(ctx, hipMemsetAsync(ctx->d_counts, 0, 2 * sizeof(int64_t) * ctx->nwaves, s), "clear counts");
+        if (!rc) rc = hip_check(ctx, hipMemsetAsync(ctx->d_fail, 0, sizeof(int32_t), s), "clear flag");
+        if (!rc) rc = hip_check(ctx, hipMemsetAsync(ctx->d_cursor, 0, sizeof(int64_t), s), "clear cursor");
+        // the host vector must outlive the copy
+        if (!rc) rc = hip_check(ctx, hipStreamSynchronize(s), "reset sync");
+    }
+    if (rc) return rc;
+    if (ctx->velocity) return hip_check(ctx, sbe::launch_haldane_vel_init(ctx->hargs, s), "init kernel");
+    return hip_check(ctx, sbe::launch_haldane_init(ctx->hargs, ctx->cfg.init_type, ctx->cfg.fermi_energy, s),
+                     "init kernel");
+}
+
 }  // namespace
+
+namespace sbe {
+int validate_inputs(const sbe_config *cfg, const sbe_model *m, std::string &err) { return validate(cfg, m, err); }
+int set_thread_error(int code, const std::string &msg) { return set_error(nullptr, code, msg); }
+bool fixed_step_two_k_per_lane(const sbe_config &cfg, int64_t nk)
+{
+    Plan p;
+    make_plan(cfg, sbe_model{}, nk, 1, 4, p);
+    return p.kpt == 2;
+}
+}  // namespace sbe
 
 extern "C" {
 
@@ -517,7 +553,6 @@ int sbe_create(const sbe_config *cfg, const sbe_model *model, int64_t k_begin, i
             ctx->aargs.a_tab = d_at;
         }
     }
-    if (!rc) rc = hip_check(ctx, hipMemsetAsync(ctx->d_series, 0, p.bytes_series, ctx->stream), "clear series");
     if (ctx->adaptive && !rc) {
         sbe::AdaptiveArgs &A = ctx->aargs;
         A.h = h;
@@ -535,21 +570,13 @@ int sbe_create(const sbe_config *cfg, const sbe_model *model, int64_t k_begin, i
         A.hstep = ctx->d_hstep;
         A.counts = ctx->d_counts;
         A.fail = ctx->d_fail;
-        const std::vector<double> h0(static_cast<size_t>(ctx->nwaves), ctx->laser.dt);  // first proposal: dt
-        rc = hip_check(ctx, hipMemcpy(ctx->d_hstep, h0.data(), h0.size() * sizeof(double), hipMemcpyHostToDevice),
-                       "upload step sizes");
-        if (!rc)
-            rc = hip_check(ctx, hipMemcpy(ctx->d_pulses, ctx->laser.pulses.data(),
-                                          sizeof(sbe::PulseConst) * ctx->laser.pulses.size(), hipMemcpyHostToDevice),
-                           "upload pulses");
-        if (!rc) rc = hip_check(ctx, hipMemsetAsync(ctx->d_counts, 0, 2 * sizeof(int64_t) * ctx->nwaves, ctx->stream), "clear counts");
-        if (!rc) rc = hip_check(ctx, hipMemsetAsync(ctx->d_fail, 0, sizeof(int32_t), ctx->stream), "clear flag");
-        if (!rc) rc = hip_check(ctx, hipMemsetAsync(ctx->d_cursor, 0, sizeof(int64_t), ctx->stream), "clear cursor");
+        rc = hip_check(ctx, hipMemcpy(ctx->d_pulses, ctx->laser.pulses.data(),
+                                      sizeof(sbe::PulseConst) * ctx->laser.pulses.size(), hipMemcpyHostToDevice),
+                       "upload pulses");
         if (!rc)
             rc = hip_check(ctx, hipMemcpy(ctx->d_aargs, &A, sizeof A, hipMemcpyHostToDevice), "upload adaptive arguments");
     }
-    if (!rc && ctx->velocity) rc = hip_check(ctx, sbe::launch_haldane_vel_init(h, ctx->stream), "init kernel");
-    else if (!rc) rc = hip_check(ctx, sbe::launch_haldane_init(h, ctx->cfg.init_type, ctx->cfg.fermi_energy, ctx->stream), "init kernel");
+    if (!rc) rc = reset_2band(ctx, ctx->stream);
     if (!rc) rc = hip_check(ctx, hipStreamSynchronize(ctx->stream), "init sync");
     if (rc) {
         set_error(nullptr, rc, ctx->err);
@@ -817,6 +844,8 @@ struct sbe_batch {
     int kpt = 1;
     sbe::AdaptiveArgs *d_ainsts = nullptr;
     bool t1 = false, deph = false;
+    sbe::SeriesJob *d_jobs = nullptr;  // [n] reductions of one launch window
+    bool multi = false;                // every member's rows fit one reduction pass
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
     std::string err;
 };
@@ -846,6 +875,7 @@ void sbe_batch_destroy(sbe_batch *b)
     if (b->d_insts) (void)hipFree(b->d_insts);
     if (b->d_ainsts) (void)hipFree(b->d_ainsts);
     if (b->d_map) (void)hipFree(b->d_map);
+    if (b->d_jobs) (void)hipFree(b->d_jobs);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
@@ -890,6 +920,9 @@ int sbe_batch_create(int32_t n, const sbe_config *cfgs, const sbe_model *models,
     hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(&b->d_insts, sizeof(sbe::HaldaneArgs) * n);
     if (e == hipSuccess) e = hipMalloc(&b->d_map, sizeof(int32_t) * (b->max_blocks + n));
+    if (e == hipSuccess) e = hipMalloc(&b->d_jobs, sizeof(sbe::SeriesJob) * n);
+    b->multi = true;
+    for (sbe_ctx *c : b->ctx) b->multi = b->multi && sbe::series_multi_ok(c->nwaves);
     if (e == hipSuccess && b->adaptive) {
         // constant for the batch's lifetime: every solve's arguments and the block -> solve map
         // over all solves (solves with no samples left return at once)
@@ -921,6 +954,8 @@ int sbe_batch_run(sbe_batch *b, int64_t max_steps)
     int64_t budget = max_steps > 0 ? max_steps : INT64_MAX;
     std::vector<sbe::HaldaneArgs> insts(n);
     std::vector<int32_t> map(b->max_blocks + n);
+    std::vector<sbe::SeriesJob> jobs;
+    jobs.reserve(n);
     while (budget > 0) {
         int64_t launch = INT64_MAX;
         for (sbe_ctx *c : b->ctx)
@@ -961,12 +996,28 @@ int sbe_batch_run(sbe_batch *b, int64_t max_steps)
         }
         if (e == hipSuccess) e = hipEventRecord(eb, b->stream);
         if (ea && eb) b->ev.push_back({ea, eb});
+        // the k sums of this window: one launch for all members when each fits one pass
+        // (the same per-step arithmetic as a member's own reduction, so the same bits)
+        jobs.clear();
+        int64_t total = 0;
         for (int i = 0; i < n && e == hipSuccess; ++i) {
             sbe_ctx *c = b->ctx[i];
             if (insts[i].nsteps <= 0) continue;
-            e = sbe::launch_series_reduce(c->d_partials, insts[i].nsteps, c->nwaves, c->nv,
-                                          c->d_series + c->steps_done * c->nv, b->stream);
+            if (b->multi) {
+                jobs.push_back({c->d_partials, c->d_series + c->steps_done * c->nv, c->nwaves, insts[i].nsteps, total});
+                total += insts[i].nsteps;
+            } else {
+                e = sbe::launch_series_reduce(c->d_partials, insts[i].nsteps, c->nwaves, c->nv,
+                                              c->d_series + c->steps_done * c->nv, b->stream);
+            }
             c->steps_done += insts[i].nsteps;
+        }
+        if (e == hipSuccess && !jobs.empty()) {
+            e = hipMemcpyAsync(b->d_jobs, jobs.data(), sizeof(sbe::SeriesJob) * jobs.size(), hipMemcpyHostToDevice,
+                               b->stream);
+            if (e == hipSuccess)
+                e = sbe::launch_series_reduce_multi(b->d_jobs, static_cast<int>(jobs.size()), total, sbe::kHaldaneNV,
+                                                    b->stream);
         }
         if (e != hipSuccess) return batch_fail(b, SBE_ERR_HIP, std::string("batch launch: ") + hipGetErrorString(e));
         budget -= launch;
@@ -979,6 +1030,18 @@ int sbe_batch_synchronize(sbe_batch *b)
     if (!b) return batch_fail(nullptr, SBE_ERR_STATE, "null batch");
     const hipError_t e = hipStreamSynchronize(b->stream);
     return e == hipSuccess ? SBE_OK : batch_fail(b, SBE_ERR_HIP, hipGetErrorString(e));
+}
+
+int sbe_batch_reset(sbe_batch *b)
+{
+    if (!b) return batch_fail(nullptr, SBE_ERR_STATE, "null batch");
+    int rc = sbe_batch_synchronize(b);
+    if (rc) return rc;
+    for (sbe_ctx *c : b->ctx) {
+        rc = reset_2band(c, b->stream);
+        if (rc) return batch_fail(b, rc, c->err);
+    }
+    return sbe_batch_synchronize(b);
 }
 
 int sbe_batch_get_info(const sbe_batch *b, int32_t i, sbe_info *info)

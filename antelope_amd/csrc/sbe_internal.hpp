@@ -185,6 +185,19 @@ hipError_t launch_haldane_init(const HaldaneArgs &a, int init_type, double ef, h
 hipError_t launch_haldane_steps(const HaldaneArgs &a, int rk_order, bool t1_on, hipStream_t s);
 hipError_t launch_series_reduce(double *partials, int64_t nsteps, int32_t nrows, int nv,
                                 double *series, hipStream_t s);
+// One k block's share of a multi-block reduction: steps [first, first +
+// nsteps) of the launch grid sum partials[nsteps][nrows][nv] into series.
+struct SeriesJob {
+    const double *partials;
+    double *series;
+    int32_t nrows;
+    int32_t nsteps;
+    int64_t first;
+};
+// true when one pass suffices (what the multi-block launch runs)
+bool series_multi_ok(int32_t nrows);
+hipError_t launch_series_reduce_multi(const SeriesJob *d_jobs, int njobs, int64_t total_steps, int nv,
+                                      hipStream_t s);
 hipError_t launch_haldane_export(const HaldaneArgs &a, double *out, hipStream_t s);
 // VelocityHamiltonian gauge (haldane_velocity.hip): band-basis state
 hipError_t launch_haldane_vel_init(const HaldaneArgs &a, hipStream_t s);
@@ -206,4 +219,9 @@ int nb_run(sbe_ctx *ctx, int64_t step_begin, int64_t step_end);
 int nb_series(sbe_ctx *ctx, double *j_inter, double *j_intra, double *rho_bar, double *band_pop);
 int nb_state(sbe_ctx *ctx, int basis, double *rho);
 constexpr int kBlock = 256;
+// sbe_api.cpp: argument checks shared by the entry points, the thread-local
+// message of sbe_last_error(NULL), and the fixed-step plan's k points per lane
+int validate_inputs(const sbe_config *cfg, const sbe_model *m, std::string &err);
+int set_thread_error(int code, const std::string &msg);
+bool fixed_step_two_k_per_lane(const sbe_config &cfg, int64_t nk);
 }  // namespace sbe

@@ -73,7 +73,40 @@ __global__ __launch_bounds__(kReduceThreads) void series_group_kernel(double *__
     if (tid < vl && vb + tid < nv) p[g0 * nv + vb + tid] = sums[tid];
 }
 
+// many k blocks (the solves of a batch) in one launch: block (x, y) works
+// for the job whose steps [first, first + nsteps) hold x, and sums that
+// step exactly as series_reduce_kernel does for the job alone (one pass)
+__global__ __launch_bounds__(kReduceThreads) void series_reduce_multi_kernel(const SeriesJob *__restrict__ jobs,
+                                                                             int njobs, int nv)
+{
+    const int64_t g = blockIdx.x;
+    int lo = 0, hi = njobs - 1;  // last job with first <= g
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (jobs[mid].first <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    const SeriesJob &j = jobs[lo];
+    const int64_t step = g - j.first;
+    if (step >= j.nsteps) return;
+    const int vl = nv < 64 ? nv : 64;
+    reduce_rows(j.partials + step * static_cast<int64_t>(j.nrows) * nv, 0, 1, j.nrows, nv,
+                j.series + step * nv + blockIdx.y * vl);
+}
+
 }  // namespace
+
+bool series_multi_ok(int32_t nrows) { return nrows <= 2 * kGroupRows; }
+
+hipError_t launch_series_reduce_multi(const SeriesJob *d_jobs, int njobs, int64_t total_steps, int nv, hipStream_t s)
+{
+    if (njobs <= 0 || total_steps <= 0) return hipSuccess;
+    const int vl = nv < 64 ? nv : 64;
+    const unsigned vblocks = static_cast<unsigned>((nv + vl - 1) / vl);
+    series_reduce_multi_kernel<<<dim3(static_cast<unsigned>(total_steps), vblocks), kReduceThreads, 0, s>>>(
+        d_jobs, njobs, nv);
+    return hipGetLastError();
+}
 
 hipError_t launch_series_reduce(double *partials, int64_t nsteps, int32_t nrows, int nv, double *series,
                                 hipStream_t s)

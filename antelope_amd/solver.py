@@ -206,6 +206,10 @@ class SBEBatch:
     def synchronize(self) -> None:
         self._chk(self._lib.sbe_batch_synchronize(self._h))
 
+    def reset(self) -> None:
+        """Every solve back to its initial state (sbe_batch_reset)."""
+        self._chk(self._lib.sbe_batch_reset(self._h))
+
     def info(self, i: int) -> A.sbe_info:
         inf = A.sbe_info()
         self._chk(self._lib.sbe_batch_get_info(self._h, i, C.byref(inf)))

@@ -286,6 +286,9 @@ int sbe_batch_create(int32_t n, const sbe_config *cfgs, const sbe_model *models,
 /* Advance every unfinished solve by up to max_steps steps (<= 0: to the end). */
 int sbe_batch_run(sbe_batch *b, int64_t max_steps);
 int sbe_batch_synchronize(sbe_batch *b);
+/* Every solve back to its initial state (as after sbe_batch_create), with
+ * no allocation or table rebuild: a batch can be run again. */
+int sbe_batch_reset(sbe_batch *b);
 int sbe_batch_get_info(const sbe_batch *b, int32_t i, sbe_info *info);
 int sbe_batch_get_series(sbe_batch *b, int32_t i, double *j_inter, double *j_intra, double *rho_bar,
                          double *band_pop);
@@ -317,6 +320,34 @@ int sbe_set_timing(sbe_ctx *ctx, int32_t enable);
 int sbe_get_timing(sbe_ctx *ctx, double *ms /*[2]*/, int64_t *launches /*[2]*/);
 
 void sbe_destroy(sbe_ctx *ctx);
+
+/* ---- propagation with no device allocation. No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops with a fixed memory budget). The 2-band
+ * (Haldane) fixed-step path of sbe_create + sbe_run (FP64 state, RK4 or
+ * RK5, LengthWannier gauge) over caller-owned device buffers:
+ *   kpts    [4][nk_stride] FP64: k_x, k_y, k_z, weight; padding k points
+ *           (index >= nk) must carry weight 0
+ *   state   [4][nk_stride] FP64 Bloch vector (n, s_x, s_y, s_z) per k point,
+ *           rho = (n I + s.sigma)/2; read, propagated, written back.
+ *           init_state != 0 first writes the initial state of sbe_create.
+ *   series  [step_end - step_begin][8] FP64, written: per step n the k sums
+ *           before the step, sum_k w (j_inter_x, j_inter_y, j_intra_x,
+ *           j_intra_y, s_x, s_y, s_z, s.d/|d|) at k + A(t_n); sbe_get_series
+ *           forms j_inter, j_intra, rho_bar and band_pop from the same values.
+ * The laser rows and the partial sums of the k reduction use one static
+ * device array of SBE_NOALLOC_SCRATCH_BYTES per device (held until the
+ * process ends); k sets whose partial rows of one step exceed it are
+ * rejected with SBE_ERR_RANGE (about 16 M k points). The call only enqueues
+ * kernels on `stream` (a hipStream_t; NULL = the default stream) and returns;
+ * the caller synchronizes. Calls that overlap on one GPU race on the shared
+ * array. Results equal sbe_run's to rounding (the laser rows are evaluated
+ * on the device instead of the host). */
+#define SBE_NOALLOC_SCRATCH_BYTES (16u << 20)
+#define SBE_NOALLOC_K_ALIGN 512
+int sbe_run_haldane_noalloc(const sbe_config *cfg, const sbe_model *model, const double *kpts, double *state,
+                            int64_t nk, int64_t nk_stride, int64_t step_begin, int64_t step_end,
+                            int32_t init_state, double *series, void *stream);
 
 /* Last error of a context (ctx may be NULL: last error of this thread,
  * e.g. a failed sbe_create). Never NULL. */
