@@ -22,15 +22,16 @@
 //    lanes of the wave.
 //  * Samples. The k-summed observables (inter/intra currents, populations,
 //    rho sums) are needed every dt, while a step covers several dt. At every
-//    step end the kernel evaluates the observables of each k point together
-//    with their exact time derivatives (ds/dt is the stage f(t+h, y5), which
-//    is also the next step's first stage; d/dt of the k + A(t) dependence is
-//    the k gradient along dA/dt), reduces both over the wave, and writes each
-//    sample inside the step as the cubic Hermite interpolant of the two wave
-//    sums at its ends. A sample then costs a few FMAs per wave instead of a
-//    full per-k evaluation; its error is the Hermite error of the observable,
-//    the same order as interpolating the state, which was measured to give
-//    the same accuracy.
+//    step end the kernel forms the observables of each k point at the step
+//    end (the phases and d of the stage at c = 1, which is evaluated at the
+//    step end itself and also gives the next step's first stage), reduces
+//    them over the wave, and keeps the last six step ends in a ring; a
+//    sample is the Lagrange interpolant through the ring's nodes (centred:
+//    samples up to node c - 2 are written when node c arrives; one-sided at
+//    the window ends). A sample then costs a few FMAs per wave instead of a
+//    per-k evaluation. On the batched workload's worst solve this is as
+//    accurate as a cubic Hermite interpolant with exact time derivatives at
+//    two step ends, at a fraction of the cost (no derivatives).
 //  * FP32 state. Near the ground state s = -d/|d| + delta with delta small,
 //    so 2 d x s is a difference of O(|d|) products that FP32 resolves only
 //    to ~1e-7 |d| absolute. The FP32 kernel therefore propagates delta =
@@ -107,14 +108,53 @@ __device__ __forceinline__ double wave_max(double x)
 }
 
 constexpr int kRow = 14;  // 12 phase values, then dA/dt (x, y)
+// c of the Cash-Karp stages 2..6 by laser row (row 3 is c = 1 and row 5 the
+// step end: both are taken at the step end itself)
+__constant__ double kRowC[6] = {0.2, 0.3, 0.6, 1.0, 0.875, 1.0};
+static_assert(CashKarp::c[1] == 0.2 && CashKarp::c[2] == 0.3 && CashKarp::c[3] == 0.6 && CashKarp::c[4] == 1.0 &&
+                  CashKarp::c[5] == 0.875,
+              "laser rows follow the Cash-Karp stage times");
 
 // sin and cos of a small angle (|x| <= 0.25) by their Taylor series to x^13
-// and x^12; the first omitted terms are below 1e-18 relative.
+// and x^12 (Horner in x^2 with the inverse factorials as constants); the
+// first omitted terms are below 1e-18 relative.
 __device__ __forceinline__ void sincos_small(double x, double &sn, double &cs)
 {
     const double x2 = x * x;
-    sn = x * (1.0 - x2 / 6.0 * (1.0 - x2 / 20.0 * (1.0 - x2 / 42.0 * (1.0 - x2 / 72.0 * (1.0 - x2 / 110.0 * (1.0 - x2 / 156.0))))));
-    cs = 1.0 - x2 / 2.0 * (1.0 - x2 / 12.0 * (1.0 - x2 / 30.0 * (1.0 - x2 / 56.0 * (1.0 - x2 / 90.0 * (1.0 - x2 / 132.0)))));
+    double ps = 1.0 / 6227020800.0;  // 1/13!
+    ps = fma(ps, x2, -1.0 / 39916800.0);
+    ps = fma(ps, x2, 1.0 / 362880.0);
+    ps = fma(ps, x2, -1.0 / 5040.0);
+    ps = fma(ps, x2, 1.0 / 120.0);
+    ps = fma(ps, x2, -1.0 / 6.0);
+    sn = fma(ps * x2, x, x);
+    double pc = 1.0 / 479001600.0;  // 1/12!
+    pc = fma(pc, x2, -1.0 / 3628800.0);
+    pc = fma(pc, x2, 1.0 / 40320.0);
+    pc = fma(pc, x2, -1.0 / 720.0);
+    pc = fma(pc, x2, 1.0 / 24.0);
+    pc = fma(pc, x2, -0.5);
+    cs = fma(pc, x2, 1.0);
+}
+
+// sin and cos of a larger angle, out of line: only pulses that move A.v by
+// more than 0.25 rad within one sample interval take it, and its argument
+// reduction would otherwise hold registers in every caller
+__device__ __attribute__((noinline)) void sincos_large(double x, double *sn, double *cs) { sincos(x, sn, cs); }
+
+// x^(-1/5) for the step-size controller: FP32 hardware log2/exp2 for the
+// first estimate, then two Newton steps in FP64 (y <- y (6 - 5 x y^5) / ... :
+// y += y (1 - x y^5) / 5), within an ulp or two of pow(x, -0.2)
+__device__ __forceinline__ double inv_fifth_root(double x)
+{
+    const float lx = __builtin_amdgcn_logf(static_cast<float>(x));
+    double y = static_cast<double>(__builtin_amdgcn_exp2f(-0.2f * lx));
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const double y2 = y * y;
+        y = fma(y * (1.0 - x * y2 * y2 * y), 0.2, y);
+    }
+    return y;
 }
 
 // Laser rows of six times (the stages 2..6 of a Cash-Karp step and its end
@@ -134,35 +174,31 @@ __device__ __forceinline__ void laser_rows(const AdaptiveArgs &A, double t, doub
     const int lane = __lane_id();
     const int s = lane / 6, j = lane - 6 * s;  // phase j of time s (lanes < 36)
     if (lane < 36) {
-        double tau = tn;
-#pragma unroll
-        for (int q = 0; q < 5; ++q)
-            if (s == q) tau = t + CashKarp::c[q + 1] * hh;
-        const double x = (tau - A.atmin) / A.dt_out;
-        int64_t i = static_cast<int64_t>(floor(x));
-        i = i > A.nt - 2 ? A.nt - 2 : i;
+        // s = 5, and the stage at c = 1 (s = 3): the step end itself
+        const double tau = s == 3 || s == 5 ? tn : fma(kRowC[s], hh, t);
+        // interval of the sample table (the host keeps nt < 2^31, so 32-bit indices)
+        const double h = A.dt_out, inv_h = A.inv_dt_out;
+        const double x = (tau - A.atmin) * inv_h;
+        const int nt2 = static_cast<int>(A.nt) - 2;
+        int i = static_cast<int>(floor(x));
+        i = i > nt2 ? nt2 : i;
         i = i < 0 ? 0 : i;  // the table holds at least two samples
         const double th = x - static_cast<double>(i), th2 = th * th, th3 = th2 * th;
-        const double h = A.dt_out;
         const double *at = A.a_tab + 4 * i;  // A_x, A_y, dA_x/dt, dA_y/dt at t_i, then at t_i+1
         const double h01 = 3.0 * th2 - 2.0 * th3, h10 = (th3 - 2.0 * th2 + th) * h, h11 = (th3 - th2) * h;
-        const double g01 = (6.0 * th - 6.0 * th2) / h, g10 = 3.0 * th2 - 4.0 * th + 1.0, g11 = 3.0 * th2 - 2.0 * th;
+        const double g01 = (6.0 * th - 6.0 * th2) * inv_h, g10 = 3.0 * th2 - 4.0 * th + 1.0, g11 = 3.0 * th2 - 2.0 * th;
         const double dAx = at[4] - at[0], dAy = at[5] - at[1];
         const double ux = h01 * dAx + h10 * at[2] + h11 * at[6];  // A(tau) - A(t_i)
         const double uy = h01 * dAy + h10 * at[3] + h11 * at[7];
         const double gx = g01 * dAx + g10 * at[2] + g11 * at[6];  // dA/dt at tau
         const double gy = g01 * dAy + g10 * at[3] + g11 * at[7];
         const HaldaneArgs &a = A.h;
-        double vx = 0.0, vy = 0.0;
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            if (j == q) vx = a.va[q][0], vy = a.va[q][1];
-            if (j == q + 3) vx = a.vb[q][0], vy = a.vb[q][1];
-        }
+        const double *vrow = j < 3 ? a.va[j] : a.vb[j - 3];
+        const double vx = vrow[0], vy = vrow[1];
         const double del = ux * vx + uy * vy;
         double sn, cs;
         if (fabs(del) <= 0.25) sincos_small(del, sn, cs);
-        else sincos(del, &sn, &cs);
+        else sincos_large(del, &sn, &cs);
         const double *ph = a.phase_tab + 12 * i + 2 * j;  // scaled e^{iA(t_i).v_j}
         const double re = ph[0] * cs - ph[1] * sn, im = ph[1] * cs + ph[0] * sn;
         rows[s * kRow + 2 * j] = static_cast<R>(re);
@@ -365,6 +401,55 @@ __device__ __forceinline__ void kahan(R &sum, R &c, R inc)
     sum = t;
 }
 
+// Samples from the last six step ends of a wave: the interpolating
+// polynomial through them (fewer than six right after a window start) in
+// Newton form on the newest nodes,
+//   p(tau) = D_0 + (tau - t_n)(D_1 + (tau - t_n-1)(D_2 + ... (tau - t_n-4) D_5)),
+// with D_k = f[t_n, ..., t_n-k] the backward divided differences. A new node
+// updates them in place, D'_0 = f_new, D'_k = (D'_k-1 - D_k-1) / (t_new - t_n+1-k),
+// and the oldest node drops out of the stencil by itself. The node times are
+// uniform (in the wave's LDS slice); each lane keeps the differences of its
+// wave-summed observable (value lane >> 3).
+struct SampleRing {
+    double D[6];  // divided differences of this lane's value, newest node first
+    double *t;    // [6] node times, newest first (LDS)
+    int cnt;      // nodes pushed in this window
+
+    __device__ __forceinline__ void push(double tn, double val)
+    {
+        double prev = D[0];
+        D[0] = val;
+#pragma unroll
+        for (int k = 1; k < 6; ++k) {
+            const double old = D[k];
+            // t[k - 1] is node n + 1 - k (before the shift below); nodes not yet pushed give 0
+            D[k] = k <= cnt ? (D[k - 1] - prev) * recip(tn - t[k - 1]) : 0.0;
+            prev = old;
+        }
+        double tt[6];
+        tt[0] = tn;
+#pragma unroll
+        for (int i = 1; i < 6; ++i) tt[i] = t[i - 1];
+        ++cnt;
+        // every lane holds the same times: the stores agree; reads come after the wave barrier
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int i = 0; i < 6; ++i) t[i] = tt[i];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+
+    // p(tau), with the node times tk[6] read from t beforehand
+    __device__ __forceinline__ double at(double tau, const double tk[6]) const
+    {
+        double acc = D[5];
+#pragma unroll
+        for (int k = 4; k >= 0; --k) acc = fma(acc, tau - tk[k], D[k]);
+        return acc;
+    }
+};
+
 // One wave owns the 64 * KPT consecutive k points [64 KPT w, 64 KPT (w+1))
 // of its solve (lane l: points 64 q + l, q < KPT); the wave is the unit of
 // step control. KPT = 2 gives each lane two independent dependency chains
@@ -492,13 +577,35 @@ __device__ void adaptive_body(const AdaptiveArgs &A, int64_t blk, const int64_t 
     int64_t nacc = 0, nrej = 0;
     bool failed = false;
 
-    // Step ends carry the wave sums of the observables and of their time
-    // derivatives (this lane: value lane >> 3); samples in between are the
-    // cubic Hermite interpolant of the two ends. At the window start: the
-    // sample itself, and the first stage.
+    // Step ends carry the wave sums of the observables (this lane: value
+    // lane >> 3) into the six-node ring; samples are its Lagrange
+    // interpolants. At the window start: the sample itself, and the first
+    // stage.
     Vec3<R> k1[KPT];
     Vec3<R> dh0[KPT];  // delta form: dh at the current step start (error scale)
-    double O0, D0;
+    __shared__ double lds_ring[kBlock / 64][6];
+    SampleRing ring;
+    ring.cnt = 0;
+    ring.t = lds_ring[threadIdx.x >> 6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) ring.D[i] = 0.0;
+    for (int i = 0; i < 6; ++i) ring.t[i] = 0.0;
+    auto emit = [&](double t_hi) {
+        double tk[6];  // the node times, from LDS once per call
+#pragma unroll
+        for (int k = 0; k < 6; ++k) tk[k] = ring.t[k];
+        double *dst = part + (m - out0) * row_stride;
+        int mi = static_cast<int>(m);  // nt < 2^31 (host check): exact conversion
+        const int mend = static_cast<int>(out1);
+        while (mi < mend) {
+            const double tm = A.atmin + A.dt_out * static_cast<double>(mi);
+            if (tm > t_hi) break;
+            *dst = ring.at(tm, tk);
+            dst += row_stride;
+            ++mi;
+        }
+        m = mi;
+    };
     {
         laser_rows<R>(A, t, 0.0, t, rows, row64);
         const R *row = rows + 5 * kRow;
@@ -520,9 +627,9 @@ __device__ void adaptive_body(const AdaptiveArgs &A, int64_t blk, const int64_t 
                            Vec3<R>{2 * k1[q].x, 2 * k1[q].y, 2 * k1[q].z}, row, a, ii, o, od);
             }
         }
-        O0 = wave_reduce8(o);
-        D0 = wave_reduce8(od);
+        const double O0 = wave_reduce8(o);
         part[(m - out0) * row_stride] = O0;
+        ring.push(t, O0);
         ++m;
     }
 
@@ -620,19 +727,8 @@ __device__ void adaptive_body(const AdaptiveArgs &A, int64_t blk, const int64_t 
                 else observe<R>(p[q], p[q].n - p[q].cn, Vec3<R>{p[q].sx - p[q].cx, p[q].sy - p[q].cy, p[q].sz - p[q].cz},
                                 Vec3<R>{2 * k1[q].x, 2 * k1[q].y, 2 * k1[q].z}, row7, a, ii, o, od);
             }
-            const double O1 = wave_reduce8(o), D1 = wave_reduce8(od);
-            // samples inside (t, tn]
-            while (m < out1) {
-                const double tm = A.atmin + A.dt_out * static_cast<double>(m);
-                if (tm > tn) break;
-                const double th = (tm - t) / hh, th2 = th * th, th3 = th2 * th;
-                const double h00 = 2.0 * th3 - 3.0 * th2 + 1.0, h01 = 3.0 * th2 - 2.0 * th3;
-                const double h10 = (th3 - 2.0 * th2 + th) * hh, h11 = (th3 - th2) * hh;
-                part[(m - out0) * row_stride] = h00 * O0 + h10 * D0 + h01 * O1 + h11 * D1;
-                ++m;
-            }
-            O0 = O1;
-            D0 = D1;
+            ring.push(tn, wave_reduce8(o));
+            if (ring.cnt >= 6) emit(ring.t[2]);  // samples up to node c - 2
             t = tn;
             ++nacc;
             const double fac = err > 0.0 ? fmin(5.0, fmax(0.2, 0.9 * pow(err, -0.2))) : 5.0;
@@ -643,6 +739,7 @@ __device__ void adaptive_body(const AdaptiveArgs &A, int64_t blk, const int64_t 
         }
     }
 
+    if (!failed) emit(t_end);  // the window end: every pending sample
     if constexpr (kDelta) {  // delta -> s at the window end (row64 holds the last step end, t_end)
 #pragma unroll
         for (int q = 0; q < KPT; ++q) {
@@ -675,6 +772,219 @@ __device__ void adaptive_body(const AdaptiveArgs &A, int64_t blk, const int64_t 
     }
 }
 
+// f/2 of the Bloch equation at d (1/|d|^2 = inv): d x y - (g2/2) y + ((g2 - g1)/2) (y.d/|d|^2) d
+template <bool DEPH>
+__device__ __forceinline__ Vec3<double> half_f_at(const Vec3<double> &d, double inv, const Vec3<double> &y, double gh2,
+                                                  double gh12)
+{
+    Vec3<double> f;
+    f.x = d.y * y.z - d.z * y.y;
+    f.y = d.z * y.x - d.x * y.z;
+    f.z = d.x * y.y - d.y * y.x;
+    if (DEPH) {
+        const double c = -gh12 * (y.x * d.x + y.y * d.y + y.z * d.z) * inv;
+        f.x += c * d.x - gh2 * y.x;
+        f.y += c * d.y - gh2 * y.y;
+        f.z += c * d.z - gh2 * y.z;
+    }
+    return f;
+}
+
+// The FP64 stepper. Per accepted step of a wave: five new stage times (the
+// stage at c = 1 is the step end, whose phases and d also give the next
+// step's first stage and the observables), the FP64 error estimate and one
+// 8-value wave reduction of the observables at the step end. Samples are
+// written two step ends later, from the six-node ring (centred stencil;
+// one-sided at the window ends, where all pending samples are written).
+template <bool DEPH, int KPT>
+__device__ void adaptive_body64(const AdaptiveArgs &A, int64_t blk, const int64_t out0, const int64_t out1)
+{
+    using CK = CashKarp;
+    const HaldaneArgs &a = A.h;
+    const int64_t ks = a.nk_stride;
+    const int lane = __lane_id();
+    const int wave = static_cast<int>(blk * (kBlock / 64) + (threadIdx.x >> 6));
+    const int64_t kbase = static_cast<int64_t>(wave) * 64 * KPT + lane;
+    double *st = static_cast<double *>(a.state);
+
+    KPoint<double> p[KPT];
+#pragma unroll
+    for (int q = 0; q < KPT; ++q) {
+        const int64_t k = kbase + 64 * q;
+        const double kx = a.kpts[k], ky = a.kpts[ks + k];
+        p[q].w = a.weight[k];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            sincos(kx * a.va[j][0] + ky * a.va[j][1], &p[q].ei[j], &p[q].er[j]);
+            sincos(kx * a.vb[j][0] + ky * a.vb[j][1], &p[q].ei[j + 3], &p[q].er[j + 3]);
+        }
+        p[q].n = st[k];
+        p[q].sx = st[ks + k];
+        p[q].sy = st[2 * ks + k];
+        p[q].sz = st[3 * ks + k];
+    }
+    const double c3 = a.c3, M0 = a.M0;
+    const double gh2 = 0.5 * a.gam2, gh12 = -0.5 * a.gam2;  // T1 off on this path
+    const double ii = a.inter_intra ? 1.0 : 0.0;
+    double *part = a.partials + static_cast<int64_t>(wave) * kHaldaneNV + (lane >> 3);
+    const int64_t row_stride = static_cast<int64_t>(a.nwaves) * kHaldaneNV;
+    __shared__ double lds_rows[kBlock / 64][6 * kRow];
+    double *rows = lds_rows[threadIdx.x >> 6];
+
+    // the arguments used in the loops, in registers (stores through part could alias A)
+    const double atmin = A.atmin, dt_out = A.dt_out, atol = A.atol, rtol = A.rtol, hmax = A.hmax, hmin = A.hmin;
+    int64_t m = out0;
+    double t = atmin + dt_out * static_cast<double>(m);
+    const double t_end = atmin + dt_out * static_cast<double>(out1);
+    const int64_t max_iter = 64 * (out1 - out0) + 1024;
+    double h = uniform(A.hstep[wave]);
+    int64_t nacc = 0, nrej = 0;
+    bool failed = false;
+    __shared__ double lds_ring[kBlock / 64][6];
+    SampleRing ring;
+    ring.cnt = 0;
+    ring.t = lds_ring[threadIdx.x >> 6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) ring.D[i] = 0.0;
+    for (int i = 0; i < 6; ++i) ring.t[i] = 0.0;
+    // samples up to t_hi from the ring
+    auto emit = [&](double t_hi) {
+        double tk[6];  // the node times, from LDS once per call
+#pragma unroll
+        for (int k = 0; k < 6; ++k) tk[k] = ring.t[k];
+        double *dst = part + (m - out0) * row_stride;
+        int mi = static_cast<int>(m);  // nt < 2^31 (host check): exact conversion
+        const int mend = static_cast<int>(out1);
+        while (mi < mend) {
+            const double tm = atmin + dt_out * static_cast<double>(mi);
+            if (tm > t_hi) break;
+            *dst = ring.at(tm, tk);
+            dst += row_stride;
+            ++mi;
+        }
+        m = mi;
+    };
+
+    Vec3<double> k1[KPT];
+    {  // the window start: its sample, the first node and the first stage
+        laser_rows<double>(A, t, 0.0, t, rows);
+        double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int q = 0; q < KPT; ++q) {
+            const Phases<double> e = phases_at(p[q], rows + 5 * kRow, c3, M0);
+            const Vec3<double> y{p[q].sx, p[q].sy, p[q].sz};
+            k1[q] = half_f_at<DEPH>(e.d, e.inv, y, gh2, gh12);
+            sample_state(e, p[q].w, p[q].n, y.x, y.y, y.z, a, ii, v);
+        }
+        const double val = wave_reduce8(v);
+        part[0] = val;
+        ring.push(t, val);
+        ++m;
+    }
+
+    int64_t iter = 0;
+    while (t < t_end) {
+        if (++iter > max_iter || !(h >= hmin)) {
+            failed = true;
+            break;
+        }
+        double hh = fmin(h, hmax);
+        const bool last = t + 1.0001 * hh >= t_end;
+        if (last) hh = t_end - t;
+        const double tn = last ? t_end : t + hh;
+        laser_rows<double>(A, t, hh, tn, rows);
+
+        Vec3<double> inc[KPT];
+        double err_k = 0.0;
+#pragma unroll
+        for (int q = 0; q < KPT; ++q) {
+            Vec3<double> kv[6];
+            kv[0] = k1[q];
+            const Vec3<double> y0{p[q].sx, p[q].sy, p[q].sz};
+#pragma unroll
+            for (int i = 1; i < 6; ++i) {
+                Vec3<double> y = y0;
+#pragma unroll
+                for (int j = 0; j < i; ++j) {
+                    if (CK::a[i][j] != 0.0) {
+                        const double aij = 2.0 * CK::a[i][j] * hh;
+                        y.x = fma(aij, kv[j].x, y.x);
+                        y.y = fma(aij, kv[j].y, y.y);
+                        y.z = fma(aij, kv[j].z, y.z);
+                    }
+                }
+                const Phases<double> e = phases_at(p[q], rows + (i - 1) * kRow, c3, M0);
+                kv[i] = half_f_at<DEPH>(e.d, e.inv, y, gh2, gh12);
+            }
+            inc[q] = Vec3<double>{0, 0, 0};
+            double ex = 0, ey = 0, ez = 0;
+#pragma unroll
+            for (int i = 0; i < 6; ++i) {
+                if (CK::b[i] != 0.0) {
+                    const double bi = 2.0 * CK::b[i] * hh;
+                    inc[q].x = fma(bi, kv[i].x, inc[q].x);
+                    inc[q].y = fma(bi, kv[i].y, inc[q].y);
+                    inc[q].z = fma(bi, kv[i].z, inc[q].z);
+                }
+                if (CK::e[i] != 0.0) {
+                    const double ei = 2.0 * CK::e[i] * hh;
+                    ex = fma(ei, kv[i].x, ex);
+                    ey = fma(ei, kv[i].y, ey);
+                    ez = fma(ei, kv[i].z, ez);
+                }
+            }
+            auto ratio = [&](double e, double y, double z) {  // 0 when the component and its error vanish
+                const double sc = atol + rtol * fmax(fabs(y), fabs(y + z));
+                return e == 0.0 ? 0.0 : fabs(e) * recip(sc);
+            };
+            err_k = nanmax(err_k, nanmax(ratio(ex, y0.x, inc[q].x),
+                                         nanmax(ratio(ey, y0.y, inc[q].y), ratio(ez, y0.z, inc[q].z))));
+        }
+        const double err = wave_max(err_k);
+
+        if (err <= 1.0) {
+            double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+            for (int q = 0; q < KPT; ++q) {
+                p[q].sx += inc[q].x;
+                p[q].sy += inc[q].y;
+                p[q].sz += inc[q].z;
+                const Vec3<double> y{p[q].sx, p[q].sy, p[q].sz};
+                // the phases of the stage at c = 1 (the step end), formed again from the same
+                // row with the same operations rather than kept through the last stage
+                const Phases<double> e1 = phases_at(p[q], rows + 5 * kRow, c3, M0);
+                k1[q] = half_f_at<DEPH>(e1.d, e1.inv, y, gh2, gh12);  // f(t + h, y5)
+                sample_state(e1, p[q].w, p[q].n, y.x, y.y, y.z, a, ii, v);
+            }
+            ring.push(tn, wave_reduce8(v));
+            if (ring.cnt >= 6) emit(ring.t[2]);  // samples up to node c - 2
+            t = tn;
+            ++nacc;
+            const double fac = err > 0.0 ? fmin(5.0, fmax(0.2, 0.9 * inv_fifth_root(err))) : 5.0;
+            if (!last) h = hh * fac;  // a shortened last step keeps the proposal it was cut from
+        } else {
+            h = hh * fmax(0.2, 0.9 * inv_fifth_root(err));  // NaN err -> 0.2: shrinks to hmin and fails
+            ++nrej;
+        }
+    }
+    if (!failed) emit(t_end);  // the window end: every pending sample
+
+#pragma unroll
+    for (int q = 0; q < KPT; ++q) {
+        const int64_t k = kbase + 64 * q;
+        st[k] = p[q].n;
+        st[ks + k] = p[q].sx;
+        st[2 * ks + k] = p[q].sy;
+        st[3 * ks + k] = p[q].sz;
+    }
+    if (lane == 0) {
+        A.hstep[wave] = h;
+        A.counts[2 * wave] += nacc;
+        A.counts[2 * wave + 1] += nrej;
+        if (failed) *A.fail = 1;
+    }
+}
+
 // Block b works for solve block_inst[b] on that solve's block
 // b - block_first[solve] (a single solve: block_inst null, solve 0). The
 // window is the next nwin samples after the solve's cursor; solves with none
@@ -682,7 +992,7 @@ __device__ void adaptive_body(const AdaptiveArgs &A, int64_t blk, const int64_t 
 // same bits alone and in a batch.
 template <typename R, bool DEPH, int KPT>
 __global__ __launch_bounds__(kBlock)
-__attribute__((amdgpu_waves_per_eu(KPT == 1 && sizeof(R) == 8 ? 2 : 1))) void haldane_adaptive_kernel(const AdaptiveArgs *__restrict__ insts,
+__attribute__((amdgpu_waves_per_eu(KPT == 1 && sizeof(R) == 8 ? 3 : 1))) void haldane_adaptive_kernel(const AdaptiveArgs *__restrict__ insts,
                                                                   const int32_t *__restrict__ block_inst,
                                                                   const int32_t *__restrict__ block_first,
                                                                   int64_t nwin)
@@ -693,7 +1003,8 @@ __attribute__((amdgpu_waves_per_eu(KPT == 1 && sizeof(R) == 8 ? 2 : 1))) void ha
     const int64_t out1 = min(out0 + nwin, A.nt);
     if (out1 <= out0) return;  // whole block
     const int64_t blk = static_cast<int64_t>(blockIdx.x) - (block_inst ? block_first[inst] : 0);
-    adaptive_body<R, DEPH, KPT>(A, blk, out0, out1);
+    if constexpr (sizeof(R) == 8) adaptive_body64<DEPH, KPT>(A, blk, out0, out1);
+    else adaptive_body<R, DEPH, KPT>(A, blk, out0, out1);
 }
 
 __global__ void adaptive_advance_kernel(const AdaptiveArgs *__restrict__ insts, int n, int64_t nwin)

@@ -393,6 +393,8 @@ int sbe_create(const sbe_config *cfg, const sbe_model *model, int64_t k_begin, i
         else if (ctx->cfg.T1 > 1e-16) why = "T1 population decay is not built for the adaptive Cash-Karp stepper";
         else if (ctx->cfg.npulses > sbe::kMaxDevicePulses)
             why = "the adaptive Cash-Karp stepper takes at most " + std::to_string(sbe::kMaxDevicePulses) + " pulses";
+        else if (ctx->laser.nt >= (int64_t(1) << 31))
+            why = "the adaptive Cash-Karp stepper takes fewer than 2^31 samples", rc = SBE_ERR_RANGE;
         else if (!(ctx->cfg.rtol > 0.0) || !(ctx->cfg.atol >= 0.0) || !std::isfinite(ctx->cfg.rtol) ||
                  !std::isfinite(ctx->cfg.atol))
             why = "calc.rtol must be > 0 and calc.atol >= 0", rc = SBE_ERR_CONFIG;
@@ -558,6 +560,7 @@ int sbe_create(const sbe_config *cfg, const sbe_model *model, int64_t k_begin, i
         A.h = h;
         A.atmin = ctx->laser.atmin;
         A.dt_out = ctx->laser.dt;
+        A.inv_dt_out = 1.0 / ctx->laser.dt;
         A.nt = ctx->laser.nt;
         A.cursor = ctx->d_cursor;
         A.rtol = ctx->cfg.rtol;

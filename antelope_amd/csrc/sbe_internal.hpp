@@ -97,8 +97,8 @@ constexpr int kMaxDevicePulses = 8;
 // per launch and one kernel serves single solves and batches alike.
 struct AdaptiveArgs {
     HaldaneArgs h;             // model, grid, state, partials (phase_tab: rows at t_n only)
-    double atmin, dt_out;
-    int64_t nt;
+    double atmin, dt_out, inv_dt_out;
+    int64_t nt;                // < 2^31 (the kernels index samples with 32 bits)
     int64_t *cursor;           // next sample; the state is at t_cursor
     double rtol, atol, hmax, hmin;
     int32_t npulses;
