@@ -103,7 +103,7 @@ class sbe_request_status(C.Structure):
                 ("t_admit", C.c_double), ("t_done", C.c_double)]
 
 
-SBE_REQ_QUEUED, SBE_REQ_RUNNING, SBE_REQ_DONE, SBE_REQ_FAILED = range(4)
+SBE_REQ_QUEUED, SBE_REQ_RUNNING, SBE_REQ_DONE, SBE_REQ_FAILED, SBE_REQ_CANCELLED = range(5)
 
 
 class sbe_step_stats(C.Structure):
@@ -147,6 +147,7 @@ SIGNATURES = {
     "sbe_server_get_series": ([C.c_void_p, C.c_int64, c_double_p, c_double_p, c_double_p, c_double_p], C.c_int),
     "sbe_server_request_error": ([C.c_void_p, C.c_int64], C.c_char_p),
     "sbe_server_release": ([C.c_void_p, C.c_int64], C.c_int),
+    "sbe_server_cancel": ([C.c_void_p, C.c_int64], C.c_int),
     "sbe_server_last_error": ([C.c_void_p], C.c_char_p),
     "sbe_server_destroy": ([C.c_void_p], None),
     "sbe_destroy": ([C.c_void_p], None),

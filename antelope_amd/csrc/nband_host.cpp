@@ -516,7 +516,8 @@ int nb_run_velocity(sbe_ctx *ctx, int64_t step_begin, int64_t step_end)
                       "velocity step kernel");
         }
         if (!rc)
-            rc = hipc(ctx, launch_series_reduce(ctx->d_partials, n, a.sample_waves, a.nv, ctx->d_series + c0 * a.nv, ctx->stream),
+            rc = hipc(ctx, launch_series_reduce(ctx->d_partials, n, a.sample_waves, a.nv, ctx->d_series + c0 * a.nv, ctx->stream,
+                                          ctx->nonfinite_d),
                       "series kernel");
         if (!rc) ctx->steps_done = c0 + n;
     }
@@ -614,7 +615,8 @@ int nb_run(sbe_ctx *ctx, int64_t step_begin, int64_t step_end)
             }
         }
         if (!rc)
-            rc = hipc(ctx, launch_series_reduce(ctx->d_partials, n, a.sample_waves, a.nv, ctx->d_series + c0 * a.nv, ctx->stream),
+            rc = hipc(ctx, launch_series_reduce(ctx->d_partials, n, a.sample_waves, a.nv, ctx->d_series + c0 * a.nv, ctx->stream,
+                                          ctx->nonfinite_d),
                       "series kernel");
         if (!rc) ctx->steps_done = c0 + n;
     }

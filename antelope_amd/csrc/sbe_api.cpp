@@ -248,6 +248,7 @@ int timed(sbe_ctx *ctx, int cls, F &&fn, const char *what)
 int reset_2band(sbe_ctx *ctx, hipStream_t s)
 {
     ctx->steps_done = 0;
+    *ctx->nonfinite_h = 0;  // no reduction of this context is in flight here
     const size_t series_bytes = static_cast<size_t>(ctx->laser.nt) * ctx->nv * sizeof(double);
     int rc = hip_check(ctx, hipMemsetAsync(ctx->d_series, 0, series_bytes, s), "clear series");
     if (ctx->adaptive && !rc) {
@@ -269,6 +270,103 @@ int reset_2band(sbe_ctx *ctx, hipStream_t s)
 }  // namespace
 
 namespace sbe {
+int create_batch_member(const sbe_config *cfg, const sbe_model *model, sbe_ctx **out)
+{
+    t_partial_bytes = kBatchPartialBytes;
+    t_batch_member = true;
+    const int rc = sbe_create(cfg, model, -1, -1, out);
+    t_partial_bytes = kPartialBytes;
+    t_batch_member = false;
+    return rc;
+}
+
+bool shared_eligible(const sbe_ctx *c)
+{
+    return c->path == PathKind::Haldane2 && !c->adaptive && !c->velocity && c->hargs.kpt == 1;
+}
+
+int shared_key(const sbe_ctx *c) { return c->cfg.rk_order * 16 + c->cfg.precision; }
+
+void free_shared(SharedLaunch &buf)
+{
+    if (buf.d_insts) (void)hipFree(buf.d_insts);
+    if (buf.d_map) (void)hipFree(buf.d_map);
+    if (buf.d_jobs) (void)hipFree(buf.d_jobs);
+    buf = SharedLaunch{};
+}
+
+int launch_shared(sbe_ctx *const *cs, const int64_t *nsteps, int n, SharedLaunch &buf, hipStream_t s, std::string &err)
+{
+    if (n <= 0) return SBE_OK;
+    int blocks = 0;
+    bool t1 = false, deph = false, multi = true;
+    for (int i = 0; i < n; ++i) {
+        const sbe_ctx *c = cs[i];
+        if (!shared_eligible(c) || shared_key(c) != shared_key(cs[0]) || nsteps[i] <= 0 ||
+            nsteps[i] > c->chunk_steps || c->steps_done + nsteps[i] > c->laser.nt) {
+            err = "shared launch: member " + std::to_string(i) + " cannot take " + std::to_string(nsteps[i]) + " steps";
+            return SBE_ERR_STATE;
+        }
+        blocks += static_cast<int>(c->nk_stride / kBlock);
+        t1 = t1 || c->hargs.gam1 != 0.0;
+        deph = deph || c->hargs.gam1 != 0.0 || c->hargs.gam2 != 0.0;
+        multi = multi && series_multi_ok(c->nwaves);
+    }
+    hipError_t e = hipSuccess;
+    if (n > buf.cap_insts || blocks + n > buf.cap_map) {  // grow: in-flight launches read the old buffers
+        e = hipStreamSynchronize(s);
+        const int ci = std::max(n, 2 * buf.cap_insts), cm = std::max(blocks + n, 2 * buf.cap_map);
+        free_shared(buf);
+        if (e == hipSuccess) e = hipMalloc(&buf.d_insts, sizeof(HaldaneArgs) * ci);
+        if (e == hipSuccess) e = hipMalloc(&buf.d_map, sizeof(int32_t) * cm);
+        if (e == hipSuccess) e = hipMalloc(&buf.d_jobs, sizeof(SeriesJob) * ci);
+        if (e != hipSuccess) {
+            free_shared(buf);
+            err = std::string("shared launch buffers: ") + hipGetErrorString(e);
+            return SBE_ERR_OOM;
+        }
+        buf.cap_insts = ci;
+        buf.cap_map = cm;
+    }
+    std::vector<HaldaneArgs> insts(n);
+    std::vector<int32_t> map(static_cast<size_t>(blocks) + n);
+    std::vector<SeriesJob> jobs;
+    int nb = 0;
+    int64_t total = 0;
+    for (int i = 0; i < n; ++i) {
+        const sbe_ctx *c = cs[i];
+        insts[i] = c->hargs;
+        insts[i].step0 = c->steps_done;
+        insts[i].nsteps = static_cast<int32_t>(nsteps[i]);
+        map[blocks + i] = nb;
+        for (int64_t q = 0; q < c->nk_stride / kBlock; ++q) map[nb++] = i;
+        jobs.push_back({c->d_partials, c->d_series + c->steps_done * c->nv, c->nwaves, insts[i].nsteps, total,
+                        c->nonfinite_d});
+        total += nsteps[i];
+    }
+    e = hipMemcpyAsync(buf.d_insts, insts.data(), sizeof(HaldaneArgs) * n, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(buf.d_map, map.data(), sizeof(int32_t) * map.size(), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+        e = launch_haldane_batch(buf.d_insts, buf.d_map, buf.d_map + blocks, blocks, cs[0]->cfg.rk_order,
+                                 cs[0]->cfg.precision, t1, deph, s);
+    if (e == hipSuccess && multi) {
+        e = hipMemcpyAsync(buf.d_jobs, jobs.data(), sizeof(SeriesJob) * n, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = launch_series_reduce_multi(buf.d_jobs, n, total, kHaldaneNV, s);
+    }
+    for (int i = 0; i < n && e == hipSuccess; ++i) {
+        sbe_ctx *c = cs[i];
+        if (!multi)
+            e = launch_series_reduce(c->d_partials, nsteps[i], c->nwaves, c->nv, c->d_series + c->steps_done * c->nv, s,
+                                     c->nonfinite_d);
+        c->steps_done += nsteps[i];
+    }
+    if (e != hipSuccess) {
+        err = std::string("shared launch: ") + hipGetErrorString(e);
+        return SBE_ERR_HIP;
+    }
+    return SBE_OK;
+}
+
 int validate_inputs(const sbe_config *cfg, const sbe_model *m, std::string &err) { return validate(cfg, m, err); }
 int set_thread_error(int code, const std::string &msg) { return set_error(nullptr, code, msg); }
 bool fixed_step_two_k_per_lane(const sbe_config &cfg, int64_t nk)
@@ -413,6 +511,13 @@ int sbe_create(const sbe_config *cfg, const sbe_model *model, int64_t k_begin, i
         return SBE_ERR_NODEVICE;
     }
     (void)hipGetDevice(&ctx->device);
+    if (hipHostMalloc(reinterpret_cast<void **>(&ctx->nonfinite_h), sizeof(int32_t), hipHostMallocMapped) != hipSuccess ||
+        hipHostGetDevicePointer(reinterpret_cast<void **>(&ctx->nonfinite_d), ctx->nonfinite_h, 0) != hipSuccess) {
+        set_error(nullptr, SBE_ERR_OOM, "pinned host flag");
+        sbe_destroy(ctx);
+        return SBE_ERR_OOM;
+    }
+    *ctx->nonfinite_h = 0;
 
     if (model->kind != SBE_MODEL_HALDANE) {
         ctx->path = sbe::PathKind::NBand;
@@ -616,7 +721,7 @@ int sbe_run(sbe_ctx *ctx, int64_t step_begin, int64_t step_end)
             if (!rc)
                 rc = timed(ctx, 1, [&] {
                     return sbe::launch_series_reduce(ctx->d_partials, n, ctx->nwaves, ctx->nv,
-                                                     ctx->d_series + s * ctx->nv, ctx->stream);
+                                                     ctx->d_series + s * ctx->nv, ctx->stream, ctx->nonfinite_d);
                 }, "series kernel");
             if (rc) return rc;
             ctx->steps_done = s + n;
@@ -636,7 +741,7 @@ int sbe_run(sbe_ctx *ctx, int64_t step_begin, int64_t step_end)
         if (!rc)
             rc = timed(ctx, 1, [&] {
                 return sbe::launch_series_reduce(ctx->d_partials, n, ctx->nwaves, ctx->nv, ctx->d_series + s * ctx->nv,
-                                                 ctx->stream);
+                                                 ctx->stream, ctx->nonfinite_d);
             }, "series kernel");
         if (rc) return rc;
         ctx->steps_done = s + n;
@@ -1007,11 +1112,12 @@ int sbe_batch_run(sbe_batch *b, int64_t max_steps)
             sbe_ctx *c = b->ctx[i];
             if (insts[i].nsteps <= 0) continue;
             if (b->multi) {
-                jobs.push_back({c->d_partials, c->d_series + c->steps_done * c->nv, c->nwaves, insts[i].nsteps, total});
+                jobs.push_back({c->d_partials, c->d_series + c->steps_done * c->nv, c->nwaves, insts[i].nsteps, total,
+                                c->nonfinite_d});
                 total += insts[i].nsteps;
             } else {
                 e = sbe::launch_series_reduce(c->d_partials, insts[i].nsteps, c->nwaves, c->nv,
-                                              c->d_series + c->steps_done * c->nv, b->stream);
+                                              c->d_series + c->steps_done * c->nv, b->stream, c->nonfinite_d);
             }
             c->steps_done += insts[i].nsteps;
         }
@@ -1097,6 +1203,7 @@ void sbe_destroy(sbe_ctx *ctx)
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     drop_events(ctx);
     for (auto &b : ctx->owned) (void)hipFree(b.ptr);
+    if (ctx->nonfinite_h) (void)hipHostFree(ctx->nonfinite_h);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }

@@ -144,6 +144,10 @@ struct sbe_ctx {
     void *d_state = nullptr;
     double *d_partials = nullptr;   // [chunk][nwaves][nv]
     double *d_series = nullptr;     // [nt][nv]
+    // set to 1 by the series reductions when a k sum is not finite (pinned
+    // host memory mapped for the device: readable without a copy once the
+    // reduction that wrote it has completed)
+    int32_t *nonfinite_h = nullptr, *nonfinite_d = nullptr;
     int64_t chunk_steps = 0;
     int32_t nwaves = 0;
     int nv = 0;
@@ -183,8 +187,9 @@ namespace sbe {
 // Launchers (haldane.hip, series.hip).
 hipError_t launch_haldane_init(const HaldaneArgs &a, int init_type, double ef, hipStream_t s);
 hipError_t launch_haldane_steps(const HaldaneArgs &a, int rk_order, bool t1_on, hipStream_t s);
+// flag (may be null): set to 1 when a sum of the launch is not finite
 hipError_t launch_series_reduce(double *partials, int64_t nsteps, int32_t nrows, int nv,
-                                double *series, hipStream_t s);
+                                double *series, hipStream_t s, int32_t *flag = nullptr);
 // One k block's share of a multi-block reduction: steps [first, first +
 // nsteps) of the launch grid sum partials[nsteps][nrows][nv] into series.
 struct SeriesJob {
@@ -193,6 +198,7 @@ struct SeriesJob {
     int32_t nrows;
     int32_t nsteps;
     int64_t first;
+    int32_t *flag;  // set to 1 on a non-finite sum (may be null)
 };
 // true when one pass suffices (what the multi-block launch runs)
 bool series_multi_ok(int32_t nrows);
@@ -222,6 +228,22 @@ constexpr int kBlock = 256;
 // sbe_api.cpp: argument checks shared by the entry points, the thread-local
 // message of sbe_last_error(NULL), and the fixed-step plan's k points per lane
 int validate_inputs(const sbe_config *cfg, const sbe_model *m, std::string &err);
+// Shared launches (the serving loop): 2-band fixed-step contexts planned as
+// batch members (one k point per lane) advance together in one launch of the
+// batch step kernel and one reduction launch, each by its own step count, in
+// stream order on s. A context gives the same bits in any shared launch and
+// alone (launch boundaries do not change the fixed-step results).
+struct SharedLaunch {
+    HaldaneArgs *d_insts = nullptr;
+    int32_t *d_map = nullptr;
+    SeriesJob *d_jobs = nullptr;
+    int cap_insts = 0, cap_map = 0;
+};
+int create_batch_member(const sbe_config *cfg, const sbe_model *model, sbe_ctx **out);
+bool shared_eligible(const sbe_ctx *c);
+int shared_key(const sbe_ctx *c);  // members of one launch share it (tableau, precision)
+int launch_shared(sbe_ctx *const *cs, const int64_t *nsteps, int n, SharedLaunch &buf, hipStream_t s, std::string &err);
+void free_shared(SharedLaunch &buf);
 int set_thread_error(int code, const std::string &msg);
 bool fixed_step_two_k_per_lane(const sbe_config &cfg, int64_t nk);
 }  // namespace sbe

@@ -24,7 +24,8 @@ constexpr int32_t kGroupRows = 4096;
 // Sum over the rows r0 + r rs, r < nr, of one step's block p, for this
 // block's values [vb, vb + vl), into out[0 .. vl). Thread (rlane, vlane) sums
 // rows rlane, rlane + rl, ... in order; the row lanes are then added in order.
-__device__ __forceinline__ void reduce_rows(const double *p, int64_t r0, int64_t rs, int64_t nr, int nv, double *out)
+__device__ __forceinline__ void reduce_rows(const double *p, int64_t r0, int64_t rs, int64_t nr, int nv, double *out,
+                                            int32_t *flag = nullptr)
 {
     __shared__ double acc[kReduceThreads];
     const int vl = nv < 64 ? nv : 64;          // values per block
@@ -42,17 +43,20 @@ __device__ __forceinline__ void reduce_rows(const double *p, int64_t r0, int64_t
         double t = 0.0;
         for (int r = 0; r < rl; ++r) t += acc[r * vl + tid];
         out[tid] = t;
+        if (flag && !isfinite(t)) *flag = 1;  // a NaN or inf anywhere in the step reaches its sum
     }
 }
 
 // one block per (step, block of 64 values): the rows 0, rs, 2 rs, ... (nr)
 __global__ __launch_bounds__(kReduceThreads) void series_reduce_kernel(const double *__restrict__ partials,
                                                                        int32_t nrows, int32_t nr, int32_t rs,
-                                                                       int nv, double *__restrict__ series)
+                                                                       int nv, double *__restrict__ series,
+                                                                       int32_t *flag)
 {
     const int64_t step = blockIdx.x;
     const int vl = nv < 64 ? nv : 64;
-    reduce_rows(partials + step * static_cast<int64_t>(nrows) * nv, 0, rs, nr, nv, series + step * nv + blockIdx.y * vl);
+    reduce_rows(partials + step * static_cast<int64_t>(nrows) * nv, 0, rs, nr, nv, series + step * nv + blockIdx.y * vl,
+                flag);
 }
 
 // first pass: group g = blockIdx.z of a step sums its rows [g G, (g + 1) G)
@@ -91,7 +95,7 @@ __global__ __launch_bounds__(kReduceThreads) void series_reduce_multi_kernel(con
     if (step >= j.nsteps) return;
     const int vl = nv < 64 ? nv : 64;
     reduce_rows(j.partials + step * static_cast<int64_t>(j.nrows) * nv, 0, 1, j.nrows, nv,
-                j.series + step * nv + blockIdx.y * vl);
+                j.series + step * nv + blockIdx.y * vl, j.flag);
 }
 
 }  // namespace
@@ -109,7 +113,7 @@ hipError_t launch_series_reduce_multi(const SeriesJob *d_jobs, int njobs, int64_
 }
 
 hipError_t launch_series_reduce(double *partials, int64_t nsteps, int32_t nrows, int nv, double *series,
-                                hipStream_t s)
+                                hipStream_t s, int32_t *flag)
 {
     if (nsteps <= 0) return hipSuccess;
     // one block per (step, block of 64 values): the summation order per value
@@ -121,10 +125,10 @@ hipError_t launch_series_reduce(double *partials, int64_t nsteps, int32_t nrows,
         series_group_kernel<<<dim3(static_cast<unsigned>(nsteps), vblocks, static_cast<unsigned>(groups)),
                               kReduceThreads, 0, s>>>(partials, nrows, nv);
         series_reduce_kernel<<<dim3(static_cast<unsigned>(nsteps), vblocks), kReduceThreads, 0, s>>>(
-            partials, nrows, groups, kGroupRows, nv, series);
+            partials, nrows, groups, kGroupRows, nv, series, flag);
     } else {
         series_reduce_kernel<<<dim3(static_cast<unsigned>(nsteps), vblocks), kReduceThreads, 0, s>>>(
-            partials, nrows, nrows, 1, nv, series);
+            partials, nrows, nrows, 1, nv, series, flag);
     }
     return hipGetLastError();
 }

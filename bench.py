@@ -59,9 +59,10 @@ PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
 VEL_FLOPS_PER_KSTEP = {"fp64": 181.0, "fp32": 181.0}
 # Executed FP64 flops per accepted k-point step of the adaptive Cash-Karp FP64
 # kernel (one k point per lane), from SQ_INSTS_VALU_{FMA,MUL,ADD}_F64 (FMA = 2)
-# on a batch of this workload's solves; includes rejected attempts and the
-# per-wave laser, controller and reduction work counted on all 64 lanes
-ADAPT_FLOPS_PER_KSTEP = {"fp64": 1697.0}
+# on the 64 x 200^2 batch (profiles/haldane_batch_64x200x200_traffic.json);
+# includes rejected attempts and the per-wave laser, controller, reduction and
+# sample work counted on all 64 lanes
+ADAPT_FLOPS_PER_KSTEP = {"fp64": 1063.0}
 
 
 def wannier_workload(precision: str = "fp64"):
@@ -338,7 +339,7 @@ def bench_serving(args) -> int:
 # counters SQ_INSTS_VALU_{FMA,MUL,ADD}_F64 (FMA = 2, x 64 lanes) on this workload's
 # generator: per RK4 step for the plain kernel, per accepted step for the adaptive one
 # (rejected attempts and the per-wave laser, controller and reduction work included).
-BATCH_FLOPS = {"plain_fp64_rk4": 385.0, "adaptive_fp64": ADAPT_FLOPS_PER_KSTEP["fp64"]}
+BATCH_FLOPS = {"plain_fp64_rk4": 392.6, "adaptive_fp64": ADAPT_FLOPS_PER_KSTEP["fp64"]}
 BATCH_KERNEL = {"plain_fp64_rk4": "haldane_batch_kernel<double, 4, false, true>",
                 "plain_fp32_rk4": "haldane_batch_kernel<float, 4, false, true>",
                 "adaptive_fp64": "haldane_adaptive_kernel<double, true, 1>",
@@ -473,6 +474,12 @@ def bench_batch(args) -> int:
             t = json.loads(prof.read_text())[best]
             rfl["traffic"] = t["hbm_bytes_per_launch"]
             rfl["traffic_source"] = str(prof.relative_to(ROOT))
+            # VALU issue: wave instructions (PMC) x 4 cycles (a wave64 VALU op on a 16-lane
+            # SIMD) over 1024 SIMDs x 2.4 GHz x the step-kernel time
+            wave_steps = ksteps / 64.0 if best.startswith("plain") else \
+                (f["accepted_block_steps"] + f["rejected_block_steps"]) * steps
+            insts = t["valu_insts_per_wave_step"] * wave_steps
+            rfl["valu_issue_frac"] = insts * 4.0 / (1024 * 2.4e9 * kt)
         except Exception:
             pass
     cpu = None
