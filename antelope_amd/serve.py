@@ -7,7 +7,8 @@ of 100^2..400^2 points with 4..20-cycle pulses and E0 log-uniform in
 [0.001, 0.006]; and a share of synthetic Wannier solves (N = 8, 16 or 32). The
 Wannier solves are reduced in size (k grids of 8^2..16^2, 2..4 cycles, a
 5 x 5 x 1 R range) because at N = 32 and 400^2 points a single request would
-run for hours on one GPU.
+run for hours on one GPU; their band counts cycle through 8, 16, 32, so every
+count occurs once three Wannier requests are drawn.
 
 `run_stream` replays the arrivals in real time: requests are submitted when
 their arrival time passes, the scheduler runs rounds in between, and the
@@ -78,6 +79,13 @@ class SBEServer:
     def release(self, rid: int) -> None:
         self._chk(self._lib.sbe_server_release(self._h, rid))
 
+    def cancel(self, rid: int) -> None:
+        """Drop a queued request, or retire a running one at the next round."""
+        self._chk(self._lib.sbe_server_cancel(self._h, rid))
+
+    def error(self, rid: int) -> str:
+        return self._lib.sbe_server_request_error(self._h, rid).decode()
+
     def close(self) -> None:
         if getattr(self, "_h", None):
             self._lib.sbe_server_destroy(self._h)
@@ -112,11 +120,14 @@ def serving_stream(n: int, rate: float = 20.0, seed: int = 5, wannier_share: flo
     t = 0.0
     reqs = []
     t1 = 0.075
+    nw = 0
     for i in range(n):
         t += -math.log(rng.uniform()) / rate
         E0 = math.exp(math.log(0.001) + rng.uniform() * (math.log(0.006) - math.log(0.001)))
         if rng.uniform() < wannier_share:
-            N = (8, 16, 32)[min(2, int(rng.uniform() * 3))]
+            N = (8, 16, 32)[nw % 3]
+            nw += 1
+            rng.uniform()  # keeps the draws of the stream as they were
             g = 8 + int(rng.uniform() * 9)
             ncyc = wannier_cycles[0] + int(rng.uniform() * (wannier_cycles[1] - wannier_cycles[0] + 1))
             model = synthetic_wannier(N, nmax=(2, 2, 0), seed=1000 + i)
@@ -155,7 +166,7 @@ def run_stream(server: SBEServer, reqs, realtime: bool = True, keep_results: boo
             if i not in ids:
                 continue
             st = server.poll(ids[i])
-            if st.state in (A.SBE_REQ_DONE, A.SBE_REQ_FAILED):
+            if st.state in (A.SBE_REQ_DONE, A.SBE_REQ_FAILED, A.SBE_REQ_CANCELLED):
                 rec = {"kind": reqs[i].kind, "nk": reqs[i].nk, "nt": st.nt, "state": st.state,
                        "latency_s": st.t_done - st.t_submit, "queue_s": (st.t_admit - st.t_submit) if st.t_admit else None,
                        "footprint": st.footprint}

@@ -198,6 +198,11 @@ def test_server_bookkeeping_without_gpu():
     assert srv.counts() == (2, 0, 0, 3 * fp)
     _expect(A.SBE_ERR_STATE, lambda: srv.release(a))        # still queued
     _expect(A.SBE_ERR_RANGE, lambda: srv.poll(12345))
+    srv.cancel(b)                                            # queued: dropped at once, nothing ran
+    assert srv.poll(b).state == A.SBE_REQ_CANCELLED and srv.counts()[0] == 1
+    _expect(A.SBE_ERR_STATE, lambda: srv.cancel(b))
+    _expect(A.SBE_ERR_RANGE, lambda: srv.cancel(12345))
+    srv.release(b)
     small = SBEServer(hbm_budget_bytes=fp - 1)
     _expect(A.SBE_ERR_OOM, lambda: small.submit(cfg, model))  # can never fit
     bad = RunConfig(npoints=[40, 40], pulses=[Pulse(E0=0.0045, w0=0.014, ncycles=2.0)], dt=-1.0)
