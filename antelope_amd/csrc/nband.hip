@@ -1238,7 +1238,62 @@ hipError_t nb_launch_psplit(const NbArgs &a, const double2 *P, uint16_t *p16, hi
     return hipGetLastError();
 }
 
+// isDipoleZero (Wannier90.h:180-194,297-387): element (m, n) of each sum carries
+// e^{ik'.(R + tau_n - tau_m)} with k' = k + A, i.e. the sum over R alone times
+// e^{ik'.(tau_n - tau_m)}; J_a = -i sum (R + tau_n - tau_m)_a H e^{...} gains
+// -i (tau_n - tau_m)_a H. One thread per (k, element) for every matrix of the
+// evaluation: first the H and H~ (no E.D term in this form), then the J with
+// the phased H of this evaluation, or hsrc (the slot before the J) if the
+// evaluation has none.
+__global__ void nb_tau_kernel(const NbArgs a, const NbEval ev, double2 *out, const double2 *hsrc)
+{
+    const int nn = a.N * a.N;
+    const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= a.nk_stride * nn) return;
+    const int64_t k = idx / nn, slot = a.nk_stride * nn;
+    const int ij = static_cast<int>(idx % nn), m = ij / a.N, n = ij % a.N;
+    double dt[3], ph = 0.0;
+    for (int c = 0; c < 3; ++c) {
+        dt[c] = a.tau[n][c] - a.tau[m][c];
+        ph += (a.kpts[3 * k + c] + ev.A[c]) * dt[c];
+    }
+    double sn, cs;
+    sincos(ph, &sn, &cs);
+    const cd e = cmk(cs, sn);
+    int jh = -1;
+    for (int j = 0; j < ev.nmat; ++j)
+        if (ev.mats[j] == kMatHtilde || ev.mats[j] == kMatH0) {
+            out[j * slot + idx] = cmul(e, out[j * slot + idx]);
+            if (ev.mats[j] == kMatH0 || jh < 0) jh = j;
+        }
+    const cd h = jh >= 0 ? out[jh * slot + idx] : hsrc[idx];
+    for (int j = 0; j < ev.nmat; ++j)
+        if (ev.mats[j] >= kMatJx) {
+            const double d = dt[ev.mats[j] - kMatJx];
+            const cd x = cmul(e, out[j * slot + idx]);
+            out[j * slot + idx] = cmk(x.x + d * h.y, x.y - d * h.x);  // x - i d h
+        }
+}
+
+static hipError_t nb_launch_sums(const NbArgs &a, const NbEval &ev, const double2 *P, double2 *out, hipStream_t s);
+
 hipError_t nb_launch_fourier(const NbArgs &a, const NbEval &ev, const double2 *P, double2 *out, hipStream_t s)
+{
+    hipError_t e = nb_launch_sums(a, ev, P, out, s);
+    if (e != hipSuccess || !a.dzero) return e;
+    const int64_t total = a.nk_stride * a.N * a.N;
+    bool has_h = false, has_j = false;
+    for (int j = 0; j < ev.nmat; ++j) {
+        has_h = has_h || ev.mats[j] < kMatJx;
+        has_j = has_j || ev.mats[j] >= kMatJx;
+    }
+    // evaluations with J but no H follow the H slot (the sample's J after a reused H)
+    const double2 *hsrc = has_j && !has_h ? out - a.nk_stride * a.N * a.N : out;
+    nb_tau_kernel<<<dim3(static_cast<unsigned>((total + 255) / 256)), 256, 0, s>>>(a, ev, out, hsrc);
+    return hipGetLastError();
+}
+
+static hipError_t nb_launch_sums(const NbArgs &a, const NbEval &ev, const double2 *P, double2 *out, hipStream_t s)
 {
     const int64_t ncols = static_cast<int64_t>(ev.nmat) * a.N * a.N;
     if (a.split16) {

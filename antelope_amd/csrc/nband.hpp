@@ -54,6 +54,10 @@ struct NbArgs {
     double2 *box_b;           // B on the box [M1 M2 M3][bcols]
     double2 *box_t1;          // first pass  [M1][M2][K3][bcols]
     double2 *box_t2;          // second pass [M1][K2][K3][bcols]
+    // isDipoleZero (Wannier90.h:180-194): the sums take the phase of R + tau_n - tau_m
+    // (tau_n = Re <0n|r|0n>) and no E.D term
+    int dzero;
+    double tau[kNbMaxBands][3];
 };
 
 // One evaluation of the Fourier sums at k + A(t) with field E(t).

@@ -108,6 +108,8 @@ BoxShape box_shape(const sbe_config &cfg, const sbe_model &m, int64_t nk, bool s
     if (am::is_analytic(m.kind) || split16 || m.nrpts < 1 || !m.rvec) return b;
     if (const char *e = std::getenv("ANTELOPE_NB_DENSE"))
         if (std::atoi(e) == 1) return b;
+    sbe_tuning tu{};
+    if (!std::getenv("ANTELOPE_NB_DENSE") && tuning_lookup(cfg, m, nk, tu) && tu.nb_sums == 2) return b;
     if (!axes_separable(cfg, m)) return b;
     int hi[3];
     for (int i = 0; i < 3; ++i) b.lo[i] = hi[i] = m.rvec[i];
@@ -310,11 +312,7 @@ int nb_validate(const sbe_model &m, std::string &err)
         err = "Wannier model needs nrpts >= 1, rvec, rweight and ham";
         return SBE_ERR_MODEL;
     }
-    if (m.is_dipole_zero) {
-        err = "the GPU Wannier path uses the dipole form (isDipoleZero = false, as Wannier90.h:124 sets)";
-        return SBE_ERR_UNSUPPORTED;
-    }
-    if (!m.pos) {
+    if (!m.pos && !m.is_dipole_zero) {
         err = "Wannier model with dipoles needs pos";
         return SBE_ERR_MODEL;
     }
@@ -374,7 +372,11 @@ int nb_setup(sbe_ctx *ctx)
     rc = hipc(ctx, hipMemcpy(dk, hk.data(), bytes_k, hipMemcpyHostToDevice), "upload k");
     if (!rc && nr) rc = hipc(ctx, hipMemcpy(dr, hr.data(), bytes_r, hipMemcpyHostToDevice), "upload R");
     if (!rc && nr) rc = hipc(ctx, hipMemcpy(dham, m.ham, static_cast<size_t>(nr) * nn * sizeof(double2), hipMemcpyHostToDevice), "upload H");
-    if (!rc && nr) rc = hipc(ctx, hipMemcpy(dpos, m.pos, static_cast<size_t>(nr) * 3 * nn * sizeof(double2), hipMemcpyHostToDevice), "upload r");
+    // isDipoleZero: no E.D term (zero positions on the device; the R = 0 diagonal gives tau)
+    if (!rc && nr && m.is_dipole_zero)
+        rc = hipc(ctx, hipMemset(dpos, 0, static_cast<size_t>(nr) * 3 * nn * sizeof(double2)), "clear r");
+    else if (!rc && nr)
+        rc = hipc(ctx, hipMemcpy(dpos, m.pos, static_cast<size_t>(nr) * 3 * nn * sizeof(double2), hipMemcpyHostToDevice), "upload r");
     if (rc) return rc;
 
     NbArgs &a = ctx->nargs;
@@ -393,7 +395,12 @@ int nb_setup(sbe_ctx *ctx)
     a.sample_waves = p.sample_waves;
     // N <= 16: stage and sample kernels on MFMA register tiles (ANTELOPE_NB_LDS=1 keeps the
     // LDS kernels, for comparison)
-    a.tile16 = N <= 16 && !(std::getenv("ANTELOPE_NB_LDS") && std::atoi(std::getenv("ANTELOPE_NB_LDS")) == 1);
+    {
+        sbe_tuning tu{};
+        const bool tuned = tuning_lookup(ctx->cfg, m, nk, tu) && tu.nb_kernels > 0;
+        a.tile16 = N <= 16 && !(tuned && tu.nb_kernels == 2);
+        if (const char *e = std::getenv("ANTELOPE_NB_LDS")) a.tile16 = N <= 16 && std::atoi(e) != 1;
+    }
     a.kpts = dk;
     a.weight = dk + 3 * p.nk_stride;
     a.rcart = dr;
@@ -409,8 +416,20 @@ int nb_setup(sbe_ctx *ctx)
     a.hmax = a.pmax = a.rmax = 0.0;
     for (size_t e = 0; e < static_cast<size_t>(nr) * nn; ++e)
         a.hmax = std::max(a.hmax, std::hypot(m.ham[2 * e], m.ham[2 * e + 1]));
-    for (size_t e = 0; e < static_cast<size_t>(nr) * 3 * nn; ++e)
-        a.pmax = std::max(a.pmax, std::hypot(m.pos[2 * e], m.pos[2 * e + 1]));
+    if (!m.is_dipole_zero)
+        for (size_t e = 0; e < static_cast<size_t>(nr) * 3 * nn; ++e)
+            a.pmax = std::max(a.pmax, std::hypot(m.pos[2 * e], m.pos[2 * e + 1]));
+    // isDipoleZero (Wannier90.h:178-194): tau_n = Re <0n|r|0n> of the R = 0 block (0 without positions)
+    a.dzero = !analytic && m.is_dipole_zero ? 1 : 0;
+    for (int n = 0; n < kNbMaxBands; ++n)
+        for (int c = 0; c < 3; ++c) a.tau[n][c] = 0.0;
+    if (a.dzero && m.pos)
+        for (int r = 0; r < nr; ++r) {
+            const int32_t *nv = m.rvec + 3 * r;
+            if (nv[0] || nv[1] || nv[2]) continue;
+            for (int n = 0; n < N; ++n)
+                for (int c = 0; c < 3; ++c) a.tau[n][c] = m.pos[2 * ((static_cast<size_t>(r) * 3 + c) * nn + n * N + n)];
+        }
     for (int e = 0; e < 3 * nr; ++e) a.rmax = std::max(a.rmax, std::fabs(hr[e]));
     a.model_kind = analytic ? m.kind : 0;
     for (int i = 0; i < 24; ++i) a.mparams[i] = analytic ? m.params[i] : 0.0;
