@@ -111,6 +111,14 @@ class sbe_step_stats(C.Structure):
                 ("rejected", C.c_int64), ("max_accepted", C.c_int64), ("min_accepted", C.c_int64)]
 
 
+class sbe_tuning(C.Structure):
+    _fields_ = [("kpt", C.c_int32), ("chunk_steps", C.c_int32), ("nb_kernels", C.c_int32), ("nb_sums", C.c_int32),
+                ("tuned", C.c_int32), ("candidates", C.c_int32), ("ms_per_step", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 # Exported C-ABI symbols with their ctypes signatures (argtypes, restype).
 SIGNATURES = {
     "sbe_create": ([C.POINTER(sbe_config), C.POINTER(sbe_model), C.c_int64, C.c_int64,
@@ -154,6 +162,12 @@ SIGNATURES = {
     "sbe_run_haldane_noalloc": ([C.POINTER(sbe_config), C.POINTER(sbe_model), C.c_void_p, C.c_void_p, C.c_int64,
                                  C.c_int64, C.c_int64, C.c_int64, C.c_int32, C.c_void_p, C.c_void_p], C.c_int),
     "sbe_last_error": ([C.c_void_p], C.c_char_p),
+    "sbe_autotune": ([C.POINTER(sbe_config), C.POINTER(sbe_model), C.c_int64, C.POINTER(sbe_tuning)], C.c_int),
+    "sbe_tuning_get": ([C.POINTER(sbe_config), C.POINTER(sbe_model), C.POINTER(sbe_tuning)], C.c_int),
+    "sbe_tuning_set": ([C.POINTER(sbe_config), C.POINTER(sbe_model), C.POINTER(sbe_tuning)], C.c_int),
+    "sbe_tuning_clear": ([], None),
+    "sbe_batch_autotune": ([C.c_int32, C.POINTER(sbe_config), C.POINTER(sbe_model), C.c_int64, C.POINTER(sbe_tuning)],
+                           C.c_int),
     "sbe_tmdc_params": ([C.c_char_p, C.c_int32, C.POINTER(sbe_model)], C.c_int),
     "sbe_hhg_points": ([C.c_int64], C.c_int64),
     "sbe_hhg_spectrum": ([C.c_int64, c_double_p, c_double_p, C.c_double, C.c_double, C.c_int32, c_double_p,

@@ -143,7 +143,7 @@ extern "C" int sbe_run_haldane_noalloc(const sbe_config *cfg, const sbe_model *m
         return fail(SBE_ERR_RANGE, "sbe_run_haldane_noalloc: steps outside [0, Nt]");
 
     // the plan of sbe_run for this many k points: k points per lane, waves
-    const int kpt = fixed_step_two_k_per_lane(*cfg, nk) ? 2 : 1;
+    const int kpt = fixed_step_two_k_per_lane(*cfg, *model, nk) ? 2 : 1;
     const int32_t nwaves = static_cast<int32_t>(nk_stride / (64 * kpt));
     const size_t row_tab = static_cast<size_t>(tab.nstage) * 12 * sizeof(double);
     const size_t row_part = static_cast<size_t>(nwaves) * kHaldaneNV * sizeof(double);

@@ -144,13 +144,16 @@ int make_plan(const sbe_config &cfg, const sbe_model &m, int64_t nk, int64_t nt,
     // 39.9 ms for 3000 steps) and loses below that (200^2: 5.1 against 4.0 ms);
     // FP32 does best with one throughout
     p.kpt = cfg.precision == SBE_PREC_FP64 && nk >= 65536 ? 2 : 1;
+    sbe_tuning tu{};
+    const bool tuned = !t_batch_member && sbe::tuning_lookup(cfg, m, nk, tu);
+    if (tuned && tu.kpt > 0 && cfg.rk_order != SBE_RK_CASH_KARP) p.kpt = tu.kpt;
     if (const char *e = std::getenv("ANTELOPE_HALDANE_KPT")) p.kpt = std::atoi(e) == 2 ? 2 : 1;
     if (t_batch_member) p.kpt = 1;
     if (cfg.rk_order == SBE_RK_CASH_KARP) {  // adaptive: 64 kpt consecutive k points per wave (step-control block)
         // one k point per lane: the FP64 kernel then fits two waves per SIMD (1.1x over
         // two chains per lane at one wave on the batched workload) and the 64-point
         // step-control blocks take longer steps than 128-point ones
-        p.kpt = 1;
+        p.kpt = tuned && tu.kpt > 0 ? tu.kpt : 1;
         if (const char *e = std::getenv("ANTELOPE_ADAPTIVE_KPT")) p.kpt = std::atoi(e) == 1 ? 1 : 2;
     }
     const int64_t unit = static_cast<int64_t>(kBlock) * p.kpt;
@@ -163,6 +166,7 @@ int make_plan(const sbe_config &cfg, const sbe_model &m, int64_t nk, int64_t nt,
     int64_t by_bytes = static_cast<int64_t>(t_partial_bytes / row);
     if (by_bytes < kMinChunkSteps) by_bytes = std::min<int64_t>(kMinChunkSteps, static_cast<int64_t>(kPartialBytesMax / row));
     p.chunk = std::max<int64_t>(1, std::min<int64_t>(nt, by_bytes));
+    if (tuned && tu.chunk_steps > 0) p.chunk = std::min<int64_t>(p.chunk, tu.chunk_steps);
     if (cfg.rk_order == SBE_RK_CASH_KARP) p.chunk = std::min<int64_t>(nt, kAdaptiveWindow);
     const size_t real = cfg.precision == SBE_PREC_FP32 ? sizeof(float) : sizeof(double);
     const int comps = cfg.precision == SBE_PREC_FP32 ? 8 : 4;
@@ -177,7 +181,6 @@ int make_plan(const sbe_config &cfg, const sbe_model &m, int64_t nk, int64_t nt,
                                 : 0;
     p.bytes_series = static_cast<size_t>(nt) * p.nv * sizeof(double);
     p.bytes_partials = static_cast<size_t>(p.chunk) * row;
-    (void)m;
     return SBE_OK;
 }
 
@@ -369,10 +372,10 @@ int launch_shared(sbe_ctx *const *cs, const int64_t *nsteps, int n, SharedLaunch
 
 int validate_inputs(const sbe_config *cfg, const sbe_model *m, std::string &err) { return validate(cfg, m, err); }
 int set_thread_error(int code, const std::string &msg) { return set_error(nullptr, code, msg); }
-bool fixed_step_two_k_per_lane(const sbe_config &cfg, int64_t nk)
+bool fixed_step_two_k_per_lane(const sbe_config &cfg, const sbe_model &m, int64_t nk)
 {
     Plan p;
-    make_plan(cfg, sbe_model{}, nk, 1, 4, p);
+    make_plan(cfg, m, nk, 1, 4, p);
     return p.kpt == 2;
 }
 }  // namespace sbe

@@ -245,5 +245,7 @@ int shared_key(const sbe_ctx *c);  // members of one launch share it (tableau, p
 int launch_shared(sbe_ctx *const *cs, const int64_t *nsteps, int n, SharedLaunch &buf, hipStream_t s, std::string &err);
 void free_shared(SharedLaunch &buf);
 int set_thread_error(int code, const std::string &msg);
-bool fixed_step_two_k_per_lane(const sbe_config &cfg, int64_t nk);
+bool fixed_step_two_k_per_lane(const sbe_config &cfg, const sbe_model &m, int64_t nk);
+// tuner.cpp: the cached plan of this problem class on the current device, if any
+bool tuning_lookup(const sbe_config &cfg, const sbe_model &m, int64_t nk, sbe_tuning &out);
 }  // namespace sbe

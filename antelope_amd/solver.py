@@ -78,6 +78,55 @@ def query_footprint(config, model, k_begin: int = -1, k_end: int = -1) -> int:
     return out.value
 
 
+def autotune(config, model, trial_steps: int = 64) -> dict:
+    """Time the plan candidates of this problem on the GPU and cache the fastest
+    for its class (sbe_autotune); returns the chosen plan."""
+    lib = load_library()
+    cfg, k1 = config.to_c()
+    mdl, k2 = model.to_c()
+    t = A.sbe_tuning()
+    _check(lib, None, lib.sbe_autotune(C.byref(cfg), C.byref(mdl), trial_steps, C.byref(t)))
+    return t.as_dict()
+
+
+def batch_autotune(configs, models, trial_steps: int = 2048) -> dict:
+    """sbe_batch_autotune: time the batch plan candidates and cache the fastest."""
+    lib = load_library()
+    n = len(configs)
+    cs, ms, keep = [], [], []
+    for c, m in zip(configs, models):
+        cc, k1 = c.to_c()
+        mm, k2 = m.to_c()
+        cs.append(cc)
+        ms.append(mm)
+        keep += [k1, k2]
+    t = A.sbe_tuning()
+    _check(lib, None, lib.sbe_batch_autotune(n, (A.sbe_config * n)(*cs), (A.sbe_model * n)(*ms), trial_steps,
+                                             C.byref(t)))
+    return t.as_dict()
+
+
+def tuning_get(config, model) -> dict:
+    lib = load_library()
+    cfg, k1 = config.to_c()
+    mdl, k2 = model.to_c()
+    t = A.sbe_tuning()
+    _check(lib, None, lib.sbe_tuning_get(C.byref(cfg), C.byref(mdl), C.byref(t)))
+    return t.as_dict()
+
+
+def tuning_set(config, model, **fields) -> None:
+    lib = load_library()
+    cfg, k1 = config.to_c()
+    mdl, k2 = model.to_c()
+    t = A.sbe_tuning(**fields)
+    _check(lib, None, lib.sbe_tuning_set(C.byref(cfg), C.byref(mdl), C.byref(t)))
+
+
+def tuning_clear() -> None:
+    load_library().sbe_tuning_clear()
+
+
 class SBESolver:
     """One k block of one SBE solve, resident on the GPU."""
 

@@ -38,6 +38,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 from antelope_amd import HaldaneModel, Pulse, RunConfig, SBESolver  # noqa: E402
+from antelope_amd.solver import autotune, batch_autotune  # noqa: E402
 from antelope_amd.spectrum import harmonic_error_db, hhg_spectrum  # noqa: E402
 
 METRIC = "k-point·RK-steps/s at ≤1e-6 rel err vs FP64 oracle; p50/p99 solve latency"
@@ -173,10 +174,11 @@ def bench_wannier(args) -> int:
     forms, res = {}, {}
     for prec in ("fp64", "split16"):
         cfg, model = wannier_workload(prec)
+        tuning = autotune(cfg, model, 8)
         r = timed_solve(cfg, model, steps, min(args.warmup, 5))
         res[prec] = r
         forms[prec] = {"k_steps_per_s": r["nk"] * r["steps"] / r["wall_s"], "wall_s": r["wall_s"],
-                       "ms_per_step": r["wall_s"] * 1e3 / r["steps"]}
+                       "ms_per_step": r["wall_s"] * 1e3 / r["steps"], "tuning": tuning}
         print(f"[bench] wannier16 {prec}: {forms[prec]['ms_per_step']:.2f} ms/step", file=sys.stderr, flush=True)
     a, b = res["fp64"]["series"], res["split16"]["series"]
     n = res["fp64"]["steps"]
@@ -399,6 +401,7 @@ def bench_batch(args) -> int:
             cfgs, _ = haldane_batch(n=n, grid=grid, rk_order=4, precision=prec)
         else:
             cfgs, _ = haldane_batch(n=n, grid=grid, rk_order=45, precision=prec, rtol=args.rtol, atol=atol)
+        tuning = batch_autotune(cfgs, models, 2048)  # plan for these members, cached for the batch below
         r = _batch_passes(cfgs, models, warmup, steps)
         res[name] = r
         if eq is None:  # oracle-equivalent k-point steps of one pass
@@ -406,7 +409,8 @@ def bench_batch(args) -> int:
         ks = eq * steps / r["wall_s"]
         f = {"k_steps_per_s": ks, "wall_s": r["wall_s"], "ms_per_pass": r["wall_s"] * 1e3 / steps,
              "step_kernel_ms_per_pass": r["timing"]["step_ms"] / steps,
-             "step_launches_per_pass": r["timing"]["launches"] / steps, "kernel": BATCH_KERNEL[name]}
+             "step_launches_per_pass": r["timing"]["launches"] / steps, "kernel": BATCH_KERNEL[name],
+             "tuning": tuning}
         if atol is not None:
             acc = sum(st.accepted for st in r["stats"])
             kpb = r["stats"][0].k_per_block
@@ -564,9 +568,11 @@ def main(argv=None) -> int:
         return bench_velocity(args)
     forms = ["fp64", "fp32"] if args.precision == "both" else [args.precision]
     res = {}
+    tunings = {}
     for p in forms:
         cfg, model = workload(args.grid, p)
         steps = args.steps if args.steps > 0 else 10 ** 9
+        tunings[p] = autotune(cfg, model, 256)  # cached for the timed context below
         r = timed_solve(cfg, model, steps, args.warmup)
         print(f"[bench] {p}: {r['steps']} steps x {r['nk']} k in {r['wall_s']:.3f} s "
               f"(step kernels {r['timing']['step_ms']:.1f} ms / {r['timing']['step_launches']} launches, "
@@ -577,7 +583,8 @@ def main(argv=None) -> int:
     for p, r in res.items():
         ks = r["nk"] * r["steps"] / r["wall_s"]
         out_forms[p] = {"k_steps_per_s": ks, "wall_s": r["wall_s"],
-                        "step_kernel_ms": r["timing"]["step_ms"], "reduce_kernel_ms": r["timing"]["reduce_ms"]}
+                        "step_kernel_ms": r["timing"]["step_ms"], "reduce_kernel_ms": r["timing"]["reduce_ms"],
+                        "tuning": tunings[p]}
     if "fp64" in res and "fp32" in res:
         a, b = res["fp64"]["series"], res["fp32"]["series"]
         n = min(res["fp64"]["steps"], res["fp32"]["steps"])

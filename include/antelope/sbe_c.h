@@ -321,6 +321,41 @@ int sbe_get_timing(sbe_ctx *ctx, double *ms /*[2]*/, int64_t *launches /*[2]*/);
 
 void sbe_destroy(sbe_ctx *ctx);
 
+/* ---- launch-plan autotuner (no reference counterpart). A context's plan has
+ * choices that do not change the method: k points per lane of the 2-band
+ * kernels, steps per launch of the fixed-step kernels, the N-band stage
+ * kernels (MFMA register tiles for N <= 16, or LDS) and the N-band Fourier
+ * sums (separable passes on product grids, or the dense GEMM). Each has a
+ * default rule. sbe_autotune times every candidate of the problem's path on
+ * throwaway contexts (trial_steps steps each, after a warm-up) and caches the
+ * fastest for this device and problem class (path, precision, tableau, bands,
+ * gauge, k points to the nearest power of two); sbe_create then plans
+ * contexts of that class with it (environment switches still win).
+ * Steps per launch never change results; k points per lane and the N-band
+ * choices change the order of sums (results agree to rounding). */
+typedef struct sbe_tuning {
+    int32_t kpt;          /* 2-band k points per lane (1, 2; 0 = default rule)      */
+    int32_t chunk_steps;  /* fixed-step steps per launch cap (0 = by memory)         */
+    int32_t nb_kernels;   /* N-band stage kernels: 0 default, 1 tiles, 2 LDS         */
+    int32_t nb_sums;      /* N-band Fourier sums: 0 default, 1 separable, 2 dense    */
+    int32_t tuned;        /* 1: from the cache (set or measured), 0: default rule    */
+    int32_t candidates;   /* plans timed by sbe_autotune                            */
+    double ms_per_step;   /* the chosen plan's trial time per step (sbe_autotune)    */
+} sbe_tuning;
+int sbe_autotune(const sbe_config *cfg, const sbe_model *model, int64_t trial_steps, sbe_tuning *out);
+/* The cached plan of this problem class on the current device (tuned = 0 and
+ * zeros: the default rules apply). */
+int sbe_tuning_get(const sbe_config *cfg, const sbe_model *model, sbe_tuning *out);
+/* Store a plan for this problem class on the current device (as sbe_autotune would). */
+int sbe_tuning_set(const sbe_config *cfg, const sbe_model *model, const sbe_tuning *tuning);
+void sbe_tuning_clear(void);
+/* The same for a batch (sbe_batch_create's arguments): the candidates are
+ * timed on throwaway batches over trial_steps samples (adaptive batches: one
+ * or two k points per lane; fixed-step members always run one), and the
+ * fastest is cached for every member's problem class. */
+int sbe_batch_autotune(int32_t n, const sbe_config *cfgs, const sbe_model *models, int64_t trial_steps,
+                       sbe_tuning *out);
+
 /* ---- propagation with no device allocation. No counterpart in the
  * reference: for callers that cannot allocate device memory (code under HIP
  * graph capture, serving loops with a fixed memory budget). The 2-band

@@ -47,6 +47,7 @@ def test_struct_layout_matches_header(tmp_path):
         "sbe_info": [f for f, _ in A.sbe_info._fields_],
         "sbe_request_status": [f for f, _ in A.sbe_request_status._fields_],
         "sbe_step_stats": [f for f, _ in A.sbe_step_stats._fields_],
+        "sbe_tuning": [f for f, _ in A.sbe_tuning._fields_],
     }
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "antelope/sbe_c.h"',
              '#include "antelope/sbe_server.h"', "int main(void){"]
@@ -209,3 +210,23 @@ def test_server_bookkeeping_without_gpu():
     _expect(A.SBE_ERR_CONFIG, lambda: srv.submit(bad, model))
     small.close()
     srv.close()
+
+
+def test_tuning_cache_without_gpu():
+    """The plan cache: empty means the default rules; a stored plan is returned for its
+    problem class (k points to the nearest power of two) and not for another; values out of
+    range are refused; clear empties it."""
+    from antelope_amd.solver import tuning_clear, tuning_get, tuning_set
+    tuning_clear()
+    m = haldane_cfg1_model()
+    cfg = small_config(n=64)
+    assert tuning_get(cfg, m)["tuned"] == 0
+    tuning_set(cfg, m, kpt=2, chunk_steps=37)
+    t = tuning_get(cfg, m)
+    assert (t["tuned"], t["kpt"], t["chunk_steps"]) == (1, 2, 37)
+    assert tuning_get(small_config(n=63), m)["kpt"] == 2          # 3969 k: the same power of two as 4096
+    assert tuning_get(small_config(n=128), m)["tuned"] == 0       # another class
+    assert tuning_get(small_config(n=64, precision="fp32"), m)["tuned"] == 0
+    _expect(A.SBE_ERR_CONFIG, lambda: tuning_set(cfg, m, kpt=3))
+    tuning_clear()
+    assert tuning_get(cfg, m)["tuned"] == 0
