@@ -119,6 +119,9 @@ class sbe_tuning(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+# sbe_gen_matrix_fn: (user, k[3], out) -> None
+sbe_gen_matrix_fn = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double))
+
 # Exported C-ABI symbols with their ctypes signatures (argtypes, restype).
 SIGNATURES = {
     "sbe_create": ([C.POINTER(sbe_config), C.POINTER(sbe_model), C.c_int64, C.c_int64,
@@ -168,6 +171,10 @@ SIGNATURES = {
     "sbe_tuning_clear": ([], None),
     "sbe_batch_autotune": ([C.c_int32, C.POINTER(sbe_config), C.POINTER(sbe_model), C.c_int64, C.POINTER(sbe_tuning)],
                            C.c_int),
+    "sbe_sample_material": ([C.c_int32, C.c_int32, C.c_int32, c_double_p, c_double_p, C.POINTER(C.c_int32),
+                             sbe_gen_matrix_fn, sbe_gen_matrix_fn, C.c_void_p, C.POINTER(sbe_model), c_double_p],
+                            C.c_int),
+    "sbe_free_sampled_material": ([C.POINTER(sbe_model)], None),
     "sbe_tmdc_params": ([C.c_char_p, C.c_int32, C.POINTER(sbe_model)], C.c_int),
     "sbe_hhg_points": ([C.c_int64], C.c_int64),
     "sbe_hhg_spectrum": ([C.c_int64, c_double_p, c_double_p, C.c_double, C.c_double, C.c_int32, c_double_p,

@@ -384,6 +384,28 @@ int sbe_run_haldane_noalloc(const sbe_config *cfg, const sbe_model *model, const
                             int64_t nk, int64_t nk_stride, int64_t step_begin, int64_t step_end,
                             int32_t init_state, double *series, void *stream);
 
+/* ---- materials given as code (BaseMaterial.h:22-78). The reference's
+ * materials are classes with virtual GenHamiltonian(complex*, k) and
+ * GenDipole(complex**, k); the GPU path takes tight-binding blocks. This
+ * samples a material's callbacks on the (2 nmax_i + 1)-point grid of each
+ * reciprocal axis and inverts the discrete Fourier transform into an
+ * SBE_MODEL_WANNIER model with hoppings |n_i| <= nmax_i along a_i. Models
+ * whose hoppings reach no further are reproduced exactly (to rounding); the
+ * largest deviation of the sampled H(k) (and D(k)) from the callbacks at 16
+ * k points off the grid is returned in *max_dev, so a caller sees a range
+ * too short. Callbacks write row-major N x N complex matrices as
+ * [re, im] pairs: gen_h writes H(k) (Hartree) to out[N][N][2], gen_d the
+ * Wannier-basis dipole D_i(k) (bohr) to out[3][N][N][2]. is_dipole_zero
+ * takes no gen_d (NULL) and uses the offsets tau ([N][3] bohr, NULL = 0):
+ * H_mn(k) = sum_R h_mn(R) e^{ik.(R + tau_n - tau_m)} (Wannier90.h:297-310).
+ * The model's arrays are owned by the library until
+ * sbe_free_sampled_material. Host only: no device is touched. */
+typedef void (*sbe_gen_matrix_fn)(void *user, const double k[3], double *out);
+int sbe_sample_material(int32_t nband, int32_t nval, int32_t is_dipole_zero, const double lattice[9],
+                        const double *tau, const int32_t nmax[3], sbe_gen_matrix_fn gen_h,
+                        sbe_gen_matrix_fn gen_d, void *user, sbe_model *out, double *max_dev);
+void sbe_free_sampled_material(sbe_model *model);
+
 /* Last error of a context (ctx may be NULL: last error of this thread,
  * e.g. a failed sbe_create). Never NULL. */
 const char *sbe_last_error(const sbe_ctx *ctx);

@@ -73,3 +73,18 @@ def test_model_without_positions(oracle):
     cfg.pulses[0].E0, cfg.pulses[0].w0 = 0.01, 0.05
     print("case: N=4 without position blocks", flush=True)
     _check(_gpu(cfg, model), oracle.run(cfg, model, nthreads=16), "no positions")
+
+
+def test_sampled_haldane_class_on_gpu(oracle):
+    """The Haldane class's H(k) (Haldane.h:86-92) sampled by sbe_sample_material
+    with its orbital offsets runs on the GPU as the Haldane path does."""
+    from antelope_amd.material import sample_material
+    h = HaldaneModel(a0=2.683, t1=0.075, t2=0.025, phi0=0.3 * math.pi, M0=2.54 * 0.025)
+    a0 = h.a0
+    L = np.array([[math.sqrt(3) * a0, 0, 0], [math.sqrt(3) / 2 * a0, 1.5 * a0, 0], [0, 0, 10.0]])
+    w, dev = sample_material(lambda k: oracle.hamiltonian(h, k), 2, L, (1, 1, 0), tau=[[0, 0, 0], [0, -a0, 0]])
+    assert dev < 1e-15
+    ax = [2 * math.pi / math.sqrt(3) / a0, 0, 0, 0, 4 * math.pi / 3 / a0, 0, 0, 0, 0]
+    cfg = small_config(n=12, ncycles=1.0, bz_axes=ax, bz_origin=[0, 0, 0])
+    print("case: sampled Haldane class, 12x12 Haldane zone, 1 cycle", flush=True)
+    _check(_gpu(cfg, w), _gpu(cfg, h), "sampled class vs Haldane path")
