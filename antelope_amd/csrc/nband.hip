@@ -317,6 +317,22 @@ static hipError_t box_fourier(const NbArgs &a, const NbEval &ev, cd *out, hipStr
     return hipGetLastError();
 }
 
+// bform and the passes along a3 and a2 of one evaluation, the second pass into t2
+// ([M1][K2][K3][bcols]); the fused step kernel does the pass along a1
+static hipError_t box_t2(const NbArgs &a, const NbEval &ev, cd *t2, hipStream_t s)
+{
+    const int nn = a.N * a.N;
+    const int64_t ncols = static_cast<int64_t>(ev.nmat) * nn;
+    const int M1 = a.boxM[0], M2 = a.boxM[1], M3 = a.boxM[2];
+    const int K2 = a.boxK[1], K3 = a.boxK[2];
+    const int64_t ncells = static_cast<int64_t>(M1) * M2 * M3;
+    nb_box_bform_kernel<<<dim3(static_cast<unsigned>((ncells * ncols + 255) / 256)), 256, 0, s>>>(a, ev, ncells, ncols);
+    box_pass_m(a.box_b, a.box_t1, a.boxE[2], M3, K3, static_cast<int64_t>(M1) * M2, 1, ncols, a.bcols, nullptr,
+               nullptr, a.nk_stride, nn, s);
+    box_pass_m(a.box_t1, t2, a.boxE[1], M2, K2, M1, K3, ncols, a.bcols, nullptr, nullptr, a.nk_stride, nn, s);
+    return hipGetLastError();
+}
+
 // ---------------------------------------- FP16-split Fourier GEMM (SPLIT16) --
 // Each real operand x (|x| <= 1 after a power-of-two scale) is carried as
 // hi = fp16(x), lo = fp16(x - hi), about 22 significant bits; a product is
@@ -1317,6 +1333,12 @@ static hipError_t nb_launch_sums(const NbArgs &a, const NbEval &ev, const double
     const dim3 grid(static_cast<unsigned>((a.nk_stride + GM - 1) / GM), static_cast<unsigned>((ncols + GN - 1) / GN));
     nb_fourier_kernel<<<grid, 256, 0, s>>>(a, ev, P, a.b64, out);
     return hipGetLastError();
+}
+
+hipError_t nb_launch_box_t2(const NbArgs &a, const NbEval &ev, double2 *t2, hipStream_t s)
+{
+    if (!a.box || a.split16 || a.dzero) return hipErrorInvalidValue;
+    return box_t2(a, ev, t2, s);
 }
 
 hipError_t nb_launch_init(const NbArgs &a, const double2 *H0, double2 *rho, double2 *U, hipStream_t s)

@@ -75,6 +75,33 @@ struct NbStage {
     double b[8];  // dt b_j (used when last)
 };
 
+// One whole RK4 step of the LengthWannier gauge for N <= 16 on a product k grid
+// with separable Fourier sums (nb16_step_kernel): the last pass of the sums
+// (along a1) runs inside the step kernel, from the second pass's output of
+// each evaluation time, so the matrices never reach HBM. t2_* are that output,
+// [M1][K2][K3][bcols] with the evaluation's matrices side by side in the columns.
+constexpr int kNbFusedWaves = 8;  // k points along a1 per block (one wave each)
+constexpr int kNbFusedMaxM1 = 8;  // lattice vectors along a1 staged in LDS
+struct NbFused {
+    const double2 *t2_prev;  // H~, H at t_n
+    const double2 *t2_j;     // J_x, J_y, J_z at t_n
+    const double2 *t2_half;  // H~, H at t_n + dt/2
+    const double2 *t2_end;   // H~, H at t_n + dt
+    double *lam;             // [nk_stride][16] band energies at the stored eigenvectors' time
+    int u_valid;             // the stored eigenvectors belong to H(t_n)
+    int lam_valid;           // lam belongs to them
+    double dt;
+    int64_t step_local;      // row block of the partial sums
+    int g1;                  // blocks per (k2, k3) column: ceil(K1 / kNbFusedWaves)
+    int nblocks;             // g1 K2 K3: partial rows per step
+};
+// bytes of the block's LDS (N bands, M1 lattice vectors along a1, nv values per sample row)
+size_t nb16_step_lds_bytes(int N, int M1, int nv);
+hipError_t nb16_launch_step(const NbArgs &a, const NbFused &f, double2 *rho, double2 *U, double *partials,
+                            hipStream_t s);
+// the separable sums of one evaluation up to the second pass: t2 [M1][K2][K3][bcols]
+hipError_t nb_launch_box_t2(const NbArgs &a, const NbEval &ev, double2 *t2, hipStream_t s);
+
 // One time step of the VelocityHamiltonian gauge: A at the stage times and
 // the tableau (coefficients already multiplied by dt).
 struct NbVelStep {

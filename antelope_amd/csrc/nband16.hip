@@ -39,6 +39,8 @@
 //    H = sum_a A_a P_a + diag(E) formed elementwise, the stage derivatives
 //    kept in registers.
 #include <cmath>
+#include <map>
+#include <mutex>
 
 #include "nband.hpp"
 
@@ -211,7 +213,11 @@ __device__ __forceinline__ double orthonormalize(Tile &U, Tile &Ut, cd *buf)
 // Warm-started eigenvectors of the Hermitian part of H (as zheevr, which
 // reads one triangle; H from FP16-split sums is Hermitian only to their
 // error): U (and its transpose Ut) in, refined U, Ut out.
-__device__ void eig_refine(const Tile &Hin, Tile &U, Tile &Ut, int N, cd *buf)
+// small: max|K| below which the last round is U (I + K) (its neglected K^2/2 and the
+// next round's O(K^2) residual are the eigenvectors' error); ortho: end with a
+// Newton-Schulz step (U unitary to rounding rather than to O(small^2))
+__device__ void eig_refine(const Tile &Hin, Tile &U, Tile &Ut, int N, cd *buf, double small_tol = 1e-8,
+                           bool ortho = true)
 {
     const int c = lane_c(), g = lane_g();
     Tile H;
@@ -255,7 +261,7 @@ __device__ void eig_refine(const Tile &Hin, Tile &U, Tile &Ut, int N, cd *buf)
             }
         }
         Tile V;
-        const bool small = kmax < 1e-8;
+        const bool small = kmax < small_tol;
         if (small) {  // I + K: the neglected K^2 / 2 is below 1e-16
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
@@ -287,16 +293,32 @@ __device__ void eig_refine(const Tile &Hin, Tile &U, Tile &Ut, int N, cd *buf)
             W = cmm(V, true, cmm(W, true, V));  // V^dag (W V)
         }
     }
-    orthonormalize(U, Ut, buf);
+    if (ortho) orthonormalize(U, Ut, buf);
 }
 
 // ------------------------------------------------------------- RK stage --
-// The stage derivative at the stage input y, given H~^T's tile and (when
-// dephasing) U, U^T: -i(H~ y - (H~ y)^dag) - g2 y - (g1 - g2) U diag(U^dag y U) U^dag.
-__device__ __forceinline__ Tile stage_derivative(const NbArgs &a, const Tile &Y, const Tile &HtT, const Tile &U,
-                                                 const Tile &Ut, cd *buf)
+// The band-diagonal part of y, U diag(U^dag y U) U^dag, and this lane's column
+// of its diagonal d = (U^dag y U)_cc (two products)
+__device__ __forceinline__ Tile band_diag(const Tile &Y, const Tile &U, const Tile &Ut, double &d)
 {
     const int g = lane_g();
+    const Tile G = cmm(Y, true, U);  // y U
+    d = col_dot(U, G);               // (U^dag y U)_cc
+    Tile Ud;  // (U diag(d))^T's tile: U^T[4kb + g][c] d_{4kb+g}
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const double dr = __shfl(d, 4 * r + g, 64);
+        Ud.re[r] = Ut.re[r] * dr;
+        Ud.im[r] = Ut.im[r] * dr;
+    }
+    return cmm(Ud, false, Ut, true);  // U diag(d) U^dag (B = U^dag: tile conj(U^T))
+}
+
+// The stage derivative at y given H~^T's tile and (dephasing on) y's band-diagonal
+// part M: -i(H~ y - (H~ y)^dag) - g2 y - (g1 - g2) M.
+__device__ __forceinline__ Tile stage_derivative_m(const NbArgs &a, const Tile &Y, const Tile &HtT, const Tile &M,
+                                                   cd *buf)
+{
     const Tile T = cmm(HtT, false, Y);  // H~ y
     const Tile Tt = transpose(T, buf);
     Tile X;  // -i (T - T^dag)
@@ -307,16 +329,6 @@ __device__ __forceinline__ Tile stage_derivative(const NbArgs &a, const Tile &Y,
         X.im[r] = -dr;
     }
     if (a.deph) {
-        const Tile G = cmm(Y, true, U);  // y U
-        const double d = col_dot(U, G);  // (U^dag y U)_cc
-        Tile Ud;  // (U diag(d))^T's tile: U^T[4kb + g][c] d_{4kb+g}
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const double dr = __shfl(d, 4 * r + g, 64);
-            Ud.re[r] = Ut.re[r] * dr;
-            Ud.im[r] = Ut.im[r] * dr;
-        }
-        const Tile M = cmm(Ud, false, Ut, true);  // U diag(d) U^dag (B = U^dag: tile conj(U^T))
         const double g2 = a.gam2, g12 = a.gam1 - a.gam2;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
@@ -325,6 +337,18 @@ __device__ __forceinline__ Tile stage_derivative(const NbArgs &a, const Tile &Y,
         }
     }
     return X;
+}
+
+// The same with U, U^T given (M formed here when dephasing)
+__device__ __forceinline__ Tile stage_derivative(const NbArgs &a, const Tile &Y, const Tile &HtT, const Tile &U,
+                                                 const Tile &Ut, cd *buf)
+{
+    Tile M{};
+    if (a.deph) {
+        double d;
+        M = band_diag(Y, U, Ut, d);
+    }
+    return stage_derivative_m(a, Y, HtT, M, buf);
 }
 
 // Stage i: y = rho + sum_j dt a_ij k_j; k_i = f(y); after the last stage
@@ -591,6 +615,259 @@ __global__ __launch_bounds__(64) void nb16_vel_step_kernel(const NbArgs a, const
     }
 }
 
+
+// ------------------------------------------------- fused RK4 step ----
+// One RK4 step per launch (SBEs.h:596-624 with the stage right-hand side of
+// SBEs.h:762-861 and the sample of SBEs.h:694-760), for a product k grid with
+// separable Fourier sums. Block = kNbFusedWaves k points that share (k2, k3)
+// and differ along a1, one wave each. The sums' last pass (along a1),
+//   X(k1, k2, k3) = sum_m E1[k1][m] T2[m][k2][k3],
+// runs here: the block stages T2's M1 rows of its column for two matrices at a
+// time in LDS and every wave forms its own tiles from them, so H~, H and J of
+// the three times of a step are read once per column instead of written and
+// read back per k point. Per wave and step: the sample at t_n (U, rho and the
+// band energies from the last step), stage 1 at t_n, stages 2 and 3 at
+// t_n + dt/2 (eigenvectors refined there), stage 4 at t_n + dt (eigenvectors
+// refined for the next sample); rho, U and the energies are the only per-k
+// HBM traffic. The eight waves' sample sums are added in LDS in wave order
+// into one partial row per block.
+// dynamic LDS of a block: the waves' transpose buffers, then two staging slots of
+// M1 N^2 complex each (at least kNbFusedWaves nv doubles: the sample sums)
+struct StepLds {
+    cd *tb;     // [kNbFusedWaves][16 kTpad]
+    cd *ys[2];  // [m][i N + j]
+};
+
+__host__ __device__ inline int64_t step_slot_elems(int N, int M1, int nv)
+{
+    const int64_t mats = static_cast<int64_t>(M1) * N * N;
+    const int64_t red = (static_cast<int64_t>(kNbFusedWaves) * nv + 3) / 4;  // the two slots hold W nv doubles
+    return mats > red ? mats : red;
+}
+
+// T2 rows m < M1 of column col for matrix j0 of src0 and j1 of src1 (src1 may
+// be null) into the staging slots; barriers on both sides
+__device__ __forceinline__ void stage_mats(const NbArgs &a, StepLds &L, const cd *src0, int j0, const cd *src1, int j1,
+                                           int64_t col)
+{
+    const int nn = a.N * a.N, M1 = a.boxM[0];
+    const int64_t mstride = static_cast<int64_t>(a.boxK[1]) * a.boxK[2] * a.bcols;  // T2 row stride of m
+    const int per = M1 * nn;
+    const int total = src1 ? 2 * per : per;
+    __syncthreads();  // the last staging's readers are done
+    for (int t = threadIdx.x; t < total; t += blockDim.x) {
+        const int slot = t >= per ? 1 : 0;
+        const int rem = t - slot * per;
+        const int m = rem / nn, ij = rem - m * nn;
+        const cd *src = slot ? src1 : src0;
+        const int j = slot ? j1 : j0;
+        L.ys[slot][rem] = src[m * mstride + col * a.bcols + j * nn + ij];
+    }
+    __syncthreads();
+}
+
+// this lane's elements of X = sum_m E1[m] Y[m] from a staging slot (trans: of X^T),
+// in the order of operations of the a1 pass of the separable sums
+__device__ __forceinline__ Tile form_tile(const cd *ys, const cd *__restrict__ e1, int M1, int N, bool trans)
+{
+    const int c = lane_c(), g = lane_g(), nn = N * N;
+    Tile T;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int i = g + 4 * r;
+        double re = 0.0, im = 0.0;
+        if (i < N && c < N) {
+            const int ij = trans ? c * N + i : i * N + c;
+#pragma unroll
+            for (int m = 0; m < kNbFusedMaxM1; ++m) {
+                if (m < M1) {
+                    const cd w = e1[m], x = ys[m * nn + ij];
+                    re = fma(w.x, x.x, fma(-w.y, x.y, re));
+                    im = fma(w.x, x.y, fma(w.y, x.x, im));
+                }
+            }
+        }
+        T.re[r] = re;
+        T.im[r] = im;
+    }
+    return T;
+}
+
+__device__ __forceinline__ void axpy_tile(Tile &Y, const Tile &X, double s, const Tile &B)
+{  // Y = B + s X
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        Y.re[r] = fma(s, X.re[r], B.re[r]);
+        Y.im[r] = fma(s, X.im[r], B.im[r]);
+    }
+}
+
+// the refinements' last-round threshold: eigenvector error O(1e-12), one rotation
+// round fewer than 1e-8 where the first round turns by ~1e-4
+constexpr double kFusedSmallK = 1e-6;
+
+__global__ __launch_bounds__(64 * kNbFusedWaves) __attribute__((amdgpu_waves_per_eu(2))) void nb16_step_kernel(
+    const NbArgs a, const NbFused f, cd *__restrict__ rho, cd *__restrict__ Ust, double *__restrict__ partials)
+{
+    extern __shared__ cd step_lds[];
+    const int N = a.N, nn = N * N;
+    const int64_t slot = step_slot_elems(N, a.boxM[0], a.nv);
+    StepLds L;
+    L.tb = step_lds;
+    L.ys[0] = step_lds + kNbFusedWaves * 16 * kTpad;
+    L.ys[1] = L.ys[0] + slot;
+    const int c = lane_c(), g = lane_g(), lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+    const int K1 = a.boxK[0], K2 = a.boxK[1], K3 = a.boxK[2], M1 = a.boxM[0];
+    // blocks of one column adjacent in dispatch order on one XCD (blocks go round robin
+    // over the eight XCDs), so the column's T2 rows are shared through that XCD's L2
+    const int nb = static_cast<int>(gridDim.x), b = static_cast<int>(blockIdx.x);
+    const int lin = nb % 8 == 0 ? (b % 8) * (nb / 8) + b / 8 : b;
+    const int col = lin / f.g1, grp = lin - col * f.g1;
+    const int k2 = col / K3, k3 = col - k2 * K3;
+    const int k1 = grp * kNbFusedWaves + w;
+    const bool active = k1 < K1;
+    const int64_t k = active ? a.box_kinv[(static_cast<int64_t>(k1) * K2 + k2) * K3 + k3] : 0;
+    const cd *e1 = a.boxE[0] + static_cast<int64_t>(active ? k1 : 0) * M1;
+    cd *buf = L.tb + w * 16 * kTpad;
+    const double dt = f.dt;
+
+    Tile R, U, Ut, Y, Acc, HtT, X;
+    double lam = 0.0, p = 0.0;
+    double cur[6] = {0, 0, 0, 0, 0, 0};
+    double pop = 0.0;
+    double racc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const double wk = active ? a.weight[k] : 0.0;
+    if (active) {
+        R = load_tile(rho + k * nn, N);
+        U = load_tile(Ust + k * nn, N, true);
+        Ut = transpose(U, buf);
+    }
+    // ---- sample at t_n
+    if (!f.u_valid || !f.lam_valid) {  // eigenvectors / energies at t_n from H(t_n)
+        stage_mats(a, L, f.t2_prev, 1, nullptr, 0, col);
+        if (active) {
+            const Tile H = form_tile(L.ys[0], e1, M1, N, false);
+            if (!f.u_valid) eig_refine(H, U, Ut, N, buf);
+            lam = col_dot(U, cmm(H, true, U));
+        }
+    } else if (active) {
+        lam = f.lam[k * 16 + c];
+    }
+    // rho's band-diagonal part: the populations, the intraband currents Tr(M J) and
+    // stage 1's dephasing
+    Tile M;
+    if (active) {
+        M = band_diag(R, U, Ut, p);  // p: populations of U's columns
+        int rank = 0;                // rank of column c among the N energies, ties by index
+        for (int j = 0; j < N; ++j) {
+            const double lj = __shfl(lam, j, 64);
+            rank += (lj < lam) || (lj == lam && j < c);
+        }
+        for (int j = 0; j < N; ++j) {
+            const int rj = __shfl(rank, j, 64);
+            const double pj = __shfl(p, j, 64);
+            if (rj == lane) pop += pj * wk;
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            racc[2 * r] = R.re[r] * wk;
+            racc[2 * r + 1] = R.im[r] * wk;
+        }
+    }
+    // Re Tr(rho J_a) and Re Tr(M J_a) = sum_c p_c Re (U^dag J_a U)_cc, elementwise from
+    // J_a read transposed (J may not be Hermitian)
+    auto current = [&](int ax, const Tile &Jt) {
+        double tot = 0.0, intra = 0.0;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            tot += R.re[r] * Jt.re[r] - R.im[r] * Jt.im[r];
+            intra += M.re[r] * Jt.re[r] - M.im[r] * Jt.im[r];
+        }
+        tot = wave_sum(tot);
+        intra = a.inter_intra ? wave_sum(intra) : 0.0;
+        cur[ax] += (tot - intra) * wk;
+        cur[3 + ax] += intra * wk;
+    };
+    stage_mats(a, L, f.t2_j, 0, f.t2_j, 1, col);
+    if (active) {
+        current(0, form_tile(L.ys[0], e1, M1, N, true));
+        current(1, form_tile(L.ys[1], e1, M1, N, true));
+    }
+    // ---- stage 1 at t_n (H~ of t_n), with J_z
+    stage_mats(a, L, f.t2_j, 2, f.t2_prev, 0, col);
+    if (active) {
+        current(2, form_tile(L.ys[0], e1, M1, N, true));
+        HtT = form_tile(L.ys[1], e1, M1, N, true);
+        X = stage_derivative_m(a, R, HtT, M, buf);
+        axpy_tile(Acc, X, dt / 6.0, R);
+        axpy_tile(Y, X, 0.5 * dt, R);
+    }
+    // ---- the block's sample sums: one partial row, waves added in order
+    __syncthreads();
+    {
+        double *red = reinterpret_cast<double *>(L.ys[0]);  // [kNbFusedWaves][nv], over both slots
+        double *mine = red + w * a.nv;
+        if (lane < 6) {
+            double v = 0.0;
+#pragma unroll
+            for (int q = 0; q < 6; ++q)
+                if (lane == q) v = cur[q];
+            mine[lane] = v;
+        }
+        if (lane < N) mine[6 + lane] = pop;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int i = g + 4 * r;
+            if (i < N && c < N) {
+                mine[6 + N + 2 * (i * N + c)] = racc[2 * r];
+                mine[6 + N + 2 * (i * N + c) + 1] = racc[2 * r + 1];
+            }
+        }
+        __syncthreads();
+        double *row = partials + (f.step_local * nb + b) * static_cast<int64_t>(a.nv);
+        for (int v = threadIdx.x; v < a.nv; v += blockDim.x) {
+            double s = 0.0;
+            for (int q = 0; q < kNbFusedWaves; ++q) s += red[q * a.nv + v];
+            row[v] = s;
+        }
+    }
+    // ---- stages 2 and 3 at t_n + dt/2
+    stage_mats(a, L, f.t2_half, 0, f.t2_half, 1, col);
+    if (active) {
+        HtT = form_tile(L.ys[0], e1, M1, N, true);
+        // within the step only: the last round at max|K| < 1e-6 and no final Newton-Schulz step
+        if (a.deph) eig_refine(form_tile(L.ys[1], e1, M1, N, false), U, Ut, N, buf, kFusedSmallK, false);
+        X = stage_derivative(a, Y, HtT, U, Ut, buf);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            Acc.re[r] = fma(dt / 3.0, X.re[r], Acc.re[r]);
+            Acc.im[r] = fma(dt / 3.0, X.im[r], Acc.im[r]);
+        }
+        axpy_tile(Y, X, 0.5 * dt, R);
+        X = stage_derivative(a, Y, HtT, U, Ut, buf);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            Acc.re[r] = fma(dt / 3.0, X.re[r], Acc.re[r]);
+            Acc.im[r] = fma(dt / 3.0, X.im[r], Acc.im[r]);
+        }
+        axpy_tile(Y, X, dt, R);
+    }
+    // ---- stage 4 at t_n + dt; eigenvectors and energies for the next sample
+    stage_mats(a, L, f.t2_end, 0, f.t2_end, 1, col);
+    if (active) {
+        HtT = form_tile(L.ys[0], e1, M1, N, true);
+        const Tile H = form_tile(L.ys[1], e1, M1, N, false);
+        eig_refine(H, U, Ut, N, buf, kFusedSmallK, true);
+        lam = col_dot(U, cmm(H, true, U));
+        X = stage_derivative(a, Y, HtT, U, Ut, buf);
+        axpy_tile(R, X, dt / 6.0, Acc);
+        store_tile(rho + k * nn, R, N);
+        store_tile(Ust + k * nn, U, N);
+        if (lane < 16) f.lam[k * 16 + c] = lam;
+    }
+}
+
 }  // namespace
 
 hipError_t nb16_launch_vel_step(const NbArgs &a, const NbVelStep &v, const double2 *P, const double *Eocc,
@@ -626,6 +903,34 @@ hipError_t nb16_launch_sample(const NbArgs &a, const double2 *mats, const double
     if (a.N > 16) return hipErrorInvalidValue;
     nb16_sample_kernel<<<dim3(static_cast<unsigned>(a.sample_waves)), 64, 0, s>>>(a, mats, rho, partials, step_local,
                                                                                  U, ureuse);
+    return hipGetLastError();
+}
+
+size_t nb16_step_lds_bytes(int N, int M1, int nv)
+{
+    return sizeof(double2) * (static_cast<size_t>(kNbFusedWaves) * 16 * kTpad + 2 * step_slot_elems(N, M1, nv));
+}
+
+hipError_t nb16_launch_step(const NbArgs &a, const NbFused &f, double2 *rho, double2 *U, double *partials,
+                            hipStream_t s)
+{
+    if (a.N > 16 || !a.box || a.boxM[0] > kNbFusedMaxM1 || f.nblocks < 1) return hipErrorInvalidValue;
+    const size_t lds = nb16_step_lds_bytes(a.N, a.boxM[0], a.nv);
+    if (lds > (size_t(64) << 10)) {  // beyond the default dynamic LDS: raise the kernel's limit, once per device
+        static std::mutex mu;
+        static std::map<int, size_t> raised;
+        int dev = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e != hipSuccess) return e;
+        std::lock_guard<std::mutex> lk(mu);
+        if (raised[dev] < lds) {
+            e = hipFuncSetAttribute(reinterpret_cast<const void *>(&nb16_step_kernel),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+            if (e != hipSuccess) return e;
+            raised[dev] = lds;
+        }
+    }
+    nb16_step_kernel<<<dim3(static_cast<unsigned>(f.nblocks)), 64 * kNbFusedWaves, lds, s>>>(a, f, rho, U, partials);
     return hipGetLastError();
 }
 

@@ -52,6 +52,9 @@ void nfree(sbe_ctx *ctx, void *ptr)
 }
 
 constexpr size_t kNbPartialBytes = size_t(128) << 20;
+// rows per group of the first reduction pass for the N-band sample rows (hundreds of
+// values per row: small groups spread a step's sum over many blocks)
+constexpr int32_t kNbReduceGroup = 128;
 
 // the matrices {H~, H, J} at k + A: Fourier sums (Wannier) or the analytic classes
 // two sample/stage times that differ only by rounding (t_n + dt against t_n+1)
@@ -139,6 +142,32 @@ BoxShape box_shape(const sbe_config &cfg, const sbe_model &m, int64_t nk, bool s
     return b;
 }
 
+// The fused RK4 step (nb16_step_kernel) replaces the stage and sample kernels when
+// one launch per step can hold everything: FP64 LengthWannier with the classical
+// RK4 tableau, N <= 16 on the register-tile kernels, a Wannier model with
+// dipoles (the dipole-free phases are an epilogue of the full sums) whose
+// separable sums cover the whole product grid with at most kNbFusedMaxM1 lattice
+// vectors along a1. ANTELOPE_NB_FUSED=0 keeps the separate kernels.
+bool fused_eligible(const sbe_config &cfg, const sbe_model &m, int64_t nk, const BoxShape &bx)
+{
+    if (const char *e = std::getenv("ANTELOPE_NB_FUSED"))
+        if (std::atoi(e) == 0) return false;
+    if (!bx.ok || cfg.rk_order != SBE_RK4 || cfg.gauge != SBE_GAUGE_LENGTH_WANNIER || cfg.precision != SBE_PREC_FP64)
+        return false;
+    if (am::is_analytic(m.kind) || m.is_dipole_zero || m.nband > 16 || bx.M[0] > kNbFusedMaxM1) return false;
+    if (const char *e = std::getenv("ANTELOPE_NB_LDS"))
+        if (std::atoi(e) == 1) return false;
+    sbe_tuning tu{};
+    if (tuning_lookup(cfg, m, nk, tu) && tu.nb_kernels == 2) return false;
+    return nk == bx.kcells();  // the whole grid (a k block of it takes the separate kernels)
+}
+
+int fused_blocks(const BoxShape &bx, int &g1)
+{
+    g1 = (bx.K[0] + kNbFusedWaves - 1) / kNbFusedWaves;
+    return g1 * bx.K[1] * bx.K[2];
+}
+
 struct NbPlan {
     int64_t nk_stride;
     int sample_waves, nv;
@@ -147,20 +176,23 @@ struct NbPlan {
     int64_t bcols;
     size_t bytes_P, bytes_rho, bytes_mats, bytes_k, bytes_tab, bytes_partials, bytes_series, bytes_split, bytes_b64;
     size_t bytes_box;
+    size_t bytes_fused;  // fused steps: four second-pass outputs and the band energies
     // offsets inside the box buffer: B, T1, T2, E_1..E_3, R map, k map
     size_t off_t1, off_t2, off_e[3], off_r, off_kinv;
 };
 
 size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 
-NbPlan nb_plan(int N, int nr, int64_t nk, int64_t nt, int nstage, bool split16, const BoxShape &bx)
+NbPlan nb_plan(int N, int nr, int64_t nk, int64_t nt, int nstage, bool split16, const BoxShape &bx, bool fused)
 {
     NbPlan p{};
     p.nk_stride = (nk + 63) / 64 * 64;
     p.sample_waves = static_cast<int>(std::min<int64_t>(nk, 2048));
     p.nv = 6 + N + 2 * N * N;
     const size_t nn = static_cast<size_t>(N) * N;
-    const size_t row = static_cast<size_t>(p.sample_waves) * p.nv * sizeof(double);
+    int g1 = 0;
+    const int64_t rows = fused ? std::max<int64_t>(p.sample_waves, fused_blocks(bx, g1)) : p.sample_waves;
+    const size_t row = static_cast<size_t>(rows) * p.nv * sizeof(double);
     p.chunk = std::max<int64_t>(1, std::min<int64_t>(nt, static_cast<int64_t>(kNbPartialBytes / row)));
     // e^{ik.R} for the dense sums; the separable form does not read it
     p.bytes_P = bx.ok ? 0 : static_cast<size_t>(p.nk_stride) * nr * sizeof(double2);
@@ -196,6 +228,11 @@ NbPlan nb_plan(int N, int nr, int64_t nk, int64_t nt, int nstage, bool split16, 
         p.off_kinv = o;
         o += align256(static_cast<size_t>(bx.kcells()) * sizeof(int32_t));
         p.bytes_box = o;
+    }
+    p.bytes_fused = 0;
+    if (fused) {
+        const size_t t2 = align256(sizeof(double2) * static_cast<size_t>(p.bcols) * bx.M[0] * bx.K[1] * bx.K[2]);
+        p.bytes_fused = 4 * t2 + align256(static_cast<size_t>(p.nk_stride) * 16 * sizeof(double));
     }
     return p;
 }
@@ -260,9 +297,10 @@ size_t nb_footprint(const sbe_config &cfg, const sbe_model &m, int64_t nk, int64
 {
     const bool split16 = cfg.precision == SBE_PREC_SPLIT16;
     const BoxShape bx = box_shape(cfg, m, nk, split16);
-    const NbPlan p = nb_plan(m.nband, am::is_analytic(m.kind) ? 0 : m.nrpts, nk, nt, nstage, split16, bx);
+    const bool fused = fused_eligible(cfg, m, nk, bx);
+    const NbPlan p = nb_plan(m.nband, am::is_analytic(m.kind) ? 0 : m.nrpts, nk, nt, nstage, split16, bx, fused);
     return p.bytes_P + p.bytes_rho + p.bytes_mats + p.bytes_k + p.bytes_tab + p.bytes_partials + p.bytes_series +
-           p.bytes_split + p.bytes_b64 + p.bytes_box;
+           p.bytes_split + p.bytes_b64 + p.bytes_box + p.bytes_fused;
 }
 
 int nb_validate(const sbe_model &m, std::string &err)
@@ -331,7 +369,8 @@ int nb_setup(sbe_ctx *ctx)
     const int64_t nk = ctx->grid.k_end - ctx->grid.k_begin;
     const bool split16 = ctx->cfg.precision == SBE_PREC_SPLIT16;
     const BoxShape bx = box_shape(ctx->cfg, m, nk, split16);
-    const NbPlan p = nb_plan(N, nr, nk, ctx->laser.nt, ctx->tab.nstage, split16, bx);
+    const bool fused = fused_eligible(ctx->cfg, m, nk, bx);
+    const NbPlan p = nb_plan(N, nr, nk, ctx->laser.nt, ctx->tab.nstage, split16, bx, fused);
     const size_t nn = static_cast<size_t>(N) * N;
     ctx->nk = nk;
     ctx->nk_stride = p.nk_stride;
@@ -349,6 +388,7 @@ int nb_setup(sbe_ctx *ctx)
     uint16_t *split = split16 ? nalloc<uint16_t>(ctx, p.bytes_split, rc) : nullptr;
     double2 *b64 = p.bytes_b64 ? nalloc<double2>(ctx, p.bytes_b64, rc) : nullptr;
     char *boxbuf = p.bytes_box ? nalloc<char>(ctx, p.bytes_box, rc) : nullptr;
+    char *fbuf = p.bytes_fused ? nalloc<char>(ctx, p.bytes_fused, rc) : nullptr;
     if (rc) return rc;
 
     // host copies of the tables: k [nk_stride][3], w, R cartesian, weights, H, r
@@ -460,6 +500,19 @@ int nb_setup(sbe_ctx *ctx)
         }
     }
 
+    // fused RK4 steps: the planned grid must have grouped into the whole box (every node one k point)
+    ctx->nbf = false;
+    if (fbuf && a.box && a.tile16 && a.boxK[0] == bx.K[0] && a.boxK[1] == bx.K[1] && a.boxK[2] == bx.K[2] &&
+        nk == bx.kcells() && nb16_step_lds_bytes(N, bx.M[0], a.nv) <= (size_t(160) << 10)) {
+        const size_t t2 = align256(sizeof(double2) * static_cast<size_t>(p.bcols) * bx.M[0] * bx.K[1] * bx.K[2]);
+        for (int i = 0; i < 4; ++i) ctx->nbf_t2[i] = reinterpret_cast<double2 *>(fbuf + i * t2);
+        ctx->nbf_lam = reinterpret_cast<double *>(fbuf + 4 * t2);
+        ctx->nbf_nblocks = fused_blocks(bx, ctx->nbf_g1);
+        ctx->nbf = true;
+    }
+    ctx->nbf_prev_valid = false;
+    ctx->nbf_lam_valid = false;
+
     a.velocity = ctx->velocity ? 1 : 0;
     // e^{ik.R}, then rho(k) from H(k) (A(atmin) = 0)
     if (!analytic && !a.box && boxbuf) {
@@ -536,7 +589,75 @@ int nb_run_velocity(sbe_ctx *ctx, int64_t step_begin, int64_t step_end)
         }
         if (!rc)
             rc = hipc(ctx, launch_series_reduce(ctx->d_partials, n, a.sample_waves, a.nv, ctx->d_series + c0 * a.nv, ctx->stream,
-                                          ctx->nonfinite_d),
+                                          ctx->nonfinite_d, kNbReduceGroup),
+                      "series kernel");
+        if (!rc) ctx->steps_done = c0 + n;
+    }
+    return rc;
+}
+
+}  // namespace
+
+namespace {
+
+// LengthWannier RK4 steps as one nb16_step_kernel launch each (after the
+// Fourier passes of the step's times)
+int nb_run_fused(sbe_ctx *ctx, int64_t step_begin, int64_t step_end)
+{
+    const NbArgs &a = ctx->nargs;
+    const double dt = ctx->laser.dt;
+    int rc = SBE_OK;
+    auto eval = [&](double t, bool hmats, double2 *t2) {
+        NbEval ev{};
+        ctx->laser.eval(t, ev.A, ev.E);
+        if (hmats) {
+            ev.nmat = 2;
+            ev.mats[0] = kMatHtilde;
+            ev.mats[1] = kMatH0;
+        } else {
+            ev.nmat = 3;
+            ev.mats[0] = kMatJx;
+            ev.mats[1] = kMatJy;
+            ev.mats[2] = kMatJz;
+        }
+        return hipc(ctx, nb_launch_box_t2(a, ev, t2, ctx->stream), "fourier passes");
+    };
+    for (int64_t c0 = step_begin; c0 < step_end && !rc; c0 += ctx->chunk_steps) {
+        const int64_t n = std::min<int64_t>(ctx->chunk_steps, step_end - c0);
+        for (int64_t s = 0; s < n && !rc; ++s) {
+            const double tn = ctx->laser.atmin + dt * static_cast<double>(c0 + s);  // antelope.cpp:299
+            // the Fourier sums of the step's times do not depend on rho: all before the step
+            if (!(ctx->nbf_prev_valid && same_time(ctx->nbf_prev_time, tn, dt))) rc = eval(tn, true, ctx->nbf_t2[0]);
+            if (!rc) rc = eval(tn, false, ctx->nbf_t2[1]);
+            if (!rc) rc = eval(tn + 0.5 * dt, true, ctx->nbf_t2[2]);
+            if (!rc) rc = eval(tn + dt, true, ctx->nbf_t2[3]);
+            if (rc) break;
+            NbFused f{};
+            f.t2_prev = ctx->nbf_t2[0];
+            f.t2_j = ctx->nbf_t2[1];
+            f.t2_half = ctx->nbf_t2[2];
+            f.t2_end = ctx->nbf_t2[3];
+            f.lam = ctx->nbf_lam;
+            f.u_valid = same_time(ctx->nb_u_time, tn, dt) ? 1 : 0;
+            f.lam_valid = f.u_valid && ctx->nbf_lam_valid && same_time(ctx->nbf_lam_time, tn, dt) ? 1 : 0;
+            f.dt = dt;
+            f.step_local = s;
+            f.g1 = ctx->nbf_g1;
+            f.nblocks = ctx->nbf_nblocks;
+            rc = hipc(ctx, nb16_launch_step(a, f, ctx->nb_rho, ctx->nb_U, ctx->d_partials, ctx->stream), "step kernel");
+            // the end of this step is the start of the next: its H~, H serve there
+            std::swap(ctx->nbf_t2[0], ctx->nbf_t2[3]);
+            ctx->nbf_prev_time = tn + dt;
+            ctx->nbf_prev_valid = true;
+            ctx->nb_u_time = tn + dt;
+            ctx->nbf_lam_time = tn + dt;
+            ctx->nbf_lam_valid = true;
+            ctx->nb_ht_valid = false;  // the separate kernels' matrices are not kept up to date
+            ctx->nb_h_valid = false;
+        }
+        if (!rc)
+            rc = hipc(ctx, launch_series_reduce(ctx->d_partials, n, ctx->nbf_nblocks, a.nv, ctx->d_series + c0 * a.nv,
+                                                ctx->stream, ctx->nonfinite_d, kNbReduceGroup),
                       "series kernel");
         if (!rc) ctx->steps_done = c0 + n;
     }
@@ -548,6 +669,7 @@ int nb_run_velocity(sbe_ctx *ctx, int64_t step_begin, int64_t step_end)
 int nb_run(sbe_ctx *ctx, int64_t step_begin, int64_t step_end)
 {
     if (ctx->velocity) return nb_run_velocity(ctx, step_begin, step_end);
+    if (ctx->nbf) return nb_run_fused(ctx, step_begin, step_end);
     const NbArgs &a = ctx->nargs;
     const Tableau &tb = ctx->tab;
     const double dt = ctx->laser.dt;
@@ -635,7 +757,7 @@ int nb_run(sbe_ctx *ctx, int64_t step_begin, int64_t step_end)
         }
         if (!rc)
             rc = hipc(ctx, launch_series_reduce(ctx->d_partials, n, a.sample_waves, a.nv, ctx->d_series + c0 * a.nv, ctx->stream,
-                                          ctx->nonfinite_d),
+                                          ctx->nonfinite_d, kNbReduceGroup),
                       "series kernel");
         if (!rc) ctx->steps_done = c0 + n;
     }

@@ -176,6 +176,15 @@ struct sbe_ctx {
     double nb_h_time = 0;        // time of the H in mats slot 1
     bool nb_h_valid = false;
     bool nb_vel_p_lds = false;   // velocity gauge: pMatrix of the k point staged in LDS
+    // fused RK4 steps (nb16_step_kernel): second-pass outputs of the separable sums
+    bool nbf = false;
+    double2 *nbf_t2[4] = {nullptr, nullptr, nullptr, nullptr};  // H~ H at t_n, J at t_n, H~ H at t + dt/2, at t + dt
+    double *nbf_lam = nullptr;   // [nk_stride][16] band energies at the stored eigenvectors' time
+    int nbf_g1 = 0, nbf_nblocks = 0;
+    bool nbf_prev_valid = false;  // nbf_t2[0] holds H~, H at nbf_prev_time
+    double nbf_prev_time = 0;
+    bool nbf_lam_valid = false;
+    double nbf_lam_time = 0;
     std::string err;
 
     // optional kernel timing: (start, stop) event pairs per kernel class
@@ -187,9 +196,10 @@ namespace sbe {
 // Launchers (haldane.hip, series.hip).
 hipError_t launch_haldane_init(const HaldaneArgs &a, int init_type, double ef, hipStream_t s);
 hipError_t launch_haldane_steps(const HaldaneArgs &a, int rk_order, bool t1_on, hipStream_t s);
-// flag (may be null): set to 1 when a sum of the launch is not finite
+// flag (may be null): set to 1 when a sum of the launch is not finite; group_rows
+// (0: the default) is the first pass's group size for steps of many rows
 hipError_t launch_series_reduce(double *partials, int64_t nsteps, int32_t nrows, int nv,
-                                double *series, hipStream_t s, int32_t *flag = nullptr);
+                                double *series, hipStream_t s, int32_t *flag = nullptr, int32_t group_rows = 0);
 // One k block's share of a multi-block reduction: steps [first, first +
 // nsteps) of the launch grid sum partials[nsteps][nrows][nv] into series.
 struct SeriesJob {

@@ -10,9 +10,10 @@
 //
 // Steps with many rows (large grids: 2^18 waves at 4096^2 k points) are
 // summed in two passes, so that one step is not left to a single block: the
-// rows split into groups of kGroupRows, each group's sum is written over the
-// group's first row, and the second pass adds the group sums in order. The
-// order still depends only on the row count.
+// rows split into groups (kGroupRows, or a caller's smaller size for wide rows),
+// each group's sum is written over the group's first row, and the second pass
+// adds the group sums in order. The order depends only on the row count and
+// the group size.
 #include "sbe_internal.hpp"
 
 namespace sbe {
@@ -63,11 +64,11 @@ __global__ __launch_bounds__(kReduceThreads) void series_reduce_kernel(const dou
 // and writes the sum over row g G. Only this block reads or writes those rows;
 // its reads are done (barrier in reduce_rows) before the write.
 __global__ __launch_bounds__(kReduceThreads) void series_group_kernel(double *__restrict__ partials, int32_t nrows,
-                                                                      int nv)
+                                                                      int nv, int32_t grows)
 {
     const int64_t step = blockIdx.x;
-    const int64_t g0 = static_cast<int64_t>(blockIdx.z) * kGroupRows;
-    const int64_t nr = nrows - g0 < kGroupRows ? nrows - g0 : kGroupRows;
+    const int64_t g0 = static_cast<int64_t>(blockIdx.z) * grows;
+    const int64_t nr = nrows - g0 < grows ? nrows - g0 : grows;
     double *p = partials + step * static_cast<int64_t>(nrows) * nv;
     __shared__ double sums[64];
     reduce_rows(p, g0, 1, nr, nv, sums);
@@ -113,19 +114,20 @@ hipError_t launch_series_reduce_multi(const SeriesJob *d_jobs, int njobs, int64_
 }
 
 hipError_t launch_series_reduce(double *partials, int64_t nsteps, int32_t nrows, int nv, double *series,
-                                hipStream_t s, int32_t *flag)
+                                hipStream_t s, int32_t *flag, int32_t group_rows)
 {
+    const int32_t G = group_rows > 0 ? group_rows : kGroupRows;
     if (nsteps <= 0) return hipSuccess;
     // one block per (step, block of 64 values): the summation order per value
     // (row lanes, then the lanes in order) does not depend on the grid
     const int vl = nv < 64 ? nv : 64;
     const unsigned vblocks = static_cast<unsigned>((nv + vl - 1) / vl);
-    if (nrows > 2 * kGroupRows) {
-        const int32_t groups = (nrows + kGroupRows - 1) / kGroupRows;
+    if (nrows > 2 * G) {
+        const int32_t groups = (nrows + G - 1) / G;
         series_group_kernel<<<dim3(static_cast<unsigned>(nsteps), vblocks, static_cast<unsigned>(groups)),
-                              kReduceThreads, 0, s>>>(partials, nrows, nv);
+                              kReduceThreads, 0, s>>>(partials, nrows, nv, G);
         series_reduce_kernel<<<dim3(static_cast<unsigned>(nsteps), vblocks), kReduceThreads, 0, s>>>(
-            partials, nrows, groups, kGroupRows, nv, series, flag);
+            partials, nrows, groups, G, nv, series, flag);
     } else {
         series_reduce_kernel<<<dim3(static_cast<unsigned>(nsteps), vblocks), kReduceThreads, 0, s>>>(
             partials, nrows, nrows, 1, nv, series, flag);

@@ -108,17 +108,19 @@ def test_laser_eval_and_footprint_without_gpu():
     assert 90_000 * 32 < b < 512 * 2 ** 20
 
 
-def _footprint_env(cfg, m, dense):
+def _footprint_env(cfg, m, dense, fused=False):
     import os
-    old = os.environ.get("ANTELOPE_NB_DENSE")
-    os.environ["ANTELOPE_NB_DENSE"] = "1" if dense else "0"
+    env = {"ANTELOPE_NB_DENSE": "1" if dense else "0", "ANTELOPE_NB_FUSED": "1" if fused else "0"}
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
     try:
         return query_footprint(cfg, m)
     finally:
-        if old is None:
-            del os.environ["ANTELOPE_NB_DENSE"]
-        else:
-            os.environ["ANTELOPE_NB_DENSE"] = old
+        for k, v in old.items():
+            if v is None:
+                del os.environ[k]
+            else:
+                os.environ[k] = v
 
 
 def test_separable_sums_footprint_without_gpu():
@@ -128,6 +130,8 @@ def test_separable_sums_footprint_without_gpu():
     [K_i][M_i], the R map and the k map, each 256-byte aligned, and drops the dense sums'
     e^{ik.R} table [nk_stride][N_R] and their B buffer [N_R padded to 32][columns]. Where
     the passes would not cost under half the dense sum (2 x 2 x 1 grid) nothing is added."""
+    import dataclasses
+
     from antelope_amd import Pulse, RunConfig
     from antelope_amd.models import synthetic_wannier
 
@@ -146,6 +150,14 @@ def test_separable_sums_footprint_without_gpu():
     nk_stride, nr = (K[0] * K[1] * K[2] + 63) // 64 * 64, cells  # the dense e^{ik.R} table is not kept
     want -= nk_stride * nr * 16 + (nr + 31) // 32 * 32 * col
     assert _footprint_env(cfg, m, dense=False) - _footprint_env(cfg, m, dense=True) == want
+    # the fused RK4 step adds four second-pass outputs [M1][K2][K3][columns] and the band
+    # energies [nk_stride][16] (FP64); its 10 partial rows per step fit in the 60 the sample
+    # kernel has
+    fused = 4 * al(M[0] * K[1] * K[2] * col) + al(nk_stride * 16 * 8)
+    assert _footprint_env(cfg, m, dense=False, fused=True) - _footprint_env(cfg, m, dense=False) == fused
+    assert _footprint_env(cfg, m, dense=True, fused=True) == _footprint_env(cfg, m, dense=True)
+    rk5 = dataclasses.replace(cfg, rk_order=5)  # the fused step is the classical RK4 tableau only
+    assert _footprint_env(rk5, m, dense=False, fused=True) == _footprint_env(rk5, m, dense=False)
     small = cfg_for((2, 2, 1))
     m1 = synthetic_wannier(N, nmax=(1, 1, 1), seed=21)
     assert _footprint_env(small, m1, dense=False) == _footprint_env(small, m1, dense=True)

@@ -1,0 +1,98 @@
+"""The fused RK4 step of the N-band LengthWannier path (nb16_step_kernel: the
+sample, the four stages and the last pass of the separable Fourier sums in one
+launch per step) against the FP64 oracle and against the separate stage and
+sample kernels (ANTELOPE_NB_FUSED=0). Bar against the oracle: 1e-10 relative
+L2 (j(t), band populations, k-summed rho, final state); against the separate
+kernels: 1e-12 (the same arithmetic, summed in another order).
+"""
+import numpy as np
+import pytest
+
+from antelope_amd import Pulse, RunConfig, SBESolver, query_footprint
+from antelope_amd.models import synthetic_wannier
+from tests.cases import rel_l2
+
+pytestmark = pytest.mark.gpu
+TOL64 = 1e-10
+
+
+def _cfg(npts, ncycles=0.5, T2=80.0, inter_intra=True, dt=0.5, E0=0.01, rk=4):
+    return RunConfig(npoints=list(npts), pulses=[Pulse(E0=E0, w0=0.05, ncycles=ncycles, envelope="sin2")],
+                     dt=dt, T2=T2, rk_order=rk, inter_intra=inter_intra)
+
+
+def _errs(got, ref):
+    e = {"j": rel_l2(got["j_inter"] + got["j_intra"], ref["j_inter"] + ref["j_intra"]),
+         "j_inter": rel_l2(got["j_inter"], ref["j_inter"]),
+         "band_pop": rel_l2(got["band_pop"], ref["band_pop"]),
+         "rho_bar": rel_l2(got["rho_bar"], ref["rho_bar"])}
+    if np.linalg.norm(ref["j_intra"]) > 0:
+        e["j_intra"] = rel_l2(got["j_intra"], ref["j_intra"])
+    return e
+
+
+def _solve(cfg, model, monkeypatch, fused, splits=None, peek=None):
+    monkeypatch.setenv("ANTELOPE_NB_FUSED", "1" if fused else "0")
+    with SBESolver(cfg, model) as s:
+        assert s.info.device_bytes == query_footprint(cfg, model)
+        if splits is None:
+            s.run()
+        else:
+            b = 0
+            for e in list(splits) + [s.nt]:
+                s.run(b, e)
+                if peek is not None and b == 0:
+                    peek(s)
+                b = e
+        s.synchronize()
+        return s.series(), s.state(), s.info.device_bytes
+
+
+@pytest.mark.parametrize("label,N,npts,kw", [
+    ("n8_box5x5x3", 8, (6, 5, 2), dict()),
+    ("n16_ragged_a1_no_dephasing_total_current", 16, (12, 3, 2), dict(T2=-1.0, inter_intra=False)),
+    ("n3_one_wave_per_block_used", 3, (5, 4, 3), dict()),
+])
+def test_fused_step_against_oracle_and_separate_kernels(oracle, monkeypatch, label, N, npts, kw):
+    model = synthetic_wannier(N, nmax=(2, 2, 1) if N <= 8 else (1, 1, 1), seed=40 + N)
+    cfg = _cfg(npts, **kw)
+    print(f"case: {label}", flush=True)
+    ref = oracle.run(cfg, model, nthreads=8, want_state=True)
+    fz, st_f, bytes_f = _solve(cfg, model, monkeypatch, fused=True)
+    sp, st_s, bytes_s = _solve(cfg, model, monkeypatch, fused=False)
+    assert bytes_f > bytes_s  # the fused plan's buffers were allocated: it ran fused
+    e_or = _errs(fz, ref)
+    e_sp = _errs(fz, sp)
+    print(label, "vs oracle", {k: f"{v:.2e}" for k, v in e_or.items()},
+          "vs separate kernels", {k: f"{v:.2e}" for k, v in e_sp.items()}, flush=True)
+    assert all(v < TOL64 for v in e_or.values()), e_or
+    assert all(v < 1e-12 for v in e_sp.values()), e_sp
+    assert rel_l2(st_f, ref["state"]) < TOL64
+    assert rel_l2(st_f, st_s) < 1e-12
+
+
+def test_fused_split_runs_and_state_reads_are_bitwise(monkeypatch):
+    """Runs split anywhere, with a band-basis state read in between, give the bits of one run."""
+    model = synthetic_wannier(8, nmax=(2, 2, 1), seed=9)
+    cfg = _cfg((6, 5, 2), ncycles=0.5)
+    print("case: split runs", flush=True)
+    one, st1, _ = _solve(cfg, model, monkeypatch, fused=True)
+    seen = []
+    two, st2, _ = _solve(cfg, model, monkeypatch, fused=True, splits=(37, 38, 90),
+                         peek=lambda s: seen.append(s.state("band")))
+    assert seen and np.all(np.isfinite(seen[0]))
+    for k in one:
+        assert np.array_equal(one[k], two[k]), k
+    assert np.array_equal(st1, st2)
+
+
+def test_fused_rk5_keeps_separate_kernels(oracle, monkeypatch):
+    """The fused step is the classical RK4 tableau; RK5 plans the separate kernels."""
+    model = synthetic_wannier(6, nmax=(2, 2, 1), seed=12)
+    cfg = _cfg((6, 5, 2), rk=5, ncycles=0.25)
+    print("case: rk5", flush=True)
+    a, _, bytes_a = _solve(cfg, model, monkeypatch, fused=True)
+    b, _, bytes_b = _solve(cfg, model, monkeypatch, fused=False)
+    assert bytes_a == bytes_b
+    ref = oracle.run(cfg, model, nthreads=8)
+    assert all(v < TOL64 for v in _errs(a, ref).values())
