@@ -228,10 +228,10 @@ __global__ void nb_box_bform_kernel(const NbArgs a, const NbEval ev, int64_t nce
 // thread stay in registers (MM >= M); E is read with uniform loads. With out set
 // (the last pass, o = 0) the result goes to out[mat][kinv[k][i]][ij] instead of Y.
 template <int MM>
-__global__ __launch_bounds__(256) void nb_box_pass_kernel(const cd *__restrict__ X, cd *__restrict__ Y,
-                                                          const cd *__restrict__ E, int M, int K, int kc, int64_t inner,
-                                                          int64_t ncols, int64_t pitch, const int32_t *__restrict__ kinv,
-                                                          cd *__restrict__ out, int64_t nk_stride, int nn)
+__device__ __forceinline__ void box_pass_body(const cd *__restrict__ X, cd *__restrict__ Y, const cd *__restrict__ E,
+                                              int M, int K, int kc, int64_t inner, int64_t ncols, int64_t pitch,
+                                              const int32_t *__restrict__ kinv, cd *__restrict__ out,
+                                              int64_t nk_stride, int nn, int kz)
 {
     const int64_t oi = blockIdx.x;
     const int64_t o = oi / inner, i = oi - o * inner;
@@ -241,7 +241,7 @@ __global__ __launch_bounds__(256) void nb_box_pass_kernel(const cd *__restrict__
 #pragma unroll
     for (int m = 0; m < MM; ++m) x[m] = m < M ? X[((o * M + m) * inner + i) * pitch + c] : cmk(0.0, 0.0);
     const int mat = static_cast<int>(c / nn), ij = static_cast<int>(c - static_cast<int64_t>(mat) * nn);
-    const int k0 = static_cast<int>(blockIdx.z) * kc, k1 = min(K, k0 + kc);
+    const int k0 = kz * kc, k1 = min(K, k0 + kc);
     for (int k = k0; k < k1; ++k) {
         const cd *e = E + static_cast<int64_t>(k) * M;
         double re = 0.0, im = 0.0;
@@ -260,6 +260,63 @@ __global__ __launch_bounds__(256) void nb_box_pass_kernel(const cd *__restrict__
             Y[((o * K + k) * inner + i) * pitch + c] = cmk(re, im);
         }
     }
+}
+
+template <int MM>
+__global__ __launch_bounds__(256) void nb_box_pass_kernel(const cd *__restrict__ X, cd *__restrict__ Y,
+                                                          const cd *__restrict__ E, int M, int K, int kc, int64_t inner,
+                                                          int64_t ncols, int64_t pitch, const int32_t *__restrict__ kinv,
+                                                          cd *__restrict__ out, int64_t nk_stride, int nn)
+{
+    box_pass_body<MM>(X, Y, E, M, K, kc, inner, ncols, pitch, kinv, out, nk_stride, nn, static_cast<int>(blockIdx.z));
+}
+
+// Up to four evaluations of the separable sums at once (the fused RK4 step's
+// evaluations of one step): B on the box, then the passes along a3 and a2,
+// block z = (evaluation, k chunk); each evaluation has its own scratch and output.
+struct BoxMulti {
+    NbEval ev[4];
+    int nev;
+    cd *b[4];   // B on the box [cells][bcols]
+    cd *t1[4];  // first pass [M1][M2][K3][bcols]
+    cd *t2[4];  // second pass [M1][K2][K3][bcols]
+};
+
+__global__ void nb_box_bform_multi_kernel(const NbArgs a, const BoxMulti bm, int64_t ncells)
+{
+    const int e = blockIdx.z;
+    const NbEval &ev = bm.ev[e];
+    const int64_t ncols = static_cast<int64_t>(ev.nmat) * a.N * a.N;
+    const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= ncells * ncols) return;
+    const int64_t cell = idx / ncols, c = idx % ncols;
+    const int r = a.box_r[cell];
+    cd v = cmk(0.0, 0.0);
+    if (r >= 0) {
+        const double *R = a.rcart + 3 * r;
+        double sn, cs;
+        sincos(ev.A[0] * R[0] + ev.A[1] * R[1] + ev.A[2] * R[2], &sn, &cs);
+        v = b_element(a, ev, r, c, cmk(cs, sn));
+    }
+    bm.b[e][cell * a.bcols + c] = v;
+}
+
+// pass 3 (second == false: B -> T1 along a3) or pass 2 (T1 -> T2 along a2)
+template <int MM>
+__global__ __launch_bounds__(256) void nb_box_pass_multi_kernel(const NbArgs a, const BoxMulti bm, bool second, int kc,
+                                                                int nz)
+{
+    const int e = static_cast<int>(blockIdx.z) / nz, kz = static_cast<int>(blockIdx.z) - e * nz;
+    const int nn = a.N * a.N;
+    const int64_t ncols = static_cast<int64_t>(bm.ev[e].nmat) * nn;
+    const int M1 = a.boxM[0], M2 = a.boxM[1], M3 = a.boxM[2], K2 = a.boxK[1], K3 = a.boxK[2];
+    if (!second)
+        box_pass_body<MM>(bm.b[e], bm.t1[e], a.boxE[2], M3, K3, kc, 1, ncols, a.bcols, nullptr, nullptr, a.nk_stride,
+                          nn, kz);
+    else
+        box_pass_body<MM>(bm.t1[e], bm.t2[e], a.boxE[1], M2, K2, kc, K3, ncols, a.bcols, nullptr, nullptr, a.nk_stride,
+                          nn, kz);
+    (void)M1;
 }
 
 // the padding k points [nk, nk_stride) of every matrix: zero, as the dense sums give them
@@ -1339,6 +1396,52 @@ hipError_t nb_launch_box_t2(const NbArgs &a, const NbEval &ev, double2 *t2, hipS
 {
     if (!a.box || a.split16 || a.dzero) return hipErrorInvalidValue;
     return box_t2(a, ev, t2, s);
+}
+
+template <int MM>
+static void box_pass_multi(const NbArgs &a, const BoxMulti &bm, bool second, int64_t maxcols, hipStream_t s)
+{
+    const int M1 = a.boxM[0], M2 = a.boxM[1], K2 = a.boxK[1], K3 = a.boxK[2];
+    const int K = second ? K2 : K3;
+    const int64_t outer = second ? M1 : static_cast<int64_t>(M1) * M2, inner = second ? K3 : 1;
+    const int64_t xy = outer * inner * ((maxcols + 255) / 256) * bm.nev;
+    const int kc = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(K, K * xy / 2048)));
+    const int nz = (K + kc - 1) / kc;
+    const dim3 grid(static_cast<unsigned>(outer * inner), static_cast<unsigned>((maxcols + 255) / 256),
+                    static_cast<unsigned>(nz * bm.nev));
+    nb_box_pass_multi_kernel<MM><<<grid, 256, 0, s>>>(a, bm, second, kc, nz);
+}
+
+template <bool SECOND>
+static void box_pass_multi_m(const NbArgs &a, const BoxMulti &bm, int64_t maxcols, hipStream_t s)
+{
+    const int M = SECOND ? a.boxM[1] : a.boxM[2];
+    if (M <= 4) box_pass_multi<4>(a, bm, SECOND, maxcols, s);
+    else if (M <= 8) box_pass_multi<8>(a, bm, SECOND, maxcols, s);
+    else if (M <= 16) box_pass_multi<16>(a, bm, SECOND, maxcols, s);
+    else box_pass_multi<32>(a, bm, SECOND, maxcols, s);
+}
+
+hipError_t nb_launch_box_t2_multi(const NbArgs &a, int nev, const NbEval *evs, double2 *const *b, double2 *const *t1,
+                                  double2 *const *t2, hipStream_t s)
+{
+    if (!a.box || a.split16 || a.dzero || nev < 1 || nev > 4) return hipErrorInvalidValue;
+    BoxMulti bm{};
+    bm.nev = nev;
+    int64_t maxcols = 0;
+    for (int e = 0; e < nev; ++e) {
+        bm.ev[e] = evs[e];
+        bm.b[e] = b[e];
+        bm.t1[e] = t1[e];
+        bm.t2[e] = t2[e];
+        maxcols = std::max<int64_t>(maxcols, static_cast<int64_t>(evs[e].nmat) * a.N * a.N);
+    }
+    const int64_t ncells = static_cast<int64_t>(a.boxM[0]) * a.boxM[1] * a.boxM[2];
+    nb_box_bform_multi_kernel<<<dim3(static_cast<unsigned>((ncells * maxcols + 255) / 256), 1, static_cast<unsigned>(nev)),
+                                256, 0, s>>>(a, bm, ncells);
+    box_pass_multi_m<false>(a, bm, maxcols, s);
+    box_pass_multi_m<true>(a, bm, maxcols, s);
+    return hipGetLastError();
 }
 
 hipError_t nb_launch_init(const NbArgs &a, const double2 *H0, double2 *rho, double2 *U, hipStream_t s)

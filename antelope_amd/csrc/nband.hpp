@@ -101,6 +101,10 @@ hipError_t nb16_launch_step(const NbArgs &a, const NbFused &f, double2 *rho, dou
                             hipStream_t s);
 // the separable sums of one evaluation up to the second pass: t2 [M1][K2][K3][bcols]
 hipError_t nb_launch_box_t2(const NbArgs &a, const NbEval &ev, double2 *t2, hipStream_t s);
+// the same for nev <= 4 evaluations in three launches, each with its own scratch b
+// [cells][bcols], t1 [M1][M2][K3][bcols] and output t2
+hipError_t nb_launch_box_t2_multi(const NbArgs &a, int nev, const NbEval *evs, double2 *const *b, double2 *const *t1,
+                                  double2 *const *t2, hipStream_t s);
 
 // One time step of the VelocityHamiltonian gauge: A at the stage times and
 // the tableau (coefficients already multiplied by dt).

@@ -314,12 +314,14 @@ __device__ __forceinline__ Tile band_diag(const Tile &Y, const Tile &U, const Ti
     return cmm(Ud, false, Ut, true);  // U diag(d) U^dag (B = U^dag: tile conj(U^T))
 }
 
-// The stage derivative at y given H~^T's tile and (dephasing on) y's band-diagonal
-// part M: -i(H~ y - (H~ y)^dag) - g2 y - (g1 - g2) M.
+// The stage derivative at y given H~^T's tile (HERM: H~'s own tile, H~ Hermitian, so
+// that H~^T's tile is its conjugate) and (dephasing on) y's band-diagonal part M:
+// -i(H~ y - (H~ y)^dag) - g2 y - (g1 - g2) M.
+template <bool HERM = false>
 __device__ __forceinline__ Tile stage_derivative_m(const NbArgs &a, const Tile &Y, const Tile &HtT, const Tile &M,
                                                    cd *buf)
 {
-    const Tile T = cmm(HtT, false, Y);  // H~ y
+    const Tile T = cmm(HtT, HERM, Y);  // H~ y
     const Tile Tt = transpose(T, buf);
     Tile X;  // -i (T - T^dag)
 #pragma unroll
@@ -632,15 +634,17 @@ __global__ __launch_bounds__(64) void nb16_vel_step_kernel(const NbArgs a, const
 // HBM traffic. The eight waves' sample sums are added in LDS in wave order
 // into one partial row per block.
 // dynamic LDS of a block: the waves' transpose buffers, then two staging slots of
-// M1 N^2 complex each (at least kNbFusedWaves nv doubles: the sample sums)
+// kNbFusedMaxM1 x 256 complex (row m of the second-pass output at m * 256, so the
+// m offsets are immediates), which also hold the sample sums (kNbFusedWaves nv doubles)
+constexpr int kFusedPitch = 256;
 struct StepLds {
     cd *tb;     // [kNbFusedWaves][16 kTpad]
-    cd *ys[2];  // [m][i N + j]
+    cd *ys[2];  // [m][i N + j], m pitch kFusedPitch
 };
 
-__host__ __device__ inline int64_t step_slot_elems(int N, int M1, int nv)
+__host__ __device__ inline int64_t step_slot_elems(int nv)
 {
-    const int64_t mats = static_cast<int64_t>(M1) * N * N;
+    const int64_t mats = static_cast<int64_t>(kNbFusedMaxM1) * kFusedPitch;
     const int64_t red = (static_cast<int64_t>(kNbFusedWaves) * nv + 3) / 4;  // the two slots hold W nv doubles
     return mats > red ? mats : red;
 }
@@ -652,45 +656,61 @@ __device__ __forceinline__ void stage_mats(const NbArgs &a, StepLds &L, const cd
 {
     const int nn = a.N * a.N, M1 = a.boxM[0];
     const int64_t mstride = static_cast<int64_t>(a.boxK[1]) * a.boxK[2] * a.bcols;  // T2 row stride of m
-    const int per = M1 * nn;
-    const int total = src1 ? 2 * per : per;
+    const int64_t cbase = col * a.bcols;
     __syncthreads();  // the last staging's readers are done
-    for (int t = threadIdx.x; t < total; t += blockDim.x) {
-        const int slot = t >= per ? 1 : 0;
-        const int rem = t - slot * per;
-        const int m = rem / nn, ij = rem - m * nn;
+    // idx = (slot, m, ij) with ij < 256 and m < 8: shifts only
+    for (int idx = threadIdx.x; idx < 2 * kNbFusedMaxM1 * kFusedPitch; idx += blockDim.x) {
+        const int slot = idx >> 11, m = (idx >> 8) & 7, ij = idx & 255;
         const cd *src = slot ? src1 : src0;
+        if (m >= M1 || ij >= nn || !src) continue;
         const int j = slot ? j1 : j0;
-        L.ys[slot][rem] = src[m * mstride + col * a.bcols + j * nn + ij];
+        L.ys[slot][m * kFusedPitch + ij] = src[m * mstride + cbase + j * nn + ij];
     }
     __syncthreads();
 }
 
-// this lane's elements of X = sum_m E1[m] Y[m] from a staging slot (trans: of X^T),
-// in the order of operations of the a1 pass of the separable sums
-__device__ __forceinline__ Tile form_tile(const cd *ys, const cd *__restrict__ e1, int M1, int N, bool trans)
+// this lane's elements of X = sum_m E1[m] Y[m] from a staging slot, in the order of
+// operations of the a1 pass of the separable sums
+__device__ __forceinline__ Tile form_tile(const cd *ys, const cd *__restrict__ e1, int M1, int N)
 {
-    const int c = lane_c(), g = lane_g(), nn = N * N;
-    Tile T;
+    const int c = lane_c(), g = lane_g();
+    int off[4];
+    bool ok[4];
+    double re[4], im[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int i = g + 4 * r;
-        double re = 0.0, im = 0.0;
-        if (i < N && c < N) {
-            const int ij = trans ? c * N + i : i * N + c;
+        ok[r] = i < N && c < N;
+        off[r] = ok[r] ? i * N + c : 0;
+        re[r] = im[r] = 0.0;
+    }
 #pragma unroll
-            for (int m = 0; m < kNbFusedMaxM1; ++m) {
-                if (m < M1) {
-                    const cd w = e1[m], x = ys[m * nn + ij];
-                    re = fma(w.x, x.x, fma(-w.y, x.y, re));
-                    im = fma(w.x, x.y, fma(w.y, x.x, im));
-                }
+    for (int m = 0; m < kNbFusedMaxM1; ++m) {
+        if (m < M1) {
+            const cd w = e1[m];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const cd x = ys[m * kFusedPitch + off[r]];
+                re[r] = fma(w.x, x.x, fma(-w.y, x.y, re[r]));
+                im[r] = fma(w.x, x.y, fma(w.y, x.x, im[r]));
             }
         }
-        T.re[r] = re;
-        T.im[r] = im;
+    }
+    Tile T;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        T.re[r] = ok[r] ? re[r] : 0.0;
+        T.im[r] = ok[r] ? im[r] : 0.0;
     }
     return T;
+}
+
+// x of lane j (j uniform over the wave)
+__device__ __forceinline__ double lane_value(double x, int j)
+{
+    const int lo = __builtin_amdgcn_readlane(__double2loint(x), j);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(x), j);
+    return __hiloint2double(hi, lo);
 }
 
 __device__ __forceinline__ void axpy_tile(Tile &Y, const Tile &X, double s, const Tile &B)
@@ -711,7 +731,7 @@ __global__ __launch_bounds__(64 * kNbFusedWaves) __attribute__((amdgpu_waves_per
 {
     extern __shared__ cd step_lds[];
     const int N = a.N, nn = N * N;
-    const int64_t slot = step_slot_elems(N, a.boxM[0], a.nv);
+    const int64_t slot = step_slot_elems(a.nv);
     StepLds L;
     L.tb = step_lds;
     L.ys[0] = step_lds + kNbFusedWaves * 16 * kTpad;
@@ -732,7 +752,7 @@ __global__ __launch_bounds__(64 * kNbFusedWaves) __attribute__((amdgpu_waves_per
     cd *buf = L.tb + w * 16 * kTpad;
     const double dt = f.dt;
 
-    Tile R, U, Ut, Y, Acc, HtT, X;
+    Tile R, U, Ut, Y, Acc, Ht, X;
     double lam = 0.0, p = 0.0;
     double cur[6] = {0, 0, 0, 0, 0, 0};
     double pop = 0.0;
@@ -747,7 +767,7 @@ __global__ __launch_bounds__(64 * kNbFusedWaves) __attribute__((amdgpu_waves_per
     if (!f.u_valid || !f.lam_valid) {  // eigenvectors / energies at t_n from H(t_n)
         stage_mats(a, L, f.t2_prev, 1, nullptr, 0, col);
         if (active) {
-            const Tile H = form_tile(L.ys[0], e1, M1, N, false);
+            const Tile H = form_tile(L.ys[0], e1, M1, N);
             if (!f.u_valid) eig_refine(H, U, Ut, N, buf);
             lam = col_dot(U, cmm(H, true, U));
         }
@@ -761,12 +781,12 @@ __global__ __launch_bounds__(64 * kNbFusedWaves) __attribute__((amdgpu_waves_per
         M = band_diag(R, U, Ut, p);  // p: populations of U's columns
         int rank = 0;                // rank of column c among the N energies, ties by index
         for (int j = 0; j < N; ++j) {
-            const double lj = __shfl(lam, j, 64);
+            const double lj = lane_value(lam, j);
             rank += (lj < lam) || (lj == lam && j < c);
         }
         for (int j = 0; j < N; ++j) {
-            const int rj = __shfl(rank, j, 64);
-            const double pj = __shfl(p, j, 64);
+            const int rj = __builtin_amdgcn_readlane(rank, j);
+            const double pj = lane_value(p, j);
             if (rj == lane) pop += pj * wk;
         }
 #pragma unroll
@@ -775,14 +795,14 @@ __global__ __launch_bounds__(64 * kNbFusedWaves) __attribute__((amdgpu_waves_per
             racc[2 * r + 1] = R.im[r] * wk;
         }
     }
-    // Re Tr(rho J_a) and Re Tr(M J_a) = sum_c p_c Re (U^dag J_a U)_cc, elementwise from
-    // J_a read transposed (J may not be Hermitian)
-    auto current = [&](int ax, const Tile &Jt) {
+    // Re Tr(rho J_a) = Re sum_ij conj(rho_ij) J_ij (rho Hermitian; J need not be) and
+    // Re Tr(M J_a) = sum_c p_c Re (U^dag J_a U)_cc the same way, elementwise on J's tile
+    auto current = [&](int ax, const Tile &J) {
         double tot = 0.0, intra = 0.0;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            tot += R.re[r] * Jt.re[r] - R.im[r] * Jt.im[r];
-            intra += M.re[r] * Jt.re[r] - M.im[r] * Jt.im[r];
+            tot += R.re[r] * J.re[r] + R.im[r] * J.im[r];
+            intra += M.re[r] * J.re[r] + M.im[r] * J.im[r];
         }
         tot = wave_sum(tot);
         intra = a.inter_intra ? wave_sum(intra) : 0.0;
@@ -791,15 +811,15 @@ __global__ __launch_bounds__(64 * kNbFusedWaves) __attribute__((amdgpu_waves_per
     };
     stage_mats(a, L, f.t2_j, 0, f.t2_j, 1, col);
     if (active) {
-        current(0, form_tile(L.ys[0], e1, M1, N, true));
-        current(1, form_tile(L.ys[1], e1, M1, N, true));
+        current(0, form_tile(L.ys[0], e1, M1, N));
+        current(1, form_tile(L.ys[1], e1, M1, N));
     }
     // ---- stage 1 at t_n (H~ of t_n), with J_z
     stage_mats(a, L, f.t2_j, 2, f.t2_prev, 0, col);
     if (active) {
-        current(2, form_tile(L.ys[0], e1, M1, N, true));
-        HtT = form_tile(L.ys[1], e1, M1, N, true);
-        X = stage_derivative_m(a, R, HtT, M, buf);
+        current(2, form_tile(L.ys[0], e1, M1, N));
+        Ht = form_tile(L.ys[1], e1, M1, N);  // H~ (Hermitian: its conjugate is H~^T's tile)
+        X = stage_derivative_m<true>(a, R, Ht, M, buf);
         axpy_tile(Acc, X, dt / 6.0, R);
         axpy_tile(Y, X, 0.5 * dt, R);
     }
@@ -835,17 +855,21 @@ __global__ __launch_bounds__(64 * kNbFusedWaves) __attribute__((amdgpu_waves_per
     // ---- stages 2 and 3 at t_n + dt/2
     stage_mats(a, L, f.t2_half, 0, f.t2_half, 1, col);
     if (active) {
-        HtT = form_tile(L.ys[0], e1, M1, N, true);
+        Ht = form_tile(L.ys[0], e1, M1, N);
         // within the step only: the last round at max|K| < 1e-6 and no final Newton-Schulz step
-        if (a.deph) eig_refine(form_tile(L.ys[1], e1, M1, N, false), U, Ut, N, buf, kFusedSmallK, false);
-        X = stage_derivative(a, Y, HtT, U, Ut, buf);
+        if (a.deph) eig_refine(form_tile(L.ys[1], e1, M1, N), U, Ut, N, buf, kFusedSmallK, false);
+        Tile Mh{};
+        double dd;
+        if (a.deph) Mh = band_diag(Y, U, Ut, dd);
+        X = stage_derivative_m<true>(a, Y, Ht, Mh, buf);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             Acc.re[r] = fma(dt / 3.0, X.re[r], Acc.re[r]);
             Acc.im[r] = fma(dt / 3.0, X.im[r], Acc.im[r]);
         }
         axpy_tile(Y, X, 0.5 * dt, R);
-        X = stage_derivative(a, Y, HtT, U, Ut, buf);
+        if (a.deph) Mh = band_diag(Y, U, Ut, dd);
+        X = stage_derivative_m<true>(a, Y, Ht, Mh, buf);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             Acc.re[r] = fma(dt / 3.0, X.re[r], Acc.re[r]);
@@ -856,11 +880,14 @@ __global__ __launch_bounds__(64 * kNbFusedWaves) __attribute__((amdgpu_waves_per
     // ---- stage 4 at t_n + dt; eigenvectors and energies for the next sample
     stage_mats(a, L, f.t2_end, 0, f.t2_end, 1, col);
     if (active) {
-        HtT = form_tile(L.ys[0], e1, M1, N, true);
-        const Tile H = form_tile(L.ys[1], e1, M1, N, false);
+        Ht = form_tile(L.ys[0], e1, M1, N);
+        const Tile H = form_tile(L.ys[1], e1, M1, N);
         eig_refine(H, U, Ut, N, buf, kFusedSmallK, true);
         lam = col_dot(U, cmm(H, true, U));
-        X = stage_derivative(a, Y, HtT, U, Ut, buf);
+        Tile Me{};
+        double dd;
+        if (a.deph) Me = band_diag(Y, U, Ut, dd);
+        X = stage_derivative_m<true>(a, Y, Ht, Me, buf);
         axpy_tile(R, X, dt / 6.0, Acc);
         store_tile(rho + k * nn, R, N);
         store_tile(Ust + k * nn, U, N);
@@ -908,7 +935,9 @@ hipError_t nb16_launch_sample(const NbArgs &a, const double2 *mats, const double
 
 size_t nb16_step_lds_bytes(int N, int M1, int nv)
 {
-    return sizeof(double2) * (static_cast<size_t>(kNbFusedWaves) * 16 * kTpad + 2 * step_slot_elems(N, M1, nv));
+    (void)N;
+    (void)M1;
+    return sizeof(double2) * (static_cast<size_t>(kNbFusedWaves) * 16 * kTpad + 2 * step_slot_elems(nv));
 }
 
 hipError_t nb16_launch_step(const NbArgs &a, const NbFused &f, double2 *rho, double2 *U, double *partials,

@@ -230,9 +230,12 @@ NbPlan nb_plan(int N, int nr, int64_t nk, int64_t nt, int nstage, bool split16, 
         p.bytes_box = o;
     }
     p.bytes_fused = 0;
-    if (fused) {
-        const size_t t2 = align256(sizeof(double2) * static_cast<size_t>(p.bcols) * bx.M[0] * bx.K[1] * bx.K[2]);
-        p.bytes_fused = 4 * t2 + align256(static_cast<size_t>(p.nk_stride) * 16 * sizeof(double));
+    if (fused) {  // four second-pass outputs, the energies, and per evaluation B and the first pass
+        const size_t col = sizeof(double2) * static_cast<size_t>(p.bcols);
+        const size_t t2 = align256(col * bx.M[0] * bx.K[1] * bx.K[2]);
+        const size_t b = align256(col * static_cast<size_t>(bx.cells()));
+        const size_t t1 = align256(col * bx.M[0] * bx.M[1] * bx.K[2]);
+        p.bytes_fused = 4 * t2 + align256(static_cast<size_t>(p.nk_stride) * 16 * sizeof(double)) + 4 * (b + t1);
     }
     return p;
 }
@@ -504,9 +507,17 @@ int nb_setup(sbe_ctx *ctx)
     ctx->nbf = false;
     if (fbuf && a.box && a.tile16 && a.boxK[0] == bx.K[0] && a.boxK[1] == bx.K[1] && a.boxK[2] == bx.K[2] &&
         nk == bx.kcells() && nb16_step_lds_bytes(N, bx.M[0], a.nv) <= (size_t(160) << 10)) {
-        const size_t t2 = align256(sizeof(double2) * static_cast<size_t>(p.bcols) * bx.M[0] * bx.K[1] * bx.K[2]);
+        const size_t col = sizeof(double2) * static_cast<size_t>(p.bcols);
+        const size_t t2 = align256(col * bx.M[0] * bx.K[1] * bx.K[2]);
+        const size_t b = align256(col * static_cast<size_t>(bx.cells()));
+        const size_t t1 = align256(col * bx.M[0] * bx.M[1] * bx.K[2]);
         for (int i = 0; i < 4; ++i) ctx->nbf_t2[i] = reinterpret_cast<double2 *>(fbuf + i * t2);
         ctx->nbf_lam = reinterpret_cast<double *>(fbuf + 4 * t2);
+        char *sc = fbuf + 4 * t2 + align256(static_cast<size_t>(p.nk_stride) * 16 * sizeof(double));
+        for (int i = 0; i < 4; ++i) {
+            ctx->nbf_b[i] = reinterpret_cast<double2 *>(sc + i * (b + t1));
+            ctx->nbf_t1[i] = reinterpret_cast<double2 *>(sc + i * (b + t1) + b);
+        }
         ctx->nbf_nblocks = fused_blocks(bx, ctx->nbf_g1);
         ctx->nbf = true;
     }
@@ -607,7 +618,7 @@ int nb_run_fused(sbe_ctx *ctx, int64_t step_begin, int64_t step_end)
     const NbArgs &a = ctx->nargs;
     const double dt = ctx->laser.dt;
     int rc = SBE_OK;
-    auto eval = [&](double t, bool hmats, double2 *t2) {
+    auto eval = [&](double t, bool hmats) {
         NbEval ev{};
         ctx->laser.eval(t, ev.A, ev.E);
         if (hmats) {
@@ -620,17 +631,29 @@ int nb_run_fused(sbe_ctx *ctx, int64_t step_begin, int64_t step_end)
             ev.mats[1] = kMatJy;
             ev.mats[2] = kMatJz;
         }
-        return hipc(ctx, nb_launch_box_t2(a, ev, t2, ctx->stream), "fourier passes");
+        return ev;
     };
     for (int64_t c0 = step_begin; c0 < step_end && !rc; c0 += ctx->chunk_steps) {
         const int64_t n = std::min<int64_t>(ctx->chunk_steps, step_end - c0);
         for (int64_t s = 0; s < n && !rc; ++s) {
             const double tn = ctx->laser.atmin + dt * static_cast<double>(c0 + s);  // antelope.cpp:299
-            // the Fourier sums of the step's times do not depend on rho: all before the step
-            if (!(ctx->nbf_prev_valid && same_time(ctx->nbf_prev_time, tn, dt))) rc = eval(tn, true, ctx->nbf_t2[0]);
-            if (!rc) rc = eval(tn, false, ctx->nbf_t2[1]);
-            if (!rc) rc = eval(tn + 0.5 * dt, true, ctx->nbf_t2[2]);
-            if (!rc) rc = eval(tn + dt, true, ctx->nbf_t2[3]);
+            // the Fourier sums of the step's times do not depend on rho: all before the step,
+            // in one batch of launches
+            NbEval evs[4];
+            double2 *t2s[4];
+            int nev = 0;
+            if (!(ctx->nbf_prev_valid && same_time(ctx->nbf_prev_time, tn, dt))) {
+                evs[nev] = eval(tn, true);
+                t2s[nev++] = ctx->nbf_t2[0];
+            }
+            evs[nev] = eval(tn, false);
+            t2s[nev++] = ctx->nbf_t2[1];
+            evs[nev] = eval(tn + 0.5 * dt, true);
+            t2s[nev++] = ctx->nbf_t2[2];
+            evs[nev] = eval(tn + dt, true);
+            t2s[nev++] = ctx->nbf_t2[3];
+            rc = hipc(ctx, nb_launch_box_t2_multi(a, nev, evs, ctx->nbf_b, ctx->nbf_t1, t2s, ctx->stream),
+                      "fourier passes");
             if (rc) break;
             NbFused f{};
             f.t2_prev = ctx->nbf_t2[0];

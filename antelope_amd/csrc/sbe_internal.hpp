@@ -180,6 +180,8 @@ struct sbe_ctx {
     bool nbf = false;
     double2 *nbf_t2[4] = {nullptr, nullptr, nullptr, nullptr};  // H~ H at t_n, J at t_n, H~ H at t + dt/2, at t + dt
     double *nbf_lam = nullptr;   // [nk_stride][16] band energies at the stored eigenvectors' time
+    double2 *nbf_b[4] = {nullptr, nullptr, nullptr, nullptr};   // per evaluation: B on the box
+    double2 *nbf_t1[4] = {nullptr, nullptr, nullptr, nullptr};  // per evaluation: first-pass output
     int nbf_g1 = 0, nbf_nblocks = 0;
     bool nbf_prev_valid = false;  // nbf_t2[0] holds H~, H at nbf_prev_time
     double nbf_prev_time = 0;

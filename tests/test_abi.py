@@ -150,10 +150,12 @@ def test_separable_sums_footprint_without_gpu():
     nk_stride, nr = (K[0] * K[1] * K[2] + 63) // 64 * 64, cells  # the dense e^{ik.R} table is not kept
     want -= nk_stride * nr * 16 + (nr + 31) // 32 * 32 * col
     assert _footprint_env(cfg, m, dense=False) - _footprint_env(cfg, m, dense=True) == want
-    # the fused RK4 step adds four second-pass outputs [M1][K2][K3][columns] and the band
-    # energies [nk_stride][16] (FP64); its 10 partial rows per step fit in the 60 the sample
-    # kernel has
-    fused = 4 * al(M[0] * K[1] * K[2] * col) + al(nk_stride * 16 * 8)
+    # the fused RK4 step adds four second-pass outputs [M1][K2][K3][columns], the band
+    # energies [nk_stride][16] (FP64) and, for each of up to four evaluations batched in one
+    # launch, B on the box and a first-pass output; its 10 partial rows per step fit in the
+    # 60 the sample kernel has
+    fused = (4 * al(M[0] * K[1] * K[2] * col) + al(nk_stride * 16 * 8)
+             + 4 * (al(cells * col) + al(M[0] * M[1] * K[2] * col)))
     assert _footprint_env(cfg, m, dense=False, fused=True) - _footprint_env(cfg, m, dense=False) == fused
     assert _footprint_env(cfg, m, dense=True, fused=True) == _footprint_env(cfg, m, dense=True)
     rk5 = dataclasses.replace(cfg, rk_order=5)  # the fused step is the classical RK4 tableau only
