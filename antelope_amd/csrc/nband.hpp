@@ -81,7 +81,7 @@ struct NbStage {
 // each evaluation time, so the matrices never reach HBM. t2_* are that output,
 // [M1][K2][K3][bcols] with the evaluation's matrices side by side in the columns.
 constexpr int kNbFusedWaves = 8;  // k points along a1 per block (one wave each)
-constexpr int kNbFusedMaxM1 = 8;  // lattice vectors along a1 staged in LDS
+constexpr int kNbFusedMaxM1 = 7;  // lattice vectors along a1 staged in LDS
 struct NbFused {
     const double2 *t2_prev;  // H~, H at t_n
     const double2 *t2_j;     // J_x, J_y, J_z at t_n

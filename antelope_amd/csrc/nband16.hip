@@ -149,17 +149,53 @@ __device__ __forceinline__ void store_tile(cd *M, const Tile &T, int N)
     }
 }
 
+// x and the value of lane l ^ 16 / l ^ 32 (gfx950 permlane swaps: no LDS)
+__device__ __forceinline__ void pair16(double x, double &a, double &b)
+{
+    const auto lo = __builtin_amdgcn_permlane16_swap(__double2loint(x), __double2loint(x), false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap(__double2hiint(x), __double2hiint(x), false, false);
+    a = __hiloint2double(hi[0], lo[0]);  // even rows: this row's own value in rows 0, 2
+    b = __hiloint2double(hi[1], lo[1]);
+}
+__device__ __forceinline__ void pair32(double x, double &a, double &b)
+{
+    const auto lo = __builtin_amdgcn_permlane32_swap(__double2loint(x), __double2loint(x), false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap(__double2hiint(x), __double2hiint(x), false, false);
+    a = __hiloint2double(hi[0], lo[0]);
+    b = __hiloint2double(hi[1], lo[1]);
+}
+// x + x(l ^ 16) and x + x(l ^ 32), the same value in both lanes of a pair
+__device__ __forceinline__ double add16(double x)
+{
+    double a, b;
+    pair16(x, a, b);
+    return a + b;
+}
+__device__ __forceinline__ double add32(double x)
+{
+    double a, b;
+    pair32(x, a, b);
+    return a + b;
+}
+
 __device__ __forceinline__ double wave_sum(double v)
 {
+    v = add32(v);
+    v = add16(v);
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    for (int o = 8; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
 }
 
 __device__ __forceinline__ double wave_max(double v)
 {
+    double a, b;
+    pair32(v, a, b);
+    v = fmax(a, b);
+    pair16(v, a, b);
+    v = fmax(a, b);
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    for (int o = 8; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
     return v;
 }
 
@@ -169,9 +205,7 @@ __device__ __forceinline__ double col_dot(const Tile &X, const Tile &Y)
     double s = 0.0;
 #pragma unroll
     for (int r = 0; r < 4; ++r) s += X.re[r] * Y.re[r] + X.im[r] * Y.im[r];
-    s += __shfl_xor(s, 16, 64);
-    s += __shfl_xor(s, 32, 64);
-    return s;
+    return add32(add16(s));
 }
 
 // The diagonal of a tile, for this lane's rows (lam_r) and column (lam_c).
@@ -633,39 +667,45 @@ __global__ __launch_bounds__(64) void nb16_vel_step_kernel(const NbArgs a, const
 // refined for the next sample); rho, U and the energies are the only per-k
 // HBM traffic. The eight waves' sample sums are added in LDS in wave order
 // into one partial row per block.
-// dynamic LDS of a block: the waves' transpose buffers, then two staging slots of
-// kNbFusedMaxM1 x 256 complex (row m of the second-pass output at m * 256, so the
-// m offsets are immediates), which also hold the sample sums (kNbFusedWaves nv doubles)
+// dynamic LDS of a block: the waves' transpose buffers, then two buffers of two
+// staging slots of kNbFusedMaxM1 x 256 complex each (row m of the second-pass output
+// at m * 256, so the m offsets are immediates). While a block computes from one
+// buffer, the other fills straight from global memory (global_load_lds); a buffer
+// also holds the block's sample sums (kNbFusedWaves nv doubles) once read.
 constexpr int kFusedPitch = 256;
-struct StepLds {
-    cd *tb;     // [kNbFusedWaves][16 kTpad]
-    cd *ys[2];  // [m][i N + j], m pitch kFusedPitch
-};
+constexpr int64_t kFusedSlot = static_cast<int64_t>(kNbFusedMaxM1) * kFusedPitch;  // complex per slot
+constexpr int64_t kFusedTb = static_cast<int64_t>(kNbFusedWaves) * 16 * kTpad;    // complex of the tb
 
-__host__ __device__ inline int64_t step_slot_elems(int nv)
-{
-    const int64_t mats = static_cast<int64_t>(kNbFusedMaxM1) * kFusedPitch;
-    const int64_t red = (static_cast<int64_t>(kNbFusedWaves) * nv + 3) / 4;  // the two slots hold W nv doubles
-    return mats > red ? mats : red;
-}
+__device__ __forceinline__ cd *fused_slot(cd *lds, int buf, int slot) { return lds + kFusedTb + (2 * buf + slot) * kFusedSlot; }
 
-// T2 rows m < M1 of column col for matrix j0 of src0 and j1 of src1 (src1 may
-// be null) into the staging slots; barriers on both sides
-__device__ __forceinline__ void stage_mats(const NbArgs &a, StepLds &L, const cd *src0, int j0, const cd *src1, int j1,
-                                           int64_t col)
+// This wave's share of filling buffer buf with matrix j0 of src0 and j1 of src1 (src1
+// may be null), rows m < M1 of column col: chunks of 64 complex go straight to LDS,
+// each lane one element; their completion is counted by vmcnt (fused_wait).
+__device__ __forceinline__ void stage_async(const NbArgs &a, cd *lds, int buf, const cd *src0, int j0, const cd *src1,
+                                            int j1, int64_t col, int w, int lane)
 {
     const int nn = a.N * a.N, M1 = a.boxM[0];
-    const int64_t mstride = static_cast<int64_t>(a.boxK[1]) * a.boxK[2] * a.bcols;  // T2 row stride of m
+    const int Q = (nn + 63) >> 6;  // chunks per row
+    const int64_t mstride = static_cast<int64_t>(a.boxK[1]) * a.boxK[2] * a.bcols;
     const int64_t cbase = col * a.bcols;
-    __syncthreads();  // the last staging's readers are done
-    // idx = (slot, m, ij) with ij < 256 and m < 8: shifts only
-    for (int idx = threadIdx.x; idx < 2 * kNbFusedMaxM1 * kFusedPitch; idx += blockDim.x) {
-        const int slot = idx >> 11, m = (idx >> 8) & 7, ij = idx & 255;
+    const int per = M1 * Q;
+    const int total = src1 ? 2 * per : per;
+    for (int ch = w; ch < total; ch += kNbFusedWaves) {  // uniform over the wave
+        const int slot = ch >= per ? 1 : 0;
+        const int rem = ch - slot * per;
+        const int m = rem / Q, q = rem - m * Q;
         const cd *src = slot ? src1 : src0;
-        if (m >= M1 || ij >= nn || !src) continue;
         const int j = slot ? j1 : j0;
-        L.ys[slot][m * kFusedPitch + ij] = src[m * mstride + cbase + j * nn + ij];
+        const int ij = q * 64 + lane;
+        cd *dst = lds + kFusedTb + (2 * buf + slot) * kFusedSlot + m * kFusedPitch + q * 64;
+        if (ij < nn) __builtin_amdgcn_global_load_lds(src + m * mstride + cbase + j * nn + ij, dst, 16, 0, 0);
     }
+}
+
+// every wave's staging loads have landed and every wave is past its last reads
+__device__ __forceinline__ void fused_wait()
+{
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
 }
 
@@ -731,11 +771,6 @@ __global__ __launch_bounds__(64 * kNbFusedWaves) __attribute__((amdgpu_waves_per
 {
     extern __shared__ cd step_lds[];
     const int N = a.N, nn = N * N;
-    const int64_t slot = step_slot_elems(a.nv);
-    StepLds L;
-    L.tb = step_lds;
-    L.ys[0] = step_lds + kNbFusedWaves * 16 * kTpad;
-    L.ys[1] = L.ys[0] + slot;
     const int c = lane_c(), g = lane_g(), lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
     const int K1 = a.boxK[0], K2 = a.boxK[1], K3 = a.boxK[2], M1 = a.boxM[0];
@@ -749,7 +784,7 @@ __global__ __launch_bounds__(64 * kNbFusedWaves) __attribute__((amdgpu_waves_per
     const bool active = k1 < K1;
     const int64_t k = active ? a.box_kinv[(static_cast<int64_t>(k1) * K2 + k2) * K3 + k3] : 0;
     const cd *e1 = a.boxE[0] + static_cast<int64_t>(active ? k1 : 0) * M1;
-    cd *buf = L.tb + w * 16 * kTpad;
+    cd *buf = step_lds + w * 16 * kTpad;
     const double dt = f.dt;
 
     Tile R, U, Ut, Y, Acc, Ht, X;
@@ -758,16 +793,21 @@ __global__ __launch_bounds__(64 * kNbFusedWaves) __attribute__((amdgpu_waves_per
     double pop = 0.0;
     double racc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     const double wk = active ? a.weight[k] : 0.0;
+    const bool need_h = !f.u_valid || !f.lam_valid;
+    // staging order: [H(t_n) if needed] [J_x J_y] [J_z H~(t_n)] [H~ H at t + dt/2] [H~ H at t + dt],
+    // buffers alternating, each filled while the previous one is read
+    if (need_h) stage_async(a, step_lds, 1, f.t2_prev, 1, nullptr, 0, col, w, lane);
+    stage_async(a, step_lds, 0, f.t2_j, 0, f.t2_j, 1, col, w, lane);
     if (active) {
         R = load_tile(rho + k * nn, N);
         U = load_tile(Ust + k * nn, N, true);
         Ut = transpose(U, buf);
     }
     // ---- sample at t_n
-    if (!f.u_valid || !f.lam_valid) {  // eigenvectors / energies at t_n from H(t_n)
-        stage_mats(a, L, f.t2_prev, 1, nullptr, 0, col);
+    if (need_h) {  // eigenvectors / energies at t_n from H(t_n)
+        fused_wait();
         if (active) {
-            const Tile H = form_tile(L.ys[0], e1, M1, N);
+            const Tile H = form_tile(fused_slot(step_lds, 1, 0), e1, M1, N);
             if (!f.u_valid) eig_refine(H, U, Ut, N, buf);
             lam = col_dot(U, cmm(H, true, U));
         }
@@ -809,24 +849,26 @@ __global__ __launch_bounds__(64 * kNbFusedWaves) __attribute__((amdgpu_waves_per
         cur[ax] += (tot - intra) * wk;
         cur[3 + ax] += intra * wk;
     };
-    stage_mats(a, L, f.t2_j, 0, f.t2_j, 1, col);
+    fused_wait();  // [J_x J_y] in buffer 0; buffer 1 read (H(t_n))
+    stage_async(a, step_lds, 1, f.t2_j, 2, f.t2_prev, 0, col, w, lane);
     if (active) {
-        current(0, form_tile(L.ys[0], e1, M1, N));
-        current(1, form_tile(L.ys[1], e1, M1, N));
+        current(0, form_tile(fused_slot(step_lds, 0, 0), e1, M1, N));
+        current(1, form_tile(fused_slot(step_lds, 0, 1), e1, M1, N));
     }
     // ---- stage 1 at t_n (H~ of t_n), with J_z
-    stage_mats(a, L, f.t2_j, 2, f.t2_prev, 0, col);
+    fused_wait();  // [J_z H~] in buffer 1; buffer 0 read
+    stage_async(a, step_lds, 0, f.t2_half, 0, f.t2_half, 1, col, w, lane);
     if (active) {
-        current(2, form_tile(L.ys[0], e1, M1, N));
-        Ht = form_tile(L.ys[1], e1, M1, N);  // H~ (Hermitian: its conjugate is H~^T's tile)
+        current(2, form_tile(fused_slot(step_lds, 1, 0), e1, M1, N));
+        Ht = form_tile(fused_slot(step_lds, 1, 1), e1, M1, N);  // H~ (Hermitian: its conjugate is H~^T's tile)
         X = stage_derivative_m<true>(a, R, Ht, M, buf);
         axpy_tile(Acc, X, dt / 6.0, R);
         axpy_tile(Y, X, 0.5 * dt, R);
     }
-    // ---- the block's sample sums: one partial row, waves added in order
+    // ---- the block's sample sums in buffer 1 (read): one partial row, waves added in order
     __syncthreads();
     {
-        double *red = reinterpret_cast<double *>(L.ys[0]);  // [kNbFusedWaves][nv], over both slots
+        double *red = reinterpret_cast<double *>(fused_slot(step_lds, 1, 0));  // [kNbFusedWaves][nv]
         double *mine = red + w * a.nv;
         if (lane < 6) {
             double v = 0.0;
@@ -847,17 +889,18 @@ __global__ __launch_bounds__(64 * kNbFusedWaves) __attribute__((amdgpu_waves_per
         __syncthreads();
         double *row = partials + (f.step_local * nb + b) * static_cast<int64_t>(a.nv);
         for (int v = threadIdx.x; v < a.nv; v += blockDim.x) {
-            double s = 0.0;
-            for (int q = 0; q < kNbFusedWaves; ++q) s += red[q * a.nv + v];
-            row[v] = s;
+            double sum = 0.0;
+            for (int q = 0; q < kNbFusedWaves; ++q) sum += red[q * a.nv + v];
+            row[v] = sum;
         }
     }
     // ---- stages 2 and 3 at t_n + dt/2
-    stage_mats(a, L, f.t2_half, 0, f.t2_half, 1, col);
+    fused_wait();  // [H~ H at t + dt/2] in buffer 0; buffer 1 read (sample sums)
+    stage_async(a, step_lds, 1, f.t2_end, 0, f.t2_end, 1, col, w, lane);
     if (active) {
-        Ht = form_tile(L.ys[0], e1, M1, N);
+        Ht = form_tile(fused_slot(step_lds, 0, 0), e1, M1, N);
         // within the step only: the last round at max|K| < 1e-6 and no final Newton-Schulz step
-        if (a.deph) eig_refine(form_tile(L.ys[1], e1, M1, N), U, Ut, N, buf, kFusedSmallK, false);
+        if (a.deph) eig_refine(form_tile(fused_slot(step_lds, 0, 1), e1, M1, N), U, Ut, N, buf, kFusedSmallK, false);
         Tile Mh{};
         double dd;
         if (a.deph) Mh = band_diag(Y, U, Ut, dd);
@@ -878,10 +921,10 @@ __global__ __launch_bounds__(64 * kNbFusedWaves) __attribute__((amdgpu_waves_per
         axpy_tile(Y, X, dt, R);
     }
     // ---- stage 4 at t_n + dt; eigenvectors and energies for the next sample
-    stage_mats(a, L, f.t2_end, 0, f.t2_end, 1, col);
+    fused_wait();  // [H~ H at t + dt] in buffer 1
     if (active) {
-        Ht = form_tile(L.ys[0], e1, M1, N);
-        const Tile H = form_tile(L.ys[1], e1, M1, N);
+        Ht = form_tile(fused_slot(step_lds, 1, 0), e1, M1, N);
+        const Tile H = form_tile(fused_slot(step_lds, 1, 1), e1, M1, N);
         eig_refine(H, U, Ut, N, buf, kFusedSmallK, true);
         lam = col_dot(U, cmm(H, true, U));
         Tile Me{};
@@ -937,7 +980,9 @@ size_t nb16_step_lds_bytes(int N, int M1, int nv)
 {
     (void)N;
     (void)M1;
-    return sizeof(double2) * (static_cast<size_t>(kNbFusedWaves) * 16 * kTpad + 2 * step_slot_elems(nv));
+    // the sample sums must fit in one buffer (two slots)
+    if (static_cast<int64_t>(kNbFusedWaves) * nv > 4 * kFusedSlot) return ~size_t(0);
+    return sizeof(double2) * static_cast<size_t>(kFusedTb + 4 * kFusedSlot);
 }
 
 hipError_t nb16_launch_step(const NbArgs &a, const NbFused &f, double2 *rho, double2 *U, double *partials,
