@@ -250,12 +250,15 @@ __device__ __forceinline__ double orthonormalize(Tile &U, Tile &Ut, cd *buf)
 // small: max|K| below which the last round is U (I + K) (its neglected K^2/2 and the
 // next round's O(K^2) residual are the eigenvectors' error); ortho: end with a
 // Newton-Schulz step (U unitary to rounding rather than to O(small^2))
+// herm: Hin is Hermitian to rounding already (FP64 sums), no symmetrization; lam_out:
+// this lane's column of diag(U^dag H U) before the last rotation (the band energies to
+// O(max|K|^2), for ranking)
 __device__ void eig_refine(const Tile &Hin, Tile &U, Tile &Ut, int N, cd *buf, double small_tol = 1e-8,
-                           bool ortho = true)
+                           bool ortho = true, bool herm = false, double *lam_out = nullptr)
 {
     const int c = lane_c(), g = lane_g();
-    Tile H;
-    {
+    Tile H = Hin;
+    if (!herm) {
         const Tile T = transpose(Hin, buf);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
@@ -264,9 +267,11 @@ __device__ void eig_refine(const Tile &Hin, Tile &U, Tile &Ut, int N, cd *buf, d
         }
     }
     Tile W = cmm(U, true, cmm(H, true, U));  // U^dag (H U)
+    double lam_last = 0.0;
     for (int it = 0; it < 30; ++it) {
         double lam_r[4], lam_c;
         diag_bcast(W, lam_r, lam_c);
+        lam_last = lam_c;
         Tile K;
         double kmax = 0.0;
 #pragma unroll
@@ -328,6 +333,7 @@ __device__ void eig_refine(const Tile &Hin, Tile &U, Tile &Ut, int N, cd *buf, d
         }
     }
     if (ortho) orthonormalize(U, Ut, buf);
+    if (lam_out) *lam_out = lam_last;
 }
 
 // ------------------------------------------------------------- RK stage --
@@ -783,9 +789,11 @@ __global__ __launch_bounds__(64 * kNbFusedWaves) __attribute__((amdgpu_waves_per
     const int k1 = grp * kNbFusedWaves + w;
     const bool active = k1 < K1;
     const int64_t k = active ? a.box_kinv[(static_cast<int64_t>(k1) * K2 + k2) * K3 + k3] : 0;
-    const cd *e1 = a.boxE[0] + static_cast<int64_t>(active ? k1 : 0) * M1;
     cd *buf = step_lds + w * 16 * kTpad;
     const double dt = f.dt;
+    // this wave's a1 factors E1[k1][m] in LDS (read as broadcasts by every tile formed)
+    cd *e1 = step_lds + kFusedTb + 4 * kFusedSlot + w * kNbFusedMaxM1;
+    if (lane < M1) e1[lane] = a.boxE[0][static_cast<int64_t>(active ? k1 : 0) * M1 + lane];
 
     Tile R, U, Ut, Y, Acc, Ht, X;
     double lam = 0.0, p = 0.0;
@@ -824,10 +832,12 @@ __global__ __launch_bounds__(64 * kNbFusedWaves) __attribute__((amdgpu_waves_per
             const double lj = lane_value(lam, j);
             rank += (lj < lam) || (lj == lam && j < c);
         }
-        for (int j = 0; j < N; ++j) {
-            const int rj = __builtin_amdgcn_readlane(rank, j);
-            const double pj = lane_value(p, j);
-            if (rj == lane) pop += pj * wk;
+        // lane rank(c) receives column c's population (forward permute; lanes c < N)
+        {
+            const int dst = (lane < N ? rank : lane) * 4;  // byte address of the receiving lane
+            const int lo = __builtin_amdgcn_ds_permute(dst, __double2loint(p));
+            const int hi = __builtin_amdgcn_ds_permute(dst, __double2hiint(p));
+            if (lane < N) pop += __hiloint2double(hi, lo) * wk;
         }
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
@@ -900,7 +910,8 @@ __global__ __launch_bounds__(64 * kNbFusedWaves) __attribute__((amdgpu_waves_per
     if (active) {
         Ht = form_tile(fused_slot(step_lds, 0, 0), e1, M1, N);
         // within the step only: the last round at max|K| < 1e-6 and no final Newton-Schulz step
-        if (a.deph) eig_refine(form_tile(fused_slot(step_lds, 0, 1), e1, M1, N), U, Ut, N, buf, kFusedSmallK, false);
+        if (a.deph)
+            eig_refine(form_tile(fused_slot(step_lds, 0, 1), e1, M1, N), U, Ut, N, buf, kFusedSmallK, false, true);
         Tile Mh{};
         double dd;
         if (a.deph) Mh = band_diag(Y, U, Ut, dd);
@@ -925,8 +936,7 @@ __global__ __launch_bounds__(64 * kNbFusedWaves) __attribute__((amdgpu_waves_per
     if (active) {
         Ht = form_tile(fused_slot(step_lds, 1, 0), e1, M1, N);
         const Tile H = form_tile(fused_slot(step_lds, 1, 1), e1, M1, N);
-        eig_refine(H, U, Ut, N, buf, kFusedSmallK, true);
-        lam = col_dot(U, cmm(H, true, U));
+        eig_refine(H, U, Ut, N, buf, kFusedSmallK, true, true, &lam);
         Tile Me{};
         double dd;
         if (a.deph) Me = band_diag(Y, U, Ut, dd);
@@ -982,7 +992,7 @@ size_t nb16_step_lds_bytes(int N, int M1, int nv)
     (void)M1;
     // the sample sums must fit in one buffer (two slots)
     if (static_cast<int64_t>(kNbFusedWaves) * nv > 4 * kFusedSlot) return ~size_t(0);
-    return sizeof(double2) * static_cast<size_t>(kFusedTb + 4 * kFusedSlot);
+    return sizeof(double2) * static_cast<size_t>(kFusedTb + 4 * kFusedSlot + kNbFusedWaves * kNbFusedMaxM1);
 }
 
 hipError_t nb16_launch_step(const NbArgs &a, const NbFused &f, double2 *rho, double2 *U, double *partials,
