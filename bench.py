@@ -194,22 +194,28 @@ def bench_wannier(args) -> int:
         forms["split16"]["hhg_cutoff_harmonic"] = hcut
     r = res["fp64"]
     ks = forms["fp64"]["k_steps_per_s"]
-    # flops per k-point RK4 step. Executed: 7 Fourier matrix sums (8 N^2 N_R each: the sample's
-    # 3 J, H~ and H at t + dt/2 and at t + dt; the rest is reused) + 3 products per stage
-    # (H~ y, y U, U diag U^dag) + 5 in the sample (H U, rho U, 3 J U), 8 N^3 each; the eigen
-    # refinement's products (~9 per new stage time, 2 per step) are not counted. The
+    # flops per k-point RK4 step of the fused step (nb16_step_kernel, one launch per step):
+    # 20 complex 16 x 16 products (8 N^3 each; rocprofv3 SQ_INSTS_MFMA = 320 per wave and
+    # step, 16 MFMAs a product: 2 for the sample's band-diagonal part, 3 per stage
+    # (H~ y and the band-diagonal part of y) past the first's 1, 3 per eigen refinement,
+    # 2 for the last Newton-Schulz step), the separable Fourier sums' passes along a3 and
+    # a2 for 7 matrices (M1 M2 K3 M3 + M1 K2 K3 M2 complex multiply-adds per column for the
+    # whole grid) and their pass along a1 inside the step kernel (8 tiles of M1 terms per
+    # k point: J_x, J_y, J_z and H~ at t, H~ and H at t + dt/2 and at t + dt). The
     # reference's algorithm for comparison: 4 stages x (H + 3 D sums + H rho) + the sample's
     # H + 3 J sums and its 8 products.
-    # On a product k grid with R on a lattice box the library evaluates each sum as three
-    # passes along the lattice axes (8 flops per complex multiply-add):
-    # M1 M2 K3 M3 + M1 K2 K3 M2 + K1 K2 K3 M1 per column for the whole grid.
     model = wannier_workload("fp64")[1]
     N, nr = 16, int(model.rvec.shape[0])
     fsum = 8 * N * N * nr
     nk = r["nk"]
     box = _box_passes(model, [48, 48, 8], nk)
-    fsum_exec = 8 * N * N * box / nk if box else fsum
-    flops = 7 * fsum_exec + (4 * 3 + 5) * 8 * N ** 3
+    rv = np.asarray(model.rvec)
+    Mb = [int(rv[:, i].max() - rv[:, i].min() + 1) for i in range(3)]
+    if box:
+        pass23 = Mb[0] * Mb[1] * 8 * Mb[2] + Mb[0] * 48 * 8 * Mb[1]
+        flops = 20 * 8 * N ** 3 + 7 * 8 * N * N * pass23 / nk + 8 * 8 * N * N * Mb[0]
+    else:
+        flops = 7 * fsum + 20 * 8 * N ** 3
     flops_ref = 4 * (4 * fsum + 8 * N ** 3) + 4 * fsum + 8 * 8 * N ** 3
     line = {"metric": METRIC, "value": ks, "unit": "k-point·RK-steps/s", "n_gpus": 1,
             "steps": r["steps"], "warmup": min(args.warmup, 5), "ms_per_step": r["wall_s"] * 1e3 / r["steps"],
@@ -223,9 +229,9 @@ def bench_wannier(args) -> int:
             "reference_equivalent_tflops": ks * flops_ref / 1e12,
             "roofline": {"bound": "mfma", "achieved": ks * flops / 1e12, "peak": PEAK_TFLOPS["fp64"],
                          "unit": "TFLOP/s", "frac": ks * flops / 1e12 / PEAK_TFLOPS["fp64"], "traffic": None,
-                         "note": "executed Fourier-sum and matrix-product flops of the whole step over the "
-                                 "step time (the eigen refinement's are not counted); the Fourier sums run as "
-                                 "separable passes, so the step is bound by the stage kernels' HBM traffic"},
+                         "note": "executed FP64 flops of the whole step (MFMA products as counted by "
+                                 "rocprofv3, the separable Fourier passes) over the step time; one fused "
+                                 "launch per step, HBM traffic about 0.5 GB per step"},
             "fourier_form": "separable box passes" if box else "dense GEMM",
             "timing": r["timing"]}
     print(json.dumps(line), flush=True)
