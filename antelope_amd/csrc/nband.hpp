@@ -106,6 +106,19 @@ hipError_t nb_launch_box_t2(const NbArgs &a, const NbEval &ev, double2 *t2, hipS
 hipError_t nb_launch_box_t2_multi(const NbArgs &a, int nev, const NbEval *evs, double2 *const *b, double2 *const *t1,
                                   double2 *const *t2, hipStream_t s);
 
+// Time-fused VelocityHamiltonian steps for N <= 16 with the classical RK4 tableau
+// (nb16_vel_fused_kernel): one wave per k point for steps [s0, s1), one partial row
+// per block of kNbFusedWaves k points and step.
+struct NbVelFused {
+    const double *A;  // [nt][4][3]: A(t_n + c_s dt)
+    int64_t s0, s1;
+    double dt;
+    int nblocks;      // ceil(nk / kNbFusedWaves)
+};
+size_t nb16_vel_fused_lds_bytes(int nv);
+hipError_t nb16_launch_vel_fused(const NbArgs &a, const NbVelFused &f, const double2 *P, const double *Eocc,
+                                 double2 *rho, double *partials, hipStream_t s);
+
 // One time step of the VelocityHamiltonian gauge: A at the stage times and
 // the tableau (coefficients already multiplied by dt).
 struct NbVelStep {

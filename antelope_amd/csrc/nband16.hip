@@ -948,6 +948,135 @@ __global__ __launch_bounds__(64 * kNbFusedWaves) __attribute__((amdgpu_waves_per
     }
 }
 
+
+// ------------------------------------- velocity gauge, time-fused ----
+// Steps [s0, s1) of the VelocityHamiltonian gauge (SBEs.h:787-805,862-871: the
+// band-basis H = sum_a A_a(t) P_a + diag(E)) with the classical RK4 tableau in one
+// launch: a wave owns one k point for every step, with pMatrix, E, the occupations
+// and rho in registers, so only the per-step partial rows reach HBM. A block's
+// kNbFusedWaves k points add their sample sums in LDS each step into one partial
+// row (waves in order). A at the stage times comes from a table of the whole pulse
+// ([nt][4][3], the host's values), read with uniform loads.
+__global__ __launch_bounds__(64 * kNbFusedWaves) __attribute__((amdgpu_waves_per_eu(2))) void nb16_vel_fused_kernel(
+    const NbArgs a, const NbVelFused f, const cd *__restrict__ P, const double *__restrict__ Eocc,
+    cd *__restrict__ rho, double *__restrict__ partials)
+{
+    extern __shared__ cd vel_lds[];
+    const int N = a.N, nn = N * N;
+    const int c = lane_c(), g = lane_g(), lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * kNbFusedWaves + w;
+    const bool active = k < a.nk;
+    const int64_t ks = a.nk_stride * nn;
+    cd *buf = vel_lds + w * 16 * kTpad;
+    double *red = reinterpret_cast<double *>(vel_lds + kNbFusedWaves * 16 * kTpad);  // [kNbFusedWaves][nv]
+    Tile R, Px, Py, Pz;
+    double Ec = 0.0, oc = 0.0, wk = 0.0;
+    if (active) {
+        R = load_tile(rho + k * nn, N);
+        // pMatrix_a read transposed: not Hermitian for the model classes whose J is not
+        Px = load_tile_t(P + k * nn, N);
+        Py = load_tile_t(P + ks + k * nn, N);
+        Pz = load_tile_t(P + 2 * ks + k * nn, N);
+        Ec = c < N ? Eocc[k * 2 * N + c] : 0.0;      // E of band c
+        oc = c < N ? Eocc[k * 2 * N + N + c] : 0.0;  // occupation of band c
+        wk = a.weight[k];
+    }
+    const double dt = f.dt;
+    for (int64_t step = f.s0; step < f.s1; ++step) {
+        // ---- sample at t_n: band populations, currents Tr(rho P_a) split into the
+        // diagonal (intraband) and off-diagonal parts, rho sums
+        double cur[6] = {0, 0, 0, 0, 0, 0};
+        double pop = 0.0;
+        if (active) {
+            double lr[4], lc;
+            diag_bcast(R, lr, lc);
+            pop = lc * wk;
+            for (int ax = 0; ax < 3; ++ax) {
+                const Tile &Pa = ax == 0 ? Px : (ax == 1 ? Py : Pz);
+                double inter = 0.0, intra = 0.0;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const double val = R.re[r] * Pa.re[r] - R.im[r] * Pa.im[r];  // Re(rho_ij P_ji)
+                    if (g + 4 * r == c && a.inter_intra) intra += val;
+                    else inter += val;
+                }
+                cur[ax] = wave_sum(inter) * wk;
+                cur[3 + ax] = wave_sum(intra) * wk;
+            }
+        }
+        double *mine = red + w * a.nv;
+        if (lane < 6) {
+            double v = 0.0;
+#pragma unroll
+            for (int q = 0; q < 6; ++q)
+                if (lane == q) v = cur[q];
+            mine[lane] = v;
+        }
+        if (lane < N) mine[6 + lane] = pop;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int i = g + 4 * r;
+            if (i < N && c < N) {
+                mine[6 + N + 2 * (i * N + c)] = active ? R.re[r] * wk : 0.0;
+                mine[6 + N + 2 * (i * N + c) + 1] = active ? R.im[r] * wk : 0.0;
+            }
+        }
+        __syncthreads();
+        double *row = partials + ((step - f.s0) * f.nblocks + blockIdx.x) * static_cast<int64_t>(a.nv);
+        for (int v = threadIdx.x; v < a.nv; v += blockDim.x) {
+            double sum = 0.0;
+            for (int q = 0; q < kNbFusedWaves; ++q) sum += red[q * a.nv + v];
+            row[v] = sum;
+        }
+        __syncthreads();  // the rows are read before the next step's sums
+        // ---- the four RK4 stages
+        if (active) {
+            const double *At = f.A + step * 12;
+            Tile Acc, Y = R, X;
+#pragma unroll
+            for (int st = 0; st < 4; ++st) {
+                const double A0 = At[3 * st], A1 = At[3 * st + 1], A2 = At[3 * st + 2];
+                Tile M;  // H^T = sum_a A_a P_a^T + diag(E)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    M.re[r] = A0 * Px.re[r] + A1 * Py.re[r] + A2 * Pz.re[r] + (g + 4 * r == c ? Ec : 0.0);
+                    M.im[r] = A0 * Px.im[r] + A1 * Py.im[r] + A2 * Pz.im[r];
+                }
+                const Tile T = cmm(M, false, Y);  // H y
+                const Tile Tt = transpose(T, buf);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const double dr = T.re[r] - Tt.re[r], di = T.im[r] + Tt.im[r];
+                    double xr = di, xi = -dr;  // -i (H y - (H y)^dag)
+                    if (a.deph) {
+                        const bool dg = g + 4 * r == c;
+                        const double gm = dg ? a.gam1 : a.gam2;
+                        xr -= gm * (Y.re[r] - (dg ? oc : 0.0));
+                        xi -= gm * Y.im[r];
+                    }
+                    X.re[r] = xr;
+                    X.im[r] = xi;
+                }
+                if (st == 0) {
+                    axpy_tile(Acc, X, dt / 6.0, R);
+                    axpy_tile(Y, X, 0.5 * dt, R);
+                } else if (st < 3) {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        Acc.re[r] = fma(dt / 3.0, X.re[r], Acc.re[r]);
+                        Acc.im[r] = fma(dt / 3.0, X.im[r], Acc.im[r]);
+                    }
+                    axpy_tile(Y, X, st == 1 ? 0.5 * dt : dt, R);
+                } else {
+                    axpy_tile(R, X, dt / 6.0, Acc);
+                }
+            }
+        }
+    }
+    if (active) store_tile(rho + k * nn, R, N);
+}
+
 }  // namespace
 
 hipError_t nb16_launch_vel_step(const NbArgs &a, const NbVelStep &v, const double2 *P, const double *Eocc,
@@ -986,6 +1115,25 @@ hipError_t nb16_launch_sample(const NbArgs &a, const double2 *mats, const double
     return hipGetLastError();
 }
 
+// dynamic LDS beyond the default limit: raise the kernel's limit, once per device and kernel
+static hipError_t allow_lds(const void *kernel, size_t lds)
+{
+    if (lds <= (size_t(64) << 10)) return hipSuccess;
+    static std::mutex mu;
+    static std::map<std::pair<int, const void *>, size_t> raised;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    std::lock_guard<std::mutex> lk(mu);
+    size_t &have = raised[{dev, kernel}];
+    if (have < lds) {
+        e = hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+        if (e != hipSuccess) return e;
+        have = lds;
+    }
+    return hipSuccess;
+}
+
 size_t nb16_step_lds_bytes(int N, int M1, int nv)
 {
     (void)N;
@@ -1000,21 +1148,26 @@ hipError_t nb16_launch_step(const NbArgs &a, const NbFused &f, double2 *rho, dou
 {
     if (a.N > 16 || !a.box || a.boxM[0] > kNbFusedMaxM1 || f.nblocks < 1) return hipErrorInvalidValue;
     const size_t lds = nb16_step_lds_bytes(a.N, a.boxM[0], a.nv);
-    if (lds > (size_t(64) << 10)) {  // beyond the default dynamic LDS: raise the kernel's limit, once per device
-        static std::mutex mu;
-        static std::map<int, size_t> raised;
-        int dev = 0;
-        hipError_t e = hipGetDevice(&dev);
-        if (e != hipSuccess) return e;
-        std::lock_guard<std::mutex> lk(mu);
-        if (raised[dev] < lds) {
-            e = hipFuncSetAttribute(reinterpret_cast<const void *>(&nb16_step_kernel),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
-            if (e != hipSuccess) return e;
-            raised[dev] = lds;
-        }
-    }
+    const hipError_t e = allow_lds(reinterpret_cast<const void *>(&nb16_step_kernel), lds);
+    if (e != hipSuccess) return e;
     nb16_step_kernel<<<dim3(static_cast<unsigned>(f.nblocks)), 64 * kNbFusedWaves, lds, s>>>(a, f, rho, U, partials);
+    return hipGetLastError();
+}
+
+size_t nb16_vel_fused_lds_bytes(int nv)
+{
+    return sizeof(double2) * static_cast<size_t>(kNbFusedWaves) * 16 * kTpad + sizeof(double) * kNbFusedWaves * nv;
+}
+
+hipError_t nb16_launch_vel_fused(const NbArgs &a, const NbVelFused &f, const double2 *P, const double *Eocc,
+                                 double2 *rho, double *partials, hipStream_t s)
+{
+    if (a.N > 16 || f.nblocks < 1 || f.s1 < f.s0) return hipErrorInvalidValue;
+    const size_t lds = nb16_vel_fused_lds_bytes(a.nv);
+    const hipError_t e = allow_lds(reinterpret_cast<const void *>(&nb16_vel_fused_kernel), lds);
+    if (e != hipSuccess) return e;
+    nb16_vel_fused_kernel<<<dim3(static_cast<unsigned>(f.nblocks)), 64 * kNbFusedWaves, lds, s>>>(a, f, P, Eocc, rho,
+                                                                                                  partials);
     return hipGetLastError();
 }
 

@@ -162,6 +162,22 @@ bool fused_eligible(const sbe_config &cfg, const sbe_model &m, int64_t nk, const
     return nk == bx.kcells();  // the whole grid (a k block of it takes the separate kernels)
 }
 
+// The time-fused velocity-gauge steps (nb16_vel_fused_kernel): FP64 with the classical
+// RK4 tableau, N <= 16 on the register-tile kernels; any model (the step has no Fourier
+// sums). ANTELOPE_NB_FUSED=0 keeps one launch per step.
+bool vel_fused_eligible(const sbe_config &cfg, const sbe_model &m, int64_t nk)
+{
+    if (const char *e = std::getenv("ANTELOPE_NB_FUSED"))
+        if (std::atoi(e) == 0) return false;
+    if (cfg.gauge != SBE_GAUGE_VELOCITY_HAMILTONIAN || cfg.rk_order != SBE_RK4 || cfg.precision != SBE_PREC_FP64 ||
+        m.nband > 16)
+        return false;
+    if (const char *e = std::getenv("ANTELOPE_NB_LDS"))
+        if (std::atoi(e) == 1) return false;
+    sbe_tuning tu{};
+    return !(tuning_lookup(cfg, m, nk, tu) && tu.nb_kernels == 2);
+}
+
 int fused_blocks(const BoxShape &bx, int &g1)
 {
     g1 = (bx.K[0] + kNbFusedWaves - 1) / kNbFusedWaves;
@@ -177,13 +193,15 @@ struct NbPlan {
     size_t bytes_P, bytes_rho, bytes_mats, bytes_k, bytes_tab, bytes_partials, bytes_series, bytes_split, bytes_b64;
     size_t bytes_box;
     size_t bytes_fused;  // fused steps: four second-pass outputs and the band energies
+    size_t bytes_vel;    // time-fused velocity steps: A at the stage times of the whole pulse
     // offsets inside the box buffer: B, T1, T2, E_1..E_3, R map, k map
     size_t off_t1, off_t2, off_e[3], off_r, off_kinv;
 };
 
 size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 
-NbPlan nb_plan(int N, int nr, int64_t nk, int64_t nt, int nstage, bool split16, const BoxShape &bx, bool fused)
+NbPlan nb_plan(int N, int nr, int64_t nk, int64_t nt, int nstage, bool split16, const BoxShape &bx, bool fused,
+               bool vel_fused = false)
 {
     NbPlan p{};
     p.nk_stride = (nk + 63) / 64 * 64;
@@ -191,7 +209,9 @@ NbPlan nb_plan(int N, int nr, int64_t nk, int64_t nt, int nstage, bool split16, 
     p.nv = 6 + N + 2 * N * N;
     const size_t nn = static_cast<size_t>(N) * N;
     int g1 = 0;
-    const int64_t rows = fused ? std::max<int64_t>(p.sample_waves, fused_blocks(bx, g1)) : p.sample_waves;
+    int64_t rows = fused ? std::max<int64_t>(p.sample_waves, fused_blocks(bx, g1)) : p.sample_waves;
+    if (vel_fused) rows = std::max<int64_t>(rows, (nk + kNbFusedWaves - 1) / kNbFusedWaves);
+    p.bytes_vel = vel_fused ? align256(static_cast<size_t>(nt) * 12 * sizeof(double)) : 0;
     const size_t row = static_cast<size_t>(rows) * p.nv * sizeof(double);
     p.chunk = std::max<int64_t>(1, std::min<int64_t>(nt, static_cast<int64_t>(kNbPartialBytes / row)));
     // e^{ik.R} for the dense sums; the separable form does not read it
@@ -301,9 +321,10 @@ size_t nb_footprint(const sbe_config &cfg, const sbe_model &m, int64_t nk, int64
     const bool split16 = cfg.precision == SBE_PREC_SPLIT16;
     const BoxShape bx = box_shape(cfg, m, nk, split16);
     const bool fused = fused_eligible(cfg, m, nk, bx);
-    const NbPlan p = nb_plan(m.nband, am::is_analytic(m.kind) ? 0 : m.nrpts, nk, nt, nstage, split16, bx, fused);
+    const bool vfused = vel_fused_eligible(cfg, m, nk);
+    const NbPlan p = nb_plan(m.nband, am::is_analytic(m.kind) ? 0 : m.nrpts, nk, nt, nstage, split16, bx, fused, vfused);
     return p.bytes_P + p.bytes_rho + p.bytes_mats + p.bytes_k + p.bytes_tab + p.bytes_partials + p.bytes_series +
-           p.bytes_split + p.bytes_b64 + p.bytes_box + p.bytes_fused;
+           p.bytes_split + p.bytes_b64 + p.bytes_box + p.bytes_fused + p.bytes_vel;
 }
 
 int nb_validate(const sbe_model &m, std::string &err)
@@ -373,7 +394,8 @@ int nb_setup(sbe_ctx *ctx)
     const bool split16 = ctx->cfg.precision == SBE_PREC_SPLIT16;
     const BoxShape bx = box_shape(ctx->cfg, m, nk, split16);
     const bool fused = fused_eligible(ctx->cfg, m, nk, bx);
-    const NbPlan p = nb_plan(N, nr, nk, ctx->laser.nt, ctx->tab.nstage, split16, bx, fused);
+    const bool vfused = vel_fused_eligible(ctx->cfg, m, nk);
+    const NbPlan p = nb_plan(N, nr, nk, ctx->laser.nt, ctx->tab.nstage, split16, bx, fused, vfused);
     const size_t nn = static_cast<size_t>(N) * N;
     ctx->nk = nk;
     ctx->nk_stride = p.nk_stride;
@@ -392,6 +414,7 @@ int nb_setup(sbe_ctx *ctx)
     double2 *b64 = p.bytes_b64 ? nalloc<double2>(ctx, p.bytes_b64, rc) : nullptr;
     char *boxbuf = p.bytes_box ? nalloc<char>(ctx, p.bytes_box, rc) : nullptr;
     char *fbuf = p.bytes_fused ? nalloc<char>(ctx, p.bytes_fused, rc) : nullptr;
+    double *vtab = p.bytes_vel ? nalloc<double>(ctx, p.bytes_vel, rc) : nullptr;
     if (rc) return rc;
 
     // host copies of the tables: k [nk_stride][3], w, R cartesian, weights, H, r
@@ -554,6 +577,22 @@ int nb_setup(sbe_ctx *ctx)
         if (!rc) rc = hipc(ctx, nb_launch_vel_init(a, ctx->nb_mats, Pv, Eocc, ctx->nb_U, ctx->nb_rho, ctx->stream), "velocity init kernel");
         const size_t lds_p = nb_vel_lds_bytes(N, ctx->tab.nstage, true);
         ctx->nb_vel_p_lds = lds_p <= (size_t(64) << 10);
+        ctx->nbv = false;
+        if (vtab && a.tile16) {  // A at the four RK4 stage times of every step, as the host evaluates them
+            const int64_t nt = ctx->laser.nt;
+            std::vector<double> hv(static_cast<size_t>(nt) * 12);
+            for (int64_t n = 0; n < nt; ++n) {
+                const double tn = ctx->laser.atmin + ctx->laser.dt * static_cast<double>(n);  // antelope.cpp:299
+                for (int st = 0; st < 4; ++st) {
+                    double E[3];
+                    ctx->laser.eval(tn + ctx->tab.c[st] * ctx->laser.dt, hv.data() + 12 * n + 3 * st, E);  // SBEs.h:612
+                }
+            }
+            if (!rc) rc = hipc(ctx, hipMemcpy(vtab, hv.data(), hv.size() * sizeof(double), hipMemcpyHostToDevice), "upload A");
+            ctx->nbv_A = vtab;
+            ctx->nbv_nblocks = static_cast<int>((nk + kNbFusedWaves - 1) / kNbFusedWaves);
+            ctx->nbv = true;
+        }
         if (!rc) rc = hipc(ctx, hipStreamSynchronize(ctx->stream), "init sync");
         return rc;
     }
@@ -580,6 +619,26 @@ int nb_run_velocity(sbe_ctx *ctx, int64_t step_begin, int64_t step_end)
     const size_t nn = static_cast<size_t>(a.N) * a.N;
     const double2 *Pv = ctx->nb_k;
     const double *Eocc = reinterpret_cast<const double *>(ctx->nb_k + 3 * static_cast<size_t>(a.nk_stride) * nn);
+    if (ctx->nbv) {  // the steps of a chunk in one launch
+        int rc = SBE_OK;
+        for (int64_t c0 = step_begin; c0 < step_end && !rc; c0 += ctx->chunk_steps) {
+            const int64_t n = std::min<int64_t>(ctx->chunk_steps, step_end - c0);
+            NbVelFused f{};
+            f.A = ctx->nbv_A;
+            f.s0 = c0;
+            f.s1 = c0 + n;
+            f.dt = dt;
+            f.nblocks = ctx->nbv_nblocks;
+            rc = hipc(ctx, nb16_launch_vel_fused(a, f, Pv, Eocc, ctx->nb_rho, ctx->d_partials, ctx->stream),
+                      "velocity step kernel");
+            if (!rc)
+                rc = hipc(ctx, launch_series_reduce(ctx->d_partials, n, ctx->nbv_nblocks, a.nv, ctx->d_series + c0 * a.nv,
+                                                    ctx->stream, ctx->nonfinite_d, kNbReduceGroup),
+                          "series kernel");
+            if (!rc) ctx->steps_done = c0 + n;
+        }
+        return rc;
+    }
     NbVelStep v{};
     v.nstage = tb.nstage;
     for (int i = 0; i < tb.nstage; ++i) {

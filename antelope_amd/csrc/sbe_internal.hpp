@@ -187,6 +187,10 @@ struct sbe_ctx {
     double nbf_prev_time = 0;
     bool nbf_lam_valid = false;
     double nbf_lam_time = 0;
+    // time-fused velocity-gauge steps (nb16_vel_fused_kernel)
+    bool nbv = false;
+    const double *nbv_A = nullptr;  // [nt][4][3] A at the RK4 stage times
+    int nbv_nblocks = 0;
     std::string err;
 
     // optional kernel timing: (start, stop) event pairs per kernel class

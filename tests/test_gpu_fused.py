@@ -96,3 +96,29 @@ def test_fused_rk5_keeps_separate_kernels(oracle, monkeypatch):
     assert bytes_a == bytes_b
     ref = oracle.run(cfg, model, nthreads=8)
     assert all(v < TOL64 for v in _errs(a, ref).values())
+
+
+@pytest.mark.parametrize("label,N,kw", [
+    ("n8_velocity_t2", 8, dict()),
+    ("n16_velocity_t1t2_total_current", 16, dict(T2=60.0, inter_intra=False)),
+])
+def test_time_fused_velocity_gauge(oracle, monkeypatch, label, N, kw):
+    """The time-fused velocity-gauge kernel (all steps of a chunk in one launch, one wave
+    per k point) against the oracle and the one-launch-per-step kernel, and split runs."""
+    import dataclasses
+    model = synthetic_wannier(N, nmax=(1, 1, 1), seed=70 + N)
+    T1 = kw.pop("T1", 150.0 if N == 16 else -1.0)
+    cfg = dataclasses.replace(_cfg((5, 4, 2), **kw), gauge="VelocityHamiltonian", T1=T1)
+    print(f"case: {label}", flush=True)
+    ref = oracle.run(cfg, model, nthreads=8, want_state=False)
+    fz, _, bytes_f = _solve(cfg, model, monkeypatch, fused=True)
+    sp, _, bytes_s = _solve(cfg, model, monkeypatch, fused=False)
+    assert bytes_f > bytes_s  # the stage-time table was allocated: it ran time-fused
+    e_or, e_sp = _errs(fz, ref), _errs(fz, sp)
+    print(label, "vs oracle", {k: f"{v:.2e}" for k, v in e_or.items()},
+          "vs one launch per step", {k: f"{v:.2e}" for k, v in e_sp.items()}, flush=True)
+    assert all(v < TOL64 for v in e_or.values()), e_or
+    assert all(v < 1e-12 for v in e_sp.values()), e_sp
+    two, _, _ = _solve(cfg, model, monkeypatch, fused=True, splits=(17, 18, 60))
+    for k in fz:
+        assert np.array_equal(fz[k], two[k]), k
