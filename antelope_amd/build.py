@@ -27,7 +27,7 @@ ARCH = os.environ.get("ANTELOPE_OFFLOAD_ARCH", "gfx950")
 # Library sources (kernels + C ABI). The driver links against the library.
 LIB_SOURCES = [
     "host_setup.cpp", "sbe_api.cpp", "nband_host.cpp", "frontend.cpp", "server.cpp", "wannier_tb.cpp", "atl.cpp", "spectrum.cpp", "tuner.cpp", "material.cpp",
-    "haldane.hip", "haldane_velocity.hip", "haldane_adaptive.hip", "nband.hip", "nband16.hip", "analytic.hip", "series.hip", "noalloc.hip",
+    "haldane.hip", "haldane_velocity.hip", "haldane_adaptive.hip", "nband.hip", "nband16.hip", "nband32.hip", "analytic.hip", "series.hip", "noalloc.hip",
 ]
 DRIVER = "antelope_mi355x"
 

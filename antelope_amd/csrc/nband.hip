@@ -225,9 +225,9 @@ __global__ void nb_box_bform_kernel(const NbArgs a, const NbEval ev, int64_t nce
 
 // One pass: Y[o][k][i][c] = sum_{m < M} E[k][m] X[o][m][i][c] for k < K; block x
 // = (o, i), block y = 256 columns, block z = kc consecutive k. The M inputs of a
-// thread stay in registers (MM >= M); E is read with uniform loads. With out set
-// (the last pass, o = 0) the result goes to out[mat][kinv[k][i]][ij] instead of Y.
-template <int MM>
+// thread stay in registers (MM >= M); E is read with uniform loads. SCATTER (the
+// last pass, o = 0): the result goes to out[mat][kinv[k][i]][ij] instead of Y.
+template <int MM, bool SCATTER>
 __device__ __forceinline__ void box_pass_body(const cd *__restrict__ X, cd *__restrict__ Y, const cd *__restrict__ E,
                                               int M, int K, int kc, int64_t inner, int64_t ncols, int64_t pitch,
                                               const int32_t *__restrict__ kinv, cd *__restrict__ out,
@@ -253,7 +253,7 @@ __device__ __forceinline__ void box_pass_body(const cd *__restrict__ X, cd *__re
                 im = fma(w.x, x[m].y, fma(w.y, x[m].x, im));
             }
         }
-        if (out) {
+        if constexpr (SCATTER) {
             const int32_t kf = kinv[static_cast<int64_t>(k) * inner + i];
             if (kf >= 0) out[(static_cast<int64_t>(mat) * nk_stride + kf) * nn + ij] = cmk(re, im);
         } else {
@@ -262,24 +262,27 @@ __device__ __forceinline__ void box_pass_body(const cd *__restrict__ X, cd *__re
     }
 }
 
-template <int MM>
+template <int MM, bool SCATTER>
 __global__ __launch_bounds__(256) void nb_box_pass_kernel(const cd *__restrict__ X, cd *__restrict__ Y,
                                                           const cd *__restrict__ E, int M, int K, int kc, int64_t inner,
                                                           int64_t ncols, int64_t pitch, const int32_t *__restrict__ kinv,
                                                           cd *__restrict__ out, int64_t nk_stride, int nn)
 {
-    box_pass_body<MM>(X, Y, E, M, K, kc, inner, ncols, pitch, kinv, out, nk_stride, nn, static_cast<int>(blockIdx.z));
+    box_pass_body<MM, SCATTER>(X, Y, E, M, K, kc, inner, ncols, pitch, kinv, out, nk_stride, nn,
+                               static_cast<int>(blockIdx.z));
 }
 
-// Up to four evaluations of the separable sums at once (the fused RK4 step's
-// evaluations of one step): B on the box, then the passes along a3 and a2,
-// block z = (evaluation, k chunk); each evaluation has its own scratch and output.
+// Up to four evaluations of the separable sums at once (the fused RK4 steps'
+// evaluations of one step): B on the box, then the passes along a3, a2 and
+// (with out set) a1, block z = (evaluation, k chunk); each evaluation has its
+// own scratch and output.
 struct BoxMulti {
     NbEval ev[4];
     int nev;
     cd *b[4];   // B on the box [cells][bcols]
     cd *t1[4];  // first pass [M1][M2][K3][bcols]
     cd *t2[4];  // second pass [M1][K2][K3][bcols]
+    cd *out[4];  // matrices [nmat][nk_stride][N][N] (the pass along a1)
 };
 
 __global__ void nb_box_bform_multi_kernel(const NbArgs a, const BoxMulti bm, int64_t ncells)
@@ -301,22 +304,34 @@ __global__ void nb_box_bform_multi_kernel(const NbArgs a, const BoxMulti bm, int
     bm.b[e][cell * a.bcols + c] = v;
 }
 
-// pass 3 (second == false: B -> T1 along a3) or pass 2 (T1 -> T2 along a2)
-template <int MM>
-__global__ __launch_bounds__(256) void nb_box_pass_multi_kernel(const NbArgs a, const BoxMulti bm, bool second, int kc,
-                                                                int nz)
+// pass 0: B -> T1 along a3; pass 1: T1 -> T2 along a2; pass 2: T2 -> the matrices
+// along a1 (scattered to the k points)
+template <int MM, int PASS>
+__global__ __launch_bounds__(256) void nb_box_pass_multi_kernel(const NbArgs a, const BoxMulti bm, int kc, int nz)
 {
     const int e = static_cast<int>(blockIdx.z) / nz, kz = static_cast<int>(blockIdx.z) - e * nz;
     const int nn = a.N * a.N;
     const int64_t ncols = static_cast<int64_t>(bm.ev[e].nmat) * nn;
-    const int M1 = a.boxM[0], M2 = a.boxM[1], M3 = a.boxM[2], K2 = a.boxK[1], K3 = a.boxK[2];
-    if (!second)
-        box_pass_body<MM>(bm.b[e], bm.t1[e], a.boxE[2], M3, K3, kc, 1, ncols, a.bcols, nullptr, nullptr, a.nk_stride,
+    const int M1 = a.boxM[0], M2 = a.boxM[1], M3 = a.boxM[2], K1 = a.boxK[0], K2 = a.boxK[1], K3 = a.boxK[2];
+    if (PASS == 0)
+        box_pass_body<MM, false>(bm.b[e], bm.t1[e], a.boxE[2], M3, K3, kc, 1, ncols, a.bcols, nullptr, nullptr, a.nk_stride,
                           nn, kz);
     else
-        box_pass_body<MM>(bm.t1[e], bm.t2[e], a.boxE[1], M2, K2, kc, K3, ncols, a.bcols, nullptr, nullptr, a.nk_stride,
+        box_pass_body<MM, false>(bm.t1[e], bm.t2[e], a.boxE[1], M2, K2, kc, K3, ncols, a.bcols, nullptr, nullptr, a.nk_stride,
                           nn, kz);
     (void)M1;
+    (void)K1;
+}
+
+// pass along a1: T2 -> the matrices, scattered to the k points
+template <int MM>
+__global__ __launch_bounds__(256) void nb_box_a1_multi_kernel(const NbArgs a, const BoxMulti bm, int kc, int nz)
+{
+    const int e = static_cast<int>(blockIdx.z) / nz, kz = static_cast<int>(blockIdx.z) - e * nz;
+    const int nn = a.N * a.N;
+    const int64_t ncols = static_cast<int64_t>(bm.ev[e].nmat) * nn;
+    box_pass_body<MM, true>(bm.t2[e], nullptr, a.boxE[0], a.boxM[0], a.boxK[0], kc, static_cast<int64_t>(a.boxK[1]) * a.boxK[2],
+                      ncols, a.bcols, a.box_kinv, bm.out[e], a.nk_stride, nn, kz);
 }
 
 // the padding k points [nk, nk_stride) of every matrix: zero, as the dense sums give them
@@ -341,7 +356,10 @@ static void box_pass(const cd *X, cd *Y, const cd *E, int M, int K, int64_t oute
     const int kc = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(K, K * xy / 2048)));
     const dim3 grid(static_cast<unsigned>(outer * inner), static_cast<unsigned>((ncols + 255) / 256),
                     static_cast<unsigned>((K + kc - 1) / kc));
-    nb_box_pass_kernel<MM><<<grid, 256, 0, s>>>(X, Y, E, M, K, kc, inner, ncols, pitch, kinv, out, nk_stride, nn);
+    if (out)
+        nb_box_pass_kernel<MM, true><<<grid, 256, 0, s>>>(X, Y, E, M, K, kc, inner, ncols, pitch, kinv, out, nk_stride, nn);
+    else
+        nb_box_pass_kernel<MM, false><<<grid, 256, 0, s>>>(X, Y, E, M, K, kc, inner, ncols, pitch, kinv, out, nk_stride, nn);
 }
 
 static void box_pass_m(const cd *X, cd *Y, const cd *E, int M, int K, int64_t outer, int64_t inner, int64_t ncols,
@@ -1399,31 +1417,35 @@ hipError_t nb_launch_box_t2(const NbArgs &a, const NbEval &ev, double2 *t2, hipS
 }
 
 template <int MM>
-static void box_pass_multi(const NbArgs &a, const BoxMulti &bm, bool second, int64_t maxcols, hipStream_t s)
+static void box_pass_multi(const NbArgs &a, const BoxMulti &bm, int pass, int64_t maxcols, hipStream_t s)
 {
-    const int M1 = a.boxM[0], M2 = a.boxM[1], K2 = a.boxK[1], K3 = a.boxK[2];
-    const int K = second ? K2 : K3;
-    const int64_t outer = second ? M1 : static_cast<int64_t>(M1) * M2, inner = second ? K3 : 1;
+    const int M1 = a.boxM[0], M2 = a.boxM[1], K1 = a.boxK[0], K2 = a.boxK[1], K3 = a.boxK[2];
+    const int K = pass == 0 ? K3 : pass == 1 ? K2 : K1;
+    const int64_t outer = pass == 0 ? static_cast<int64_t>(M1) * M2 : pass == 1 ? M1 : 1;
+    const int64_t inner = pass == 0 ? 1 : pass == 1 ? K3 : static_cast<int64_t>(K2) * K3;
     const int64_t xy = outer * inner * ((maxcols + 255) / 256) * bm.nev;
     const int kc = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(K, K * xy / 2048)));
     const int nz = (K + kc - 1) / kc;
     const dim3 grid(static_cast<unsigned>(outer * inner), static_cast<unsigned>((maxcols + 255) / 256),
                     static_cast<unsigned>(nz * bm.nev));
-    nb_box_pass_multi_kernel<MM><<<grid, 256, 0, s>>>(a, bm, second, kc, nz);
+    if (pass == 0) nb_box_pass_multi_kernel<MM, 0><<<grid, 256, 0, s>>>(a, bm, kc, nz);
+    else if (pass == 1) nb_box_pass_multi_kernel<MM, 1><<<grid, 256, 0, s>>>(a, bm, kc, nz);
+    else nb_box_a1_multi_kernel<MM><<<grid, 256, 0, s>>>(a, bm, kc, nz);
 }
 
-template <bool SECOND>
-static void box_pass_multi_m(const NbArgs &a, const BoxMulti &bm, int64_t maxcols, hipStream_t s)
+static void box_pass_multi_m(const NbArgs &a, const BoxMulti &bm, int pass, int64_t maxcols, hipStream_t s)
 {
-    const int M = SECOND ? a.boxM[1] : a.boxM[2];
-    if (M <= 4) box_pass_multi<4>(a, bm, SECOND, maxcols, s);
-    else if (M <= 8) box_pass_multi<8>(a, bm, SECOND, maxcols, s);
-    else if (M <= 16) box_pass_multi<16>(a, bm, SECOND, maxcols, s);
-    else box_pass_multi<32>(a, bm, SECOND, maxcols, s);
+    const int M = a.boxM[2 - pass];
+    if (M <= 4) box_pass_multi<4>(a, bm, pass, maxcols, s);
+    else if (M <= 8) box_pass_multi<8>(a, bm, pass, maxcols, s);
+    else if (M <= 16) box_pass_multi<16>(a, bm, pass, maxcols, s);
+    else box_pass_multi<32>(a, bm, pass, maxcols, s);
 }
 
-hipError_t nb_launch_box_t2_multi(const NbArgs &a, int nev, const NbEval *evs, double2 *const *b, double2 *const *t1,
-                                  double2 *const *t2, hipStream_t s)
+// B and the passes of up to four evaluations; out == nullptr: the passes along a3
+// and a2 into t2, otherwise the one along a1 too, into out[e]
+static hipError_t box_multi(const NbArgs &a, int nev, const NbEval *evs, double2 *const *b, double2 *const *t1,
+                            double2 *const *t2, double2 *const *out, hipStream_t s)
 {
     if (!a.box || a.split16 || a.dzero || nev < 1 || nev > 4) return hipErrorInvalidValue;
     BoxMulti bm{};
@@ -1434,14 +1456,31 @@ hipError_t nb_launch_box_t2_multi(const NbArgs &a, int nev, const NbEval *evs, d
         bm.b[e] = b[e];
         bm.t1[e] = t1[e];
         bm.t2[e] = t2[e];
+        bm.out[e] = out ? out[e] : nullptr;
         maxcols = std::max<int64_t>(maxcols, static_cast<int64_t>(evs[e].nmat) * a.N * a.N);
     }
     const int64_t ncells = static_cast<int64_t>(a.boxM[0]) * a.boxM[1] * a.boxM[2];
     nb_box_bform_multi_kernel<<<dim3(static_cast<unsigned>((ncells * maxcols + 255) / 256), 1, static_cast<unsigned>(nev)),
                                 256, 0, s>>>(a, bm, ncells);
-    box_pass_multi_m<false>(a, bm, maxcols, s);
-    box_pass_multi_m<true>(a, bm, maxcols, s);
+    box_pass_multi_m(a, bm, 0, maxcols, s);
+    box_pass_multi_m(a, bm, 1, maxcols, s);
+    if (out) box_pass_multi_m(a, bm, 2, maxcols, s);
     return hipGetLastError();
+}
+
+hipError_t nb_launch_box_t2_multi(const NbArgs &a, int nev, const NbEval *evs, double2 *const *b, double2 *const *t1,
+                                  double2 *const *t2, hipStream_t s)
+{
+    return box_multi(a, nev, evs, b, t1, t2, nullptr, s);
+}
+
+hipError_t nb_launch_box_mats_multi(const NbArgs &a, int nev, const NbEval *evs, double2 *const *b, double2 *const *t1,
+                                    double2 *const *t2, double2 *const *out, hipStream_t s)
+{
+    if (!out) return hipErrorInvalidValue;
+    for (int e = 0; e < nev && e < 4; ++e)
+        if (!out[e]) return hipErrorInvalidValue;
+    return box_multi(a, nev, evs, b, t1, t2, out, s);
 }
 
 hipError_t nb_launch_init(const NbArgs &a, const double2 *H0, double2 *rho, double2 *U, hipStream_t s)

@@ -105,6 +105,27 @@ hipError_t nb_launch_box_t2(const NbArgs &a, const NbEval &ev, double2 *t2, hipS
 // [cells][bcols], t1 [M1][M2][K3][bcols] and output t2
 hipError_t nb_launch_box_t2_multi(const NbArgs &a, int nev, const NbEval *evs, double2 *const *b, double2 *const *t1,
                                   double2 *const *t2, hipStream_t s);
+// The same with the pass along a1 as well, each evaluation's matrices into out[e]
+// ([nmat][nk_stride][N][N]; the padding k points are not written)
+hipError_t nb_launch_box_mats_multi(const NbArgs &a, int nev, const NbEval *evs, double2 *const *b, double2 *const *t1,
+                                    double2 *const *t2, double2 *const *out, hipStream_t s);
+
+// One whole RK4 step of the LengthWannier gauge for 16 < N <= 32 (nb32_step_kernel):
+// four waves per k point, each one 16x16 quadrant of every matrix; the matrices of the
+// step's times from the Fourier-sum outputs ([nk_stride][N][N] each, j: [3][nk_stride][N][N]).
+constexpr int kNb32MaxBlocks = 256;  // blocks of the N <= 32 step: one per CU, k points walked in turn
+struct NbStep32 {
+    const double2 *ht_prev, *h_prev;  // H~, H at t_n (h_prev read when u_valid or lam_valid is 0)
+    const double2 *j;                 // J_x, J_y, J_z at t_n
+    const double2 *ht_half, *h_half;  // at t_n + dt/2
+    const double2 *ht_end, *h_end;    // at t_n + dt
+    double *lam;                      // [nk_stride][32] band energies at the stored eigenvectors' time
+    int u_valid, lam_valid;
+    double dt;
+    int64_t step_local;
+};
+hipError_t nb32_launch_step(const NbArgs &a, const NbStep32 &f, int nblocks, double2 *rho, double2 *U, double *partials,
+                            hipStream_t s);
 
 // Time-fused VelocityHamiltonian steps for N <= 16 with the classical RK4 tableau
 // (nb16_vel_fused_kernel): one wave per k point for steps [s0, s1), one partial row

@@ -178,6 +178,17 @@ bool vel_fused_eligible(const sbe_config &cfg, const sbe_model &m, int64_t nk)
     return !(tuning_lookup(cfg, m, nk, tu) && tu.nb_kernels == 2);
 }
 
+// The fused RK4 step for 16 < N <= 32 (nb32_step_kernel): FP64 LengthWannier with the
+// classical RK4 tableau and a Wannier model with dipoles (dense or separable sums);
+// ANTELOPE_NB_FUSED=0 keeps the separate kernels.
+bool nb32_eligible(const sbe_config &cfg, const sbe_model &m)
+{
+    if (const char *e = std::getenv("ANTELOPE_NB_FUSED"))
+        if (std::atoi(e) == 0) return false;
+    return cfg.rk_order == SBE_RK4 && cfg.gauge == SBE_GAUGE_LENGTH_WANNIER && cfg.precision == SBE_PREC_FP64 &&
+           !am::is_analytic(m.kind) && !m.is_dipole_zero && m.nband > 16 && m.nband <= 32;
+}
+
 int fused_blocks(const BoxShape &bx, int &g1)
 {
     g1 = (bx.K[0] + kNbFusedWaves - 1) / kNbFusedWaves;
@@ -201,7 +212,7 @@ struct NbPlan {
 size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 
 NbPlan nb_plan(int N, int nr, int64_t nk, int64_t nt, int nstage, bool split16, const BoxShape &bx, bool fused,
-               bool vel_fused = false)
+               bool vel_fused = false, bool nb32 = false)
 {
     NbPlan p{};
     p.nk_stride = (nk + 63) / 64 * 64;
@@ -256,6 +267,14 @@ NbPlan nb_plan(int N, int nr, int64_t nk, int64_t nt, int nstage, bool split16, 
         const size_t b = align256(col * static_cast<size_t>(bx.cells()));
         const size_t t1 = align256(col * bx.M[0] * bx.M[1] * bx.K[2]);
         p.bytes_fused = 4 * t2 + align256(static_cast<size_t>(p.nk_stride) * 16 * sizeof(double)) + 4 * (b + t1);
+    }
+    if (nb32) {  // the energies, and with the separable sums per evaluation B and both passes
+        p.bytes_fused = align256(static_cast<size_t>(p.nk_stride) * 32 * sizeof(double));
+        if (bx.ok) {
+            const size_t col = sizeof(double2) * static_cast<size_t>(p.bcols);
+            p.bytes_fused += 4 * (align256(col * static_cast<size_t>(bx.cells())) +
+                                  align256(col * bx.M[0] * bx.M[1] * bx.K[2]) + align256(col * bx.M[0] * bx.K[1] * bx.K[2]));
+        }
     }
     return p;
 }
@@ -322,7 +341,9 @@ size_t nb_footprint(const sbe_config &cfg, const sbe_model &m, int64_t nk, int64
     const BoxShape bx = box_shape(cfg, m, nk, split16);
     const bool fused = fused_eligible(cfg, m, nk, bx);
     const bool vfused = vel_fused_eligible(cfg, m, nk);
-    const NbPlan p = nb_plan(m.nband, am::is_analytic(m.kind) ? 0 : m.nrpts, nk, nt, nstage, split16, bx, fused, vfused);
+    const bool f32 = nb32_eligible(cfg, m);
+    const NbPlan p =
+        nb_plan(m.nband, am::is_analytic(m.kind) ? 0 : m.nrpts, nk, nt, nstage, split16, bx, fused, vfused, f32);
     return p.bytes_P + p.bytes_rho + p.bytes_mats + p.bytes_k + p.bytes_tab + p.bytes_partials + p.bytes_series +
            p.bytes_split + p.bytes_b64 + p.bytes_box + p.bytes_fused + p.bytes_vel;
 }
@@ -395,7 +416,8 @@ int nb_setup(sbe_ctx *ctx)
     const BoxShape bx = box_shape(ctx->cfg, m, nk, split16);
     const bool fused = fused_eligible(ctx->cfg, m, nk, bx);
     const bool vfused = vel_fused_eligible(ctx->cfg, m, nk);
-    const NbPlan p = nb_plan(N, nr, nk, ctx->laser.nt, ctx->tab.nstage, split16, bx, fused, vfused);
+    const bool f32 = nb32_eligible(ctx->cfg, m);
+    const NbPlan p = nb_plan(N, nr, nk, ctx->laser.nt, ctx->tab.nstage, split16, bx, fused, vfused, f32);
     const size_t nn = static_cast<size_t>(N) * N;
     ctx->nk = nk;
     ctx->nk_stride = p.nk_stride;
@@ -546,6 +568,31 @@ int nb_setup(sbe_ctx *ctx)
     }
     ctx->nbf_prev_valid = false;
     ctx->nbf_lam_valid = false;
+    // fused RK4 steps for 16 < N <= 32: H~ H pairs in mats slots 0-1 and stage-derivative
+    // slots 0-1, 2-3 (J in mats slots 2-4); with the separable sums, per evaluation scratch
+    ctx->nb32 = false;
+    if (f32 && fbuf && ctx->tab.nstage == 4) {
+        const size_t slot = static_cast<size_t>(p.nk_stride) * nn;
+        ctx->nb32_pair[0] = ctx->nb_mats;
+        ctx->nb32_pair[1] = ctx->nb_k;
+        ctx->nb32_pair[2] = ctx->nb_k + 2 * slot;
+        ctx->nb32_lam = reinterpret_cast<double *>(fbuf);
+        for (int i = 0; i < 4; ++i) ctx->nb32_b[i] = ctx->nb32_t1[i] = ctx->nb32_t2[i] = nullptr;
+        if (a.box) {
+            const size_t col = sizeof(double2) * static_cast<size_t>(p.bcols);
+            const size_t b = align256(col * static_cast<size_t>(bx.cells()));
+            const size_t t1 = align256(col * bx.M[0] * bx.M[1] * bx.K[2]);
+            const size_t t2 = align256(col * bx.M[0] * bx.K[1] * bx.K[2]);
+            char *sc = fbuf + align256(static_cast<size_t>(p.nk_stride) * 32 * sizeof(double));
+            for (int i = 0; i < 4; ++i) {
+                ctx->nb32_b[i] = reinterpret_cast<double2 *>(sc + i * (b + t1 + t2));
+                ctx->nb32_t1[i] = reinterpret_cast<double2 *>(sc + i * (b + t1 + t2) + b);
+                ctx->nb32_t2[i] = reinterpret_cast<double2 *>(sc + i * (b + t1 + t2) + b + t1);
+            }
+        }
+        ctx->nb32_blocks = static_cast<int>(std::min<int64_t>(nk, kNb32MaxBlocks));
+        ctx->nb32 = true;
+    }
 
     a.velocity = ctx->velocity ? 1 : 0;
     // e^{ik.R}, then rho(k) from H(k) (A(atmin) = 0)
@@ -746,12 +793,97 @@ int nb_run_fused(sbe_ctx *ctx, int64_t step_begin, int64_t step_end)
     return rc;
 }
 
+// LengthWannier RK4 steps for 16 < N <= 32: the matrices of the step's times (one
+// batch of launches with the separable sums), then one nb32_step_kernel launch
+int nb_run_fused32(sbe_ctx *ctx, int64_t step_begin, int64_t step_end)
+{
+    const NbArgs &a = ctx->nargs;
+    const double dt = ctx->laser.dt;
+    const size_t slot = static_cast<size_t>(a.nk_stride) * a.N * a.N;
+    int rc = SBE_OK;
+    auto eval = [&](double t, bool hmats, bool h0) {
+        NbEval ev{};
+        ctx->laser.eval(t, ev.A, ev.E);
+        if (hmats) {
+            ev.nmat = h0 ? 2 : 1;
+            ev.mats[0] = kMatHtilde;
+            ev.mats[1] = kMatH0;
+        } else {
+            ev.nmat = 3;
+            ev.mats[0] = kMatJx;
+            ev.mats[1] = kMatJy;
+            ev.mats[2] = kMatJz;
+        }
+        return ev;
+    };
+    double2 *jm = ctx->nb_mats + 2 * slot;
+    for (int64_t c0 = step_begin; c0 < step_end && !rc; c0 += ctx->chunk_steps) {
+        const int64_t n = std::min<int64_t>(ctx->chunk_steps, step_end - c0);
+        for (int64_t s = 0; s < n && !rc; ++s) {
+            const double tn = ctx->laser.atmin + dt * static_cast<double>(c0 + s);  // antelope.cpp:299
+            NbEval evs[4];
+            double2 *outs[4];
+            int nev = 0;
+            if (!(ctx->nbf_prev_valid && same_time(ctx->nbf_prev_time, tn, dt))) {
+                evs[nev] = eval(tn, true, true);
+                outs[nev++] = ctx->nb32_pair[0];
+            }
+            evs[nev] = eval(tn, false, false);
+            outs[nev++] = jm;
+            evs[nev] = eval(tn + 0.5 * dt, true, a.deph != 0);  // H at the midpoint only for dephasing
+            outs[nev++] = ctx->nb32_pair[1];
+            evs[nev] = eval(tn + dt, true, true);
+            outs[nev++] = ctx->nb32_pair[2];
+            if (a.box) {
+                rc = hipc(ctx, nb_launch_box_mats_multi(a, nev, evs, ctx->nb32_b, ctx->nb32_t1, ctx->nb32_t2, outs,
+                                                        ctx->stream),
+                          "fourier passes");
+            } else {
+                for (int e = 0; e < nev && !rc; ++e)
+                    rc = hipc(ctx, eval_mats(a, evs[e], ctx->nb_P, outs[e], ctx->stream), "fourier kernel");
+            }
+            if (rc) break;
+            NbStep32 f{};
+            f.ht_prev = ctx->nb32_pair[0];
+            f.h_prev = ctx->nb32_pair[0] + slot;
+            f.j = jm;
+            f.ht_half = ctx->nb32_pair[1];
+            f.h_half = ctx->nb32_pair[1] + slot;
+            f.ht_end = ctx->nb32_pair[2];
+            f.h_end = ctx->nb32_pair[2] + slot;
+            f.lam = ctx->nb32_lam;
+            f.u_valid = same_time(ctx->nb_u_time, tn, dt) ? 1 : 0;
+            f.lam_valid = f.u_valid && ctx->nbf_lam_valid && same_time(ctx->nbf_lam_time, tn, dt) ? 1 : 0;
+            f.dt = dt;
+            f.step_local = s;
+            rc = hipc(ctx, nb32_launch_step(a, f, ctx->nb32_blocks, ctx->nb_rho, ctx->nb_U, ctx->d_partials, ctx->stream),
+                      "step kernel");
+            // the end of this step is the start of the next: its H~, H serve there
+            std::swap(ctx->nb32_pair[0], ctx->nb32_pair[2]);
+            ctx->nbf_prev_time = tn + dt;
+            ctx->nbf_prev_valid = true;
+            ctx->nb_u_time = tn + dt;
+            ctx->nbf_lam_time = tn + dt;
+            ctx->nbf_lam_valid = true;
+            ctx->nb_ht_valid = false;  // the separate kernels' matrices are not kept up to date
+            ctx->nb_h_valid = false;
+        }
+        if (!rc)
+            rc = hipc(ctx, launch_series_reduce(ctx->d_partials, n, ctx->nb32_blocks, a.nv, ctx->d_series + c0 * a.nv,
+                                                ctx->stream, ctx->nonfinite_d, kNbReduceGroup),
+                      "series kernel");
+        if (!rc) ctx->steps_done = c0 + n;
+    }
+    return rc;
+}
+
 }  // namespace
 
 int nb_run(sbe_ctx *ctx, int64_t step_begin, int64_t step_end)
 {
     if (ctx->velocity) return nb_run_velocity(ctx, step_begin, step_end);
     if (ctx->nbf) return nb_run_fused(ctx, step_begin, step_end);
+    if (ctx->nb32) return nb_run_fused32(ctx, step_begin, step_end);
     const NbArgs &a = ctx->nargs;
     const Tableau &tb = ctx->tab;
     const double dt = ctx->laser.dt;
@@ -903,6 +1035,7 @@ int nb_state(sbe_ctx *ctx, int basis, double *rho)
         double2 *out = ctx->nb_mats + static_cast<size_t>(a.nk_stride) * nn;
         ctx->nb_ht_valid = false;
         ctx->nb_h_valid = false;
+        if (ctx->nb32) ctx->nbf_prev_valid = false;  // an H~ H pair may live in slots 0-1
         int rc = hipc(ctx, eval_mats(a, ev, ctx->nb_P, ctx->nb_mats, ctx->stream), "fourier kernel");
         if (!rc) rc = hipc(ctx, nb_launch_band_export(a, ctx->nb_mats, ctx->nb_rho, out, ctx->stream), "band export kernel");
         if (!rc) rc = hipc(ctx, hipMemcpyAsync(rho, out, bytes, hipMemcpyDeviceToHost, ctx->stream), "copy state");

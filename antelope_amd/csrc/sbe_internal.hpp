@@ -187,6 +187,15 @@ struct sbe_ctx {
     double nbf_prev_time = 0;
     bool nbf_lam_valid = false;
     double nbf_lam_time = 0;
+    // fused RK4 steps for 16 < N <= 32 (nb32_step_kernel): the matrices of a step's
+    // times in mats and the stage-derivative slots, H~ H pairs rotating
+    bool nb32 = false;
+    double2 *nb32_pair[3] = {nullptr, nullptr, nullptr};  // H~ H at t_n, t_n + dt/2, t_n + dt
+    double *nb32_lam = nullptr;  // [nk_stride][32] band energies at the stored eigenvectors' time
+    double2 *nb32_b[4] = {nullptr, nullptr, nullptr, nullptr};   // separable sums: per evaluation B,
+    double2 *nb32_t1[4] = {nullptr, nullptr, nullptr, nullptr};  // first and second pass
+    double2 *nb32_t2[4] = {nullptr, nullptr, nullptr, nullptr};
+    int nb32_blocks = 0;
     // time-fused velocity-gauge steps (nb16_vel_fused_kernel)
     bool nbv = false;
     const double *nbv_A = nullptr;  // [nt][4][3] A at the RK4 stage times

@@ -165,6 +165,32 @@ def test_separable_sums_footprint_without_gpu():
     assert _footprint_env(small, m1, dense=False) == _footprint_env(small, m1, dense=True)
 
 
+def test_fused_step_n32_footprint_without_gpu():
+    """For 16 < N <= 32 the fused RK4 step keeps its matrices in the mats and stage slots
+    the separate kernels have, and adds the band energies [nk_stride][32] (FP64) and, with
+    the separable sums, per evaluation (up to four batched) B on the box and both pass
+    outputs; with the dense sums only the energies. RK5 and the SPLIT16 sums keep the separate
+    kernels."""
+    import dataclasses
+
+    from antelope_amd import Pulse, RunConfig
+    from antelope_amd.models import synthetic_wannier
+
+    N, M, K = 24, (5, 5, 1), (12, 12, 1)
+    m = synthetic_wannier(N, nmax=(2, 2, 0), seed=3)
+    cfg = RunConfig(npoints=list(K[:2]), pulses=[Pulse(E0=0.003, w0=0.014, ncycles=2.0, envelope="sin2")],
+                    dt=0.25, T2=220.0, rk_order=4, inter_intra=True)
+    al = lambda x: (x + 255) // 256 * 256
+    col = 16 * ((4 * N * N + 63) // 64 * 64)
+    nk_stride = (K[0] * K[1] + 63) // 64 * 64
+    lam = al(nk_stride * 32 * 8)
+    scratch = 4 * (al(M[0] * M[1] * M[2] * col) + al(M[0] * M[1] * K[2] * col) + al(M[0] * K[1] * K[2] * col))
+    assert _footprint_env(cfg, m, dense=False, fused=True) - _footprint_env(cfg, m, dense=False) == lam + scratch
+    assert _footprint_env(cfg, m, dense=True, fused=True) - _footprint_env(cfg, m, dense=True) == lam
+    for other in (dataclasses.replace(cfg, rk_order=5), dataclasses.replace(cfg, precision="split16")):
+        assert _footprint_env(other, m, dense=False, fused=True) == _footprint_env(other, m, dense=False)
+
+
 def test_separable_sums_eligibility_without_gpu():
     """The separable form needs theta_j = k.a_j to depend on grid index j alone. On a
     hexagonal lattice, calc.BZaxes given as Cartesian axes break that (the footprint is

@@ -122,3 +122,49 @@ def test_time_fused_velocity_gauge(oracle, monkeypatch, label, N, kw):
     two, _, _ = _solve(cfg, model, monkeypatch, fused=True, splits=(17, 18, 60))
     for k in fz:
         assert np.array_equal(fz[k], two[k]), k
+
+
+@pytest.mark.parametrize("label,N,npts,nmax,kw,dense", [
+    ("n24_box6x5x2", 24, (6, 5, 2), (1, 1, 1), dict(), False),
+    ("n32_grid8x8_no_dephasing_total_current", 32, (8, 8), (2, 2, 0), dict(T2=-1.0, inter_intra=False), False),
+    ("n20_dense_sums_t1t2", 20, (5, 4, 3), (1, 1, 1), dict(T1=150.0), True),
+    ("n17_300k_more_k_than_blocks", 17, (20, 15), (1, 1, 0), dict(), False),
+])
+def test_fused_step_n32_against_oracle_and_separate_kernels(oracle, monkeypatch, label, N, npts, nmax, kw, dense):
+    """The fused RK4 step for 16 < N <= 32 (nb32_step_kernel: four waves per k point, one
+    16 x 16 quadrant of every matrix each, after one batch of Fourier launches per step)
+    against the oracle and the separate stage and sample kernels."""
+    import dataclasses
+    model = synthetic_wannier(N, nmax=nmax, seed=50 + N)
+    T1 = kw.pop("T1", -1.0)
+    cfg = dataclasses.replace(_cfg(npts, **kw), T1=T1)
+    if dense:
+        monkeypatch.setenv("ANTELOPE_NB_DENSE", "1")
+    print(f"case: {label}", flush=True)
+    ref = oracle.run(cfg, model, nthreads=8, want_state=True)
+    fz, st_f, bytes_f = _solve(cfg, model, monkeypatch, fused=True)
+    sp, st_s, bytes_s = _solve(cfg, model, monkeypatch, fused=False)
+    assert bytes_f > bytes_s  # the band energies were allocated: it ran fused
+    e_or, e_sp = _errs(fz, ref), _errs(fz, sp)
+    print(label, "vs oracle", {k: f"{v:.2e}" for k, v in e_or.items()},
+          "vs separate kernels", {k: f"{v:.2e}" for k, v in e_sp.items()}, flush=True)
+    assert all(v < TOL64 for v in e_or.values()), e_or
+    assert all(v < 1e-12 for v in e_sp.values()), e_sp
+    assert rel_l2(st_f, ref["state"]) < TOL64
+    assert rel_l2(st_f, st_s) < 1e-12
+
+
+def test_fused_step_n32_split_runs_and_state_reads_are_bitwise(monkeypatch):
+    """Runs of the N <= 32 fused step split anywhere, with a band-basis state read in
+    between (it reuses the matrix slots the step keeps H~ and H in), give the bits of one run."""
+    model = synthetic_wannier(24, nmax=(1, 1, 1), seed=19)
+    cfg = _cfg((6, 5, 2), ncycles=0.5)
+    print("case: n24 split runs", flush=True)
+    one, st1, _ = _solve(cfg, model, monkeypatch, fused=True)
+    seen = []
+    two, st2, _ = _solve(cfg, model, monkeypatch, fused=True, splits=(37, 38, 90),
+                         peek=lambda s: seen.append(s.state("band")))
+    assert seen and np.all(np.isfinite(seen[0]))
+    for k in one:
+        assert np.array_equal(one[k], two[k]), k
+    assert np.array_equal(st1, st2)
