@@ -1474,6 +1474,185 @@ hipError_t nb_launch_box_t2_multi(const NbArgs &a, int nev, const NbEval *evs, d
     return box_multi(a, nev, evs, b, t1, t2, nullptr, s);
 }
 
+// Up to four evaluations of the dense sums in one launch, for R lists of at most MM
+// vectors: a thread owns one column c of one evaluation, forms its B column
+// w_R e^{iA.R} X_R[c] in registers (e^{iA.R} once per block in LDS) and sums
+// out[k][c] = sum_R e^{ik.R} B_R[c] over a chunk of kc k points (e^{ik.R} read
+// with uniform loads). block y: 256 columns, block z: (evaluation, k chunk).
+struct DenseMulti {
+    NbEval ev[4];
+    int nev;
+    cd *out[4];
+};
+
+// one evaluation's columns [blockIdx.y * 256, + 256) at k points [k0, k1) into out
+// ([nmat][nk_stride][N][N]); phase: LDS for MM values
+template <int MM>
+__device__ __forceinline__ void dense_eval_body(const NbArgs &a, const NbEval &ev, const cd *__restrict__ P,
+                                                cd *__restrict__ out_base, int64_t k0, int64_t k1, cd *phase)
+{
+    const int nr = a.nr, nn = a.N * a.N;
+    if (static_cast<int>(threadIdx.x) < nr) {
+        const double *R = a.rcart + 3 * threadIdx.x;
+        double sn, cs;
+        sincos(ev.A[0] * R[0] + ev.A[1] * R[1] + ev.A[2] * R[2], &sn, &cs);
+        phase[threadIdx.x] = cmk(cs, sn);
+    }
+    __syncthreads();
+    const int64_t ncols = static_cast<int64_t>(ev.nmat) * nn;
+    const int64_t c = static_cast<int64_t>(blockIdx.y) * 256 + threadIdx.x;
+    if (c >= ncols) return;  // no barriers below
+    cd x[MM];
+#pragma unroll
+    for (int m = 0; m < MM; ++m) x[m] = m < nr ? b_element(a, ev, m, c, phase[m]) : cmk(0.0, 0.0);
+    const int mat = static_cast<int>(c / nn), ij = static_cast<int>(c - static_cast<int64_t>(mat) * nn);
+    cd *out = out_base + static_cast<int64_t>(mat) * a.nk_stride * nn + ij;
+    for (int64_t k = k0; k < k1; ++k) {
+        const cd *pk = P + k * nr;
+        double re = 0.0, im = 0.0;
+#pragma unroll
+        for (int m = 0; m < MM; ++m) {
+            if (m < nr) {
+                const cd w = pk[m];
+                re = fma(w.x, x[m].x, fma(-w.y, x[m].y, re));
+                im = fma(w.x, x[m].y, fma(w.y, x[m].x, im));
+            }
+        }
+        out[k * nn] = cmk(re, im);
+    }
+}
+
+template <int MM>
+__global__ __launch_bounds__(256) void nb_dense_multi_kernel(const NbArgs a, const DenseMulti dm,
+                                                             const cd *__restrict__ P, int kc, int nz)
+{
+    __shared__ cd phase[MM];
+    const int e = static_cast<int>(blockIdx.z) / nz, kz = static_cast<int>(blockIdx.z) - e * nz;
+    const int64_t k0 = static_cast<int64_t>(kz) * kc;
+    dense_eval_body<MM>(a, dm.ev[e], P, dm.out[e], k0, min(a.nk, k0 + kc), phase);
+}
+
+// The matrices of a run of RK4 steps in one launch: block z = (step, evaluation): J at
+// t_n, H~ (and H with dephasing) at t_n + dt/2, H~ and H at t_n + dt, with A and E at
+// those times from the host-evaluated table; block y = 64 columns (one per lane, the B
+// column w_R e^{iA.R} X_R[c] formed in registers). The k points go in tiles of 64 whose
+// e^{ik.R} rows are staged in LDS and read as broadcasts; wave w sums the tile's k points
+// w, w + 4, ... two at a time (four independent accumulation chains).
+constexpr int kDenseKTile = 64;
+template <int MM>
+__global__ __launch_bounds__(256) void nb_dense_steps_kernel(const NbArgs a, const NbDenseSteps d,
+                                                             const cd *__restrict__ P)
+{
+    __shared__ cd phase[MM];
+    __shared__ cd pt[kDenseKTile * MM];
+    const int z = static_cast<int>(blockIdx.z), sl = z / 3, typ = z - 3 * sl;
+    const int64_t step = d.s0 + sl;
+    NbEval ev{};
+    const double *lt = d.laser + (step * 3 + typ) * 6;
+    for (int i = 0; i < 3; ++i) {
+        ev.A[i] = lt[i];
+        ev.E[i] = lt[3 + i];
+    }
+    cd *out;
+    if (typ == 0) {
+        ev.nmat = 3;
+        ev.mats[0] = kMatJx;
+        ev.mats[1] = kMatJy;
+        ev.mats[2] = kMatJz;
+        out = d.j + static_cast<int64_t>(sl) * 3 * d.slot;
+    } else {
+        ev.nmat = typ == 1 && !d.deph ? 1 : 2;
+        ev.mats[0] = kMatHtilde;
+        ev.mats[1] = kMatH0;
+        out = typ == 1 ? d.half + static_cast<int64_t>(sl) * 2 * d.slot : d.ring + (step % d.nring) * 2 * d.slot;
+    }
+    const int nr = a.nr, nn = a.N * a.N;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (static_cast<int>(threadIdx.x) < nr) {
+        const double *R = a.rcart + 3 * threadIdx.x;
+        double sn, cs;
+        sincos(ev.A[0] * R[0] + ev.A[1] * R[1] + ev.A[2] * R[2], &sn, &cs);
+        phase[threadIdx.x] = cmk(cs, sn);
+    }
+    __syncthreads();
+    const int64_t ncols = static_cast<int64_t>(ev.nmat) * nn;
+    const int64_t c = static_cast<int64_t>(blockIdx.y) * 64 + lane;
+    const bool live = c < ncols;
+    cd x[MM];
+#pragma unroll
+    for (int m = 0; m < MM; ++m) x[m] = live && m < nr ? b_element(a, ev, m, c, phase[m]) : cmk(0.0, 0.0);
+    const int mat = live ? static_cast<int>(c / nn) : 0;
+    const int ij = live ? static_cast<int>(c - static_cast<int64_t>(mat) * nn) : 0;
+    cd *o = out + static_cast<int64_t>(mat) * a.nk_stride * nn + ij;
+    for (int64_t k0 = 0; k0 < a.nk; k0 += kDenseKTile) {
+        const int kt = static_cast<int>(min<int64_t>(kDenseKTile, a.nk - k0));
+        __syncthreads();  // the previous tile's reads are done
+        for (int e = threadIdx.x; e < kt * nr; e += 256) pt[(e / nr) * MM + e % nr] = P[k0 * nr + e];
+        __syncthreads();
+        for (int kk = w; kk < kt; kk += 8) {
+            const int kk2 = kk + 4 < kt ? kk + 4 : kk;  // a second k point (the same one at a ragged end)
+            const cd *p1 = pt + kk * MM, *p2 = pt + kk2 * MM;
+            double r1a = 0.0, i1a = 0.0, r2a = 0.0, i2a = 0.0;
+#pragma unroll
+            for (int m = 0; m < MM; ++m) {
+                if (m < nr) {
+                    const cd u = p1[m], v = p2[m];
+                    r1a = fma(u.x, x[m].x, fma(-u.y, x[m].y, r1a));
+                    i1a = fma(u.x, x[m].y, fma(u.y, x[m].x, i1a));
+                    r2a = fma(v.x, x[m].x, fma(-v.y, x[m].y, r2a));
+                    i2a = fma(v.x, x[m].y, fma(v.y, x[m].x, i2a));
+                }
+            }
+            if (live) {
+                o[(k0 + kk) * nn] = cmk(r1a, i1a);
+                if (kk2 != kk) o[(k0 + kk2) * nn] = cmk(r2a, i2a);
+            }
+        }
+    }
+}
+
+hipError_t nb_launch_dense_steps(const NbArgs &a, const NbDenseSteps &d, const double2 *P, hipStream_t s)
+{
+    if (a.model_kind || a.split16 || a.dzero || a.nr < 1 || a.nr > kNbDenseMultiMaxR || !P || d.ns < 1 ||
+        d.nring < 1 || d.s0 < 0)
+        return hipErrorInvalidValue;
+    const dim3 grid(1, static_cast<unsigned>((3 * a.N * a.N + 63) / 64), static_cast<unsigned>(3 * d.ns));
+    if (a.nr <= 16) nb_dense_steps_kernel<16><<<grid, 256, 0, s>>>(a, d, P);
+    else nb_dense_steps_kernel<32><<<grid, 256, 0, s>>>(a, d, P);
+    return hipGetLastError();
+}
+
+template <int MM>
+static void dense_multi(const NbArgs &a, const DenseMulti &dm, const cd *P, int64_t maxcols, hipStream_t s)
+{
+    // k chunks: about two blocks per CU in all (each block forms its columns of B once)
+    const int64_t ny = (maxcols + 255) / 256;
+    const int64_t nz = std::max<int64_t>(1, std::min<int64_t>(a.nk, 512 / (ny * dm.nev)));
+    const int kc = static_cast<int>((a.nk + nz - 1) / nz);
+    const int nzc = static_cast<int>((a.nk + kc - 1) / kc);
+    const dim3 grid(1, static_cast<unsigned>(ny), static_cast<unsigned>(nzc * dm.nev));
+    nb_dense_multi_kernel<MM><<<grid, 256, 0, s>>>(a, dm, P, kc, nzc);
+}
+
+hipError_t nb_launch_dense_multi(const NbArgs &a, int nev, const NbEval *evs, const double2 *P, double2 *const *out,
+                                 hipStream_t s)
+{
+    if (a.model_kind || a.split16 || a.dzero || a.nr < 1 || a.nr > kNbDenseMultiMaxR || !P || nev < 1 || nev > 4)
+        return hipErrorInvalidValue;
+    DenseMulti dm{};
+    dm.nev = nev;
+    int64_t maxcols = 0;
+    for (int e = 0; e < nev; ++e) {
+        if (!out[e]) return hipErrorInvalidValue;
+        dm.ev[e] = evs[e];
+        dm.out[e] = out[e];
+        maxcols = std::max<int64_t>(maxcols, static_cast<int64_t>(evs[e].nmat) * a.N * a.N);
+    }
+    if (a.nr <= 16) dense_multi<16>(a, dm, P, maxcols, s);
+    else dense_multi<32>(a, dm, P, maxcols, s);
+    return hipGetLastError();
+}
+
 hipError_t nb_launch_box_mats_multi(const NbArgs &a, int nev, const NbEval *evs, double2 *const *b, double2 *const *t1,
                                     double2 *const *t2, double2 *const *out, hipStream_t s)
 {

@@ -110,6 +110,27 @@ hipError_t nb_launch_box_t2_multi(const NbArgs &a, int nev, const NbEval *evs, d
 hipError_t nb_launch_box_mats_multi(const NbArgs &a, int nev, const NbEval *evs, double2 *const *b, double2 *const *t1,
                                     double2 *const *t2, double2 *const *out, hipStream_t s);
 
+// Up to four evaluations of the dense sums in one launch (Wannier models with at most
+// kNbDenseMultiMaxR lattice vectors; P = e^{ik.R} [nk_stride][nr]), each into out[e]
+// ([nmat][nk_stride][N][N]; the padding k points are not written)
+constexpr int kNbDenseMultiMaxR = 32;
+hipError_t nb_launch_dense_multi(const NbArgs &a, int nev, const NbEval *evs, const double2 *P, double2 *const *out,
+                                 hipStream_t s);
+
+// The matrices of steps [s0, s0 + ns) of the RK4 LengthWannier gauge in one launch
+// (the dense sums, the same R lists): J at t_n into j [ns][3][slot], H~ (and H with
+// dephasing) at t_n + dt/2 into half [ns][2][slot], H~ and H at t_n + dt into the ring
+// [nring][2][slot] at step mod nring (the pair of the step before s0 is kept there when
+// nring > ns). laser: [nt][3][6], A and E at t_n, t_n + dt/2, t_n + dt.
+struct NbDenseSteps {
+    const double *laser;
+    int64_t s0;
+    int ns, nring, deph;
+    int64_t slot;  // nk_stride N^2
+    double2 *j, *half, *ring;
+};
+hipError_t nb_launch_dense_steps(const NbArgs &a, const NbDenseSteps &d, const double2 *P, hipStream_t s);
+
 // One whole RK4 step of the LengthWannier gauge for 16 < N <= 32 (nb32_step_kernel):
 // four waves per k point, each one 16x16 quadrant of every matrix; the matrices of the
 // step's times from the Fourier-sum outputs ([nk_stride][N][N] each, j: [3][nk_stride][N][N]).

@@ -331,10 +331,36 @@ __device__ __forceinline__ void q_axpy(Tile &Y, const Tile &X, double s, const T
     }
 }
 
+constexpr int kQStage = 1024;  // complex per staged matrix (N <= 32)
+
+// the k point's matrices src0 and (if given) src1 into staging buffer b, asynchronously:
+// chunks of 64 complex dealt over the four waves, one 16-byte global-to-LDS load per
+// lane; their completion is counted by vmcnt (stage_wait)
+__device__ __forceinline__ void stage_async(cd *stg, int b, const cd *src0, const cd *src1, int nn, int w, int lane)
+{
+    const int Q = (nn + 63) >> 6;
+    const int total = src1 ? 2 * Q : Q;
+    for (int ch = w; ch < total; ch += 4) {  // uniform over the wave
+        const int m = ch >= Q ? 1 : 0, q = ch - m * Q;
+        const cd *src = m ? src1 : src0;
+        const int ij = q * 64 + lane;
+        if (ij < nn) __builtin_amdgcn_global_load_lds(src + ij, stg + (2 * b + m) * kQStage + q * 64, 16, 0, 0);
+    }
+}
+
+// every wave's staging (and other) loads have landed and every wave is past its earlier reads
+__device__ __forceinline__ void stage_wait()
+{
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+}
+
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1))) void nb32_step_kernel(
     const NbArgs a, const NbStep32 f, cd *__restrict__ rho, cd *__restrict__ Ust, double *__restrict__ partials)
 {
     __shared__ cd slots[4 * kQSlot];
+    __shared__ cd stg[4 * kQStage];     // staging: buffers A (0, 1) and B (2, 3)
+    __shared__ double rsum[2 * kQStage];  // the block's sum_k w rho (each element owned by one lane)
     __shared__ double scal[2 * 128];
     const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
     Q L{slots, scal, w >> 1, w & 1, 0, a.N};
@@ -342,27 +368,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1))) void n
     const int c = lane_c(), g = lane_g(), lane = threadIdx.x & 63;
     const int64_t slot = a.nk_stride * nn;
     const double dt = f.dt;
-    // the block's partial row: sum_k w rho accumulates in place (each element owned by one lane)
-    double *row = partials + (f.step_local * gridDim.x + blockIdx.x) * static_cast<int64_t>(a.nv);
-    double *rbar = row + 6 + N;
+    const bool need_h = !f.u_valid || !f.lam_valid;
+    const cd *A0 = stg, *A1 = stg + kQStage, *B0 = stg + 2 * kQStage, *B1 = stg + 3 * kQStage;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-        if (q_valid(L, r)) {
-            const int e = (16 * L.p + g + 4 * r) * N + 16 * L.q + c;
-            rbar[2 * e] = 0.0;
-            rbar[2 * e + 1] = 0.0;
-        }
+        const int e = (16 * L.p + g + 4 * r) * N + 16 * L.q + c;
+        if (q_valid(L, r)) rsum[2 * e] = rsum[2 * e + 1] = 0.0;
     }
     double curl = 0.0;  // wave 0, lane a < 6: current component a (inter x, y, z, intra x, y, z)
     double pop = 0.0;   // wave 0, lane m < N: population of the m-th lowest band
+    // staging: A = [J_x, H(t_n) if needed], B = [J_y, J_z], A = [H~(t_n), H(t + dt/2) if
+    // dephasing], B = [H~(t + dt/2), H(t + dt)], A = [H~(t + dt)]; each buffer refilled
+    // after the wait that follows its last reads, while the other one is in use
     for (int64_t k = blockIdx.x; k < a.nk; k += gridDim.x) {
+        const int64_t kn = k * nn;
         const double wk = a.weight[k];
-        Tile R = q_load(L, rho + k * nn, false);
-        Tile U = q_load(L, Ust + k * nn, true);
+        stage_async(stg, 0, f.j + kn, need_h ? f.h_prev + kn : nullptr, nn, w, lane);
+        Tile R = q_load(L, rho + kn, false);
+        Tile U = q_load(L, Ust + kn, true);
+        stage_wait();
+        stage_async(stg, 1, f.j + slot + kn, f.j + 2 * slot + kn, nn, w, lane);
         double lam;  // energy of column 16q + c
-        if (!f.u_valid || !f.lam_valid) {
-            if (!f.u_valid) q_refine(L, f.h_prev + k * nn, U, 1e-8, true, lam);
-            const Tile HU = q_prod(L, q_load(L, f.h_prev + k * nn, false), false, U, false);
+        if (need_h) {
+            if (!f.u_valid) q_refine(L, A1, U, 1e-8, true, lam);
+            const Tile HU = q_prod(L, q_load(L, A1, false), false, U, false);
             lam = q_col_dot(L, U, HU);
         } else {
             lam = f.lam[k * 32 + 16 * L.q + c];
@@ -391,19 +420,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1))) void n
             __syncthreads();
             if (w == 0 && lane < N) pop += S[64 + lane] * wk;
             L.next();
-            __syncthreads();  // the next exchange reuses this area two phases on
         }
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
+            const int e = (16 * L.p + g + 4 * r) * N + 16 * L.q + c;
             if (q_valid(L, r)) {
-                const int e = (16 * L.p + g + 4 * r) * N + 16 * L.q + c;
-                rbar[2 * e] = fma(R.re[r], wk, rbar[2 * e]);
-                rbar[2 * e + 1] = fma(R.im[r], wk, rbar[2 * e + 1]);
+                rsum[2 * e] = fma(R.re[r], wk, rsum[2 * e]);
+                rsum[2 * e + 1] = fma(R.im[r], wk, rsum[2 * e + 1]);
             }
         }
         // currents: Re Tr(rho J) = Re sum conj(rho_ij) J_ij, Re Tr(M J) likewise (rho, M Hermitian)
-        for (int ax = 0; ax < 3; ++ax) {
-            const Tile J = q_load(L, f.j + ax * slot + k * nn, false);
+        auto current = [&](int ax, const cd *Js) {
+            const Tile J = q_load(L, Js, false);
             double tot = 0.0, intra = 0.0;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
@@ -414,22 +442,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1))) void n
             intra = a.inter_intra ? q_sum(L, wave_sum(intra)) : 0.0;
             if (lane == ax) curl += (tot - intra) * wk;
             if (lane == 3 + ax) curl += intra * wk;
-        }
+        };
+        current(0, A0);
+        stage_wait();  // B = [J_y, J_z]; A free
+        stage_async(stg, 0, f.ht_prev + kn, a.deph ? f.h_half + kn : nullptr, nn, w, lane);
+        current(1, B0);
+        current(2, B1);
+        stage_wait();  // A = [H~(t_n), H(t + dt/2)]; B free
+        stage_async(stg, 1, f.ht_half + kn, f.h_end + kn, nn, w, lane);
         // stage 1 at t_n
-        Tile X = q_stage(L, a, R, q_load(L, f.ht_prev + k * nn, false), M);
+        Tile X = q_stage(L, a, R, q_load(L, A0, false), M);
         Tile Acc, Y;
         q_axpy(Acc, X, dt / 6.0, R);
         q_axpy(Y, X, 0.5 * dt, R);
         // stages 2 and 3 at t_n + dt/2
         if (a.deph) {
             double dd;
-            q_refine(L, f.h_half + k * nn, U, 1e-6, false, dd);
+            q_refine(L, A1, U, 1e-6, false, dd);
         }
+        stage_wait();  // B = [H~(t + dt/2), H(t + dt)]; A free
+        stage_async(stg, 0, f.ht_end + kn, nullptr, nn, w, lane);
         for (int st = 1; st <= 2; ++st) {
             Tile Mh{};
             double dd;
             if (a.deph) Mh = q_band_diag(L, Y, U, dd);
-            X = q_stage(L, a, Y, q_load(L, f.ht_half + k * nn, false), Mh);
+            X = q_stage(L, a, Y, q_load(L, B0, false), Mh);
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 Acc.re[r] = fma(dt / 3.0, X.re[r], Acc.re[r]);
@@ -438,22 +475,34 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1))) void n
             q_axpy(Y, X, st == 1 ? 0.5 * dt : dt, R);
         }
         // stage 4 at t_n + dt; eigenvectors and energies for the next sample
-        q_refine(L, f.h_end + k * nn, U, 1e-6, true, lam);
+        q_refine(L, B1, U, 1e-6, true, lam);
+        stage_wait();  // A = [H~(t + dt)]
         {
             Tile Me{};
             double dd;
             if (a.deph) Me = q_band_diag(L, Y, U, dd);
-            X = q_stage(L, a, Y, q_load(L, f.ht_end + k * nn, false), Me);
+            X = q_stage(L, a, Y, q_load(L, A0, false), Me);
         }
         q_axpy(R, X, dt / 6.0, Acc);
-        q_store(L, rho + k * nn, R);
-        q_store(L, Ust + k * nn, U);
+        q_store(L, rho + kn, R);
+        q_store(L, Ust + kn, U);
         if (g == 0 && L.p == 0) f.lam[k * 32 + 16 * L.q + c] = lam;
+        // the next k point's first staging writes A and B: every wave is past its reads of
+        // both here (the last exchange of stage 4 has a barrier after them)
     }
-    // currents and populations from wave 0 (the rho sums are in place)
+    // the block's partial row
+    double *row = partials + (f.step_local * gridDim.x + blockIdx.x) * static_cast<int64_t>(a.nv);
     if (w == 0) {
         if (lane < 6) row[lane] = curl;
         if (lane < N) row[6 + lane] = pop;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int e = (16 * L.p + g + 4 * r) * N + 16 * L.q + c;
+        if (q_valid(L, r)) {
+            row[6 + N + 2 * e] = rsum[2 * e];
+            row[6 + N + 2 * e + 1] = rsum[2 * e + 1];
+        }
     }
 }
 

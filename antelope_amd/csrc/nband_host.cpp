@@ -189,6 +189,24 @@ bool nb32_eligible(const sbe_config &cfg, const sbe_model &m)
            !am::is_analytic(m.kind) && !m.is_dipole_zero && m.nband > 16 && m.nband <= 32;
 }
 
+// The N <= 32 fused step's matrices in one dense launch for all evaluations of a step
+// (nb_launch_dense_multi) when the R list is short and the k points few: there the
+// separable passes cost more in launches than they save in arithmetic.
+bool nb32_dense_multi_n(int nr, int64_t nk) { return nr >= 1 && nr <= kNbDenseMultiMaxR && nk <= 1024; }
+
+// steps whose matrices one dense launch evaluates: about 256 MB of matrices (seven
+// N x N per k point and step), at least one, at most the pulse; ANTELOPE_NB_EVAL_STEPS
+// sets it (the results do not depend on it)
+constexpr size_t kNb32StepBytes = size_t(256) << 20;
+int nb32_steps_per_eval(int64_t nk_stride, int N, int64_t nt)
+{
+    const size_t per = 7 * static_cast<size_t>(nk_stride) * N * N * sizeof(double2);
+    int64_t S = static_cast<int64_t>(kNb32StepBytes / per);
+    if (const char *e = std::getenv("ANTELOPE_NB_EVAL_STEPS"))
+        if (std::atoi(e) > 0) S = std::atoi(e);
+    return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(nt, S)));
+}
+
 int fused_blocks(const BoxShape &bx, int &g1)
 {
     g1 = (bx.K[0] + kNbFusedWaves - 1) / kNbFusedWaves;
@@ -225,8 +243,10 @@ NbPlan nb_plan(int N, int nr, int64_t nk, int64_t nt, int nstage, bool split16, 
     p.bytes_vel = vel_fused ? align256(static_cast<size_t>(nt) * 12 * sizeof(double)) : 0;
     const size_t row = static_cast<size_t>(rows) * p.nv * sizeof(double);
     p.chunk = std::max<int64_t>(1, std::min<int64_t>(nt, static_cast<int64_t>(kNbPartialBytes / row)));
-    // e^{ik.R} for the dense sums; the separable form does not read it
-    p.bytes_P = bx.ok ? 0 : static_cast<size_t>(p.nk_stride) * nr * sizeof(double2);
+    // e^{ik.R} for the dense sums (also the N <= 32 fused step's one-launch sums); the
+    // separable form does not read it
+    const bool dm32 = nb32 && nb32_dense_multi_n(nr, nk);
+    p.bytes_P = bx.ok && !dm32 ? 0 : static_cast<size_t>(p.nk_stride) * nr * sizeof(double2);
     p.bytes_rho = static_cast<size_t>(p.nk_stride) * nn * sizeof(double2);
     p.bytes_mats = 5 * p.bytes_rho;  // H~, H, J_x, J_y, J_z
     p.bytes_k = static_cast<size_t>(nstage + 1) * p.bytes_rho;  // stage derivatives + eigenvectors
@@ -270,7 +290,12 @@ NbPlan nb_plan(int N, int nr, int64_t nk, int64_t nt, int nstage, bool split16, 
     }
     if (nb32) {  // the energies, and with the separable sums per evaluation B and both passes
         p.bytes_fused = align256(static_cast<size_t>(p.nk_stride) * 32 * sizeof(double));
-        if (bx.ok) {
+        if (dm32) {  // the matrices of a run of steps (7 per step, 2 more in the ring), A and E of the pulse
+            const int S = nb32_steps_per_eval(p.nk_stride, N, nt);
+            p.bytes_fused += align256((7 * static_cast<size_t>(S) + 2) * p.bytes_rho) +
+                             align256(static_cast<size_t>(nt) * 18 * sizeof(double));
+        }
+        if (bx.ok && !dm32) {
             const size_t col = sizeof(double2) * static_cast<size_t>(p.bcols);
             p.bytes_fused += 4 * (align256(col * static_cast<size_t>(bx.cells())) +
                                   align256(col * bx.M[0] * bx.M[1] * bx.K[2]) + align256(col * bx.M[0] * bx.K[1] * bx.K[2]));
@@ -577,8 +602,28 @@ int nb_setup(sbe_ctx *ctx)
         ctx->nb32_pair[1] = ctx->nb_k;
         ctx->nb32_pair[2] = ctx->nb_k + 2 * slot;
         ctx->nb32_lam = reinterpret_cast<double *>(fbuf);
+        ctx->nb32_dm = nb32_dense_multi_n(nr, nk);
+        if (ctx->nb32_dm) {
+            const int64_t nt = ctx->laser.nt;
+            const int S = nb32_steps_per_eval(p.nk_stride, N, nt);
+            char *mb = fbuf + align256(static_cast<size_t>(p.nk_stride) * 32 * sizeof(double));
+            ctx->nb32_S = S;
+            ctx->nb32_j = reinterpret_cast<double2 *>(mb);
+            ctx->nb32_half = ctx->nb32_j + 3 * static_cast<size_t>(S) * slot;
+            ctx->nb32_ring = ctx->nb32_half + 2 * static_cast<size_t>(S) * slot;
+            double *lt = reinterpret_cast<double *>(mb + align256((7 * static_cast<size_t>(S) + 2) * p.bytes_rho));
+            std::vector<double> hl(static_cast<size_t>(nt) * 18);
+            for (int64_t n = 0; n < nt; ++n) {  // the times of the step loop's evaluations
+                const double tn = ctx->laser.atmin + ctx->laser.dt * static_cast<double>(n);  // antelope.cpp:299
+                const double ts[3] = {tn, tn + 0.5 * ctx->laser.dt, tn + ctx->laser.dt};
+                for (int j = 0; j < 3; ++j) ctx->laser.eval(ts[j], hl.data() + 18 * n + 6 * j, hl.data() + 18 * n + 6 * j + 3);
+            }
+            rc = hipc(ctx, hipMemcpy(lt, hl.data(), hl.size() * sizeof(double), hipMemcpyHostToDevice), "upload laser");
+            if (rc) return rc;
+            ctx->nb32_laser = lt;
+        }
         for (int i = 0; i < 4; ++i) ctx->nb32_b[i] = ctx->nb32_t1[i] = ctx->nb32_t2[i] = nullptr;
-        if (a.box) {
+        if (a.box && !ctx->nb32_dm) {
             const size_t col = sizeof(double2) * static_cast<size_t>(p.bcols);
             const size_t b = align256(col * static_cast<size_t>(bx.cells()));
             const size_t t1 = align256(col * bx.M[0] * bx.M[1] * bx.K[2]);
@@ -602,11 +647,11 @@ int nb_setup(sbe_ctx *ctx)
         // sums' e^{ik.R} table and B buffer replace the box buffer, as the dense plan has them
         nfree(ctx, boxbuf);
         boxbuf = nullptr;
-        ctx->nb_P = nalloc<double2>(ctx, static_cast<size_t>(p.nk_stride) * nr * sizeof(double2), rc);
+        if (!ctx->nb_P) ctx->nb_P = nalloc<double2>(ctx, static_cast<size_t>(p.nk_stride) * nr * sizeof(double2), rc);
         a.b64 = nalloc<double2>(ctx, sizeof(double2) * static_cast<size_t>(p.nr_pad) * p.bcols, rc);
         if (rc) return rc;
     }
-    if (!analytic && !a.box) rc = hipc(ctx, nb_launch_phase(a, ctx->nb_P, ctx->stream), "phase kernel");
+    if (!analytic && ctx->nb_P) rc = hipc(ctx, nb_launch_phase(a, ctx->nb_P, ctx->stream), "phase kernel");
     if (!rc && split16) rc = hipc(ctx, nb_launch_psplit(a, ctx->nb_P, split, ctx->stream), "phase split kernel");
     NbEval ev{};
     ev.nmat = 1;
@@ -817,6 +862,72 @@ int nb_run_fused32(sbe_ctx *ctx, int64_t step_begin, int64_t step_end)
         return ev;
     };
     double2 *jm = ctx->nb_mats + 2 * slot;
+    auto launch_step = [&](const NbStep32 &f0, double tn) {
+        NbStep32 f = f0;
+        f.lam = ctx->nb32_lam;
+        f.u_valid = same_time(ctx->nb_u_time, tn, dt) ? 1 : 0;
+        f.lam_valid = f.u_valid && ctx->nbf_lam_valid && same_time(ctx->nbf_lam_time, tn, dt) ? 1 : 0;
+        f.dt = dt;
+        const int r = hipc(ctx, nb32_launch_step(a, f, ctx->nb32_blocks, ctx->nb_rho, ctx->nb_U, ctx->d_partials, ctx->stream),
+                           "step kernel");
+        ctx->nbf_prev_time = tn + dt;
+        ctx->nbf_prev_valid = true;
+        ctx->nb_u_time = tn + dt;
+        ctx->nbf_lam_time = tn + dt;
+        ctx->nbf_lam_valid = true;
+        ctx->nb_ht_valid = false;  // the separate kernels' matrices are not kept up to date
+        ctx->nb_h_valid = false;
+        return r;
+    };
+    if (ctx->nb32_dm) {  // the matrices of up to nb32_S steps per launch; one step launch per step
+        const int nring = ctx->nb32_S + 1;
+        auto pair = [&](int64_t step) { return ctx->nb32_ring + ((step % nring + nring) % nring) * 2 * slot; };
+        for (int64_t c0 = step_begin; c0 < step_end && !rc; c0 += ctx->chunk_steps) {
+            const int64_t n = std::min<int64_t>(ctx->chunk_steps, step_end - c0);
+            for (int64_t s = 0; s < n && !rc;) {
+                const int ns = static_cast<int>(std::min<int64_t>(ctx->nb32_S, n - s));
+                const int64_t step0 = c0 + s;
+                const double t0 = ctx->laser.atmin + dt * static_cast<double>(step0);  // antelope.cpp:299
+                if (!(ctx->nbf_prev_valid && same_time(ctx->nbf_prev_time, t0, dt))) {
+                    NbEval ev = eval(t0, true, true);
+                    double2 *out = pair(step0 - 1);
+                    rc = hipc(ctx, nb_launch_dense_multi(a, 1, &ev, ctx->nb_P, &out, ctx->stream), "fourier kernel");
+                }
+                NbDenseSteps d{};
+                d.laser = ctx->nb32_laser;
+                d.s0 = step0;
+                d.ns = ns;
+                d.nring = nring;
+                d.deph = a.deph;
+                d.slot = static_cast<int64_t>(slot);
+                d.j = ctx->nb32_j;
+                d.half = ctx->nb32_half;
+                d.ring = ctx->nb32_ring;
+                if (!rc) rc = hipc(ctx, nb_launch_dense_steps(a, d, ctx->nb_P, ctx->stream), "fourier kernel");
+                for (int i = 0; i < ns && !rc; ++i) {
+                    const int64_t step = step0 + i;
+                    const double tn = ctx->laser.atmin + dt * static_cast<double>(step);
+                    NbStep32 f{};
+                    f.ht_prev = pair(step - 1);
+                    f.h_prev = f.ht_prev + slot;
+                    f.j = ctx->nb32_j + 3 * static_cast<size_t>(i) * slot;
+                    f.ht_half = ctx->nb32_half + 2 * static_cast<size_t>(i) * slot;
+                    f.h_half = f.ht_half + slot;
+                    f.ht_end = pair(step);
+                    f.h_end = f.ht_end + slot;
+                    f.step_local = s + i;
+                    rc = launch_step(f, tn);
+                }
+                s += ns;
+            }
+            if (!rc)
+                rc = hipc(ctx, launch_series_reduce(ctx->d_partials, n, ctx->nb32_blocks, a.nv, ctx->d_series + c0 * a.nv,
+                                                    ctx->stream, ctx->nonfinite_d, kNbReduceGroup),
+                          "series kernel");
+            if (!rc) ctx->steps_done = c0 + n;
+        }
+        return rc;
+    }
     for (int64_t c0 = step_begin; c0 < step_end && !rc; c0 += ctx->chunk_steps) {
         const int64_t n = std::min<int64_t>(ctx->chunk_steps, step_end - c0);
         for (int64_t s = 0; s < n && !rc; ++s) {
@@ -851,22 +962,10 @@ int nb_run_fused32(sbe_ctx *ctx, int64_t step_begin, int64_t step_end)
             f.h_half = ctx->nb32_pair[1] + slot;
             f.ht_end = ctx->nb32_pair[2];
             f.h_end = ctx->nb32_pair[2] + slot;
-            f.lam = ctx->nb32_lam;
-            f.u_valid = same_time(ctx->nb_u_time, tn, dt) ? 1 : 0;
-            f.lam_valid = f.u_valid && ctx->nbf_lam_valid && same_time(ctx->nbf_lam_time, tn, dt) ? 1 : 0;
-            f.dt = dt;
             f.step_local = s;
-            rc = hipc(ctx, nb32_launch_step(a, f, ctx->nb32_blocks, ctx->nb_rho, ctx->nb_U, ctx->d_partials, ctx->stream),
-                      "step kernel");
+            rc = launch_step(f, tn);
             // the end of this step is the start of the next: its H~, H serve there
             std::swap(ctx->nb32_pair[0], ctx->nb32_pair[2]);
-            ctx->nbf_prev_time = tn + dt;
-            ctx->nbf_prev_valid = true;
-            ctx->nb_u_time = tn + dt;
-            ctx->nbf_lam_time = tn + dt;
-            ctx->nbf_lam_valid = true;
-            ctx->nb_ht_valid = false;  // the separate kernels' matrices are not kept up to date
-            ctx->nb_h_valid = false;
         }
         if (!rc)
             rc = hipc(ctx, launch_series_reduce(ctx->d_partials, n, ctx->nb32_blocks, a.nv, ctx->d_series + c0 * a.nv,
@@ -1035,7 +1134,7 @@ int nb_state(sbe_ctx *ctx, int basis, double *rho)
         double2 *out = ctx->nb_mats + static_cast<size_t>(a.nk_stride) * nn;
         ctx->nb_ht_valid = false;
         ctx->nb_h_valid = false;
-        if (ctx->nb32) ctx->nbf_prev_valid = false;  // an H~ H pair may live in slots 0-1
+        if (ctx->nb32 && !ctx->nb32_dm) ctx->nbf_prev_valid = false;  // an H~ H pair may live in slots 0-1
         int rc = hipc(ctx, eval_mats(a, ev, ctx->nb_P, ctx->nb_mats, ctx->stream), "fourier kernel");
         if (!rc) rc = hipc(ctx, nb_launch_band_export(a, ctx->nb_mats, ctx->nb_rho, out, ctx->stream), "band export kernel");
         if (!rc) rc = hipc(ctx, hipMemcpyAsync(rho, out, bytes, hipMemcpyDeviceToHost, ctx->stream), "copy state");

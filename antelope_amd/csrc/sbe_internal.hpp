@@ -196,6 +196,12 @@ struct sbe_ctx {
     double2 *nb32_t1[4] = {nullptr, nullptr, nullptr, nullptr};  // first and second pass
     double2 *nb32_t2[4] = {nullptr, nullptr, nullptr, nullptr};
     int nb32_blocks = 0;
+    // short R lists, few k points: the matrices of up to nb32_S steps in one dense launch,
+    // J and the midpoint pair per step, the end pairs in a ring of nb32_S + 1
+    bool nb32_dm = false;
+    int nb32_S = 0;
+    double2 *nb32_j = nullptr, *nb32_half = nullptr, *nb32_ring = nullptr;
+    const double *nb32_laser = nullptr;  // [nt][3][6] A, E at t_n, t_n + dt/2, t_n + dt
     // time-fused velocity-gauge steps (nb16_vel_fused_kernel)
     bool nbv = false;
     const double *nbv_A = nullptr;  // [nt][4][3] A at the RK4 stage times

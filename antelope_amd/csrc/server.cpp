@@ -357,10 +357,13 @@ int sbe_server_step(sbe_server *s)
                     r.status = SBE_ERR_HIP, r.err = "hipEventCreate failed";
             r.slice = std::min(s->slice, r.ctx->chunk_steps);
         } else {
-            if (r.ctx->path == sbe::PathKind::NBand) {
-                // every N-band step is a dozen small launches: short slices keep the hardware
-                // queue this request shares with others from filling with thousands of them
+            if (r.ctx->path == sbe::PathKind::NBand && !(r.ctx->nbf || r.ctx->nb32 || r.ctx->nbv)) {
+                // every N-band step on the separate kernels is a dozen small launches: short
+                // slices keep the hardware queue this request shares with others from filling
+                // with thousands of them (the fused steps take one or two launches per step)
                 r.slice = std::max<int64_t>(1, s->slice / 32);
+            } else if (r.ctx->path == sbe::PathKind::NBand) {
+                r.slice = s->slice;
             } else if (r.ctx->adaptive) {
                 // whole windows of the adaptive stepper, so the result equals a run on its own
                 const int64_t w = r.ctx->chunk_steps;

@@ -167,26 +167,46 @@ def test_separable_sums_footprint_without_gpu():
 
 def test_fused_step_n32_footprint_without_gpu():
     """For 16 < N <= 32 the fused RK4 step keeps its matrices in the mats and stage slots
-    the separate kernels have, and adds the band energies [nk_stride][32] (FP64) and, with
-    the separable sums, per evaluation (up to four batched) B on the box and both pass
-    outputs; with the dense sums only the energies. RK5 and the SPLIT16 sums keep the separate
-    kernels."""
+    the separate kernels have and adds the band energies [nk_stride][32] (FP64). With at
+    most 32 lattice vectors and at most 1024 k points the evaluations of a run of steps
+    are one dense launch, which reads the e^{ik.R} table [nk_stride][N_R] (kept by the
+    dense sums anyway); otherwise, with the separable sums, each of up to four batched
+    evaluations has B on the box and both pass outputs. RK5 and the SPLIT16 sums keep
+    the separate kernels."""
     import dataclasses
 
     from antelope_amd import Pulse, RunConfig
     from antelope_amd.models import synthetic_wannier
 
-    N, M, K = 24, (5, 5, 1), (12, 12, 1)
-    m = synthetic_wannier(N, nmax=(2, 2, 0), seed=3)
-    cfg = RunConfig(npoints=list(K[:2]), pulses=[Pulse(E0=0.003, w0=0.014, ncycles=2.0, envelope="sin2")],
-                    dt=0.25, T2=220.0, rk_order=4, inter_intra=True)
+    def cfg_for(npts):
+        return RunConfig(npoints=list(npts), pulses=[Pulse(E0=0.003, w0=0.014, ncycles=2.0, envelope="sin2")],
+                         dt=0.25, T2=220.0, rk_order=4, inter_intra=True)
+
     al = lambda x: (x + 255) // 256 * 256
+    N = 24
     col = 16 * ((4 * N * N + 63) // 64 * 64)
-    nk_stride = (K[0] * K[1] + 63) // 64 * 64
+    # 25 lattice vectors, 144 k points: the matrices of S steps per dense launch (J, the
+    # midpoint pair, the end pair in a ring of S + 1), S for about 256 MB of them, and the
+    # pulse's A and E at the three evaluation times of every step
+    m = synthetic_wannier(N, nmax=(2, 2, 0), seed=3)
+    cfg = cfg_for((12, 12))
+    nt = laser_eval(cfg, [0.0])[3]
+    nk_stride = (144 + 63) // 64 * 64
     lam = al(nk_stride * 32 * 8)
+    mat = nk_stride * N * N * 16
+    S = max(1, min(nt, (256 << 20) // (7 * mat)))
+    steps = al((7 * S + 2) * mat) + al(nt * 18 * 8)
+    assert _footprint_env(cfg, m, dense=False, fused=True) - _footprint_env(cfg, m, dense=False) == \
+        lam + steps + nk_stride * 25 * 16
+    assert _footprint_env(cfg, m, dense=True, fused=True) - _footprint_env(cfg, m, dense=True) == lam + steps
+    # 75 lattice vectors on a 5 x 5 x 3 box, 6 x 5 x 2 grid: the separable passes, batched
+    M, K = (5, 5, 3), (6, 5, 2)
+    m75 = synthetic_wannier(N, nmax=(2, 2, 1), seed=3)
+    cfg75 = cfg_for(K)
+    nk_stride = (60 + 63) // 64 * 64
     scratch = 4 * (al(M[0] * M[1] * M[2] * col) + al(M[0] * M[1] * K[2] * col) + al(M[0] * K[1] * K[2] * col))
-    assert _footprint_env(cfg, m, dense=False, fused=True) - _footprint_env(cfg, m, dense=False) == lam + scratch
-    assert _footprint_env(cfg, m, dense=True, fused=True) - _footprint_env(cfg, m, dense=True) == lam
+    assert _footprint_env(cfg75, m75, dense=False, fused=True) - _footprint_env(cfg75, m75, dense=False) == \
+        al(nk_stride * 32 * 8) + scratch
     for other in (dataclasses.replace(cfg, rk_order=5), dataclasses.replace(cfg, precision="split16")):
         assert _footprint_env(other, m, dense=False, fused=True) == _footprint_env(other, m, dense=False)
 

@@ -125,7 +125,8 @@ def test_time_fused_velocity_gauge(oracle, monkeypatch, label, N, kw):
 
 
 @pytest.mark.parametrize("label,N,npts,nmax,kw,dense", [
-    ("n24_box6x5x2", 24, (6, 5, 2), (1, 1, 1), dict(), False),
+    ("n24_box6x5x2_75R_separable_passes", 24, (6, 5, 2), (2, 2, 1), dict(), False),
+    ("n24_27R_one_dense_launch", 24, (6, 5, 2), (1, 1, 1), dict(), False),
     ("n32_grid8x8_no_dephasing_total_current", 32, (8, 8), (2, 2, 0), dict(T2=-1.0, inter_intra=False), False),
     ("n20_dense_sums_t1t2", 20, (5, 4, 3), (1, 1, 1), dict(T1=150.0), True),
     ("n17_300k_more_k_than_blocks", 17, (20, 15), (1, 1, 0), dict(), False),
@@ -154,12 +155,16 @@ def test_fused_step_n32_against_oracle_and_separate_kernels(oracle, monkeypatch,
     assert rel_l2(st_f, st_s) < 1e-12
 
 
-def test_fused_step_n32_split_runs_and_state_reads_are_bitwise(monkeypatch):
+@pytest.mark.parametrize("label,nmax", [("n24_27R_steps_per_launch", (1, 1, 1)),
+                                         ("n24_75R_separable_passes", (2, 2, 1))])
+def test_fused_step_n32_split_runs_and_state_reads_are_bitwise(monkeypatch, label, nmax):
     """Runs of the N <= 32 fused step split anywhere, with a band-basis state read in
-    between (it reuses the matrix slots the step keeps H~ and H in), give the bits of one run."""
-    model = synthetic_wannier(24, nmax=(1, 1, 1), seed=19)
+    between (it reuses matrix slots the step may keep H~ and H in), give the bits of one
+    run; with the dense sums of many steps per launch, so does any number of steps per
+    launch (1 and 5: the ring of end pairs wraps)."""
+    model = synthetic_wannier(24, nmax=nmax, seed=19)
     cfg = _cfg((6, 5, 2), ncycles=0.5)
-    print("case: n24 split runs", flush=True)
+    print(f"case: {label} split runs", flush=True)
     one, st1, _ = _solve(cfg, model, monkeypatch, fused=True)
     seen = []
     two, st2, _ = _solve(cfg, model, monkeypatch, fused=True, splits=(37, 38, 90),
@@ -168,3 +173,10 @@ def test_fused_step_n32_split_runs_and_state_reads_are_bitwise(monkeypatch):
     for k in one:
         assert np.array_equal(one[k], two[k]), k
     assert np.array_equal(st1, st2)
+    if nmax == (1, 1, 1):
+        for S in ("1", "5"):
+            monkeypatch.setenv("ANTELOPE_NB_EVAL_STEPS", S)
+            three, st3, _ = _solve(cfg, model, monkeypatch, fused=True, splits=(37,))
+            for k in one:
+                assert np.array_equal(one[k], three[k]), (S, k)
+            assert np.array_equal(st1, st3)
