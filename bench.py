@@ -16,7 +16,8 @@ dt = 0.25 per solve).
 Other workloads: --workload haldane (300 x 300 Haldane, elliptic 12-cycle
 pulse, FP64 and FP32 state; a step is one RK4 step of all 90,000 k points),
 wannier16 (N = 16 Wannier model on 48 x 48 x 8), velocity (the velocity
-gauge) and serving (a seeded request stream).
+gauge), serving (a seeded request stream) and wannier-requests (the stream's
+Wannier requests solved one at a time, fused and separate kernels).
 
 Prints ONE JSON line. Synthetic, seeded inputs only.
 
@@ -525,6 +526,50 @@ def rel_l2(x, ref):
     return float(np.linalg.norm(x - ref) / d) if d > 0 else float(np.linalg.norm(x))
 
 
+def bench_wannier_requests(args) -> int:
+    """The serving stream's Wannier requests (synthetic N = 8, 16, 32 models with 25 R on
+    8^2..16^2 grids, 2..4-cycle pulses, dt = 0.25) solved one at a time over the whole
+    pulse: the fused RK4 step kernels beside the separate stage and sample kernels
+    (ANTELOPE_NB_FUSED=0). Reports the solve time of each; value is the N = 32 request's
+    k-point RK steps/s with the fused kernels."""
+    from antelope_amd.serve import serving_stream
+    reqs, seen = [], set()
+    for r in serving_stream(args.requests, rate=20.0, seed=5, wannier_share=0.1):
+        if r.kind.startswith("wannier") and r.kind not in seen:
+            seen.add(r.kind)
+            reqs.append(r)
+    forms = {}
+    for fused in ("1", "0"):
+        os.environ["ANTELOPE_NB_FUSED"] = fused
+        for r in reqs:
+            with SBESolver(r.config, r.model) as w:  # first-launch code loading outside the timing
+                w.run(0, min(8, w.nt))
+                w.synchronize()
+            with SBESolver(r.config, r.model) as s:
+                gpu_sync()
+                t0 = time.perf_counter()
+                s.run()
+                s.synchronize()
+                wall = time.perf_counter() - t0
+                nk = s.info.k_end - s.info.k_begin
+                key = f"{r.kind}_{'fused' if fused == '1' else 'separate'}"
+                forms[key] = {"nk": nk, "steps": s.nt, "solve_s": wall, "us_per_step": wall * 1e6 / s.nt,
+                              "k_steps_per_s": nk * s.nt / wall}
+                print(f"[bench] {key}: nk={nk} {s.nt} steps in {wall:.3f} s ({wall * 1e6 / s.nt:.1f} us/step)",
+                      file=sys.stderr, flush=True)
+    os.environ.pop("ANTELOPE_NB_FUSED", None)
+    top = forms["wannier32_fused"]
+    line = {"metric": METRIC, "value": top["k_steps_per_s"], "unit": "k-point·RK-steps/s", "n_gpus": 1,
+            "steps": top["steps"], "warmup": 8, "ms_per_step": top["us_per_step"] * 1e-3,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic (seeded Wannier models and pulses of the serving stream)",
+            "config": {"workload": "the serving stream's Wannier requests (N = 8, 16, 32, 25 R), one at a time, "
+                                   "whole pulse, RK4 dt = 0.25, T2 = 220", "nk": top["nk"]},
+            "forms": forms, "roofline": None, "cpu_baseline": None}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -532,7 +577,8 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=None,
                     help="untimed steps (default: 1 batch pass; 200 time steps for --workload haldane)")
     ap.add_argument("--grid", type=int, default=300)
-    ap.add_argument("--workload", choices=["batch", "haldane", "wannier16", "serving", "velocity"], default="batch")
+    ap.add_argument("--workload", choices=["batch", "haldane", "wannier16", "serving", "velocity", "wannier-requests"],
+                    default="batch")
     ap.add_argument("--precision", choices=["both", "fp64", "fp32"], default="both")
     ap.add_argument("--requests", type=int, default=100, help="serving: requests in the stream")
     ap.add_argument("--instances", type=int, default=64, help="batch: solves in the batch")
@@ -568,6 +614,8 @@ def main(argv=None) -> int:
         return bench_wannier(args)
     if args.workload == "serving":
         return bench_serving(args)
+    if args.workload == "wannier-requests":
+        return bench_wannier_requests(args)
     if args.workload == "batch":
         return bench_batch(args)
     if args.workload == "velocity":
