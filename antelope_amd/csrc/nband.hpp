@@ -155,10 +155,14 @@ struct NbVelFused {
     const double *A;  // [nt][4][3]: A(t_n + c_s dt)
     int64_t s0, s1;
     double dt;
-    int nblocks;      // ceil(nk / kNbFusedWaves)
+    int nblocks;      // partial rows per step: ceil(nk / kNbFusedWaves) for N <= 16, nk above
 };
 size_t nb16_vel_fused_lds_bytes(int nv);
 hipError_t nb16_launch_vel_fused(const NbArgs &a, const NbVelFused &f, const double2 *P, const double *Eocc,
+                                 double2 *rho, double *partials, hipStream_t s);
+// the same for 16 < N <= 32: one block of four waves per k point (nblocks = nk)
+constexpr int64_t kNb32VelMaxK = 1024;
+hipError_t nb32_launch_vel_fused(const NbArgs &a, const NbVelFused &f, const double2 *P, const double *Eocc,
                                  double2 *rho, double *partials, hipStream_t s);
 
 // One time step of the VelocityHamiltonian gauge: A at the stage times and

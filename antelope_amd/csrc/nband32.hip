@@ -506,6 +506,114 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1))) void n
     }
 }
 
+// Steps [s0, s1) of the VelocityHamiltonian gauge (SBEs.h:787-805,862-871: the
+// band-basis H = sum_a A_a(t) P_a + diag(E)) with the classical RK4 tableau in one launch,
+// for 16 < N <= 32: a block of four waves owns one k point for every step, each wave one
+// quadrant of rho and of pMatrix in registers (no re-reads); per step the block writes its
+// partial row (the currents summed over its waves in order, populations and rho directly
+// by the lanes that hold them). A at the stage times from the host-evaluated table.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1))) void nb32_vel_fused_kernel(
+    const NbArgs a, const NbVelFused f, const cd *__restrict__ P, const double *__restrict__ Eocc,
+    cd *__restrict__ rho, double *__restrict__ partials)
+{
+    __shared__ cd slots[4 * kQSlot];
+    __shared__ double scal[2 * 128];
+    const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+    Q L{slots, scal, w >> 1, w & 1, 0, a.N};
+    const int N = a.N, nn = N * N;
+    const int c = lane_c(), g = lane_g(), lane = threadIdx.x & 63;
+    const int64_t k = blockIdx.x, kn = k * nn;
+    const int64_t ks = a.nk_stride * nn;
+    Tile R = q_load(L, rho + kn, false);
+    const Tile Px = q_load(L, P + kn, false), Py = q_load(L, P + ks + kn, false), Pz = q_load(L, P + 2 * ks + kn, false);
+    double Er[4], orr[4];  // E and occupation of this lane's rows (used on the diagonal)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int i = 16 * L.p + g + 4 * r;
+        Er[r] = i < N ? Eocc[k * 2 * N + i] : 0.0;
+        orr[r] = i < N ? Eocc[k * 2 * N + N + i] : 0.0;
+    }
+    const double wk = a.weight[k], dt = f.dt;
+    for (int64_t step = f.s0; step < f.s1; ++step) {
+        // ---- sample at t_n: Re Tr(rho P_a) = sum_ij Re(conj(rho_ij) P_ij) (rho Hermitian),
+        // split into the diagonal (intraband) and off-diagonal parts; populations; rho
+        double *row = partials + ((step - f.s0) * f.nblocks + blockIdx.x) * static_cast<int64_t>(a.nv);
+        double part[6];
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+            const Tile &Pa = ax == 0 ? Px : (ax == 1 ? Py : Pz);
+            double inter = 0.0, intra = 0.0;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const double val = R.re[r] * Pa.re[r] + R.im[r] * Pa.im[r];
+                if (q_diagonal(L, r) && a.inter_intra) intra += val;
+                else inter += val;
+            }
+            part[ax] = wave_sum(inter);
+            part[3 + ax] = wave_sum(intra);
+        }
+        {
+            double *S = L.scal();
+            if (lane == 0)
+#pragma unroll
+                for (int j = 0; j < 6; ++j) S[8 * w + j] = part[j];
+            __syncthreads();
+            if (w == 0 && lane < 6) row[lane] = (((S[lane] + S[8 + lane]) + S[16 + lane]) + S[24 + lane]) * wk;
+            L.next();
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            if (!q_valid(L, r)) continue;
+            const int i = 16 * L.p + g + 4 * r, j = 16 * L.q + c;
+            if (i == j) row[6 + i] = R.re[r] * wk;
+            row[6 + N + 2 * (i * N + j)] = R.re[r] * wk;
+            row[6 + N + 2 * (i * N + j) + 1] = R.im[r] * wk;
+        }
+        // ---- the four RK4 stages
+        const double *At = f.A + step * 12;
+        Tile Acc, Y = R, X;
+#pragma unroll 1
+        for (int st = 0; st < 4; ++st) {
+            const double A0 = At[3 * st], A1 = At[3 * st + 1], A2 = At[3 * st + 2];
+            Tile H;  // this quadrant of sum_a A_a P_a + diag(E)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                H.re[r] = A0 * Px.re[r] + A1 * Py.re[r] + A2 * Pz.re[r] + (q_diagonal(L, r) ? Er[r] : 0.0);
+                H.im[r] = A0 * Px.im[r] + A1 * Py.im[r] + A2 * Pz.im[r];
+            }
+            const Tile T = q_prod(L, H, false, Y, false);  // H y
+            const Tile Td = q_adjoint(L, T);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const double dr = T.re[r] - Td.re[r], di = T.im[r] - Td.im[r];
+                double xr = di, xi = -dr;  // -i (H y - (H y)^dag)
+                if (a.deph) {
+                    const bool dg = q_diagonal(L, r);
+                    const double gm = dg ? a.gam1 : a.gam2;
+                    xr -= gm * (Y.re[r] - (dg ? orr[r] : 0.0));
+                    xi -= gm * Y.im[r];
+                }
+                X.re[r] = xr;
+                X.im[r] = xi;
+            }
+            if (st == 0) {
+                q_axpy(Acc, X, dt / 6.0, R);
+                q_axpy(Y, X, 0.5 * dt, R);
+            } else if (st < 3) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    Acc.re[r] = fma(dt / 3.0, X.re[r], Acc.re[r]);
+                    Acc.im[r] = fma(dt / 3.0, X.im[r], Acc.im[r]);
+                }
+                q_axpy(Y, X, st == 1 ? 0.5 * dt : dt, R);
+            } else {
+                q_axpy(R, X, dt / 6.0, Acc);
+            }
+        }
+    }
+    q_store(L, rho + kn, R);
+}
+
 }  // namespace
 
 hipError_t nb32_launch_step(const NbArgs &a, const NbStep32 &f, int nblocks, double2 *rho, double2 *U, double *partials,
@@ -513,6 +621,14 @@ hipError_t nb32_launch_step(const NbArgs &a, const NbStep32 &f, int nblocks, dou
 {
     if (a.N <= 16 || a.N > 32 || nblocks < 1) return hipErrorInvalidValue;
     nb32_step_kernel<<<dim3(static_cast<unsigned>(nblocks)), 256, 0, s>>>(a, f, rho, U, partials);
+    return hipGetLastError();
+}
+
+hipError_t nb32_launch_vel_fused(const NbArgs &a, const NbVelFused &f, const double2 *P, const double *Eocc,
+                                 double2 *rho, double *partials, hipStream_t s)
+{
+    if (a.N <= 16 || a.N > 32 || f.nblocks != a.nk || f.s1 <= f.s0) return hipErrorInvalidValue;
+    nb32_vel_fused_kernel<<<dim3(static_cast<unsigned>(f.nblocks)), 256, 0, s>>>(a, f, P, Eocc, rho, partials);
     return hipGetLastError();
 }
 

@@ -162,15 +162,16 @@ bool fused_eligible(const sbe_config &cfg, const sbe_model &m, int64_t nk, const
     return nk == bx.kcells();  // the whole grid (a k block of it takes the separate kernels)
 }
 
-// The time-fused velocity-gauge steps (nb16_vel_fused_kernel): FP64 with the classical
-// RK4 tableau, N <= 16 on the register-tile kernels; any model (the step has no Fourier
-// sums). ANTELOPE_NB_FUSED=0 keeps one launch per step.
+// The time-fused velocity-gauge steps (nb16_vel_fused_kernel for N <= 16 on the
+// register-tile kernels, nb32_vel_fused_kernel for 16 < N <= 32 up to kNb32VelMaxK k
+// points, one block per k point): FP64 with the classical RK4 tableau; any model (the step
+// has no Fourier sums). ANTELOPE_NB_FUSED=0 keeps one launch per step.
 bool vel_fused_eligible(const sbe_config &cfg, const sbe_model &m, int64_t nk)
 {
     if (const char *e = std::getenv("ANTELOPE_NB_FUSED"))
         if (std::atoi(e) == 0) return false;
     if (cfg.gauge != SBE_GAUGE_VELOCITY_HAMILTONIAN || cfg.rk_order != SBE_RK4 || cfg.precision != SBE_PREC_FP64 ||
-        m.nband > 16)
+        m.nband > 32 || (m.nband > 16 && nk > kNb32VelMaxK))
         return false;
     if (const char *e = std::getenv("ANTELOPE_NB_LDS"))
         if (std::atoi(e) == 1) return false;
@@ -239,7 +240,7 @@ NbPlan nb_plan(int N, int nr, int64_t nk, int64_t nt, int nstage, bool split16, 
     const size_t nn = static_cast<size_t>(N) * N;
     int g1 = 0;
     int64_t rows = fused ? std::max<int64_t>(p.sample_waves, fused_blocks(bx, g1)) : p.sample_waves;
-    if (vel_fused) rows = std::max<int64_t>(rows, (nk + kNbFusedWaves - 1) / kNbFusedWaves);
+    if (vel_fused) rows = std::max<int64_t>(rows, N > 16 ? nk : (nk + kNbFusedWaves - 1) / kNbFusedWaves);
     p.bytes_vel = vel_fused ? align256(static_cast<size_t>(nt) * 12 * sizeof(double)) : 0;
     const size_t row = static_cast<size_t>(rows) * p.nv * sizeof(double);
     p.chunk = std::max<int64_t>(1, std::min<int64_t>(nt, static_cast<int64_t>(kNbPartialBytes / row)));
@@ -670,7 +671,7 @@ int nb_setup(sbe_ctx *ctx)
         const size_t lds_p = nb_vel_lds_bytes(N, ctx->tab.nstage, true);
         ctx->nb_vel_p_lds = lds_p <= (size_t(64) << 10);
         ctx->nbv = false;
-        if (vtab && a.tile16) {  // A at the four RK4 stage times of every step, as the host evaluates them
+        if (vtab && (a.tile16 || N > 16)) {  // A at the four RK4 stage times of every step, as the host evaluates them
             const int64_t nt = ctx->laser.nt;
             std::vector<double> hv(static_cast<size_t>(nt) * 12);
             for (int64_t n = 0; n < nt; ++n) {
@@ -682,7 +683,7 @@ int nb_setup(sbe_ctx *ctx)
             }
             if (!rc) rc = hipc(ctx, hipMemcpy(vtab, hv.data(), hv.size() * sizeof(double), hipMemcpyHostToDevice), "upload A");
             ctx->nbv_A = vtab;
-            ctx->nbv_nblocks = static_cast<int>((nk + kNbFusedWaves - 1) / kNbFusedWaves);
+            ctx->nbv_nblocks = static_cast<int>(N > 16 ? nk : (nk + kNbFusedWaves - 1) / kNbFusedWaves);
             ctx->nbv = true;
         }
         if (!rc) rc = hipc(ctx, hipStreamSynchronize(ctx->stream), "init sync");
@@ -721,7 +722,9 @@ int nb_run_velocity(sbe_ctx *ctx, int64_t step_begin, int64_t step_end)
             f.s1 = c0 + n;
             f.dt = dt;
             f.nblocks = ctx->nbv_nblocks;
-            rc = hipc(ctx, nb16_launch_vel_fused(a, f, Pv, Eocc, ctx->nb_rho, ctx->d_partials, ctx->stream),
+            rc = hipc(ctx,
+                      a.N > 16 ? nb32_launch_vel_fused(a, f, Pv, Eocc, ctx->nb_rho, ctx->d_partials, ctx->stream)
+                               : nb16_launch_vel_fused(a, f, Pv, Eocc, ctx->nb_rho, ctx->d_partials, ctx->stream),
                       "velocity step kernel");
             if (!rc)
                 rc = hipc(ctx, launch_series_reduce(ctx->d_partials, n, ctx->nbv_nblocks, a.nv, ctx->d_series + c0 * a.nv,

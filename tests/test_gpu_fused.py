@@ -101,10 +101,13 @@ def test_fused_rk5_keeps_separate_kernels(oracle, monkeypatch):
 @pytest.mark.parametrize("label,N,kw", [
     ("n8_velocity_t2", 8, dict()),
     ("n16_velocity_t1t2_total_current", 16, dict(T2=60.0, inter_intra=False)),
+    ("n24_velocity_t2_four_waves_per_k", 24, dict()),
+    ("n32_velocity_t1t2_total_current", 32, dict(T1=150.0, T2=60.0, inter_intra=False)),
 ])
 def test_time_fused_velocity_gauge(oracle, monkeypatch, label, N, kw):
-    """The time-fused velocity-gauge kernel (all steps of a chunk in one launch, one wave
-    per k point) against the oracle and the one-launch-per-step kernel, and split runs."""
+    """The time-fused velocity-gauge kernels (all steps of a chunk in one launch; one wave
+    per k point for N <= 16, a block of four waves per k point above) against the oracle
+    and the one-launch-per-step kernel, and split runs."""
     import dataclasses
     model = synthetic_wannier(N, nmax=(1, 1, 1), seed=70 + N)
     T1 = kw.pop("T1", 150.0 if N == 16 else -1.0)
@@ -180,3 +183,25 @@ def test_fused_step_n32_split_runs_and_state_reads_are_bitwise(monkeypatch, labe
             for k in one:
                 assert np.array_equal(one[k], three[k]), (S, k)
             assert np.array_equal(st1, st3)
+
+
+@pytest.mark.parametrize("label,nmax", [("n20_27R_dense_steps", (1, 1, 1)), ("n20_75R_separable_passes", (2, 2, 1))])
+def test_fused_step_n32_k_blocks(oracle, monkeypatch, label, nmax):
+    """The N <= 32 fused step on k blocks of the grid (one rank's share each, ragged):
+    the blocks' series add up to the oracle's whole-grid series."""
+    model = synthetic_wannier(20, nmax=nmax, seed=31)
+    cfg = _cfg((6, 5, 2), ncycles=0.25)
+    monkeypatch.setenv("ANTELOPE_NB_FUSED", "1")
+    print(f"case: {label} k blocks", flush=True)
+    ref = oracle.run(cfg, model, nthreads=8)
+    tot = None
+    for b, e in [(0, 23), (23, 41), (41, 60)]:
+        with SBESolver(cfg, model, b, e) as s:
+            assert s.info.device_bytes == query_footprint(cfg, model, b, e)
+            s.run(0, 40)
+            s.run(40, s.nt)
+            g = s.series()
+        tot = g if tot is None else {k: tot[k] + g[k] for k in g}
+    e_or = _errs(tot, ref)
+    print(label, "k blocks vs oracle", {k: f"{v:.2e}" for k, v in e_or.items()}, flush=True)
+    assert all(v < TOL64 for v in e_or.values()), e_or
