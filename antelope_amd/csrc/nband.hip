@@ -323,6 +323,70 @@ __global__ __launch_bounds__(256) void nb_box_pass_multi_kernel(const NbArgs a, 
     (void)K1;
 }
 
+// evaluation e of a run of RK4 steps (NbBoxSteps): its A, E, matrices and outputs
+__device__ __forceinline__ NbEval box_step_eval(const NbBoxSteps &d, int e, cd *&t2)
+{
+    const int sl = e / 3, typ = e - 3 * sl;
+    const int64_t step = d.s0 + sl;
+    NbEval ev{};
+    const double *lt = d.laser + (step * 3 + typ) * 6;
+    for (int i = 0; i < 3; ++i) {
+        ev.A[i] = lt[i];
+        ev.E[i] = lt[3 + i];
+    }
+    if (typ == 0) {
+        ev.nmat = 3;
+        ev.mats[0] = kMatJx;
+        ev.mats[1] = kMatJy;
+        ev.mats[2] = kMatJz;
+        t2 = d.tj + sl * d.t2_stride;
+    } else {
+        ev.nmat = 2;
+        ev.mats[0] = kMatHtilde;
+        ev.mats[1] = kMatH0;
+        t2 = typ == 1 ? d.th + sl * d.t2_stride : d.ring + (step % d.nring) * d.t2_stride;
+    }
+    return ev;
+}
+
+__global__ void nb_box_steps_bform_kernel(const NbArgs a, const NbBoxSteps d, int64_t ncells)
+{
+    const int e = blockIdx.z;
+    cd *t2;
+    const NbEval ev = box_step_eval(d, e, t2);
+    const int64_t ncols = static_cast<int64_t>(ev.nmat) * a.N * a.N;
+    const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= ncells * ncols) return;
+    const int64_t cell = idx / ncols, c = idx % ncols;
+    const int r = a.box_r[cell];
+    cd v = cmk(0.0, 0.0);
+    if (r >= 0) {
+        const double *R = a.rcart + 3 * r;
+        double sn, cs;
+        sincos(ev.A[0] * R[0] + ev.A[1] * R[1] + ev.A[2] * R[2], &sn, &cs);
+        v = b_element(a, ev, r, c, cmk(cs, sn));
+    }
+    (d.b + e * d.b_stride)[cell * a.bcols + c] = v;
+}
+
+template <int MM, int PASS>
+__global__ __launch_bounds__(256) void nb_box_steps_pass_kernel(const NbArgs a, const NbBoxSteps d, int kc, int nz)
+{
+    const int e = static_cast<int>(blockIdx.z) / nz, kz = static_cast<int>(blockIdx.z) - e * nz;
+    cd *t2;
+    const NbEval ev = box_step_eval(d, e, t2);
+    const int nn = a.N * a.N;
+    const int64_t ncols = static_cast<int64_t>(ev.nmat) * nn;
+    const int M2 = a.boxM[1], M3 = a.boxM[2], K2 = a.boxK[1], K3 = a.boxK[2];
+    cd *t1 = d.t1 + e * d.t1_stride;
+    if (PASS == 0)
+        box_pass_body<MM, false>(d.b + e * d.b_stride, t1, a.boxE[2], M3, K3, kc, 1, ncols, a.bcols, nullptr, nullptr,
+                                 a.nk_stride, nn, kz);
+    else
+        box_pass_body<MM, false>(t1, t2, a.boxE[1], M2, K2, kc, K3, ncols, a.bcols, nullptr, nullptr, a.nk_stride, nn,
+                                 kz);
+}
+
 // pass along a1: T2 -> the matrices, scattered to the k points
 template <int MM>
 __global__ __launch_bounds__(256) void nb_box_a1_multi_kernel(const NbArgs a, const BoxMulti bm, int kc, int nz)
@@ -1650,6 +1714,45 @@ hipError_t nb_launch_dense_multi(const NbArgs &a, int nev, const NbEval *evs, co
     }
     if (a.nr <= 16) dense_multi<16>(a, dm, P, maxcols, s);
     else dense_multi<32>(a, dm, P, maxcols, s);
+    return hipGetLastError();
+}
+
+template <int MM, int PASS>
+static void box_steps_pass(const NbArgs &a, const NbBoxSteps &d, int64_t maxcols, hipStream_t s)
+{
+    const int M1 = a.boxM[0], M2 = a.boxM[1], K2 = a.boxK[1], K3 = a.boxK[2];
+    const int K = PASS == 0 ? K3 : K2;
+    const int64_t outer = PASS == 0 ? static_cast<int64_t>(M1) * M2 : M1, inner = PASS == 0 ? 1 : K3;
+    const int64_t nev = 3 * static_cast<int64_t>(d.ns);
+    const int64_t xy = outer * inner * ((maxcols + 255) / 256) * nev;
+    const int kc = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(K, K * xy / 2048)));
+    const int nz = (K + kc - 1) / kc;
+    const dim3 grid(static_cast<unsigned>(outer * inner), static_cast<unsigned>((maxcols + 255) / 256),
+                    static_cast<unsigned>(nz * nev));
+    nb_box_steps_pass_kernel<MM, PASS><<<grid, 256, 0, s>>>(a, d, kc, nz);
+}
+
+template <int PASS>
+static void box_steps_pass_m(const NbArgs &a, const NbBoxSteps &d, int64_t maxcols, hipStream_t s)
+{
+    const int M = a.boxM[2 - PASS];
+    if (M <= 4) box_steps_pass<4, PASS>(a, d, maxcols, s);
+    else if (M <= 8) box_steps_pass<8, PASS>(a, d, maxcols, s);
+    else if (M <= 16) box_steps_pass<16, PASS>(a, d, maxcols, s);
+    else box_steps_pass<32, PASS>(a, d, maxcols, s);
+}
+
+hipError_t nb_launch_box_steps(const NbArgs &a, const NbBoxSteps &d, hipStream_t s)
+{
+    if (!a.box || a.split16 || a.dzero || d.ns < 1 || d.nring < 1 || d.s0 < 0 || 3 * static_cast<int64_t>(d.ns) > 65535)
+        return hipErrorInvalidValue;
+    const int64_t maxcols = 3 * static_cast<int64_t>(a.N) * a.N;
+    const int64_t ncells = static_cast<int64_t>(a.boxM[0]) * a.boxM[1] * a.boxM[2];
+    nb_box_steps_bform_kernel<<<dim3(static_cast<unsigned>((ncells * maxcols + 255) / 256), 1,
+                                     static_cast<unsigned>(3 * d.ns)),
+                                256, 0, s>>>(a, d, ncells);
+    box_steps_pass_m<0>(a, d, maxcols, s);
+    box_steps_pass_m<1>(a, d, maxcols, s);
     return hipGetLastError();
 }
 

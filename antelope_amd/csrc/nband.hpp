@@ -105,6 +105,20 @@ hipError_t nb_launch_box_t2(const NbArgs &a, const NbEval &ev, double2 *t2, hipS
 // [cells][bcols], t1 [M1][M2][K3][bcols] and output t2
 hipError_t nb_launch_box_t2_multi(const NbArgs &a, int nev, const NbEval *evs, double2 *const *b, double2 *const *t1,
                                   double2 *const *t2, hipStream_t s);
+// The separable sums up to the second pass for a run of RK4 steps [s0, s0 + ns) in three
+// launches (bform, the passes along a3 and a2), evaluation e = 3 sl + typ: typ 0 J at t_n
+// into tj + sl t2_stride, typ 1 H~ H at t_n + dt/2 into th + sl t2_stride, typ 2 H~ H at
+// t_n + dt into ring + (step mod nring) t2_stride (the pair of the step before s0 stays in the
+// ring when nring > ns); per evaluation scratch b + e b_stride, t1 + e t1_stride. laser:
+// [nt][3][6], A and E at t_n, t_n + dt/2, t_n + dt.
+struct NbBoxSteps {
+    const double *laser;
+    int64_t s0;
+    int ns, nring;
+    double2 *b, *t1, *tj, *th, *ring;
+    int64_t b_stride, t1_stride, t2_stride;  // elements
+};
+hipError_t nb_launch_box_steps(const NbArgs &a, const NbBoxSteps &d, hipStream_t s);
 // The same with the pass along a1 as well, each evaluation's matrices into out[e]
 // ([nmat][nk_stride][N][N]; the padding k points are not written)
 hipError_t nb_launch_box_mats_multi(const NbArgs &a, int nev, const NbEval *evs, double2 *const *b, double2 *const *t1,

@@ -199,6 +199,18 @@ bool nb32_dense_multi_n(int nr, int64_t nk) { return nr >= 1 && nr <= kNbDenseMu
 // N x N per k point and step), at least one, at most the pulse; ANTELOPE_NB_EVAL_STEPS
 // sets it (the results do not depend on it)
 constexpr size_t kNb32StepBytes = size_t(256) << 20;
+// steps whose separable passes (up to the second) one batch of launches evaluates for the
+// N <= 16 fused step: about 256 MB of second-pass outputs and scratch (three evaluations
+// per step), at least one, at most the pulse and 21845 (three evaluations a step in one grid
+// dimension); ANTELOPE_NB_EVAL_STEPS sets it (the results do not depend on it)
+int nb16_steps_per_eval(size_t t2, size_t b, size_t t1, int64_t nt)
+{
+    int64_t S = static_cast<int64_t>(kNb32StepBytes / (3 * (t2 + b + t1)));
+    if (const char *e = std::getenv("ANTELOPE_NB_EVAL_STEPS"))
+        if (std::atoi(e) > 0) S = std::atoi(e);
+    return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>({nt, S, 21845})));
+}
+
 int nb32_steps_per_eval(int64_t nk_stride, int N, int64_t nt)
 {
     const size_t per = 7 * static_cast<size_t>(nk_stride) * N * N * sizeof(double2);
@@ -282,12 +294,15 @@ NbPlan nb_plan(int N, int nr, int64_t nk, int64_t nt, int nstage, bool split16, 
         p.bytes_box = o;
     }
     p.bytes_fused = 0;
-    if (fused) {  // four second-pass outputs, the energies, and per evaluation B and the first pass
+    if (fused) {  // second-pass outputs of S steps (J, the midpoint pair, the end pairs in a ring
+        // of S + 1), the energies, per evaluation B and the first pass, A and E of the pulse
         const size_t col = sizeof(double2) * static_cast<size_t>(p.bcols);
         const size_t t2 = align256(col * bx.M[0] * bx.K[1] * bx.K[2]);
         const size_t b = align256(col * static_cast<size_t>(bx.cells()));
         const size_t t1 = align256(col * bx.M[0] * bx.M[1] * bx.K[2]);
-        p.bytes_fused = 4 * t2 + align256(static_cast<size_t>(p.nk_stride) * 16 * sizeof(double)) + 4 * (b + t1);
+        const size_t S = static_cast<size_t>(nb16_steps_per_eval(t2, b, t1, nt));
+        p.bytes_fused = (3 * S + 1) * t2 + align256(static_cast<size_t>(p.nk_stride) * 16 * sizeof(double)) +
+                        3 * S * (b + t1) + align256(static_cast<size_t>(nt) * 18 * sizeof(double));
     }
     if (nb32) {  // the energies, and with the separable sums per evaluation B and both passes
         p.bytes_fused = align256(static_cast<size_t>(p.nk_stride) * 32 * sizeof(double));
@@ -582,13 +597,33 @@ int nb_setup(sbe_ctx *ctx)
         const size_t t2 = align256(col * bx.M[0] * bx.K[1] * bx.K[2]);
         const size_t b = align256(col * static_cast<size_t>(bx.cells()));
         const size_t t1 = align256(col * bx.M[0] * bx.M[1] * bx.K[2]);
-        for (int i = 0; i < 4; ++i) ctx->nbf_t2[i] = reinterpret_cast<double2 *>(fbuf + i * t2);
-        ctx->nbf_lam = reinterpret_cast<double *>(fbuf + 4 * t2);
-        char *sc = fbuf + 4 * t2 + align256(static_cast<size_t>(p.nk_stride) * 16 * sizeof(double));
-        for (int i = 0; i < 4; ++i) {
-            ctx->nbf_b[i] = reinterpret_cast<double2 *>(sc + i * (b + t1));
-            ctx->nbf_t1[i] = reinterpret_cast<double2 *>(sc + i * (b + t1) + b);
+        const int64_t nt = ctx->laser.nt;
+        const int S = nb16_steps_per_eval(t2, b, t1, nt);
+        char *q = fbuf;
+        ctx->nbf_S = S;
+        ctx->nbf_t2_stride = static_cast<int64_t>(t2 / sizeof(double2));
+        ctx->nbf_tj = reinterpret_cast<double2 *>(q);
+        ctx->nbf_th = reinterpret_cast<double2 *>(q + S * t2);
+        ctx->nbf_ring = reinterpret_cast<double2 *>(q + 2 * S * t2);
+        q += (3 * static_cast<size_t>(S) + 1) * t2;
+        ctx->nbf_lam = reinterpret_cast<double *>(q);
+        q += align256(static_cast<size_t>(p.nk_stride) * 16 * sizeof(double));
+        ctx->nbf_bs = reinterpret_cast<double2 *>(q);
+        ctx->nbf_b_stride = static_cast<int64_t>(b / sizeof(double2));
+        q += 3 * static_cast<size_t>(S) * b;
+        ctx->nbf_t1s = reinterpret_cast<double2 *>(q);
+        ctx->nbf_t1_stride = static_cast<int64_t>(t1 / sizeof(double2));
+        q += 3 * static_cast<size_t>(S) * t1;
+        double *lt = reinterpret_cast<double *>(q);
+        std::vector<double> hl(static_cast<size_t>(nt) * 18);
+        for (int64_t n = 0; n < nt; ++n) {  // the times of the step loop's evaluations
+            const double tn = ctx->laser.atmin + ctx->laser.dt * static_cast<double>(n);  // antelope.cpp:299
+            const double ts[3] = {tn, tn + 0.5 * ctx->laser.dt, tn + ctx->laser.dt};
+            for (int j = 0; j < 3; ++j) ctx->laser.eval(ts[j], hl.data() + 18 * n + 6 * j, hl.data() + 18 * n + 6 * j + 3);
         }
+        rc = hipc(ctx, hipMemcpy(lt, hl.data(), hl.size() * sizeof(double), hipMemcpyHostToDevice), "upload laser");
+        if (rc) return rc;
+        ctx->nbf_laser = lt;
         ctx->nbf_nblocks = fused_blocks(bx, ctx->nbf_g1);
         ctx->nbf = true;
     }
@@ -771,66 +806,68 @@ int nb_run_fused(sbe_ctx *ctx, int64_t step_begin, int64_t step_end)
 {
     const NbArgs &a = ctx->nargs;
     const double dt = ctx->laser.dt;
+    const int nring = ctx->nbf_S + 1;
+    const int64_t t2s = ctx->nbf_t2_stride;
+    auto pair = [&](int64_t step) { return ctx->nbf_ring + ((step % nring + nring) % nring) * t2s; };
     int rc = SBE_OK;
-    auto eval = [&](double t, bool hmats) {
-        NbEval ev{};
-        ctx->laser.eval(t, ev.A, ev.E);
-        if (hmats) {
-            ev.nmat = 2;
-            ev.mats[0] = kMatHtilde;
-            ev.mats[1] = kMatH0;
-        } else {
-            ev.nmat = 3;
-            ev.mats[0] = kMatJx;
-            ev.mats[1] = kMatJy;
-            ev.mats[2] = kMatJz;
-        }
-        return ev;
-    };
     for (int64_t c0 = step_begin; c0 < step_end && !rc; c0 += ctx->chunk_steps) {
         const int64_t n = std::min<int64_t>(ctx->chunk_steps, step_end - c0);
-        for (int64_t s = 0; s < n && !rc; ++s) {
-            const double tn = ctx->laser.atmin + dt * static_cast<double>(c0 + s);  // antelope.cpp:299
-            // the Fourier sums of the step's times do not depend on rho: all before the step,
-            // in one batch of launches
-            NbEval evs[4];
-            double2 *t2s[4];
-            int nev = 0;
-            if (!(ctx->nbf_prev_valid && same_time(ctx->nbf_prev_time, tn, dt))) {
-                evs[nev] = eval(tn, true);
-                t2s[nev++] = ctx->nbf_t2[0];
+        for (int64_t s = 0; s < n && !rc;) {
+            // the separable sums of up to nbf_S steps' times (they do not depend on rho), then
+            // one step launch per step
+            const int ns = static_cast<int>(std::min<int64_t>(ctx->nbf_S, n - s));
+            const int64_t step0 = c0 + s;
+            const double t0 = ctx->laser.atmin + dt * static_cast<double>(step0);  // antelope.cpp:299
+            if (!(ctx->nbf_prev_valid && same_time(ctx->nbf_prev_time, t0, dt))) {  // H~, H at t_n of the first step
+                NbEval ev{};
+                ctx->laser.eval(t0, ev.A, ev.E);
+                ev.nmat = 2;
+                ev.mats[0] = kMatHtilde;
+                ev.mats[1] = kMatH0;
+                double2 *out = pair(step0 - 1);
+                rc = hipc(ctx, nb_launch_box_t2_multi(a, 1, &ev, &ctx->nbf_bs, &ctx->nbf_t1s, &out, ctx->stream),
+                          "fourier passes");
             }
-            evs[nev] = eval(tn, false);
-            t2s[nev++] = ctx->nbf_t2[1];
-            evs[nev] = eval(tn + 0.5 * dt, true);
-            t2s[nev++] = ctx->nbf_t2[2];
-            evs[nev] = eval(tn + dt, true);
-            t2s[nev++] = ctx->nbf_t2[3];
-            rc = hipc(ctx, nb_launch_box_t2_multi(a, nev, evs, ctx->nbf_b, ctx->nbf_t1, t2s, ctx->stream),
-                      "fourier passes");
-            if (rc) break;
-            NbFused f{};
-            f.t2_prev = ctx->nbf_t2[0];
-            f.t2_j = ctx->nbf_t2[1];
-            f.t2_half = ctx->nbf_t2[2];
-            f.t2_end = ctx->nbf_t2[3];
-            f.lam = ctx->nbf_lam;
-            f.u_valid = same_time(ctx->nb_u_time, tn, dt) ? 1 : 0;
-            f.lam_valid = f.u_valid && ctx->nbf_lam_valid && same_time(ctx->nbf_lam_time, tn, dt) ? 1 : 0;
-            f.dt = dt;
-            f.step_local = s;
-            f.g1 = ctx->nbf_g1;
-            f.nblocks = ctx->nbf_nblocks;
-            rc = hipc(ctx, nb16_launch_step(a, f, ctx->nb_rho, ctx->nb_U, ctx->d_partials, ctx->stream), "step kernel");
-            // the end of this step is the start of the next: its H~, H serve there
-            std::swap(ctx->nbf_t2[0], ctx->nbf_t2[3]);
-            ctx->nbf_prev_time = tn + dt;
-            ctx->nbf_prev_valid = true;
-            ctx->nb_u_time = tn + dt;
-            ctx->nbf_lam_time = tn + dt;
-            ctx->nbf_lam_valid = true;
-            ctx->nb_ht_valid = false;  // the separate kernels' matrices are not kept up to date
-            ctx->nb_h_valid = false;
+            NbBoxSteps d{};
+            d.laser = ctx->nbf_laser;
+            d.s0 = step0;
+            d.ns = ns;
+            d.nring = nring;
+            d.b = ctx->nbf_bs;
+            d.t1 = ctx->nbf_t1s;
+            d.tj = ctx->nbf_tj;
+            d.th = ctx->nbf_th;
+            d.ring = ctx->nbf_ring;
+            d.b_stride = ctx->nbf_b_stride;
+            d.t1_stride = ctx->nbf_t1_stride;
+            d.t2_stride = t2s;
+            if (!rc) rc = hipc(ctx, nb_launch_box_steps(a, d, ctx->stream), "fourier passes");
+            for (int i = 0; i < ns && !rc; ++i) {
+                const int64_t step = step0 + i;
+                const double tn = ctx->laser.atmin + dt * static_cast<double>(step);
+                NbFused f{};
+                f.t2_prev = pair(step - 1);
+                f.t2_j = ctx->nbf_tj + i * t2s;
+                f.t2_half = ctx->nbf_th + i * t2s;
+                f.t2_end = pair(step);
+                f.lam = ctx->nbf_lam;
+                f.u_valid = same_time(ctx->nb_u_time, tn, dt) ? 1 : 0;
+                f.lam_valid = f.u_valid && ctx->nbf_lam_valid && same_time(ctx->nbf_lam_time, tn, dt) ? 1 : 0;
+                f.dt = dt;
+                f.step_local = s + i;
+                f.g1 = ctx->nbf_g1;
+                f.nblocks = ctx->nbf_nblocks;
+                rc = hipc(ctx, nb16_launch_step(a, f, ctx->nb_rho, ctx->nb_U, ctx->d_partials, ctx->stream), "step kernel");
+                // the end of this step is the start of the next: its H~, H serve there
+                ctx->nbf_prev_time = tn + dt;
+                ctx->nbf_prev_valid = true;
+                ctx->nb_u_time = tn + dt;
+                ctx->nbf_lam_time = tn + dt;
+                ctx->nbf_lam_valid = true;
+                ctx->nb_ht_valid = false;  // the separate kernels' matrices are not kept up to date
+                ctx->nb_h_valid = false;
+            }
+            s += ns;
         }
         if (!rc)
             rc = hipc(ctx, launch_series_reduce(ctx->d_partials, n, ctx->nbf_nblocks, a.nv, ctx->d_series + c0 * a.nv,

@@ -178,12 +178,17 @@ struct sbe_ctx {
     bool nb_vel_p_lds = false;   // velocity gauge: pMatrix of the k point staged in LDS
     // fused RK4 steps (nb16_step_kernel): second-pass outputs of the separable sums
     bool nbf = false;
-    double2 *nbf_t2[4] = {nullptr, nullptr, nullptr, nullptr};  // H~ H at t_n, J at t_n, H~ H at t + dt/2, at t + dt
+    // second-pass outputs of up to nbf_S steps: J at t_n and H~ H at t_n + dt/2 per step, the
+    // end pairs H~ H at t_n + dt in a ring of nbf_S + 1 (step mod ring)
+    int nbf_S = 0;
+    double2 *nbf_tj = nullptr, *nbf_th = nullptr, *nbf_ring = nullptr;
+    int64_t nbf_t2_stride = 0;
     double *nbf_lam = nullptr;   // [nk_stride][16] band energies at the stored eigenvectors' time
-    double2 *nbf_b[4] = {nullptr, nullptr, nullptr, nullptr};   // per evaluation: B on the box
-    double2 *nbf_t1[4] = {nullptr, nullptr, nullptr, nullptr};  // per evaluation: first-pass output
+    double2 *nbf_bs = nullptr, *nbf_t1s = nullptr;  // per evaluation: B on the box, first-pass output
+    int64_t nbf_b_stride = 0, nbf_t1_stride = 0;
+    const double *nbf_laser = nullptr;  // [nt][3][6] A, E at t_n, t_n + dt/2, t_n + dt
     int nbf_g1 = 0, nbf_nblocks = 0;
-    bool nbf_prev_valid = false;  // nbf_t2[0] holds H~, H at nbf_prev_time
+    bool nbf_prev_valid = false;  // the ring holds H~, H at nbf_prev_time for the step that starts there
     double nbf_prev_time = 0;
     bool nbf_lam_valid = false;
     double nbf_lam_time = 0;

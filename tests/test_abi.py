@@ -150,12 +150,16 @@ def test_separable_sums_footprint_without_gpu():
     nk_stride, nr = (K[0] * K[1] * K[2] + 63) // 64 * 64, cells  # the dense e^{ik.R} table is not kept
     want -= nk_stride * nr * 16 + (nr + 31) // 32 * 32 * col
     assert _footprint_env(cfg, m, dense=False) - _footprint_env(cfg, m, dense=True) == want
-    # the fused RK4 step adds four second-pass outputs [M1][K2][K3][columns], the band
-    # energies [nk_stride][16] (FP64) and, for each of up to four evaluations batched in one
-    # launch, B on the box and a first-pass output; its 10 partial rows per step fit in the
-    # 60 the sample kernel has
-    fused = (4 * al(M[0] * K[1] * K[2] * col) + al(nk_stride * 16 * 8)
-             + 4 * (al(cells * col) + al(M[0] * M[1] * K[2] * col)))
+    # the fused RK4 step evaluates the separable sums of S steps per batch of launches: it
+    # adds their second-pass outputs [M1][K2][K3][columns] (J and the midpoint pair per step,
+    # the end pairs in a ring of S + 1), the band energies [nk_stride][16] (FP64), B on the
+    # box and a first-pass output per evaluation (three a step), and A, E at the three times
+    # of every step; S for about 256 MB of them; its 10 partial rows per step fit in the 60
+    # the sample kernel has
+    nt = laser_eval(cfg, [0.0])[3]
+    t2, b, t1 = al(M[0] * K[1] * K[2] * col), al(cells * col), al(M[0] * M[1] * K[2] * col)
+    S = max(1, min(nt, 21845, (256 << 20) // (3 * (t2 + b + t1))))
+    fused = (3 * S + 1) * t2 + al(nk_stride * 16 * 8) + 3 * S * (b + t1) + al(nt * 18 * 8)
     assert _footprint_env(cfg, m, dense=False, fused=True) - _footprint_env(cfg, m, dense=False) == fused
     assert _footprint_env(cfg, m, dense=True, fused=True) == _footprint_env(cfg, m, dense=True)
     rk5 = dataclasses.replace(cfg, rk_order=5)  # the fused step is the classical RK4 tableau only

@@ -72,7 +72,9 @@ def test_fused_step_against_oracle_and_separate_kernels(oracle, monkeypatch, lab
 
 
 def test_fused_split_runs_and_state_reads_are_bitwise(monkeypatch):
-    """Runs split anywhere, with a band-basis state read in between, give the bits of one run."""
+    """Runs split anywhere, with a band-basis state read in between, give the bits of one run;
+    so does any number of steps whose separable passes run in one batch of launches (1 and
+    7: the ring of end pairs wraps)."""
     model = synthetic_wannier(8, nmax=(2, 2, 1), seed=9)
     cfg = _cfg((6, 5, 2), ncycles=0.5)
     print("case: split runs", flush=True)
@@ -84,6 +86,12 @@ def test_fused_split_runs_and_state_reads_are_bitwise(monkeypatch):
     for k in one:
         assert np.array_equal(one[k], two[k]), k
     assert np.array_equal(st1, st2)
+    for S in ("1", "7"):
+        monkeypatch.setenv("ANTELOPE_NB_EVAL_STEPS", S)
+        three, st3, _ = _solve(cfg, model, monkeypatch, fused=True, splits=(37,))
+        for k in one:
+            assert np.array_equal(one[k], three[k]), (S, k)
+        assert np.array_equal(st1, st3)
 
 
 def test_fused_rk5_keeps_separate_kernels(oracle, monkeypatch):
