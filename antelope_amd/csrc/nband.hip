@@ -71,21 +71,18 @@ __global__ void nb_phase_kernel(const NbArgs a, cd *__restrict__ P)
 // --------------------------------------------------------- Fourier GEMM --
 constexpr int GM = 64, GN = 64, GK = 32;
 
-// B element (row r, column c) of the evaluation: c = mat * N^2 + ij
-__device__ __forceinline__ cd b_element(const NbArgs &a, const NbEval &ev, int r, int64_t c, cd phase)
+// B element of matrix kind `type` (row r, element ij) at field E
+__device__ __forceinline__ cd b_value(const NbArgs &a, const double E[3], int type, int r, int ij, cd phase)
 {
     const int nn = a.N * a.N;
-    const int mat = static_cast<int>(c / nn);
-    const int ij = static_cast<int>(c % nn);
-    const int type = ev.mats[mat];
     const cd h = a.ham[static_cast<int64_t>(r) * nn + ij];
     cd x;
     if (type == kMatHtilde) {
         x = h;
         const cd *p = a.pos + static_cast<int64_t>(r) * 3 * nn + ij;
-        x = cadd(x, cscale(p[0], ev.E[0]));
-        x = cadd(x, cscale(p[nn], ev.E[1]));
-        x = cadd(x, cscale(p[2 * nn], ev.E[2]));
+        x = cadd(x, cscale(p[0], E[0]));
+        x = cadd(x, cscale(p[nn], E[1]));
+        x = cadd(x, cscale(p[2 * nn], E[2]));
     } else if (type == kMatH0) {
         x = h;
     } else {  // J_a = -i R_a H (Wannier90.h:373-384)
@@ -93,6 +90,14 @@ __device__ __forceinline__ cd b_element(const NbArgs &a, const NbEval &ev, int r
         x = cmk(h.y * Ra, -h.x * Ra);
     }
     return cmul(cscale(phase, a.rweight[r]), x);
+}
+
+// B element (row r, column c) of the evaluation: c = mat * N^2 + ij
+__device__ __forceinline__ cd b_element(const NbArgs &a, const NbEval &ev, int r, int64_t c, cd phase)
+{
+    const int nn = a.N * a.N;
+    const int mat = static_cast<int>(c / nn);
+    return b_value(a, ev.E, ev.mats[mat], r, static_cast<int>(c % nn), phase);
 }
 
 typedef double d4 __attribute__((ext_vector_type(4)));
@@ -354,7 +359,8 @@ __global__ void nb_box_steps_bform_kernel(const NbArgs a, const NbBoxSteps d, in
     const int e = blockIdx.z;
     cd *t2;
     const NbEval ev = box_step_eval(d, e, t2);
-    const int64_t ncols = static_cast<int64_t>(ev.nmat) * a.N * a.N;
+    const int nn = a.N * a.N;
+    const int64_t ncols = static_cast<int64_t>(ev.nmat) * nn;
     const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (idx >= ncells * ncols) return;
     const int64_t cell = idx / ncols, c = idx % ncols;
@@ -364,7 +370,10 @@ __global__ void nb_box_steps_bform_kernel(const NbArgs a, const NbBoxSteps d, in
         const double *R = a.rcart + 3 * r;
         double sn, cs;
         sincos(ev.A[0] * R[0] + ev.A[1] * R[1] + ev.A[2] * R[2], &sn, &cs);
-        v = b_element(a, ev, r, c, cmk(cs, sn));
+        // the kind without indexing the evaluation's list (J_x, J_y, J_z; H~, H)
+        const int mat = static_cast<int>(c / nn);
+        const int type = ev.nmat == 3 ? kMatJx + mat : (mat == 0 ? kMatHtilde : kMatH0);
+        v = b_value(a, ev.E, type, r, static_cast<int>(c - static_cast<int64_t>(mat) * nn), cmk(cs, sn));
     }
     (d.b + e * d.b_stride)[cell * a.bcols + c] = v;
 }
@@ -1642,11 +1651,13 @@ __global__ __launch_bounds__(256) void nb_dense_steps_kernel(const NbArgs a, con
     const int64_t ncols = static_cast<int64_t>(ev.nmat) * nn;
     const int64_t c = static_cast<int64_t>(blockIdx.y) * 64 + lane;
     const bool live = c < ncols;
-    cd x[MM];
-#pragma unroll
-    for (int m = 0; m < MM; ++m) x[m] = live && m < nr ? b_element(a, ev, m, c, phase[m]) : cmk(0.0, 0.0);
     const int mat = live ? static_cast<int>(c / nn) : 0;
     const int ij = live ? static_cast<int>(c - static_cast<int64_t>(mat) * nn) : 0;
+    // the kind without indexing the evaluation's list (J_x, J_y, J_z; H~, H)
+    const int type = typ == 0 ? kMatJx + mat : (mat == 0 ? kMatHtilde : kMatH0);
+    cd x[MM];
+#pragma unroll
+    for (int m = 0; m < MM; ++m) x[m] = live && m < nr ? b_value(a, ev.E, type, m, ij, phase[m]) : cmk(0.0, 0.0);
     cd *o = out + static_cast<int64_t>(mat) * a.nk_stride * nn + ij;
     for (int64_t k0 = 0; k0 < a.nk; k0 += kDenseKTile) {
         const int kt = static_cast<int>(min<int64_t>(kDenseKTile, a.nk - k0));
