@@ -80,7 +80,7 @@ struct NbStage {
 // (along a1) runs inside the step kernel, from the second pass's output of
 // each evaluation time, so the matrices never reach HBM. t2_* are that output,
 // [M1][K2][K3][bcols] with the evaluation's matrices side by side in the columns.
-constexpr int kNbFusedWaves = 8;  // k points along a1 per block (one wave each)
+constexpr int kNbFusedWaves = 8;  // k points along a1 per block (one wave each); 4 on small grids
 constexpr int kNbFusedMaxM1 = 7;  // lattice vectors along a1 staged in LDS
 struct NbFused {
     const double2 *t2_prev;  // H~, H at t_n
@@ -92,11 +92,13 @@ struct NbFused {
     int lam_valid;           // lam belongs to them
     double dt;
     int64_t step_local;      // row block of the partial sums
-    int g1;                  // blocks per (k2, k3) column: ceil(K1 / kNbFusedWaves)
+    int g1;                  // blocks per (k2, k3) column: ceil(K1 / waves)
     int nblocks;             // g1 K2 K3: partial rows per step
+    int waves;               // k points along a1 per block: 8, or 4 where 8 gives fewer blocks than CUs
 };
-// bytes of the block's LDS (N bands, M1 lattice vectors along a1, nv values per sample row)
-size_t nb16_step_lds_bytes(int N, int M1, int nv);
+// bytes of the block's LDS (N bands, M1 lattice vectors along a1, nv values per sample row,
+// waves per block)
+size_t nb16_step_lds_bytes(int N, int M1, int nv, int waves);
 hipError_t nb16_launch_step(const NbArgs &a, const NbFused &f, double2 *rho, double2 *U, double *partials,
                             hipStream_t s);
 // the separable sums of one evaluation up to the second pass: t2 [M1][K2][K3][bcols]

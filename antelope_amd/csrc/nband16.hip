@@ -488,8 +488,9 @@ __global__ __launch_bounds__(64) void nb16_vel_step_kernel(const NbArgs a, const
 // ------------------------------------------------- fused RK4 step ----
 // One RK4 step per launch (SBEs.h:596-624 with the stage right-hand side of
 // SBEs.h:762-861 and the sample of SBEs.h:694-760), for a product k grid with
-// separable Fourier sums. Block = kNbFusedWaves k points that share (k2, k3)
-// and differ along a1, one wave each. The sums' last pass (along a1),
+// separable Fourier sums. Block = W k points that share (k2, k3) and differ
+// along a1, one wave each (W = 8; W = 4 where the grid has fewer blocks than
+// CUs, so each wave has a SIMD of its own). The sums' last pass (along a1),
 //   X(k1, k2, k3) = sum_m E1[k1][m] T2[m][k2][k3],
 // runs here: the block stages T2's M1 rows of its column for two matrices at a
 // time in LDS and every wave forms its own tiles from them, so H~, H and J of
@@ -498,22 +499,25 @@ __global__ __launch_bounds__(64) void nb16_vel_step_kernel(const NbArgs a, const
 // band energies from the last step), stage 1 at t_n, stages 2 and 3 at
 // t_n + dt/2 (eigenvectors refined there), stage 4 at t_n + dt (eigenvectors
 // refined for the next sample); rho, U and the energies are the only per-k
-// HBM traffic. The eight waves' sample sums are added in LDS in wave order
-// into one partial row per block.
+// HBM traffic. The W waves' sample sums are added in LDS in wave order into
+// one partial row per block.
 // dynamic LDS of a block: the waves' transpose buffers, then two buffers of two
 // staging slots of kNbFusedMaxM1 x 256 complex each (row m of the second-pass output
 // at m * 256, so the m offsets are immediates). While a block computes from one
 // buffer, the other fills straight from global memory (global_load_lds); a buffer
-// also holds the block's sample sums (kNbFusedWaves nv doubles) once read.
+// also holds the block's sample sums (W nv doubles) once read.
 constexpr int kFusedPitch = 256;
 constexpr int64_t kFusedSlot = static_cast<int64_t>(kNbFusedMaxM1) * kFusedPitch;  // complex per slot
-constexpr int64_t kFusedTb = static_cast<int64_t>(kNbFusedWaves) * 16 * kTpad;    // complex of the tb
+template <int W>
+constexpr int64_t fused_tb() { return static_cast<int64_t>(W) * 16 * kTpad; }  // complex of the tb
 
-__device__ __forceinline__ cd *fused_slot(cd *lds, int buf, int slot) { return lds + kFusedTb + (2 * buf + slot) * kFusedSlot; }
+template <int W>
+__device__ __forceinline__ cd *fused_slot(cd *lds, int buf, int slot) { return lds + fused_tb<W>() + (2 * buf + slot) * kFusedSlot; }
 
 // This wave's share of filling buffer buf with matrix j0 of src0 and j1 of src1 (src1
 // may be null), rows m < M1 of column col: chunks of 64 complex go straight to LDS,
 // each lane one element; their completion is counted by vmcnt (fused_wait).
+template <int W>
 __device__ __forceinline__ void stage_async(const NbArgs &a, cd *lds, int buf, const cd *src0, int j0, const cd *src1,
                                             int j1, int64_t col, int w, int lane)
 {
@@ -523,14 +527,14 @@ __device__ __forceinline__ void stage_async(const NbArgs &a, cd *lds, int buf, c
     const int64_t cbase = col * a.bcols;
     const int per = M1 * Q;
     const int total = src1 ? 2 * per : per;
-    for (int ch = w; ch < total; ch += kNbFusedWaves) {  // uniform over the wave
+    for (int ch = w; ch < total; ch += W) {  // uniform over the wave
         const int slot = ch >= per ? 1 : 0;
         const int rem = ch - slot * per;
         const int m = rem / Q, q = rem - m * Q;
         const cd *src = slot ? src1 : src0;
         const int j = slot ? j1 : j0;
         const int ij = q * 64 + lane;
-        cd *dst = lds + kFusedTb + (2 * buf + slot) * kFusedSlot + m * kFusedPitch + q * 64;
+        cd *dst = lds + fused_tb<W>() + (2 * buf + slot) * kFusedSlot + m * kFusedPitch + q * 64;
         if (ij < nn) __builtin_amdgcn_global_load_lds(src + m * mstride + cbase + j * nn + ij, dst, 16, 0, 0);
     }
 }
@@ -599,7 +603,8 @@ __device__ __forceinline__ void axpy_tile(Tile &Y, const Tile &X, double s, cons
 // round fewer than 1e-8 where the first round turns by ~1e-4
 constexpr double kFusedSmallK = 1e-6;
 
-__global__ __launch_bounds__(64 * kNbFusedWaves) __attribute__((amdgpu_waves_per_eu(2))) void nb16_step_kernel(
+template <int W>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2))) void nb16_step_kernel(
     const NbArgs a, const NbFused f, cd *__restrict__ rho, cd *__restrict__ Ust, double *__restrict__ partials)
 {
     extern __shared__ cd step_lds[];
@@ -613,13 +618,13 @@ __global__ __launch_bounds__(64 * kNbFusedWaves) __attribute__((amdgpu_waves_per
     const int lin = nb % 8 == 0 ? (b % 8) * (nb / 8) + b / 8 : b;
     const int col = lin / f.g1, grp = lin - col * f.g1;
     const int k2 = col / K3, k3 = col - k2 * K3;
-    const int k1 = grp * kNbFusedWaves + w;
+    const int k1 = grp * W + w;
     const bool active = k1 < K1;
     const int64_t k = active ? a.box_kinv[(static_cast<int64_t>(k1) * K2 + k2) * K3 + k3] : 0;
     cd *buf = step_lds + w * 16 * kTpad;
     const double dt = f.dt;
     // this wave's a1 factors E1[k1][m] in LDS (read as broadcasts by every tile formed)
-    cd *e1 = step_lds + kFusedTb + 4 * kFusedSlot + w * kNbFusedMaxM1;
+    cd *e1 = step_lds + fused_tb<W>() + 4 * kFusedSlot + w * kNbFusedMaxM1;
     if (lane < M1) e1[lane] = a.boxE[0][static_cast<int64_t>(active ? k1 : 0) * M1 + lane];
 
     Tile R, U, Ut, Y, Acc, Ht, X;
@@ -631,8 +636,8 @@ __global__ __launch_bounds__(64 * kNbFusedWaves) __attribute__((amdgpu_waves_per
     const bool need_h = !f.u_valid || !f.lam_valid;
     // staging order: [H(t_n) if needed] [J_x J_y] [J_z H~(t_n)] [H~ H at t + dt/2] [H~ H at t + dt],
     // buffers alternating, each filled while the previous one is read
-    if (need_h) stage_async(a, step_lds, 1, f.t2_prev, 1, nullptr, 0, col, w, lane);
-    stage_async(a, step_lds, 0, f.t2_j, 0, f.t2_j, 1, col, w, lane);
+    if (need_h) stage_async<W>(a, step_lds, 1, f.t2_prev, 1, nullptr, 0, col, w, lane);
+    stage_async<W>(a, step_lds, 0, f.t2_j, 0, f.t2_j, 1, col, w, lane);
     if (active) {
         R = load_tile(rho + k * nn, N);
         U = load_tile(Ust + k * nn, N, true);
@@ -642,7 +647,7 @@ __global__ __launch_bounds__(64 * kNbFusedWaves) __attribute__((amdgpu_waves_per
     if (need_h) {  // eigenvectors / energies at t_n from H(t_n)
         fused_wait();
         if (active) {
-            const Tile H = form_tile(fused_slot(step_lds, 1, 0), e1, M1, N);
+            const Tile H = form_tile(fused_slot<W>(step_lds, 1, 0), e1, M1, N);
             if (!f.u_valid) eig_refine(H, U, Ut, N, buf);
             lam = col_dot(U, cmm(H, true, U));
         }
@@ -687,17 +692,17 @@ __global__ __launch_bounds__(64 * kNbFusedWaves) __attribute__((amdgpu_waves_per
         cur[3 + ax] += intra * wk;
     };
     fused_wait();  // [J_x J_y] in buffer 0; buffer 1 read (H(t_n))
-    stage_async(a, step_lds, 1, f.t2_j, 2, f.t2_prev, 0, col, w, lane);
+    stage_async<W>(a, step_lds, 1, f.t2_j, 2, f.t2_prev, 0, col, w, lane);
     if (active) {
-        current(0, form_tile(fused_slot(step_lds, 0, 0), e1, M1, N));
-        current(1, form_tile(fused_slot(step_lds, 0, 1), e1, M1, N));
+        current(0, form_tile(fused_slot<W>(step_lds, 0, 0), e1, M1, N));
+        current(1, form_tile(fused_slot<W>(step_lds, 0, 1), e1, M1, N));
     }
     // ---- stage 1 at t_n (H~ of t_n), with J_z
     fused_wait();  // [J_z H~] in buffer 1; buffer 0 read
-    stage_async(a, step_lds, 0, f.t2_half, 0, f.t2_half, 1, col, w, lane);
+    stage_async<W>(a, step_lds, 0, f.t2_half, 0, f.t2_half, 1, col, w, lane);
     if (active) {
-        current(2, form_tile(fused_slot(step_lds, 1, 0), e1, M1, N));
-        Ht = form_tile(fused_slot(step_lds, 1, 1), e1, M1, N);  // H~ (Hermitian: its conjugate is H~^T's tile)
+        current(2, form_tile(fused_slot<W>(step_lds, 1, 0), e1, M1, N));
+        Ht = form_tile(fused_slot<W>(step_lds, 1, 1), e1, M1, N);  // H~ (Hermitian: its conjugate is H~^T's tile)
         X = stage_derivative_m<true>(a, R, Ht, M, buf);
         axpy_tile(Acc, X, dt / 6.0, R);
         axpy_tile(Y, X, 0.5 * dt, R);
@@ -705,7 +710,7 @@ __global__ __launch_bounds__(64 * kNbFusedWaves) __attribute__((amdgpu_waves_per
     // ---- the block's sample sums in buffer 1 (read): one partial row, waves added in order
     __syncthreads();
     {
-        double *red = reinterpret_cast<double *>(fused_slot(step_lds, 1, 0));  // [kNbFusedWaves][nv]
+        double *red = reinterpret_cast<double *>(fused_slot<W>(step_lds, 1, 0));  // [W][nv]
         double *mine = red + w * a.nv;
         if (lane < 6) {
             double v = 0.0;
@@ -727,18 +732,18 @@ __global__ __launch_bounds__(64 * kNbFusedWaves) __attribute__((amdgpu_waves_per
         double *row = partials + (f.step_local * nb + b) * static_cast<int64_t>(a.nv);
         for (int v = threadIdx.x; v < a.nv; v += blockDim.x) {
             double sum = 0.0;
-            for (int q = 0; q < kNbFusedWaves; ++q) sum += red[q * a.nv + v];
+            for (int q = 0; q < W; ++q) sum += red[q * a.nv + v];
             row[v] = sum;
         }
     }
     // ---- stages 2 and 3 at t_n + dt/2
     fused_wait();  // [H~ H at t + dt/2] in buffer 0; buffer 1 read (sample sums)
-    stage_async(a, step_lds, 1, f.t2_end, 0, f.t2_end, 1, col, w, lane);
+    stage_async<W>(a, step_lds, 1, f.t2_end, 0, f.t2_end, 1, col, w, lane);
     if (active) {
-        Ht = form_tile(fused_slot(step_lds, 0, 0), e1, M1, N);
+        Ht = form_tile(fused_slot<W>(step_lds, 0, 0), e1, M1, N);
         // within the step only: the last round at max|K| < 1e-6 and no final Newton-Schulz step
         if (a.deph)
-            eig_refine(form_tile(fused_slot(step_lds, 0, 1), e1, M1, N), U, Ut, N, buf, kFusedSmallK, false, true);
+            eig_refine(form_tile(fused_slot<W>(step_lds, 0, 1), e1, M1, N), U, Ut, N, buf, kFusedSmallK, false, true);
         Tile Mh{};
         double dd;
         if (a.deph) Mh = band_diag(Y, U, Ut, dd);
@@ -761,8 +766,8 @@ __global__ __launch_bounds__(64 * kNbFusedWaves) __attribute__((amdgpu_waves_per
     // ---- stage 4 at t_n + dt; eigenvectors and energies for the next sample
     fused_wait();  // [H~ H at t + dt] in buffer 1
     if (active) {
-        Ht = form_tile(fused_slot(step_lds, 1, 0), e1, M1, N);
-        const Tile H = form_tile(fused_slot(step_lds, 1, 1), e1, M1, N);
+        Ht = form_tile(fused_slot<W>(step_lds, 1, 0), e1, M1, N);
+        const Tile H = form_tile(fused_slot<W>(step_lds, 1, 1), e1, M1, N);
         eig_refine(H, U, Ut, N, buf, kFusedSmallK, true, true, &lam);
         Tile Me{};
         double dd;
@@ -961,24 +966,33 @@ static hipError_t allow_lds(const void *kernel, size_t lds)
     return hipSuccess;
 }
 
-size_t nb16_step_lds_bytes(int N, int M1, int nv)
+size_t nb16_step_lds_bytes(int N, int M1, int nv, int waves)
 {
     (void)N;
     (void)M1;
     // the sample sums must fit in one buffer (two slots)
-    if (static_cast<int64_t>(kNbFusedWaves) * nv > 4 * kFusedSlot) return ~size_t(0);
-    return sizeof(double2) * static_cast<size_t>(kFusedTb + 4 * kFusedSlot + kNbFusedWaves * kNbFusedMaxM1);
+    if (static_cast<int64_t>(waves) * nv > 2 * kFusedSlot * 2) return ~size_t(0);
+    return sizeof(double2) * static_cast<size_t>(static_cast<int64_t>(waves) * 16 * kTpad + 4 * kFusedSlot +
+                                                 static_cast<int64_t>(waves) * kNbFusedMaxM1);
+}
+
+template <int W>
+static hipError_t launch_step_w(const NbArgs &a, const NbFused &f, double2 *rho, double2 *U, double *partials,
+                                hipStream_t s)
+{
+    const size_t lds = nb16_step_lds_bytes(a.N, a.boxM[0], a.nv, W);
+    const hipError_t e = allow_lds(reinterpret_cast<const void *>(&nb16_step_kernel<W>), lds);
+    if (e != hipSuccess) return e;
+    nb16_step_kernel<W><<<dim3(static_cast<unsigned>(f.nblocks)), 64 * W, lds, s>>>(a, f, rho, U, partials);
+    return hipGetLastError();
 }
 
 hipError_t nb16_launch_step(const NbArgs &a, const NbFused &f, double2 *rho, double2 *U, double *partials,
                             hipStream_t s)
 {
-    if (a.N > 16 || !a.box || a.boxM[0] > kNbFusedMaxM1 || f.nblocks < 1) return hipErrorInvalidValue;
-    const size_t lds = nb16_step_lds_bytes(a.N, a.boxM[0], a.nv);
-    const hipError_t e = allow_lds(reinterpret_cast<const void *>(&nb16_step_kernel), lds);
-    if (e != hipSuccess) return e;
-    nb16_step_kernel<<<dim3(static_cast<unsigned>(f.nblocks)), 64 * kNbFusedWaves, lds, s>>>(a, f, rho, U, partials);
-    return hipGetLastError();
+    if (a.N > 16 || !a.box || a.boxM[0] > kNbFusedMaxM1 || f.nblocks < 1 || (f.waves != 4 && f.waves != 8))
+        return hipErrorInvalidValue;
+    return f.waves == 4 ? launch_step_w<4>(a, f, rho, U, partials, s) : launch_step_w<8>(a, f, rho, U, partials, s);
 }
 
 size_t nb16_vel_fused_lds_bytes(int nv)

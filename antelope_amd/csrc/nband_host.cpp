@@ -220,9 +220,20 @@ int nb32_steps_per_eval(int64_t nk_stride, int N, int64_t nt)
     return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(nt, S)));
 }
 
+// k points along a1 per block of the fused N <= 16 step: 8, or 4 where 8 would give fewer
+// blocks than the GPU has CUs (small grids: a wave per SIMD shortens the step, whose time is
+// then one k point's chain of products)
+constexpr int kNbCUs = 256;
+int fused_waves(const BoxShape &bx)
+{
+    const int64_t b8 = static_cast<int64_t>((bx.K[0] + 7) / 8) * bx.K[1] * bx.K[2];
+    return b8 < kNbCUs ? 4 : 8;
+}
+
 int fused_blocks(const BoxShape &bx, int &g1)
 {
-    g1 = (bx.K[0] + kNbFusedWaves - 1) / kNbFusedWaves;
+    const int w = fused_waves(bx);
+    g1 = (bx.K[0] + w - 1) / w;
     return g1 * bx.K[1] * bx.K[2];
 }
 
@@ -592,7 +603,7 @@ int nb_setup(sbe_ctx *ctx)
     // fused RK4 steps: the planned grid must have grouped into the whole box (every node one k point)
     ctx->nbf = false;
     if (fbuf && a.box && a.tile16 && a.boxK[0] == bx.K[0] && a.boxK[1] == bx.K[1] && a.boxK[2] == bx.K[2] &&
-        nk == bx.kcells() && nb16_step_lds_bytes(N, bx.M[0], a.nv) <= (size_t(160) << 10)) {
+        nk == bx.kcells() && nb16_step_lds_bytes(N, bx.M[0], a.nv, fused_waves(bx)) <= (size_t(160) << 10)) {
         const size_t col = sizeof(double2) * static_cast<size_t>(p.bcols);
         const size_t t2 = align256(col * bx.M[0] * bx.K[1] * bx.K[2]);
         const size_t b = align256(col * static_cast<size_t>(bx.cells()));
@@ -625,6 +636,7 @@ int nb_setup(sbe_ctx *ctx)
         if (rc) return rc;
         ctx->nbf_laser = lt;
         ctx->nbf_nblocks = fused_blocks(bx, ctx->nbf_g1);
+        ctx->nbf_waves = fused_waves(bx);
         ctx->nbf = true;
     }
     ctx->nbf_prev_valid = false;
@@ -857,6 +869,7 @@ int nb_run_fused(sbe_ctx *ctx, int64_t step_begin, int64_t step_end)
                 f.step_local = s + i;
                 f.g1 = ctx->nbf_g1;
                 f.nblocks = ctx->nbf_nblocks;
+                f.waves = ctx->nbf_waves;
                 rc = hipc(ctx, nb16_launch_step(a, f, ctx->nb_rho, ctx->nb_U, ctx->d_partials, ctx->stream), "step kernel");
                 // the end of this step is the start of the next: its H~, H serve there
                 ctx->nbf_prev_time = tn + dt;

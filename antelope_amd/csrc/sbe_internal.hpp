@@ -187,7 +187,7 @@ struct sbe_ctx {
     double2 *nbf_bs = nullptr, *nbf_t1s = nullptr;  // per evaluation: B on the box, first-pass output
     int64_t nbf_b_stride = 0, nbf_t1_stride = 0;
     const double *nbf_laser = nullptr;  // [nt][3][6] A, E at t_n, t_n + dt/2, t_n + dt
-    int nbf_g1 = 0, nbf_nblocks = 0;
+    int nbf_g1 = 0, nbf_nblocks = 0, nbf_waves = 8;
     bool nbf_prev_valid = false;  // the ring holds H~, H at nbf_prev_time for the step that starts there
     double nbf_prev_time = 0;
     bool nbf_lam_valid = false;

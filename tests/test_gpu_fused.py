@@ -50,10 +50,13 @@ def _solve(cfg, model, monkeypatch, fused, splits=None, peek=None):
 
 @pytest.mark.parametrize("label,N,npts,kw", [
     ("n8_box5x5x3", 8, (6, 5, 2), dict()),
+    ("n8_8x16x16_eight_waves_per_block", 8, (8, 16, 16), dict(ncycles=0.25)),
     ("n16_ragged_a1_no_dephasing_total_current", 16, (12, 3, 2), dict(T2=-1.0, inter_intra=False)),
     ("n3_one_wave_per_block_used", 3, (5, 4, 3), dict()),
 ])
 def test_fused_step_against_oracle_and_separate_kernels(oracle, monkeypatch, label, N, npts, kw):
+    """Blocks of 4 waves on grids that give fewer blocks of 8 than CUs (the small cases),
+    of 8 on the 8 x 16 x 16 grid (256 blocks)."""
     model = synthetic_wannier(N, nmax=(2, 2, 1) if N <= 8 else (1, 1, 1), seed=40 + N)
     cfg = _cfg(npts, **kw)
     print(f"case: {label}", flush=True)
