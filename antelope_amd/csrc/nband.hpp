@@ -171,9 +171,10 @@ struct NbVelFused {
     const double *A;  // [nt][4][3]: A(t_n + c_s dt)
     int64_t s0, s1;
     double dt;
-    int nblocks;      // partial rows per step: ceil(nk / kNbFusedWaves) for N <= 16, nk above
+    int nblocks;      // partial rows per step: ceil(nk / waves) for N <= 16, nk above
+    int waves;        // N <= 16: k points per block, 8, or 4 where 8 gives fewer blocks than CUs
 };
-size_t nb16_vel_fused_lds_bytes(int nv);
+size_t nb16_vel_fused_lds_bytes(int nv, int waves);
 hipError_t nb16_launch_vel_fused(const NbArgs &a, const NbVelFused &f, const double2 *P, const double *Eocc,
                                  double2 *rho, double *partials, hipStream_t s);
 // the same for 16 < N <= 32: one block of four waves per k point (nblocks = nk)

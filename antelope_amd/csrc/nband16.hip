@@ -789,7 +789,8 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2))) voi
 // kNbFusedWaves k points add their sample sums in LDS each step into one partial
 // row (waves in order). A at the stage times comes from a table of the whole pulse
 // ([nt][4][3], the host's values), read with uniform loads.
-__global__ __launch_bounds__(64 * kNbFusedWaves) __attribute__((amdgpu_waves_per_eu(2))) void nb16_vel_fused_kernel(
+template <int W>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2))) void nb16_vel_fused_kernel(
     const NbArgs a, const NbVelFused f, const cd *__restrict__ P, const double *__restrict__ Eocc,
     cd *__restrict__ rho, double *__restrict__ partials)
 {
@@ -797,11 +798,11 @@ __global__ __launch_bounds__(64 * kNbFusedWaves) __attribute__((amdgpu_waves_per
     const int N = a.N, nn = N * N;
     const int c = lane_c(), g = lane_g(), lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
-    const int64_t k = static_cast<int64_t>(blockIdx.x) * kNbFusedWaves + w;
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * W + w;
     const bool active = k < a.nk;
     const int64_t ks = a.nk_stride * nn;
     cd *buf = vel_lds + w * 16 * kTpad;
-    double *red = reinterpret_cast<double *>(vel_lds + kNbFusedWaves * 16 * kTpad);  // [kNbFusedWaves][nv]
+    double *red = reinterpret_cast<double *>(vel_lds + W * 16 * kTpad);  // [W][nv]
     Tile R, Px, Py, Pz;
     double Ec = 0.0, oc = 0.0, wk = 0.0;
     if (active) {
@@ -858,7 +859,7 @@ __global__ __launch_bounds__(64 * kNbFusedWaves) __attribute__((amdgpu_waves_per
         double *row = partials + ((step - f.s0) * f.nblocks + blockIdx.x) * static_cast<int64_t>(a.nv);
         for (int v = threadIdx.x; v < a.nv; v += blockDim.x) {
             double sum = 0.0;
-            for (int q = 0; q < kNbFusedWaves; ++q) sum += red[q * a.nv + v];
+            for (int q = 0; q < W; ++q) sum += red[q * a.nv + v];
             row[v] = sum;
         }
         __syncthreads();  // the rows are read before the next step's sums
@@ -995,21 +996,28 @@ hipError_t nb16_launch_step(const NbArgs &a, const NbFused &f, double2 *rho, dou
     return f.waves == 4 ? launch_step_w<4>(a, f, rho, U, partials, s) : launch_step_w<8>(a, f, rho, U, partials, s);
 }
 
-size_t nb16_vel_fused_lds_bytes(int nv)
+size_t nb16_vel_fused_lds_bytes(int nv, int waves)
 {
-    return sizeof(double2) * static_cast<size_t>(kNbFusedWaves) * 16 * kTpad + sizeof(double) * kNbFusedWaves * nv;
+    return sizeof(double2) * static_cast<size_t>(waves) * 16 * kTpad + sizeof(double) * waves * nv;
+}
+
+template <int W>
+static hipError_t launch_vel_fused_w(const NbArgs &a, const NbVelFused &f, const double2 *P, const double *Eocc,
+                                     double2 *rho, double *partials, hipStream_t s)
+{
+    const size_t lds = nb16_vel_fused_lds_bytes(a.nv, W);
+    const hipError_t e = allow_lds(reinterpret_cast<const void *>(&nb16_vel_fused_kernel<W>), lds);
+    if (e != hipSuccess) return e;
+    nb16_vel_fused_kernel<W><<<dim3(static_cast<unsigned>(f.nblocks)), 64 * W, lds, s>>>(a, f, P, Eocc, rho, partials);
+    return hipGetLastError();
 }
 
 hipError_t nb16_launch_vel_fused(const NbArgs &a, const NbVelFused &f, const double2 *P, const double *Eocc,
                                  double2 *rho, double *partials, hipStream_t s)
 {
-    if (a.N > 16 || f.nblocks < 1 || f.s1 < f.s0) return hipErrorInvalidValue;
-    const size_t lds = nb16_vel_fused_lds_bytes(a.nv);
-    const hipError_t e = allow_lds(reinterpret_cast<const void *>(&nb16_vel_fused_kernel), lds);
-    if (e != hipSuccess) return e;
-    nb16_vel_fused_kernel<<<dim3(static_cast<unsigned>(f.nblocks)), 64 * kNbFusedWaves, lds, s>>>(a, f, P, Eocc, rho,
-                                                                                                  partials);
-    return hipGetLastError();
+    if (a.N > 16 || f.nblocks < 1 || f.s1 < f.s0 || (f.waves != 4 && f.waves != 8)) return hipErrorInvalidValue;
+    return f.waves == 4 ? launch_vel_fused_w<4>(a, f, P, Eocc, rho, partials, s)
+                        : launch_vel_fused_w<8>(a, f, P, Eocc, rho, partials, s);
 }
 
 }  // namespace sbe

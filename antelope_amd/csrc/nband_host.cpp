@@ -230,6 +230,9 @@ int fused_waves(const BoxShape &bx)
     return b8 < kNbCUs ? 4 : 8;
 }
 
+// the same rule for the time-fused velocity steps (one k point per wave, k in order)
+int vel_waves(int64_t nk) { return (nk + 7) / 8 < kNbCUs ? 4 : 8; }
+
 int fused_blocks(const BoxShape &bx, int &g1)
 {
     const int w = fused_waves(bx);
@@ -263,7 +266,7 @@ NbPlan nb_plan(int N, int nr, int64_t nk, int64_t nt, int nstage, bool split16, 
     const size_t nn = static_cast<size_t>(N) * N;
     int g1 = 0;
     int64_t rows = fused ? std::max<int64_t>(p.sample_waves, fused_blocks(bx, g1)) : p.sample_waves;
-    if (vel_fused) rows = std::max<int64_t>(rows, N > 16 ? nk : (nk + kNbFusedWaves - 1) / kNbFusedWaves);
+    if (vel_fused) rows = std::max<int64_t>(rows, N > 16 ? nk : (nk + vel_waves(nk) - 1) / vel_waves(nk));
     p.bytes_vel = vel_fused ? align256(static_cast<size_t>(nt) * 12 * sizeof(double)) : 0;
     const size_t row = static_cast<size_t>(rows) * p.nv * sizeof(double);
     p.chunk = std::max<int64_t>(1, std::min<int64_t>(nt, static_cast<int64_t>(kNbPartialBytes / row)));
@@ -730,7 +733,8 @@ int nb_setup(sbe_ctx *ctx)
             }
             if (!rc) rc = hipc(ctx, hipMemcpy(vtab, hv.data(), hv.size() * sizeof(double), hipMemcpyHostToDevice), "upload A");
             ctx->nbv_A = vtab;
-            ctx->nbv_nblocks = static_cast<int>(N > 16 ? nk : (nk + kNbFusedWaves - 1) / kNbFusedWaves);
+            ctx->nbv_waves = vel_waves(nk);
+            ctx->nbv_nblocks = static_cast<int>(N > 16 ? nk : (nk + ctx->nbv_waves - 1) / ctx->nbv_waves);
             ctx->nbv = true;
         }
         if (!rc) rc = hipc(ctx, hipStreamSynchronize(ctx->stream), "init sync");
@@ -769,6 +773,7 @@ int nb_run_velocity(sbe_ctx *ctx, int64_t step_begin, int64_t step_end)
             f.s1 = c0 + n;
             f.dt = dt;
             f.nblocks = ctx->nbv_nblocks;
+            f.waves = ctx->nbv_waves;
             rc = hipc(ctx,
                       a.N > 16 ? nb32_launch_vel_fused(a, f, Pv, Eocc, ctx->nb_rho, ctx->d_partials, ctx->stream)
                                : nb16_launch_vel_fused(a, f, Pv, Eocc, ctx->nb_rho, ctx->d_partials, ctx->stream),

@@ -210,7 +210,7 @@ struct sbe_ctx {
     // time-fused velocity-gauge steps (nb16_vel_fused_kernel)
     bool nbv = false;
     const double *nbv_A = nullptr;  // [nt][4][3] A at the RK4 stage times
-    int nbv_nblocks = 0;
+    int nbv_nblocks = 0, nbv_waves = 8;
     std::string err;
 
     // optional kernel timing: (start, stop) event pairs per kernel class

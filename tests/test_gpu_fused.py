@@ -114,6 +114,7 @@ def test_fused_rk5_keeps_separate_kernels(oracle, monkeypatch):
     ("n16_velocity_t1t2_total_current", 16, dict(T2=60.0, inter_intra=False)),
     ("n24_velocity_t2_four_waves_per_k", 24, dict()),
     ("n32_velocity_t1t2_total_current", 32, dict(T1=150.0, T2=60.0, inter_intra=False)),
+    ("n8_velocity_2048k_eight_waves_per_block", 8, dict(npts=(16, 16, 8), ncycles=0.25)),
 ])
 def test_time_fused_velocity_gauge(oracle, monkeypatch, label, N, kw):
     """The time-fused velocity-gauge kernels (all steps of a chunk in one launch; one wave
@@ -122,7 +123,8 @@ def test_time_fused_velocity_gauge(oracle, monkeypatch, label, N, kw):
     import dataclasses
     model = synthetic_wannier(N, nmax=(1, 1, 1), seed=70 + N)
     T1 = kw.pop("T1", 150.0 if N == 16 else -1.0)
-    cfg = dataclasses.replace(_cfg((5, 4, 2), **kw), gauge="VelocityHamiltonian", T1=T1)
+    npts = kw.pop("npts", (5, 4, 2))  # blocks of 4 waves for N <= 16 below 2048 k points, of 8 above
+    cfg = dataclasses.replace(_cfg(npts, **kw), gauge="VelocityHamiltonian", T1=T1)
     print(f"case: {label}", flush=True)
     ref = oracle.run(cfg, model, nthreads=8, want_state=False)
     fz, _, bytes_f = _solve(cfg, model, monkeypatch, fused=True)
