@@ -195,29 +195,31 @@ bool nb32_eligible(const sbe_config &cfg, const sbe_model &m)
 // separable passes cost more in launches than they save in arithmetic.
 bool nb32_dense_multi_n(int nr, int64_t nk) { return nr >= 1 && nr <= kNbDenseMultiMaxR && nk <= 1024; }
 
-// steps whose matrices one dense launch evaluates: about 256 MB of matrices (seven
-// N x N per k point and step), at least one, at most the pulse; ANTELOPE_NB_EVAL_STEPS
-// sets it (the results do not depend on it)
-constexpr size_t kNb32StepBytes = size_t(256) << 20;
-// steps whose separable passes (up to the second) one batch of launches evaluates for the
-// N <= 16 fused step: about 256 MB of second-pass outputs and scratch (three evaluations
-// per step), at least one, at most the pulse and 21845 (three evaluations a step in one grid
-// dimension); ANTELOPE_NB_EVAL_STEPS sets it (the results do not depend on it)
-int nb16_steps_per_eval(size_t t2, size_t b, size_t t1, int64_t nt)
+// Steps whose Fourier evaluations one batch of launches covers (the fused RK4 steps):
+// at most kNbEvalSteps (a few launches per hundred steps) and about kNbEvalBytes of outputs
+// and scratch, at least one, at most the pulse; ANTELOPE_NB_EVAL_STEPS sets it (the results
+// do not depend on it).
+constexpr int64_t kNbEvalSteps = 64;
+constexpr size_t kNbEvalBytes = size_t(256) << 20;
+int64_t eval_steps(size_t bytes_per_step, int64_t nt)
 {
-    int64_t S = static_cast<int64_t>(kNb32StepBytes / (3 * (t2 + b + t1)));
+    int64_t S = std::min<int64_t>(kNbEvalSteps, static_cast<int64_t>(kNbEvalBytes / bytes_per_step));
     if (const char *e = std::getenv("ANTELOPE_NB_EVAL_STEPS"))
         if (std::atoi(e) > 0) S = std::atoi(e);
-    return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>({nt, S, 21845})));
+    return std::max<int64_t>(1, std::min<int64_t>(nt, S));
 }
 
+// N <= 16: the separable passes up to the second of three evaluations per step (at most
+// 21845 steps: three evaluations a step in one grid dimension)
+int nb16_steps_per_eval(size_t t2, size_t b, size_t t1, int64_t nt)
+{
+    return static_cast<int>(std::min<int64_t>(eval_steps(3 * (t2 + b + t1), nt), 21845));
+}
+
+// 16 < N <= 32, one dense launch: seven N x N matrices per k point and step
 int nb32_steps_per_eval(int64_t nk_stride, int N, int64_t nt)
 {
-    const size_t per = 7 * static_cast<size_t>(nk_stride) * N * N * sizeof(double2);
-    int64_t S = static_cast<int64_t>(kNb32StepBytes / per);
-    if (const char *e = std::getenv("ANTELOPE_NB_EVAL_STEPS"))
-        if (std::atoi(e) > 0) S = std::atoi(e);
-    return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(nt, S)));
+    return static_cast<int>(eval_steps(7 * static_cast<size_t>(nk_stride) * N * N * sizeof(double2), nt));
 }
 
 // k points along a1 per block of the fused N <= 16 step: 8, or 4 where 8 would give fewer

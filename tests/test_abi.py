@@ -154,11 +154,11 @@ def test_separable_sums_footprint_without_gpu():
     # adds their second-pass outputs [M1][K2][K3][columns] (J and the midpoint pair per step,
     # the end pairs in a ring of S + 1), the band energies [nk_stride][16] (FP64), B on the
     # box and a first-pass output per evaluation (three a step), and A, E at the three times
-    # of every step; S for about 256 MB of them; its 10 partial rows per step fit in the 60
-    # the sample kernel has
+    # of every step; S for about 256 MB of them, at most 64; its 10 partial rows per step fit
+    # in the 60 the sample kernel has
     nt = laser_eval(cfg, [0.0])[3]
     t2, b, t1 = al(M[0] * K[1] * K[2] * col), al(cells * col), al(M[0] * M[1] * K[2] * col)
-    S = max(1, min(nt, 21845, (256 << 20) // (3 * (t2 + b + t1))))
+    S = max(1, min(nt, 21845, 64, (256 << 20) // (3 * (t2 + b + t1))))
     fused = (3 * S + 1) * t2 + al(nk_stride * 16 * 8) + 3 * S * (b + t1) + al(nt * 18 * 8)
     assert _footprint_env(cfg, m, dense=False, fused=True) - _footprint_env(cfg, m, dense=False) == fused
     assert _footprint_env(cfg, m, dense=True, fused=True) == _footprint_env(cfg, m, dense=True)
@@ -190,15 +190,15 @@ def test_fused_step_n32_footprint_without_gpu():
     N = 24
     col = 16 * ((4 * N * N + 63) // 64 * 64)
     # 25 lattice vectors, 144 k points: the matrices of S steps per dense launch (J, the
-    # midpoint pair, the end pair in a ring of S + 1), S for about 256 MB of them, and the
-    # pulse's A and E at the three evaluation times of every step
+    # midpoint pair, the end pair in a ring of S + 1), S for about 256 MB of them (at most
+    # 64), and the pulse's A and E at the three evaluation times of every step
     m = synthetic_wannier(N, nmax=(2, 2, 0), seed=3)
     cfg = cfg_for((12, 12))
     nt = laser_eval(cfg, [0.0])[3]
     nk_stride = (144 + 63) // 64 * 64
     lam = al(nk_stride * 32 * 8)
     mat = nk_stride * N * N * 16
-    S = max(1, min(nt, (256 << 20) // (7 * mat)))
+    S = max(1, min(nt, 64, (256 << 20) // (7 * mat)))
     steps = al((7 * S + 2) * mat) + al(nt * 18 * 8)
     assert _footprint_env(cfg, m, dense=False, fused=True) - _footprint_env(cfg, m, dense=False) == \
         lam + steps + nk_stride * 25 * 16
