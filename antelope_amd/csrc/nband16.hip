@@ -619,8 +619,10 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2))) voi
     const int col = lin / f.g1, grp = lin - col * f.g1;
     const int k2 = col / K3, k3 = col - k2 * K3;
     const int k1 = grp * W + w;
-    const bool active = k1 < K1;
-    const int64_t k = active ? a.box_kinv[(static_cast<int64_t>(k1) * K2 + k2) * K3 + k3] : 0;
+    // a k block of the grid leaves holes in its box (k map -1): those waves only stage
+    const int64_t kmap = k1 < K1 ? a.box_kinv[(static_cast<int64_t>(k1) * K2 + k2) * K3 + k3] : -1;
+    const bool active = kmap >= 0;
+    const int64_t k = active ? kmap : 0;
     cd *buf = step_lds + w * 16 * kTpad;
     const double dt = f.dt;
     // this wave's a1 factors E1[k1][m] in LDS (read as broadcasts by every tile formed)

@@ -145,9 +145,9 @@ BoxShape box_shape(const sbe_config &cfg, const sbe_model &m, int64_t nk, bool s
 // The fused RK4 step (nb16_step_kernel) replaces the stage and sample kernels when
 // one launch per step can hold everything: FP64 LengthWannier with the classical
 // RK4 tableau, N <= 16 on the register-tile kernels, a Wannier model with
-// dipoles (the dipole-free phases are an epilogue of the full sums) whose
-// separable sums cover the whole product grid with at most kNbFusedMaxM1 lattice
-// vectors along a1. ANTELOPE_NB_FUSED=0 keeps the separate kernels.
+// dipoles (the dipole-free phases are an epilogue of the full sums) whose k points
+// (the whole product grid or a k block of it) take the separable sums with at most
+// kNbFusedMaxM1 lattice vectors along a1. ANTELOPE_NB_FUSED=0 keeps the separate kernels.
 bool fused_eligible(const sbe_config &cfg, const sbe_model &m, int64_t nk, const BoxShape &bx)
 {
     if (const char *e = std::getenv("ANTELOPE_NB_FUSED"))
@@ -159,7 +159,7 @@ bool fused_eligible(const sbe_config &cfg, const sbe_model &m, int64_t nk, const
         if (std::atoi(e) == 1) return false;
     sbe_tuning tu{};
     if (tuning_lookup(cfg, m, nk, tu) && tu.nb_kernels == 2) return false;
-    return nk == bx.kcells();  // the whole grid (a k block of it takes the separate kernels)
+    return true;  // the whole grid, or a k block of it (holes in the box are idle waves)
 }
 
 // The time-fused velocity-gauge steps (nb16_vel_fused_kernel for N <= 16 on the
@@ -607,8 +607,7 @@ int nb_setup(sbe_ctx *ctx)
 
     // fused RK4 steps: the planned grid must have grouped into the whole box (every node one k point)
     ctx->nbf = false;
-    if (fbuf && a.box && a.tile16 && a.boxK[0] == bx.K[0] && a.boxK[1] == bx.K[1] && a.boxK[2] == bx.K[2] &&
-        nk == bx.kcells() && nb16_step_lds_bytes(N, bx.M[0], a.nv, fused_waves(bx)) <= (size_t(160) << 10)) {
+    if (fbuf && a.box && a.tile16 && nb16_step_lds_bytes(N, bx.M[0], a.nv, fused_waves(bx)) <= (size_t(160) << 10)) {
         const size_t col = sizeof(double2) * static_cast<size_t>(p.bcols);
         const size_t t2 = align256(col * bx.M[0] * bx.K[1] * bx.K[2]);
         const size_t b = align256(col * static_cast<size_t>(bx.cells()));
@@ -640,8 +639,10 @@ int nb_setup(sbe_ctx *ctx)
         rc = hipc(ctx, hipMemcpy(lt, hl.data(), hl.size() * sizeof(double), hipMemcpyHostToDevice), "upload laser");
         if (rc) return rc;
         ctx->nbf_laser = lt;
-        ctx->nbf_nblocks = fused_blocks(bx, ctx->nbf_g1);
+        // blocks over the k points' own box (a k block of the grid may span fewer k1, k2, k3)
         ctx->nbf_waves = fused_waves(bx);
+        ctx->nbf_g1 = (a.boxK[0] + ctx->nbf_waves - 1) / ctx->nbf_waves;
+        ctx->nbf_nblocks = ctx->nbf_g1 * a.boxK[1] * a.boxK[2];
         ctx->nbf = true;
     }
     ctx->nbf_prev_valid = false;

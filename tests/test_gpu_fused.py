@@ -218,3 +218,27 @@ def test_fused_step_n32_k_blocks(oracle, monkeypatch, label, nmax):
     e_or = _errs(tot, ref)
     print(label, "k blocks vs oracle", {k: f"{v:.2e}" for k, v in e_or.items()}, flush=True)
     assert all(v < TOL64 for v in e_or.values()), e_or
+
+
+def test_fused_step_on_k_blocks(oracle, monkeypatch):
+    """The N <= 16 fused step on ragged k blocks of the grid (one rank's share each: the
+    blocks' boxes have holes, idle waves): each block runs fused (its footprint holds the
+    fused buffers) and the blocks' series add up to the oracle's whole-grid series."""
+    model = synthetic_wannier(8, nmax=(2, 2, 1), seed=44)
+    cfg = _cfg((8, 6, 2), ncycles=0.25)  # blocks of at least 26 k points: the separable sums pay
+    print("case: n8 k blocks fused", flush=True)
+    ref = oracle.run(cfg, model, nthreads=8)
+    tot = None
+    for b, e in [(0, 40), (40, 70), (70, 96)]:
+        monkeypatch.setenv("ANTELOPE_NB_FUSED", "0")
+        sep = query_footprint(cfg, model, b, e)
+        monkeypatch.setenv("ANTELOPE_NB_FUSED", "1")
+        with SBESolver(cfg, model, b, e) as s:
+            assert s.info.device_bytes == query_footprint(cfg, model, b, e) > sep
+            s.run(0, 30)
+            s.run(30, s.nt)
+            g = s.series()
+        tot = g if tot is None else {k: tot[k] + g[k] for k in g}
+    e_or = _errs(tot, ref)
+    print("n8 k blocks fused vs oracle", {k: f"{v:.2e}" for k, v in e_or.items()}, flush=True)
+    assert all(v < TOL64 for v in e_or.values()), e_or
