@@ -49,6 +49,7 @@ double now_s()
 
 constexpr int kDepth = 2;                    // launches in flight per lane or request
 constexpr double kSliceKSteps = 67108864.0;  // 2-band work per own-stream slice, about 1.5 ms of device time
+constexpr int64_t kNbSliceKSteps = 65536;    // fused N-band work per slice (256 steps of 256 k points)
 constexpr size_t kParkMax = 8;               // parked contexts freed together
 
 struct Request {
@@ -363,7 +364,10 @@ int sbe_server_step(sbe_server *s)
                 // with thousands of them (the fused steps take one or two launches per step)
                 r.slice = std::max<int64_t>(1, s->slice / 32);
             } else if (r.ctx->path == sbe::PathKind::NBand) {
-                r.slice = s->slice;
+                // fused steps: whole slices on the serving-size grids, fewer steps per slice on
+                // large ones (about kNbSliceKSteps k-point steps, a few ms of device time)
+                const int64_t by_work = kNbSliceKSteps / std::max<int64_t>(1, inf.nk_total);
+                r.slice = std::max<int64_t>(1, std::min<int64_t>(s->slice, by_work));
             } else if (r.ctx->adaptive) {
                 // whole windows of the adaptive stepper, so the result equals a run on its own
                 const int64_t w = r.ctx->chunk_steps;
