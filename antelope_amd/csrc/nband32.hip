@@ -88,8 +88,9 @@ __device__ Tile q_prod(Q &L, const Tile &X, bool adjX, const Tile &Y, bool adjY)
     for (int r = 0; r < 2; ++r) {
         // A_pr^T's tile: A = X: transposed read of X_pr; A = X^dag: A_pr^T = conj(X_rp)
         const Tile Aop = adjX ? q_get(SA, r, L.p, false, true) : q_get(SA, L.p, r, true, false);
-        // B_rq's tile: B = Y: Y_rq; B = Y^dag: element (i, c) = conj(Y[16q + c][16r + i])
-        const Tile Bop = adjY ? q_get(SB, L.q, r, true, true) : q_get(SB, r, L.q, false, false);
+        // B_rq's tile: B = Y: Y_rq (this wave's own Y for r = p); B = Y^dag: element (i, c) =
+        // conj(Y[16q + c][16r + i])
+        const Tile Bop = adjY ? q_get(SB, L.q, r, true, true) : (r == L.p ? Y : q_get(SB, r, L.q, false, false));
         mma_acc(re, im, Aop, Bop);
     }
     L.next();
@@ -429,7 +430,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1))) void n
                 rsum[2 * e + 1] = fma(R.im[r], wk, rsum[2 * e + 1]);
             }
         }
-        // currents: Re Tr(rho J) = Re sum conj(rho_ij) J_ij, Re Tr(M J) likewise (rho, M Hermitian)
+        // currents: Re Tr(rho J) = Re sum conj(rho_ij) J_ij, Re Tr(M J) likewise (rho, M
+        // Hermitian): the waves' sums of the three axes, then one exchange for all six
+        double part[6];
         auto current = [&](int ax, const cd *Js) {
             const Tile J = q_load(L, Js, false);
             double tot = 0.0, intra = 0.0;
@@ -438,16 +441,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1))) void n
                 tot += R.re[r] * J.re[r] + R.im[r] * J.im[r];
                 intra += M.re[r] * J.re[r] + M.im[r] * J.im[r];
             }
-            tot = q_sum(L, wave_sum(tot));
-            intra = a.inter_intra ? q_sum(L, wave_sum(intra)) : 0.0;
-            if (lane == ax) curl += (tot - intra) * wk;
-            if (lane == 3 + ax) curl += intra * wk;
+            part[ax] = wave_sum(tot);
+            part[3 + ax] = a.inter_intra ? wave_sum(intra) : 0.0;
         };
         current(0, A0);
         stage_wait();  // B = [J_y, J_z]; A free
         stage_async(stg, 0, f.ht_prev + kn, a.deph ? f.h_half + kn : nullptr, nn, w, lane);
         current(1, B0);
         current(2, B1);
+        {
+            double *S = L.scal();
+            if (lane == 0)
+#pragma unroll
+                for (int j = 0; j < 6; ++j) S[8 * w + j] = part[j];
+            __syncthreads();
+            if (w == 0 && lane < 3) {  // waves added in order
+                const double tot = ((S[lane] + S[8 + lane]) + S[16 + lane]) + S[24 + lane];
+                const double intra = ((S[3 + lane] + S[11 + lane]) + S[19 + lane]) + S[27 + lane];
+                curl += (tot - intra) * wk;  // lane a: inter a
+            }
+            if (w == 0 && lane >= 3 && lane < 6) {
+                const double intra = ((S[lane] + S[8 + lane]) + S[16 + lane]) + S[24 + lane];
+                curl += intra * wk;  // lane 3 + a: intra a
+            }
+            L.next();
+        }
         stage_wait();  // A = [H~(t_n), H(t + dt/2)]; B free
         stage_async(stg, 1, f.ht_half + kn, f.h_end + kn, nn, w, lane);
         // stage 1 at t_n
