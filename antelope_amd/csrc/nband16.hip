@@ -110,13 +110,13 @@ __device__ void eig_refine(const Tile &Hin, Tile &U, Tile &Ut, int N, cd *buf, d
             if (i != c && i < N && c < N && w2 > 0.0) {
                 const double D = lam_c - lam_r[r];
                 const double s = (D > 0.0 || (D == 0.0 && i < c)) ? 1.0 : -1.0;
-                const double den = 0.5 * D + s * sqrt(0.25 * D * D + w2);
-                K.re[r] = W.re[r] / den;
-                K.im[r] = W.im[r] / den;
-                kmax = fmax(kmax, sqrt(K.re[r] * K.re[r] + K.im[r] * K.im[r]));
+                const double inv = 1.0 / (0.5 * D + s * sqrt(0.25 * D * D + w2));
+                K.re[r] = W.re[r] * inv;
+                K.im[r] = W.im[r] * inv;
+                kmax = fmax(kmax, K.re[r] * K.re[r] + K.im[r] * K.im[r]);  // |K|^2 here, the root once below
             }
         }
-        kmax = wave_max(kmax);
+        kmax = sqrt(wave_max(kmax));
         if (kmax == 0.0) break;
         if (kmax > 0.5) {  // far from converged (a cold-like start): damped rotations
             const double f = 0.5 / kmax;

@@ -244,13 +244,13 @@ __device__ void q_refine(Q &L, const cd *Hk, Tile &U, double small_tol, bool ort
             if (i != cg && i < N && cg < N && w2 > 0.0) {
                 const double D = lam_c - lam_r[r];
                 const double s = (D > 0.0 || (D == 0.0 && i < cg)) ? 1.0 : -1.0;
-                const double den = 0.5 * D + s * sqrt(0.25 * D * D + w2);
-                K.re[r] = W.re[r] / den;
-                K.im[r] = W.im[r] / den;
-                kmax = fmax(kmax, sqrt(K.re[r] * K.re[r] + K.im[r] * K.im[r]));
+                const double inv = 1.0 / (0.5 * D + s * sqrt(0.25 * D * D + w2));
+                K.re[r] = W.re[r] * inv;
+                K.im[r] = W.im[r] * inv;
+                kmax = fmax(kmax, K.re[r] * K.re[r] + K.im[r] * K.im[r]);  // |K|^2 here, the root once below
             }
         }
-        kmax = q_max(L, kmax);
+        kmax = sqrt(q_max(L, kmax));
         if (kmax == 0.0) break;
         if (kmax > 0.5) {  // far from converged: damped rotations
             const double fct = 0.5 / kmax;
