@@ -179,7 +179,7 @@ __global__ __launch_bounds__(256) void nb_fourier_kernel(const NbArgs a, const N
 #pragma unroll
                 for (int j = 0; j < 2; ++j) {
                     accr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i].x, bv[j].x, accr[i][j], 0, 0, 0);
-                    accr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-av[i].y, bv[j].y, accr[i][j], 0, 0, 0);
+                    accr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i].y, bv[j].y, accr[i][j], 0, 0, 1);  // 1: A negated
                     acci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i].x, bv[j].y, acci[i][j], 0, 0, 0);
                     acci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i].y, bv[j].x, acci[i][j], 0, 0, 0);
                 }
@@ -659,7 +659,7 @@ __device__ void wmm(cd *C, const cd *A, const cd *B, int N, bool adjA, bool adjB
                 if (i < N && l < N) x = adjA ? cconj(A[l * N + i]) : A[i * N + l];
                 if (l < N && j < N) y = adjB ? cconj(B[j * N + l]) : B[l * N + j];
                 re = __builtin_amdgcn_mfma_f64_16x16x4f64(x.x, y.x, re, 0, 0, 0);
-                re = __builtin_amdgcn_mfma_f64_16x16x4f64(-x.y, y.y, re, 0, 0, 0);
+                re = __builtin_amdgcn_mfma_f64_16x16x4f64(x.y, y.y, re, 0, 0, 1);  // 1: A negated
                 im = __builtin_amdgcn_mfma_f64_16x16x4f64(x.x, y.y, im, 0, 0, 0);
                 im = __builtin_amdgcn_mfma_f64_16x16x4f64(x.y, y.x, im, 0, 0, 0);
             }

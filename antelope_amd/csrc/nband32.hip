@@ -70,7 +70,7 @@ __device__ __forceinline__ void mma_acc(d4 &re, d4 &im, const Tile &Aop, const T
     for (int kb = 0; kb < 4; ++kb) {
         const double ar = Aop.re[kb], ai = Aop.im[kb], br = B.re[kb], bi = B.im[kb];
         re = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, re, 0, 0, 0);
-        re = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, bi, re, 0, 0, 0);
+        re = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, bi, re, 0, 0, 1);  // 1: A negated
         im = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bi, im, 0, 0, 0);
         im = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, br, im, 0, 0, 0);
     }

@@ -25,19 +25,24 @@ constexpr int kTpad = 17;  // LDS row pitch (complex) of the transpose buffer
 __device__ __forceinline__ int lane_c() { return threadIdx.x & 15; }
 __device__ __forceinline__ int lane_g() { return (threadIdx.x >> 4) & 3; }
 
-// C = A B. Aop holds A^T's tile (register kb = A[c][4 kb + g]); conjA and
-// conjB conjugate the operands as they are read.
-__device__ __forceinline__ Tile cmm(const Tile &Aop, bool conjA, const Tile &B, bool conjB = false)
+// C = A B. Aop holds A^T's tile (register kb = A[c][4 kb + g]); CA and CB
+// conjugate the operands. The signs go into the MFMA's operand negation (its
+// last argument: 1 negates A, 2 negates B), not into separate instructions.
+template <bool CA, bool CB>
+__device__ __forceinline__ Tile cmm_t(const Tile &Aop, const Tile &B)
 {
+    // re += ar br - ai' bi',  im += ar bi' + ai' br  with ai' = (CA ? -1 : 1) ai, bi' likewise
+    constexpr int kNegRe2 = (CA != CB) ? 0 : 1;  // -ai' bi' = -(+-1)(+-1) ai bi
+    constexpr int kNegIm1 = CB ? 2 : 0;
+    constexpr int kNegIm2 = CA ? 1 : 0;
     d4 re = d4{0.0, 0.0, 0.0, 0.0}, im = d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int kb = 0; kb < 4; ++kb) {
-        const double ar = Aop.re[kb], ai = conjA ? -Aop.im[kb] : Aop.im[kb];
-        const double br = B.re[kb], bi = conjB ? -B.im[kb] : B.im[kb];
+        const double ar = Aop.re[kb], ai = Aop.im[kb], br = B.re[kb], bi = B.im[kb];
         re = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, re, 0, 0, 0);
-        re = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, bi, re, 0, 0, 0);
-        im = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bi, im, 0, 0, 0);
-        im = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, br, im, 0, 0, 0);
+        re = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, bi, re, 0, 0, kNegRe2);
+        im = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bi, im, 0, 0, kNegIm1);
+        im = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, br, im, 0, 0, kNegIm2);
     }
     Tile C;
 #pragma unroll
@@ -46,6 +51,12 @@ __device__ __forceinline__ Tile cmm(const Tile &Aop, bool conjA, const Tile &B, 
         C.im[r] = im[r];
     }
     return C;
+}
+
+__device__ __forceinline__ Tile cmm(const Tile &Aop, bool conjA, const Tile &B, bool conjB = false)
+{
+    if (conjA) return conjB ? cmm_t<true, true>(Aop, B) : cmm_t<true, false>(Aop, B);
+    return conjB ? cmm_t<false, true>(Aop, B) : cmm_t<false, false>(Aop, B);
 }
 
 __device__ __forceinline__ void wave_lds_order()
