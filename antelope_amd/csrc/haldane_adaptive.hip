@@ -80,6 +80,14 @@ __device__ __forceinline__ double uniform(double x)
     return __hiloint2double(hi, lo);
 }
 
+// x of lane l
+__device__ __forceinline__ double lane_value(double x, int l)
+{
+    const int lo = __builtin_amdgcn_readlane(__double2loint(x), l);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(x), l);
+    return __hiloint2double(hi, lo);
+}
+
 // 1/x and 1/sqrt(x) for x > 0: in FP64 the hardware estimate and two Newton
 // steps (full precision up to the last bit or two), in FP32 the IEEE forms
 __device__ __forceinline__ double recip(double x)
@@ -417,13 +425,17 @@ struct SampleRing {
 
     __device__ __forceinline__ void push(double tn, double val)
     {
+        // 1/(tn - t[k]) for k < 5 (wave-uniform): lane k forms the k-th, the
+        // other lanes read it from there. t[k] is node n - k (before the shift
+        // below); nodes not yet pushed give 0 and an unused factor.
+        const int lane = __lane_id();
+        const double rl = recip(tn - t[lane < 5 ? lane : 0]);
         double prev = D[0];
         D[0] = val;
 #pragma unroll
         for (int k = 1; k < 6; ++k) {
             const double old = D[k];
-            // t[k - 1] is node n + 1 - k (before the shift below); nodes not yet pushed give 0
-            D[k] = k <= cnt ? (D[k - 1] - prev) * recip(tn - t[k - 1]) : 0.0;
+            D[k] = k <= cnt ? (D[k - 1] - prev) * lane_value(rl, k - 1) : 0.0;
             prev = old;
         }
         double tt[6];
@@ -447,6 +459,33 @@ struct SampleRing {
 #pragma unroll
         for (int k = 4; k >= 0; --k) acc = fma(acc, tau - tk[k], D[k]);
         return acc;
+    }
+
+    // Writes the samples mi, mi + 1, ... (below mend) whose times
+    // atmin + n dt_out do not exceed t_hi, and returns the next one. The eight
+    // lanes of a group hold the same value's ring, so lane l evaluates sample
+    // mi + (l & 7) of value l >> 3: eight samples per pass, each with the
+    // arithmetic of a single evaluation. part points at this lane's value
+    // column of sample row out0.
+    __device__ __forceinline__ int emit(int mi, int mend, double atmin, double dt_out, double t_hi, double *part,
+                                        int64_t out0, int64_t row_stride) const
+    {
+        double tk[6];  // the node times, from LDS once per call
+#pragma unroll
+        for (int k = 0; k < 6; ++k) tk[k] = t[k];
+        const int j = __lane_id() & 7;
+        while (mi < mend) {
+            const bool in = j < mend - mi;
+            const int mj = in ? mi + j : mi;
+            const double tm = atmin + dt_out * static_cast<double>(mj);
+            const bool ok = in && !(tm > t_hi);
+            // the times grow with j, so the lanes that write are a prefix of each group
+            const int n = __popcll(__ballot(ok) & 0xffull);
+            if (ok) part[(static_cast<int64_t>(mj) - out0) * row_stride] = at(tm, tk);
+            mi += n;
+            if (n < 8) break;
+        }
+        return mi;
     }
 };
 
@@ -590,21 +629,8 @@ __device__ void adaptive_body(const AdaptiveArgs &A, int64_t blk, const int64_t 
 #pragma unroll
     for (int i = 0; i < 6; ++i) ring.D[i] = 0.0;
     for (int i = 0; i < 6; ++i) ring.t[i] = 0.0;
-    auto emit = [&](double t_hi) {
-        double tk[6];  // the node times, from LDS once per call
-#pragma unroll
-        for (int k = 0; k < 6; ++k) tk[k] = ring.t[k];
-        double *dst = part + (m - out0) * row_stride;
-        int mi = static_cast<int>(m);  // nt < 2^31 (host check): exact conversion
-        const int mend = static_cast<int>(out1);
-        while (mi < mend) {
-            const double tm = A.atmin + A.dt_out * static_cast<double>(mi);
-            if (tm > t_hi) break;
-            *dst = ring.at(tm, tk);
-            dst += row_stride;
-            ++mi;
-        }
-        m = mi;
+    auto emit = [&](double t_hi) {  // nt < 2^31 (host check): exact conversions
+        m = ring.emit(static_cast<int>(m), static_cast<int>(out1), A.atmin, A.dt_out, t_hi, part, out0, row_stride);
     };
     {
         laser_rows<R>(A, t, 0.0, t, rows, row64);
@@ -773,6 +799,30 @@ __device__ void adaptive_body(const AdaptiveArgs &A, int64_t blk, const int64_t 
 }
 
 // f/2 of the Bloch equation at d (1/|d|^2 = inv): d x y - (g2/2) y + ((g2 - g1)/2) (y.d/|d|^2) d
+// d(k + A) and 1/|d|^2 (0 at a gap closing) at a stage time: the sums of
+// phases_at, each component formed as one FMA chain (a stage needs no single
+// phase, only d)
+__device__ __forceinline__ void stage_d_inv(const KPoint<double> &p, const double *row, double c3, double M0,
+                                            Vec3<double> &d, double &inv)
+{
+    double dx = p.er[0] * row[0], dy = p.er[0] * row[1], sb = p.er[3] * row[7];
+    dx = fma(-p.ei[0], row[1], dx);
+    dy = fma(p.ei[0], row[0], dy);
+    sb = fma(p.ei[3], row[6], sb);
+#pragma unroll
+    for (int j = 1; j < 3; ++j) {
+        dx = fma(p.er[j], row[2 * j], dx);
+        dx = fma(-p.ei[j], row[2 * j + 1], dx);
+        dy = fma(p.er[j], row[2 * j + 1], dy);
+        dy = fma(p.ei[j], row[2 * j], dy);
+        sb = fma(p.er[j + 3], row[7 + 2 * j], sb);
+        sb = fma(p.ei[j + 3], row[6 + 2 * j], sb);
+    }
+    d = Vec3<double>{dx, dy, fma(-c3, sb, M0)};
+    const double dd = fma(d.x, d.x, fma(d.y, d.y, d.z * d.z));
+    inv = dd > 0.0 ? rcp_nr(dd) : 0.0;
+}
+
 template <bool DEPH>
 __device__ __forceinline__ Vec3<double> half_f_at(const Vec3<double> &d, double inv, const Vec3<double> &y, double gh2,
                                                   double gh12)
@@ -848,21 +898,8 @@ __device__ void adaptive_body64(const AdaptiveArgs &A, int64_t blk, const int64_
     for (int i = 0; i < 6; ++i) ring.D[i] = 0.0;
     for (int i = 0; i < 6; ++i) ring.t[i] = 0.0;
     // samples up to t_hi from the ring
-    auto emit = [&](double t_hi) {
-        double tk[6];  // the node times, from LDS once per call
-#pragma unroll
-        for (int k = 0; k < 6; ++k) tk[k] = ring.t[k];
-        double *dst = part + (m - out0) * row_stride;
-        int mi = static_cast<int>(m);  // nt < 2^31 (host check): exact conversion
-        const int mend = static_cast<int>(out1);
-        while (mi < mend) {
-            const double tm = atmin + dt_out * static_cast<double>(mi);
-            if (tm > t_hi) break;
-            *dst = ring.at(tm, tk);
-            dst += row_stride;
-            ++mi;
-        }
-        m = mi;
+    auto emit = [&](double t_hi) {  // nt < 2^31 (host check): exact conversions
+        m = ring.emit(static_cast<int>(m), static_cast<int>(out1), atmin, dt_out, t_hi, part, out0, row_stride);
     };
 
     Vec3<double> k1[KPT];
@@ -913,8 +950,10 @@ __device__ void adaptive_body64(const AdaptiveArgs &A, int64_t blk, const int64_
                         y.z = fma(aij, kv[j].z, y.z);
                     }
                 }
-                const Phases<double> e = phases_at(p[q], rows + (i - 1) * kRow, c3, M0);
-                kv[i] = half_f_at<DEPH>(e.d, e.inv, y, gh2, gh12);
+                Vec3<double> d;
+                double inv;
+                stage_d_inv(p[q], rows + (i - 1) * kRow, c3, M0, d, inv);
+                kv[i] = half_f_at<DEPH>(d, inv, y, gh2, gh12);
             }
             inc[q] = Vec3<double>{0, 0, 0};
             double ex = 0, ey = 0, ez = 0;
@@ -950,8 +989,8 @@ __device__ void adaptive_body64(const AdaptiveArgs &A, int64_t blk, const int64_
                 p[q].sy += inc[q].y;
                 p[q].sz += inc[q].z;
                 const Vec3<double> y{p[q].sx, p[q].sy, p[q].sz};
-                // the phases of the stage at c = 1 (the step end), formed again from the same
-                // row with the same operations rather than kept through the last stage
+                // the phases at the step end (the time of the stage at c = 1, which formed only
+                // d), from the step end's row rather than kept through the last stage
                 const Phases<double> e1 = phases_at(p[q], rows + 5 * kRow, c3, M0);
                 k1[q] = half_f_at<DEPH>(e1.d, e1.inv, y, gh2, gh12);  // f(t + h, y5)
                 sample_state(e1, p[q].w, p[q].n, y.x, y.y, y.z, a, ii, v);
