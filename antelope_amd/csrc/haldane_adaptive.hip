@@ -126,21 +126,34 @@ static_assert(CashKarp::c[1] == 0.2 && CashKarp::c[2] == 0.3 && CashKarp::c[3] =
 // sin and cos of a small angle (|x| <= 0.25) by their Taylor series to x^13
 // and x^12 (Horner in x^2 with the inverse factorials as constants); the
 // first omitted terms are below 1e-18 relative.
+// A double constant formed in scalar registers where it is used (two
+// s_mov_b32 per use). Loop-invariant literals would otherwise be kept in
+// vector registers for the whole loop, which pushes the stepper into spills.
+template <unsigned long long B>
+__device__ __forceinline__ double sconst()
+{
+    int lo, hi;
+    asm volatile("s_mov_b32 %0, %1" : "=s"(lo) : "i"(static_cast<int>(B & 0xffffffffull)));
+    asm volatile("s_mov_b32 %0, %1" : "=s"(hi) : "i"(static_cast<int>(B >> 32)));
+    return __hiloint2double(hi, lo);
+}
+#define SBE_SCONST(v) sconst<__builtin_bit_cast(unsigned long long, static_cast<double>(v))>()
+
 __device__ __forceinline__ void sincos_small(double x, double &sn, double &cs)
 {
     const double x2 = x * x;
-    double ps = 1.0 / 6227020800.0;  // 1/13!
-    ps = fma(ps, x2, -1.0 / 39916800.0);
-    ps = fma(ps, x2, 1.0 / 362880.0);
-    ps = fma(ps, x2, -1.0 / 5040.0);
-    ps = fma(ps, x2, 1.0 / 120.0);
-    ps = fma(ps, x2, -1.0 / 6.0);
+    double ps = SBE_SCONST(1.0 / 6227020800.0);  // 1/13!
+    ps = fma(ps, x2, SBE_SCONST(-1.0 / 39916800.0));
+    ps = fma(ps, x2, SBE_SCONST(1.0 / 362880.0));
+    ps = fma(ps, x2, SBE_SCONST(-1.0 / 5040.0));
+    ps = fma(ps, x2, SBE_SCONST(1.0 / 120.0));
+    ps = fma(ps, x2, SBE_SCONST(-1.0 / 6.0));
     sn = fma(ps * x2, x, x);
-    double pc = 1.0 / 479001600.0;  // 1/12!
-    pc = fma(pc, x2, -1.0 / 3628800.0);
-    pc = fma(pc, x2, 1.0 / 40320.0);
-    pc = fma(pc, x2, -1.0 / 720.0);
-    pc = fma(pc, x2, 1.0 / 24.0);
+    double pc = SBE_SCONST(1.0 / 479001600.0);  // 1/12!
+    pc = fma(pc, x2, SBE_SCONST(-1.0 / 3628800.0));
+    pc = fma(pc, x2, SBE_SCONST(1.0 / 40320.0));
+    pc = fma(pc, x2, SBE_SCONST(-1.0 / 720.0));
+    pc = fma(pc, x2, SBE_SCONST(1.0 / 24.0));
     pc = fma(pc, x2, -0.5);
     cs = fma(pc, x2, 1.0);
 }
@@ -165,6 +178,44 @@ __device__ __forceinline__ double inv_fifth_root(double x)
     return y;
 }
 
+// What one lane needs for the laser rows, read once per launch: the lane's
+// time (stage fraction, or the step end) and lattice vector, its column of
+// the phase table, and the table constants. The step loop then reads only
+// the two table rows its times fall between, with one memory round trip.
+struct LaserLane {
+    const double *a_tab;      // [nt][4]: A - A(t_min) (x, y) and dA/dt (x, y) at t_n
+    const double *phase_col;  // phase_tab + 2 j: scaled e^{iA(t_n).v_j} at row n x 12
+    double atmin, h, inv_h;   // sample grid
+    double c;                 // stage fraction of time s (kRowC)
+    double vx, vy;            // v_j: a_j (j < 3) or b_{j-3}
+    int nt2;                  // last interval index (nt - 2)
+    int s, j;                 // time s, phase j (lanes < 36)
+    bool active, end;         // lane < 36; time s is the step end itself (s = 3, 5)
+};
+
+__device__ __forceinline__ LaserLane laser_lane(const AdaptiveArgs &A)
+{
+    LaserLane L;
+    const int lane = __lane_id();
+    L.s = lane / 6;
+    L.j = lane - 6 * L.s;
+    L.active = lane < 36;
+    // s = 5, and the stage at c = 1 (s = 3): the step end itself
+    L.end = L.s == 3 || L.s == 5;
+    L.c = kRowC[L.active ? L.s : 0];
+    const HaldaneArgs &a = A.h;
+    const double *vrow = L.j < 3 ? a.va[L.j] : a.vb[L.j - 3];
+    L.vx = vrow[0];
+    L.vy = vrow[1];
+    L.a_tab = A.a_tab;
+    L.phase_col = a.phase_tab + 2 * L.j;
+    L.atmin = A.atmin;
+    L.h = A.dt_out;
+    L.inv_h = A.inv_dt_out;
+    L.nt2 = static_cast<int>(A.nt) - 2;  // the host keeps nt < 2^31, so 32-bit indices
+    return L;
+}
+
 // Laser rows of six times (the stages 2..6 of a Cash-Karp step and its end
 // t + h, or six copies of one time) in the layout of the sample table:
 // t1 e^{iA.a_j} (j = 0..2) then e^{iA.b_j}, as (re, im) pairs, followed by
@@ -176,39 +227,37 @@ __device__ __forceinline__ double inv_fifth_root(double x)
 // The rows go to this wave's LDS slice, from which every lane reads them as
 // broadcasts.
 template <typename R>
-__device__ __forceinline__ void laser_rows(const AdaptiveArgs &A, double t, double hh, double tn, R *rows /*[6][kRow]*/,
+__device__ __forceinline__ void laser_rows(const LaserLane &L, double t, double hh, double tn, R *rows /*[6][kRow]*/,
                                            double *row64 = nullptr /*[6][kRow]: FP64 copies of the rows*/)
 {
-    const int lane = __lane_id();
-    const int s = lane / 6, j = lane - 6 * s;  // phase j of time s (lanes < 36)
-    if (lane < 36) {
-        // s = 5, and the stage at c = 1 (s = 3): the step end itself
-        const double tau = s == 3 || s == 5 ? tn : fma(kRowC[s], hh, t);
-        // interval of the sample table (the host keeps nt < 2^31, so 32-bit indices)
-        const double h = A.dt_out, inv_h = A.inv_dt_out;
-        const double x = (tau - A.atmin) * inv_h;
-        const int nt2 = static_cast<int>(A.nt) - 2;
+    const int s = L.s, j = L.j;
+    if (L.active) {
+        const double tau = L.end ? tn : fma(L.c, hh, t);
+        // interval of the sample table
+        const double h = L.h, inv_h = L.inv_h;
+        const double x = (tau - L.atmin) * inv_h;
         int i = static_cast<int>(floor(x));
-        i = i > nt2 ? nt2 : i;
+        i = i > L.nt2 ? L.nt2 : i;
         i = i < 0 ? 0 : i;  // the table holds at least two samples
+        const double *at = L.a_tab + 4 * i;  // A_x, A_y, dA_x/dt, dA_y/dt at t_i, then at t_i+1
+        double T[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) T[q] = at[q];
+        const double *ph = L.phase_col + 12 * i;  // scaled e^{iA(t_i).v_j}, read with the A rows
+        const double ph0 = ph[0], ph1 = ph[1];
         const double th = x - static_cast<double>(i), th2 = th * th, th3 = th2 * th;
-        const double *at = A.a_tab + 4 * i;  // A_x, A_y, dA_x/dt, dA_y/dt at t_i, then at t_i+1
         const double h01 = 3.0 * th2 - 2.0 * th3, h10 = (th3 - 2.0 * th2 + th) * h, h11 = (th3 - th2) * h;
         const double g01 = (6.0 * th - 6.0 * th2) * inv_h, g10 = 3.0 * th2 - 4.0 * th + 1.0, g11 = 3.0 * th2 - 2.0 * th;
-        const double dAx = at[4] - at[0], dAy = at[5] - at[1];
-        const double ux = h01 * dAx + h10 * at[2] + h11 * at[6];  // A(tau) - A(t_i)
-        const double uy = h01 * dAy + h10 * at[3] + h11 * at[7];
-        const double gx = g01 * dAx + g10 * at[2] + g11 * at[6];  // dA/dt at tau
-        const double gy = g01 * dAy + g10 * at[3] + g11 * at[7];
-        const HaldaneArgs &a = A.h;
-        const double *vrow = j < 3 ? a.va[j] : a.vb[j - 3];
-        const double vx = vrow[0], vy = vrow[1];
-        const double del = ux * vx + uy * vy;
+        const double dAx = T[4] - T[0], dAy = T[5] - T[1];
+        const double ux = h01 * dAx + h10 * T[2] + h11 * T[6];  // A(tau) - A(t_i)
+        const double uy = h01 * dAy + h10 * T[3] + h11 * T[7];
+        const double gx = g01 * dAx + g10 * T[2] + g11 * T[6];  // dA/dt at tau
+        const double gy = g01 * dAy + g10 * T[3] + g11 * T[7];
+        const double del = ux * L.vx + uy * L.vy;
         double sn, cs;
         if (fabs(del) <= 0.25) sincos_small(del, sn, cs);
         else sincos_large(del, &sn, &cs);
-        const double *ph = a.phase_tab + 12 * i + 2 * j;  // scaled e^{iA(t_i).v_j}
-        const double re = ph[0] * cs - ph[1] * sn, im = ph[1] * cs + ph[0] * sn;
+        const double re = ph0 * cs - ph1 * sn, im = ph1 * cs + ph0 * sn;
         rows[s * kRow + 2 * j] = static_cast<R>(re);
         rows[s * kRow + 2 * j + 1] = static_cast<R>(im);
         if (j == 0) {
@@ -559,6 +608,7 @@ __device__ void adaptive_body(const AdaptiveArgs &A, int64_t blk, const int64_t 
 
     __shared__ R lds_rows[kBlock / 64][6 * kRow];
     R *rows = lds_rows[threadIdx.x >> 6];
+    const LaserLane L = laser_lane(A);
     // eps of the reference vector: 2 % of a bound on |d| over the zone, so
     // that its rate stays below |dd/dt| / eps where the gap closes
     const double dmax = 3.0 * fabs(a.t1) + fabs(a.M0) + 3.0 * fabs(a.c3);
@@ -633,7 +683,7 @@ __device__ void adaptive_body(const AdaptiveArgs &A, int64_t blk, const int64_t 
         m = ring.emit(static_cast<int>(m), static_cast<int>(out1), A.atmin, A.dt_out, t_hi, part, out0, row_stride);
     };
     {
-        laser_rows<R>(A, t, 0.0, t, rows, row64);
+        laser_rows<R>(L, t, 0.0, t, rows, row64);
         const R *row = rows + 5 * kRow;
         double o[8] = {0, 0, 0, 0, 0, 0, 0, 0}, od[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
@@ -669,7 +719,7 @@ __device__ void adaptive_body(const AdaptiveArgs &A, int64_t blk, const int64_t 
         const bool last = t + 1.0001 * hh >= t_end;
         if (last) hh = t_end - t;
         const double tn = last ? t_end : t + hh;
-        laser_rows<R>(A, t, hh, tn, rows, row64);
+        laser_rows<R>(L, t, hh, tn, rows, row64);
 
         Vec3<R> inc[KPT], y5[KPT], kv[KPT][6], dh1[KPT];
         double err_k = 0.0;
@@ -799,7 +849,7 @@ __device__ void adaptive_body(const AdaptiveArgs &A, int64_t blk, const int64_t 
 }
 
 // f/2 of the Bloch equation at d (1/|d|^2 = inv): d x y - (g2/2) y + ((g2 - g1)/2) (y.d/|d|^2) d
-// d(k + A) and 1/|d|^2 (0 at a gap closing) at a stage time: the sums of
+// d(k + A) and 1/|d|^2 at a stage time: the sums of
 // phases_at, each component formed as one FMA chain (a stage needs no single
 // phase, only d)
 __device__ __forceinline__ void stage_d_inv(const KPoint<double> &p, const double *row, double c3, double M0,
@@ -820,7 +870,8 @@ __device__ __forceinline__ void stage_d_inv(const KPoint<double> &p, const doubl
     }
     d = Vec3<double>{dx, dy, fma(-c3, sb, M0)};
     const double dd = fma(d.x, d.x, fma(d.y, d.y, d.z * d.z));
-    inv = dd > 0.0 ? rcp_nr(dd) : 0.0;
+    // d = 0 (a gap closing) gives a finite inv and a zero dephasing projection
+    inv = rcp_nr(fmax(dd, 1e-300));
 }
 
 template <bool DEPH>
@@ -880,6 +931,7 @@ __device__ void adaptive_body64(const AdaptiveArgs &A, int64_t blk, const int64_
     const int64_t row_stride = static_cast<int64_t>(a.nwaves) * kHaldaneNV;
     __shared__ double lds_rows[kBlock / 64][6 * kRow];
     double *rows = lds_rows[threadIdx.x >> 6];
+    const LaserLane L = laser_lane(A);
 
     // the arguments used in the loops, in registers (stores through part could alias A)
     const double atmin = A.atmin, dt_out = A.dt_out, atol = A.atol, rtol = A.rtol, hmax = A.hmax, hmin = A.hmin;
@@ -904,7 +956,7 @@ __device__ void adaptive_body64(const AdaptiveArgs &A, int64_t blk, const int64_
 
     Vec3<double> k1[KPT];
     {  // the window start: its sample, the first node and the first stage
-        laser_rows<double>(A, t, 0.0, t, rows);
+        laser_rows<double>(L, t, 0.0, t, rows);
         double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
         for (int q = 0; q < KPT; ++q) {
@@ -929,7 +981,7 @@ __device__ void adaptive_body64(const AdaptiveArgs &A, int64_t blk, const int64_
         const bool last = t + 1.0001 * hh >= t_end;
         if (last) hh = t_end - t;
         const double tn = last ? t_end : t + hh;
-        laser_rows<double>(A, t, hh, tn, rows);
+        laser_rows<double>(L, t, hh, tn, rows);
 
         Vec3<double> inc[KPT];
         double err_k = 0.0;
