@@ -139,23 +139,26 @@ __device__ __forceinline__ double sconst()
 }
 #define SBE_SCONST(v) sconst<__builtin_bit_cast(unsigned long long, static_cast<double>(v))>()
 
+// sin and cos of |x| <= 0.25 by their Taylor series (to x^13 and x^12: the
+// first omitted terms are below 1e-19 relative). The series in x^2 are summed
+// from the smallest term up, with the powers of x^2 formed once for both and
+// every coefficient a multiplicand (a scalar-register operand, no copies).
 __device__ __forceinline__ void sincos_small(double x, double &sn, double &cs)
 {
-    const double x2 = x * x;
-    double ps = SBE_SCONST(1.0 / 6227020800.0);  // 1/13!
-    ps = fma(ps, x2, SBE_SCONST(-1.0 / 39916800.0));
-    ps = fma(ps, x2, SBE_SCONST(1.0 / 362880.0));
-    ps = fma(ps, x2, SBE_SCONST(-1.0 / 5040.0));
-    ps = fma(ps, x2, SBE_SCONST(1.0 / 120.0));
-    ps = fma(ps, x2, SBE_SCONST(-1.0 / 6.0));
-    sn = fma(ps * x2, x, x);
-    double pc = SBE_SCONST(1.0 / 479001600.0);  // 1/12!
-    pc = fma(pc, x2, SBE_SCONST(-1.0 / 3628800.0));
-    pc = fma(pc, x2, SBE_SCONST(1.0 / 40320.0));
-    pc = fma(pc, x2, SBE_SCONST(-1.0 / 720.0));
-    pc = fma(pc, x2, SBE_SCONST(1.0 / 24.0));
-    pc = fma(pc, x2, -0.5);
-    cs = fma(pc, x2, 1.0);
+    const double x2 = x * x, x4 = x2 * x2, x6 = x4 * x2, x8 = x4 * x4, x10 = x8 * x2, x12 = x8 * x4;
+    double ps = SBE_SCONST(1.0 / 6227020800.0) * x12;  // 1/13!
+    ps = fma(SBE_SCONST(-1.0 / 39916800.0), x10, ps);
+    ps = fma(SBE_SCONST(1.0 / 362880.0), x8, ps);
+    ps = fma(SBE_SCONST(-1.0 / 5040.0), x6, ps);
+    ps = fma(SBE_SCONST(1.0 / 120.0), x4, ps);
+    ps = fma(SBE_SCONST(-1.0 / 6.0), x2, ps);  // sin x / x - 1
+    sn = fma(ps, x, x);
+    double pc = SBE_SCONST(1.0 / 479001600.0) * x12;  // 1/12!
+    pc = fma(SBE_SCONST(-1.0 / 3628800.0), x10, pc);
+    pc = fma(SBE_SCONST(1.0 / 40320.0), x8, pc);
+    pc = fma(SBE_SCONST(-1.0 / 720.0), x6, pc);
+    pc = fma(SBE_SCONST(1.0 / 24.0), x4, pc);
+    cs = fma(-0.5, x2, pc) + 1.0;  // cos x
 }
 
 // sin and cos of a larger angle, out of line: only pulses that move A.v by
