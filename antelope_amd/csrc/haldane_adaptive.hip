@@ -108,11 +108,22 @@ __device__ __forceinline__ float rsqrt_fast(float x) { return 1.0f / sqrtf(x); }
 // max that propagates NaN (fmax would drop it and accept a non-finite step)
 __device__ __forceinline__ double nanmax(double a, double b) { return (a != a || a > b) ? a : b; }
 
+// Maximum over the wave (NaN if any lane holds one), in every lane and in
+// scalar registers: DPP within each row of 16 lanes, then the row swaps, so
+// no step waits on an LDS permute. Requires all 64 lanes.
 __device__ __forceinline__ double wave_max(double x)
 {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) x = nanmax(x, __shfl_xor(x, off, 64));
-    return uniform(x);
+    x = nanmax(x, dpp_f64<0xB1>(x));   // quad_perm [1,0,3,2]
+    x = nanmax(x, dpp_f64<0x4E>(x));   // quad_perm [2,3,0,1]
+    x = nanmax(x, dpp_f64<0x141>(x));  // row_half_mirror: the other quad of the half row
+    x = nanmax(x, dpp_f64<0x140>(x));  // row_mirror: the other half row
+    double a = x, b = x;
+    swap16(a, b);  // a: rows 0, 0, 2, 2; b: rows 1, 1, 3, 3
+    x = nanmax(a, b);
+    a = x;
+    b = x;
+    swap32(a, b);  // a: rows 0, 1, 0, 1; b: rows 2, 3, 2, 3
+    return uniform(nanmax(a, b));
 }
 
 constexpr int kRow = 14;  // 12 phase values, then dA/dt (x, y)
