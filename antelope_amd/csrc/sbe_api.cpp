@@ -955,9 +955,19 @@ struct sbe_batch {
     int kpt = 1;
     sbe::AdaptiveArgs *d_ainsts = nullptr;
     bool t1 = false, deph = false;
-    sbe::SeriesJob *d_jobs = nullptr;  // [n] reductions of one launch window
+    sbe::SeriesJob *d_jobs = nullptr;  // [2][n] reductions of one launch window, by window parity
     bool multi = false;                // every member's rows fit one reduction pass
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+    // Windows alternate between two partial-row buffers per member (the
+    // member's own, then part2), and each window's k sums run on rstream while
+    // the next window steps: a window waits for the reduction that last read its
+    // buffer, and every run ends with the step stream waiting for the last one.
+    hipStream_t rstream = nullptr;
+    std::vector<double *> part2;         // second partial-row buffer per member
+    sbe::SeriesJob *h_jobs = nullptr;    // [2][n] pinned host copies of the reductions' jobs
+    hipEvent_t win_done[2] = {nullptr, nullptr}, red_done[2] = {nullptr, nullptr};
+    bool red_pending[2] = {false, false};
+    int64_t windows = 0;                 // windows launched so far (parity = windows & 1)
     std::string err;
 };
 
@@ -982,11 +992,20 @@ void sbe_batch_destroy(sbe_batch *b)
         (void)hipEventDestroy(p.first);
         (void)hipEventDestroy(p.second);
     }
+    if (b->rstream) (void)hipStreamSynchronize(b->rstream);
     for (sbe_ctx *c : b->ctx) sbe_destroy(c);
+    for (double *q : b->part2)
+        if (q) (void)hipFree(q);
     if (b->d_insts) (void)hipFree(b->d_insts);
     if (b->d_ainsts) (void)hipFree(b->d_ainsts);
     if (b->d_map) (void)hipFree(b->d_map);
     if (b->d_jobs) (void)hipFree(b->d_jobs);
+    if (b->h_jobs) (void)hipHostFree(b->h_jobs);
+    for (int p = 0; p < 2; ++p) {
+        if (b->win_done[p]) (void)hipEventDestroy(b->win_done[p]);
+        if (b->red_done[p]) (void)hipEventDestroy(b->red_done[p]);
+    }
+    if (b->rstream) (void)hipStreamDestroy(b->rstream);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
@@ -1029,24 +1048,38 @@ int sbe_batch_create(int32_t n, const sbe_config *cfgs, const sbe_model *models,
     b->rk = cfgs[0].rk_order;
     b->prec = cfgs[0].precision;
     hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->rstream, hipStreamNonBlocking);
+    for (int p = 0; p < 2 && e == hipSuccess; ++p) {
+        e = hipEventCreateWithFlags(&b->win_done[p], hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&b->red_done[p], hipEventDisableTiming);
+    }
     if (e == hipSuccess) e = hipMalloc(&b->d_insts, sizeof(sbe::HaldaneArgs) * n);
     if (e == hipSuccess) e = hipMalloc(&b->d_map, sizeof(int32_t) * (b->max_blocks + n));
-    if (e == hipSuccess) e = hipMalloc(&b->d_jobs, sizeof(sbe::SeriesJob) * n);
+    if (e == hipSuccess) e = hipMalloc(&b->d_jobs, sizeof(sbe::SeriesJob) * 2 * n);
+    if (e == hipSuccess) e = hipHostMalloc(&b->h_jobs, sizeof(sbe::SeriesJob) * 2 * n, hipHostMallocDefault);
+    b->part2.assign(n, nullptr);
+    for (int i = 0; i < n && e == hipSuccess; ++i) {
+        const sbe_ctx *c = b->ctx[i];
+        e = hipMalloc(&b->part2[i], static_cast<size_t>(c->chunk_steps) * c->nwaves * c->nv * sizeof(double));
+    }
     b->multi = true;
     for (sbe_ctx *c : b->ctx) b->multi = b->multi && sbe::series_multi_ok(c->nwaves);
     if (e == hipSuccess && b->adaptive) {
         // constant for the batch's lifetime: every solve's arguments and the block -> solve map
         // over all solves (solves with no samples left return at once)
-        e = hipMalloc(&b->d_ainsts, sizeof(sbe::AdaptiveArgs) * n);
-        std::vector<sbe::AdaptiveArgs> ai(n);
+        // [2][n]: the arguments of even windows, then of odd windows (partial rows in part2)
+        e = hipMalloc(&b->d_ainsts, sizeof(sbe::AdaptiveArgs) * 2 * n);
+        std::vector<sbe::AdaptiveArgs> ai(2 * n);
         std::vector<int32_t> map(b->max_blocks + n);
         int nb = 0;
         for (int i = 0; i < n; ++i) {
             ai[i] = b->ctx[i]->aargs;
+            ai[n + i] = b->ctx[i]->aargs;
+            ai[n + i].h.partials = b->part2[i];
             map[b->max_blocks + i] = nb;
             for (int64_t q = 0; q < b->ctx[i]->nk_stride / (kBlock * b->kpt); ++q) map[nb++] = i;
         }
-        if (e == hipSuccess) e = hipMemcpy(b->d_ainsts, ai.data(), sizeof(sbe::AdaptiveArgs) * n, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(b->d_ainsts, ai.data(), sizeof(sbe::AdaptiveArgs) * 2 * n, hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemcpy(b->d_map, map.data(), sizeof(int32_t) * map.size(), hipMemcpyHostToDevice);
     }
     if (e != hipSuccess) {
@@ -1065,8 +1098,6 @@ int sbe_batch_run(sbe_batch *b, int64_t max_steps)
     int64_t budget = max_steps > 0 ? max_steps : INT64_MAX;
     std::vector<sbe::HaldaneArgs> insts(n);
     std::vector<int32_t> map(b->max_blocks + n);
-    std::vector<sbe::SeriesJob> jobs;
-    jobs.reserve(n);
     while (budget > 0) {
         int64_t launch = INT64_MAX;
         for (sbe_ctx *c : b->ctx)
@@ -1074,10 +1105,12 @@ int sbe_batch_run(sbe_batch *b, int64_t max_steps)
         if (launch == INT64_MAX) break;  // all done
         if (b->adaptive) launch = kAdaptiveWindow;  // every solve on the fixed window grid
         launch = std::min(launch, budget);
+        const int par = static_cast<int>(b->windows & 1);
         int nb = 0;
         for (int i = 0; i < n; ++i) {
             sbe_ctx *c = b->ctx[i];
             insts[i] = c->hargs;
+            if (par) insts[i].partials = b->part2[i];
             insts[i].step0 = c->steps_done;
             insts[i].nsteps = static_cast<int32_t>(std::min<int64_t>(launch, c->laser.nt - c->steps_done));
             if (b->adaptive) continue;  // its block map is static (set up at creation)
@@ -1088,7 +1121,9 @@ int sbe_batch_run(sbe_batch *b, int64_t max_steps)
         // solve -> first block table sits after the block -> solve table
         int32_t *d_first = b->d_map + b->max_blocks;
         hipError_t e = hipSuccess;
-        if (!b->adaptive) {  // the adaptive batch keeps its arguments and block map on the device (set up once)
+        // this parity's buffers are free once the reduction that last read them has run
+        if (b->red_pending[par]) e = hipStreamWaitEvent(b->stream, b->red_done[par], 0);
+        if (e == hipSuccess && !b->adaptive) {  // the adaptive batch keeps its arguments and block map on the device (set up once)
             e = hipMemcpyAsync(b->d_insts, insts.data(), sizeof(sbe::HaldaneArgs) * n, hipMemcpyHostToDevice, b->stream);
             if (e == hipSuccess) e = hipMemcpyAsync(b->d_map, map.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, b->stream);
             if (e == hipSuccess)
@@ -1098,41 +1133,57 @@ int sbe_batch_run(sbe_batch *b, int64_t max_steps)
         if (e == hipSuccess) e = hipEventCreate(&ea);
         if (e == hipSuccess) e = hipEventCreate(&eb);
         if (e == hipSuccess) e = hipEventRecord(ea, b->stream);
+        sbe::AdaptiveArgs *d_ai = b->d_ainsts + static_cast<int64_t>(par) * n;
         if (e == hipSuccess && b->adaptive) {
-            e = sbe::launch_haldane_adaptive(b->d_ainsts, b->d_map, d_first, b->max_blocks, launch, b->prec, b->deph,
-                                             b->kpt, b->stream);
-            if (e == hipSuccess) e = sbe::launch_adaptive_advance(b->d_ainsts, n, launch, b->stream);
+            e = sbe::launch_haldane_adaptive(d_ai, b->d_map, d_first, b->max_blocks, launch, b->prec, b->deph, b->kpt,
+                                             b->stream);
+            if (e == hipSuccess) e = sbe::launch_adaptive_advance(d_ai, n, launch, b->stream);
         } else if (e == hipSuccess) {
             e = sbe::launch_haldane_batch(b->d_insts, b->d_map, d_first, nb, b->rk, b->prec, b->t1, b->deph, b->stream);
         }
         if (e == hipSuccess) e = hipEventRecord(eb, b->stream);
         if (ea && eb) b->ev.push_back({ea, eb});
-        // the k sums of this window: one launch for all members when each fits one pass
-        // (the same per-step arithmetic as a member's own reduction, so the same bits)
-        jobs.clear();
+        if (e == hipSuccess) e = hipEventRecord(b->win_done[par], b->stream);
+        // the k sums of this window, on the reduction stream: one launch for all members when
+        // each fits one pass (the same per-step arithmetic as a member's own reduction, so the
+        // same bits)
+        if (e == hipSuccess) e = hipStreamWaitEvent(b->rstream, b->win_done[par], 0);
+        // the pinned job list of this parity was last read by the reduction two windows back
+        if (e == hipSuccess && b->red_pending[par]) e = hipEventSynchronize(b->red_done[par]);
+        sbe::SeriesJob *h_jobs = b->h_jobs + static_cast<int64_t>(par) * n;
+        sbe::SeriesJob *d_jobs = b->d_jobs + static_cast<int64_t>(par) * n;
+        int njobs = 0;
         int64_t total = 0;
         for (int i = 0; i < n && e == hipSuccess; ++i) {
             sbe_ctx *c = b->ctx[i];
             if (insts[i].nsteps <= 0) continue;
+            double *rows = par ? b->part2[i] : c->d_partials;
             if (b->multi) {
-                jobs.push_back({c->d_partials, c->d_series + c->steps_done * c->nv, c->nwaves, insts[i].nsteps, total,
-                                c->nonfinite_d});
+                h_jobs[njobs++] = {rows, c->d_series + c->steps_done * c->nv, c->nwaves, insts[i].nsteps, total,
+                                   c->nonfinite_d};
                 total += insts[i].nsteps;
             } else {
-                e = sbe::launch_series_reduce(c->d_partials, insts[i].nsteps, c->nwaves, c->nv,
-                                              c->d_series + c->steps_done * c->nv, b->stream, c->nonfinite_d);
+                e = sbe::launch_series_reduce(rows, insts[i].nsteps, c->nwaves, c->nv,
+                                              c->d_series + c->steps_done * c->nv, b->rstream, c->nonfinite_d);
             }
             c->steps_done += insts[i].nsteps;
         }
-        if (e == hipSuccess && !jobs.empty()) {
-            e = hipMemcpyAsync(b->d_jobs, jobs.data(), sizeof(sbe::SeriesJob) * jobs.size(), hipMemcpyHostToDevice,
-                               b->stream);
+        if (e == hipSuccess && njobs > 0) {
+            e = hipMemcpyAsync(d_jobs, h_jobs, sizeof(sbe::SeriesJob) * njobs, hipMemcpyHostToDevice, b->rstream);
             if (e == hipSuccess)
-                e = sbe::launch_series_reduce_multi(b->d_jobs, static_cast<int>(jobs.size()), total, sbe::kHaldaneNV,
-                                                    b->stream);
+                e = sbe::launch_series_reduce_multi(d_jobs, njobs, total, sbe::kHaldaneNV, b->rstream);
         }
+        if (e == hipSuccess) e = hipEventRecord(b->red_done[par], b->rstream);
         if (e != hipSuccess) return batch_fail(b, SBE_ERR_HIP, std::string("batch launch: ") + hipGetErrorString(e));
+        b->red_pending[par] = true;
+        ++b->windows;
         budget -= launch;
+    }
+    // the step stream completes no earlier than the last reduction (synchronize and reset wait on it)
+    const int last = static_cast<int>((b->windows + 1) & 1);
+    if (b->windows > 0) {
+        const hipError_t e = hipStreamWaitEvent(b->stream, b->red_done[last], 0);
+        if (e != hipSuccess) return batch_fail(b, SBE_ERR_HIP, std::string("batch launch: ") + hipGetErrorString(e));
     }
     return SBE_OK;
 }
