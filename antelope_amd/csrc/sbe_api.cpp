@@ -944,30 +944,42 @@ int sbe_get_info(const sbe_ctx *ctx, sbe_info *info)
 
 }  // extern "C"
 
+// A group of a batch's solves with its own streams: its launch windows follow
+// each other on `stream`, its k-sum reductions run on `rstream` while its next
+// window steps, and the windows alternate between two partial-row buffers per
+// solve (a window waits for the reduction that last read its buffer). The
+// adaptive batch runs two groups (solves of even and of odd index), so that one
+// group's launch tail overlaps the other's next window; each solve still takes
+// the same window sequence, so the groups change no result.
+struct BatchGroup {
+    std::vector<int> solves;        // the group's solves (batch indices)
+    int arg0 = 0;                   // its first argument in each parity half of d_ainsts
+    int nblocks = 0;                // adaptive: its blocks
+    int32_t *d_map = nullptr;       // adaptive: [nblocks] block -> solve (in the group), then [n_g] first blocks
+    hipStream_t stream = nullptr, rstream = nullptr;
+    hipEvent_t win_done[2] = {nullptr, nullptr}, red_done[2] = {nullptr, nullptr};
+    bool red_pending[2] = {false, false};
+    int64_t windows = 0;                 // windows launched (parity = windows & 1)
+    sbe::SeriesJob *h_jobs = nullptr;    // [2][n_g] pinned job lists of the reductions
+    sbe::SeriesJob *d_jobs = nullptr;    // [2][n_g]
+};
+
 struct sbe_batch {
     std::vector<sbe_ctx *> ctx;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;  // the first group's step stream; every run ends with it joined to the others
     sbe::HaldaneArgs *d_insts = nullptr;
-    int32_t *d_map = nullptr;  // [2][max_blocks]: block -> solve, solve -> first block
+    int32_t *d_map = nullptr;  // fixed step: [2][max_blocks]: block -> solve, solve -> first block
     int max_blocks = 0;
     int rk = 0, prec = 0;
     bool adaptive = false;
     int kpt = 1;
-    sbe::AdaptiveArgs *d_ainsts = nullptr;
+    sbe::AdaptiveArgs *d_ainsts = nullptr;  // adaptive: [2][n], by window parity, group by group
     bool t1 = false, deph = false;
-    sbe::SeriesJob *d_jobs = nullptr;  // [2][n] reductions of one launch window, by window parity
-    bool multi = false;                // every member's rows fit one reduction pass
+    bool multi = false;  // every member's rows fit one reduction pass
+    std::vector<BatchGroup> groups;
+    std::vector<double *> part2;  // second partial-row buffer per member (odd windows)
+    hipEvent_t run_start = nullptr;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
-    // Windows alternate between two partial-row buffers per member (the
-    // member's own, then part2), and each window's k sums run on rstream while
-    // the next window steps: a window waits for the reduction that last read its
-    // buffer, and every run ends with the step stream waiting for the last one.
-    hipStream_t rstream = nullptr;
-    std::vector<double *> part2;         // second partial-row buffer per member
-    sbe::SeriesJob *h_jobs = nullptr;    // [2][n] pinned host copies of the reductions' jobs
-    hipEvent_t win_done[2] = {nullptr, nullptr}, red_done[2] = {nullptr, nullptr};
-    bool red_pending[2] = {false, false};
-    int64_t windows = 0;                 // windows launched so far (parity = windows & 1)
     std::string err;
 };
 
@@ -987,26 +999,32 @@ const char *sbe_batch_last_error(const sbe_batch *b) { return b ? b->err.c_str()
 void sbe_batch_destroy(sbe_batch *b)
 {
     if (!b) return;
-    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (BatchGroup &g : b->groups) {
+        if (g.stream) (void)hipStreamSynchronize(g.stream);
+        if (g.rstream) (void)hipStreamSynchronize(g.rstream);
+    }
     for (auto &p : b->ev) {
         (void)hipEventDestroy(p.first);
         (void)hipEventDestroy(p.second);
     }
-    if (b->rstream) (void)hipStreamSynchronize(b->rstream);
     for (sbe_ctx *c : b->ctx) sbe_destroy(c);
     for (double *q : b->part2)
         if (q) (void)hipFree(q);
     if (b->d_insts) (void)hipFree(b->d_insts);
     if (b->d_ainsts) (void)hipFree(b->d_ainsts);
     if (b->d_map) (void)hipFree(b->d_map);
-    if (b->d_jobs) (void)hipFree(b->d_jobs);
-    if (b->h_jobs) (void)hipHostFree(b->h_jobs);
-    for (int p = 0; p < 2; ++p) {
-        if (b->win_done[p]) (void)hipEventDestroy(b->win_done[p]);
-        if (b->red_done[p]) (void)hipEventDestroy(b->red_done[p]);
+    if (b->run_start) (void)hipEventDestroy(b->run_start);
+    for (BatchGroup &g : b->groups) {
+        if (g.d_map) (void)hipFree(g.d_map);
+        if (g.d_jobs) (void)hipFree(g.d_jobs);
+        if (g.h_jobs) (void)hipHostFree(g.h_jobs);
+        for (int p = 0; p < 2; ++p) {
+            if (g.win_done[p]) (void)hipEventDestroy(g.win_done[p]);
+            if (g.red_done[p]) (void)hipEventDestroy(g.red_done[p]);
+        }
+        if (g.rstream) (void)hipStreamDestroy(g.rstream);
+        if (g.stream) (void)hipStreamDestroy(g.stream);
     }
-    if (b->rstream) (void)hipStreamDestroy(b->rstream);
-    if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
 
@@ -1047,16 +1065,28 @@ int sbe_batch_create(int32_t n, const sbe_config *cfgs, const sbe_model *models,
     t_batch_member = false;
     b->rk = cfgs[0].rk_order;
     b->prec = cfgs[0].precision;
-    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->rstream, hipStreamNonBlocking);
-    for (int p = 0; p < 2 && e == hipSuccess; ++p) {
-        e = hipEventCreateWithFlags(&b->win_done[p], hipEventDisableTiming);
-        if (e == hipSuccess) e = hipEventCreateWithFlags(&b->red_done[p], hipEventDisableTiming);
+    // groups: two for the adaptive batch (even and odd solves), one otherwise
+    const int ng = b->adaptive && n >= 2 ? 2 : 1;
+    b->groups.resize(ng);
+    for (int i = 0; i < n; ++i) b->groups[i % ng].solves.push_back(i);
+    hipError_t e = hipEventCreateWithFlags(&b->run_start, hipEventDisableTiming);
+    int arg0 = 0;
+    for (BatchGroup &g : b->groups) {
+        g.arg0 = arg0;
+        arg0 += static_cast<int>(g.solves.size());
+        const size_t nj = 2 * g.solves.size();
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&g.rstream, hipStreamNonBlocking);
+        for (int p = 0; p < 2 && e == hipSuccess; ++p) {
+            e = hipEventCreateWithFlags(&g.win_done[p], hipEventDisableTiming);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&g.red_done[p], hipEventDisableTiming);
+        }
+        if (e == hipSuccess) e = hipMalloc(&g.d_jobs, sizeof(sbe::SeriesJob) * nj);
+        if (e == hipSuccess) e = hipHostMalloc(&g.h_jobs, sizeof(sbe::SeriesJob) * nj, hipHostMallocDefault);
     }
+    b->stream = b->groups[0].stream;
     if (e == hipSuccess) e = hipMalloc(&b->d_insts, sizeof(sbe::HaldaneArgs) * n);
     if (e == hipSuccess) e = hipMalloc(&b->d_map, sizeof(int32_t) * (b->max_blocks + n));
-    if (e == hipSuccess) e = hipMalloc(&b->d_jobs, sizeof(sbe::SeriesJob) * 2 * n);
-    if (e == hipSuccess) e = hipHostMalloc(&b->h_jobs, sizeof(sbe::SeriesJob) * 2 * n, hipHostMallocDefault);
     b->part2.assign(n, nullptr);
     for (int i = 0; i < n && e == hipSuccess; ++i) {
         const sbe_ctx *c = b->ctx[i];
@@ -1065,22 +1095,29 @@ int sbe_batch_create(int32_t n, const sbe_config *cfgs, const sbe_model *models,
     b->multi = true;
     for (sbe_ctx *c : b->ctx) b->multi = b->multi && sbe::series_multi_ok(c->nwaves);
     if (e == hipSuccess && b->adaptive) {
-        // constant for the batch's lifetime: every solve's arguments and the block -> solve map
-        // over all solves (solves with no samples left return at once)
-        // [2][n]: the arguments of even windows, then of odd windows (partial rows in part2)
+        // constant for the batch's lifetime: every solve's arguments, [2][n] by window parity
+        // (odd windows write part2) and group by group, and each group's block -> solve map
+        // (solves with no samples left return at once)
         e = hipMalloc(&b->d_ainsts, sizeof(sbe::AdaptiveArgs) * 2 * n);
         std::vector<sbe::AdaptiveArgs> ai(2 * n);
-        std::vector<int32_t> map(b->max_blocks + n);
-        int nb = 0;
-        for (int i = 0; i < n; ++i) {
-            ai[i] = b->ctx[i]->aargs;
-            ai[n + i] = b->ctx[i]->aargs;
-            ai[n + i].h.partials = b->part2[i];
-            map[b->max_blocks + i] = nb;
-            for (int64_t q = 0; q < b->ctx[i]->nk_stride / (kBlock * b->kpt); ++q) map[nb++] = i;
+        for (BatchGroup &g : b->groups) {
+            const int ngs = static_cast<int>(g.solves.size());
+            std::vector<int32_t> map;
+            std::vector<int32_t> first(ngs);
+            for (int r = 0; r < ngs; ++r) {
+                const int i = g.solves[r];
+                ai[g.arg0 + r] = b->ctx[i]->aargs;
+                ai[n + g.arg0 + r] = b->ctx[i]->aargs;
+                ai[n + g.arg0 + r].h.partials = b->part2[i];
+                first[r] = static_cast<int32_t>(map.size());
+                for (int64_t q = 0; q < b->ctx[i]->nk_stride / (kBlock * b->kpt); ++q) map.push_back(r);
+            }
+            g.nblocks = static_cast<int>(map.size());
+            map.insert(map.end(), first.begin(), first.end());
+            if (e == hipSuccess) e = hipMalloc(&g.d_map, sizeof(int32_t) * map.size());
+            if (e == hipSuccess) e = hipMemcpy(g.d_map, map.data(), sizeof(int32_t) * map.size(), hipMemcpyHostToDevice);
         }
         if (e == hipSuccess) e = hipMemcpy(b->d_ainsts, ai.data(), sizeof(sbe::AdaptiveArgs) * 2 * n, hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemcpy(b->d_map, map.data(), sizeof(int32_t) * map.size(), hipMemcpyHostToDevice);
     }
     if (e != hipSuccess) {
         const std::string m = std::string("batch set-up: ") + hipGetErrorString(e);
@@ -1091,6 +1128,93 @@ int sbe_batch_create(int32_t n, const sbe_config *cfgs, const sbe_model *models,
     return SBE_OK;
 }
 
+namespace {
+// One launch window of group g: the step launch on g.stream (its partial rows in
+// this window's parity buffers), then the group's k sums on g.rstream.
+hipError_t batch_window(sbe_batch *b, BatchGroup &g, int64_t launch, std::vector<sbe::HaldaneArgs> &insts,
+                        std::vector<int32_t> &map)
+{
+    const int n = static_cast<int>(b->ctx.size());
+    const int par = static_cast<int>(g.windows & 1);
+    int nb = 0;
+    for (int i : g.solves) {
+        sbe_ctx *c = b->ctx[i];
+        insts[i] = c->hargs;
+        if (par) insts[i].partials = b->part2[i];
+        insts[i].step0 = c->steps_done;
+        insts[i].nsteps = static_cast<int32_t>(std::min<int64_t>(launch, c->laser.nt - c->steps_done));
+        if (b->adaptive) continue;  // its block maps are static (set up at creation)
+        map[b->max_blocks + i] = nb;
+        if (insts[i].nsteps > 0)
+            for (int64_t q = 0; q < c->nk_stride / kBlock; ++q) map[nb++] = i;
+    }
+    // this parity's buffers are free once the reduction that last read them has run
+    hipError_t e = g.red_pending[par] ? hipStreamWaitEvent(g.stream, g.red_done[par], 0) : hipSuccess;
+    // solve -> first block table sits after the block -> solve table
+    int32_t *d_first = b->d_map + b->max_blocks;
+    if (e == hipSuccess && !b->adaptive) {  // (one group) the adaptive batch keeps its arguments on the device
+        e = hipMemcpyAsync(b->d_insts, insts.data(), sizeof(sbe::HaldaneArgs) * n, hipMemcpyHostToDevice, g.stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(b->d_map, map.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, g.stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(d_first, map.data() + b->max_blocks, sizeof(int32_t) * n, hipMemcpyHostToDevice, g.stream);
+    }
+    hipEvent_t ea = nullptr, eb = nullptr;
+    if (e == hipSuccess) e = hipEventCreate(&ea);
+    if (e == hipSuccess) e = hipEventCreate(&eb);
+    if (e == hipSuccess) e = hipEventRecord(ea, g.stream);
+    const int ngs = static_cast<int>(g.solves.size());
+    sbe::AdaptiveArgs *d_ai = b->d_ainsts + static_cast<int64_t>(par) * n + g.arg0;
+    if (e == hipSuccess && b->adaptive) {
+        e = sbe::launch_haldane_adaptive(d_ai, g.d_map, g.d_map + g.nblocks, g.nblocks, launch, b->prec, b->deph,
+                                         b->kpt, g.stream);
+        if (e == hipSuccess) e = sbe::launch_adaptive_advance(d_ai, ngs, launch, g.stream);
+    } else if (e == hipSuccess) {
+        e = sbe::launch_haldane_batch(b->d_insts, b->d_map, d_first, nb, b->rk, b->prec, b->t1, b->deph, g.stream);
+    }
+    if (e == hipSuccess) e = hipEventRecord(eb, g.stream);
+    if (ea && eb) b->ev.push_back({ea, eb});
+    else {
+        if (ea) (void)hipEventDestroy(ea);
+        if (eb) (void)hipEventDestroy(eb);
+    }
+    if (e == hipSuccess) e = hipEventRecord(g.win_done[par], g.stream);
+    // the k sums of this window: one launch for the group when each member fits one pass (the
+    // same per-step arithmetic as a member's own reduction, so the same bits)
+    if (e == hipSuccess) e = hipStreamWaitEvent(g.rstream, g.win_done[par], 0);
+    // the pinned job list of this parity was last read by the reduction two windows back
+    if (e == hipSuccess && g.red_pending[par]) e = hipEventSynchronize(g.red_done[par]);
+    sbe::SeriesJob *h_jobs = g.h_jobs + static_cast<int64_t>(par) * ngs;
+    sbe::SeriesJob *d_jobs = g.d_jobs + static_cast<int64_t>(par) * ngs;
+    int njobs = 0;
+    int64_t total = 0;
+    for (int i : g.solves) {
+        if (e != hipSuccess) break;
+        sbe_ctx *c = b->ctx[i];
+        if (insts[i].nsteps <= 0) continue;
+        double *rows = par ? b->part2[i] : c->d_partials;
+        if (b->multi) {
+            h_jobs[njobs++] = {rows, c->d_series + c->steps_done * c->nv, c->nwaves, insts[i].nsteps, total,
+                               c->nonfinite_d};
+            total += insts[i].nsteps;
+        } else {
+            e = sbe::launch_series_reduce(rows, insts[i].nsteps, c->nwaves, c->nv, c->d_series + c->steps_done * c->nv,
+                                          g.rstream, c->nonfinite_d);
+        }
+        c->steps_done += insts[i].nsteps;
+    }
+    if (e == hipSuccess && njobs > 0) {
+        e = hipMemcpyAsync(d_jobs, h_jobs, sizeof(sbe::SeriesJob) * njobs, hipMemcpyHostToDevice, g.rstream);
+        if (e == hipSuccess) e = sbe::launch_series_reduce_multi(d_jobs, njobs, total, sbe::kHaldaneNV, g.rstream);
+    }
+    if (e == hipSuccess) e = hipEventRecord(g.red_done[par], g.rstream);
+    if (e == hipSuccess) {
+        g.red_pending[par] = true;
+        ++g.windows;
+    }
+    return e;
+}
+}  // namespace
+
 int sbe_batch_run(sbe_batch *b, int64_t max_steps)
 {
     if (!b) return batch_fail(nullptr, SBE_ERR_STATE, "null batch");
@@ -1098,93 +1222,31 @@ int sbe_batch_run(sbe_batch *b, int64_t max_steps)
     int64_t budget = max_steps > 0 ? max_steps : INT64_MAX;
     std::vector<sbe::HaldaneArgs> insts(n);
     std::vector<int32_t> map(b->max_blocks + n);
-    while (budget > 0) {
+    // work enqueued on the first stream before this run (a reset) precedes every group
+    hipError_t e = hipEventRecord(b->run_start, b->stream);
+    for (size_t gi = 1; gi < b->groups.size() && e == hipSuccess; ++gi)
+        e = hipStreamWaitEvent(b->groups[gi].stream, b->run_start, 0);
+    while (budget > 0 && e == hipSuccess) {
         int64_t launch = INT64_MAX;
         for (sbe_ctx *c : b->ctx)
             if (c->steps_done < c->laser.nt) launch = std::min<int64_t>(launch, c->chunk_steps);
         if (launch == INT64_MAX) break;  // all done
         if (b->adaptive) launch = kAdaptiveWindow;  // every solve on the fixed window grid
         launch = std::min(launch, budget);
-        const int par = static_cast<int>(b->windows & 1);
-        int nb = 0;
-        for (int i = 0; i < n; ++i) {
-            sbe_ctx *c = b->ctx[i];
-            insts[i] = c->hargs;
-            if (par) insts[i].partials = b->part2[i];
-            insts[i].step0 = c->steps_done;
-            insts[i].nsteps = static_cast<int32_t>(std::min<int64_t>(launch, c->laser.nt - c->steps_done));
-            if (b->adaptive) continue;  // its block map is static (set up at creation)
-            map[b->max_blocks + i] = nb;
-            if (insts[i].nsteps > 0)
-                for (int64_t q = 0; q < c->nk_stride / kBlock; ++q) map[nb++] = i;
+        for (BatchGroup &g : b->groups) {
+            bool active = false;
+            for (int i : g.solves) active = active || b->ctx[i]->steps_done < b->ctx[i]->laser.nt;
+            if (active && e == hipSuccess) e = batch_window(b, g, launch, insts, map);
         }
-        // solve -> first block table sits after the block -> solve table
-        int32_t *d_first = b->d_map + b->max_blocks;
-        hipError_t e = hipSuccess;
-        // this parity's buffers are free once the reduction that last read them has run
-        if (b->red_pending[par]) e = hipStreamWaitEvent(b->stream, b->red_done[par], 0);
-        if (e == hipSuccess && !b->adaptive) {  // the adaptive batch keeps its arguments and block map on the device (set up once)
-            e = hipMemcpyAsync(b->d_insts, insts.data(), sizeof(sbe::HaldaneArgs) * n, hipMemcpyHostToDevice, b->stream);
-            if (e == hipSuccess) e = hipMemcpyAsync(b->d_map, map.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, b->stream);
-            if (e == hipSuccess)
-                e = hipMemcpyAsync(d_first, map.data() + b->max_blocks, sizeof(int32_t) * n, hipMemcpyHostToDevice, b->stream);
-        }
-        hipEvent_t ea = nullptr, eb = nullptr;
-        if (e == hipSuccess) e = hipEventCreate(&ea);
-        if (e == hipSuccess) e = hipEventCreate(&eb);
-        if (e == hipSuccess) e = hipEventRecord(ea, b->stream);
-        sbe::AdaptiveArgs *d_ai = b->d_ainsts + static_cast<int64_t>(par) * n;
-        if (e == hipSuccess && b->adaptive) {
-            e = sbe::launch_haldane_adaptive(d_ai, b->d_map, d_first, b->max_blocks, launch, b->prec, b->deph, b->kpt,
-                                             b->stream);
-            if (e == hipSuccess) e = sbe::launch_adaptive_advance(d_ai, n, launch, b->stream);
-        } else if (e == hipSuccess) {
-            e = sbe::launch_haldane_batch(b->d_insts, b->d_map, d_first, nb, b->rk, b->prec, b->t1, b->deph, b->stream);
-        }
-        if (e == hipSuccess) e = hipEventRecord(eb, b->stream);
-        if (ea && eb) b->ev.push_back({ea, eb});
-        if (e == hipSuccess) e = hipEventRecord(b->win_done[par], b->stream);
-        // the k sums of this window, on the reduction stream: one launch for all members when
-        // each fits one pass (the same per-step arithmetic as a member's own reduction, so the
-        // same bits)
-        if (e == hipSuccess) e = hipStreamWaitEvent(b->rstream, b->win_done[par], 0);
-        // the pinned job list of this parity was last read by the reduction two windows back
-        if (e == hipSuccess && b->red_pending[par]) e = hipEventSynchronize(b->red_done[par]);
-        sbe::SeriesJob *h_jobs = b->h_jobs + static_cast<int64_t>(par) * n;
-        sbe::SeriesJob *d_jobs = b->d_jobs + static_cast<int64_t>(par) * n;
-        int njobs = 0;
-        int64_t total = 0;
-        for (int i = 0; i < n && e == hipSuccess; ++i) {
-            sbe_ctx *c = b->ctx[i];
-            if (insts[i].nsteps <= 0) continue;
-            double *rows = par ? b->part2[i] : c->d_partials;
-            if (b->multi) {
-                h_jobs[njobs++] = {rows, c->d_series + c->steps_done * c->nv, c->nwaves, insts[i].nsteps, total,
-                                   c->nonfinite_d};
-                total += insts[i].nsteps;
-            } else {
-                e = sbe::launch_series_reduce(rows, insts[i].nsteps, c->nwaves, c->nv,
-                                              c->d_series + c->steps_done * c->nv, b->rstream, c->nonfinite_d);
-            }
-            c->steps_done += insts[i].nsteps;
-        }
-        if (e == hipSuccess && njobs > 0) {
-            e = hipMemcpyAsync(d_jobs, h_jobs, sizeof(sbe::SeriesJob) * njobs, hipMemcpyHostToDevice, b->rstream);
-            if (e == hipSuccess)
-                e = sbe::launch_series_reduce_multi(d_jobs, njobs, total, sbe::kHaldaneNV, b->rstream);
-        }
-        if (e == hipSuccess) e = hipEventRecord(b->red_done[par], b->rstream);
-        if (e != hipSuccess) return batch_fail(b, SBE_ERR_HIP, std::string("batch launch: ") + hipGetErrorString(e));
-        b->red_pending[par] = true;
-        ++b->windows;
         budget -= launch;
     }
-    // the step stream completes no earlier than the last reduction (synchronize and reset wait on it)
-    const int last = static_cast<int>((b->windows + 1) & 1);
-    if (b->windows > 0) {
-        const hipError_t e = hipStreamWaitEvent(b->stream, b->red_done[last], 0);
-        if (e != hipSuccess) return batch_fail(b, SBE_ERR_HIP, std::string("batch launch: ") + hipGetErrorString(e));
+    // the first stream completes no earlier than every group's last reduction (synchronize and
+    // reset wait on it)
+    for (BatchGroup &g : b->groups) {
+        if (e != hipSuccess || g.windows == 0) continue;
+        e = hipStreamWaitEvent(b->stream, g.red_done[(g.windows + 1) & 1], 0);
     }
+    if (e != hipSuccess) return batch_fail(b, SBE_ERR_HIP, std::string("batch launch: ") + hipGetErrorString(e));
     return SBE_OK;
 }
 
@@ -1237,15 +1299,29 @@ int sbe_batch_get_timing(sbe_batch *b, double *ms, int64_t *launches)
 {
     if (!b) return batch_fail(nullptr, SBE_ERR_STATE, "null batch");
     int rc = sbe_batch_synchronize(b);
-    double sum = 0.0;
+    // the time during which at least one step launch ran: the union of the launch
+    // intervals (the groups' launches overlap), from their offsets to the first start
+    std::vector<std::pair<double, double>> iv;
+    iv.reserve(b->ev.size());
     for (auto &p : b->ev) {
-        float t = 0.f;
-        if (!rc && hipEventElapsedTime(&t, p.first, p.second) != hipSuccess) rc = batch_fail(b, SBE_ERR_HIP, "event timing");
-        sum += t;
+        float t0 = 0.f, t1 = 0.f;
+        if (!rc && (hipEventElapsedTime(&t0, b->ev[0].first, p.first) != hipSuccess ||
+                    hipEventElapsedTime(&t1, b->ev[0].first, p.second) != hipSuccess))
+            rc = batch_fail(b, SBE_ERR_HIP, "event timing");
+        iv.push_back({t0, t1});
+    }
+    std::sort(iv.begin(), iv.end());
+    double busy = 0.0, hi = -1e300;
+    for (const auto &v : iv) {
+        if (v.second <= hi) continue;
+        busy += v.second - std::max(v.first, hi);
+        hi = v.second;
+    }
+    for (auto &p : b->ev) {
         (void)hipEventDestroy(p.first);
         (void)hipEventDestroy(p.second);
     }
-    if (ms) *ms = sum;
+    if (ms) *ms = rc ? 0.0 : busy;
     if (launches) *launches = static_cast<int64_t>(b->ev.size());
     b->ev.clear();
     return rc;
