@@ -947,15 +947,19 @@ int sbe_get_info(const sbe_ctx *ctx, sbe_info *info)
 // A group of a batch's solves with its own streams: its launch windows follow
 // each other on `stream`, its k-sum reductions run on `rstream` while its next
 // window steps, and the windows alternate between two partial-row buffers per
-// solve (a window waits for the reduction that last read its buffer). The
-// adaptive batch runs two groups (solves of even and of odd index), so that one
-// group's launch tail overlaps the other's next window; each solve still takes
-// the same window sequence, so the groups change no result.
+// solve (a window waits for the reduction that last read its buffer). A batch
+// of two or more solves runs two groups (solves of even and of odd index), so
+// that one group's launch tail overlaps the other's next window; each solve
+// still takes the same window sequence, so the groups change no result.
 struct BatchGroup {
     std::vector<int> solves;        // the group's solves (batch indices)
     int arg0 = 0;                   // its first argument in each parity half of d_ainsts
     int nblocks = 0;                // adaptive: its blocks
-    int32_t *d_map = nullptr;       // adaptive: [nblocks] block -> solve (in the group), then [n_g] first blocks
+    int32_t *d_map = nullptr;       // adaptive: [nblocks] block -> solve (in the group), then [n_g] first blocks;
+                                    // fixed step: [max_blocks] block -> solve, then [n] first blocks (per window)
+    sbe::HaldaneArgs *d_insts = nullptr;   // fixed step: [n] the window's arguments (the group's solves)
+    std::vector<sbe::HaldaneArgs> h_insts;  // fixed step: host copies
+    std::vector<int32_t> h_map;
     hipStream_t stream = nullptr, rstream = nullptr;
     hipEvent_t win_done[2] = {nullptr, nullptr}, red_done[2] = {nullptr, nullptr};
     bool red_pending[2] = {false, false};
@@ -967,8 +971,6 @@ struct BatchGroup {
 struct sbe_batch {
     std::vector<sbe_ctx *> ctx;
     hipStream_t stream = nullptr;  // the first group's step stream; every run ends with it joined to the others
-    sbe::HaldaneArgs *d_insts = nullptr;
-    int32_t *d_map = nullptr;  // fixed step: [2][max_blocks]: block -> solve, solve -> first block
     int max_blocks = 0;
     int rk = 0, prec = 0;
     bool adaptive = false;
@@ -1010,12 +1012,11 @@ void sbe_batch_destroy(sbe_batch *b)
     for (sbe_ctx *c : b->ctx) sbe_destroy(c);
     for (double *q : b->part2)
         if (q) (void)hipFree(q);
-    if (b->d_insts) (void)hipFree(b->d_insts);
     if (b->d_ainsts) (void)hipFree(b->d_ainsts);
-    if (b->d_map) (void)hipFree(b->d_map);
     if (b->run_start) (void)hipEventDestroy(b->run_start);
     for (BatchGroup &g : b->groups) {
         if (g.d_map) (void)hipFree(g.d_map);
+        if (g.d_insts) (void)hipFree(g.d_insts);
         if (g.d_jobs) (void)hipFree(g.d_jobs);
         if (g.h_jobs) (void)hipHostFree(g.h_jobs);
         for (int p = 0; p < 2; ++p) {
@@ -1065,8 +1066,8 @@ int sbe_batch_create(int32_t n, const sbe_config *cfgs, const sbe_model *models,
     t_batch_member = false;
     b->rk = cfgs[0].rk_order;
     b->prec = cfgs[0].precision;
-    // groups: two for the adaptive batch (even and odd solves), one otherwise
-    const int ng = b->adaptive && n >= 2 ? 2 : 1;
+    // groups: even and odd solves (one group for a single solve)
+    const int ng = n >= 2 ? 2 : 1;
     b->groups.resize(ng);
     for (int i = 0; i < n; ++i) b->groups[i % ng].solves.push_back(i);
     hipError_t e = hipEventCreateWithFlags(&b->run_start, hipEventDisableTiming);
@@ -1085,8 +1086,13 @@ int sbe_batch_create(int32_t n, const sbe_config *cfgs, const sbe_model *models,
         if (e == hipSuccess) e = hipHostMalloc(&g.h_jobs, sizeof(sbe::SeriesJob) * nj, hipHostMallocDefault);
     }
     b->stream = b->groups[0].stream;
-    if (e == hipSuccess) e = hipMalloc(&b->d_insts, sizeof(sbe::HaldaneArgs) * n);
-    if (e == hipSuccess) e = hipMalloc(&b->d_map, sizeof(int32_t) * (b->max_blocks + n));
+    if (!b->adaptive)
+        for (BatchGroup &g : b->groups) {
+            g.h_insts.resize(n);
+            g.h_map.resize(b->max_blocks + n);
+            if (e == hipSuccess) e = hipMalloc(&g.d_insts, sizeof(sbe::HaldaneArgs) * n);
+            if (e == hipSuccess) e = hipMalloc(&g.d_map, sizeof(int32_t) * (b->max_blocks + n));
+        }
     b->part2.assign(n, nullptr);
     for (int i = 0; i < n && e == hipSuccess; ++i) {
         const sbe_ctx *c = b->ctx[i];
@@ -1131,10 +1137,12 @@ int sbe_batch_create(int32_t n, const sbe_config *cfgs, const sbe_model *models,
 namespace {
 // One launch window of group g: the step launch on g.stream (its partial rows in
 // this window's parity buffers), then the group's k sums on g.rstream.
-hipError_t batch_window(sbe_batch *b, BatchGroup &g, int64_t launch, std::vector<sbe::HaldaneArgs> &insts,
-                        std::vector<int32_t> &map)
+hipError_t batch_window(sbe_batch *b, BatchGroup &g, int64_t launch, std::vector<sbe::HaldaneArgs> &args)
 {
     const int n = static_cast<int>(b->ctx.size());
+    // the fixed-step path uploads this group's arguments and block map from its own host copies
+    std::vector<sbe::HaldaneArgs> &insts = b->adaptive ? args : g.h_insts;
+    std::vector<int32_t> &map = g.h_map;
     const int par = static_cast<int>(g.windows & 1);
     int nb = 0;
     for (int i : g.solves) {
@@ -1151,10 +1159,10 @@ hipError_t batch_window(sbe_batch *b, BatchGroup &g, int64_t launch, std::vector
     // this parity's buffers are free once the reduction that last read them has run
     hipError_t e = g.red_pending[par] ? hipStreamWaitEvent(g.stream, g.red_done[par], 0) : hipSuccess;
     // solve -> first block table sits after the block -> solve table
-    int32_t *d_first = b->d_map + b->max_blocks;
-    if (e == hipSuccess && !b->adaptive) {  // (one group) the adaptive batch keeps its arguments on the device
-        e = hipMemcpyAsync(b->d_insts, insts.data(), sizeof(sbe::HaldaneArgs) * n, hipMemcpyHostToDevice, g.stream);
-        if (e == hipSuccess) e = hipMemcpyAsync(b->d_map, map.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, g.stream);
+    int32_t *d_first = b->adaptive ? nullptr : g.d_map + b->max_blocks;
+    if (e == hipSuccess && !b->adaptive) {  // the adaptive batch keeps its arguments on the device
+        e = hipMemcpyAsync(g.d_insts, insts.data(), sizeof(sbe::HaldaneArgs) * n, hipMemcpyHostToDevice, g.stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(g.d_map, map.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, g.stream);
         if (e == hipSuccess)
             e = hipMemcpyAsync(d_first, map.data() + b->max_blocks, sizeof(int32_t) * n, hipMemcpyHostToDevice, g.stream);
     }
@@ -1169,7 +1177,7 @@ hipError_t batch_window(sbe_batch *b, BatchGroup &g, int64_t launch, std::vector
                                          b->kpt, g.stream);
         if (e == hipSuccess) e = sbe::launch_adaptive_advance(d_ai, ngs, launch, g.stream);
     } else if (e == hipSuccess) {
-        e = sbe::launch_haldane_batch(b->d_insts, b->d_map, d_first, nb, b->rk, b->prec, b->t1, b->deph, g.stream);
+        e = sbe::launch_haldane_batch(g.d_insts, g.d_map, d_first, nb, b->rk, b->prec, b->t1, b->deph, g.stream);
     }
     if (e == hipSuccess) e = hipEventRecord(eb, g.stream);
     if (ea && eb) b->ev.push_back({ea, eb});
@@ -1221,7 +1229,6 @@ int sbe_batch_run(sbe_batch *b, int64_t max_steps)
     const int n = static_cast<int>(b->ctx.size());
     int64_t budget = max_steps > 0 ? max_steps : INT64_MAX;
     std::vector<sbe::HaldaneArgs> insts(n);
-    std::vector<int32_t> map(b->max_blocks + n);
     // work enqueued on the first stream before this run (a reset) precedes every group
     hipError_t e = hipEventRecord(b->run_start, b->stream);
     for (size_t gi = 1; gi < b->groups.size() && e == hipSuccess; ++gi)
@@ -1236,7 +1243,7 @@ int sbe_batch_run(sbe_batch *b, int64_t max_steps)
         for (BatchGroup &g : b->groups) {
             bool active = false;
             for (int i : g.solves) active = active || b->ctx[i]->steps_done < b->ctx[i]->laser.nt;
-            if (active && e == hipSuccess) e = batch_window(b, g, launch, insts, map);
+            if (active && e == hipSuccess) e = batch_window(b, g, launch, insts);
         }
         budget -= launch;
     }

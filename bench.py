@@ -460,8 +460,8 @@ def bench_batch(args) -> int:
     f = forms[best]
     r = res[best]
     value = f["k_steps_per_s"]
-    # step-kernel busy time of the timed passes: the union of the launch intervals (HIP events; the
-    # adaptive batch runs its solves in two groups whose launches overlap)
+    # step-kernel busy time of the timed passes: the union of the launch intervals (HIP events; a
+    # batch runs its solves in two groups whose launches overlap)
     kt = r["timing"]["step_ms"] * 1e-3
     launches = max(r["timing"]["launches"], 1)
     # the bound that limits the step kernels: FP64 VALU issue (the state stays in registers for a

@@ -278,10 +278,9 @@ int sbe_laser_eval(const sbe_config *cfg, int64_t n, const double *t,
 /* ---- batched solves (no reference counterpart: the reference runs one
  * solve per MPI job). n independent 2-band solves, each with its own
  * configuration, model, k grid and pulse length, advanced together: every
- * launch of the step kernel covers all unfinished solves (the adaptive
- * stepper: all unfinished solves of one of two groups, even and odd index,
- * whose launch windows overlap), and solves that are done drop out of the
- * launch grid. All solves must share rk_order and precision. Results are read
+ * launch of the step kernel covers the unfinished solves of one of two groups
+ * (even and odd index, whose launch windows overlap), and solves that are
+ * done drop out of the launch grid. All solves must share rk_order and precision. Results are read
  * per solve exactly as from a single context. */
 typedef struct sbe_batch sbe_batch;
 int sbe_batch_create(int32_t n, const sbe_config *cfgs, const sbe_model *models, sbe_batch **out);
@@ -295,8 +294,8 @@ int sbe_batch_get_info(const sbe_batch *b, int32_t i, sbe_info *info);
 int sbe_batch_get_series(sbe_batch *b, int32_t i, double *j_inter, double *j_intra, double *rho_bar,
                          double *band_pop);
 /* Time during which at least one batch step launch ran since the last call (the
- * union of their HIP-event intervals: an adaptive batch runs its solves in two
- * groups whose launches overlap), and the number of launches. */
+ * union of their HIP-event intervals: the two groups' launches overlap), and
+ * the number of launches. */
 int sbe_batch_get_timing(sbe_batch *b, double *ms, int64_t *launches);
 const char *sbe_batch_last_error(const sbe_batch *b);
 void sbe_batch_destroy(sbe_batch *b);
