@@ -948,9 +948,9 @@ int sbe_get_info(const sbe_ctx *ctx, sbe_info *info)
 // each other on `stream`, its k-sum reductions run on `rstream` while its next
 // window steps, and the windows alternate between two partial-row buffers per
 // solve (a window waits for the reduction that last read its buffer). A batch
-// of two or more solves runs two groups (solves of even and of odd index), so
-// that one group's launch tail overlaps the other's next window; each solve
-// still takes the same window sequence, so the groups change no result.
+// of two or more solves runs two groups of about equal work, so that one
+// group's launch tail overlaps the other's next window; each solve still takes
+// the same window sequence, so the groups change no result.
 struct BatchGroup {
     std::vector<int> solves;        // the group's solves (batch indices)
     int arg0 = 0;                   // its first argument in each parity half of d_ainsts
@@ -1066,10 +1066,26 @@ int sbe_batch_create(int32_t n, const sbe_config *cfgs, const sbe_model *models,
     t_batch_member = false;
     b->rk = cfgs[0].rk_order;
     b->prec = cfgs[0].precision;
-    // groups: even and odd solves (one group for a single solve)
+    // groups of about equal work (k points x samples): longest first, each solve to the lighter
+    // group (one group for a single solve); within a group, solves in index order
     const int ng = n >= 2 ? 2 : 1;
     b->groups.resize(ng);
-    for (int i = 0; i < n; ++i) b->groups[i % ng].solves.push_back(i);
+    {
+        std::vector<int> order(n);
+        std::vector<double> work(n);
+        for (int i = 0; i < n; ++i) {
+            order[i] = i;
+            work[i] = static_cast<double>(b->ctx[i]->nk_stride) * static_cast<double>(b->ctx[i]->laser.nt);
+        }
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return work[x] > work[y]; });
+        double load[2] = {0.0, 0.0};
+        for (int i : order) {
+            const int g = ng == 2 && load[1] < load[0] ? 1 : 0;
+            load[g] += work[i];
+            b->groups[g].solves.push_back(i);
+        }
+        for (BatchGroup &g : b->groups) std::sort(g.solves.begin(), g.solves.end());
+    }
     hipError_t e = hipEventCreateWithFlags(&b->run_start, hipEventDisableTiming);
     int arg0 = 0;
     for (BatchGroup &g : b->groups) {

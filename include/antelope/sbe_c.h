@@ -279,8 +279,8 @@ int sbe_laser_eval(const sbe_config *cfg, int64_t n, const double *t,
  * solve per MPI job). n independent 2-band solves, each with its own
  * configuration, model, k grid and pulse length, advanced together: every
  * launch of the step kernel covers the unfinished solves of one of two groups
- * (even and odd index, whose launch windows overlap), and solves that are
- * done drop out of the launch grid. All solves must share rk_order and precision. Results are read
+ * (of about equal work, k points x samples; their launch windows overlap), and
+ * solves that are done drop out of the launch grid. All solves must share rk_order and precision. Results are read
  * per solve exactly as from a single context. */
 typedef struct sbe_batch sbe_batch;
 int sbe_batch_create(int32_t n, const sbe_config *cfgs, const sbe_model *models, sbe_batch **out);
