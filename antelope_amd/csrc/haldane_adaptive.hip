@@ -867,7 +867,7 @@ __device__ void adaptive_body(const AdaptiveArgs &A, int64_t blk, const int64_t 
 // phases_at, each component formed as one FMA chain (a stage needs no single
 // phase, only d)
 __device__ __forceinline__ void stage_d_inv(const KPoint<double> &p, const double *row, double c3, double M0,
-                                            Vec3<double> &d, double &inv)
+                                            Vec3<double> &d, double &inv, double &dd)
 {
     double dx = p.er[0] * row[0], dy = p.er[0] * row[1], sb = p.er[3] * row[7];
     dx = fma(-p.ei[0], row[1], dx);
@@ -883,9 +883,22 @@ __device__ __forceinline__ void stage_d_inv(const KPoint<double> &p, const doubl
         sb = fma(p.ei[j + 3], row[6 + 2 * j], sb);
     }
     d = Vec3<double>{dx, dy, fma(-c3, sb, M0)};
-    const double dd = fma(d.x, d.x, fma(d.y, d.y, d.z * d.z));
+    dd = fma(d.x, d.x, fma(d.y, d.y, d.z * d.z));
     // d = 0 (a gap closing) gives a finite inv and a zero dephasing projection
     inv = rcp_nr(fmax(dd, 1e-300));
+}
+
+// The single phases t1 e^{i(k+A).a_j} and e^{i(k+A).b_j} of a row (the
+// products of phases_at, without the sums)
+__device__ __forceinline__ void step_end_phases(const KPoint<double> &p, const double *row, Phases<double> &e)
+{
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        e.ar[j] = fma(p.er[j], row[2 * j], -(p.ei[j] * row[2 * j + 1]));
+        e.ai[j] = fma(p.er[j], row[2 * j + 1], p.ei[j] * row[2 * j]);
+        e.br[j] = fma(p.er[j + 3], row[6 + 2 * j], -(p.ei[j + 3] * row[7 + 2 * j]));
+        e.bi[j] = fma(p.er[j + 3], row[7 + 2 * j], p.ei[j + 3] * row[6 + 2 * j]);
+    }
 }
 
 template <bool DEPH>
@@ -998,6 +1011,8 @@ __device__ void adaptive_body64(const AdaptiveArgs &A, int64_t blk, const int64_
         laser_rows<double>(L, t, hh, tn, rows);
 
         Vec3<double> inc[KPT];
+        Vec3<double> d_end[KPT];  // d, 1/|d|^2 and |d|^2 at the step end (the stage at c = 1)
+        double inv_end[KPT], dd_end[KPT];
         double err_k = 0.0;
 #pragma unroll
         for (int q = 0; q < KPT; ++q) {
@@ -1017,9 +1032,14 @@ __device__ void adaptive_body64(const AdaptiveArgs &A, int64_t blk, const int64_
                     }
                 }
                 Vec3<double> d;
-                double inv;
-                stage_d_inv(p[q], rows + (i - 1) * kRow, c3, M0, d, inv);
+                double inv, dd;
+                stage_d_inv(p[q], rows + (i - 1) * kRow, c3, M0, d, inv, dd);
                 kv[i] = half_f_at<DEPH>(d, inv, y, gh2, gh12);
+                if (CK::c[i] == 1.0) {
+                    d_end[q] = d;
+                    inv_end[q] = inv;
+                    dd_end[q] = dd;
+                }
             }
             inc[q] = Vec3<double>{0, 0, 0};
             double ex = 0, ey = 0, ez = 0;
@@ -1055,9 +1075,13 @@ __device__ void adaptive_body64(const AdaptiveArgs &A, int64_t blk, const int64_
                 p[q].sy += inc[q].y;
                 p[q].sz += inc[q].z;
                 const Vec3<double> y{p[q].sx, p[q].sy, p[q].sz};
-                // the phases at the step end (the time of the stage at c = 1, which formed only
-                // d), from the step end's row rather than kept through the last stage
-                const Phases<double> e1 = phases_at(p[q], rows + 5 * kRow, c3, M0);
+                // the single phases at the step end from its row (the stage at c = 1 formed
+                // only their sums: d, |d|^2 and 1/|d|^2, kept from there)
+                Phases<double> e1;
+                step_end_phases(p[q], rows + 5 * kRow, e1);
+                e1.d = d_end[q];
+                e1.dd = dd_end[q];
+                e1.inv = inv_end[q];
                 k1[q] = half_f_at<DEPH>(e1.d, e1.inv, y, gh2, gh12);  // f(t + h, y5)
                 sample_state(e1, p[q].w, p[q].n, y.x, y.y, y.z, a, ii, v);
             }
