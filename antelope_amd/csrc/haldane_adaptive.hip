@@ -564,7 +564,7 @@ __device__ void adaptive_body(const AdaptiveArgs &A, int64_t blk, const int64_t 
     const HaldaneArgs &a = A.h;
     const int64_t ks = a.nk_stride;
     const int lane = __lane_id();
-    const int wave = static_cast<int>(blk * (kBlock / 64) + (threadIdx.x >> 6));
+    const int wave = static_cast<int>(blk * (kAdaptiveBlock / 64) + (threadIdx.x >> 6));
     const int64_t kbase = static_cast<int64_t>(wave) * 64 * KPT + lane;
     R *st = static_cast<R *>(a.state);
 
@@ -620,14 +620,14 @@ __device__ void adaptive_body(const AdaptiveArgs &A, int64_t blk, const int64_t 
     double *part = a.partials + static_cast<int64_t>(wave) * kHaldaneNV + (lane >> 3);
     const int64_t row_stride = static_cast<int64_t>(a.nwaves) * kHaldaneNV;
 
-    __shared__ R lds_rows[kBlock / 64][6 * kRow];
+    __shared__ R lds_rows[kAdaptiveBlock / 64][6 * kRow];
     R *rows = lds_rows[threadIdx.x >> 6];
     const LaserLane L = laser_lane(A);
     // eps of the reference vector: 2 % of a bound on |d| over the zone, so
     // that its rate stays below |dd/dt| / eps where the gap closes
     const double dmax = 3.0 * fabs(a.t1) + fabs(a.M0) + 3.0 * fabs(a.c3);
     const double eps2 = (0.02 * dmax) * (0.02 * dmax);
-    __shared__ double lds_row64[kBlock / 64][kDelta ? 6 * kRow : 1];
+    __shared__ double lds_row64[kAdaptiveBlock / 64][kDelta ? 6 * kRow : 1];
     double *row64 = kDelta ? lds_row64[threadIdx.x >> 6] : nullptr;
     const double *end64 = row64 + 5 * kRow;  // the step end (window start: its time)
     // FP64 dh and its rate at the step end
@@ -686,7 +686,7 @@ __device__ void adaptive_body(const AdaptiveArgs &A, int64_t blk, const int64_t 
     // stage.
     Vec3<R> k1[KPT];
     Vec3<R> dh0[KPT];  // delta form: dh at the current step start (error scale)
-    __shared__ double lds_ring[kBlock / 64][6];
+    __shared__ double lds_ring[kAdaptiveBlock / 64][6];
     SampleRing ring;
     ring.cnt = 0;
     ring.t = lds_ring[threadIdx.x >> 6];
@@ -931,7 +931,7 @@ __device__ void adaptive_body64(const AdaptiveArgs &A, int64_t blk, const int64_
     const HaldaneArgs &a = A.h;
     const int64_t ks = a.nk_stride;
     const int lane = __lane_id();
-    const int wave = static_cast<int>(blk * (kBlock / 64) + (threadIdx.x >> 6));
+    const int wave = static_cast<int>(blk * (kAdaptiveBlock / 64) + (threadIdx.x >> 6));
     const int64_t kbase = static_cast<int64_t>(wave) * 64 * KPT + lane;
     double *st = static_cast<double *>(a.state);
 
@@ -956,7 +956,7 @@ __device__ void adaptive_body64(const AdaptiveArgs &A, int64_t blk, const int64_
     const double ii = a.inter_intra ? 1.0 : 0.0;
     double *part = a.partials + static_cast<int64_t>(wave) * kHaldaneNV + (lane >> 3);
     const int64_t row_stride = static_cast<int64_t>(a.nwaves) * kHaldaneNV;
-    __shared__ double lds_rows[kBlock / 64][6 * kRow];
+    __shared__ double lds_rows[kAdaptiveBlock / 64][6 * kRow];
     double *rows = lds_rows[threadIdx.x >> 6];
     const LaserLane L = laser_lane(A);
 
@@ -969,7 +969,7 @@ __device__ void adaptive_body64(const AdaptiveArgs &A, int64_t blk, const int64_
     double h = uniform(A.hstep[wave]);
     int64_t nacc = 0, nrej = 0;
     bool failed = false;
-    __shared__ double lds_ring[kBlock / 64][6];
+    __shared__ double lds_ring[kAdaptiveBlock / 64][6];
     SampleRing ring;
     ring.cnt = 0;
     ring.t = lds_ring[threadIdx.x >> 6];
@@ -1120,7 +1120,7 @@ __device__ void adaptive_body64(const AdaptiveArgs &A, int64_t blk, const int64_
 // left leave at once. One kernel serves both cases, so a solve gives the
 // same bits alone and in a batch.
 template <typename R, bool DEPH, int KPT>
-__global__ __launch_bounds__(kBlock)
+__global__ __launch_bounds__(kAdaptiveBlock)
 __attribute__((amdgpu_waves_per_eu(KPT == 1 && sizeof(R) == 8 ? 3 : 1))) void haldane_adaptive_kernel(const AdaptiveArgs *__restrict__ insts,
                                                                   const int32_t *__restrict__ block_inst,
                                                                   const int32_t *__restrict__ block_first,
@@ -1148,8 +1148,8 @@ template <typename R, int KPT>
 void launch_adaptive(const AdaptiveArgs *d, const int32_t *bi, const int32_t *bf, int nblocks, int64_t nwin, bool deph,
                      hipStream_t s)
 {
-    if (deph) haldane_adaptive_kernel<R, true, KPT><<<nblocks, kBlock, 0, s>>>(d, bi, bf, nwin);
-    else haldane_adaptive_kernel<R, false, KPT><<<nblocks, kBlock, 0, s>>>(d, bi, bf, nwin);
+    if (deph) haldane_adaptive_kernel<R, true, KPT><<<nblocks, kAdaptiveBlock, 0, s>>>(d, bi, bf, nwin);
+    else haldane_adaptive_kernel<R, false, KPT><<<nblocks, kAdaptiveBlock, 0, s>>>(d, bi, bf, nwin);
 }
 
 }  // namespace

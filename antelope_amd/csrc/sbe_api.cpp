@@ -712,7 +712,7 @@ int sbe_run(sbe_ctx *ctx, int64_t step_begin, int64_t step_end)
     if (ctx->path == sbe::PathKind::NBand) return sbe::nb_run(ctx, step_begin, step_end);
     if (ctx->adaptive) {
         const bool deph = ctx->hargs.gam2 != 0.0;
-        const int nblocks = static_cast<int>(ctx->nk_stride / (kBlock * ctx->hargs.kpt));
+        const int nblocks = static_cast<int>(ctx->nk_stride / (sbe::kAdaptiveBlock * ctx->hargs.kpt));
         for (int64_t s = step_begin; s < step_end; s += ctx->chunk_steps) {
             const int64_t n = std::min<int64_t>(ctx->chunk_steps, step_end - s);
             // the kernel reads the window start from the device cursor (== s)
@@ -1057,7 +1057,8 @@ int sbe_batch_create(int32_t n, const sbe_config *cfgs, const sbe_model *models,
             return batch_fail(nullptr, rc, m);
         }
         b->ctx.push_back(c);
-        b->max_blocks += static_cast<int>(c->nk_stride / (kBlock * (b->adaptive ? c->hargs.kpt : 1)));
+        b->max_blocks += static_cast<int>(b->adaptive ? c->nk_stride / (sbe::kAdaptiveBlock * c->hargs.kpt)
+                                                     : c->nk_stride / kBlock);
         b->kpt = c->hargs.kpt;
         b->t1 = b->t1 || c->hargs.gam1 != 0.0;
         b->deph = b->deph || c->hargs.gam1 != 0.0 || c->hargs.gam2 != 0.0;
@@ -1132,7 +1133,7 @@ int sbe_batch_create(int32_t n, const sbe_config *cfgs, const sbe_model *models,
                 ai[n + g.arg0 + r] = b->ctx[i]->aargs;
                 ai[n + g.arg0 + r].h.partials = b->part2[i];
                 first[r] = static_cast<int32_t>(map.size());
-                for (int64_t q = 0; q < b->ctx[i]->nk_stride / (kBlock * b->kpt); ++q) map.push_back(r);
+                for (int64_t q = 0; q < b->ctx[i]->nk_stride / (sbe::kAdaptiveBlock * b->kpt); ++q) map.push_back(r);
             }
             g.nblocks = static_cast<int>(map.size());
             map.insert(map.end(), first.begin(), first.end());

@@ -261,6 +261,9 @@ int nb_run(sbe_ctx *ctx, int64_t step_begin, int64_t step_end);
 int nb_series(sbe_ctx *ctx, double *j_inter, double *j_intra, double *rho_bar, double *band_pop);
 int nb_state(sbe_ctx *ctx, int basis, double *rho);
 constexpr int kBlock = 256;
+// The adaptive stepper's launch unit: one wave per block, so a wave slot frees
+// as soon as its wave ends (the waves of a window take different step counts).
+constexpr int kAdaptiveBlock = 64;
 // sbe_api.cpp: argument checks shared by the entry points, the thread-local
 // message of sbe_last_error(NULL), and the fixed-step plan's k points per lane
 int validate_inputs(const sbe_config *cfg, const sbe_model *m, std::string &err);
