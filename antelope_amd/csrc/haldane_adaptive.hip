@@ -108,6 +108,10 @@ __device__ __forceinline__ float rsqrt_fast(float x) { return 1.0f / sqrtf(x); }
 // max that propagates NaN (fmax would drop it and accept a non-finite step)
 __device__ __forceinline__ double nanmax(double a, double b) { return (a != a || a > b) ? a : b; }
 
+// Maximum over the wave of values that are >= 0 or +inf (never NaN), in every
+// lane and in scalar registers: wave_max with the hardware maximum.
+__device__ __forceinline__ double wave_max_nonneg(double x);
+
 // Maximum over the wave (NaN if any lane holds one), in every lane and in
 // scalar registers: DPP within each row of 16 lanes, then the row swaps, so
 // no step waits on an LDS permute. Requires all 64 lanes.
@@ -124,6 +128,21 @@ __device__ __forceinline__ double wave_max(double x)
     b = x;
     swap32(a, b);  // a: rows 0, 1, 0, 1; b: rows 2, 3, 2, 3
     return uniform(nanmax(a, b));
+}
+
+__device__ __forceinline__ double wave_max_nonneg(double x)
+{
+    x = fmax(x, dpp_f64<0xB1>(x));   // quad_perm [1,0,3,2]
+    x = fmax(x, dpp_f64<0x4E>(x));   // quad_perm [2,3,0,1]
+    x = fmax(x, dpp_f64<0x141>(x));  // row_half_mirror
+    x = fmax(x, dpp_f64<0x140>(x));  // row_mirror
+    double a = x, b = x;
+    swap16(a, b);
+    x = fmax(a, b);
+    a = x;
+    b = x;
+    swap32(a, b);
+    return uniform(fmax(a, b));
 }
 
 constexpr int kRow = 14;  // 12 phase values, then dA/dt (x, y)
@@ -1058,14 +1077,20 @@ __device__ void adaptive_body64(const AdaptiveArgs &A, int64_t blk, const int64_
                     ez = fma(ei, kv[i].z, ez);
                 }
             }
-            auto ratio = [&](double e, double y, double z) {  // 0 when the component and its error vanish
+            // |e| / scale; a zero scale (atol = 0 with a vanishing component and increment)
+            // counts as 1e-300, so a vanishing error still gives 0
+            auto ratio = [&](double e, double y, double z) {
                 const double sc = atol + rtol * fmax(fabs(y), fabs(y + z));
-                return e == 0.0 ? 0.0 : fabs(e) * recip(sc);
+                return fabs(e) * recip(fmax(sc, 1e-300));
             };
-            err_k = nanmax(err_k, nanmax(ratio(ex, y0.x, inc[q].x),
-                                         nanmax(ratio(ey, y0.y, inc[q].y), ratio(ez, y0.z, inc[q].z))));
+            const double rx = ratio(ex, y0.x, inc[q].x), ry = ratio(ey, y0.y, inc[q].y),
+                         rz = ratio(ez, y0.z, inc[q].z);
+            // a NaN ratio (a non-finite state) counts as +inf: rejected, the step shrinks to
+            // hmin and the solve fails, as for NaN; the maxima then need no NaN test
+            const double sum = rx + ry + rz;
+            err_k = fmax(err_k, sum == sum ? fmax(fmax(rx, ry), rz) : __builtin_inf());
         }
-        const double err = wave_max(err_k);
+        const double err = wave_max_nonneg(err_k);
 
         if (err <= 1.0) {
             double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
