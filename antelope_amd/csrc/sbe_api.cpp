@@ -1068,7 +1068,7 @@ int sbe_batch_create(int32_t n, const sbe_config *cfgs, const sbe_model *models,
     b->rk = cfgs[0].rk_order;
     b->prec = cfgs[0].precision;
     // groups of about equal work (k points x samples): longest first, each solve to the lighter
-    // group (one group for a single solve); within a group, solves in index order
+    // group (one group for a single solve)
     const int ng = n >= 2 ? 2 : 1;
     b->groups.resize(ng);
     {
@@ -1085,7 +1085,16 @@ int sbe_batch_create(int32_t n, const sbe_config *cfgs, const sbe_model *models,
             load[g] += work[i];
             b->groups[g].solves.push_back(i);
         }
-        for (BatchGroup &g : b->groups) std::sort(g.solves.begin(), g.solves.end());
+        // within a group, the strongest fields first (they take the most steps per window), so
+        // the slowest blocks are dispatched early rather than left for the launch's tail
+        auto field = [&](int i) {
+            double e = 0.0;
+            const sbe_config &cf = cfgs[i];
+            for (int q = 0; q < cf.npulses; ++q) e = std::max(e, std::fabs(cf.pulses[q].E0));
+            return e;
+        };
+        for (BatchGroup &g : b->groups)
+            std::stable_sort(g.solves.begin(), g.solves.end(), [&](int x, int y) { return field(x) > field(y); });
     }
     hipError_t e = hipEventCreateWithFlags(&b->run_start, hipEventDisableTiming);
     int arg0 = 0;
