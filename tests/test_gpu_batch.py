@@ -48,3 +48,30 @@ def test_batch_member_large_fp64_grid():
                 den = np.linalg.norm(alone[k])
                 err = np.linalg.norm(got[i][k] - alone[k]) / den if den > 0 else np.abs(got[i][k]).max()
                 assert err < 1e-12, (k, err)
+
+
+@pytest.mark.parametrize("rk_order", [4, 45])
+def test_batch_timing_is_busy_time(rk_order):
+    """The batch runs its solves in two groups whose step launches overlap; the
+    reported step time is the union of the launch intervals, so it is positive
+    and no longer than the wall time around the run (a sum over overlapping
+    launches could exceed it). Running it again gives the same series."""
+    import time
+    cfgs, models = haldane_batch(n=6, grid=40, cycles=(2, 4), rk_order=rk_order, precision="fp64", seed=3)
+    with SBEBatch(cfgs, models) as b:
+        b.timing()                      # clears anything recorded before
+        t0 = time.perf_counter()
+        b.run()
+        b.synchronize()
+        wall_ms = (time.perf_counter() - t0) * 1e3
+        tim = b.timing()
+        first = [b.series(i) for i in range(len(cfgs))]
+        b.reset()
+        b.run()
+        again = [b.series(i) for i in range(len(cfgs))]
+    print("case: batch timing rk", rk_order, tim, "wall ms", wall_ms, flush=True)
+    assert tim["launches"] >= 2
+    assert 0.0 < tim["step_ms"] <= wall_ms
+    for i in range(len(cfgs)):
+        for k in first[i]:
+            assert np.array_equal(first[i][k], again[i][k]), (i, k)
