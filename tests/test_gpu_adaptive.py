@@ -89,6 +89,34 @@ def test_batch_equals_single_solves():
             assert (gst.accepted, gst.rejected) == (st.accepted, st.rejected)
 
 
+def test_batch_groups_partial_runs_and_rerun():
+    """Five solves of one to six cycles (one to five launch windows) in the batch's two
+    solve groups, advanced in parts that end on window boundaries (2048 samples: the
+    windows are part of the method, so a part ending elsewhere defines other windows),
+    then reset and run again: every solve's series and step counts equal its solve
+    alone, both times."""
+    e0 = [0.006, 0.001, 0.004, 0.0025, 0.005]
+    cyc = [6.0, 1.0, 4.0, 2.5, 5.0]
+    grids = [16, 12, 20, 8, 14]
+    cs = [_cfg(E0=e, ncycles=n, G=g, ellip=0.3 + 0.1 * i) for i, (e, n, g) in enumerate(zip(e0, cyc, grids))]
+    ms = [MODEL] * len(cs)
+    print("case: batch of 5 adaptive solves, 1..6 cycles, runs of 2048 and 4096 samples", flush=True)
+    with SBEBatch(cs, ms) as b:
+        runs = []
+        for _ in range(2):
+            b.run(2048)
+            b.run(4096)
+            b.run()
+            runs.append([(b.series(i), b.step_stats(i)) for i in range(len(cs))])
+            b.reset()
+    for i, (c, m) in enumerate(zip(cs, ms)):
+        s, st = _gpu(c, m)
+        for gs, gst in (runs[0][i], runs[1][i]):
+            for key in s:
+                assert np.array_equal(gs[key], s[key]), (i, key)
+            assert (gst.accepted, gst.rejected) == (st.accepted, st.rejected), i
+
+
 def test_step_size_underflow_is_reported():
     print("case: rtol 1e-30 (steps shrink below 1e-6 dt)", flush=True)
     with SBESolver(_cfg(rtol=1e-30, G=8), MODEL) as s:
