@@ -690,16 +690,18 @@ def main(argv=None) -> int:
     tim = r["timing"]
     kst = r["nk"] * r["steps"]  # oracle-equivalent k-point steps (every form)
     sec = tim["step_ms"] * 1e-3
+    kname = KERNEL_NAME[best]
+    if best == "adaptive_fp64":  # the launch plan's k points per lane
+        kname = f"haldane_adaptive_kernel<double, true, {max(1, tunings[best].get('kpt', 1))}>"
     ach = kst * ALGO_BYTES_PER_KSTEP[best] / sec / 1e9
-    rfl = {"bound": "hbm", "kernel": KERNEL_NAME[best], "achieved": ach, "peak": PEAK_HBM_GBS,
-           "unit": "GB/s", "frac": ach / PEAK_HBM_GBS, "traffic": None,
-           "algo_bytes_per_kstep": ALGO_BYTES_PER_KSTEP[best],
-           "avg_launch_ms": tim["step_ms"] / max(tim["step_launches"], 1),
-           "launches": tim["step_launches"],
-           "note": "time-fused kernel: rho stays in registers for a whole launch, so the measured HBM "
-                   "traffic is far below the algorithmic bytes; its real limit is the VALU (compute_roofline)"}
-    # executed FP64 flops: per k-point step of the fixed-step kernels, per accepted k-point step
-    # of the adaptive one
+    hbm_rfl = {"bound": "hbm", "kernel": kname, "achieved": ach, "peak": PEAK_HBM_GBS,
+               "unit": "GB/s", "frac": ach / PEAK_HBM_GBS, "traffic": None,
+               "algo_bytes_per_kstep": ALGO_BYTES_PER_KSTEP[best],
+               "note": "time-fused kernel: rho stays in registers for a whole launch, so the measured HBM "
+                       "traffic is far below the algorithmic bytes and HBM is not what limits it"}
+    # the bound that limits it: FP64 VALU issue. Executed FP64 flops per k-point step of the
+    # fixed-step kernels, per accepted k-point step of the adaptive one (counted with one k
+    # point per lane on the 64 x 200^2 batch)
     if best == "adaptive_fp64":
         fpk = ADAPT_FLOPS_PER_KSTEP["fp64"]
         kfl = out_forms[best]["accepted_block_steps"] * out_forms[best]["k_per_block"]
@@ -707,14 +709,19 @@ def main(argv=None) -> int:
         fpk = FLOPS_PER_KSTEP[best]
         kfl = kst
     fl = kfl * fpk / sec / 1e12
-    comp = {"bound": "valu", "achieved": fl, "peak": PEAK_TFLOPS["fp64"], "unit": "TFLOP/s",
-            "frac": fl / PEAK_TFLOPS["fp64"], "flops_per_kstep": fpk,
-            "note": "peak is the FP64 vector rate; packed FP32 is not used, so FP32 VALU issues at the same rate"}
+    rfl = {"bound": "valu", "kernel": kname, "achieved": fl, "peak": PEAK_TFLOPS["fp64"], "unit": "TFLOP/s",
+           "frac": fl / PEAK_TFLOPS["fp64"], "flops_per_kstep": fpk, "traffic": None,
+           "avg_launch_ms": tim["step_ms"] / max(tim["step_launches"], 1),
+           "launches": tim["step_launches"],
+           "note": "executed FP64 flops (rocprofv3 SQ_INSTS_VALU_*_F64, FMA = 2) per "
+                   + ("accepted " if best == "adaptive_fp64" else "") + "k-point step over the step kernels' "
+                   "HIP-event time; peak = FP64 vector rate (packed FP32 is not used, so FP32 issues at it too)"
+                   + ("; flops counted with one k point per lane" if best == "adaptive_fp64" else "")}
     prof = ROOT / "profiles" / "haldane_steps_300x300_traffic.json"
     if prof.exists() and args.grid == 300:
         try:
             t = json.loads(prof.read_text())[best]
-            rfl["traffic"] = t["hbm_bytes_per_launch"]
+            rfl["traffic"] = hbm_rfl["traffic"] = t["hbm_bytes_per_launch"]
             rfl["traffic_source"] = str(prof.relative_to(ROOT))
         except Exception:
             pass
@@ -724,7 +731,8 @@ def main(argv=None) -> int:
     for f, rr in res.items():
         if f in out_forms and rr["timing"]["step_ms"] > 0:
             k_s = rr["nk"] * rr["steps"] / (rr["timing"]["step_ms"] * 1e-3)
-            out_forms[f]["kernel"] = KERNEL_NAME[f]
+            out_forms[f]["kernel"] = KERNEL_NAME[f] if f != "adaptive_fp64" else \
+                f"haldane_adaptive_kernel<double, true, {max(1, tunings[f].get('kpt', 1))}>"
             out_forms[f]["hbm_roofline_frac_algorithmic"] = k_s * ALGO_BYTES_PER_KSTEP[f] / 1e9 / PEAK_HBM_GBS
             if f in FLOPS_PER_KSTEP:
                 out_forms[f]["valu_frac_fp64_peak"] = k_s * FLOPS_PER_KSTEP[f] / 1e12 / PEAK_TFLOPS["fp64"]
@@ -754,7 +762,7 @@ def main(argv=None) -> int:
                  "(an adaptive step spans several dt)",
         "forms": out_forms,
         "roofline": rfl,
-        "compute_roofline": comp,
+        "hbm_roofline": hbm_rfl,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
